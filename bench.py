@@ -1,0 +1,278 @@
+"""Benchmark: fwd+bwd Mpix/s of the Gaussian-splatting render path on one MI355X.
+
+Workload (default ``hd1m``): N=1,000,000 Gaussians, 1080x1920, fx=fy=1500,
+c=(960,540), block 16, seeded synthetic scene (``jaxsplat_amd.scenes``), L2
+loss against a seeded U(0,1) target.  One step = render forward (projection,
+binning/sort, rasterize) + loss + backward (rasterize adjoint, projection
+adjoint) through the public API ``jaxsplat_amd.render`` and torch autograd.
+Inputs are resident in HBM before the timed region.
+
+Prints ONE JSON line.  Besides the contract fields it carries:
+* ``roofline``: the dominant kernel (rasterize backward) -- algorithmic bytes
+  per launch / its average duration measured with HIP events on the launch
+  stream, against the 8 TB/s HBM peak;
+* ``cpu_baseline``: the float64 NumPy oracle timed on a bounded sample of the
+  same workload on this host (1 core);
+* ``stages``: per-stage milliseconds of the same pipeline.
+"""
+
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="hd1m", choices=["small", "hd1m", "dense3m"])
+    ap.add_argument("--cpu-baseline", dest="cpu_baseline", action="store_true", default=True)
+    ap.add_argument("--no-cpu-baseline", dest="cpu_baseline", action="store_false")
+    ap.add_argument("--cpu-tiles", type=int, default=24)
+    ap.add_argument("--stage-iters", type=int, default=10)
+    return ap.parse_args()
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(the path shards by view/scene with one gradient all-reduce; see DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import jaxsplat_amd
+    from jaxsplat_amd import _lib, scenes
+    from jaxsplat_amd._ops import _bin, _p, _stream
+
+    if not torch.cuda.is_available():
+        print("bench.py needs a GPU", file=sys.stderr)
+        sys.exit(1)
+    _lib.lib()  # fail loudly if the HIP library is missing
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+
+    sc = scenes.make(args.workload)
+    cam = sc.camera
+    Himg, Wimg = cam.img_shape
+    tgt_seed = {"small": 2000, "hd1m": 2001, "dense3m": 2002}[args.workload]
+    target = torch.from_numpy(scenes.target_image(cam.img_shape, tgt_seed)).to(dev)
+    params = [torch.from_numpy(a).to(dev).requires_grad_(True) for a in sc.params()]
+    viewmat = torch.from_numpy(cam.viewmat).to(dev)
+    background = torch.from_numpy(sc.background).to(dev)
+    kw = dict(viewmat=viewmat, background=background, img_shape=cam.img_shape, f=cam.f, c=cam.c,
+              glob_scale=sc.glob_scale, clip_thresh=sc.clip_thresh, block_size=sc.block_width)
+
+    def step():
+        for p in params:
+            p.grad = None
+        img = jaxsplat_amd.render(*params, **kw)
+        loss = ((img - target) ** 2).mean()
+        loss.backward()
+        return loss
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record()
+    for _ in range(args.steps):
+        step()
+    e1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = e0.elapsed_time(e1) / args.steps
+    mpix = Himg * Wimg / (ms * 1e-3) / 1e6
+
+    stages, roof, extra = stage_breakdown(torch, sc, cam, params, target, args.stage_iters)
+
+    cpu = cpu_baseline(sc, cam, tgt_seed, args.cpu_tiles) if args.cpu_baseline else None
+
+    rec = {
+        "metric": "fwd+bwd Mpix/s at 1M Gaussians 1080p, 1 GPU; % HBM roofline; batched vs plain",
+        "value": round(mpix, 2),
+        "unit": "Mpix/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded scene and U(0,1) target)",
+        "config": {"workload": args.workload, "num_gaussians": sc.n, "img_shape": [Himg, Wimg],
+                   "block_width": sc.block_width, "f": list(cam.f), "c": list(cam.c),
+                   "path": "plain single view (jaxsplat_amd.render + torch autograd)",
+                   "parallelism": "none (1 GPU)"},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "stages_ms": stages,
+        "stats": extra,
+        "wall_s": round(wall, 3),
+    }
+    print(json.dumps(rec), flush=True)
+
+
+def stage_breakdown(torch, sc, cam, params, target, iters):
+    """Run the same pipeline through the C ABI with HIP events between stages."""
+    from jaxsplat_amd import _lib
+    from jaxsplat_amd._ops import _bin, _p, _stream
+    L = _lib.lib()
+    dev = params[0].device
+    N = sc.n
+    Himg, Wimg = cam.img_shape
+    means3d, scales, quats, colors, opac = [p.detach() for p in params]
+    viewmat = torch.from_numpy(cam.viewmat).to(dev)
+    bg = torch.from_numpy(sc.background).to(dev)
+    d = _lib.make_descriptor(N, cam.img_shape, cam.f, cam.c, sc.glob_scale, sc.clip_thresh,
+                             sc.block_width)
+    f32 = dict(dtype=torch.float32, device=dev)
+    i32 = dict(dtype=torch.int32, device=dev)
+    xys, depths, conics = torch.empty((N, 2), **f32), torch.empty((N, 1), **f32), \
+        torch.empty((N, 3), **f32)
+    radii, nth, cum = torch.empty((N, 1), **i32), torch.empty((N, 1), **i32), \
+        torch.empty((N, 1), **i32)
+    wsb = L.jaxsplat_project_workspace_size(ctypes.byref(d))
+    ws = torch.empty(wsb, dtype=torch.uint8, device=dev)
+    fT, fI = torch.empty((Himg, Wimg), **f32), torch.empty((Himg, Wimg), **i32)
+    img = torch.empty((Himg, Wimg, 3), **f32)
+    vc, vo, vxy, vcn = (torch.empty((N, 3), **f32), torch.empty((N, 1), **f32),
+                        torch.empty((N, 2), **f32), torch.empty((N, 3), **f32))
+    vm, vs, vq = torch.empty((N, 3), **f32), torch.empty((N, 3), **f32), torch.empty((N, 4), **f32)
+    names = ["project_fwd", "bin_sort", "raster_fwd", "loss_grad", "raster_bwd", "project_bwd"]
+    acc = {k: 0.0 for k in names}
+    s = _stream(dev)
+    for it in range(iters + 2):
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(len(names) + 1)]
+        ev[0].record()
+        _lib.check(L.jaxsplat_project_forward(
+            ctypes.byref(d), _p(means3d), _p(scales), _p(quats), _p(viewmat), None, _p(xys),
+            _p(depths), _p(radii), _p(conics), None, _p(nth), _p(cum), _p(ws), wsb, s))
+        ev[1].record()
+        isect = int(cum[-1, 0].item())
+        gids, bins, _ = _bin(d, xys, depths, radii, isect, dev)
+        ev[2].record()
+        _lib.check(L.jaxsplat_rasterize_forward_binned(
+            ctypes.byref(d), _p(gids), _p(bins), isect, _p(xys), _p(conics), _p(colors), _p(opac),
+            _p(bg), _p(fT), _p(fI), _p(img), s))
+        ev[3].record()
+        v_img = (2.0 / img.numel()) * (img - target)
+        ev[4].record()
+        _lib.check(L.jaxsplat_rasterize_backward_binned(
+            ctypes.byref(d), _p(gids), _p(bins), isect, _p(xys), _p(conics), _p(colors), _p(opac),
+            _p(bg), _p(fT), _p(fI), _p(v_img), None, _p(vc), _p(vo), _p(vxy), None, _p(vcn), 0, s))
+        ev[5].record()
+        _lib.check(L.jaxsplat_project_backward(
+            ctypes.byref(d), _p(means3d), _p(scales), _p(quats), _p(viewmat), _p(radii),
+            _p(conics), None, _p(vxy), None, _p(vcn), None, _p(vm), _p(vs), _p(vq), None, None,
+            0, s))
+        ev[6].record()
+        torch.cuda.synchronize()
+        if it >= 2:
+            for k, n in enumerate(names):
+                acc[n] += ev[k].elapsed_time(ev[k + 1]) / iters
+    # algorithmic bytes of the dominant kernel (rasterize backward), per launch:
+    #   40 B per consumed intersection (id 4 + xy 8 + opacity 4 + conic 12 + color 12)
+    # + 20 B per pixel (final T 4 + final idx 4 + v_img 12)
+    # + 36 B per visible Gaussian (v_rgb 12 + v_opacity 4 + v_xy 8 + v_conic 12 written)
+    # + 8 B per tile (bin read)
+    bw = sc.block_width
+    tiles_x, tiles_y = (Wimg + bw - 1) // bw, (Himg + bw - 1) // bw
+    bins_np = bins.cpu().numpy().astype(np.int64)
+    fI_np = fI.cpu().numpy().astype(np.int64)
+    pad = np.full((tiles_y * bw, tiles_x * bw), -1, np.int64)
+    pad[:Himg, :Wimg] = fI_np
+    tmax = pad.reshape(tiles_y, bw, tiles_x, bw).max(axis=(1, 3)).reshape(-1)
+    L_b = int(np.clip(tmax - bins_np[:, 0] + 1, 0, bins_np[:, 1] - bins_np[:, 0]).sum())
+    V = int((radii > 0).sum().item())
+    T = tiles_x * tiles_y
+    bytes_bwd = 40 * L_b + 20 * Himg * Wimg + 36 * V + 8 * T
+    t_bwd = acc["raster_bwd"] * 1e-3
+    achieved = bytes_bwd / t_bwd / 1e9
+    roof = {"kernel": "raster_bwd_kernel", "bound": "hbm", "achieved": round(achieved, 2),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+            "traffic": None, "avg_ms": round(acc["raster_bwd"], 4),
+            "bytes_per_launch": bytes_bwd}
+    # whole-step algorithmic bytes (B_alg) for the combined fraction
+    P = 2 if T > 256 else 1
+    L_f_tiles = tmax  # forward consumed prefix bounded by the last contributor (lower bound)
+    B_alg = 232 * N + 20 * V + (20 + 24 * P) * isect + 40 * 2 * L_b + 24 * T + 40 * Himg * Wimg
+    extra = {"num_intersects": isect, "visible": V, "L_b": L_b,
+             "B_alg_bytes": int(B_alg), "stage_sum_ms": round(sum(acc.values()), 4),
+             "step_frac_hbm": round(B_alg / (sum(acc.values()) * 1e-3) / 1e9 / HBM_PEAK_GBS, 5)}
+    return {k: round(v, 4) for k, v in acc.items()}, roof, extra
+
+
+def cpu_baseline(sc, cam, tgt_seed, n_tiles):
+    """Oracle (float64 NumPy, 1 core) on a bounded sample of the same workload."""
+    import oracle
+    from jaxsplat_amd import scenes
+    Himg, Wimg = cam.img_shape
+    bw = sc.block_width
+    kw = dict(img_shape=cam.img_shape, f=cam.f, c=cam.c, glob_scale=sc.glob_scale,
+              clip_thresh=sc.clip_thresh, block_width=bw)
+    t0 = time.perf_counter()
+    pr = oracle.project.project_fwd(sc.means3d, sc.scales, sc.quats, cam.viewmat, **kw)
+    ks, gs, bins = oracle.binning.bin_and_sort(pr["xys"].astype(np.float32),
+                                               pr["depths"].astype(np.float32), pr["radii"],
+                                               pr["cum_tiles_hit"], img_shape=cam.img_shape,
+                                               block_width=bw)
+    t_pre = time.perf_counter() - t0
+    T = bins.shape[0]
+    rng = np.random.default_rng(0)
+    nonempty = np.nonzero(bins[:, 1] > bins[:, 0])[0]
+    tiles = np.sort(rng.choice(nonempty, size=min(n_tiles, nonempty.size), replace=False))
+    target = scenes.target_image(cam.img_shape, tgt_seed)
+    t1 = time.perf_counter()
+    f = oracle.rasterize.rasterize_fwd(gs, bins, pr["xys"], pr["conics"], sc.colors,
+                                       sc.opacities, sc.background, img_shape=cam.img_shape,
+                                       block_width=bw, tiles=tiles)
+    v_img = 2.0 * (f["img"] - target) / target.size
+    b = oracle.rasterize.rasterize_bwd(gs, bins, pr["xys"], pr["conics"], sc.colors, sc.opacities,
+                                       sc.background, f["final_Ts"], f["final_idx"], v_img,
+                                       np.zeros(cam.img_shape), img_shape=cam.img_shape,
+                                       block_width=bw, tiles=tiles)
+    vproj = oracle.project.project_bwd(sc.means3d, sc.scales, sc.quats, cam.viewmat, pr["radii"],
+                                       pr["conics"], pr["compensation"], b["v_xy"],
+                                       np.zeros(sc.n), b["v_conic"], np.zeros(sc.n),
+                                       img_shape=cam.img_shape, f=cam.f, glob_scale=sc.glob_scale)
+    t_tiles = time.perf_counter() - t1
+    px = tiles.size * bw * bw
+    t_eff = t_tiles + t_pre * tiles.size / T
+    return {"value": round(px / t_eff / 1e6, 5), "unit": "Mpix/s", "cores": 1, "kind": "port",
+            "sample": (f"{args_workload(sc)}: float64 oracle; projection + binning of all "
+                       f"{sc.n} Gaussians ({t_pre:.1f} s, prorated {tiles.size}/{T} tiles) + "
+                       f"rasterize fwd+bwd of {tiles.size} seeded random non-empty tiles "
+                       f"({t_tiles:.1f} s) + projection bwd"),
+            "host_cpu": cpu_model(), "host_logical_cpus": os.cpu_count()}
+
+
+def args_workload(sc):
+    return sc.name
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for ln in fh:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+if __name__ == "__main__":
+    main()

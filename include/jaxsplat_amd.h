@@ -1,0 +1,189 @@
+/*
+ * jaxsplat_amd -- C ABI of the MI355X (gfx950) differentiable Gaussian-splatting
+ * path: projection, tile binning/sort and rasterization, forward and backward.
+ *
+ * Every pointer argument is a device pointer (HBM) unless stated otherwise;
+ * `stream` is a hipStream_t (NULL = default stream).  Arrays are dense,
+ * row-major float32/int32 in the layout of the reference's JAX API
+ * (yklcs/jaxsplat jaxsplat/_project/abstract.py:7-21,57-77 and
+ * jaxsplat/_rasterize/abstract.py:7-24,62-85).  No call allocates device
+ * memory except the four reference-compatible custom-call targets, which keep
+ * a grow-only workspace per (device, stream) because their signature carries
+ * none.  All typed entry points return a jaxsplat_status; on error
+ * jaxsplat_last_error() returns a thread-local message.
+ */
+#ifndef JAXSPLAT_AMD_H
+#define JAXSPLAT_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#define JAXSPLAT_ALIGN16 alignas(16)
+#else
+#define JAXSPLAT_ALIGN16 _Alignas(16)
+#endif
+
+typedef enum jaxsplat_status {
+    JAXSPLAT_OK = 0,
+    JAXSPLAT_ERR_INVALID = 1,   /* bad descriptor / argument */
+    JAXSPLAT_ERR_HIP = 2,       /* HIP runtime error */
+    JAXSPLAT_ERR_CAPACITY = 3,  /* look-back timeout or intersection overflow */
+    JAXSPLAT_ERR_WORKSPACE = 4  /* workspace missing or too small */
+} jaxsplat_status;
+
+/* Adjoint convention of the backward entry points. */
+typedef enum jaxsplat_adjoint {
+    JAXSPLAT_ADJOINT_REFERENCE = 0,  /* bit-for-bit reference semantics */
+    JAXSPLAT_ADJOINT_CONSISTENT = 1  /* exact adjoint of the forward */
+} jaxsplat_adjoint;
+
+typedef struct jaxsplat_dim3 {
+    uint32_t x, y, z;
+} jaxsplat_dim3;
+
+/*
+ * Opaque call descriptor.  Same 80-byte layout as the reference's
+ * ops::Descriptor (lib/ops.h:9-22) packed by make_descriptor
+ * (lib/ffi.cu:15-52): img_shape is (W, H, 1), intrins is (fx, fy, cx, cy),
+ * grid_dim_2d is the tile grid.
+ */
+typedef struct jaxsplat_descriptor {
+    uint32_t num_points;
+    jaxsplat_dim3 img_shape;
+    JAXSPLAT_ALIGN16 float intrins[4];
+    float glob_scale;
+    float clip_thresh;
+    uint32_t block_width;
+    uint32_t grid_dim_1d;
+    uint32_t block_dim_1d;
+    jaxsplat_dim3 grid_dim_2d;
+    jaxsplat_dim3 block_dim_2d;
+} jaxsplat_descriptor;
+
+/* Library version string (host memory). */
+const char* jaxsplat_version(void);
+/* Thread-local message of the last failing call, "" if none (host memory). */
+const char* jaxsplat_last_error(void);
+
+/*
+ * Fill *out; replaces make_descriptor (lib/ffi.cu:15-52).  img_h/img_w are the
+ * reference's img_shape (H, W).  Rejects block_width outside [1, 16] (the
+ * reference's LDS arrays hold 256 entries, lib/common.h:9) and a negative
+ * clip_thresh (reference keys would sign-extend into the tile id,
+ * lib/kernels/forward.cu:168-173).
+ */
+int jaxsplat_make_descriptor(jaxsplat_descriptor* out, uint32_t num_points, uint32_t img_h,
+                             uint32_t img_w, float fx, float fy, float cx, float cy,
+                             float glob_scale, float clip_thresh, uint32_t block_width);
+
+/* ------------------------------------------------------------------------ */
+/* Reference-compatible XLA custom-call targets.                             */
+/* Signature of ops::*::xla (lib/ops.h:45-95), registered by the reference   */
+/* as "project_fwd", "project_bwd", "rasterize_fwd", "rasterize_bwd"         */
+/* (lib/ffi.cu:4-13).  `buffers` holds the operands then the results in the   */
+/* reference lowering order (lib/ops.cu:246-258, 336-354, 408-419,           */
+/* 502-519); `opaque` is a jaxsplat_descriptor.  Errors are reported through */
+/* jaxsplat_last_error() (the signature has no status).                      */
+/* ------------------------------------------------------------------------ */
+void jaxsplat_project_fwd(void* stream, void** buffers, const char* opaque, size_t opaque_len);
+void jaxsplat_project_bwd(void* stream, void** buffers, const char* opaque, size_t opaque_len);
+void jaxsplat_rasterize_fwd(void* stream, void** buffers, const char* opaque, size_t opaque_len);
+void jaxsplat_rasterize_bwd(void* stream, void** buffers, const char* opaque, size_t opaque_len);
+
+/* ------------------------------------------------------------------------ */
+/* Typed entry points (caller-owned workspace, nothing allocated).           */
+/* ------------------------------------------------------------------------ */
+
+/* Workspace bytes for jaxsplat_project_forward. */
+size_t jaxsplat_project_workspace_size(const jaxsplat_descriptor* d);
+
+/*
+ * Projection forward; replaces ops::project::fwd::xla (lib/ops.cu:12-61) and
+ * project_gaussians_fwd (lib/kernels/forward.cu:13-131) + the CUB inclusive
+ * scan (lib/ops.cu:607-643).  Shapes: means3d/scales (N,3), quats (N,4),
+ * viewmat (4,4); outputs cov3d (N,6) [nullable], xys (N,2), depths (N),
+ * radii (N), conics (N,3), compensation (N) [nullable], num_tiles_hit (N),
+ * cum_tiles_hit (N, inclusive).  Every output row is written (culled
+ * Gaussians get the reference's zero values).
+ */
+int jaxsplat_project_forward(const jaxsplat_descriptor* d, const float* means3d,
+                             const float* scales, const float* quats, const float* viewmat,
+                             float* cov3d, float* xys, float* depths, int32_t* radii,
+                             float* conics, float* compensation, int32_t* num_tiles_hit,
+                             int32_t* cum_tiles_hit, void* workspace, size_t workspace_bytes,
+                             void* stream);
+
+/*
+ * Projection backward; replaces ops::project::bwd::xla (lib/ops.cu:63-105) and
+ * project_gaussians_bwd (lib/kernels/backward.cu:345-418).  v_depth,
+ * v_compensation (with compensation), v_cov2d and v_cov3d are nullable.
+ */
+int jaxsplat_project_backward(const jaxsplat_descriptor* d, const float* means3d,
+                              const float* scales, const float* quats, const float* viewmat,
+                              const int32_t* radii, const float* conics,
+                              const float* compensation, const float* v_xy,
+                              const float* v_depth, const float* v_conic,
+                              const float* v_compensation, float* v_mean3d, float* v_scale,
+                              float* v_quat, float* v_cov2d, float* v_cov3d, int adjoint,
+                              void* stream);
+
+/* Workspace bytes for jaxsplat_bin_and_sort with up to num_intersects pairs. */
+size_t jaxsplat_bin_workspace_size(const jaxsplat_descriptor* d, uint32_t num_intersects);
+
+/*
+ * Intersection emission, (tile, depth) sort and tile bin edges; replaces
+ * ops::sort_and_bin (lib/ops.cu:645-793) with map_gaussian_to_intersects
+ * (lib/kernels/forward.cu:135-179), CUB SortPairs and get_tile_bin_edges
+ * (forward.cu:184-208).  num_intersects = cum_tiles_hit[N-1].  Outputs:
+ * gaussian_ids_sorted (num_intersects), tile_ids_sorted (num_intersects,
+ * nullable; the reference key is (tile << 32) | bits(depths[id])) and
+ * tile_bins (tiles, 2) with empty tiles (0, 0).
+ */
+int jaxsplat_bin_and_sort(const jaxsplat_descriptor* d, const float* xys, const float* depths,
+                          const int32_t* radii, uint32_t num_intersects,
+                          int32_t* gaussian_ids_sorted, int32_t* tile_ids_sorted,
+                          int32_t* tile_bins, void* workspace, size_t workspace_bytes,
+                          void* stream);
+
+/*
+ * Synchronise `stream` and report a look-back timeout recorded in the
+ * workspace of the last jaxsplat_bin_and_sort (JAXSPLAT_ERR_CAPACITY).
+ */
+int jaxsplat_bin_check(const jaxsplat_descriptor* d, uint32_t num_intersects,
+                       const void* workspace, void* stream);
+
+/*
+ * Rasterize forward over binned intersections; replaces rasterize_fwd
+ * (lib/kernels/forward.cu:339-462) and the zero-intersection path
+ * (lib/ops.cu:119-127).  Outputs final_Ts (H,W), final_idx (H,W),
+ * out_img (H,W,3).
+ */
+int jaxsplat_rasterize_forward_binned(const jaxsplat_descriptor* d,
+                                      const int32_t* gaussian_ids_sorted,
+                                      const int32_t* tile_bins, uint32_t num_intersects,
+                                      const float* xys, const float* conics,
+                                      const float* colors, const float* opacities,
+                                      const float* background, float* final_Ts,
+                                      int32_t* final_idx, float* out_img, void* stream);
+
+/*
+ * Rasterize backward over binned intersections; replaces rasterize_bwd
+ * (lib/kernels/backward.cu:153-343) and lib/ops.cu:184-186.  Gradient outputs
+ * are overwritten (zeroed, then accumulated); v_xy_abs is nullable and a NULL
+ * v_out_img_alpha means a zero cotangent for the alpha channel.
+ */
+int jaxsplat_rasterize_backward_binned(
+    const jaxsplat_descriptor* d, const int32_t* gaussian_ids_sorted, const int32_t* tile_bins,
+    uint32_t num_intersects, const float* xys, const float* conics, const float* colors,
+    const float* opacities, const float* background, const float* final_Ts,
+    const int32_t* final_idx, const float* v_out_img, const float* v_out_img_alpha,
+    float* v_colors, float* v_opacity, float* v_xy, float* v_xy_abs, float* v_conic,
+    int adjoint, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* JAXSPLAT_AMD_H */

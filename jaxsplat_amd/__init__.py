@@ -1,0 +1,35 @@
+"""jaxsplat_amd: MI355X-native differentiable 3D Gaussian splatting.
+
+Drop-in for the reference API of yklcs/jaxsplat (``jaxsplat/__init__.py:6-73``):
+``render``, ``project`` and ``rasterize`` with the same keyword-only arguments,
+outputs and cotangent routing, on torch GPU tensors, backed by hand-written
+HIP kernels for gfx950 reached through the C ABI of ``libjaxsplat_amd.so``.
+
+``adjoint="reference"`` (default) reproduces the reference backward exactly,
+including its fwd/bwd alpha-clamp mismatch and its unclamped EWA Jacobian;
+``adjoint="consistent"`` gives the exact adjoint of the forward.
+"""
+
+from __future__ import annotations
+
+from ._ops import ShapeDtypeError, bin_and_sort, project, rasterize
+
+__all__ = ["render", "project", "rasterize", "bin_and_sort", "ShapeDtypeError"]
+
+
+def render(means3d, scales, quats, colors, opacities, *, viewmat, background, img_shape, f, c,
+           glob_scale, clip_thresh, block_size=16, adjoint="reference"):
+    """Render 3D Gaussians to an (H, W, 3) image, differentiably.
+
+    Same contract as the reference ``render`` (``jaxsplat/__init__.py:6-70``):
+    means3d (N,3), scales (N,3), quats (N,4) (w first, normalised), colors
+    (N,3), opacities (N,1); viewmat (4,4) row-major world-to-camera;
+    background (3,); img_shape (H, W); f (fx, fy); c (cx, cy); glob_scale;
+    clip_thresh; block_size in [1, 16] (tile side).  Differentiable in means3d,
+    scales, quats, colors and opacities.
+    """
+    xys, depths, radii, conics, _num_tiles_hit, cum_tiles_hit = project(
+        means3d, scales, quats, viewmat, img_shape=img_shape, f=f, c=c, glob_scale=glob_scale,
+        clip_thresh=clip_thresh, block_width=block_size, adjoint=adjoint)
+    return rasterize(colors, opacities, background, xys, depths, radii, conics, cum_tiles_hit,
+                     img_shape=img_shape, block_width=block_size, adjoint=adjoint)

@@ -1,0 +1,126 @@
+"""ctypes binding of ``libjaxsplat_amd.so`` (the C ABI in ``include/jaxsplat_amd.h``).
+
+The shared library is built in-tree by ``jaxsplat_amd._build``.  Loading fails
+loudly when it is missing: there is no CPU fallback for any entry point.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import threading
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libjaxsplat_amd.so"
+
+OK = 0
+ERR_INVALID = 1
+ERR_HIP = 2
+ERR_CAPACITY = 3
+ERR_WORKSPACE = 4
+ADJOINT = {"reference": 0, "consistent": 1}
+
+# Every extern "C" symbol declared in include/jaxsplat_amd.h.
+EXPORTS = (
+    "jaxsplat_version", "jaxsplat_last_error", "jaxsplat_make_descriptor",
+    "jaxsplat_project_fwd", "jaxsplat_project_bwd", "jaxsplat_rasterize_fwd",
+    "jaxsplat_rasterize_bwd", "jaxsplat_project_workspace_size",
+    "jaxsplat_project_forward", "jaxsplat_project_backward",
+    "jaxsplat_bin_workspace_size", "jaxsplat_bin_and_sort", "jaxsplat_bin_check",
+    "jaxsplat_rasterize_forward_binned", "jaxsplat_rasterize_backward_binned",
+)
+
+
+class Dim3(ctypes.Structure):
+    _fields_ = [("x", ctypes.c_uint32), ("y", ctypes.c_uint32), ("z", ctypes.c_uint32)]
+
+
+class Descriptor(ctypes.Structure):
+    """Mirror of ``jaxsplat_descriptor`` (80 bytes, reference ``lib/ops.h:9-22``)."""
+    _fields_ = [
+        ("num_points", ctypes.c_uint32),
+        ("img_shape", Dim3),
+        ("intrins", ctypes.c_float * 4),      # offset 16 (16-byte aligned in C)
+        ("glob_scale", ctypes.c_float),
+        ("clip_thresh", ctypes.c_float),
+        ("block_width", ctypes.c_uint32),
+        ("grid_dim_1d", ctypes.c_uint32),
+        ("block_dim_1d", ctypes.c_uint32),
+        ("grid_dim_2d", Dim3),
+        ("block_dim_2d", Dim3),
+        ("_pad", ctypes.c_uint32),
+    ]
+
+
+assert ctypes.sizeof(Descriptor) == 80
+
+
+class JaxsplatError(RuntimeError):
+    def __init__(self, code, message):
+        super().__init__(f"jaxsplat_amd error {code}: {message}")
+        self.code = code
+
+
+_lock = threading.Lock()
+_lib = None
+
+P = ctypes.c_void_p
+U32 = ctypes.c_uint32
+I32 = ctypes.c_int
+F32 = ctypes.c_float
+SZ = ctypes.c_size_t
+DP = ctypes.POINTER(Descriptor)
+
+_SIGS = {
+    "jaxsplat_version": (ctypes.c_char_p, []),
+    "jaxsplat_last_error": (ctypes.c_char_p, []),
+    "jaxsplat_make_descriptor": (I32, [DP, U32, U32, U32, F32, F32, F32, F32, F32, F32, U32]),
+    "jaxsplat_project_fwd": (None, [P, ctypes.POINTER(P), ctypes.c_char_p, SZ]),
+    "jaxsplat_project_bwd": (None, [P, ctypes.POINTER(P), ctypes.c_char_p, SZ]),
+    "jaxsplat_rasterize_fwd": (None, [P, ctypes.POINTER(P), ctypes.c_char_p, SZ]),
+    "jaxsplat_rasterize_bwd": (None, [P, ctypes.POINTER(P), ctypes.c_char_p, SZ]),
+    "jaxsplat_project_workspace_size": (SZ, [DP]),
+    "jaxsplat_project_forward": (I32, [DP] + [P] * 12 + [P, SZ, P]),
+    "jaxsplat_project_backward": (I32, [DP] + [P] * 16 + [I32, P]),
+    "jaxsplat_bin_workspace_size": (SZ, [DP, U32]),
+    "jaxsplat_bin_and_sort": (I32, [DP, P, P, P, U32, P, P, P, P, SZ, P]),
+    "jaxsplat_bin_check": (I32, [DP, U32, P, P]),
+    "jaxsplat_rasterize_forward_binned": (I32, [DP, P, P, U32] + [P] * 8 + [P]),
+    "jaxsplat_rasterize_backward_binned": (I32, [DP, P, P, U32] + [P] * 14 + [I32, P]),
+}
+
+
+def lib():
+    """Load (once) and return the ctypes library; raises if it was not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    with _lock:
+        if _lib is None:
+            if not LIB_PATH.exists():
+                raise ImportError(
+                    f"{LIB_PATH} is missing: build it with `python -m jaxsplat_amd._build` "
+                    "(hipcc, gfx950).  jaxsplat_amd has no CPU fallback.")
+            L = ctypes.CDLL(str(LIB_PATH))
+            for name, (res, args) in _SIGS.items():
+                fn = getattr(L, name)
+                fn.restype = res
+                fn.argtypes = args
+            _lib = L
+    return _lib
+
+
+def check(code):
+    if code != OK:
+        raise JaxsplatError(code, lib().jaxsplat_last_error().decode())
+
+
+def make_descriptor(num_points, img_shape, f, c, glob_scale, clip_thresh, block_width):
+    d = Descriptor()
+    H, W = img_shape
+    code = lib().jaxsplat_make_descriptor(ctypes.byref(d), int(num_points), int(H), int(W),
+                                          float(f[0]), float(f[1]), float(c[0]), float(c[1]),
+                                          float(glob_scale), float(clip_thresh),
+                                          int(block_width))
+    if code != OK:
+        raise ValueError(lib().jaxsplat_last_error().decode())
+    return d

@@ -1,0 +1,222 @@
+"""Differentiable ``project`` / ``rasterize`` on torch device memory.
+
+Mirrors the reference's ``jax.custom_vjp`` pair for each op
+(``jaxsplat/_project/__init__.py:56-208`` and
+``jaxsplat/_rasterize/__init__.py:40-190``) with ``torch.autograd.Function``:
+the forward calls the HIP kernels through the C ABI, saves residuals, and the
+backward calls the hand-written adjoint kernels.  Cotangent routing follows the
+reference: ``project`` is differentiable in means3d, scales, quats (``None``
+for viewmat, ``_project/__init__.py:205``); ``rasterize`` in colors,
+opacities, xys, conics (``None`` for background, depths, radii,
+cum_tiles_hit, ``_rasterize/__init__.py:187``).  The dropped outputs
+(compensation, img_alpha) carry zero cotangents, as in the reference.
+
+Difference in residuals (same results): the sorted Gaussian ids and tile bins
+of the forward are kept for the backward instead of sorting twice
+(``lib/ops.cu:188-198``), and cov3d is recomputed in the backward instead of
+being stored.
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import torch
+
+from . import _lib
+
+
+class ShapeDtypeError(AssertionError, TypeError):
+    """Raised where the reference's abstract evaluation asserts (``_types.py:30-33``)."""
+
+
+def _expect(name, t, shape, dtype):
+    if not isinstance(t, torch.Tensor):
+        raise ShapeDtypeError(f"{name}: expected a torch.Tensor, got {type(t).__name__}")
+    if tuple(t.shape) != tuple(shape) or t.dtype != dtype:
+        raise ShapeDtypeError(
+            f"{name}: expected shape {tuple(shape)} {dtype}, got {tuple(t.shape)} {t.dtype}")
+    if t.device.type != "cuda":
+        raise ShapeDtypeError(f"{name}: expected a GPU tensor, got device {t.device}")
+
+
+def _p(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def _stream(device):
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def _adjoint(mode):
+    try:
+        return _lib.ADJOINT[mode]
+    except KeyError:
+        raise ValueError(f"adjoint must be one of {sorted(_lib.ADJOINT)}, got {mode!r}") from None
+
+
+class _Project(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, cfg, means3d, scales, quats, viewmat):
+        img_shape, f, c, glob_scale, clip_thresh, block_width, adjoint = cfg
+        N = means3d.shape[0]
+        dev = means3d.device
+        d = _lib.make_descriptor(N, img_shape, f, c, glob_scale, clip_thresh, block_width)
+        L = _lib.lib()
+        xys = torch.empty((N, 2), dtype=torch.float32, device=dev)
+        depths = torch.empty((N, 1), dtype=torch.float32, device=dev)
+        radii = torch.empty((N, 1), dtype=torch.int32, device=dev)
+        conics = torch.empty((N, 3), dtype=torch.float32, device=dev)
+        nth = torch.empty((N, 1), dtype=torch.int32, device=dev)
+        cum = torch.empty((N, 1), dtype=torch.int32, device=dev)
+        ws_bytes = L.jaxsplat_project_workspace_size(ctypes.byref(d))
+        ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=dev)
+        with torch.cuda.device(dev):
+            _lib.check(L.jaxsplat_project_forward(
+                ctypes.byref(d), _p(means3d), _p(scales), _p(quats), _p(viewmat), None,
+                _p(xys), _p(depths), _p(radii), _p(conics), None, _p(nth), _p(cum),
+                _p(ws), ws_bytes, _stream(dev)))
+        ctx.d = d
+        ctx.adjoint = adjoint
+        ctx.save_for_backward(means3d, scales, quats, viewmat, radii, conics)
+        ctx.mark_non_differentiable(radii, nth, cum)
+        return xys, depths, radii, conics, nth, cum
+
+    @staticmethod
+    def backward(ctx, v_xys, v_depths, _v_radii, v_conics, _v_nth, _v_cum):
+        means3d, scales, quats, viewmat, radii, conics = ctx.saved_tensors
+        N = means3d.shape[0]
+        dev = means3d.device
+        v_xys = torch.zeros((N, 2), dtype=torch.float32, device=dev) if v_xys is None \
+            else v_xys.contiguous()
+        v_conics = torch.zeros((N, 3), dtype=torch.float32, device=dev) if v_conics is None \
+            else v_conics.contiguous()
+        v_depths = v_depths.contiguous() if v_depths is not None else None
+        v_mean3d = torch.empty_like(means3d)
+        v_scale = torch.empty_like(scales)
+        v_quat = torch.empty_like(quats)
+        L = _lib.lib()
+        with torch.cuda.device(dev):
+            _lib.check(L.jaxsplat_project_backward(
+                ctypes.byref(ctx.d), _p(means3d), _p(scales), _p(quats), _p(viewmat),
+                _p(radii), _p(conics), None, _p(v_xys), _p(v_depths), _p(v_conics), None,
+                _p(v_mean3d), _p(v_scale), _p(v_quat), None, None, ctx.adjoint,
+                _stream(dev)))
+        return None, v_mean3d, v_scale, v_quat, None
+
+
+def project(means3d, scales, quats, viewmat, *, img_shape, f, c, glob_scale, clip_thresh,
+            block_width, adjoint="reference"):
+    """Project 3D Gaussians to the image plane (reference ``_project/__init__.py:56-87``).
+
+    Returns ``(xys (N,2), depths (N,1), radii (N,1) int32, conics (N,3),
+    num_tiles_hit (N,1) int32, cum_tiles_hit (N,1) int32)``.  The reference
+    types the two counts uint32; torch's uint32 lacks arithmetic, so they are
+    int32 here (the values are identical, all < 2^31).
+    """
+    N = means3d.shape[0] if isinstance(means3d, torch.Tensor) else -1
+    _expect("means3d", means3d, (N, 3), torch.float32)
+    _expect("scales", scales, (N, 3), torch.float32)
+    _expect("quats", quats, (N, 4), torch.float32)
+    _expect("viewmat", viewmat, (4, 4), torch.float32)
+    cfg = (tuple(img_shape), tuple(f), tuple(c), float(glob_scale), float(clip_thresh),
+           int(block_width), _adjoint(adjoint))
+    return _Project.apply(cfg, means3d.contiguous(), scales.contiguous(), quats.contiguous(),
+                          viewmat.contiguous())
+
+
+def bin_and_sort(xys, depths, radii, cum_tiles_hit, *, img_shape, block_width,
+                 return_tiles=False):
+    """Run the binning stage alone (tests, tools).  Returns
+    ``(gaussian_ids_sorted, tile_bins (T,2)[, tile_ids_sorted], num_intersects)``."""
+    N = xys.shape[0]
+    dev = xys.device
+    d = _lib.make_descriptor(N, img_shape, (1.0, 1.0), (0.0, 0.0), 1.0, 0.0, block_width)
+    isect = int(cum_tiles_hit.reshape(-1)[-1].item()) if N > 0 else 0
+    return _bin(d, xys, depths, radii, isect, dev, return_tiles)
+
+
+def _bin(d, xys, depths, radii, isect, dev, return_tiles=False):
+    L = _lib.lib()
+    ntiles = d.grid_dim_2d.x * d.grid_dim_2d.y
+    gids = torch.empty(max(isect, 1), dtype=torch.int32, device=dev)
+    tiles = torch.empty(max(isect, 1), dtype=torch.int32, device=dev) if return_tiles else None
+    bins = torch.empty((ntiles, 2), dtype=torch.int32, device=dev)
+    ws_bytes = L.jaxsplat_bin_workspace_size(ctypes.byref(d), isect)
+    ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=dev)
+    with torch.cuda.device(dev):
+        _lib.check(L.jaxsplat_bin_and_sort(
+            ctypes.byref(d), _p(xys), _p(depths), _p(radii), isect, _p(gids), _p(tiles),
+            _p(bins), _p(ws), ws_bytes, _stream(dev)))
+    out = (gids[:isect], bins)
+    if return_tiles:
+        out = out + (tiles[:isect],)
+    return out + (isect,)
+
+
+class _Rasterize(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, cfg, colors, opacities, background, xys, depths, radii, conics,
+                cum_tiles_hit):
+        img_shape, block_width, adjoint = cfg
+        N = colors.shape[0]
+        dev = colors.device
+        H, W = img_shape
+        d = _lib.make_descriptor(N, img_shape, (1.0, 1.0), (0.0, 0.0), 1.0, 0.0, block_width)
+        isect = int(cum_tiles_hit[-1, 0].item()) if N > 0 else 0
+        gids, bins, _ = _bin(d, xys, depths, radii, isect, dev)
+        final_Ts = torch.empty((H, W, 1), dtype=torch.float32, device=dev)
+        final_idx = torch.empty((H, W, 1), dtype=torch.int32, device=dev)
+        img = torch.empty((H, W, 3), dtype=torch.float32, device=dev)
+        L = _lib.lib()
+        with torch.cuda.device(dev):
+            _lib.check(L.jaxsplat_rasterize_forward_binned(
+                ctypes.byref(d), _p(gids), _p(bins), isect, _p(xys), _p(conics), _p(colors),
+                _p(opacities), _p(background), _p(final_Ts), _p(final_idx), _p(img),
+                _stream(dev)))
+        ctx.d = d
+        ctx.isect = isect
+        ctx.adjoint = adjoint
+        ctx.save_for_backward(colors, opacities, background, xys, conics, gids, bins, final_Ts,
+                              final_idx)
+        return img
+
+    @staticmethod
+    def backward(ctx, v_img):
+        colors, opacities, background, xys, conics, gids, bins, final_Ts, final_idx = \
+            ctx.saved_tensors
+        dev = colors.device
+        v_img = v_img.contiguous()
+        v_colors = torch.empty_like(colors)
+        v_opacity = torch.empty_like(opacities)
+        v_xy = torch.empty_like(xys)
+        v_conic = torch.empty_like(conics)
+        L = _lib.lib()
+        with torch.cuda.device(dev):
+            _lib.check(L.jaxsplat_rasterize_backward_binned(
+                ctypes.byref(ctx.d), _p(gids), _p(bins), ctx.isect, _p(xys), _p(conics),
+                _p(colors), _p(opacities), _p(background), _p(final_Ts), _p(final_idx),
+                _p(v_img), None, _p(v_colors), _p(v_opacity), _p(v_xy), None, _p(v_conic),
+                ctx.adjoint, _stream(dev)))
+        return None, v_colors, v_opacity, None, v_xy, None, None, v_conic, None
+
+
+def rasterize(colors, opacities, background, xys, depths, radii, conics, cum_tiles_hit, *,
+              img_shape, block_width, adjoint="reference"):
+    """Rasterize projected Gaussians (reference ``_rasterize/__init__.py:40-68``).
+
+    Returns the (H, W, 3) image.
+    """
+    N = colors.shape[0] if isinstance(colors, torch.Tensor) else -1
+    _expect("colors", colors, (N, 3), torch.float32)
+    _expect("opacities", opacities, (N, 1), torch.float32)
+    _expect("background", background, (3,), torch.float32)
+    _expect("xys", xys, (N, 2), torch.float32)
+    _expect("depths", depths, (N, 1), torch.float32)
+    _expect("radii", radii, (N, 1), torch.int32)
+    _expect("conics", conics, (N, 3), torch.float32)
+    _expect("cum_tiles_hit", cum_tiles_hit, (N, 1), torch.int32)
+    cfg = (tuple(img_shape), int(block_width), _adjoint(adjoint))
+    return _Rasterize.apply(cfg, colors.contiguous(), opacities.contiguous(),
+                            background.contiguous(), xys.contiguous(), depths.contiguous(),
+                            radii.contiguous(), conics.contiguous(), cum_tiles_hit.contiguous())

@@ -1,0 +1,234 @@
+// Tile binning for gfx950: intersection emission, the (tile, depth) sort and
+// tile bin edges.
+//
+// Replaces the reference's ops::sort_and_bin (lib/ops.cu:645-793):
+// map_gaussian_to_intersects (lib/kernels/forward.cu:135-179) emits
+// key = (tile << 32) | depth_bits per (Gaussian, tile), CUB sorts the 64-bit
+// keys stably on [0, 32 + msb) and get_tile_bin_edges (forward.cu:184-208)
+// marks [start, end) per tile.  Because emission is in ascending Gaussian id,
+// the reference's final order is the lexicographic order
+// (tile, depth bits, gaussian id).
+//
+// This implementation produces exactly that order with far less sort traffic:
+//  1. depth_keys: one 32-bit key per Gaussian (depth bits, or 0xFFFFFFFF for
+//     Gaussians with radius 0, which emit nothing);
+//  2. a stable onesweep sort of the N Gaussians by depth key (values = index),
+//     giving the (depth bits, gaussian id) order;
+//  3. emit: a single-pass kernel walks the Gaussians in that order, scans
+//     their tile counts with a decoupled look-back and writes
+//     (tile id, gaussian id) pairs; within a workgroup the slots are spread
+//     evenly over lanes (binary search over the workgroup's offsets), so a
+//     Gaussian covering hundreds of tiles does not serialise one lane;
+//  4. a stable onesweep sort of the pairs on the tile id bits only
+//     (2 passes at 1080p/16x16 instead of 6 passes over 64-bit keys);
+//  5. tile_bins_kernel: [start, end) edges from the sorted tile ids, empty
+//     tiles (0, 0).
+// Stability of step 4 on input already ordered by (depth bits, id) yields
+// (tile, depth bits, id): identical to the reference order.
+#include "common.hpp"
+
+namespace jsa {
+namespace {
+
+__global__ void depth_keys_kernel(int n, const float* __restrict__ depths,
+                                  const int* __restrict__ radii, uint32_t* __restrict__ keys) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    keys[g] = radii[g] > 0 ? __float_as_uint(depths[g]) : 0xFFFFFFFFu;
+}
+
+// forward.cu:152-161 / helpers.h:9-32, recomputed from xys and the integer radius.
+__device__ __forceinline__ void tile_rect(const Camera& c, float2 xy, int rad, int& x0, int& x1,
+                                          int& y0, int& y1) {
+    const float bwf = (float)c.block_width;
+    const float tcx = xy.x / bwf, tcy = xy.y / bwf, tr = (float)rad / bwf;
+    x0 = min(max(0, (int)(tcx - tr)), c.tiles_x);
+    x1 = min(max(0, (int)(tcx + tr + 1.f)), c.tiles_x);
+    y0 = min(max(0, (int)(tcy - tr)), c.tiles_y);
+    y1 = min(max(0, (int)(tcy + tr + 1.f)), c.tiles_y);
+}
+
+__global__ __launch_bounds__(256) void emit_kernel(
+    Camera c, const uint32_t* __restrict__ perm, const float2* __restrict__ xys,
+    const int* __restrict__ radii, int isect_cap, uint32_t* __restrict__ tile_out,
+    uint32_t* __restrict__ gid_out, uint64_t* __restrict__ status, uint32_t* __restrict__ counter,
+    uint32_t* __restrict__ err) {
+    __shared__ uint32_t s_part;
+    __shared__ uint32_t s_wave_tot[4];
+    __shared__ uint64_t s_excl;
+    __shared__ uint32_t s_off[256];
+    __shared__ uint32_t s_gid[256];
+    __shared__ int s_x0[256], s_y0[256], s_w[256];
+    const int tid = threadIdx.x;
+    const int lane = lane_id();
+    const int wave = tid >> 6;
+    if (tid == 0) s_part = atomicAdd(counter, 1u);
+    __syncthreads();
+    const int part = (int)s_part;
+    const int r = part * 256 + tid;
+
+    int count = 0, x0 = 0, y0 = 0, w = 1;
+    uint32_t g = 0;
+    if (r < c.num_points) {
+        g = perm[r];
+        const int rad = radii[g];
+        if (rad > 0) {
+            int x1, y1;
+            tile_rect(c, xys[g], rad, x0, x1, y0, y1);
+            w = max(x1 - x0, 1);
+            count = (x1 - x0) * (y1 - y0);
+        }
+    }
+    const int incl = wave_inclusive_sum(count);
+    if (lane == 63) s_wave_tot[wave] = (uint32_t)incl;
+    __syncthreads();
+    uint32_t wave_off = 0, total = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t v = s_wave_tot[k];
+        wave_off += (k < wave) ? v : 0u;
+        total += v;
+    }
+    s_off[tid] = wave_off + (uint32_t)(incl - count);
+    s_gid[tid] = g;
+    s_x0[tid] = x0;
+    s_y0[tid] = y0;
+    s_w[tid] = w;
+    if (tid == 0) {
+        store_agent(status + part, (part == 0 ? kFlagPre : kFlagAgg) | (uint64_t)total);
+        if (part == 0) s_excl = 0;
+    }
+    if (wave == 0 && part > 0) {
+        const uint64_t excl = lookback64(status, part, err);
+        if (lane == 0) {
+            store_agent(status + part, kFlagPre | (excl + total));
+            s_excl = excl;
+        }
+    }
+    __syncthreads();
+    const uint64_t gbase = s_excl;
+    for (uint32_t s = tid; s < total; s += 256) {
+        // last entry with offset <= s (entries with zero count share the next offset)
+        int lo = 0, hi = 255;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (s_off[mid] <= s) lo = mid; else hi = mid - 1;
+        }
+        const int k = (int)(s - s_off[lo]);
+        const int wv = s_w[lo];
+        const int tyy = s_y0[lo] + k / wv;
+        const int txx = s_x0[lo] + k - (k / wv) * wv;
+        const uint64_t pos = gbase + s;
+        if (pos < (uint64_t)isect_cap) {
+            tile_out[pos] = (uint32_t)(tyy * c.tiles_x + txx);
+            gid_out[pos] = s_gid[lo];
+        }
+    }
+}
+
+__global__ void tile_bins_kernel(const uint32_t* __restrict__ tiles, int n_host,
+                                 const int* __restrict__ n_dev, int2* __restrict__ bins) {
+    const int n = n_dev ? *n_dev : n_host;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t t = tiles[i];
+    if (i == 0) bins[t].x = 0;
+    if (i == n - 1) bins[t].y = n;
+    if (i > 0) {
+        const uint32_t p = tiles[i - 1];
+        if (p != t) {
+            bins[p].y = i;
+            bins[t].x = i;
+        }
+    }
+}
+
+constexpr size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+struct BinLayout {
+    // zeroed region
+    size_t err, emit_counter, emit_status, depth_sort, tile_sort, zero_end;
+    // plain region
+    size_t dkeys, dtmp_k, dtmp_v, dsorted_k, perm, tiles, gids, ttmp_k, ttmp_v, tiles_sorted, end;
+};
+
+BinLayout layout(const BinPlan& p) {
+    BinLayout L;
+    size_t o = 0;
+    L.err = o; o += 16;
+    L.emit_counter = o; o += 16;
+    L.emit_status = o; o = align256(o + 8ull * ((p.n + 255) / 256 + 1));
+    L.depth_sort = o; o = align256(o + p.depth_sort.scratch_bytes);
+    L.tile_sort = o; o = align256(o + p.tile_sort.scratch_bytes);
+    L.zero_end = o;
+    const size_t n4 = align256(4ull * (p.n > 0 ? p.n : 1));
+    const size_t i4 = align256(4ull * (p.isect_cap > 0 ? p.isect_cap : 1));
+    L.dkeys = o; o += n4;
+    L.dtmp_k = o; o += n4;
+    L.dtmp_v = o; o += n4;
+    L.dsorted_k = o; o += n4;
+    L.perm = o; o += n4;
+    L.tiles = o; o += i4;
+    L.gids = o; o += i4;
+    L.ttmp_k = o; o += i4;
+    L.ttmp_v = o; o += i4;
+    L.tiles_sorted = o; o += i4;
+    L.end = o;
+    return L;
+}
+
+int bitlen(uint32_t v) { return v ? 32 - __builtin_clz(v) : 0; }
+
+}  // namespace
+
+BinPlan bin_plan(const Camera& c, int isect_cap) {
+    BinPlan p;
+    p.n = c.num_points;
+    p.isect_cap = isect_cap;
+    p.num_tiles = c.tiles_x * c.tiles_y;
+    p.tile_bits = bitlen((uint32_t)(p.num_tiles > 1 ? p.num_tiles - 1 : 1));
+    p.depth_sort = sort_plan(p.n, 0, 32);
+    p.tile_sort = sort_plan(isect_cap, 0, p.tile_bits);
+    p.scratch_bytes = layout(p).end;
+    return p;
+}
+
+const uint32_t* bin_error_word(const BinPlan& p, const void* scratch) {
+    return reinterpret_cast<const uint32_t*>(static_cast<const uint8_t*>(scratch) + layout(p).err);
+}
+
+hipError_t bin_and_sort(const BinPlan& p, const Camera& c, const float* xys, const float* depths,
+                        const int* radii, int num_isect_host, const int* num_isect_dev,
+                        int* gids_sorted, int* tiles_sorted, int* tile_bins, void* scratch,
+                        hipStream_t s) {
+    const BinLayout L = layout(p);
+    uint8_t* b = static_cast<uint8_t*>(scratch);
+    hipError_t e = fill_zero(b, L.zero_end, s);
+    if (e != hipSuccess) return e;
+    e = fill_zero(tile_bins, 8ull * p.num_tiles, s);
+    if (e != hipSuccess) return e;
+    if (p.n <= 0) return hipSuccess;
+    uint32_t* err = reinterpret_cast<uint32_t*>(b + L.err);
+    auto U = [&](size_t off) { return reinterpret_cast<uint32_t*>(b + off); };
+
+    depth_keys_kernel<<<(p.n + 255) / 256, 256, 0, s>>>(p.n, depths, radii, U(L.dkeys));
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    e = sort_pairs(p.depth_sort, U(L.dkeys), nullptr, U(L.dsorted_k), U(L.perm), U(L.dtmp_k),
+                   U(L.dtmp_v), p.n, nullptr, b + L.depth_sort, err, s);
+    if (e != hipSuccess) return e;
+    emit_kernel<<<(p.n + 255) / 256, 256, 0, s>>>(
+        c, U(L.perm), reinterpret_cast<const float2*>(xys), radii, p.isect_cap, U(L.tiles),
+        U(L.gids), reinterpret_cast<uint64_t*>(b + L.emit_status), U(L.emit_counter), err);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    uint32_t* tiles_out = tiles_sorted ? reinterpret_cast<uint32_t*>(tiles_sorted) : U(L.tiles_sorted);
+    e = sort_pairs(p.tile_sort, U(L.tiles), U(L.gids), tiles_out,
+                   reinterpret_cast<uint32_t*>(gids_sorted), U(L.ttmp_k), U(L.ttmp_v),
+                   num_isect_host, num_isect_dev, b + L.tile_sort, err, s);
+    if (e != hipSuccess) return e;
+    const int nb = (p.isect_cap + 255) / 256;
+    tile_bins_kernel<<<nb > 0 ? nb : 1, 256, 0, s>>>(tiles_out, num_isect_host, num_isect_dev,
+                                                   reinterpret_cast<int2*>(tile_bins));
+    return hipGetLastError();
+}
+
+}  // namespace jsa

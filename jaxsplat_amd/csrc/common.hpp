@@ -1,0 +1,230 @@
+// Shared device utilities for the gfx950 (CDNA4) Gaussian-splatting kernels:
+// wave64 reductions on DPP, ballot helpers, the decoupled look-back status
+// words used by the single-pass scans and the onesweep radix sort, and the
+// internal host-side launch interface shared by the .hip translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+
+namespace jsa {
+
+constexpr int kWave = 64;
+constexpr int kMaxBlockWidth = 16;                 // tile side, bw*bw <= 256 threads
+constexpr int kMaxTileThreads = kMaxBlockWidth * kMaxBlockWidth;
+constexpr uint32_t kSpinLimit = 1u << 22;          // bounded look-back waits
+
+// ---------------------------------------------------------------------------
+// Error codes shared with the C ABI
+// ---------------------------------------------------------------------------
+enum Status : int {
+    kOk = 0,
+    kErrInvalid = 1,
+    kErrHip = 2,
+    kErrCapacity = 3,
+    kErrWorkspace = 4,
+};
+
+// ---------------------------------------------------------------------------
+// Lane helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+
+// DPP controls (GFX9 encoding).
+constexpr int kDppQuadPerm1032 = 0xB1;   // quad_perm [1,0,3,2]
+constexpr int kDppQuadPerm2301 = 0x4E;   // quad_perm [2,3,0,1]
+constexpr int kDppRowRor4 = 0x124;
+constexpr int kDppRowRor8 = 0x128;
+constexpr int kDppRowBcast15 = 0x142;
+constexpr int kDppRowBcast31 = 0x143;
+
+template <int CTRL, int ROW_MASK = 0xF>
+__device__ __forceinline__ float dpp_f(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(
+        0, __builtin_bit_cast(int, v), CTRL, ROW_MASK, 0xF, false));
+}
+template <int CTRL, int ROW_MASK = 0xF>
+__device__ __forceinline__ int dpp_i(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, ROW_MASK, 0xF, false);
+}
+
+// Sum over all 64 lanes; the total ends up in lane 63.  Requires a full EXEC
+// mask (call from wave-uniform control flow).
+__device__ __forceinline__ float wave_sum_to_lane63(float v) {
+    v += dpp_f<kDppQuadPerm1032>(v);
+    v += dpp_f<kDppQuadPerm2301>(v);
+    v += dpp_f<kDppRowRor4>(v);
+    v += dpp_f<kDppRowRor8>(v);
+    v += dpp_f<kDppRowBcast15, 0xA>(v);
+    v += dpp_f<kDppRowBcast31, 0xC>(v);
+    return v;
+}
+
+__device__ __forceinline__ float wave_sum(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(
+        __builtin_bit_cast(int, wave_sum_to_lane63(v)), 63));
+}
+
+__device__ __forceinline__ int wave_max_i(int v) {
+    v = max(v, __shfl_xor(v, 1));
+    v = max(v, __shfl_xor(v, 2));
+    v = max(v, __shfl_xor(v, 4));
+    v = max(v, __shfl_xor(v, 8));
+    v = max(v, __shfl_xor(v, 16));
+    v = max(v, __shfl_xor(v, 32));
+    return v;
+}
+
+// Inclusive prefix sum over the wave (Kogge-Stone on shuffles).
+template <typename T>
+__device__ __forceinline__ T wave_inclusive_sum(T v) {
+    const int lane = lane_id();
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        T n = __shfl_up(v, o);
+        if (lane >= o) v += n;
+    }
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// Device-scope coherent loads/stores for look-back status words
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ T load_agent(const T* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ void store_agent(T* p, T v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// 64-bit status word of a single-pass scan partition:
+// bits 63..62 flag (0 = not ready, 1 = aggregate, 2 = inclusive prefix),
+// bits 61..0 value.
+constexpr uint64_t kFlagAgg = 1ull << 62;
+constexpr uint64_t kFlagPre = 2ull << 62;
+constexpr uint64_t kValMask = (1ull << 62) - 1;
+
+// Wave-parallel decoupled look-back: returns the exclusive prefix of
+// partition `part` (> 0).  Executed by one full wave; every lane returns the
+// same value.  Waits are bounded by kSpinLimit; on overflow *err is set and
+// the partial prefix is returned (the result is then invalid but the kernel
+// terminates).
+__device__ inline uint64_t lookback64(const uint64_t* status, int part, uint32_t* err) {
+    const int lane = lane_id();
+    uint64_t excl = 0;
+    int j = part - 1;
+    uint32_t spins = 0;
+    while (true) {
+        const int idx = j - lane;
+        uint64_t s = idx >= 0 ? load_agent(status + idx) : kFlagPre;
+        const uint64_t flag = s >> 62;
+        const uint64_t not_ready = ballot(flag == 0);
+        const uint64_t pre = ballot(flag == 2);
+        const uint64_t need = pre ? ((pre & (~pre + 1)) << 1) - 1 : ~0ull;
+        if (not_ready & need) {
+            if (++spins > kSpinLimit) {
+                if (lane == 0) atomicOr(err, 1u);
+                return excl;
+            }
+            __builtin_amdgcn_s_sleep(1);
+            continue;
+        }
+        uint64_t v = ((1ull << lane) & need) ? (s & kValMask) : 0;
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
+        excl += v;
+        if (pre) break;
+        j -= kWave;
+    }
+    return excl;
+}
+
+// ---------------------------------------------------------------------------
+// Host-side internal interface (implemented in the .hip files)
+// ---------------------------------------------------------------------------
+struct Camera {
+    int num_points;
+    int img_w, img_h;
+    int block_width;
+    int tiles_x, tiles_y;
+    float fx, fy, cx, cy;
+    float tan_fovx, tan_fovy;
+    float glob_scale, clip_thresh;
+};
+
+// Zero `bytes` (multiple of 4) starting at p with a fill kernel (never a
+// hipMemsetAsync, so the call is safe inside captured graphs).
+hipError_t fill_zero(void* p, size_t bytes, hipStream_t s);
+hipError_t fill_value_f32x3(float* dst, size_t n_pixels, const float* rgb, hipStream_t s);
+
+// ---- projection ----
+size_t project_fwd_scratch_bytes(int n);
+hipError_t project_fwd(const Camera& c, const float* means3d, const float* scales,
+                       const float* quats, const float* viewmat, float* cov3d,
+                       float* xys, float* depths, int* radii, float* conics,
+                       float* compensation, int* num_tiles_hit, int* cum_tiles_hit,
+                       void* scratch, hipStream_t s);
+hipError_t project_bwd(const Camera& c, const float* means3d, const float* scales,
+                       const float* quats, const float* viewmat, const int* radii,
+                       const float* conics, const float* compensation,
+                       const float* v_xy, const float* v_depth, const float* v_conic,
+                       const float* v_compensation, float* v_mean3d, float* v_scale,
+                       float* v_quat, float* v_cov2d, float* v_cov3d, int consistent,
+                       hipStream_t s);
+
+// ---- onesweep radix sort (uint32 keys, uint32 values) ----
+struct SortPlan {
+    int n_cap;          // capacity (upper bound on n) used for launch shapes
+    int begin_bit, end_bit;
+    int passes;
+    int parts;          // partitions per pass at capacity
+    size_t scratch_bytes;
+};
+SortPlan sort_plan(int n_cap, int begin_bit, int end_bit);
+// Sorts (keys_in, vals_in) -> (keys_out, vals_out) stably on [begin, end).
+// vals_in == nullptr means values = index.  n is read from n_dev when non-null.
+// tmp_keys/tmp_vals must hold n_cap entries (ping-pong buffers).  The scratch
+// area must be zeroed by the caller (sort_zero_bytes of it) before the call.
+size_t sort_zero_bytes(const SortPlan& p);
+hipError_t sort_pairs(const SortPlan& p, const uint32_t* keys_in, const uint32_t* vals_in,
+                      uint32_t* keys_out, uint32_t* vals_out, uint32_t* tmp_keys,
+                      uint32_t* tmp_vals, int n_host, const int* n_dev, void* scratch,
+                      uint32_t* err, hipStream_t s);
+
+// ---- binning ----
+struct BinPlan {
+    int n;              // Gaussians
+    int isect_cap;      // intersection capacity
+    int num_tiles;
+    int tile_bits;
+    SortPlan depth_sort;
+    SortPlan tile_sort;
+    size_t scratch_bytes;
+};
+BinPlan bin_plan(const Camera& c, int isect_cap);
+hipError_t bin_and_sort(const BinPlan& p, const Camera& c, const float* xys,
+                        const float* depths, const int* radii, int num_isect_host,
+                        const int* num_isect_dev, int* gids_sorted, int* tiles_sorted,
+                        int* tile_bins, void* scratch, hipStream_t s);
+// status word of the last bin_and_sort using `scratch` (non-zero = look-back timeout)
+const uint32_t* bin_error_word(const BinPlan& p, const void* scratch);
+
+// ---- rasterization ----
+hipError_t raster_fwd(const Camera& c, const int* gids_sorted, const int* tile_bins,
+                      const float* xys, const float* conics, const float* colors,
+                      const float* opacities, const float* background, float* final_Ts,
+                      int* final_idx, float* out_img, hipStream_t s);
+hipError_t raster_bwd(const Camera& c, const int* gids_sorted, const int* tile_bins,
+                      const float* xys, const float* conics, const float* colors,
+                      const float* opacities, const float* background,
+                      const float* final_Ts, const int* final_idx, const float* v_out,
+                      const float* v_out_alpha, float* v_colors, float* v_opacity,
+                      float* v_xy, float* v_xy_abs, float* v_conic, int consistent,
+                      hipStream_t s);
+
+}  // namespace jsa

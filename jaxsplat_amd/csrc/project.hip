@@ -1,0 +1,480 @@
+// Per-Gaussian projection, forward and backward, for gfx950.
+//
+// Forward semantics follow the reference kernel project_gaussians_fwd
+// (lib/kernels/forward.cu:13-131) with scale_rot_to_cov3d (forward.cu:547-568),
+// project_cov3d_ewa (forward.cu:465-544) and the helpers of
+// lib/kernels/helpers.h:9-57,101-153,194-211.  Backward follows
+// project_gaussians_bwd (lib/kernels/backward.cu:345-418) with
+// project_cov3d_ewa_vjp (:421-524), scale_rot_to_cov3d_vjp (:528-570) and the
+// VJP helpers (helpers.h:60-98,131-138,155-192).
+//
+// This translation unit is compiled with -ffp-contract=off: every float
+// operation is rounded on its own, in the order written, so the float32
+// restatement of the oracle reproduces the forward bit for bit.  Products of
+// 3x3 matrices always sum over the inner index in the order 0, 1, 2.
+//
+// Design:
+//  * one lane per Gaussian, 256-lane workgroups; outputs of every Gaussian are
+//    written explicitly (culled ones get zeros), so no output memsets are
+//    needed;
+//  * the inclusive scan of num_tiles_hit (cum_tiles_hit) is fused into the
+//    forward kernel as a single-pass decoupled look-back scan: partitions are
+//    taken from an atomic counter, a wave publishes its aggregate and looks
+//    back over 64 predecessors at a time;
+//  * cov3d and compensation outputs are optional (nullptr skips the store);
+//    the backward recomputes cov3d from scales/quats.
+#include "common.hpp"
+
+namespace jsa {
+namespace {
+
+struct M3 {
+    float m[3][3];  // [row][col]
+};
+
+__device__ __forceinline__ M3 mm(const M3& a, const M3& b) {
+    M3 r;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            r.m[i][j] = a.m[i][0] * b.m[0][j] + a.m[i][1] * b.m[1][j] + a.m[i][2] * b.m[2][j];
+    return r;
+}
+
+__device__ __forceinline__ M3 tr(const M3& a) {
+    M3 r;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) r.m[i][j] = a.m[j][i];
+    return r;
+}
+
+// helpers.h:140-153; quaternion (w, x, y, z), not normalised.
+__device__ __forceinline__ M3 quat_to_rotmat(float w, float x, float y, float z) {
+    M3 R;
+    R.m[0][0] = 1.f - 2.f * (y * y + z * z);
+    R.m[0][1] = 2.f * (x * y - w * z);
+    R.m[0][2] = 2.f * (x * z + w * y);
+    R.m[1][0] = 2.f * (x * y + w * z);
+    R.m[1][1] = 1.f - 2.f * (x * x + z * z);
+    R.m[1][2] = 2.f * (y * z - w * x);
+    R.m[2][0] = 2.f * (x * z - w * y);
+    R.m[2][1] = 2.f * (y * z + w * x);
+    R.m[2][2] = 1.f - 2.f * (x * x + y * y);
+    return R;
+}
+
+__device__ __forceinline__ M3 scale_mat(float sx, float sy, float sz, float g) {
+    M3 S;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) S.m[i][j] = 0.f;
+    S.m[0][0] = g * sx;
+    S.m[1][1] = g * sy;
+    S.m[2][2] = g * sz;
+    return S;
+}
+
+// forward.cu:547-568: upper triangle of (R S)(R S)^T.
+__device__ __forceinline__ void cov3d_of(const float* s, const float* q, float g, float cov[6]) {
+    const M3 R = quat_to_rotmat(q[0], q[1], q[2], q[3]);
+    const M3 S = scale_mat(s[0], s[1], s[2], g);
+    const M3 M = mm(R, S);
+    const M3 V = mm(M, tr(M));
+    cov[0] = V.m[0][0];
+    cov[1] = V.m[0][1];
+    cov[2] = V.m[0][2];
+    cov[3] = V.m[1][1];
+    cov[4] = V.m[1][2];
+    cov[5] = V.m[2][2];
+}
+
+__device__ __forceinline__ void load_view(const float* __restrict__ vm, M3& W, float p[3]) {
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) W.m[r][c] = vm[4 * r + c];
+        p[r] = vm[4 * r + 3];
+    }
+}
+
+// helpers.h:101-108
+__device__ __forceinline__ void transform(const M3& W, const float p[3], float x, float y,
+                                          float z, float t[3]) {
+#pragma unroll
+    for (int r = 0; r < 3; ++r) t[r] = W.m[r][0] * x + W.m[r][1] * y + W.m[r][2] * z + p[r];
+}
+
+__device__ __forceinline__ M3 sym3(const float c[6]) {
+    M3 V;
+    V.m[0][0] = c[0]; V.m[0][1] = c[1]; V.m[0][2] = c[2];
+    V.m[1][0] = c[1]; V.m[1][1] = c[3]; V.m[1][2] = c[4];
+    V.m[2][0] = c[2]; V.m[2][1] = c[4]; V.m[2][2] = c[5];
+    return V;
+}
+
+__device__ __forceinline__ M3 ewa_jacobian(float fx, float fy, float tx, float ty, float rz,
+                                           float rz2) {
+    M3 J;
+    J.m[0][0] = fx * rz; J.m[0][1] = 0.f; J.m[0][2] = -fx * tx * rz2;
+    J.m[1][0] = 0.f; J.m[1][1] = fy * rz; J.m[1][2] = -fy * ty * rz2;
+    J.m[2][0] = 0.f; J.m[2][1] = 0.f; J.m[2][2] = 0.f;
+    return J;
+}
+
+__global__ __launch_bounds__(256) void project_fwd_kernel(
+    Camera c, const float* __restrict__ means3d, const float* __restrict__ scales,
+    const float* __restrict__ quats, const float* __restrict__ viewmat,
+    float* __restrict__ cov3d_out, float* __restrict__ xys, float* __restrict__ depths,
+    int* __restrict__ radii, float* __restrict__ conics, float* __restrict__ comp_out,
+    int* __restrict__ num_tiles_hit, int* __restrict__ cum_tiles_hit,
+    uint64_t* __restrict__ status, uint32_t* __restrict__ counter, uint32_t* __restrict__ err) {
+    __shared__ uint32_t s_part;
+    __shared__ uint32_t s_wave_tot[4];
+    __shared__ uint64_t s_excl;
+    const int tid = threadIdx.x;
+    const int lane = lane_id();
+    const int wave = tid >> 6;
+    if (tid == 0) s_part = atomicAdd(counter, 1u);
+    __syncthreads();
+    const int part = (int)s_part;
+    const int idx = part * 256 + tid;
+
+    int count = 0;
+    if (idx < c.num_points) {
+        M3 W;
+        float pt[3];
+        load_view(viewmat, W, pt);
+        const float mx = means3d[3 * idx + 0];
+        const float my = means3d[3 * idx + 1];
+        const float mz = means3d[3 * idx + 2];
+        float t[3];
+        transform(W, pt, mx, my, mz, t);                  // clip_near_plane, helpers.h:204-211
+        float cov[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        float conic[3] = {0.f, 0.f, 0.f};
+        float cx = 0.f, cy = 0.f, depth = 0.f, comp = 0.f;
+        int rad_i = 0;
+        if (!(t[2] <= c.clip_thresh)) {
+            const float s[3] = {scales[3 * idx + 0], scales[3 * idx + 1], scales[3 * idx + 2]};
+            const float4 q4 = reinterpret_cast<const float4*>(quats)[idx];
+            const float q[4] = {q4.x, q4.y, q4.z, q4.w};
+            cov3d_of(s, q, c.glob_scale, cov);
+            // project_cov3d_ewa, forward.cu:465-544
+            const float lim_x = 1.3f * c.tan_fovx;
+            const float lim_y = 1.3f * c.tan_fovy;
+            const float tx = t[2] * fminf(lim_x, fmaxf(-lim_x, t[0] / t[2]));
+            const float ty = t[2] * fminf(lim_y, fmaxf(-lim_y, t[1] / t[2]));
+            const float rz = 1.f / t[2];
+            const float rz2 = rz * rz;
+            const M3 J = ewa_jacobian(c.fx, c.fy, tx, ty, rz, rz2);
+            const M3 T = mm(J, W);
+            const M3 C2 = mm(mm(T, sym3(cov)), tr(T));
+            const float c00 = C2.m[0][0], c11 = C2.m[1][1], c01 = C2.m[1][0];
+            const float det_orig = c00 * c11 - c01 * c01;
+            const float a = c00 + 0.3f, b = c01, cc = c11 + 0.3f;
+            const float det_blur = a * cc - b * b;
+            const float comp_v = sqrtf(fmaxf(0.f, det_orig / det_blur));
+            // compute_cov2d_bounds, helpers.h:34-57
+            const float det = a * cc - b * b;
+            if (det != 0.f) {
+                const float inv_det = 1.f / det;
+                conic[0] = cc * inv_det;
+                conic[1] = -b * inv_det;
+                conic[2] = a * inv_det;
+                const float hb = 0.5f * (a + cc);
+                const float v1 = hb + sqrtf(fmaxf(0.1f, hb * hb - det));
+                const float v2 = hb - sqrtf(fmaxf(0.1f, hb * hb - det));
+                const float radius = ceilf(3.f * sqrtf(fmaxf(v1, v2)));
+                // project_pix, helpers.h:122-128
+                const float rw = 1.f / (t[2] + 1e-6f);
+                const float px = t[0] * rw * c.fx + c.cx;
+                const float py = t[1] * rw * c.fy + c.cy;
+                // get_tile_bbox, helpers.h:9-32
+                const float bwf = (float)c.block_width;
+                const float tcx = px / bwf, tcy = py / bwf, tr_ = radius / bwf;
+                const int x0 = min(max(0, (int)(tcx - tr_)), c.tiles_x);
+                const int x1 = min(max(0, (int)(tcx + tr_ + 1.f)), c.tiles_x);
+                const int y0 = min(max(0, (int)(tcy - tr_)), c.tiles_y);
+                const int y1 = min(max(0, (int)(tcy + tr_ + 1.f)), c.tiles_y);
+                const int area = (x1 - x0) * (y1 - y0);
+                if (area > 0) {
+                    count = area;
+                    depth = t[2];
+                    rad_i = (int)radius;
+                    cx = px;
+                    cy = py;
+                    comp = comp_v;
+                }
+            }
+        }
+        if (cov3d_out) {
+            float2* o = reinterpret_cast<float2*>(cov3d_out + 6 * (size_t)idx);
+            o[0] = make_float2(cov[0], cov[1]);
+            o[1] = make_float2(cov[2], cov[3]);
+            o[2] = make_float2(cov[4], cov[5]);
+        }
+        reinterpret_cast<float2*>(xys)[idx] = make_float2(cx, cy);
+        depths[idx] = depth;
+        radii[idx] = rad_i;
+        conics[3 * idx + 0] = conic[0];
+        conics[3 * idx + 1] = conic[1];
+        conics[3 * idx + 2] = conic[2];
+        if (comp_out) comp_out[idx] = comp;
+        num_tiles_hit[idx] = count;
+    }
+
+    // ---- single-pass inclusive scan of the counts (decoupled look-back) ----
+    const int incl = wave_inclusive_sum(count);
+    if (lane == 63) s_wave_tot[wave] = (uint32_t)incl;
+    __syncthreads();
+    uint32_t wave_off = 0, total = 0;
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+        const uint32_t v = s_wave_tot[w];
+        wave_off += (w < wave) ? v : 0u;
+        total += v;
+    }
+    if (tid == 0) {
+        store_agent(status + part, (part == 0 ? kFlagPre : kFlagAgg) | (uint64_t)total);
+        if (part == 0) s_excl = 0;
+    }
+    if (wave == 0 && part > 0) {
+        const uint64_t excl = lookback64(status, part, err);
+        if (lane == 0) {
+            store_agent(status + part, kFlagPre | (excl + total));
+            s_excl = excl;
+        }
+    }
+    __syncthreads();
+    if (idx < c.num_points) cum_tiles_hit[idx] = (int)(s_excl + wave_off + (uint32_t)incl);
+}
+
+// helpers.h:155-192 with vR[row][col]; the reference indexes v_R[col][row].
+__device__ __forceinline__ void quat_vjp(const float q[4], const M3& vR, float out[4]) {
+    const float w = q[0], x = q[1], y = q[2], z = q[3];
+#define VR(col, row) vR.m[row][col]
+    out[0] = 2.f * (x * (VR(1, 2) - VR(2, 1)) + y * (VR(2, 0) - VR(0, 2)) + z * (VR(0, 1) - VR(1, 0)));
+    out[1] = 2.f * (-2.f * x * (VR(1, 1) + VR(2, 2)) + y * (VR(0, 1) + VR(1, 0)) +
+                    z * (VR(0, 2) + VR(2, 0)) + w * (VR(1, 2) - VR(2, 1)));
+    out[2] = 2.f * (x * (VR(0, 1) + VR(1, 0)) - 2.f * y * (VR(0, 0) + VR(2, 2)) +
+                    z * (VR(1, 2) + VR(2, 1)) + w * (VR(2, 0) - VR(0, 2)));
+    out[3] = 2.f * (x * (VR(0, 2) + VR(2, 0)) + y * (VR(1, 2) + VR(2, 1)) -
+                    2.f * z * (VR(0, 0) + VR(1, 1)) + w * (VR(0, 1) - VR(1, 0)));
+#undef VR
+}
+
+__global__ __launch_bounds__(256) void project_bwd_kernel(
+    Camera c, const float* __restrict__ means3d, const float* __restrict__ scales,
+    const float* __restrict__ quats, const float* __restrict__ viewmat,
+    const int* __restrict__ radii, const float* __restrict__ conics,
+    const float* __restrict__ compensation, const float* __restrict__ v_xy,
+    const float* __restrict__ v_depth, const float* __restrict__ v_conic,
+    const float* __restrict__ v_compensation, float* __restrict__ v_mean3d,
+    float* __restrict__ v_scale, float* __restrict__ v_quat, float* __restrict__ v_cov2d_out,
+    float* __restrict__ v_cov3d_out, int consistent) {
+    const int idx = blockIdx.x * 256 + threadIdx.x;
+    if (idx >= c.num_points) return;
+    float vmean[3] = {0.f, 0.f, 0.f}, vsc[3] = {0.f, 0.f, 0.f}, vq[4] = {0.f, 0.f, 0.f, 0.f};
+    float vc2[3] = {0.f, 0.f, 0.f}, vc3[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    if (radii[idx] > 0) {                                      // backward.cu:369-371
+        M3 W;
+        float pt[3];
+        load_view(viewmat, W, pt);
+        const float m[3] = {means3d[3 * idx + 0], means3d[3 * idx + 1], means3d[3 * idx + 2]};
+        float t[3];
+        transform(W, pt, m[0], m[1], m[2], t);
+        // v_mean from v_xy: project_pix_vjp (helpers.h:131-138) then R^T (:90-98)
+        const float2 gxy = reinterpret_cast<const float2*>(v_xy)[idx];
+        const float rw = 1.f / (t[2] + 1e-6f);
+        const float vpx = c.fx * gxy.x, vpy = c.fy * gxy.y;
+        const float vv[3] = {vpx * rw, vpy * rw, -(vpx * t[0] + vpy * t[1]) * rw * rw};
+        const float vz = v_depth ? v_depth[idx] : 0.f;
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            vmean[k] = W.m[0][k] * vv[0] + W.m[1][k] * vv[1] + W.m[2][k] * vv[2] + W.m[2][k] * vz;
+        // v_cov2d (helpers.h:60-87)
+        const float a = conics[3 * idx + 0], b = conics[3 * idx + 1], cc = conics[3 * idx + 2];
+        const float ga = v_conic[3 * idx + 0], gb = v_conic[3 * idx + 1] * 0.5f,
+                    gc = v_conic[3 * idx + 2];
+        const float xg00 = a * ga + b * gb, xg01 = a * gb + b * gc;
+        const float xg10 = b * ga + cc * gb, xg11 = b * gb + cc * gc;
+        vc2[0] = -(xg00 * a + xg01 * b);
+        vc2[1] = -(xg00 * b + xg01 * cc) - (xg10 * a + xg11 * b);
+        vc2[2] = -(xg10 * b + xg11 * cc);
+        if (v_compensation) {
+            const float cp = compensation[idx];
+            const float inv_det = a * cc - b * b;
+            const float omc = 1.f - cp * cp;
+            const float vsq = v_compensation[idx] * 0.5f / (cp + 1e-6f);
+            vc2[0] += vsq * (omc * a - 0.3f * inv_det);
+            vc2[1] += 2.f * vsq * (omc * b);
+            vc2[2] += vsq * (omc * cc - 0.3f * inv_det);
+        }
+        // EWA VJP, backward.cu:421-524
+        const float s[3] = {scales[3 * idx + 0], scales[3 * idx + 1], scales[3 * idx + 2]};
+        const float4 q4 = reinterpret_cast<const float4*>(quats)[idx];
+        const float q[4] = {q4.x, q4.y, q4.z, q4.w};
+        float cov[6];
+        cov3d_of(s, q, c.glob_scale, cov);
+        float tx = t[0], ty = t[1];
+        bool inx = true, iny = true;
+        float cxv = 0.f, cyv = 0.f;
+        if (consistent) {
+            const float lim_x = 1.3f * c.tan_fovx, lim_y = 1.3f * c.tan_fovy;
+            const float ux = t[0] / t[2], uy = t[1] / t[2];
+            inx = ux > -lim_x && ux < lim_x;
+            iny = uy > -lim_y && uy < lim_y;
+            cxv = fminf(lim_x, fmaxf(-lim_x, ux));
+            cyv = fminf(lim_y, fmaxf(-lim_y, uy));
+            tx = t[2] * cxv;
+            ty = t[2] * cyv;
+        }
+        const float rz = 1.f / t[2];
+        const float rz2 = rz * rz;
+        const M3 J = ewa_jacobian(c.fx, c.fy, tx, ty, rz, rz2);
+        const M3 V = sym3(cov);
+        M3 G;
+        G.m[0][0] = vc2[0]; G.m[0][1] = 0.5f * vc2[1]; G.m[0][2] = 0.f;
+        G.m[1][0] = 0.5f * vc2[1]; G.m[1][1] = vc2[2]; G.m[1][2] = 0.f;
+        G.m[2][0] = 0.f; G.m[2][1] = 0.f; G.m[2][2] = 0.f;
+        const M3 T = mm(J, W);
+        const M3 vV = mm(mm(tr(T), G), T);
+        const M3 vT1 = mm(mm(G, T), tr(V));
+        const M3 vT2 = mm(mm(tr(G), T), V);
+        M3 vT;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) vT.m[i][j] = vT1.m[i][j] + vT2.m[i][j];
+        vc3[0] = vV.m[0][0];
+        vc3[1] = vV.m[0][1] + vV.m[1][0];
+        vc3[2] = vV.m[0][2] + vV.m[2][0];
+        vc3[3] = vV.m[1][1];
+        vc3[4] = vV.m[1][2] + vV.m[2][1];
+        vc3[5] = vV.m[2][2];
+        const M3 vJ = mm(vT, tr(W));
+        const float rz3 = rz2 * rz;
+        float vtx = -c.fx * rz2 * vJ.m[0][2];
+        float vty = -c.fy * rz2 * vJ.m[1][2];
+        float vtz = -c.fx * rz2 * vJ.m[0][0] + 2.f * c.fx * tx * rz3 * vJ.m[0][2] -
+                    c.fy * rz2 * vJ.m[1][1] + 2.f * c.fy * ty * rz3 * vJ.m[1][2];
+        if (consistent) {
+            vtz = vtz + (inx ? 0.f : cxv) * vtx + (iny ? 0.f : cyv) * vty;
+            vtx = inx ? vtx : 0.f;
+            vty = iny ? vty : 0.f;
+        }
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            vmean[k] = vmean[k] + (vtx * W.m[0][k] + vty * W.m[1][k] + vtz * W.m[2][k]);
+        // scale/rotation VJP, backward.cu:528-570
+        const M3 R = quat_to_rotmat(q[0], q[1], q[2], q[3]);
+        const M3 S = scale_mat(s[0], s[1], s[2], c.glob_scale);
+        const M3 M = mm(R, S);
+        M3 vVs;
+        vVs.m[0][0] = vc3[0]; vVs.m[0][1] = 0.5f * vc3[1]; vVs.m[0][2] = 0.5f * vc3[2];
+        vVs.m[1][0] = 0.5f * vc3[1]; vVs.m[1][1] = vc3[3]; vVs.m[1][2] = 0.5f * vc3[4];
+        vVs.m[2][0] = 0.5f * vc3[2]; vVs.m[2][1] = 0.5f * vc3[4]; vVs.m[2][2] = vc3[5];
+        M3 vM = mm(vVs, M);
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) vM.m[i][j] = 2.f * vM.m[i][j];
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            vsc[k] = (R.m[0][k] * vM.m[0][k] + R.m[1][k] * vM.m[1][k] + R.m[2][k] * vM.m[2][k]) *
+                     c.glob_scale;
+        const M3 vR = mm(vM, S);
+        quat_vjp(q, vR, vq);
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        v_mean3d[3 * idx + k] = vmean[k];
+        v_scale[3 * idx + k] = vsc[k];
+    }
+    reinterpret_cast<float4*>(v_quat)[idx] = make_float4(vq[0], vq[1], vq[2], vq[3]);
+    if (v_cov2d_out) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) v_cov2d_out[3 * idx + k] = vc2[k];
+    }
+    if (v_cov3d_out) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) v_cov3d_out[6 * idx + k] = vc3[k];
+    }
+}
+
+__global__ void fill_zero_kernel(uint32_t* __restrict__ p, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        p[i] = 0u;
+}
+
+__global__ void fill_rgb_kernel(float* __restrict__ dst, size_t n_pix, const float* __restrict__ rgb) {
+    const float r = rgb[0], g = rgb[1], b = rgb[2];
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n_pix;
+         i += (size_t)gridDim.x * blockDim.x) {
+        dst[3 * i + 0] = r;
+        dst[3 * i + 1] = g;
+        dst[3 * i + 2] = b;
+    }
+}
+
+}  // namespace
+
+hipError_t fill_zero(void* p, size_t bytes, hipStream_t s) {
+    const size_t n = bytes / 4;
+    if (n == 0) return hipSuccess;
+    const int blocks = (int)((n + 255) / 256 < 2048 ? (n + 255) / 256 : 2048);
+    fill_zero_kernel<<<blocks, 256, 0, s>>>(static_cast<uint32_t*>(p), n);
+    return hipGetLastError();
+}
+
+hipError_t fill_value_f32x3(float* dst, size_t n_pixels, const float* rgb, hipStream_t s) {
+    if (n_pixels == 0) return hipSuccess;
+    const int blocks = (int)((n_pixels + 255) / 256 < 2048 ? (n_pixels + 255) / 256 : 2048);
+    fill_rgb_kernel<<<blocks, 256, 0, s>>>(dst, n_pixels, rgb);
+    return hipGetLastError();
+}
+
+// scratch layout: [counter u32 | err u32 | pad to 16 | status u64 x parts]
+size_t project_fwd_scratch_bytes(int n) {
+    const size_t parts = ((size_t)n + 255) / 256;
+    return 16 + 8 * (parts > 0 ? parts : 1);
+}
+
+hipError_t project_fwd(const Camera& c, const float* means3d, const float* scales,
+                       const float* quats, const float* viewmat, float* cov3d, float* xys,
+                       float* depths, int* radii, float* conics, float* compensation,
+                       int* num_tiles_hit, int* cum_tiles_hit, void* scratch, hipStream_t s) {
+    if (c.num_points <= 0) return hipSuccess;
+    const int parts = (c.num_points + 255) / 256;
+    uint8_t* base = static_cast<uint8_t*>(scratch);
+    hipError_t e = fill_zero(base, project_fwd_scratch_bytes(c.num_points), s);
+    if (e != hipSuccess) return e;
+    uint32_t* counter = reinterpret_cast<uint32_t*>(base);
+    uint32_t* err = counter + 1;
+    uint64_t* status = reinterpret_cast<uint64_t*>(base + 16);
+    project_fwd_kernel<<<parts, 256, 0, s>>>(c, means3d, scales, quats, viewmat, cov3d, xys,
+                                             depths, radii, conics, compensation, num_tiles_hit,
+                                             cum_tiles_hit, status, counter, err);
+    return hipGetLastError();
+}
+
+hipError_t project_bwd(const Camera& c, const float* means3d, const float* scales,
+                       const float* quats, const float* viewmat, const int* radii,
+                       const float* conics, const float* compensation, const float* v_xy,
+                       const float* v_depth, const float* v_conic, const float* v_compensation,
+                       float* v_mean3d, float* v_scale, float* v_quat, float* v_cov2d,
+                       float* v_cov3d, int consistent, hipStream_t s) {
+    if (c.num_points <= 0) return hipSuccess;
+    const int blocks = (c.num_points + 255) / 256;
+    project_bwd_kernel<<<blocks, 256, 0, s>>>(c, means3d, scales, quats, viewmat, radii, conics,
+                                              compensation, v_xy, v_depth, v_conic,
+                                              v_compensation, v_mean3d, v_scale, v_quat, v_cov2d,
+                                              v_cov3d, consistent);
+    return hipGetLastError();
+}
+
+}  // namespace jsa
