@@ -40,6 +40,9 @@ def parse():
     ap.add_argument("--no-cpu-baseline", dest="cpu_baseline", action="store_false")
     ap.add_argument("--cpu-tiles", type=int, default=24)
     ap.add_argument("--stage-iters", type=int, default=10)
+    ap.add_argument("--copy-calibration-mb", type=int, default=0,
+                    help="run a device-to-device copy of this many MiB first (HBM counter "
+                         "calibration under rocprofv3 --pmc)")
     return ap.parse_args()
 
 
@@ -61,6 +64,15 @@ def main():
     _lib.lib()  # fail loudly if the HIP library is missing
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
+
+    if args.copy_calibration_mb:
+        n = args.copy_calibration_mb * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     sc = scenes.make(args.workload)
     cam = sc.camera

@@ -63,6 +63,17 @@ __device__ __forceinline__ float wave_sum_to_lane63(float v) {
     return v;
 }
 
+// Sum over each 16-lane row; every lane of a row ends with the row's total.
+__device__ __forceinline__ float row_sum16(float v) {
+    v += dpp_f<kDppQuadPerm1032>(v);
+    v += dpp_f<kDppQuadPerm2301>(v);
+    v += dpp_f<kDppRowRor4>(v);
+    v += dpp_f<kDppRowRor8>(v);
+    return v;
+}
+
+__device__ __forceinline__ int uniform_i(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
 __device__ __forceinline__ float wave_sum(float v) {
     return __builtin_bit_cast(float, __builtin_amdgcn_readlane(
         __builtin_bit_cast(int, wave_sum_to_lane63(v)), 63));

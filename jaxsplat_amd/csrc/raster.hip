@@ -40,17 +40,66 @@ __device__ __forceinline__ int xcd_tile(int b, int num_tiles) {
     return (b & 7) * per + (b >> 3);
 }
 
-__device__ __forceinline__ int threads_for(int bw) { return ((bw * bw + 63) / 64) * 64; }
+__device__ __forceinline__ uint64_t uniform64(uint64_t v) {
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
 
+// Pixel-centre box of the 64 consecutive tile pixels p0..p0+63 (row-major in
+// the bw x bw tile), clipped to the image; ok = false if none is inside.
+__device__ __forceinline__ float4 group_rect(int p0, int bw, int x_org, int y_org, int W, int H,
+                                             bool& ok) {
+    const int last = min(p0 + 63, bw * bw - 1);
+    ok = p0 <= last;
+    const int r0 = p0 / bw, r1 = last / bw;
+    const int c0 = (r0 == r1) ? p0 - r0 * bw : 0;
+    const int c1 = (r0 == r1) ? last - r1 * bw : bw - 1;
+    const int x0 = x_org + c0, x1 = min(x_org + c1, W - 1);
+    const int y0 = y_org + r0, y1 = min(y_org + r1, H - 1);
+    ok = ok && x0 <= x1 && y0 <= y1;
+    return make_float4((float)x0 + 0.5f, (float)x1 + 0.5f, (float)y0 + 0.5f, (float)y1 + 0.5f);
+}
+
+// Half extents of the axis-aligned box holding every pixel centre at which
+// o * exp(-sigma) >= 1/255, i.e. sigma <= ln(255 o).  The threshold is
+// inflated by 0.01 and the extents by 0.01% + 1e-3 px so the box also holds
+// every pixel the float32 per-pixel test accepts.  Returns false when no pixel
+// can pass (o <= ~1/255).  Degenerate conics give an unbounded box.
+__device__ __forceinline__ bool alpha_box(float a, float b, float c, float o, float& ex,
+                                          float& ey) {
+    const float smax = __logf(255.f * o) + 0.01f;
+    if (!(smax > 0.f)) return false;
+    const float det = a * c - b * b;
+    if (!(det > 0.f)) {
+        ex = ey = __builtin_inff();
+        return true;
+    }
+    const float k = 2.f * smax * __builtin_amdgcn_rcpf(det);
+    ex = sqrtf(fmaxf(k * c, 0.f)) * 1.0001f + 1e-3f;
+    ey = sqrtf(fmaxf(k * a, 0.f)) * 1.0001f + 1e-3f;
+    return true;
+}
+
+__device__ __forceinline__ bool box_hits(float gx, float gy, float ex, float ey, float4 r) {
+    return gx + ex >= r.x && gx - ex <= r.y && gy + ey >= r.z && gy - ey <= r.w;
+}
+
+// ---------------------------------------------------------------------------
+// Forward: one pixel per lane, one 64-pixel group per wave.
+// ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void raster_fwd_kernel(
     Camera c, const int* __restrict__ gids, const int2* __restrict__ bins,
     const float2* __restrict__ xys, const float* __restrict__ conics,
     const float* __restrict__ colors, const float* __restrict__ opac,
     const float* __restrict__ bg, float* __restrict__ final_Ts, int* __restrict__ final_idx,
     float* __restrict__ out_img) {
-    __shared__ float4 s_a[kMaxTileThreads];
-    __shared__ float4 s_b[kMaxTileThreads];
-    __shared__ float4 s_c[kMaxTileThreads];
+    __shared__ float4 s_a[kMaxTileThreads];   // x, y, opacity, id
+    __shared__ float4 s_b[kMaxTileThreads];   // conic a, b, c
+    __shared__ float4 s_c[kMaxTileThreads];   // colour
+    __shared__ uint64_t s_mask[4][4];         // [group][64-entry word]
+    __shared__ float4 s_rect[4];
+    __shared__ int s_rok[4];
     __shared__ int s_cnt[4];
 
     const int num_tiles = c.tiles_x * c.tiles_y;
@@ -63,12 +112,18 @@ __global__ __launch_bounds__(256) void raster_fwd_kernel(
     const int wave = tid >> 6;
     const int nwaves = nthr >> 6;
     const int ty = tile / c.tiles_x, tx = tile - ty * c.tiles_x;
+    const int x_org = tx * bw, y_org = ty * bw;
     const bool pixel = tid < bw * bw;
-    const int i = ty * bw + (pixel ? tid / bw : 0);
-    const int j = tx * bw + (pixel ? tid % bw : 0);
+    const int i = y_org + (pixel ? tid / bw : 0);
+    const int j = x_org + (pixel ? tid % bw : 0);
     const bool inside = pixel && i < c.img_h && j < c.img_w;
     const float px = (float)j + 0.5f;
     const float py = (float)i + 0.5f;
+    if (tid < nwaves) {
+        bool ok;
+        s_rect[tid] = group_rect(tid * 64, bw, x_org, y_org, c.img_w, c.img_h, ok);
+        s_rok[tid] = ok;
+    }
 
     const int2 range = bins[tile];
     float T = 1.f;
@@ -84,37 +139,56 @@ __global__ __launch_bounds__(256) void raster_fwd_kernel(
         for (int w = 0; w < nwaves; ++w) ndone += s_cnt[w];
         if (ndone == nthr) break;
         const int idx = bstart + tid;
+        uint32_t bits = 0;
         if (idx < range.y) {
             const int g = gids[idx];
             const float2 xy = xys[g];
-            s_a[tid] = make_float4(xy.x, xy.y, opac[g], __int_as_float(g));
-            s_b[tid] = make_float4(conics[3 * g + 0], conics[3 * g + 1], conics[3 * g + 2], 0.f);
+            const float o = opac[g];
+            const float ca = conics[3 * g + 0], cb_ = conics[3 * g + 1], cc = conics[3 * g + 2];
+            s_a[tid] = make_float4(xy.x, xy.y, o, __int_as_float(g));
+            s_b[tid] = make_float4(ca, cb_, cc, 0.f);
             s_c[tid] = make_float4(colors[3 * g + 0], colors[3 * g + 1], colors[3 * g + 2], 0.f);
+            float ex, ey;
+            if (alpha_box(ca, cb_, cc, o, ex, ey)) {
+                for (int G = 0; G < nwaves; ++G)
+                    if (s_rok[G] && box_hits(xy.x, xy.y, ex, ey, s_rect[G])) bits |= 1u << G;
+            }
+        }
+        for (int G = 0; G < nwaves; ++G) {
+            const uint64_t m = ballot((bits >> G) & 1u);
+            if (lane == 0) s_mask[G][wave] = m;
         }
         __syncthreads();
-        const int nb = min(nthr, range.y - bstart);
-        for (int t = 0; t < nb; ++t) {
-            if (ballot(!done) == 0) break;
-            const float4 a = s_a[t];
-            const float4 b = s_b[t];
-            const float dx = a.x - px, dy = a.y - py;
-            const float sigma = 0.5f * (b.x * dx * dx + b.z * dy * dy) + b.y * dx * dy;
-            const float alpha = fminf(0.999f, a.z * __expf(-sigma));
-            if (done || sigma < 0.f || alpha < kAlphaMin) continue;
-            const float next_T = T * (1.f - alpha);
-            if (next_T <= kTEps) {
-                done = true;
-                continue;
+        bool live = true;
+        for (int q = 0; q < nwaves && live; ++q) {
+            uint64_t m = uniform64(s_mask[wave][q]);
+            while (m) {
+                if (ballot(!done) == 0) {
+                    live = false;
+                    break;
+                }
+                const int t = q * 64 + __ffsll((unsigned long long)m) - 1;
+                m &= m - 1;
+                const float4 a = s_a[t];
+                const float4 b = s_b[t];
+                const float dx = a.x - px, dy = a.y - py;
+                const float sigma = 0.5f * (b.x * dx * dx + b.z * dy * dy) + b.y * dx * dy;
+                const float alpha = fminf(0.999f, a.z * __expf(-sigma));
+                if (done || sigma < 0.f || alpha < kAlphaMin) continue;
+                const float next_T = T * (1.f - alpha);
+                if (next_T <= kTEps) {
+                    done = true;
+                    continue;
+                }
+                const float vis = alpha * T;
+                const float4 col = s_c[t];
+                cr += col.x * vis;
+                cg += col.y * vis;
+                cb += col.z * vis;
+                T = next_T;
+                cur = bstart + t;
             }
-            const float vis = alpha * T;
-            const float4 col = s_c[t];
-            cr += col.x * vis;
-            cg += col.y * vis;
-            cb += col.z * vis;
-            T = next_T;
-            cur = bstart + t;
         }
-        // protect s_cnt / LDS batch before the next round overwrites them
         __syncthreads();
     }
     if (inside) {
@@ -127,8 +201,58 @@ __global__ __launch_bounds__(256) void raster_fwd_kernel(
     }
 }
 
+// ---------------------------------------------------------------------------
+// Backward: two pixels per lane (slots), so each (Gaussian, wave) reduction
+// covers 128 pixels; 64-pixel groups = (slot, wave).
+// ---------------------------------------------------------------------------
+struct PixState {
+    float px, py, T_final, T, vo_r, vo_g, vo_b, bgdot, buf_r, buf_g, buf_b;
+    int fin;
+    bool inside;
+};
+
+template <bool CONSISTENT, int NV>
+__device__ __forceinline__ bool bwd_pixel(PixState& s, const float4 a, const float4 b,
+                                          const float4 col, int k, float* g) {
+    constexpr float kClamp = CONSISTENT ? 0.999f : 0.99f;
+    const float dx = a.x - s.px, dy = a.y - s.py;
+    const float sigma = 0.5f * (b.x * dx * dx + b.z * dy * dy) + b.y * dx * dy;
+    const float vis = __expf(-sigma);
+    const float raw = a.z * vis;
+    const float alpha = fminf(kClamp, raw);
+    const bool valid = s.inside && (k <= s.fin) && sigma >= 0.f && alpha >= kAlphaMin;
+    if (!valid) return false;
+    const float ra = __builtin_amdgcn_rcpf(1.f - alpha);
+    s.T *= ra;
+    const float fac = alpha * s.T;
+    g[0] += fac * s.vo_r;
+    g[1] += fac * s.vo_g;
+    g[2] += fac * s.vo_b;
+    const float v_alpha = (col.x * s.T - s.buf_r * ra) * s.vo_r +
+                          (col.y * s.T - s.buf_g * ra) * s.vo_g +
+                          (col.z * s.T - s.buf_b * ra) * s.vo_b + ra * s.bgdot;
+    s.buf_r += col.x * fac;
+    s.buf_g += col.y * fac;
+    s.buf_b += col.z * fac;
+    const float slope = (CONSISTENT && raw >= kClamp) ? 0.f : 1.f;
+    const float v_sigma = -a.z * vis * v_alpha * slope;
+    g[3] += 0.5f * v_sigma * dx * dx;
+    g[4] += v_sigma * dx * dy;
+    g[5] += 0.5f * v_sigma * dy * dy;
+    const float gx = v_sigma * (b.x * dx + b.y * dy);
+    const float gy = v_sigma * (b.y * dx + b.z * dy);
+    g[6] += gx;
+    g[7] += gy;
+    if (NV == 11) {
+        g[8] += fabsf(gx);
+        g[9] += fabsf(gy);
+    }
+    g[NV - 1] += vis * v_alpha * slope;
+    return true;
+}
+
 template <bool CONSISTENT, bool ABS>
-__global__ __launch_bounds__(256) void raster_bwd_kernel(
+__global__ __launch_bounds__(128) void raster_bwd_kernel(
     Camera c, const int* __restrict__ gids, const int2* __restrict__ bins,
     const float2* __restrict__ xys, const float* __restrict__ conics,
     const float* __restrict__ colors, const float* __restrict__ opac,
@@ -138,44 +262,68 @@ __global__ __launch_bounds__(256) void raster_bwd_kernel(
     float* __restrict__ v_opacity, float* __restrict__ v_xy, float* __restrict__ v_xy_abs,
     float* __restrict__ v_conic) {
     constexpr int NV = ABS ? 11 : 9;     // rgb3, conic3, xy2, [xy_abs2], opacity
-    constexpr float kClamp = CONSISTENT ? 0.999f : 0.99f;
-    __shared__ float4 s_a[kMaxTileThreads];
-    __shared__ float4 s_b[kMaxTileThreads];
-    __shared__ float4 s_c[kMaxTileThreads];
-    __shared__ float s_g[kMaxTileThreads][NV];
-    __shared__ int s_flag[kMaxTileThreads];
-    __shared__ int s_wmax[4];
+    constexpr int NB = 128;              // batch entries (one per lane)
+    __shared__ float4 s_a[NB];           // x, y, opacity, id
+    __shared__ float4 s_b[NB];           // conic a, b, c
+    __shared__ float4 s_c[NB];           // colour
+    __shared__ float s_g[NB][NV][4];     // per 16-lane row partial sums
+    __shared__ int s_flag[NB];
+    __shared__ uint64_t s_mask[4][2];    // [group = slot * nwaves + wave][64-entry word]
+    __shared__ float4 s_rect[4];
+    __shared__ int s_rok[4];
+    __shared__ int s_wmax[2];
 
     const int num_tiles = c.tiles_x * c.tiles_y;
     const int tile = xcd_tile(blockIdx.x, num_tiles);
     if (tile >= num_tiles) return;
     const int bw = c.block_width;
-    const int nthr = blockDim.x;
+    const int nthr = blockDim.x;         // 64 or 128 (two pixels per lane)
     const int tid = threadIdx.x;
     const int lane = lane_id();
+    const int row = lane >> 4;
     const int wave = tid >> 6;
     const int nwaves = nthr >> 6;
+    const int ngroups = 2 * nwaves;
     const int ty = tile / c.tiles_x, tx = tile - ty * c.tiles_x;
-    const bool pixel = tid < bw * bw;
-    const int i = ty * bw + (pixel ? tid / bw : 0);
-    const int j = tx * bw + (pixel ? tid % bw : 0);
-    const bool inside = pixel && i < c.img_h && j < c.img_w;
-    const float px = (float)j + 0.5f;
-    const float py = (float)i + 0.5f;
-    const int pix = inside ? i * c.img_w + j : 0;
+    const int x_org = tx * bw, y_org = ty * bw;
+    const float bgr = bg[0], bgg = bg[1], bgb = bg[2];
 
-    const float T_final = inside ? final_Ts[pix] : 0.f;
-    float T = T_final;
-    const int bin_final = inside ? final_idx[pix] : -1;
-    const float vo_r = inside ? v_out_img[3 * pix + 0] : 0.f;
-    const float vo_g = inside ? v_out_img[3 * pix + 1] : 0.f;
-    const float vo_b = inside ? v_out_img[3 * pix + 2] : 0.f;
-    const float vo_a = (inside && v_out_alpha) ? v_out_alpha[pix] : 0.f;
-    // background term of v_alpha is loop invariant: T_final * ra * (v_out_alpha - bg . v_out)
-    const float bgdot = T_final * (vo_a - (bg[0] * vo_r + bg[1] * vo_g + bg[2] * vo_b));
-    float buf_r = 0.f, buf_g = 0.f, buf_b = 0.f;
-
-    const int wave_fin = wave_max_i(bin_final);
+    PixState ps[2];
+    int my_fin = -1;
+#pragma unroll
+    for (int sl = 0; sl < 2; ++sl) {
+        const int p = sl * nthr + tid;
+        const bool pixel = p < bw * bw;
+        const int i = y_org + (pixel ? p / bw : 0);
+        const int j = x_org + (pixel ? p % bw : 0);
+        PixState& s = ps[sl];
+        s.inside = pixel && i < c.img_h && j < c.img_w;
+        s.px = (float)j + 0.5f;
+        s.py = (float)i + 0.5f;
+        const int pix = s.inside ? i * c.img_w + j : 0;
+        s.T_final = s.inside ? final_Ts[pix] : 0.f;
+        s.T = s.T_final;
+        s.fin = s.inside ? final_idx[pix] : -1;
+        s.vo_r = s.inside ? v_out_img[3 * pix + 0] : 0.f;
+        s.vo_g = s.inside ? v_out_img[3 * pix + 1] : 0.f;
+        s.vo_b = s.inside ? v_out_img[3 * pix + 2] : 0.f;
+        const float vo_a = (s.inside && v_out_alpha) ? v_out_alpha[pix] : 0.f;
+        // background term of v_alpha: T_final * ra * (v_out_alpha - bg . v_out)
+        s.bgdot = s.T_final * (vo_a - (bgr * s.vo_r + bgg * s.vo_g + bgb * s.vo_b));
+        s.buf_r = s.buf_g = s.buf_b = 0.f;
+        my_fin = max(my_fin, s.fin);
+    }
+    if (tid < ngroups) {
+        bool ok;
+        const int sl = tid / nwaves, w = tid - sl * nwaves;
+        s_rect[tid] = group_rect(sl * nthr + w * 64, bw, x_org, y_org, c.img_w, c.img_h, ok);
+        s_rok[tid] = ok;
+    }
+#pragma unroll
+    for (int v = 0; v < NV; ++v)
+        *reinterpret_cast<float4*>(&s_g[tid][v][0]) = make_float4(0.f, 0.f, 0.f, 0.f);
+    s_flag[tid] = 0;
+    const int wave_fin = uniform_i(wave_max_i(my_fin));
     if (lane == 0) s_wmax[wave] = wave_fin;
     __syncthreads();
     int block_fin = -1;
@@ -184,86 +332,92 @@ __global__ __launch_bounds__(256) void raster_bwd_kernel(
     const int2 range = bins[tile];
     const int last = min(range.y - 1, block_fin);
     for (int bend = last; bend >= range.x; bend -= nthr) {
-        __syncthreads();
+        // load entry e = tid (sorted index bend - e, back to front)
         const int idx = bend - tid;
+        uint32_t bits = 0;
         if (idx >= range.x) {
             const int g = gids[idx];
             const float2 xy = xys[g];
-            s_a[tid] = make_float4(xy.x, xy.y, opac[g], __int_as_float(g));
-            s_b[tid] = make_float4(conics[3 * g + 0], conics[3 * g + 1], conics[3 * g + 2], 0.f);
+            const float o = opac[g];
+            const float ca = conics[3 * g + 0], cb_ = conics[3 * g + 1], cc = conics[3 * g + 2];
+            s_a[tid] = make_float4(xy.x, xy.y, o, __int_as_float(g));
+            s_b[tid] = make_float4(ca, cb_, cc, 0.f);
             s_c[tid] = make_float4(colors[3 * g + 0], colors[3 * g + 1], colors[3 * g + 2], 0.f);
-        }
-        for (int q = 0; q < NV; ++q) s_g[tid][q] = 0.f;
-        s_flag[tid] = 0;
-        __syncthreads();
-        const int nb = min(nthr, bend - range.x + 1);
-        const int t0 = max(0, bend - wave_fin);
-        for (int t = t0; t < nb; ++t) {
-            const int k = bend - t;
-            const float4 a = s_a[t];
-            const float4 b = s_b[t];
-            const float dx = a.x - px, dy = a.y - py;
-            const float sigma = 0.5f * (b.x * dx * dx + b.z * dy * dy) + b.y * dx * dy;
-            const float vis = __expf(-sigma);
-            const float raw = a.z * vis;
-            const float alpha = fminf(kClamp, raw);
-            const bool valid = (k <= bin_final) && sigma >= 0.f && alpha >= kAlphaMin;
-            if (ballot(valid) == 0) continue;
-            float g[NV];
-#pragma unroll
-            for (int q = 0; q < NV; ++q) g[q] = 0.f;
-            if (valid) {
-                const float ra = 1.f / (1.f - alpha);
-                T *= ra;
-                const float fac = alpha * T;
-                const float4 col = s_c[t];
-                g[0] = fac * vo_r;
-                g[1] = fac * vo_g;
-                g[2] = fac * vo_b;
-                const float v_alpha = (col.x * T - buf_r * ra) * vo_r +
-                                      (col.y * T - buf_g * ra) * vo_g +
-                                      (col.z * T - buf_b * ra) * vo_b + ra * bgdot;
-                buf_r += col.x * fac;
-                buf_g += col.y * fac;
-                buf_b += col.z * fac;
-                const float slope = (CONSISTENT && raw >= kClamp) ? 0.f : 1.f;
-                const float v_sigma = -a.z * vis * v_alpha * slope;
-                g[3] = 0.5f * v_sigma * dx * dx;
-                g[4] = v_sigma * dx * dy;
-                g[5] = 0.5f * v_sigma * dy * dy;
-                g[6] = v_sigma * (b.x * dx + b.y * dy);
-                g[7] = v_sigma * (b.y * dx + b.z * dy);
-                if (ABS) {
-                    g[8] = fabsf(g[6]);
-                    g[9] = fabsf(g[7]);
-                }
-                g[NV - 1] = vis * v_alpha * slope;
+            float ex, ey;
+            if (alpha_box(ca, cb_, cc, o, ex, ey)) {
+                for (int G = 0; G < ngroups; ++G)
+                    if (s_rok[G] && box_hits(xy.x, xy.y, ex, ey, s_rect[G])) bits |= 1u << G;
             }
+        }
+        for (int G = 0; G < ngroups; ++G) {
+            const uint64_t m = ballot((bits >> G) & 1u);
+            if (lane == 0) s_mask[G][wave] = m;
+        }
+        __syncthreads();
+        const int nb = uniform_i(min(nthr, bend - range.x + 1));
+        const int t0 = uniform_i(max(0, bend - wave_fin));   // entries t < t0 lie after every fin of this wave
+        const int nwords = (nb + 63) >> 6;
+        for (int q = t0 >> 6; q < nwords; ++q) {
+            uint64_t m0 = uniform64(s_mask[wave][q]);
+            uint64_t m1 = uniform64(s_mask[nwaves + wave][q]);
+            const int lo = t0 - q * 64;
+            if (lo > 0) {
+                const uint64_t keep = ~((1ull << lo) - 1ull);
+                m0 &= keep;
+                m1 &= keep;
+            }
+            uint64_t m = m0 | m1;
+            while (m) {
+                const int b = __ffsll((unsigned long long)m) - 1;
+                m &= m - 1;
+                const int t = q * 64 + b;
+                const int k = bend - t;
+                const float4 a = s_a[t];
+                const float4 bb = s_b[t];
+                const float4 col = s_c[t];
+                float g[NV];
 #pragma unroll
-            for (int q = 0; q < NV; ++q) g[q] = wave_sum_to_lane63(g[q]);
-            if (lane == 63) {
+                for (int v = 0; v < NV; ++v) g[v] = 0.f;
+                bool any = false;
+                if ((m0 >> b) & 1ull) any = bwd_pixel<CONSISTENT, NV>(ps[0], a, bb, col, k, g);
+                if ((m1 >> b) & 1ull) any |= bwd_pixel<CONSISTENT, NV>(ps[1], a, bb, col, k, g);
+                if (ballot(any) == 0) continue;
+                // 16-lane row sums; each row leader adds into its own LDS slot
 #pragma unroll
-                for (int q = 0; q < NV; ++q) atomicAdd(&s_g[t][q], g[q]);
-                s_flag[t] = 1;
+                for (int v = 0; v < NV; ++v) g[v] = row_sum16(g[v]);
+                if ((lane & 15) == 15) {
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) atomicAdd(&s_g[t][v][row], g[v]);
+                    if (row == 3) s_flag[t] = 1;
+                }
             }
         }
         __syncthreads();
         if (tid < nb && s_flag[tid]) {
-            const int gid = __float_as_int(s_a[tid].w);
-            unsafeAtomicAdd(v_rgb + 3 * gid + 0, s_g[tid][0]);
-            unsafeAtomicAdd(v_rgb + 3 * gid + 1, s_g[tid][1]);
-            unsafeAtomicAdd(v_rgb + 3 * gid + 2, s_g[tid][2]);
-            unsafeAtomicAdd(v_conic + 3 * gid + 0, s_g[tid][3]);
-            unsafeAtomicAdd(v_conic + 3 * gid + 1, s_g[tid][4]);
-            unsafeAtomicAdd(v_conic + 3 * gid + 2, s_g[tid][5]);
-            unsafeAtomicAdd(v_xy + 2 * gid + 0, s_g[tid][6]);
-            unsafeAtomicAdd(v_xy + 2 * gid + 1, s_g[tid][7]);
-            if (ABS) {
-                unsafeAtomicAdd(v_xy_abs + 2 * gid + 0, s_g[tid][8]);
-                unsafeAtomicAdd(v_xy_abs + 2 * gid + 1, s_g[tid][9]);
+            float r[NV];
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                const float4 p4 = *reinterpret_cast<const float4*>(&s_g[tid][v][0]);
+                r[v] = (p4.x + p4.y) + (p4.z + p4.w);
+                *reinterpret_cast<float4*>(&s_g[tid][v][0]) = make_float4(0.f, 0.f, 0.f, 0.f);
             }
-            unsafeAtomicAdd(v_opacity + gid, s_g[tid][NV - 1]);
+            s_flag[tid] = 0;
+            const int gid = __float_as_int(s_a[tid].w);
+            unsafeAtomicAdd(v_rgb + 3 * gid + 0, r[0]);
+            unsafeAtomicAdd(v_rgb + 3 * gid + 1, r[1]);
+            unsafeAtomicAdd(v_rgb + 3 * gid + 2, r[2]);
+            unsafeAtomicAdd(v_conic + 3 * gid + 0, r[3]);
+            unsafeAtomicAdd(v_conic + 3 * gid + 1, r[4]);
+            unsafeAtomicAdd(v_conic + 3 * gid + 2, r[5]);
+            unsafeAtomicAdd(v_xy + 2 * gid + 0, r[6]);
+            unsafeAtomicAdd(v_xy + 2 * gid + 1, r[7]);
+            if (ABS) {
+                unsafeAtomicAdd(v_xy_abs + 2 * gid + 0, r[8]);
+                unsafeAtomicAdd(v_xy_abs + 2 * gid + 1, r[9]);
+            }
+            unsafeAtomicAdd(v_opacity + gid, r[NV - 1]);
         }
+        __syncthreads();
     }
 }
 
@@ -344,7 +498,9 @@ hipError_t raster_bwd(const Camera& c, const int* gids_sorted, const int* tile_b
                       const int* final_idx, const float* v_out, const float* v_out_alpha,
                       float* v_colors, float* v_opacity, float* v_xy, float* v_xy_abs,
                       float* v_conic, int consistent, hipStream_t s) {
-    const int nthr = ((c.block_width * c.block_width + 63) / 64) * 64;
+    // two pixels per lane
+    const int half = (c.block_width * c.block_width + 1) / 2;
+    const int nthr = ((half + 63) / 64) * 64;
     const int grid = tile_grid(c);
     const int2* b2 = reinterpret_cast<const int2*>(tile_bins);
     const float2* x2 = reinterpret_cast<const float2*>(xys);
