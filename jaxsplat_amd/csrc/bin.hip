@@ -48,52 +48,73 @@ __device__ __forceinline__ void tile_rect(const Camera& c, float2 xy, int rad, i
     y1 = min(max(0, (int)(tcy + tr + 1.f)), c.tiles_y);
 }
 
+constexpr int kEmitPer = 4;                  // Gaussians per lane
+constexpr int kEmitPart = 256 * kEmitPer;    // Gaussians per scan partition
+
+// Emission: the workgroup owns 1024 consecutive Gaussians of the depth order
+// (4 groups of 64 per wave).  A single-pass decoupled look-back scan gives the
+// workgroup's first output slot; each wave then expands its groups 64 slots
+// at a time: the lane whose tile run starts inside the chunk marks that
+// position in a per-wave LDS row, an inclusive max-scan (DPP) gives every
+// slot its owning lane, and the owner's (id, rectangle, offset) is read from
+// LDS.  Stores of one chunk are 64 consecutive slots.
 __global__ __launch_bounds__(256) void emit_kernel(
     Camera c, const uint32_t* __restrict__ perm, const float2* __restrict__ xys,
     const int* __restrict__ radii, int isect_cap, uint32_t* __restrict__ tile_out,
     uint32_t* __restrict__ gid_out, uint64_t* __restrict__ status, uint32_t* __restrict__ counter,
     uint32_t* __restrict__ err) {
     __shared__ uint32_t s_part;
-    __shared__ uint32_t s_wave_tot[4];
+    __shared__ uint32_t s_wave_tot[kEmitPer][4];
     __shared__ uint64_t s_excl;
-    __shared__ uint32_t s_off[256];
-    __shared__ uint32_t s_gid[256];
-    __shared__ int s_x0[256], s_y0[256], s_w[256];
+    __shared__ int s_mark[4][64];
+    __shared__ int4 s_own[4][64];        // (gid, x0 | y0 << 16, width, group offset)
     const int tid = threadIdx.x;
     const int lane = lane_id();
     const int wave = tid >> 6;
     if (tid == 0) s_part = atomicAdd(counter, 1u);
     __syncthreads();
     const int part = (int)s_part;
-    const int r = part * 256 + tid;
 
-    int count = 0, x0 = 0, y0 = 0, w = 1;
-    uint32_t g = 0;
-    if (r < c.num_points) {
-        g = perm[r];
-        const int rad = radii[g];
-        if (rad > 0) {
-            int x1, y1;
-            tile_rect(c, xys[g], rad, x0, x1, y0, y1);
-            w = max(x1 - x0, 1);
-            count = (x1 - x0) * (y1 - y0);
-        }
-    }
-    const int incl = wave_inclusive_sum(count);
-    if (lane == 63) s_wave_tot[wave] = (uint32_t)incl;
-    __syncthreads();
-    uint32_t wave_off = 0, total = 0;
+    // element e = k*256 + tid of the partition (depth rank part*kEmitPart + e)
+    int cnt[kEmitPer], incl[kEmitPer], rx0[kEmitPer], ry0[kEmitPer], rw[kEmitPer];
+    uint32_t gg[kEmitPer];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const uint32_t v = s_wave_tot[k];
-        wave_off += (k < wave) ? v : 0u;
-        total += v;
+    for (int k = 0; k < kEmitPer; ++k) {
+        const int r = part * kEmitPart + k * 256 + tid;
+        int count = 0, x0 = 0, y0 = 0, w = 1;
+        uint32_t g = 0;
+        if (r < c.num_points) {
+            g = perm[r];
+            const int rad = radii[g];
+            if (rad > 0) {
+                int x1, y1;
+                tile_rect(c, xys[g], rad, x0, x1, y0, y1);
+                w = max(x1 - x0, 1);
+                count = (x1 - x0) * (y1 - y0);
+            }
+        }
+        cnt[k] = count;
+        rx0[k] = x0;
+        ry0[k] = y0;
+        rw[k] = w;
+        gg[k] = g;
+        incl[k] = wave_inclusive_sum(count);
+        if (lane == 63) s_wave_tot[k][wave] = (uint32_t)incl[k];
     }
-    s_off[tid] = wave_off + (uint32_t)(incl - count);
-    s_gid[tid] = g;
-    s_x0[tid] = x0;
-    s_y0[tid] = y0;
-    s_w[tid] = w;
+    __syncthreads();
+    uint32_t total = 0, grp_off[kEmitPer];
+#pragma unroll
+    for (int k = 0; k < kEmitPer; ++k) {
+        uint32_t w_off = 0, slab = 0;
+#pragma unroll
+        for (int w = 0; w < 4; ++w) {
+            const uint32_t v = s_wave_tot[k][w];
+            w_off += (w < wave) ? v : 0u;
+            slab += v;
+        }
+        grp_off[k] = total + w_off;
+        total += slab;
+    }
     if (tid == 0) {
         store_agent(status + part, (part == 0 ? kFlagPre : kFlagAgg) | (uint64_t)total);
         if (part == 0) s_excl = 0;
@@ -107,31 +128,53 @@ __global__ __launch_bounds__(256) void emit_kernel(
     }
     __syncthreads();
     const uint64_t gbase = s_excl;
-    for (uint32_t s = tid; s < total; s += 256) {
-        // last entry with offset <= s (entries with zero count share the next offset)
-        int lo = 0, hi = 255;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (s_off[mid] <= s) lo = mid; else hi = mid - 1;
-        }
-        const int k = (int)(s - s_off[lo]);
-        const int wv = s_w[lo];
-        const int tyy = s_y0[lo] + k / wv;
-        const int txx = s_x0[lo] + k - (k / wv) * wv;
-        const uint64_t pos = gbase + s;
-        if (pos < (uint64_t)isect_cap) {
-            tile_out[pos] = (uint32_t)(tyy * c.tiles_x + txx);
-            gid_out[pos] = s_gid[lo];
+
+#pragma unroll
+    for (int k = 0; k < kEmitPer; ++k) {
+        const int gtotal = uniform_i(__shfl(incl[k], 63));
+        const int ex = incl[k] - cnt[k];
+        s_own[wave][lane] = make_int4((int)gg[k], rx0[k] | (ry0[k] << 16), rw[k], ex);
+        wave_lds_sync();
+        const uint64_t obase = gbase + uniform_i((int)grp_off[k]);
+        int carry = 0;
+        for (int chunk = 0; chunk < gtotal; chunk += 64) {
+            s_mark[wave][lane] = 0;
+            wave_lds_sync();
+            if (cnt[k] > 0 && ex >= chunk && ex < chunk + 64) s_mark[wave][ex - chunk] = lane + 1;
+            wave_lds_sync();
+            int v = wave_max_scan_nonneg(s_mark[wave][lane]);
+            const int owner = v > 0 ? v - 1 : carry;
+            carry = __builtin_amdgcn_readlane(owner, 63);
+            const int slot = chunk + lane;
+            if (slot < gtotal) {
+                const int4 od = s_own[wave][owner];
+                const int kk = slot - od.w;
+                const int q = kk / od.z;
+                const int ty = (od.y >> 16) + q;
+                const int tx = (od.y & 0xFFFF) + (kk - q * od.z);
+                const uint64_t pos = obase + (uint64_t)slot;
+                if (pos < (uint64_t)isect_cap) {
+                    tile_out[pos] = (uint32_t)(ty * c.tiles_x + tx);
+                    gid_out[pos] = (uint32_t)od.x;
+                }
+            }
+            wave_lds_sync();
         }
     }
 }
 
 __global__ void tile_bins_kernel(const uint32_t* __restrict__ tiles, int n_host,
-                                 const int* __restrict__ n_dev, int2* __restrict__ bins) {
+                                 const int* __restrict__ n_dev, int num_tiles,
+                                 int2* __restrict__ bins, uint32_t* __restrict__ err) {
     const int n = n_dev ? *n_dev : n_host;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint32_t t = tiles[i];
+    const uint32_t p_ = i > 0 ? tiles[i - 1] : 0u;
+    if (t >= (uint32_t)num_tiles || p_ >= (uint32_t)num_tiles) {   // never for valid input
+        atomicOr(err, 4u);
+        return;
+    }
     if (i == 0) bins[t].x = 0;
     if (i == n - 1) bins[t].y = n;
     if (i > 0) {
@@ -147,9 +190,10 @@ constexpr size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
 struct BinLayout {
     // zeroed region
-    size_t err, emit_counter, emit_status, depth_sort, tile_sort, zero_end;
+    size_t err, emit_counter, emit_status, zero_end;
     // plain region
-    size_t dkeys, dtmp_k, dtmp_v, dsorted_k, perm, tiles, gids, ttmp_k, ttmp_v, tiles_sorted, end;
+    size_t depth_sort, tile_sort, dkeys, dtmp_k, dtmp_v, dsorted_k, perm, tiles, gids, ttmp_k,
+        ttmp_v, tiles_sorted, end;
 };
 
 BinLayout layout(const BinPlan& p) {
@@ -157,10 +201,10 @@ BinLayout layout(const BinPlan& p) {
     size_t o = 0;
     L.err = o; o += 16;
     L.emit_counter = o; o += 16;
-    L.emit_status = o; o = align256(o + 8ull * ((p.n + 255) / 256 + 1));
+    L.emit_status = o; o = align256(o + 8ull * ((p.n + kEmitPart - 1) / kEmitPart + 1));
+    L.zero_end = o;
     L.depth_sort = o; o = align256(o + p.depth_sort.scratch_bytes);
     L.tile_sort = o; o = align256(o + p.tile_sort.scratch_bytes);
-    L.zero_end = o;
     const size_t n4 = align256(4ull * (p.n > 0 ? p.n : 1));
     const size_t i4 = align256(4ull * (p.isect_cap > 0 ? p.isect_cap : 1));
     L.dkeys = o; o += n4;
@@ -216,7 +260,7 @@ hipError_t bin_and_sort(const BinPlan& p, const Camera& c, const float* xys, con
     e = sort_pairs(p.depth_sort, U(L.dkeys), nullptr, U(L.dsorted_k), U(L.perm), U(L.dtmp_k),
                    U(L.dtmp_v), p.n, nullptr, b + L.depth_sort, err, s);
     if (e != hipSuccess) return e;
-    emit_kernel<<<(p.n + 255) / 256, 256, 0, s>>>(
+    emit_kernel<<<(p.n + kEmitPart - 1) / kEmitPart, 256, 0, s>>>(
         c, U(L.perm), reinterpret_cast<const float2*>(xys), radii, p.isect_cap, U(L.tiles),
         U(L.gids), reinterpret_cast<uint64_t*>(b + L.emit_status), U(L.emit_counter), err);
     if ((e = hipGetLastError()) != hipSuccess) return e;
@@ -227,7 +271,8 @@ hipError_t bin_and_sort(const BinPlan& p, const Camera& c, const float* xys, con
     if (e != hipSuccess) return e;
     const int nb = (p.isect_cap + 255) / 256;
     tile_bins_kernel<<<nb > 0 ? nb : 1, 256, 0, s>>>(tiles_out, num_isect_host, num_isect_dev,
-                                                   reinterpret_cast<int2*>(tile_bins));
+                                                   p.num_tiles,
+                                                   reinterpret_cast<int2*>(tile_bins), err);
     return hipGetLastError();
 }
 

@@ -74,6 +74,28 @@ __device__ __forceinline__ float row_sum16(float v) {
 
 __device__ __forceinline__ int uniform_i(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// Orders LDS traffic between the lanes of one wave: without it the compiler
+// may forward a lane's own earlier store to a later load of the same address
+// and miss a store made by another lane in between.  Emits no instruction
+// (a wave's LDS operations already execute in program order).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Inclusive max-scan over the 64 lanes for non-negative values (0 is the
+// identity): Hillis-Steele row shifts, then the two row broadcasts.
+__device__ __forceinline__ int wave_max_scan_nonneg(int v) {
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x111, 0xF, 0xF, true));   // row_shr:1
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x112, 0xF, 0xF, true));   // row_shr:2
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x114, 0xF, 0xF, true));   // row_shr:4
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x118, 0xF, 0xF, true));   // row_shr:8
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, kDppRowBcast15, 0xA, 0xF, false));
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, kDppRowBcast31, 0xC, 0xF, false));
+    return v;
+}
+
 __device__ __forceinline__ float wave_sum(float v) {
     return __builtin_bit_cast(float, __builtin_amdgcn_readlane(
         __builtin_bit_cast(int, wave_sum_to_lane63(v)), 63));

@@ -125,6 +125,96 @@ __device__ __forceinline__ M3 ewa_jacobian(float fx, float fy, float tx, float t
     return J;
 }
 
+constexpr int kProjPer = 4;                   // Gaussians per lane
+constexpr int kProjPart = 256 * kProjPer;    // Gaussians per scan partition
+
+// Projection of Gaussian idx (forward.cu:13-131); writes every output row and
+// returns num_tiles_hit.
+__device__ __forceinline__ int project_one(
+    const Camera& c, int idx, const M3& W, const float* pt, const float* __restrict__ means3d,
+    const float* __restrict__ scales, const float* __restrict__ quats,
+    float* __restrict__ cov3d_out, float* __restrict__ xys, float* __restrict__ depths,
+    int* __restrict__ radii, float* __restrict__ conics, float* __restrict__ comp_out,
+    int* __restrict__ num_tiles_hit) {
+    const float mx = means3d[3 * idx + 0];
+    const float my = means3d[3 * idx + 1];
+    const float mz = means3d[3 * idx + 2];
+    float t[3];
+    transform(W, pt, mx, my, mz, t);                  // clip_near_plane, helpers.h:204-211
+    float cov[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    float conic[3] = {0.f, 0.f, 0.f};
+    float cx = 0.f, cy = 0.f, depth = 0.f, comp = 0.f;
+    int rad_i = 0, count = 0;
+    if (!(t[2] <= c.clip_thresh)) {
+        const float s[3] = {scales[3 * idx + 0], scales[3 * idx + 1], scales[3 * idx + 2]};
+        const float4 q4 = reinterpret_cast<const float4*>(quats)[idx];
+        const float q[4] = {q4.x, q4.y, q4.z, q4.w};
+        cov3d_of(s, q, c.glob_scale, cov);
+        // project_cov3d_ewa, forward.cu:465-544
+        const float lim_x = 1.3f * c.tan_fovx;
+        const float lim_y = 1.3f * c.tan_fovy;
+        const float tx = t[2] * fminf(lim_x, fmaxf(-lim_x, t[0] / t[2]));
+        const float ty = t[2] * fminf(lim_y, fmaxf(-lim_y, t[1] / t[2]));
+        const float rz = 1.f / t[2];
+        const float rz2 = rz * rz;
+        const M3 J = ewa_jacobian(c.fx, c.fy, tx, ty, rz, rz2);
+        const M3 T = mm(J, W);
+        const M3 C2 = mm(mm(T, sym3(cov)), tr(T));
+        const float c00 = C2.m[0][0], c11 = C2.m[1][1], c01 = C2.m[1][0];
+        const float det_orig = c00 * c11 - c01 * c01;
+        const float a = c00 + 0.3f, b = c01, cc = c11 + 0.3f;
+        const float det_blur = a * cc - b * b;
+        const float comp_v = sqrtf(fmaxf(0.f, det_orig / det_blur));
+        // compute_cov2d_bounds, helpers.h:34-57
+        const float det = a * cc - b * b;
+        if (det != 0.f) {
+            const float inv_det = 1.f / det;
+            conic[0] = cc * inv_det;
+            conic[1] = -b * inv_det;
+            conic[2] = a * inv_det;
+            const float hb = 0.5f * (a + cc);
+            const float v1 = hb + sqrtf(fmaxf(0.1f, hb * hb - det));
+            const float v2 = hb - sqrtf(fmaxf(0.1f, hb * hb - det));
+            const float radius = ceilf(3.f * sqrtf(fmaxf(v1, v2)));
+            // project_pix, helpers.h:122-128
+            const float rw = 1.f / (t[2] + 1e-6f);
+            const float px = t[0] * rw * c.fx + c.cx;
+            const float py = t[1] * rw * c.fy + c.cy;
+            // get_tile_bbox, helpers.h:9-32
+            const float bwf = (float)c.block_width;
+            const float tcx = px / bwf, tcy = py / bwf, tr_ = radius / bwf;
+            const int x0 = min(max(0, (int)(tcx - tr_)), c.tiles_x);
+            const int x1 = min(max(0, (int)(tcx + tr_ + 1.f)), c.tiles_x);
+            const int y0 = min(max(0, (int)(tcy - tr_)), c.tiles_y);
+            const int y1 = min(max(0, (int)(tcy + tr_ + 1.f)), c.tiles_y);
+            const int area = (x1 - x0) * (y1 - y0);
+            if (area > 0) {
+                count = area;
+                depth = t[2];
+                rad_i = (int)radius;
+                cx = px;
+                cy = py;
+                comp = comp_v;
+            }
+        }
+    }
+    if (cov3d_out) {
+        float2* o = reinterpret_cast<float2*>(cov3d_out + 6 * (size_t)idx);
+        o[0] = make_float2(cov[0], cov[1]);
+        o[1] = make_float2(cov[2], cov[3]);
+        o[2] = make_float2(cov[4], cov[5]);
+    }
+    reinterpret_cast<float2*>(xys)[idx] = make_float2(cx, cy);
+    depths[idx] = depth;
+    radii[idx] = rad_i;
+    conics[3 * idx + 0] = conic[0];
+    conics[3 * idx + 1] = conic[1];
+    conics[3 * idx + 2] = conic[2];
+    if (comp_out) comp_out[idx] = comp;
+    num_tiles_hit[idx] = count;
+    return count;
+}
+
 __global__ __launch_bounds__(256) void project_fwd_kernel(
     Camera c, const float* __restrict__ means3d, const float* __restrict__ scales,
     const float* __restrict__ quats, const float* __restrict__ viewmat,
@@ -133,7 +223,7 @@ __global__ __launch_bounds__(256) void project_fwd_kernel(
     int* __restrict__ num_tiles_hit, int* __restrict__ cum_tiles_hit,
     uint64_t* __restrict__ status, uint32_t* __restrict__ counter, uint32_t* __restrict__ err) {
     __shared__ uint32_t s_part;
-    __shared__ uint32_t s_wave_tot[4];
+    __shared__ uint32_t s_wave_tot[kProjPer][4];
     __shared__ uint64_t s_excl;
     const int tid = threadIdx.x;
     const int lane = lane_id();
@@ -141,101 +231,37 @@ __global__ __launch_bounds__(256) void project_fwd_kernel(
     if (tid == 0) s_part = atomicAdd(counter, 1u);
     __syncthreads();
     const int part = (int)s_part;
-    const int idx = part * 256 + tid;
+    M3 W;
+    float pt[3];
+    load_view(viewmat, W, pt);
 
-    int count = 0;
-    if (idx < c.num_points) {
-        M3 W;
-        float pt[3];
-        load_view(viewmat, W, pt);
-        const float mx = means3d[3 * idx + 0];
-        const float my = means3d[3 * idx + 1];
-        const float mz = means3d[3 * idx + 2];
-        float t[3];
-        transform(W, pt, mx, my, mz, t);                  // clip_near_plane, helpers.h:204-211
-        float cov[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-        float conic[3] = {0.f, 0.f, 0.f};
-        float cx = 0.f, cy = 0.f, depth = 0.f, comp = 0.f;
-        int rad_i = 0;
-        if (!(t[2] <= c.clip_thresh)) {
-            const float s[3] = {scales[3 * idx + 0], scales[3 * idx + 1], scales[3 * idx + 2]};
-            const float4 q4 = reinterpret_cast<const float4*>(quats)[idx];
-            const float q[4] = {q4.x, q4.y, q4.z, q4.w};
-            cov3d_of(s, q, c.glob_scale, cov);
-            // project_cov3d_ewa, forward.cu:465-544
-            const float lim_x = 1.3f * c.tan_fovx;
-            const float lim_y = 1.3f * c.tan_fovy;
-            const float tx = t[2] * fminf(lim_x, fmaxf(-lim_x, t[0] / t[2]));
-            const float ty = t[2] * fminf(lim_y, fmaxf(-lim_y, t[1] / t[2]));
-            const float rz = 1.f / t[2];
-            const float rz2 = rz * rz;
-            const M3 J = ewa_jacobian(c.fx, c.fy, tx, ty, rz, rz2);
-            const M3 T = mm(J, W);
-            const M3 C2 = mm(mm(T, sym3(cov)), tr(T));
-            const float c00 = C2.m[0][0], c11 = C2.m[1][1], c01 = C2.m[1][0];
-            const float det_orig = c00 * c11 - c01 * c01;
-            const float a = c00 + 0.3f, b = c01, cc = c11 + 0.3f;
-            const float det_blur = a * cc - b * b;
-            const float comp_v = sqrtf(fmaxf(0.f, det_orig / det_blur));
-            // compute_cov2d_bounds, helpers.h:34-57
-            const float det = a * cc - b * b;
-            if (det != 0.f) {
-                const float inv_det = 1.f / det;
-                conic[0] = cc * inv_det;
-                conic[1] = -b * inv_det;
-                conic[2] = a * inv_det;
-                const float hb = 0.5f * (a + cc);
-                const float v1 = hb + sqrtf(fmaxf(0.1f, hb * hb - det));
-                const float v2 = hb - sqrtf(fmaxf(0.1f, hb * hb - det));
-                const float radius = ceilf(3.f * sqrtf(fmaxf(v1, v2)));
-                // project_pix, helpers.h:122-128
-                const float rw = 1.f / (t[2] + 1e-6f);
-                const float px = t[0] * rw * c.fx + c.cx;
-                const float py = t[1] * rw * c.fy + c.cy;
-                // get_tile_bbox, helpers.h:9-32
-                const float bwf = (float)c.block_width;
-                const float tcx = px / bwf, tcy = py / bwf, tr_ = radius / bwf;
-                const int x0 = min(max(0, (int)(tcx - tr_)), c.tiles_x);
-                const int x1 = min(max(0, (int)(tcx + tr_ + 1.f)), c.tiles_x);
-                const int y0 = min(max(0, (int)(tcy - tr_)), c.tiles_y);
-                const int y1 = min(max(0, (int)(tcy + tr_ + 1.f)), c.tiles_y);
-                const int area = (x1 - x0) * (y1 - y0);
-                if (area > 0) {
-                    count = area;
-                    depth = t[2];
-                    rad_i = (int)radius;
-                    cx = px;
-                    cy = py;
-                    comp = comp_v;
-                }
-            }
-        }
-        if (cov3d_out) {
-            float2* o = reinterpret_cast<float2*>(cov3d_out + 6 * (size_t)idx);
-            o[0] = make_float2(cov[0], cov[1]);
-            o[1] = make_float2(cov[2], cov[3]);
-            o[2] = make_float2(cov[4], cov[5]);
-        }
-        reinterpret_cast<float2*>(xys)[idx] = make_float2(cx, cy);
-        depths[idx] = depth;
-        radii[idx] = rad_i;
-        conics[3 * idx + 0] = conic[0];
-        conics[3 * idx + 1] = conic[1];
-        conics[3 * idx + 2] = conic[2];
-        if (comp_out) comp_out[idx] = comp;
-        num_tiles_hit[idx] = count;
-    }
-
-    // ---- single-pass inclusive scan of the counts (decoupled look-back) ----
-    const int incl = wave_inclusive_sum(count);
-    if (lane == 63) s_wave_tot[wave] = (uint32_t)incl;
-    __syncthreads();
-    uint32_t wave_off = 0, total = 0;
+    // element k*256 + tid of the partition, k = 0..kProjPer-1 (coalesced slabs)
+    int incl[kProjPer];
+    int cnt[kProjPer];
 #pragma unroll
-    for (int w = 0; w < 4; ++w) {
-        const uint32_t v = s_wave_tot[w];
-        wave_off += (w < wave) ? v : 0u;
-        total += v;
+    for (int k = 0; k < kProjPer; ++k) {
+        const int idx = part * kProjPart + k * 256 + tid;
+        cnt[k] = idx < c.num_points
+                     ? project_one(c, idx, W, pt, means3d, scales, quats, cov3d_out, xys,
+                                   depths, radii, conics, comp_out, num_tiles_hit)
+                     : 0;
+        incl[k] = wave_inclusive_sum(cnt[k]);
+        if (lane == 63) s_wave_tot[k][wave] = (uint32_t)incl[k];
+    }
+    __syncthreads();
+    // ---- single-pass inclusive scan of the counts (decoupled look-back) ----
+    uint32_t total = 0, off[kProjPer];
+#pragma unroll
+    for (int k = 0; k < kProjPer; ++k) {
+        uint32_t w_off = 0, slab = 0;
+#pragma unroll
+        for (int w = 0; w < 4; ++w) {
+            const uint32_t v = s_wave_tot[k][w];
+            w_off += (w < wave) ? v : 0u;
+            slab += v;
+        }
+        off[k] = total + w_off;
+        total += slab;
     }
     if (tid == 0) {
         store_agent(status + part, (part == 0 ? kFlagPre : kFlagAgg) | (uint64_t)total);
@@ -249,7 +275,12 @@ __global__ __launch_bounds__(256) void project_fwd_kernel(
         }
     }
     __syncthreads();
-    if (idx < c.num_points) cum_tiles_hit[idx] = (int)(s_excl + wave_off + (uint32_t)incl);
+    const uint64_t base = s_excl;
+#pragma unroll
+    for (int k = 0; k < kProjPer; ++k) {
+        const int idx = part * kProjPart + k * 256 + tid;
+        if (idx < c.num_points) cum_tiles_hit[idx] = (int)(base + off[k] + (uint32_t)incl[k]);
+    }
 }
 
 // helpers.h:155-192 with vR[row][col]; the reference indexes v_R[col][row].
@@ -440,7 +471,7 @@ hipError_t fill_value_f32x3(float* dst, size_t n_pixels, const float* rgb, hipSt
 
 // scratch layout: [counter u32 | err u32 | pad to 16 | status u64 x parts]
 size_t project_fwd_scratch_bytes(int n) {
-    const size_t parts = ((size_t)n + 255) / 256;
+    const size_t parts = ((size_t)n + kProjPart - 1) / kProjPart;
     return 16 + 8 * (parts > 0 ? parts : 1);
 }
 
@@ -449,7 +480,7 @@ hipError_t project_fwd(const Camera& c, const float* means3d, const float* scale
                        float* depths, int* radii, float* conics, float* compensation,
                        int* num_tiles_hit, int* cum_tiles_hit, void* scratch, hipStream_t s) {
     if (c.num_points <= 0) return hipSuccess;
-    const int parts = (c.num_points + 255) / 256;
+    const int parts = (c.num_points + kProjPart - 1) / kProjPart;
     uint8_t* base = static_cast<uint8_t*>(scratch);
     hipError_t e = fill_zero(base, project_fwd_scratch_bytes(c.num_points), s);
     if (e != hipSuccess) return e;

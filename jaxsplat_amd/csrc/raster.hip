@@ -140,8 +140,8 @@ __global__ __launch_bounds__(256) void raster_fwd_kernel(
         if (ndone == nthr) break;
         const int idx = bstart + tid;
         uint32_t bits = 0;
-        if (idx < range.y) {
-            const int g = gids[idx];
+        const int g = idx < range.y ? gids[idx] : -1;
+        if ((unsigned)g < (unsigned)c.num_points) {
             const float2 xy = xys[g];
             const float o = opac[g];
             const float ca = conics[3 * g + 0], cb_ = conics[3 * g + 1], cc = conics[3 * g + 2];
@@ -335,8 +335,8 @@ __global__ __launch_bounds__(128) void raster_bwd_kernel(
         // load entry e = tid (sorted index bend - e, back to front)
         const int idx = bend - tid;
         uint32_t bits = 0;
-        if (idx >= range.x) {
-            const int g = gids[idx];
+        const int g = idx >= range.x ? gids[idx] : -1;
+        if ((unsigned)g < (unsigned)c.num_points) {
             const float2 xy = xys[g];
             const float o = opac[g];
             const float ca = conics[3 * g + 0], cb_ = conics[3 * g + 1], cc = conics[3 * g + 2];

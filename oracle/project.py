@@ -236,6 +236,26 @@ def quat_to_rotmat_vjp(q, vR):
     return np.stack([vw, vx, vy, vz], axis=1)
 
 
+def cov2d_to_conic_vjp(conics, v_conic):
+    """``helpers.h:60-71``: v_cov2d = -X G X packed as (xx, xy + yx, yy)."""
+    a, b, c_ = conics[:, 0], conics[:, 1], conics[:, 2]
+    X = [[a, b], [b, c_]]
+    G = [[v_conic[:, 0], v_conic[:, 1] / 2.0], [v_conic[:, 1] / 2.0, v_conic[:, 2]]]
+    XG = [[X[r][0] * G[0][cc] + X[r][1] * G[1][cc] for cc in range(2)] for r in range(2)]
+    VS = [[-(XG[r][0] * X[0][cc] + XG[r][1] * X[1][cc]) for cc in range(2)] for r in range(2)]
+    return np.stack([VS[0][0], VS[0][1] + VS[1][0], VS[1][1]], axis=1)
+
+
+def cov2d_to_compensation_vjp(conics, compensation, v_compensation):
+    """``helpers.h:73-87``: d comp^2 / d cov2d = (1 - comp^2) conic - 0.3 I det(conic)."""
+    a, b, c_ = conics[:, 0], conics[:, 1], conics[:, 2]
+    inv_det = a * c_ - b * b
+    omc = 1.0 - compensation * compensation
+    vsq = v_compensation * 0.5 / (compensation + 1e-6)
+    return np.stack([vsq * (omc * a - 0.3 * inv_det), 2.0 * vsq * (omc * b),
+                     vsq * (omc * c_ - 0.3 * inv_det)], axis=1)
+
+
 def project_bwd(means3d, scales, quats, viewmat, radii, conics, compensation,
                 v_xy, v_depth, v_conic, v_compensation, *, img_shape, f,
                 glob_scale, mode="reference"):
@@ -286,18 +306,8 @@ def project_bwd(means3d, scales, quats, viewmat, radii, conics, compensation,
     vmean = [vmean[c] + v[8 + c] * vdep for c in range(3)]     # backward.cu:385-388
 
     # v_cov2d from v_conic (helpers.h:60-71) and v_compensation (:73-87)
-    a, b, c_ = cn[:, 0], cn[:, 1], cn[:, 2]
-    X = [[a, b], [b, c_]]
-    G = [[vcon[:, 0], vcon[:, 1] / 2.0], [vcon[:, 1] / 2.0, vcon[:, 2]]]
-    XG = [[X[r][0] * G[0][cc] + X[r][1] * G[1][cc] for cc in range(2)] for r in range(2)]
-    VS = [[-(XG[r][0] * X[0][cc] + XG[r][1] * X[1][cc]) for cc in range(2)] for r in range(2)]
-    vc2 = [VS[0][0], VS[0][1] + VS[1][0], VS[1][1]]
-    inv_det = a * c_ - b * b
-    omc = 1.0 - comp * comp
-    vsq = vcomp * 0.5 / (comp + 1e-6)
-    vc2 = [vc2[0] + vsq * (omc * a - 0.3 * inv_det),
-           vc2[1] + 2.0 * vsq * (omc * b),
-           vc2[2] + vsq * (omc * c_ - 0.3 * inv_det)]
+    v2 = cov2d_to_conic_vjp(cn, vcon) + cov2d_to_compensation_vjp(cn, comp, vcomp)
+    vc2 = [v2[:, 0], v2[:, 1], v2[:, 2]]
 
     # EWA VJP (backward.cu:421-524)
     cov3d = scale_rot_to_cov3d(s, glob_scale, q, dt)

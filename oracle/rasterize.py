@@ -228,6 +228,7 @@ def rasterize_fwd_bruteforce(gids_sorted, tile_bins, xys, conics, colors,
     H, W = img_shape
     bw = block_width
     tiles_x = (W + bw - 1) // bw
+    opacities = np.asarray(opacities, np.float64).reshape(-1)
     img = np.zeros((H, W, 3))
     fT = np.zeros((H, W))
     fI = np.zeros((H, W), np.int32)
@@ -269,6 +270,7 @@ def rasterize_bwd_bruteforce(gids_sorted, tile_bins, xys, conics, colors,
     bw = block_width
     tiles_x = (W + bw - 1) // bw
     N = len(xys)
+    opacities = np.asarray(opacities, np.float64).reshape(-1)
     vc = np.zeros((N, 3))
     vo = np.zeros(N)
     vxy = np.zeros((N, 2))
