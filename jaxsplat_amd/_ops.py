@@ -114,13 +114,14 @@ def project(means3d, scales, quats, viewmat, *, img_shape, f, c, glob_scale, cli
     types the two counts uint32; torch's uint32 lacks arithmetic, so they are
     int32 here (the values are identical, all < 2^31).
     """
+    adj = _adjoint(adjoint)
     N = means3d.shape[0] if isinstance(means3d, torch.Tensor) else -1
     _expect("means3d", means3d, (N, 3), torch.float32)
     _expect("scales", scales, (N, 3), torch.float32)
     _expect("quats", quats, (N, 4), torch.float32)
     _expect("viewmat", viewmat, (4, 4), torch.float32)
     cfg = (tuple(img_shape), tuple(f), tuple(c), float(glob_scale), float(clip_thresh),
-           int(block_width), _adjoint(adjoint))
+           int(block_width), adj)
     return _Project.apply(cfg, means3d.contiguous(), scales.contiguous(), quats.contiguous(),
                           viewmat.contiguous())
 
@@ -207,6 +208,7 @@ def rasterize(colors, opacities, background, xys, depths, radii, conics, cum_til
 
     Returns the (H, W, 3) image.
     """
+    adj = _adjoint(adjoint)
     N = colors.shape[0] if isinstance(colors, torch.Tensor) else -1
     _expect("colors", colors, (N, 3), torch.float32)
     _expect("opacities", opacities, (N, 1), torch.float32)
@@ -216,7 +218,7 @@ def rasterize(colors, opacities, background, xys, depths, radii, conics, cum_til
     _expect("radii", radii, (N, 1), torch.int32)
     _expect("conics", conics, (N, 3), torch.float32)
     _expect("cum_tiles_hit", cum_tiles_hit, (N, 1), torch.int32)
-    cfg = (tuple(img_shape), int(block_width), _adjoint(adjoint))
+    cfg = (tuple(img_shape), int(block_width), adj)
     return _Rasterize.apply(cfg, colors.contiguous(), opacities.contiguous(),
                             background.contiguous(), xys.contiguous(), depths.contiguous(),
                             radii.contiguous(), conics.contiguous(), cum_tiles_hit.contiguous())
