@@ -38,7 +38,7 @@ def parse():
     ap.add_argument("--workload", default="hd1m", choices=["small", "hd1m", "dense3m"])
     ap.add_argument("--cpu-baseline", dest="cpu_baseline", action="store_true", default=True)
     ap.add_argument("--no-cpu-baseline", dest="cpu_baseline", action="store_false")
-    ap.add_argument("--cpu-tiles", type=int, default=24)
+    ap.add_argument("--cpu-tiles", type=int, default=200)
     ap.add_argument("--stage-iters", type=int, default=10)
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="run a device-to-device copy of this many MiB first (HBM counter "
@@ -215,7 +215,7 @@ def stage_breakdown(torch, sc, cam, params, target, iters):
     achieved = bytes_bwd / t_bwd / 1e9
     roof = {"kernel": "raster_bwd_kernel", "bound": "hbm", "achieved": round(achieved, 2),
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-            "traffic": None, "avg_ms": round(acc["raster_bwd"], 4),
+            "traffic": measured_traffic(sc.name), "avg_ms": round(acc["raster_bwd"], 4),
             "bytes_per_launch": bytes_bwd}
     # whole-step algorithmic bytes (B_alg) for the combined fraction
     P = 2 if T > 256 else 1
@@ -225,6 +225,20 @@ def stage_breakdown(torch, sc, cam, params, target, iters):
              "B_alg_bytes": int(B_alg), "stage_sum_ms": round(sum(acc.values()), 4),
              "step_frac_hbm": round(B_alg / (sum(acc.values()) * 1e-3) / 1e9 / HBM_PEAK_GBS, 5)}
     return {k: round(v, 4) for k, v in acc.items()}, roof, extra
+
+
+def measured_traffic(workload):
+    """HBM bytes per launch of the dominant kernel from the committed
+    rocprofv3 FETCH_SIZE/WRITE_SIZE runs (scaled by a copy-kernel
+    calibration), or None when no measurement for this workload exists."""
+    p = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                     f"{workload}_traffic.json")
+    try:
+        with open(p) as fh:
+            rec = json.load(fh)
+        return rec["bytes_per_launch"] if rec.get("kernel") == "raster_bwd_kernel" else None
+    except (OSError, ValueError, KeyError):
+        return None
 
 
 def cpu_baseline(sc, cam, tgt_seed, n_tiles):

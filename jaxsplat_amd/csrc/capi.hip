@@ -337,6 +337,7 @@ int jaxsplat_rasterize_backward_binned(
 // ---------------------------------------------------------------------------
 
 void jaxsplat_project_fwd(void* stream, void** buffers, const char* opaque, size_t opaque_len) {
+    g_err.clear();
     const jaxsplat_descriptor* d = unpack(opaque, opaque_len);
     if (!d) return;
     int st = 0;
@@ -351,6 +352,7 @@ void jaxsplat_project_fwd(void* stream, void** buffers, const char* opaque, size
 }
 
 void jaxsplat_project_bwd(void* stream, void** buffers, const char* opaque, size_t opaque_len) {
+    g_err.clear();
     const jaxsplat_descriptor* d = unpack(opaque, opaque_len);
     if (!d) return;
     // operands: means3d, scales, quats, viewmat, cov3ds, xys, radii, conics,
@@ -380,6 +382,7 @@ static int legacy_bin(const jaxsplat_descriptor* d, void* stream, const float* x
 }
 
 void jaxsplat_rasterize_fwd(void* stream, void** buffers, const char* opaque, size_t opaque_len) {
+    g_err.clear();
     const jaxsplat_descriptor* d = unpack(opaque, opaque_len);
     if (!d || validate(d)) return;
     // operands: colors, opacities, background, xys, depths, radii, conics,
@@ -395,6 +398,7 @@ void jaxsplat_rasterize_fwd(void* stream, void** buffers, const char* opaque, si
 }
 
 void jaxsplat_rasterize_bwd(void* stream, void** buffers, const char* opaque, size_t opaque_len) {
+    g_err.clear();
     const jaxsplat_descriptor* d = unpack(opaque, opaque_len);
     if (!d || validate(d)) return;
     // operands: colors, opacities, background, xys, depths, radii, conics,
