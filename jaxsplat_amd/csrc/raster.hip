@@ -17,6 +17,9 @@
 //  * Gaussian batches staged in LDS as three float4 records
 //    (x, y, opacity, id) (a, b, c, -) (r, g, b, -) so each lane reads a batch
 //    entry with 128-bit broadcast LDS loads;
+//  * per-(Gaussian, 64-pixel group) culling bits built once per batch: the
+//    exact minimum of sigma over the group's pixel rectangle against
+//    ln(255 o); each wave then walks only its set bits (scalar s_ff1);
 //  * per-wave early exit from the batch loop (64-lane ballot) and a per-tile
 //    exit once every pixel is done;
 //  * backward: the batch walk starts at the tile's largest final_idx (nothing
@@ -61,28 +64,65 @@ __device__ __forceinline__ float4 group_rect(int p0, int bw, int x_org, int y_or
     return make_float4((float)x0 + 0.5f, (float)x1 + 0.5f, (float)y0 + 0.5f, (float)y1 + 0.5f);
 }
 
-// Half extents of the axis-aligned box holding every pixel centre at which
-// o * exp(-sigma) >= 1/255, i.e. sigma <= ln(255 o).  The threshold is
-// inflated by 0.01 and the extents by 0.01% + 1e-3 px so the box also holds
-// every pixel the float32 per-pixel test accepts.  Returns false when no pixel
-// can pass (o <= ~1/255).  Degenerate conics give an unbounded box.
-__device__ __forceinline__ bool alpha_box(float a, float b, float c, float o, float& ex,
-                                          float& ey) {
-    const float smax = __logf(255.f * o) + 0.01f;
-    if (!(smax > 0.f)) return false;
-    const float det = a * c - b * b;
-    if (!(det > 0.f)) {
-        ex = ey = __builtin_inff();
-        return true;
-    }
-    const float k = 2.f * smax * __builtin_amdgcn_rcpf(det);
-    ex = sqrtf(fmaxf(k * c, 0.f)) * 1.0001f + 1e-3f;
-    ey = sqrtf(fmaxf(k * a, 0.f)) * 1.0001f + 1e-3f;
-    return true;
+// Culling of a (Gaussian, pixel group) pair.  A pixel can pass the per-pixel
+// test only where o * exp(-sigma) >= 1/255, i.e. sigma <= ln(255 o).  The test
+// compares that threshold (plus 0.01, far above float32 rounding of sigma)
+// with the exact minimum of the quadratic form sigma over the group's
+// pixel-centre rectangle, so a culled pair is one the per-pixel test would
+// reject anyway.  Conics that are not positive definite are never culled.
+struct CullInfo {
+    float gx, gy, a, b, c, ra, rc, smax;
+    bool any;       // false: no pixel can pass (o <= ~1/255)
+    bool exact;     // false: degenerate conic, keep every group
+};
+
+__device__ __forceinline__ CullInfo cull_info(float gx, float gy, float a, float b, float c,
+                                              float o) {
+    CullInfo k;
+    k.gx = gx;
+    k.gy = gy;
+    k.a = a;
+    k.b = b;
+    k.c = c;
+    k.smax = __logf(255.f * o) + 0.01f;
+    k.any = k.smax > 0.f;
+    k.exact = a > 0.f && c > 0.f && a * c - b * b > 0.f;
+    k.ra = __builtin_amdgcn_rcpf(a);
+    k.rc = __builtin_amdgcn_rcpf(c);
+    return k;
 }
 
-__device__ __forceinline__ bool box_hits(float gx, float gy, float ex, float ey, float4 r) {
-    return gx + ex >= r.x && gx - ex <= r.y && gy + ey >= r.z && gy - ey <= r.w;
+__device__ __forceinline__ float quad(const CullInfo& k, float dx, float dy) {
+    return 0.5f * (k.a * dx * dx + k.c * dy * dy) + k.b * dx * dy;
+}
+
+__device__ __forceinline__ bool group_hit(const CullInfo& k, float4 r) {
+    if (!k.any) return false;
+    if (!k.exact) return true;
+    if (k.gx >= r.x && k.gx <= r.y && k.gy >= r.z && k.gy <= r.w) return true;
+    // centre outside: the minimum lies on the boundary; minimise along each edge
+    float best;
+    {
+        const float dx = r.x - k.gx;
+        const float dy = fminf(fmaxf(-k.b * dx * k.rc, r.z - k.gy), r.w - k.gy);
+        best = quad(k, dx, dy);
+    }
+    {
+        const float dx = r.y - k.gx;
+        const float dy = fminf(fmaxf(-k.b * dx * k.rc, r.z - k.gy), r.w - k.gy);
+        best = fminf(best, quad(k, dx, dy));
+    }
+    {
+        const float dy = r.z - k.gy;
+        const float dx = fminf(fmaxf(-k.b * dy * k.ra, r.x - k.gx), r.y - k.gx);
+        best = fminf(best, quad(k, dx, dy));
+    }
+    {
+        const float dy = r.w - k.gy;
+        const float dx = fminf(fmaxf(-k.b * dy * k.ra, r.x - k.gx), r.y - k.gx);
+        best = fminf(best, quad(k, dx, dy));
+    }
+    return best <= k.smax;
 }
 
 // ---------------------------------------------------------------------------
@@ -148,11 +188,9 @@ __global__ __launch_bounds__(256) void raster_fwd_kernel(
             s_a[tid] = make_float4(xy.x, xy.y, o, __int_as_float(g));
             s_b[tid] = make_float4(ca, cb_, cc, 0.f);
             s_c[tid] = make_float4(colors[3 * g + 0], colors[3 * g + 1], colors[3 * g + 2], 0.f);
-            float ex, ey;
-            if (alpha_box(ca, cb_, cc, o, ex, ey)) {
-                for (int G = 0; G < nwaves; ++G)
-                    if (s_rok[G] && box_hits(xy.x, xy.y, ex, ey, s_rect[G])) bits |= 1u << G;
-            }
+            const CullInfo ci = cull_info(xy.x, xy.y, ca, cb_, cc, o);
+            for (int G = 0; G < nwaves; ++G)
+                if (s_rok[G] && group_hit(ci, s_rect[G])) bits |= 1u << G;
         }
         for (int G = 0; G < nwaves; ++G) {
             const uint64_t m = ballot((bits >> G) & 1u);
@@ -343,11 +381,9 @@ __global__ __launch_bounds__(128) void raster_bwd_kernel(
             s_a[tid] = make_float4(xy.x, xy.y, o, __int_as_float(g));
             s_b[tid] = make_float4(ca, cb_, cc, 0.f);
             s_c[tid] = make_float4(colors[3 * g + 0], colors[3 * g + 1], colors[3 * g + 2], 0.f);
-            float ex, ey;
-            if (alpha_box(ca, cb_, cc, o, ex, ey)) {
-                for (int G = 0; G < ngroups; ++G)
-                    if (s_rok[G] && box_hits(xy.x, xy.y, ex, ey, s_rect[G])) bits |= 1u << G;
-            }
+            const CullInfo ci = cull_info(xy.x, xy.y, ca, cb_, cc, o);
+            for (int G = 0; G < ngroups; ++G)
+                if (s_rok[G] && group_hit(ci, s_rect[G])) bits |= 1u << G;
         }
         for (int G = 0; G < ngroups; ++G) {
             const uint64_t m = ballot((bits >> G) & 1u);
