@@ -13,7 +13,11 @@ Prints ONE JSON line.  Besides the contract fields it carries:
   stream, against the 8 TB/s HBM peak;
 * ``cpu_baseline``: the float64 NumPy oracle timed on a bounded sample of the
   same workload on this host (1 core);
-* ``stages``: per-stage milliseconds of the same pipeline.
+* ``stages``: per-stage milliseconds of the same pipeline;
+* ``batched``: the multi-view workload ``orbit64`` (64 cameras around one
+  500,000-Gaussian scene, 800x800 each): one HIP-graph replay of the batched
+  forward + L2 gradient + backward of all 64 views (``jaxsplat_amd.batch``)
+  against 64 plain ``render`` fwd+bwd calls, both in Mpix/s over all views.
 """
 
 from __future__ import annotations
@@ -40,6 +44,9 @@ def parse():
     ap.add_argument("--no-cpu-baseline", dest="cpu_baseline", action="store_false")
     ap.add_argument("--cpu-tiles", type=int, default=200)
     ap.add_argument("--stage-iters", type=int, default=10)
+    ap.add_argument("--batched", dest="batched", action="store_true", default=True)
+    ap.add_argument("--no-batched", dest="batched", action="store_false")
+    ap.add_argument("--batched-steps", type=int, default=5)
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="run a device-to-device copy of this many MiB first (HBM counter "
                          "calibration under rocprofv3 --pmc)")
@@ -109,6 +116,8 @@ def main():
 
     stages, roof, extra = stage_breakdown(torch, sc, cam, params, target, args.stage_iters)
 
+    batched = batched_vs_plain(torch, args.batched_steps) if args.batched else None
+
     cpu = cpu_baseline(sc, cam, tgt_seed, args.cpu_tiles) if args.cpu_baseline else None
 
     rec = {
@@ -132,6 +141,7 @@ def main():
         "cpu_baseline": cpu,
         "stages_ms": stages,
         "stats": extra,
+        "batched": batched,
         "wall_s": round(wall, 3),
     }
     print(json.dumps(rec), flush=True)
@@ -225,6 +235,116 @@ def stage_breakdown(torch, sc, cam, params, target, iters):
              "B_alg_bytes": int(B_alg), "stage_sum_ms": round(sum(acc.values()), 4),
              "step_frac_hbm": round(B_alg / (sum(acc.values()) * 1e-3) / 1e9 / HBM_PEAK_GBS, 5)}
     return {k: round(v, 4) for k, v in acc.items()}, roof, extra
+
+
+def batched_vs_plain(torch, steps, warmup=2):
+    """orbit64: all 64 views per step, batched in one HIP graph vs 64 plain
+    render calls.  Both steps compute the L2 loss gradient against seeded
+    targets and the full backward to the 500k Gaussians."""
+    import jaxsplat_amd
+    from jaxsplat_amd import BatchPlan, ViewSpec, _lib, scenes
+    from jaxsplat_amd._ops import _p, _stream
+    dev = torch.device("cuda:0")
+    sc = scenes.make("orbit64")
+    V = len(sc.cameras)
+    Hh, Ww = sc.cameras[0].img_shape
+    P = V * Hh * Ww
+    views = [ViewSpec(0, c.img_shape, c.f, c.c) for c in sc.cameras]
+    params = [torch.from_numpy(a).to(dev) for a in sc.params()]
+    vms = torch.from_numpy(np.stack([c.viewmat for c in sc.cameras])).to(dev)
+    bgs = torch.zeros((V, 3), dtype=torch.float32, device=dev)
+    target = torch.from_numpy(np.concatenate(
+        [scenes.target_image((Hh, Ww), 2003 + v).reshape(-1, 3) for v in range(V)])).to(dev)
+    L = _lib.lib()
+    # size the intersection capacity from one eager forward
+    probe = BatchPlan([sc.n], views, glob_scale=sc.glob_scale, clip_thresh=sc.clip_thresh,
+                      block_width=sc.block_width, isect_capacity=1 << 20, device=dev)
+    fT = torch.empty((P,), dtype=torch.float32, device=dev)
+    fI = torch.empty((P,), dtype=torch.int32, device=dev)
+    img = torch.empty((P, 3), dtype=torch.float32, device=dev)
+    _lib.check(L.jaxsplat_batch_forward(probe._h, *[_p(x) for x in params], _p(vms), _p(bgs),
+                                        _p(fT), _p(fI), _p(img), _p(probe.workspace),
+                                        probe.workspace_bytes, _stream(dev)))
+    try:
+        probe.check()
+    except _lib.JaxsplatError as e:
+        if e.code != _lib.ERR_CAPACITY:
+            raise
+    isect = probe.last_intersections
+    del probe
+    plan = BatchPlan([sc.n], views, glob_scale=sc.glob_scale, clip_thresh=sc.clip_thresh,
+                     block_width=sc.block_width, isect_capacity=int(isect * 1.1) + 1024,
+                     device=dev)
+    grads = [torch.empty_like(x) for x in params]
+    scale = 2.0 / (3.0 * Hh * Ww)           # d mean((img - target)^2) per view
+
+    def step():
+        s = _stream(dev)
+        _lib.check(L.jaxsplat_batch_forward(plan._h, *[_p(x) for x in params], _p(vms),
+                                            _p(bgs), _p(fT), _p(fI), _p(img),
+                                            _p(plan.workspace), plan.workspace_bytes, s))
+        v_img = (img - target) * scale
+        _lib.check(L.jaxsplat_batch_backward(plan._h, *[_p(x) for x in params], _p(vms),
+                                             _p(bgs), _p(fT), _p(fI), _p(v_img),
+                                             *[_p(g) for g in grads], 0, _p(plan.workspace),
+                                             plan.workspace_bytes, s))
+
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        for _ in range(warmup):
+            step()
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    plan.check()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        step()
+    for _ in range(warmup):
+        graph.replay()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        graph.replay()
+    e1.record()
+    torch.cuda.synchronize()
+    ms_b = e0.elapsed_time(e1) / steps
+    plan.check()
+
+    # plain: one render + loss + backward per view (the reference's usage)
+    pp = [x.clone().requires_grad_(True) for x in params]
+    kw = dict(background=bgs[0], glob_scale=sc.glob_scale, clip_thresh=sc.clip_thresh,
+              block_size=sc.block_width)
+    tv = target.view(V, Hh, Ww, 3)
+
+    def plain_step():
+        for x in pp:
+            x.grad = None
+        for v, c in enumerate(sc.cameras):
+            out = jaxsplat_amd.render(*pp, viewmat=vms[v], img_shape=c.img_shape, f=c.f, c=c.c,
+                                      **kw)
+            ((out - tv[v]) ** 2).mean().backward()
+
+    for _ in range(warmup):
+        plain_step()
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(steps):
+        plain_step()
+    e1.record()
+    torch.cuda.synchronize()
+    ms_p = e0.elapsed_time(e1) / steps
+    # the two steps produce the same gradients (float atomics aside)
+    gb = torch.cat([g.reshape(-1) for g in grads])
+    gp = torch.cat([x.grad.reshape(-1) for x in pp])
+    rel = float((gb - gp).abs().max() / gp.abs().max().clamp_min(1e-30))
+    return {"workload": "orbit64", "views": V, "num_gaussians": sc.n, "img_shape": [Hh, Ww],
+            "num_intersects": isect, "ms_batched_graph": round(ms_b, 3),
+            "ms_plain_64_calls": round(ms_p, 3),
+            "mpix_s_batched": round(P / (ms_b * 1e-3) / 1e6, 1),
+            "mpix_s_plain": round(P / (ms_p * 1e-3) / 1e6, 1),
+            "speedup": round(ms_p / ms_b, 3), "grad_max_rel_diff": rel}
 
 
 def measured_traffic(workload):

@@ -182,6 +182,78 @@ int jaxsplat_rasterize_backward_binned(
     float* v_colors, float* v_opacity, float* v_xy, float* v_xy_abs, float* v_conic,
     int adjoint, void* stream);
 
+/* ------------------------------------------------------------------------
+ * Batched rendering: many views of one or more scenes in one pass.
+ *
+ * No counterpart in the reference, which renders one view per call
+ * (jaxsplat/__init__.py:6-70, two sorts and several host syncs each).  A
+ * batch plan fixes the layout; every forward/backward then runs the whole
+ * batch with kernel launches only (no allocation, no host synchronisation),
+ * so a caller may capture a full step in one HIP graph.
+ *
+ * Layout: the parameter arrays (means3d (G,3), scales (G,3), quats (G,4),
+ * colors (G,3), opacities (G,1)) hold the scenes end to end
+ * (scene s = rows [first_gaussian, first_gaussian + num_gaussians), with
+ * first_gaussian of scene s equal to the rows of scenes 0..s-1).  Views are
+ * grouped by ascending scene; view v has a row-major 4x4 viewmats[v] and a
+ * background backgrounds[v] (3 floats).  The image-sized buffers (out_img
+ * (P,3), final_Ts (P), final_idx (P)) hold the views' H x W images end to end;
+ * jaxsplat_batch_pixel_offset gives each view's first pixel.
+ *
+ * Per view the result is the plain path's render of that view (same kernels,
+ * same per-tile order), except that a view with no intersections has
+ * final_Ts = 1 (the plain path writes 0 there, as the reference does).
+ * ------------------------------------------------------------------------ */
+typedef struct jaxsplat_scene {
+    int32_t first_gaussian;
+    int32_t num_gaussians;
+} jaxsplat_scene;
+
+typedef struct jaxsplat_view {
+    int32_t scene;
+    int32_t img_h, img_w;
+    float fx, fy, cx, cy;
+} jaxsplat_view;
+
+typedef struct jaxsplat_batch jaxsplat_batch;
+
+/* Validates the layout and uploads the view table (allocates a small device
+ * table on the current device; this is the only allocation of a batch).
+ * isect_capacity bounds the total (view, tile) intersections of one forward. */
+int jaxsplat_batch_create(jaxsplat_batch** out, const jaxsplat_scene* scenes, int32_t num_scenes,
+                          const jaxsplat_view* views, int32_t num_views, float glob_scale,
+                          float clip_thresh, uint32_t block_width, int64_t isect_capacity);
+void jaxsplat_batch_destroy(jaxsplat_batch* b);
+size_t jaxsplat_batch_workspace_size(const jaxsplat_batch* b);
+int64_t jaxsplat_batch_num_pixels(const jaxsplat_batch* b);
+int64_t jaxsplat_batch_num_gaussians(const jaxsplat_batch* b);
+int64_t jaxsplat_batch_pixel_offset(const jaxsplat_batch* b, int32_t view);
+
+/* Forward of every view.  The workspace keeps the residuals the backward
+ * reads: pass the same workspace to jaxsplat_batch_backward with no other
+ * forward in between. */
+int jaxsplat_batch_forward(const jaxsplat_batch* b, const float* means3d, const float* scales,
+                           const float* quats, const float* colors, const float* opacities,
+                           const float* viewmats, const float* backgrounds, float* final_Ts,
+                           int32_t* final_idx, float* out_img, void* workspace,
+                           size_t workspace_bytes, void* stream);
+
+/* Backward of every view for the cotangent v_out_img (P,3): writes the
+ * gradients of all G Gaussians (summed over the views of their scene). */
+int jaxsplat_batch_backward(const jaxsplat_batch* b, const float* means3d, const float* scales,
+                            const float* quats, const float* colors, const float* opacities,
+                            const float* viewmats, const float* backgrounds,
+                            const float* final_Ts, const int32_t* final_idx,
+                            const float* v_out_img, float* v_means3d, float* v_scales,
+                            float* v_quats, float* v_colors, float* v_opacities, int adjoint,
+                            void* workspace, size_t workspace_bytes, void* stream);
+
+/* Synchronous status of the last forward in `workspace`: the exact number of
+ * intersections, and JAXSPLAT_ERR_CAPACITY if it exceeded isect_capacity (the
+ * forward's outputs are then invalid) or a look-back wait timed out. */
+int jaxsplat_batch_check(const jaxsplat_batch* b, const void* workspace,
+                         int64_t* num_intersects, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -13,8 +13,10 @@ including its fwd/bwd alpha-clamp mismatch and its unclamped EWA Jacobian;
 from __future__ import annotations
 
 from ._ops import ShapeDtypeError, bin_and_sort, project, rasterize
+from .batch import BatchPlan, BatchRenderer, ViewSpec, render_batch
 
-__all__ = ["render", "project", "rasterize", "bin_and_sort", "ShapeDtypeError"]
+__all__ = ["render", "project", "rasterize", "bin_and_sort", "ShapeDtypeError", "BatchPlan",
+           "BatchRenderer", "ViewSpec", "render_batch"]
 
 
 def render(means3d, scales, quats, colors, opacities, *, viewmat, background, img_shape, f, c,

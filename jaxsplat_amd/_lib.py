@@ -27,6 +27,9 @@ EXPORTS = (
     "jaxsplat_project_forward", "jaxsplat_project_backward",
     "jaxsplat_bin_workspace_size", "jaxsplat_bin_and_sort", "jaxsplat_bin_check",
     "jaxsplat_rasterize_forward_binned", "jaxsplat_rasterize_backward_binned",
+    "jaxsplat_batch_create", "jaxsplat_batch_destroy", "jaxsplat_batch_workspace_size",
+    "jaxsplat_batch_num_pixels", "jaxsplat_batch_num_gaussians", "jaxsplat_batch_pixel_offset",
+    "jaxsplat_batch_forward", "jaxsplat_batch_backward", "jaxsplat_batch_check",
 )
 
 
@@ -52,6 +55,18 @@ class Descriptor(ctypes.Structure):
 
 
 assert ctypes.sizeof(Descriptor) == 80
+
+
+class Scene(ctypes.Structure):
+    """Mirror of ``jaxsplat_scene``."""
+    _fields_ = [("first_gaussian", ctypes.c_int32), ("num_gaussians", ctypes.c_int32)]
+
+
+class View(ctypes.Structure):
+    """Mirror of ``jaxsplat_view``."""
+    _fields_ = [("scene", ctypes.c_int32), ("img_h", ctypes.c_int32), ("img_w", ctypes.c_int32),
+                ("fx", ctypes.c_float), ("fy", ctypes.c_float), ("cx", ctypes.c_float),
+                ("cy", ctypes.c_float)]
 
 
 class JaxsplatError(RuntimeError):
@@ -86,6 +101,16 @@ _SIGS = {
     "jaxsplat_bin_check": (I32, [DP, U32, P, P]),
     "jaxsplat_rasterize_forward_binned": (I32, [DP, P, P, U32] + [P] * 8 + [P]),
     "jaxsplat_rasterize_backward_binned": (I32, [DP, P, P, U32] + [P] * 14 + [I32, P]),
+    "jaxsplat_batch_create": (I32, [ctypes.POINTER(P), ctypes.POINTER(Scene), I32,
+                                    ctypes.POINTER(View), I32, F32, F32, U32, ctypes.c_int64]),
+    "jaxsplat_batch_destroy": (None, [P]),
+    "jaxsplat_batch_workspace_size": (SZ, [P]),
+    "jaxsplat_batch_num_pixels": (ctypes.c_int64, [P]),
+    "jaxsplat_batch_num_gaussians": (ctypes.c_int64, [P]),
+    "jaxsplat_batch_pixel_offset": (ctypes.c_int64, [P, I32]),
+    "jaxsplat_batch_forward": (I32, [P] + [P] * 10 + [P, SZ, P]),
+    "jaxsplat_batch_backward": (I32, [P] + [P] * 15 + [I32, P, SZ, P]),
+    "jaxsplat_batch_check": (I32, [P, P, ctypes.POINTER(ctypes.c_int64), P]),
 }
 
 

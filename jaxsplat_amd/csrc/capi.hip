@@ -2,6 +2,7 @@
 // include/jaxsplat_amd.h): argument validation, descriptor -> camera
 // conversion, workspace carving, and the four reference-compatible XLA
 // custom-call targets.
+#include <algorithm>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -9,6 +10,7 @@
 #include <mutex>
 #include <string>
 #include <utility>
+#include <vector>
 
 #include "../../include/jaxsplat_amd.h"
 #include "common.hpp"
@@ -413,6 +415,289 @@ void jaxsplat_rasterize_bwd(void* stream, void** buffers, const char* opaque, si
     jaxsplat_rasterize_backward_binned(d, gids, bins, isect, F(3), F(6), F(0), F(1), F(2), F(8),
                                        I(9), F(10), F(11), F(12), F(13), F(14), F(15), F(16),
                                        JAXSPLAT_ADJOINT_REFERENCE, stream);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batched rendering
+// ---------------------------------------------------------------------------
+
+struct jaxsplat_batch {
+    int device = 0;
+    int num_scenes = 0, num_views = 0;
+    int n_items = 0, n_gauss = 0, max_view_n = 0, max_scene_n = 0;
+    int total_tiles = 0, max_view_tiles = 0;
+    long long num_pixels = 0;
+    float glob_scale = 1.f, clip_thresh = 0.f;
+    int block_width = 16;
+    jsa::BatchBinPlan bin{};
+    std::vector<jsa::ViewRec> views;
+    std::vector<jsa::SceneRec> scenes;
+    jsa::ViewRec* d_views = nullptr;
+    jsa::SceneRec* d_scenes = nullptr;
+    // workspace layout
+    size_t xys = 0, conics = 0, radii = 0, dkeys = 0, v_xy = 0, v_conic = 0, items = 0,
+           bins = 0, bin_scratch = 0, ws_bytes = 0;
+};
+
+namespace {
+
+jsa::RasterGrid batch_grid(const jaxsplat_batch* b) {
+    jsa::RasterGrid g{};
+    g.v0 = b->views[0];
+    g.views = b->d_views;
+    g.num_views = b->num_views;
+    g.total_tiles = b->total_tiles;
+    g.block_width = b->block_width;
+    g.n_items = b->n_items;
+    return g;
+}
+
+int check_batch_ws(const jaxsplat_batch* b, const void* ws, size_t bytes, const char* where) {
+    if (!b) return fail(JAXSPLAT_ERR_INVALID, "%s: batch is NULL", where);
+    if (!ws || bytes < b->ws_bytes)
+        return fail(JAXSPLAT_ERR_WORKSPACE, "%s: workspace too small (%zu < %zu)", where, bytes,
+                    b->ws_bytes);
+    return JAXSPLAT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int jaxsplat_batch_create(jaxsplat_batch** out, const jaxsplat_scene* scenes, int32_t num_scenes,
+                          const jaxsplat_view* views, int32_t num_views, float glob_scale,
+                          float clip_thresh, uint32_t block_width, int64_t isect_capacity) {
+    g_err.clear();
+    if (!out) return fail(JAXSPLAT_ERR_INVALID, "out is NULL");
+    *out = nullptr;
+    if (!scenes || num_scenes < 1) return fail(JAXSPLAT_ERR_INVALID, "need at least one scene");
+    if (!views || num_views < 1 || num_views > 4096)
+        return fail(JAXSPLAT_ERR_INVALID, "num_views must be in [1, 4096], got %d", num_views);
+    if (block_width < 1 || block_width > 16)
+        return fail(JAXSPLAT_ERR_INVALID, "block_width must be in [1, 16], got %u", block_width);
+    if (!(clip_thresh >= 0.f)) return fail(JAXSPLAT_ERR_INVALID, "clip_thresh must be >= 0");
+    if (isect_capacity < 1 || isect_capacity > 0x7FFFFF00LL)
+        return fail(JAXSPLAT_ERR_INVALID, "isect_capacity must be in [1, 2^31 - 256]");
+    auto b = new jaxsplat_batch();
+    b->num_scenes = num_scenes;
+    b->num_views = num_views;
+    b->glob_scale = glob_scale;
+    b->clip_thresh = clip_thresh;
+    b->block_width = (int)block_width;
+    long long g = 0;
+    for (int s = 0; s < num_scenes; ++s) {
+        if (scenes[s].num_gaussians < 0 || scenes[s].first_gaussian != g) {
+            delete b;
+            return fail(JAXSPLAT_ERR_INVALID,
+                        "scene %d: scenes must hold consecutive rows (first_gaussian %d, "
+                        "expected %lld)", s, scenes[s].first_gaussian, g);
+        }
+        g += scenes[s].num_gaussians;
+        jsa::SceneRec r{scenes[s].first_gaussian, scenes[s].num_gaussians, 0, 0};
+        b->scenes.push_back(r);
+        b->max_scene_n = std::max(b->max_scene_n, scenes[s].num_gaussians);
+    }
+    if (g > (1LL << 30)) {
+        delete b;
+        return fail(JAXSPLAT_ERR_INVALID, "too many Gaussians (%lld)", g);
+    }
+    b->n_gauss = (int)g;
+    long long items = 0, tiles = 0, pix = 0;
+    int prev_scene = 0;
+    for (int s = 0; s < num_scenes; ++s) b->scenes[s].v_begin = b->scenes[s].v_end = 0;
+    for (int v = 0; v < num_views; ++v) {
+        const jaxsplat_view& w = views[v];
+        if (w.scene < prev_scene || w.scene >= num_scenes) {
+            delete b;
+            return fail(JAXSPLAT_ERR_INVALID, "view %d: views must be grouped by ascending scene",
+                        v);
+        }
+        if (w.img_h < 1 || w.img_w < 1 || !(w.fx > 0.f) || !(w.fy > 0.f)) {
+            delete b;
+            return fail(JAXSPLAT_ERR_INVALID, "view %d: bad image size or focal length", v);
+        }
+        if (w.scene != prev_scene || v == 0) b->scenes[w.scene].v_begin = v;
+        b->scenes[w.scene].v_end = v + 1;
+        prev_scene = w.scene;
+        jsa::ViewRec r{};
+        r.img_w = w.img_w;
+        r.img_h = w.img_h;
+        r.tiles_x = (w.img_w + (int)block_width - 1) / (int)block_width;
+        r.tiles_y = (w.img_h + (int)block_width - 1) / (int)block_width;
+        r.fx = w.fx;
+        r.fy = w.fy;
+        r.cx = w.cx;
+        r.cy = w.cy;
+        r.tan_fovx = (float)(0.5 * (double)w.img_w / (double)w.fx);
+        r.tan_fovy = (float)(0.5 * (double)w.img_h / (double)w.fy);
+        r.g0 = scenes[w.scene].first_gaussian;
+        r.n = scenes[w.scene].num_gaussians;
+        r.item_off = (int)items;
+        r.tile_off = (int)tiles;
+        r.view = v;
+        r.pix_off = pix;
+        items += r.n;
+        tiles += (long long)r.tiles_x * r.tiles_y;
+        pix += (long long)w.img_w * w.img_h;
+        b->max_view_n = std::max(b->max_view_n, r.n);
+        b->max_view_tiles = std::max(b->max_view_tiles, r.tiles_x * r.tiles_y);
+        b->views.push_back(r);
+    }
+    if (items > 0x7FFFFF00LL || tiles > (1LL << 30)) {
+        delete b;
+        return fail(JAXSPLAT_ERR_INVALID, "batch too large (%lld items, %lld tiles)", items,
+                    tiles);
+    }
+    b->n_items = (int)items;
+    b->total_tiles = (int)tiles;
+    b->num_pixels = pix;
+    b->bin = jsa::batch_bin_plan(b->n_items, num_views, b->max_view_tiles, (int)isect_capacity);
+    if (b->bin.tile_bits + b->bin.view_bits > 31) {
+        delete b;
+        return fail(JAXSPLAT_ERR_INVALID, "views x tiles exceed 2^31 bins");
+    }
+    for (int v = 0; v < num_views; ++v) b->views[v].bin_off = v << b->bin.tile_bits;
+    // workspace: item rows, backward item gradients, sorted ids, bins, binning scratch
+    size_t o = 0;
+    const size_t n = (size_t)(b->n_items > 0 ? b->n_items : 1);
+    b->xys = o; o = align256(o + 8 * n);
+    b->conics = o; o = align256(o + 12 * n);
+    b->radii = o; o = align256(o + 4 * n);
+    b->dkeys = o; o = align256(o + 4 * n);
+    b->v_xy = o; o = align256(o + 8 * n);
+    b->v_conic = o; o = align256(o + 12 * n);
+    b->items = o; o = align256(o + 4 * (size_t)isect_capacity);
+    b->bins = o; o = align256(o + 8 * (size_t)b->bin.num_bins);
+    b->bin_scratch = o; o = align256(o + b->bin.scratch_bytes);
+    b->ws_bytes = o;
+    // device tables
+    hipError_t e = hipGetDevice(&b->device);
+    if (e == hipSuccess) e = hipMalloc(&b->d_views, sizeof(jsa::ViewRec) * num_views);
+    if (e == hipSuccess) e = hipMalloc(&b->d_scenes, sizeof(jsa::SceneRec) * num_scenes);
+    if (e == hipSuccess)
+        e = hipMemcpy(b->d_views, b->views.data(), sizeof(jsa::ViewRec) * num_views,
+                      hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(b->d_scenes, b->scenes.data(), sizeof(jsa::SceneRec) * num_scenes,
+                      hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        jaxsplat_batch_destroy(b);
+        return hip_fail(e, "batch_create");
+    }
+    *out = b;
+    return JAXSPLAT_OK;
+}
+
+void jaxsplat_batch_destroy(jaxsplat_batch* b) {
+    if (!b) return;
+    if (b->d_views) (void)hipFree(b->d_views);
+    if (b->d_scenes) (void)hipFree(b->d_scenes);
+    delete b;
+}
+
+size_t jaxsplat_batch_workspace_size(const jaxsplat_batch* b) { return b ? b->ws_bytes : 0; }
+int64_t jaxsplat_batch_num_pixels(const jaxsplat_batch* b) { return b ? b->num_pixels : 0; }
+int64_t jaxsplat_batch_num_gaussians(const jaxsplat_batch* b) { return b ? b->n_gauss : 0; }
+int64_t jaxsplat_batch_pixel_offset(const jaxsplat_batch* b, int32_t view) {
+    if (!b || view < 0 || view > b->num_views) return -1;
+    return view == b->num_views ? b->num_pixels : b->views[view].pix_off;
+}
+
+int jaxsplat_batch_forward(const jaxsplat_batch* b, const float* means3d, const float* scales,
+                           const float* quats, const float* colors, const float* opacities,
+                           const float* viewmats, const float* backgrounds, float* final_Ts,
+                           int32_t* final_idx, float* out_img, void* workspace,
+                           size_t workspace_bytes, void* stream) {
+    g_err.clear();
+    if (int st = check_batch_ws(b, workspace, workspace_bytes, "batch_forward")) return st;
+    if (!viewmats || !backgrounds || !final_Ts || !final_idx || !out_img ||
+        (b->n_gauss > 0 && (!means3d || !scales || !quats || !colors || !opacities)))
+        return fail(JAXSPLAT_ERR_INVALID, "batch_forward: NULL required pointer");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    uint8_t* w = static_cast<uint8_t*>(workspace);
+    float* xys = reinterpret_cast<float*>(w + b->xys);
+    float* conics = reinterpret_cast<float*>(w + b->conics);
+    int* radii = reinterpret_cast<int*>(w + b->radii);
+    uint32_t* dkeys = reinterpret_cast<uint32_t*>(w + b->dkeys);
+    int* items = reinterpret_cast<int*>(w + b->items);
+    int* bins = reinterpret_cast<int*>(w + b->bins);
+    JSA_CHECK_HIP(jsa::project_batch_fwd(b->d_views, b->num_views, b->max_view_n, b->glob_scale,
+                                         b->clip_thresh, b->block_width, means3d, scales, quats,
+                                         viewmats, xys, conics, radii, dkeys, s),
+                  "batch_forward: project");
+    JSA_CHECK_HIP(jsa::batch_bin(b->bin, b->d_views, b->block_width, xys, radii, dkeys, items,
+                                 bins, w + b->bin_scratch, s),
+                  "batch_forward: bin");
+    JSA_CHECK_HIP(jsa::raster_fwd_grid(batch_grid(b), items, bins, xys, conics, colors, opacities,
+                                       backgrounds, final_Ts, final_idx, out_img, s),
+                  "batch_forward: rasterize");
+    return JAXSPLAT_OK;
+}
+
+int jaxsplat_batch_backward(const jaxsplat_batch* b, const float* means3d, const float* scales,
+                            const float* quats, const float* colors, const float* opacities,
+                            const float* viewmats, const float* backgrounds,
+                            const float* final_Ts, const int32_t* final_idx,
+                            const float* v_out_img, float* v_means3d, float* v_scales,
+                            float* v_quats, float* v_colors, float* v_opacities, int adjoint,
+                            void* workspace, size_t workspace_bytes, void* stream) {
+    g_err.clear();
+    if (int st = check_batch_ws(b, workspace, workspace_bytes, "batch_backward")) return st;
+    if (adjoint != JAXSPLAT_ADJOINT_REFERENCE && adjoint != JAXSPLAT_ADJOINT_CONSISTENT)
+        return fail(JAXSPLAT_ERR_INVALID, "unknown adjoint mode %d", adjoint);
+    if (!viewmats || !backgrounds || !final_Ts || !final_idx || !v_out_img ||
+        (b->n_gauss > 0 && (!means3d || !scales || !quats || !colors || !opacities ||
+                            !v_means3d || !v_scales || !v_quats || !v_colors || !v_opacities)))
+        return fail(JAXSPLAT_ERR_INVALID, "batch_backward: NULL required pointer");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    uint8_t* w = static_cast<uint8_t*>(workspace);
+    const float* xys = reinterpret_cast<const float*>(w + b->xys);
+    const float* conics = reinterpret_cast<const float*>(w + b->conics);
+    const int* radii = reinterpret_cast<const int*>(w + b->radii);
+    float* v_xy = reinterpret_cast<float*>(w + b->v_xy);
+    float* v_conic = reinterpret_cast<float*>(w + b->v_conic);
+    const int* items = reinterpret_cast<const int*>(w + b->items);
+    const int* bins = reinterpret_cast<const int*>(w + b->bins);
+    const size_t G = (size_t)b->n_gauss, n = (size_t)b->n_items;
+    float* ptrs[5] = {v_colors, v_opacities, v_xy, v_conic, nullptr};
+    const size_t cnt[5] = {3 * G, G, 2 * n, 3 * n, 0};
+    JSA_CHECK_HIP(jsa::zero_grads(ptrs, cnt, 5, s), "batch_backward: zero");
+    const int cons = adjoint == JAXSPLAT_ADJOINT_CONSISTENT;
+    JSA_CHECK_HIP(jsa::raster_bwd_grid(batch_grid(b), items, bins, xys, conics, colors, opacities,
+                                       backgrounds, final_Ts, final_idx, v_out_img, nullptr,
+                                       v_colors, v_opacities, v_xy, nullptr, v_conic, cons, s),
+                  "batch_backward: rasterize");
+    JSA_CHECK_HIP(jsa::project_batch_bwd(b->d_scenes, b->num_scenes, b->max_scene_n, b->d_views,
+                                         b->glob_scale, b->clip_thresh, means3d, scales, quats,
+                                         viewmats, radii, conics, v_xy, v_conic, v_means3d,
+                                         v_scales, v_quats, cons, s),
+                  "batch_backward: project");
+    return JAXSPLAT_OK;
+}
+
+int jaxsplat_batch_check(const jaxsplat_batch* b, const void* workspace,
+                         int64_t* num_intersects, void* stream) {
+    g_err.clear();
+    if (int st = check_batch_ws(b, workspace, b ? b->ws_bytes : 0, "batch_check")) return st;
+    const uint8_t* w = static_cast<const uint8_t*>(workspace) + b->bin_scratch;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    uint32_t err = 0;
+    unsigned long long total = 0;
+    JSA_CHECK_HIP(hipMemcpyAsync(&err, jsa::batch_bin_error_word(b->bin, w), 4,
+                                 hipMemcpyDeviceToHost, s),
+                  "batch_check");
+    JSA_CHECK_HIP(hipMemcpyAsync(&total, jsa::batch_bin_total(b->bin, w), 8,
+                                 hipMemcpyDeviceToHost, s),
+                  "batch_check");
+    JSA_CHECK_HIP(hipStreamSynchronize(s), "batch_check");
+    if (num_intersects) *num_intersects = (int64_t)total;
+    if (err & 8u)
+        return fail(JAXSPLAT_ERR_CAPACITY, "intersections %llu exceed the batch capacity %d",
+                    total, b->bin.isect_cap);
+    if (err) return fail(JAXSPLAT_ERR_CAPACITY, "binning error (code %u)", err);
+    return JAXSPLAT_OK;
 }
 
 }  // extern "C"

@@ -190,6 +190,42 @@ struct Camera {
     float glob_scale, clip_thresh;
 };
 
+// One view of a render.  A plain render is a single record with zero offsets;
+// a batch is a device table of records, ordered by scene, whose items, tiles,
+// tile bins and pixels are laid end to end.
+struct alignas(16) ViewRec {
+    int img_w, img_h, tiles_x, tiles_y;
+    float fx, fy, cx, cy;
+    float tan_fovx, tan_fovy;
+    int g0, n;            // the scene's Gaussians: rows [g0, g0 + n) of the parameter arrays
+    int item_off;         // first projected item; item = item_off + (gaussian - g0)
+    int tile_off;         // first tile in the batch's tile numbering
+    int bin_off;          // first tile bin (view << tile_bits)
+    int view;             // row of viewmats and backgrounds
+    long long pix_off;    // first pixel in the image-sized buffers
+    long long reserved;
+};
+static_assert(sizeof(ViewRec) == 80, "ViewRec layout");
+
+// Scenes of a batch: Gaussians [g0, g0 + n) seen by views [v_begin, v_end).
+struct SceneRec {
+    int g0, n, v_begin, v_end;
+};
+
+// Launch description of the rasterizer: one view (views == nullptr, v0 used)
+// or a device table of num_views records covering total_tiles tiles.
+struct RasterGrid {
+    ViewRec v0;
+    const ViewRec* views;
+    int num_views;
+    int total_tiles;
+    int block_width;
+    int n_items;          // sorted entries name items in [0, n_items)
+};
+
+ViewRec single_view(const Camera& c);
+RasterGrid single_grid(const Camera& c);
+
 // Zero `bytes` (multiple of 4) starting at p with a fill kernel (never a
 // hipMemsetAsync, so the call is safe inside captured graphs).
 hipError_t fill_zero(void* p, size_t bytes, hipStream_t s);
@@ -247,6 +283,45 @@ hipError_t bin_and_sort(const BinPlan& p, const Camera& c, const float* xys,
 // status word of the last bin_and_sort using `scratch` (non-zero = look-back timeout)
 const uint32_t* bin_error_word(const BinPlan& p, const void* scratch);
 
+// ---- batched projection (views of scenes, see ViewRec) ----
+// Forward: items of every view; writes xys, conics, radii and the depth sort
+// key (depth bits, 0xFFFFFFFF for items that emit nothing).
+hipError_t project_batch_fwd(const ViewRec* views, int num_views, int max_view_n,
+                             float glob_scale, float clip_thresh, int block_width,
+                             const float* means3d, const float* scales, const float* quats,
+                             const float* viewmats, float* xys, float* conics, int* radii,
+                             uint32_t* depth_keys, hipStream_t s);
+// Backward: one lane per Gaussian, summing the contributions of its scene's
+// views; writes v_mean3d, v_scale, v_quat for every Gaussian of every scene.
+hipError_t project_batch_bwd(const SceneRec* scenes, int num_scenes, int max_scene_n,
+                             const ViewRec* views, float glob_scale, float clip_thresh,
+                             const float* means3d, const float* scales, const float* quats,
+                             const float* viewmats, const int* radii, const float* conics,
+                             const float* v_xy, const float* v_conic, float* v_mean3d,
+                             float* v_scale, float* v_quat, int consistent, hipStream_t s);
+
+// ---- batched binning ----
+struct BatchBinPlan {
+    int n_items;
+    int num_views;
+    int isect_cap;
+    int tile_bits, view_bits;
+    int num_bins;       // num_views << tile_bits
+    SortPlan depth_sort, view_sort, tile_sort;
+    size_t scratch_bytes;
+};
+BatchBinPlan batch_bin_plan(int n_items, int num_views, int max_view_tiles, int isect_cap);
+// Sorted item ids of every (view, tile) and bins[view << tile_bits | tile].
+// The scratch keeps the intersection count clamped to the capacity (the
+// length the sorts and bins use), the exact total (64 bits) and an error word:
+// bit 1 look-back wait limit, bit 4 bad tile key, bit 8 capacity overflow.
+hipError_t batch_bin(const BatchBinPlan& p, const ViewRec* views, int block_width,
+                     const float* xys, const int* radii, const uint32_t* depth_keys,
+                     int* items_sorted, int* tile_bins, void* scratch, hipStream_t s);
+const uint32_t* batch_bin_error_word(const BatchBinPlan& p, const void* scratch);
+const int* batch_bin_count(const BatchBinPlan& p, const void* scratch);
+const unsigned long long* batch_bin_total(const BatchBinPlan& p, const void* scratch);
+
 // ---- rasterization ----
 hipError_t raster_fwd(const Camera& c, const int* gids_sorted, const int* tile_bins,
                       const float* xys, const float* conics, const float* colors,
@@ -259,5 +334,19 @@ hipError_t raster_bwd(const Camera& c, const int* gids_sorted, const int* tile_b
                       const float* v_out_alpha, float* v_colors, float* v_opacity,
                       float* v_xy, float* v_xy_abs, float* v_conic, int consistent,
                       hipStream_t s);
+// The same kernels over a RasterGrid (one view or a batch).  Item-indexed
+// arrays: xys, conics, v_xy, v_xy_abs, v_conic; Gaussian-indexed: colors,
+// opacities, v_colors, v_opacity; backgrounds are rows of 3 per view.
+hipError_t raster_fwd_grid(const RasterGrid& g, const int* items_sorted, const int* tile_bins,
+                           const float* xys, const float* conics, const float* colors,
+                           const float* opacities, const float* backgrounds, float* final_Ts,
+                           int* final_idx, float* out_img, hipStream_t s);
+hipError_t raster_bwd_grid(const RasterGrid& g, const int* items_sorted, const int* tile_bins,
+                           const float* xys, const float* conics, const float* colors,
+                           const float* opacities, const float* backgrounds,
+                           const float* final_Ts, const int* final_idx, const float* v_out,
+                           const float* v_out_alpha, float* v_colors, float* v_opacity,
+                           float* v_xy, float* v_xy_abs, float* v_conic, int consistent,
+                           hipStream_t s);
 
 }  // namespace jsa

@@ -14,15 +14,20 @@
 // 3x3 matrices always sum over the inner index in the order 0, 1, 2.
 //
 // Design:
-//  * one lane per Gaussian, 256-lane workgroups; outputs of every Gaussian are
-//    written explicitly (culled ones get zeros), so no output memsets are
-//    needed;
+//  * forward: four Gaussians per lane in coalesced 256-wide slabs, 256-lane
+//    workgroups; outputs of every Gaussian are written explicitly (culled ones
+//    get zeros), so no output memsets are needed;
 //  * the inclusive scan of num_tiles_hit (cum_tiles_hit) is fused into the
 //    forward kernel as a single-pass decoupled look-back scan: partitions are
 //    taken from an atomic counter, a wave publishes its aggregate and looks
 //    back over 64 predecessors at a time;
 //  * cov3d and compensation outputs are optional (nullptr skips the store);
-//    the backward recomputes cov3d from scales/quats.
+//    the backward (one lane per Gaussian) recomputes cov3d from scales/quats;
+//  * batched variants (views of scenes, a device table of ViewRec): the
+//    forward runs one grid row per view with the same per-Gaussian code and
+//    writes the depth sort key instead of a scan; the backward runs one lane
+//    per Gaussian over all views of its scene, summing the per-view
+//    contributions in registers.
 #include "common.hpp"
 
 namespace jsa {
@@ -128,14 +133,18 @@ __device__ __forceinline__ M3 ewa_jacobian(float fx, float fy, float tx, float t
 constexpr int kProjPer = 4;                   // Gaussians per lane
 constexpr int kProjPart = 256 * kProjPer;    // Gaussians per scan partition
 
-// Projection of Gaussian idx (forward.cu:13-131); writes every output row and
-// returns num_tiles_hit.
-__device__ __forceinline__ int project_one(
-    const Camera& c, int idx, const M3& W, const float* pt, const float* __restrict__ means3d,
-    const float* __restrict__ scales, const float* __restrict__ quats,
-    float* __restrict__ cov3d_out, float* __restrict__ xys, float* __restrict__ depths,
-    int* __restrict__ radii, float* __restrict__ conics, float* __restrict__ comp_out,
-    int* __restrict__ num_tiles_hit) {
+// Projected attributes of one Gaussian (culled ones keep the zeros).
+struct Proj {
+    float cov[6], conic[3];
+    float cx, cy, depth, comp;
+    int rad, count;
+};
+
+// Projection of Gaussian idx (forward.cu:13-131).
+__device__ __forceinline__ Proj project_core(const Camera& c, int idx, const M3& W,
+                                             const float* pt, const float* __restrict__ means3d,
+                                             const float* __restrict__ scales,
+                                             const float* __restrict__ quats) {
     const float mx = means3d[3 * idx + 0];
     const float my = means3d[3 * idx + 1];
     const float mz = means3d[3 * idx + 2];
@@ -198,21 +207,102 @@ __device__ __forceinline__ int project_one(
             }
         }
     }
+    Proj r;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) r.cov[k] = cov[k];
+    r.conic[0] = conic[0];
+    r.conic[1] = conic[1];
+    r.conic[2] = conic[2];
+    r.cx = cx;
+    r.cy = cy;
+    r.depth = depth;
+    r.comp = comp;
+    r.rad = rad_i;
+    r.count = count;
+    return r;
+}
+
+// Plain projection of Gaussian idx: writes every output row and returns
+// num_tiles_hit.
+__device__ __forceinline__ int project_one(
+    const Camera& c, int idx, const M3& W, const float* pt, const float* __restrict__ means3d,
+    const float* __restrict__ scales, const float* __restrict__ quats,
+    float* __restrict__ cov3d_out, float* __restrict__ xys, float* __restrict__ depths,
+    int* __restrict__ radii, float* __restrict__ conics, float* __restrict__ comp_out,
+    int* __restrict__ num_tiles_hit) {
+    const Proj r = project_core(c, idx, W, pt, means3d, scales, quats);
+    const float* cov = r.cov;
+    const float* conic = r.conic;
     if (cov3d_out) {
         float2* o = reinterpret_cast<float2*>(cov3d_out + 6 * (size_t)idx);
         o[0] = make_float2(cov[0], cov[1]);
         o[1] = make_float2(cov[2], cov[3]);
         o[2] = make_float2(cov[4], cov[5]);
     }
-    reinterpret_cast<float2*>(xys)[idx] = make_float2(cx, cy);
-    depths[idx] = depth;
-    radii[idx] = rad_i;
+    reinterpret_cast<float2*>(xys)[idx] = make_float2(r.cx, r.cy);
+    depths[idx] = r.depth;
+    radii[idx] = r.rad;
     conics[3 * idx + 0] = conic[0];
     conics[3 * idx + 1] = conic[1];
     conics[3 * idx + 2] = conic[2];
-    if (comp_out) comp_out[idx] = comp;
-    num_tiles_hit[idx] = count;
-    return count;
+    if (comp_out) comp_out[idx] = r.comp;
+    num_tiles_hit[idx] = r.count;
+    return r.count;
+}
+
+// Camera of one view of a batch (the block width and scalars are shared).
+__device__ __forceinline__ Camera view_camera(const ViewRec& v, float glob_scale,
+                                              float clip_thresh, int block_width) {
+    Camera c;
+    c.num_points = v.n;
+    c.img_w = v.img_w;
+    c.img_h = v.img_h;
+    c.block_width = block_width;
+    c.tiles_x = v.tiles_x;
+    c.tiles_y = v.tiles_y;
+    c.fx = v.fx;
+    c.fy = v.fy;
+    c.cx = v.cx;
+    c.cy = v.cy;
+    c.tan_fovx = v.tan_fovx;
+    c.tan_fovy = v.tan_fovy;
+    c.glob_scale = glob_scale;
+    c.clip_thresh = clip_thresh;
+    return c;
+}
+
+// Batched forward: blockIdx.y = view, kProjPer Gaussians per lane of the
+// view's scene.  Writes the item rows the binning and rasterizer read, and the
+// depth sort key; no scan (the emission scans in depth order itself).
+__global__ __launch_bounds__(256) void project_batch_fwd_kernel(
+    const ViewRec* __restrict__ views, float glob_scale, float clip_thresh, int block_width,
+    const float* __restrict__ means3d, const float* __restrict__ scales,
+    const float* __restrict__ quats, const float* __restrict__ viewmats,
+    float* __restrict__ xys, float* __restrict__ conics, int* __restrict__ radii,
+    uint32_t* __restrict__ dkeys) {
+    const ViewRec vr = views[blockIdx.y];
+    const int first = blockIdx.x * kProjPart;
+    if (first >= vr.n) return;
+    const Camera c = view_camera(vr, glob_scale, clip_thresh, block_width);
+    M3 W;
+    float pt[3];
+    load_view(viewmats + 16 * vr.view, W, pt);
+    const float* m3 = means3d + 3 * (size_t)vr.g0;
+    const float* sc = scales + 3 * (size_t)vr.g0;
+    const float* qt = quats + 4 * (size_t)vr.g0;
+#pragma unroll
+    for (int k = 0; k < kProjPer; ++k) {
+        const int local = first + k * 256 + (int)threadIdx.x;
+        if (local >= vr.n) break;
+        const Proj r = project_core(c, local, W, pt, m3, sc, qt);
+        const size_t it = (size_t)vr.item_off + local;
+        reinterpret_cast<float2*>(xys)[it] = make_float2(r.cx, r.cy);
+        conics[3 * it + 0] = r.conic[0];
+        conics[3 * it + 1] = r.conic[1];
+        conics[3 * it + 2] = r.conic[2];
+        radii[it] = r.rad;
+        dkeys[it] = r.rad > 0 ? __float_as_uint(r.depth) : 0xFFFFFFFFu;
+    }
 }
 
 __global__ __launch_bounds__(256) void project_fwd_kernel(
@@ -297,6 +387,116 @@ __device__ __forceinline__ void quat_vjp(const float q[4], const M3& vR, float o
 #undef VR
 }
 
+// VJP of one view's projection of one Gaussian with camera-space mean from
+// (W, pt, m) and 3D covariance cov: the contribution to v_mean3d (from v_xy,
+// v_depth and the EWA Jacobian) and v_cov3d, plus v_cov2d.
+// backward.cu:345-524 with helpers.h:60-98,131-138.
+__device__ __forceinline__ void view_vjp(const Camera& c, const M3& W, const float* pt,
+                                         const float m[3], const float cov[6], float a, float b,
+                                         float cc, float2 gxy, float vz, float ga, float gb_raw,
+                                         float gc, bool has_comp, float cp, float vcomp,
+                                         int consistent, float vmean[3], float vc2[3],
+                                         float vc3[6]) {
+    float t[3];
+    transform(W, pt, m[0], m[1], m[2], t);
+    // v_mean from v_xy: project_pix_vjp (helpers.h:131-138) then R^T (:90-98)
+    const float rw = 1.f / (t[2] + 1e-6f);
+    const float vpx = c.fx * gxy.x, vpy = c.fy * gxy.y;
+    const float vv[3] = {vpx * rw, vpy * rw, -(vpx * t[0] + vpy * t[1]) * rw * rw};
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+        vmean[k] = W.m[0][k] * vv[0] + W.m[1][k] * vv[1] + W.m[2][k] * vv[2] + W.m[2][k] * vz;
+    // v_cov2d (helpers.h:60-87)
+    const float gb = gb_raw * 0.5f;
+    const float xg00 = a * ga + b * gb, xg01 = a * gb + b * gc;
+    const float xg10 = b * ga + cc * gb, xg11 = b * gb + cc * gc;
+    vc2[0] = -(xg00 * a + xg01 * b);
+    vc2[1] = -(xg00 * b + xg01 * cc) - (xg10 * a + xg11 * b);
+    vc2[2] = -(xg10 * b + xg11 * cc);
+    if (has_comp) {
+        const float inv_det = a * cc - b * b;
+        const float omc = 1.f - cp * cp;
+        const float vsq = vcomp * 0.5f / (cp + 1e-6f);
+        vc2[0] += vsq * (omc * a - 0.3f * inv_det);
+        vc2[1] += 2.f * vsq * (omc * b);
+        vc2[2] += vsq * (omc * cc - 0.3f * inv_det);
+    }
+    // EWA VJP, backward.cu:421-524
+    float tx = t[0], ty = t[1];
+    bool inx = true, iny = true;
+    float cxv = 0.f, cyv = 0.f;
+    if (consistent) {
+        const float lim_x = 1.3f * c.tan_fovx, lim_y = 1.3f * c.tan_fovy;
+        const float ux = t[0] / t[2], uy = t[1] / t[2];
+        inx = ux > -lim_x && ux < lim_x;
+        iny = uy > -lim_y && uy < lim_y;
+        cxv = fminf(lim_x, fmaxf(-lim_x, ux));
+        cyv = fminf(lim_y, fmaxf(-lim_y, uy));
+        tx = t[2] * cxv;
+        ty = t[2] * cyv;
+    }
+    const float rz = 1.f / t[2];
+    const float rz2 = rz * rz;
+    const M3 J = ewa_jacobian(c.fx, c.fy, tx, ty, rz, rz2);
+    const M3 V = sym3(cov);
+    M3 G;
+    G.m[0][0] = vc2[0]; G.m[0][1] = 0.5f * vc2[1]; G.m[0][2] = 0.f;
+    G.m[1][0] = 0.5f * vc2[1]; G.m[1][1] = vc2[2]; G.m[1][2] = 0.f;
+    G.m[2][0] = 0.f; G.m[2][1] = 0.f; G.m[2][2] = 0.f;
+    const M3 T = mm(J, W);
+    const M3 vV = mm(mm(tr(T), G), T);
+    const M3 vT1 = mm(mm(G, T), tr(V));
+    const M3 vT2 = mm(mm(tr(G), T), V);
+    M3 vT;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) vT.m[i][j] = vT1.m[i][j] + vT2.m[i][j];
+    vc3[0] = vV.m[0][0];
+    vc3[1] = vV.m[0][1] + vV.m[1][0];
+    vc3[2] = vV.m[0][2] + vV.m[2][0];
+    vc3[3] = vV.m[1][1];
+    vc3[4] = vV.m[1][2] + vV.m[2][1];
+    vc3[5] = vV.m[2][2];
+    const M3 vJ = mm(vT, tr(W));
+    const float rz3 = rz2 * rz;
+    float vtx = -c.fx * rz2 * vJ.m[0][2];
+    float vty = -c.fy * rz2 * vJ.m[1][2];
+    float vtz = -c.fx * rz2 * vJ.m[0][0] + 2.f * c.fx * tx * rz3 * vJ.m[0][2] -
+                c.fy * rz2 * vJ.m[1][1] + 2.f * c.fy * ty * rz3 * vJ.m[1][2];
+    if (consistent) {
+        vtz = vtz + (inx ? 0.f : cxv) * vtx + (iny ? 0.f : cyv) * vty;
+        vtx = inx ? vtx : 0.f;
+        vty = iny ? vty : 0.f;
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+        vmean[k] = vmean[k] + (vtx * W.m[0][k] + vty * W.m[1][k] + vtz * W.m[2][k]);
+}
+
+// Scale/rotation VJP of cov3d = (R S)(R S)^T, backward.cu:528-570.
+__device__ __forceinline__ void cov3d_vjp(const float s[3], const float q[4], float glob_scale,
+                                          const float vc3[6], float vsc[3], float vq[4]) {
+    const M3 R = quat_to_rotmat(q[0], q[1], q[2], q[3]);
+    const M3 S = scale_mat(s[0], s[1], s[2], glob_scale);
+    const M3 M = mm(R, S);
+    M3 vVs;
+    vVs.m[0][0] = vc3[0]; vVs.m[0][1] = 0.5f * vc3[1]; vVs.m[0][2] = 0.5f * vc3[2];
+    vVs.m[1][0] = 0.5f * vc3[1]; vVs.m[1][1] = vc3[3]; vVs.m[1][2] = 0.5f * vc3[4];
+    vVs.m[2][0] = 0.5f * vc3[2]; vVs.m[2][1] = 0.5f * vc3[4]; vVs.m[2][2] = vc3[5];
+    M3 vM = mm(vVs, M);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) vM.m[i][j] = 2.f * vM.m[i][j];
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+        vsc[k] = (R.m[0][k] * vM.m[0][k] + R.m[1][k] * vM.m[1][k] + R.m[2][k] * vM.m[2][k]) *
+                 glob_scale;
+    const M3 vR = mm(vM, S);
+    quat_vjp(q, vR, vq);
+}
+
 __global__ __launch_bounds__(256) void project_bwd_kernel(
     Camera c, const float* __restrict__ means3d, const float* __restrict__ scales,
     const float* __restrict__ quats, const float* __restrict__ viewmat,
@@ -315,110 +515,18 @@ __global__ __launch_bounds__(256) void project_bwd_kernel(
         float pt[3];
         load_view(viewmat, W, pt);
         const float m[3] = {means3d[3 * idx + 0], means3d[3 * idx + 1], means3d[3 * idx + 2]};
-        float t[3];
-        transform(W, pt, m[0], m[1], m[2], t);
-        // v_mean from v_xy: project_pix_vjp (helpers.h:131-138) then R^T (:90-98)
-        const float2 gxy = reinterpret_cast<const float2*>(v_xy)[idx];
-        const float rw = 1.f / (t[2] + 1e-6f);
-        const float vpx = c.fx * gxy.x, vpy = c.fy * gxy.y;
-        const float vv[3] = {vpx * rw, vpy * rw, -(vpx * t[0] + vpy * t[1]) * rw * rw};
-        const float vz = v_depth ? v_depth[idx] : 0.f;
-#pragma unroll
-        for (int k = 0; k < 3; ++k)
-            vmean[k] = W.m[0][k] * vv[0] + W.m[1][k] * vv[1] + W.m[2][k] * vv[2] + W.m[2][k] * vz;
-        // v_cov2d (helpers.h:60-87)
-        const float a = conics[3 * idx + 0], b = conics[3 * idx + 1], cc = conics[3 * idx + 2];
-        const float ga = v_conic[3 * idx + 0], gb = v_conic[3 * idx + 1] * 0.5f,
-                    gc = v_conic[3 * idx + 2];
-        const float xg00 = a * ga + b * gb, xg01 = a * gb + b * gc;
-        const float xg10 = b * ga + cc * gb, xg11 = b * gb + cc * gc;
-        vc2[0] = -(xg00 * a + xg01 * b);
-        vc2[1] = -(xg00 * b + xg01 * cc) - (xg10 * a + xg11 * b);
-        vc2[2] = -(xg10 * b + xg11 * cc);
-        if (v_compensation) {
-            const float cp = compensation[idx];
-            const float inv_det = a * cc - b * b;
-            const float omc = 1.f - cp * cp;
-            const float vsq = v_compensation[idx] * 0.5f / (cp + 1e-6f);
-            vc2[0] += vsq * (omc * a - 0.3f * inv_det);
-            vc2[1] += 2.f * vsq * (omc * b);
-            vc2[2] += vsq * (omc * cc - 0.3f * inv_det);
-        }
-        // EWA VJP, backward.cu:421-524
         const float s[3] = {scales[3 * idx + 0], scales[3 * idx + 1], scales[3 * idx + 2]};
         const float4 q4 = reinterpret_cast<const float4*>(quats)[idx];
         const float q[4] = {q4.x, q4.y, q4.z, q4.w};
         float cov[6];
         cov3d_of(s, q, c.glob_scale, cov);
-        float tx = t[0], ty = t[1];
-        bool inx = true, iny = true;
-        float cxv = 0.f, cyv = 0.f;
-        if (consistent) {
-            const float lim_x = 1.3f * c.tan_fovx, lim_y = 1.3f * c.tan_fovy;
-            const float ux = t[0] / t[2], uy = t[1] / t[2];
-            inx = ux > -lim_x && ux < lim_x;
-            iny = uy > -lim_y && uy < lim_y;
-            cxv = fminf(lim_x, fmaxf(-lim_x, ux));
-            cyv = fminf(lim_y, fmaxf(-lim_y, uy));
-            tx = t[2] * cxv;
-            ty = t[2] * cyv;
-        }
-        const float rz = 1.f / t[2];
-        const float rz2 = rz * rz;
-        const M3 J = ewa_jacobian(c.fx, c.fy, tx, ty, rz, rz2);
-        const M3 V = sym3(cov);
-        M3 G;
-        G.m[0][0] = vc2[0]; G.m[0][1] = 0.5f * vc2[1]; G.m[0][2] = 0.f;
-        G.m[1][0] = 0.5f * vc2[1]; G.m[1][1] = vc2[2]; G.m[1][2] = 0.f;
-        G.m[2][0] = 0.f; G.m[2][1] = 0.f; G.m[2][2] = 0.f;
-        const M3 T = mm(J, W);
-        const M3 vV = mm(mm(tr(T), G), T);
-        const M3 vT1 = mm(mm(G, T), tr(V));
-        const M3 vT2 = mm(mm(tr(G), T), V);
-        M3 vT;
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-#pragma unroll
-            for (int j = 0; j < 3; ++j) vT.m[i][j] = vT1.m[i][j] + vT2.m[i][j];
-        vc3[0] = vV.m[0][0];
-        vc3[1] = vV.m[0][1] + vV.m[1][0];
-        vc3[2] = vV.m[0][2] + vV.m[2][0];
-        vc3[3] = vV.m[1][1];
-        vc3[4] = vV.m[1][2] + vV.m[2][1];
-        vc3[5] = vV.m[2][2];
-        const M3 vJ = mm(vT, tr(W));
-        const float rz3 = rz2 * rz;
-        float vtx = -c.fx * rz2 * vJ.m[0][2];
-        float vty = -c.fy * rz2 * vJ.m[1][2];
-        float vtz = -c.fx * rz2 * vJ.m[0][0] + 2.f * c.fx * tx * rz3 * vJ.m[0][2] -
-                    c.fy * rz2 * vJ.m[1][1] + 2.f * c.fy * ty * rz3 * vJ.m[1][2];
-        if (consistent) {
-            vtz = vtz + (inx ? 0.f : cxv) * vtx + (iny ? 0.f : cyv) * vty;
-            vtx = inx ? vtx : 0.f;
-            vty = iny ? vty : 0.f;
-        }
-#pragma unroll
-        for (int k = 0; k < 3; ++k)
-            vmean[k] = vmean[k] + (vtx * W.m[0][k] + vty * W.m[1][k] + vtz * W.m[2][k]);
-        // scale/rotation VJP, backward.cu:528-570
-        const M3 R = quat_to_rotmat(q[0], q[1], q[2], q[3]);
-        const M3 S = scale_mat(s[0], s[1], s[2], c.glob_scale);
-        const M3 M = mm(R, S);
-        M3 vVs;
-        vVs.m[0][0] = vc3[0]; vVs.m[0][1] = 0.5f * vc3[1]; vVs.m[0][2] = 0.5f * vc3[2];
-        vVs.m[1][0] = 0.5f * vc3[1]; vVs.m[1][1] = vc3[3]; vVs.m[1][2] = 0.5f * vc3[4];
-        vVs.m[2][0] = 0.5f * vc3[2]; vVs.m[2][1] = 0.5f * vc3[4]; vVs.m[2][2] = vc3[5];
-        M3 vM = mm(vVs, M);
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-#pragma unroll
-            for (int j = 0; j < 3; ++j) vM.m[i][j] = 2.f * vM.m[i][j];
-#pragma unroll
-        for (int k = 0; k < 3; ++k)
-            vsc[k] = (R.m[0][k] * vM.m[0][k] + R.m[1][k] * vM.m[1][k] + R.m[2][k] * vM.m[2][k]) *
-                     c.glob_scale;
-        const M3 vR = mm(vM, S);
-        quat_vjp(q, vR, vq);
+        const bool has_comp = v_compensation != nullptr;
+        view_vjp(c, W, pt, m, cov, conics[3 * idx + 0], conics[3 * idx + 1], conics[3 * idx + 2],
+                 reinterpret_cast<const float2*>(v_xy)[idx], v_depth ? v_depth[idx] : 0.f,
+                 v_conic[3 * idx + 0], v_conic[3 * idx + 1], v_conic[3 * idx + 2], has_comp,
+                 has_comp ? compensation[idx] : 0.f, has_comp ? v_compensation[idx] : 0.f,
+                 consistent, vmean, vc2, vc3);
+        cov3d_vjp(s, q, c.glob_scale, vc3, vsc, vq);
     }
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
@@ -434,6 +542,61 @@ __global__ __launch_bounds__(256) void project_bwd_kernel(
 #pragma unroll
         for (int k = 0; k < 6; ++k) v_cov3d_out[6 * idx + k] = vc3[k];
     }
+}
+
+// Batched backward: blockIdx.y = scene, one lane per Gaussian; the views of
+// the scene are walked in order and their v_mean3d / v_cov3d contributions
+// summed in registers (the scale/rotation VJP is linear in v_cov3d, so it
+// runs once on the sum).  No atomics; every Gaussian of the scene is written.
+__global__ __launch_bounds__(256) void project_batch_bwd_kernel(
+    const SceneRec* __restrict__ scenes, const ViewRec* __restrict__ views, float glob_scale,
+    float clip_thresh, const float* __restrict__ means3d, const float* __restrict__ scales,
+    const float* __restrict__ quats, const float* __restrict__ viewmats,
+    const int* __restrict__ radii, const float* __restrict__ conics,
+    const float* __restrict__ v_xy, const float* __restrict__ v_conic,
+    float* __restrict__ v_mean3d, float* __restrict__ v_scale, float* __restrict__ v_quat,
+    int consistent) {
+    const SceneRec sc = scenes[blockIdx.y];
+    const int k = blockIdx.x * 256 + (int)threadIdx.x;
+    if (k >= sc.n) return;
+    const size_t g = (size_t)sc.g0 + k;
+    const float m[3] = {means3d[3 * g + 0], means3d[3 * g + 1], means3d[3 * g + 2]};
+    const float s[3] = {scales[3 * g + 0], scales[3 * g + 1], scales[3 * g + 2]};
+    const float4 q4 = reinterpret_cast<const float4*>(quats)[g];
+    const float q[4] = {q4.x, q4.y, q4.z, q4.w};
+    float vmean[3] = {0.f, 0.f, 0.f}, vc3[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    float cov[6];
+    bool any = false;
+    for (int v = sc.v_begin; v < sc.v_end; ++v) {
+        const ViewRec vr = views[v];
+        const size_t it = (size_t)vr.item_off + k;
+        if (radii[it] <= 0) continue;
+        if (!any) {
+            cov3d_of(s, q, glob_scale, cov);
+            any = true;
+        }
+        const Camera c = view_camera(vr, glob_scale, clip_thresh, 1);
+        M3 W;
+        float pt[3];
+        load_view(viewmats + 16 * vr.view, W, pt);
+        float vm[3], v2[3], v3[6];
+        view_vjp(c, W, pt, m, cov, conics[3 * it + 0], conics[3 * it + 1], conics[3 * it + 2],
+                 reinterpret_cast<const float2*>(v_xy)[it], 0.f, v_conic[3 * it + 0],
+                 v_conic[3 * it + 1], v_conic[3 * it + 2], false, 0.f, 0.f, consistent, vm, v2,
+                 v3);
+#pragma unroll
+        for (int j = 0; j < 3; ++j) vmean[j] += vm[j];
+#pragma unroll
+        for (int j = 0; j < 6; ++j) vc3[j] += v3[j];
+    }
+    float vsc[3] = {0.f, 0.f, 0.f}, vq[4] = {0.f, 0.f, 0.f, 0.f};
+    if (any) cov3d_vjp(s, q, glob_scale, vc3, vsc, vq);
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        v_mean3d[3 * g + j] = vmean[j];
+        v_scale[3 * g + j] = vsc[j];
+    }
+    reinterpret_cast<float4*>(v_quat)[g] = make_float4(vq[0], vq[1], vq[2], vq[3]);
 }
 
 __global__ void fill_zero_kernel(uint32_t* __restrict__ p, size_t n) {
@@ -505,6 +668,34 @@ hipError_t project_bwd(const Camera& c, const float* means3d, const float* scale
                                               compensation, v_xy, v_depth, v_conic,
                                               v_compensation, v_mean3d, v_scale, v_quat, v_cov2d,
                                               v_cov3d, consistent);
+    return hipGetLastError();
+}
+
+hipError_t project_batch_fwd(const ViewRec* views, int num_views, int max_view_n,
+                             float glob_scale, float clip_thresh, int block_width,
+                             const float* means3d, const float* scales, const float* quats,
+                             const float* viewmats, float* xys, float* conics, int* radii,
+                             uint32_t* depth_keys, hipStream_t s) {
+    if (num_views <= 0 || max_view_n <= 0) return hipSuccess;
+    const dim3 grid((max_view_n + kProjPart - 1) / kProjPart, num_views);
+    project_batch_fwd_kernel<<<grid, 256, 0, s>>>(views, glob_scale, clip_thresh, block_width,
+                                                  means3d, scales, quats, viewmats, xys, conics,
+                                                  radii, depth_keys);
+    return hipGetLastError();
+}
+
+hipError_t project_batch_bwd(const SceneRec* scenes, int num_scenes, int max_scene_n,
+                             const ViewRec* views, float glob_scale, float clip_thresh,
+                             const float* means3d, const float* scales, const float* quats,
+                             const float* viewmats, const int* radii, const float* conics,
+                             const float* v_xy, const float* v_conic, float* v_mean3d,
+                             float* v_scale, float* v_quat, int consistent, hipStream_t s) {
+    if (num_scenes <= 0 || max_scene_n <= 0) return hipSuccess;
+    const dim3 grid((max_scene_n + 255) / 256, num_scenes);
+    project_batch_bwd_kernel<<<grid, 256, 0, s>>>(scenes, views, glob_scale, clip_thresh,
+                                                  means3d, scales, quats, viewmats, radii,
+                                                  conics, v_xy, v_conic, v_mean3d, v_scale,
+                                                  v_quat, consistent);
     return hipGetLastError();
 }
 

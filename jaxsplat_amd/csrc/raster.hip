@@ -49,6 +49,19 @@ __device__ __forceinline__ uint64_t uniform64(uint64_t v) {
     return ((uint64_t)hi << 32) | lo;
 }
 
+// The view that owns tile gt of the grid's tile numbering (binary search over
+// the ascending tile_off of the view table; uniform per workgroup).
+__device__ __forceinline__ ViewRec resolve_view(const RasterGrid& g, int gt) {
+    if (g.views == nullptr) return g.v0;
+    int lo = 0, hi = g.num_views;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (g.views[mid].tile_off <= gt) lo = mid;
+        else hi = mid;
+    }
+    return g.views[lo];
+}
+
 // Pixel-centre box of the 64 consecutive tile pixels p0..p0+63 (row-major in
 // the bw x bw tile), clipped to the image; ok = false if none is inside.
 __device__ __forceinline__ float4 group_rect(int p0, int bw, int x_org, int y_org, int W, int H,
@@ -129,10 +142,10 @@ __device__ __forceinline__ bool group_hit(const CullInfo& k, float4 r) {
 // Forward: one pixel per lane, one 64-pixel group per wave.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void raster_fwd_kernel(
-    Camera c, const int* __restrict__ gids, const int2* __restrict__ bins,
+    RasterGrid rg, const int* __restrict__ gids, const int2* __restrict__ bins,
     const float2* __restrict__ xys, const float* __restrict__ conics,
     const float* __restrict__ colors, const float* __restrict__ opac,
-    const float* __restrict__ bg, float* __restrict__ final_Ts, int* __restrict__ final_idx,
+    const float* __restrict__ bgs, float* __restrict__ final_Ts, int* __restrict__ final_idx,
     float* __restrict__ out_img) {
     __shared__ float4 s_a[kMaxTileThreads];   // x, y, opacity, id
     __shared__ float4 s_b[kMaxTileThreads];   // conic a, b, c
@@ -142,30 +155,32 @@ __global__ __launch_bounds__(256) void raster_fwd_kernel(
     __shared__ int s_rok[4];
     __shared__ int s_cnt[4];
 
-    const int num_tiles = c.tiles_x * c.tiles_y;
-    const int tile = xcd_tile(blockIdx.x, num_tiles);
-    if (tile >= num_tiles) return;
-    const int bw = c.block_width;
+    const int gt = xcd_tile(blockIdx.x, rg.total_tiles);
+    if (gt >= rg.total_tiles) return;
+    const ViewRec vr = resolve_view(rg, gt);
+    const int tile = gt - vr.tile_off;
+    const int gshift = vr.g0 - vr.item_off;      // gaussian = item + gshift
+    const int bw = rg.block_width;
     const int nthr = blockDim.x;
     const int tid = threadIdx.x;
     const int lane = lane_id();
     const int wave = tid >> 6;
     const int nwaves = nthr >> 6;
-    const int ty = tile / c.tiles_x, tx = tile - ty * c.tiles_x;
+    const int ty = tile / vr.tiles_x, tx = tile - ty * vr.tiles_x;
     const int x_org = tx * bw, y_org = ty * bw;
     const bool pixel = tid < bw * bw;
     const int i = y_org + (pixel ? tid / bw : 0);
     const int j = x_org + (pixel ? tid % bw : 0);
-    const bool inside = pixel && i < c.img_h && j < c.img_w;
+    const bool inside = pixel && i < vr.img_h && j < vr.img_w;
     const float px = (float)j + 0.5f;
     const float py = (float)i + 0.5f;
     if (tid < nwaves) {
         bool ok;
-        s_rect[tid] = group_rect(tid * 64, bw, x_org, y_org, c.img_w, c.img_h, ok);
+        s_rect[tid] = group_rect(tid * 64, bw, x_org, y_org, vr.img_w, vr.img_h, ok);
         s_rok[tid] = ok;
     }
 
-    const int2 range = bins[tile];
+    const int2 range = bins[vr.bin_off + tile];
     float T = 1.f;
     float cr = 0.f, cg = 0.f, cb = 0.f;
     int cur = 0;
@@ -180,12 +195,14 @@ __global__ __launch_bounds__(256) void raster_fwd_kernel(
         if (ndone == nthr) break;
         const int idx = bstart + tid;
         uint32_t bits = 0;
-        const int g = idx < range.y ? gids[idx] : -1;
-        if ((unsigned)g < (unsigned)c.num_points) {
-            const float2 xy = xys[g];
+        const int it = idx < range.y ? gids[idx] : -1;
+        if ((unsigned)(it - vr.item_off) < (unsigned)vr.n) {    // an item of this view
+            const int g = it + gshift;
+            const float2 xy = xys[it];
             const float o = opac[g];
-            const float ca = conics[3 * g + 0], cb_ = conics[3 * g + 1], cc = conics[3 * g + 2];
-            s_a[tid] = make_float4(xy.x, xy.y, o, __int_as_float(g));
+            const float ca = conics[3 * it + 0], cb_ = conics[3 * it + 1],
+                        cc = conics[3 * it + 2];
+            s_a[tid] = make_float4(xy.x, xy.y, o, __int_as_float(it));
             s_b[tid] = make_float4(ca, cb_, cc, 0.f);
             s_c[tid] = make_float4(colors[3 * g + 0], colors[3 * g + 1], colors[3 * g + 2], 0.f);
             const CullInfo ci = cull_info(xy.x, xy.y, ca, cb_, cc, o);
@@ -230,7 +247,8 @@ __global__ __launch_bounds__(256) void raster_fwd_kernel(
         __syncthreads();
     }
     if (inside) {
-        const int pix = i * c.img_w + j;
+        const size_t pix = (size_t)vr.pix_off + (size_t)i * vr.img_w + j;
+        const float* bg = bgs + 3 * vr.view;
         final_Ts[pix] = T;
         final_idx[pix] = cur;
         out_img[3 * pix + 0] = cr + T * bg[0];
@@ -291,10 +309,10 @@ __device__ __forceinline__ bool bwd_pixel(PixState& s, const float4 a, const flo
 
 template <bool CONSISTENT, bool ABS>
 __global__ __launch_bounds__(128) void raster_bwd_kernel(
-    Camera c, const int* __restrict__ gids, const int2* __restrict__ bins,
+    RasterGrid rg, const int* __restrict__ gids, const int2* __restrict__ bins,
     const float2* __restrict__ xys, const float* __restrict__ conics,
     const float* __restrict__ colors, const float* __restrict__ opac,
-    const float* __restrict__ bg, const float* __restrict__ final_Ts,
+    const float* __restrict__ bgs, const float* __restrict__ final_Ts,
     const int* __restrict__ final_idx, const float* __restrict__ v_out_img,
     const float* __restrict__ v_out_alpha, float* __restrict__ v_rgb,
     float* __restrict__ v_opacity, float* __restrict__ v_xy, float* __restrict__ v_xy_abs,
@@ -311,10 +329,12 @@ __global__ __launch_bounds__(128) void raster_bwd_kernel(
     __shared__ int s_rok[4];
     __shared__ int s_wmax[2];
 
-    const int num_tiles = c.tiles_x * c.tiles_y;
-    const int tile = xcd_tile(blockIdx.x, num_tiles);
-    if (tile >= num_tiles) return;
-    const int bw = c.block_width;
+    const int gt = xcd_tile(blockIdx.x, rg.total_tiles);
+    if (gt >= rg.total_tiles) return;
+    const ViewRec vr = resolve_view(rg, gt);
+    const int tile = gt - vr.tile_off;
+    const int gshift = vr.g0 - vr.item_off;      // gaussian = item + gshift
+    const int bw = rg.block_width;
     const int nthr = blockDim.x;         // 64 or 128 (two pixels per lane)
     const int tid = threadIdx.x;
     const int lane = lane_id();
@@ -322,8 +342,9 @@ __global__ __launch_bounds__(128) void raster_bwd_kernel(
     const int wave = tid >> 6;
     const int nwaves = nthr >> 6;
     const int ngroups = 2 * nwaves;
-    const int ty = tile / c.tiles_x, tx = tile - ty * c.tiles_x;
+    const int ty = tile / vr.tiles_x, tx = tile - ty * vr.tiles_x;
     const int x_org = tx * bw, y_org = ty * bw;
+    const float* bg = bgs + 3 * vr.view;
     const float bgr = bg[0], bgg = bg[1], bgb = bg[2];
 
     PixState ps[2];
@@ -335,10 +356,10 @@ __global__ __launch_bounds__(128) void raster_bwd_kernel(
         const int i = y_org + (pixel ? p / bw : 0);
         const int j = x_org + (pixel ? p % bw : 0);
         PixState& s = ps[sl];
-        s.inside = pixel && i < c.img_h && j < c.img_w;
+        s.inside = pixel && i < vr.img_h && j < vr.img_w;
         s.px = (float)j + 0.5f;
         s.py = (float)i + 0.5f;
-        const int pix = s.inside ? i * c.img_w + j : 0;
+        const size_t pix = s.inside ? (size_t)vr.pix_off + (size_t)i * vr.img_w + j : 0;
         s.T_final = s.inside ? final_Ts[pix] : 0.f;
         s.T = s.T_final;
         s.fin = s.inside ? final_idx[pix] : -1;
@@ -354,7 +375,7 @@ __global__ __launch_bounds__(128) void raster_bwd_kernel(
     if (tid < ngroups) {
         bool ok;
         const int sl = tid / nwaves, w = tid - sl * nwaves;
-        s_rect[tid] = group_rect(sl * nthr + w * 64, bw, x_org, y_org, c.img_w, c.img_h, ok);
+        s_rect[tid] = group_rect(sl * nthr + w * 64, bw, x_org, y_org, vr.img_w, vr.img_h, ok);
         s_rok[tid] = ok;
     }
 #pragma unroll
@@ -367,18 +388,20 @@ __global__ __launch_bounds__(128) void raster_bwd_kernel(
     int block_fin = -1;
     for (int w = 0; w < nwaves; ++w) block_fin = max(block_fin, s_wmax[w]);
 
-    const int2 range = bins[tile];
+    const int2 range = bins[vr.bin_off + tile];
     const int last = min(range.y - 1, block_fin);
     for (int bend = last; bend >= range.x; bend -= nthr) {
         // load entry e = tid (sorted index bend - e, back to front)
         const int idx = bend - tid;
         uint32_t bits = 0;
-        const int g = idx >= range.x ? gids[idx] : -1;
-        if ((unsigned)g < (unsigned)c.num_points) {
-            const float2 xy = xys[g];
+        const int it = idx >= range.x ? gids[idx] : -1;
+        if ((unsigned)(it - vr.item_off) < (unsigned)vr.n) {    // an item of this view
+            const int g = it + gshift;
+            const float2 xy = xys[it];
             const float o = opac[g];
-            const float ca = conics[3 * g + 0], cb_ = conics[3 * g + 1], cc = conics[3 * g + 2];
-            s_a[tid] = make_float4(xy.x, xy.y, o, __int_as_float(g));
+            const float ca = conics[3 * it + 0], cb_ = conics[3 * it + 1],
+                        cc = conics[3 * it + 2];
+            s_a[tid] = make_float4(xy.x, xy.y, o, __int_as_float(it));
             s_b[tid] = make_float4(ca, cb_, cc, 0.f);
             s_c[tid] = make_float4(colors[3 * g + 0], colors[3 * g + 1], colors[3 * g + 2], 0.f);
             const CullInfo ci = cull_info(xy.x, xy.y, ca, cb_, cc, o);
@@ -438,18 +461,19 @@ __global__ __launch_bounds__(128) void raster_bwd_kernel(
                 *reinterpret_cast<float4*>(&s_g[tid][v][0]) = make_float4(0.f, 0.f, 0.f, 0.f);
             }
             s_flag[tid] = 0;
-            const int gid = __float_as_int(s_a[tid].w);
+            const int it = __float_as_int(s_a[tid].w);
+            const int gid = it + gshift;
             unsafeAtomicAdd(v_rgb + 3 * gid + 0, r[0]);
             unsafeAtomicAdd(v_rgb + 3 * gid + 1, r[1]);
             unsafeAtomicAdd(v_rgb + 3 * gid + 2, r[2]);
-            unsafeAtomicAdd(v_conic + 3 * gid + 0, r[3]);
-            unsafeAtomicAdd(v_conic + 3 * gid + 1, r[4]);
-            unsafeAtomicAdd(v_conic + 3 * gid + 2, r[5]);
-            unsafeAtomicAdd(v_xy + 2 * gid + 0, r[6]);
-            unsafeAtomicAdd(v_xy + 2 * gid + 1, r[7]);
+            unsafeAtomicAdd(v_conic + 3 * it + 0, r[3]);
+            unsafeAtomicAdd(v_conic + 3 * it + 1, r[4]);
+            unsafeAtomicAdd(v_conic + 3 * it + 2, r[5]);
+            unsafeAtomicAdd(v_xy + 2 * it + 0, r[6]);
+            unsafeAtomicAdd(v_xy + 2 * it + 1, r[7]);
             if (ABS) {
-                unsafeAtomicAdd(v_xy_abs + 2 * gid + 0, r[8]);
-                unsafeAtomicAdd(v_xy_abs + 2 * gid + 1, r[9]);
+                unsafeAtomicAdd(v_xy_abs + 2 * it + 0, r[8]);
+                unsafeAtomicAdd(v_xy_abs + 2 * it + 1, r[9]);
             }
             unsafeAtomicAdd(v_opacity + gid, r[NV - 1]);
         }
@@ -486,12 +510,36 @@ __global__ void raster_empty_kernel(size_t n_pix, const float* __restrict__ bg,
     }
 }
 
-int tile_grid(const Camera& c) {
-    const int nt = c.tiles_x * c.tiles_y;
-    return 8 * ((nt + 7) / 8);
-}
+int tile_grid(int total_tiles) { return 8 * ((total_tiles + 7) / 8); }
 
 }  // namespace
+
+ViewRec single_view(const Camera& c) {
+    ViewRec v{};
+    v.img_w = c.img_w;
+    v.img_h = c.img_h;
+    v.tiles_x = c.tiles_x;
+    v.tiles_y = c.tiles_y;
+    v.fx = c.fx;
+    v.fy = c.fy;
+    v.cx = c.cx;
+    v.cy = c.cy;
+    v.tan_fovx = c.tan_fovx;
+    v.tan_fovy = c.tan_fovy;
+    v.n = c.num_points;
+    return v;
+}
+
+RasterGrid single_grid(const Camera& c) {
+    RasterGrid g{};
+    g.v0 = single_view(c);
+    g.views = nullptr;
+    g.num_views = 1;
+    g.total_tiles = c.tiles_x * c.tiles_y;
+    g.block_width = c.block_width;
+    g.n_items = c.num_points;
+    return g;
+}
 
 hipError_t raster_empty(const Camera& c, const float* background, float* final_Ts,
                         int* final_idx, float* out_img, hipStream_t s) {
@@ -516,15 +564,52 @@ hipError_t zero_grads(float* const* ptrs, const size_t* counts, int k, hipStream
     return hipGetLastError();
 }
 
+hipError_t raster_fwd_grid(const RasterGrid& g, const int* items_sorted, const int* tile_bins,
+                           const float* xys, const float* conics, const float* colors,
+                           const float* opacities, const float* backgrounds, float* final_Ts,
+                           int* final_idx, float* out_img, hipStream_t s) {
+    if (g.total_tiles <= 0) return hipSuccess;
+    const int nthr = ((g.block_width * g.block_width + 63) / 64) * 64;
+    raster_fwd_kernel<<<tile_grid(g.total_tiles), nthr, 0, s>>>(
+        g, items_sorted, reinterpret_cast<const int2*>(tile_bins),
+        reinterpret_cast<const float2*>(xys), conics, colors, opacities, backgrounds, final_Ts,
+        final_idx, out_img);
+    return hipGetLastError();
+}
+
 hipError_t raster_fwd(const Camera& c, const int* gids_sorted, const int* tile_bins,
                       const float* xys, const float* conics, const float* colors,
                       const float* opacities, const float* background, float* final_Ts,
                       int* final_idx, float* out_img, hipStream_t s) {
-    const int nthr = ((c.block_width * c.block_width + 63) / 64) * 64;
-    raster_fwd_kernel<<<tile_grid(c), nthr, 0, s>>>(
-        c, gids_sorted, reinterpret_cast<const int2*>(tile_bins),
-        reinterpret_cast<const float2*>(xys), conics, colors, opacities, background, final_Ts,
-        final_idx, out_img);
+    return raster_fwd_grid(single_grid(c), gids_sorted, tile_bins, xys, conics, colors,
+                           opacities, background, final_Ts, final_idx, out_img, s);
+}
+
+hipError_t raster_bwd_grid(const RasterGrid& g, const int* items_sorted, const int* tile_bins,
+                           const float* xys, const float* conics, const float* colors,
+                           const float* opacities, const float* background,
+                           const float* final_Ts, const int* final_idx, const float* v_out,
+                           const float* v_out_alpha, float* v_colors, float* v_opacity,
+                           float* v_xy, float* v_xy_abs, float* v_conic, int consistent,
+                           hipStream_t s) {
+    if (g.total_tiles <= 0) return hipSuccess;
+    // two pixels per lane
+    const int half = (g.block_width * g.block_width + 1) / 2;
+    const int nthr = ((half + 63) / 64) * 64;
+    const int grid = tile_grid(g.total_tiles);
+    const int* gids_sorted = items_sorted;
+    const int2* b2 = reinterpret_cast<const int2*>(tile_bins);
+    const float2* x2 = reinterpret_cast<const float2*>(xys);
+#define JSA_LAUNCH(CONS, ABS)                                                                 \
+    raster_bwd_kernel<CONS, ABS><<<grid, nthr, 0, s>>>(                                      \
+        g, gids_sorted, b2, x2, conics, colors, opacities, background, final_Ts, final_idx,  \
+        v_out, v_out_alpha, v_colors, v_opacity, v_xy, v_xy_abs, v_conic)
+    if (consistent) {
+        if (v_xy_abs) JSA_LAUNCH(true, true); else JSA_LAUNCH(true, false);
+    } else {
+        if (v_xy_abs) JSA_LAUNCH(false, true); else JSA_LAUNCH(false, false);
+    }
+#undef JSA_LAUNCH
     return hipGetLastError();
 }
 
@@ -534,23 +619,9 @@ hipError_t raster_bwd(const Camera& c, const int* gids_sorted, const int* tile_b
                       const int* final_idx, const float* v_out, const float* v_out_alpha,
                       float* v_colors, float* v_opacity, float* v_xy, float* v_xy_abs,
                       float* v_conic, int consistent, hipStream_t s) {
-    // two pixels per lane
-    const int half = (c.block_width * c.block_width + 1) / 2;
-    const int nthr = ((half + 63) / 64) * 64;
-    const int grid = tile_grid(c);
-    const int2* b2 = reinterpret_cast<const int2*>(tile_bins);
-    const float2* x2 = reinterpret_cast<const float2*>(xys);
-#define JSA_LAUNCH(CONS, ABS)                                                                 \
-    raster_bwd_kernel<CONS, ABS><<<grid, nthr, 0, s>>>(                                      \
-        c, gids_sorted, b2, x2, conics, colors, opacities, background, final_Ts, final_idx,  \
-        v_out, v_out_alpha, v_colors, v_opacity, v_xy, v_xy_abs, v_conic)
-    if (consistent) {
-        if (v_xy_abs) JSA_LAUNCH(true, true); else JSA_LAUNCH(true, false);
-    } else {
-        if (v_xy_abs) JSA_LAUNCH(false, true); else JSA_LAUNCH(false, false);
-    }
-#undef JSA_LAUNCH
-    return hipGetLastError();
+    return raster_bwd_grid(single_grid(c), gids_sorted, tile_bins, xys, conics, colors,
+                           opacities, background, final_Ts, final_idx, v_out, v_out_alpha,
+                           v_colors, v_opacity, v_xy, v_xy_abs, v_conic, consistent, s);
 }
 
 }  // namespace jsa

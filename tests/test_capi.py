@@ -74,3 +74,35 @@ def test_workspace_sizes_scale():
 
 def test_version_string():
     assert b"gfx950" in _lib.lib().jaxsplat_version()
+
+
+def _batch_args(scenes=((0, 10), (10, 5)), views=((0, 32, 32), (1, 16, 24)), bw=16, cap=1000):
+    sc = (_lib.Scene * len(scenes))()
+    for i, (a, n) in enumerate(scenes):
+        sc[i].first_gaussian, sc[i].num_gaussians = a, n
+    vw = (_lib.View * len(views))()
+    for i, (s, h, w) in enumerate(views):
+        vw[i].scene, vw[i].img_h, vw[i].img_w = s, h, w
+        vw[i].fx = vw[i].fy = 50.0
+        vw[i].cx, vw[i].cy = w / 2, h / 2
+    return sc, len(scenes), vw, len(views), 1.0, 0.01, bw, cap
+
+
+@pytest.mark.parametrize("kw,msg", [
+    (dict(scenes=((0, 10), (12, 5))), b"consecutive"),
+    (dict(views=((1, 32, 32), (0, 16, 24))), b"grouped"),
+    (dict(views=((0, 0, 32),)), b"image size"),
+    (dict(views=((2, 8, 8),)), b"grouped"),
+    (dict(bw=17), b"block_width"),
+    (dict(cap=0), b"isect_capacity"),
+    (dict(views=()), b"num_views"),
+])
+def test_batch_create_rejects(kw, msg):
+    """Batch layout validation runs before any device call, so it is
+    checked here without a GPU."""
+    L = _lib.lib()
+    h = ctypes.c_void_p()
+    code = L.jaxsplat_batch_create(ctypes.byref(h), *_batch_args(**kw))
+    assert code == _lib.ERR_INVALID
+    assert msg in L.jaxsplat_last_error()
+    assert not h.value
