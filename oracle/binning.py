@@ -123,3 +123,27 @@ def bin_and_sort_bruteforce(xys, depths, radii, *, img_shape, block_width):
             bins[t, 0] = p
         bins[t, 1] = p + 1
     return keys, gids, bins
+
+
+def tile_lists(xys, depths, radii, tiles, *, img_shape, block_width):
+    """Sorted Gaussian ids of each tile in ``tiles``: the run of that tile in
+    the reference's sorted intersections (emission of ``forward.cu:135-179``
+    restricted to one tile, in the order (depth bits, id) of the stable sort
+    of ``ops.cu:722-760``), without emitting or sorting the other tiles."""
+    H, W = img_shape
+    tiles_x = (W + block_width - 1) // block_width
+    tiles_y = (H + block_width - 1) // block_width
+    xys = np.asarray(xys, np.float32).reshape(-1, 2)
+    depths = np.asarray(depths, np.float32).reshape(-1)
+    radii = np.asarray(radii, np.int32).reshape(-1)
+    vis = np.nonzero(radii > 0)[0]
+    xmin, xmax, ymin, ymax = tile_rect(xys[vis, 0], xys[vis, 1],
+                                       radii[vis].astype(np.float32),
+                                       tiles_x, tiles_y, block_width)
+    out = []
+    for t in tiles:
+        ty, tx = divmod(int(t), tiles_x)
+        g = vis[(xmin <= tx) & (tx < xmax) & (ymin <= ty) & (ty < ymax)]
+        dbits = depths[g].view(np.int32)
+        out.append(g[np.lexsort((g, dbits))].astype(np.int32))
+    return out

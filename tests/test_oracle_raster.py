@@ -54,6 +54,26 @@ def test_binning_invariants():
     assert not bins[empty].any()
 
 
+@pytest.mark.parametrize("bw", [8, 16])
+def test_tile_lists_match_sorted_runs(bw):
+    """Per-tile lists (used where the full sort is too large) equal the runs of
+    the full emission + stable sort, for every tile, empty ones included."""
+    sc = scenes.make("small")
+    cam = sc.camera
+    pr = oracle.project.project_fwd(sc.means3d, sc.scales * 3, sc.quats, cam.viewmat,
+                                    dtype=np.float32, img_shape=cam.img_shape, f=cam.f, c=cam.c,
+                                    glob_scale=1.0, clip_thresh=0.01, block_width=bw)
+    _, gs, bins = oracle.binning.bin_and_sort(pr["xys"], pr["depths"], pr["radii"],
+                                              pr["cum_tiles_hit"], img_shape=cam.img_shape,
+                                              block_width=bw)
+    tiles = np.arange(bins.shape[0])
+    lists = oracle.binning.tile_lists(pr["xys"], pr["depths"], pr["radii"], tiles,
+                                      img_shape=cam.img_shape, block_width=bw)
+    assert sum(x.size for x in lists) == gs.size > 0
+    for t, lst in zip(tiles, lists):
+        np.testing.assert_array_equal(lst, gs[bins[t, 0]:bins[t, 1]])
+
+
 @pytest.mark.parametrize("opac", [None, lambda o: np.full_like(o, 0.995)])
 def test_raster_fwd_matches_bruteforce(opac):
     sc, kw, pr, ks, gs, bins, op = _tiny(opac=opac, scale=12.0)
