@@ -15,9 +15,12 @@ Prints ONE JSON line.  Besides the contract fields it carries:
   same workload on this host (1 core);
 * ``stages``: per-stage milliseconds of the same pipeline;
 * ``batched``: the multi-view workload ``orbit64`` (64 cameras around one
-  500,000-Gaussian scene, 800x800 each): one HIP-graph replay of the batched
-  forward + L2 gradient + backward of all 64 views (``jaxsplat_amd.batch``)
-  against 64 plain ``render`` fwd+bwd calls, both in Mpix/s over all views.
+  500,000-Gaussian scene, 800x800 each) and the uneven multi-scene workload
+  ``multi32`` (32 scenes of 10k-2M Gaussians, one camera each, mixed image
+  sizes): one HIP-graph replay of the autotuned batched forward + L2 gradient
+  + backward of all views (``jaxsplat_amd.batch``, ``jaxsplat_amd.tune``)
+  against one plain ``render`` fwd+bwd call per view, both in Mpix/s over
+  all views.
 """
 
 from __future__ import annotations
@@ -47,6 +50,7 @@ def parse():
     ap.add_argument("--batched", dest="batched", action="store_true", default=True)
     ap.add_argument("--no-batched", dest="batched", action="store_false")
     ap.add_argument("--batched-steps", type=int, default=5)
+    ap.add_argument("--batched-workloads", default="orbit64,multi32")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="run a device-to-device copy of this many MiB first (HBM counter "
                          "calibration under rocprofv3 --pmc)")
@@ -116,7 +120,8 @@ def main():
 
     stages, roof, extra = stage_breakdown(torch, sc, cam, params, target, args.stage_iters)
 
-    batched = batched_vs_plain(torch, args.batched_steps) if args.batched else None
+    batched = [batched_vs_plain(torch, w, args.batched_steps)
+               for w in args.batched_workloads.split(",") if w] if args.batched else None
 
     cpu = cpu_baseline(sc, cam, tgt_seed, args.cpu_tiles) if args.cpu_baseline else None
 
@@ -237,53 +242,68 @@ def stage_breakdown(torch, sc, cam, params, target, iters):
     return {k: round(v, 4) for k, v in acc.items()}, roof, extra
 
 
-def batched_vs_plain(torch, steps, warmup=2):
-    """orbit64: all 64 views per step, batched in one HIP graph vs 64 plain
-    render calls.  Both steps compute the L2 loss gradient against seeded
-    targets and the full backward to the 500k Gaussians."""
+def _batched_workload(torch, name, dev):
+    """Scenes, views, parameters and targets of a batched workload."""
+    from jaxsplat_amd import ViewSpec, scenes
+    if name == "orbit64":
+        sc = scenes.make("orbit64")
+        sizes = [sc.n]
+        cams = [(0, c) for c in sc.cameras]
+        params = sc.params()
+        meta = dict(num_gaussians=sc.n, img_shape=list(sc.cameras[0].img_shape))
+        glob, clip = sc.glob_scale, sc.clip_thresh
+    elif name == "multi32":
+        ms = scenes.make_multi()
+        sizes = [m.n for m in ms]
+        cams = [(i, m.camera) for i, m in enumerate(ms)]
+        params = [np.concatenate(x) for x in zip(*[m.params() for m in ms])]
+        meta = dict(num_gaussians=int(sum(sizes)), scene_sizes=sizes,
+                    img_shapes=[list(m.camera.img_shape) for m in ms])
+        glob, clip = ms[0].glob_scale, ms[0].clip_thresh
+    else:
+        raise ValueError(name)
+    views = [ViewSpec(s_, c.img_shape, c.f, c.c) for s_, c in cams]
+    params = [torch.from_numpy(np.ascontiguousarray(a)).to(dev) for a in params]
+    vms = torch.from_numpy(np.stack([c.viewmat for _, c in cams])).to(dev)
+    bgs = torch.zeros((len(cams), 3), dtype=torch.float32, device=dev)
+    seed0 = {"orbit64": 2003, "multi32": 2004}[name]
+    targets = [torch.from_numpy(scenes.target_image(c.img_shape, seed0 + v)).to(dev)
+               for v, (_, c) in enumerate(cams)]
+    return sizes, views, cams, params, vms, bgs, targets, meta, glob, clip
+
+
+def batched_vs_plain(torch, name, steps, warmup=2):
+    """All views of a batched workload per step: the batched path (autotuned,
+    one HIP graph replay of forward + L2 gradient + backward) against one
+    plain render fwd+bwd call per view (the reference's usage)."""
     import jaxsplat_amd
-    from jaxsplat_amd import BatchPlan, ViewSpec, _lib, scenes
+    from jaxsplat_amd import _lib
     from jaxsplat_amd._ops import _p, _stream
+    from jaxsplat_amd.tune import autotune, tuned_plan
     dev = torch.device("cuda:0")
-    sc = scenes.make("orbit64")
-    V = len(sc.cameras)
-    Hh, Ww = sc.cameras[0].img_shape
-    P = V * Hh * Ww
-    views = [ViewSpec(0, c.img_shape, c.f, c.c) for c in sc.cameras]
-    params = [torch.from_numpy(a).to(dev) for a in sc.params()]
-    vms = torch.from_numpy(np.stack([c.viewmat for c in sc.cameras])).to(dev)
-    bgs = torch.zeros((V, 3), dtype=torch.float32, device=dev)
-    target = torch.from_numpy(np.concatenate(
-        [scenes.target_image((Hh, Ww), 2003 + v).reshape(-1, 3) for v in range(V)])).to(dev)
+    sizes, views, cams, params, vms, bgs, targets, meta, glob, clip = \
+        _batched_workload(torch, name, dev)
+    V = len(views)
+    tuning = autotune(sizes, views, *params, vms, bgs, glob_scale=glob, clip_thresh=clip)
+    plan = tuned_plan(tuning, sizes, views, glob_scale=glob, clip_thresh=clip, device=dev)
+    P = plan.num_pixels
     L = _lib.lib()
-    # size the intersection capacity from one eager forward
-    probe = BatchPlan([sc.n], views, glob_scale=sc.glob_scale, clip_thresh=sc.clip_thresh,
-                      block_width=sc.block_width, isect_capacity=1 << 20, device=dev)
+    target = torch.cat([t.reshape(-1, 3) for t in targets])
+    # per-pixel weight 2 / (3 H W) of its view: d mean((img - target)^2)
+    wpix = torch.cat([torch.full((t.shape[0] * t.shape[1], 1), 2.0 / t.numel(),
+                                 dtype=torch.float32, device=dev) for t in targets])
     fT = torch.empty((P,), dtype=torch.float32, device=dev)
     fI = torch.empty((P,), dtype=torch.int32, device=dev)
     img = torch.empty((P, 3), dtype=torch.float32, device=dev)
-    _lib.check(L.jaxsplat_batch_forward(probe._h, *[_p(x) for x in params], _p(vms), _p(bgs),
-                                        _p(fT), _p(fI), _p(img), _p(probe.workspace),
-                                        probe.workspace_bytes, _stream(dev)))
-    try:
-        probe.check()
-    except _lib.JaxsplatError as e:
-        if e.code != _lib.ERR_CAPACITY:
-            raise
-    isect = probe.last_intersections
-    del probe
-    plan = BatchPlan([sc.n], views, glob_scale=sc.glob_scale, clip_thresh=sc.clip_thresh,
-                     block_width=sc.block_width, isect_capacity=int(isect * 1.1) + 1024,
-                     device=dev)
+    v_img = torch.empty((P, 3), dtype=torch.float32, device=dev)
     grads = [torch.empty_like(x) for x in params]
-    scale = 2.0 / (3.0 * Hh * Ww)           # d mean((img - target)^2) per view
 
     def step():
         s = _stream(dev)
         _lib.check(L.jaxsplat_batch_forward(plan._h, *[_p(x) for x in params], _p(vms),
                                             _p(bgs), _p(fT), _p(fI), _p(img),
                                             _p(plan.workspace), plan.workspace_bytes, s))
-        v_img = (img - target) * scale
+        torch.mul(torch.sub(img, target), wpix, out=v_img)
         _lib.check(L.jaxsplat_batch_backward(plan._h, *[_p(x) for x in params], _p(vms),
                                              _p(bgs), _p(fT), _p(fI), _p(v_img),
                                              *[_p(g) for g in grads], 0, _p(plan.workspace),
@@ -296,7 +316,7 @@ def batched_vs_plain(torch, steps, warmup=2):
             step()
     torch.cuda.current_stream().wait_stream(side)
     torch.cuda.synchronize()
-    plan.check()
+    isect = plan.check()
     graph = torch.cuda.CUDAGraph()
     with torch.cuda.graph(graph):
         step()
@@ -312,19 +332,18 @@ def batched_vs_plain(torch, steps, warmup=2):
     ms_b = e0.elapsed_time(e1) / steps
     plan.check()
 
-    # plain: one render + loss + backward per view (the reference's usage)
     pp = [x.clone().requires_grad_(True) for x in params]
-    kw = dict(background=bgs[0], glob_scale=sc.glob_scale, clip_thresh=sc.clip_thresh,
-              block_size=sc.block_width)
-    tv = target.view(V, Hh, Ww, 3)
+    offs = np.concatenate([[0], np.cumsum(sizes)])
 
     def plain_step():
         for x in pp:
             x.grad = None
-        for v, c in enumerate(sc.cameras):
-            out = jaxsplat_amd.render(*pp, viewmat=vms[v], img_shape=c.img_shape, f=c.f, c=c.c,
-                                      **kw)
-            ((out - tv[v]) ** 2).mean().backward()
+        for v, (sidx, c) in enumerate(cams):
+            a, b = int(offs[sidx]), int(offs[sidx + 1])
+            out = jaxsplat_amd.render(*[x[a:b] for x in pp], viewmat=vms[v], background=bgs[v],
+                                      img_shape=c.img_shape, f=c.f, c=c.c, glob_scale=glob,
+                                      clip_thresh=clip, block_size=tuning.block_width)
+            ((out - targets[v]) ** 2).mean().backward()
 
     for _ in range(warmup):
         plain_step()
@@ -335,16 +354,18 @@ def batched_vs_plain(torch, steps, warmup=2):
     e1.record()
     torch.cuda.synchronize()
     ms_p = e0.elapsed_time(e1) / steps
-    # the two steps produce the same gradients (float atomics aside)
     gb = torch.cat([g.reshape(-1) for g in grads])
     gp = torch.cat([x.grad.reshape(-1) for x in pp])
     rel = float((gb - gp).abs().max() / gp.abs().max().clamp_min(1e-30))
-    return {"workload": "orbit64", "views": V, "num_gaussians": sc.n, "img_shape": [Hh, Ww],
-            "num_intersects": isect, "ms_batched_graph": round(ms_b, 3),
-            "ms_plain_64_calls": round(ms_p, 3),
-            "mpix_s_batched": round(P / (ms_b * 1e-3) / 1e6, 1),
-            "mpix_s_plain": round(P / (ms_p * 1e-3) / 1e6, 1),
-            "speedup": round(ms_p / ms_b, 3), "grad_max_rel_diff": rel}
+    return dict(workload=name, views=V, **meta, num_intersects=isect,
+                tuning={"block_width": tuning.block_width, "fwd_depth": tuning.fwd_depth,
+                        "bwd_depth": tuning.bwd_depth,
+                        "candidates_ms": {f"bw{k[0]}_{k[1]}{k[2]}": round(v, 3)
+                                          for k, v in tuning.table.items()}},
+                ms_batched_graph=round(ms_b, 3), ms_plain_per_view_calls=round(ms_p, 3),
+                mpix_s_batched=round(P / (ms_b * 1e-3) / 1e6, 1),
+                mpix_s_plain=round(P / (ms_p * 1e-3) / 1e6, 1),
+                speedup=round(ms_p / ms_b, 3), grad_max_rel_diff=rel)
 
 
 def measured_traffic(workload):

@@ -225,6 +225,11 @@ int jaxsplat_batch_create(jaxsplat_batch** out, const jaxsplat_scene* scenes, in
                           float clip_thresh, uint32_t block_width, int64_t isect_capacity);
 void jaxsplat_batch_destroy(jaxsplat_batch* b);
 size_t jaxsplat_batch_workspace_size(const jaxsplat_batch* b);
+/* Gaussians staged in LDS per batch by the rasterize forward / backward:
+ * 64, 128 or 256, or 0 for the workgroup size (bw^2 rounded up to whole
+ * waves; half of it in the backward).  The results do not depend on it; the
+ * autotuner of the Python layer picks the fastest. */
+int jaxsplat_batch_set_depths(jaxsplat_batch* b, int32_t fwd_depth, int32_t bwd_depth);
 int64_t jaxsplat_batch_num_pixels(const jaxsplat_batch* b);
 int64_t jaxsplat_batch_num_gaussians(const jaxsplat_batch* b);
 int64_t jaxsplat_batch_pixel_offset(const jaxsplat_batch* b, int32_t view);
@@ -253,6 +258,43 @@ int jaxsplat_batch_backward(const jaxsplat_batch* b, const float* means3d, const
  * forward's outputs are then invalid) or a look-back wait timed out. */
 int jaxsplat_batch_check(const jaxsplat_batch* b, const void* workspace,
                          int64_t* num_intersects, void* stream);
+
+/* Device address of the status word of the last forward in `workspace`
+ * (non-zero after a capacity overflow or a binning error): kernels that must
+ * not consume an invalid render, like jaxsplat_fit_adam, read it. */
+const uint32_t* jaxsplat_batch_status_word(const jaxsplat_batch* b, const void* workspace);
+
+/* ------------------------------------------------------------------------
+ * Image-fitting loop (the training step of examples/single_image.py:26-40,
+ * 143-166): parameter activations, the L2 loss gradient and Adam, fused for
+ * the device.  With the batched render they make a step of kernel launches
+ * only, which a caller may capture in one HIP graph.
+ * ------------------------------------------------------------------------ */
+
+/* quats / |quats|, sigmoid(colors), sigmoid(opacities) for n Gaussians
+ * (single_image.py:143-166; means and scales are used raw). */
+int jaxsplat_fit_activate(int64_t n, const float* quats_raw, const float* colors_raw,
+                          const float* opacities_raw, float* quats, float* colors,
+                          float* opacities, void* stream);
+
+/* v_img = 2 (img - target) / n and loss_sum = sum (img - target)^2 over n
+ * floats (the gradient of mean((img - target)^2), single_image.py:26-29);
+ * loss_sum is cleared first. */
+int jaxsplat_fit_l2(int64_t n, const float* img, const float* target, float* v_img,
+                    float* loss_sum, void* stream);
+
+/* One Adam step (optax.adam: m = b1 m + (1-b1) g, v = b2 v + (1-b2) g^2,
+ * p -= lr m^ / (sqrt(v^) + eps)) on the raw parameters, with g the gradient
+ * of the activated parameters pulled back through the activations.  moment1 /
+ * moment2 hold 14 floats per Gaussian (means 3, scales 3, quats 4, colors 3,
+ * opacity 1); *step (device) counts applied steps for the bias correction.  A
+ * non-NULL skip word that reads non-zero makes the step a no-op. */
+int jaxsplat_fit_adam(int64_t n, float* means3d, float* scales, float* quats_raw,
+                      float* colors_raw, float* opacities_raw, const float* v_means3d,
+                      const float* v_scales, const float* v_quats, const float* v_colors,
+                      const float* v_opacities, float* moment1, float* moment2, int32_t* step,
+                      const uint32_t* skip, float lr, float beta1, float beta2, float eps,
+                      void* stream);
 
 #ifdef __cplusplus
 }

@@ -20,7 +20,7 @@ __all__ = ["render", "project", "rasterize", "bin_and_sort", "ShapeDtypeError", 
 
 
 def render(means3d, scales, quats, colors, opacities, *, viewmat, background, img_shape, f, c,
-           glob_scale, clip_thresh, block_size=16, adjoint="reference"):
+           glob_scale, clip_thresh, block_size=16, adjoint="reference", antialiased=False):
     """Render 3D Gaussians to an (H, W, 3) image, differentiably.
 
     Same contract as the reference ``render`` (``jaxsplat/__init__.py:6-70``):
@@ -29,9 +29,17 @@ def render(means3d, scales, quats, colors, opacities, *, viewmat, background, im
     background (3,); img_shape (H, W); f (fx, fy); c (cx, cy); glob_scale;
     clip_thresh; block_size in [1, 16] (tile side).  Differentiable in means3d,
     scales, quats, colors and opacities.
+
+    ``antialiased=True`` (not in the reference API) blends with
+    ``opacities * compensation``, the compensation that the reference's
+    projection computes and drops (``forward.cu:537-543``); gradients flow
+    through both factors.
     """
-    xys, depths, radii, conics, _num_tiles_hit, cum_tiles_hit = project(
-        means3d, scales, quats, viewmat, img_shape=img_shape, f=f, c=c, glob_scale=glob_scale,
-        clip_thresh=clip_thresh, block_width=block_size, adjoint=adjoint)
+    out = project(means3d, scales, quats, viewmat, img_shape=img_shape, f=f, c=c,
+                  glob_scale=glob_scale, clip_thresh=clip_thresh, block_width=block_size,
+                  adjoint=adjoint, return_compensation=antialiased)
+    xys, depths, radii, conics, _num_tiles_hit, cum_tiles_hit = out[:6]
+    if antialiased:
+        opacities = opacities * out[6]
     return rasterize(colors, opacities, background, xys, depths, radii, conics, cum_tiles_hit,
                      img_shape=img_shape, block_width=block_size, adjoint=adjoint)

@@ -28,8 +28,11 @@ EXPORTS = (
     "jaxsplat_bin_workspace_size", "jaxsplat_bin_and_sort", "jaxsplat_bin_check",
     "jaxsplat_rasterize_forward_binned", "jaxsplat_rasterize_backward_binned",
     "jaxsplat_batch_create", "jaxsplat_batch_destroy", "jaxsplat_batch_workspace_size",
+    "jaxsplat_batch_set_depths",
     "jaxsplat_batch_num_pixels", "jaxsplat_batch_num_gaussians", "jaxsplat_batch_pixel_offset",
     "jaxsplat_batch_forward", "jaxsplat_batch_backward", "jaxsplat_batch_check",
+    "jaxsplat_batch_status_word", "jaxsplat_fit_activate", "jaxsplat_fit_l2",
+    "jaxsplat_fit_adam",
 )
 
 
@@ -105,12 +108,17 @@ _SIGS = {
                                     ctypes.POINTER(View), I32, F32, F32, U32, ctypes.c_int64]),
     "jaxsplat_batch_destroy": (None, [P]),
     "jaxsplat_batch_workspace_size": (SZ, [P]),
+    "jaxsplat_batch_set_depths": (I32, [P, I32, I32]),
     "jaxsplat_batch_num_pixels": (ctypes.c_int64, [P]),
     "jaxsplat_batch_num_gaussians": (ctypes.c_int64, [P]),
     "jaxsplat_batch_pixel_offset": (ctypes.c_int64, [P, I32]),
     "jaxsplat_batch_forward": (I32, [P] + [P] * 10 + [P, SZ, P]),
     "jaxsplat_batch_backward": (I32, [P] + [P] * 15 + [I32, P, SZ, P]),
     "jaxsplat_batch_check": (I32, [P, P, ctypes.POINTER(ctypes.c_int64), P]),
+    "jaxsplat_batch_status_word": (P, [P, P]),
+    "jaxsplat_fit_activate": (I32, [ctypes.c_int64] + [P] * 6 + [P]),
+    "jaxsplat_fit_l2": (I32, [ctypes.c_int64] + [P] * 4 + [P]),
+    "jaxsplat_fit_adam": (I32, [ctypes.c_int64] + [P] * 14 + [F32, F32, F32, F32, P]),
 }
 
 

@@ -69,22 +69,24 @@ class _Project(torch.autograd.Function):
         conics = torch.empty((N, 3), dtype=torch.float32, device=dev)
         nth = torch.empty((N, 1), dtype=torch.int32, device=dev)
         cum = torch.empty((N, 1), dtype=torch.int32, device=dev)
+        comp = torch.empty((N, 1), dtype=torch.float32, device=dev)
         ws_bytes = L.jaxsplat_project_workspace_size(ctypes.byref(d))
         ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=dev)
         with torch.cuda.device(dev):
             _lib.check(L.jaxsplat_project_forward(
                 ctypes.byref(d), _p(means3d), _p(scales), _p(quats), _p(viewmat), None,
-                _p(xys), _p(depths), _p(radii), _p(conics), None, _p(nth), _p(cum),
+                _p(xys), _p(depths), _p(radii), _p(conics), _p(comp), _p(nth), _p(cum),
                 _p(ws), ws_bytes, _stream(dev)))
         ctx.d = d
         ctx.adjoint = adjoint
-        ctx.save_for_backward(means3d, scales, quats, viewmat, radii, conics)
+        ctx.save_for_backward(means3d, scales, quats, viewmat, radii, conics, comp)
         ctx.mark_non_differentiable(radii, nth, cum)
-        return xys, depths, radii, conics, nth, cum
+        ctx.set_materialize_grads(False)
+        return xys, depths, radii, conics, nth, cum, comp
 
     @staticmethod
-    def backward(ctx, v_xys, v_depths, _v_radii, v_conics, _v_nth, _v_cum):
-        means3d, scales, quats, viewmat, radii, conics = ctx.saved_tensors
+    def backward(ctx, v_xys, v_depths, _v_radii, v_conics, _v_nth, _v_cum, v_comp):
+        means3d, scales, quats, viewmat, radii, conics, comp = ctx.saved_tensors
         N = means3d.shape[0]
         dev = means3d.device
         v_xys = torch.zeros((N, 2), dtype=torch.float32, device=dev) if v_xys is None \
@@ -92,6 +94,7 @@ class _Project(torch.autograd.Function):
         v_conics = torch.zeros((N, 3), dtype=torch.float32, device=dev) if v_conics is None \
             else v_conics.contiguous()
         v_depths = v_depths.contiguous() if v_depths is not None else None
+        v_comp = v_comp.contiguous() if v_comp is not None else None
         v_mean3d = torch.empty_like(means3d)
         v_scale = torch.empty_like(scales)
         v_quat = torch.empty_like(quats)
@@ -99,20 +102,25 @@ class _Project(torch.autograd.Function):
         with torch.cuda.device(dev):
             _lib.check(L.jaxsplat_project_backward(
                 ctypes.byref(ctx.d), _p(means3d), _p(scales), _p(quats), _p(viewmat),
-                _p(radii), _p(conics), None, _p(v_xys), _p(v_depths), _p(v_conics), None,
-                _p(v_mean3d), _p(v_scale), _p(v_quat), None, None, ctx.adjoint,
+                _p(radii), _p(conics), _p(comp), _p(v_xys), _p(v_depths), _p(v_conics),
+                _p(v_comp), _p(v_mean3d), _p(v_scale), _p(v_quat), None, None, ctx.adjoint,
                 _stream(dev)))
         return None, v_mean3d, v_scale, v_quat, None
 
 
 def project(means3d, scales, quats, viewmat, *, img_shape, f, c, glob_scale, clip_thresh,
-            block_width, adjoint="reference"):
+            block_width, adjoint="reference", return_compensation=False):
     """Project 3D Gaussians to the image plane (reference ``_project/__init__.py:56-87``).
 
     Returns ``(xys (N,2), depths (N,1), radii (N,1) int32, conics (N,3),
     num_tiles_hit (N,1) int32, cum_tiles_hit (N,1) int32)``.  The reference
     types the two counts uint32; torch's uint32 lacks arithmetic, so they are
     int32 here (the values are identical, all < 2^31).
+
+    ``return_compensation=True`` appends the differentiable compensation
+    (N,1) = sqrt(det(cov2d) / det(cov2d + 0.3 I)) that the reference computes
+    and drops (``forward.cu:537-543``, ``_project/__init__.py:78-87``); its
+    cotangent reaches the projection backward (``helpers.h:73-87``).
     """
     adj = _adjoint(adjoint)
     N = means3d.shape[0] if isinstance(means3d, torch.Tensor) else -1
@@ -122,8 +130,9 @@ def project(means3d, scales, quats, viewmat, *, img_shape, f, c, glob_scale, cli
     _expect("viewmat", viewmat, (4, 4), torch.float32)
     cfg = (tuple(img_shape), tuple(f), tuple(c), float(glob_scale), float(clip_thresh),
            int(block_width), adj)
-    return _Project.apply(cfg, means3d.contiguous(), scales.contiguous(), quats.contiguous(),
-                          viewmat.contiguous())
+    out = _Project.apply(cfg, means3d.contiguous(), scales.contiguous(), quats.contiguous(),
+                         viewmat.contiguous())
+    return out if return_compensation else out[:6]
 
 
 def bin_and_sort(xys, depths, radii, cum_tiles_hit, *, img_shape, block_width,

@@ -90,6 +90,7 @@ class BatchPlan:
         self.workspace_bytes = int(L.jaxsplat_batch_workspace_size(h))
         self.workspace = torch.empty(self.workspace_bytes, dtype=torch.uint8, device=self.device)
         self.generation = 0          # bumped by every forward; the backward checks it
+        self.depths = (0, 0)
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -103,6 +104,12 @@ class BatchPlan:
     @property
     def num_views(self):
         return len(self.views)
+
+    def set_depths(self, fwd_depth=0, bwd_depth=0):
+        """Gaussians staged in LDS per batch by the rasterize forward / backward
+        (64, 128 or 256; 0 = workgroup size).  Results do not depend on it."""
+        _lib.check(_lib.lib().jaxsplat_batch_set_depths(self._h, int(fwd_depth), int(bwd_depth)))
+        self.depths = (int(fwd_depth), int(bwd_depth))
 
     def split(self, flat):
         """Per-view (H, W, C) views of a (num_pixels, C) buffer."""

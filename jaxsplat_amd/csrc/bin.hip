@@ -12,14 +12,14 @@
 // This implementation produces exactly that order with far less sort traffic:
 //  1. depth_keys: one 32-bit key per Gaussian (depth bits, or 0xFFFFFFFF for
 //     Gaussians with radius 0, which emit nothing);
-//  2. a stable onesweep sort of the N Gaussians by depth key (values = index),
+//  2. a stable LSD radix sort of the N Gaussians by depth key (values = index),
 //     giving the (depth bits, gaussian id) order;
 //  3. emit: a single-pass kernel walks the Gaussians in that order, scans
 //     their tile counts with a decoupled look-back and writes
-//     (tile id, gaussian id) pairs; within a workgroup the slots are spread
-//     evenly over lanes (binary search over the workgroup's offsets), so a
+//     (tile id, gaussian id) pairs; each wave spreads its slots evenly over
+//     its lanes (owner lookup by an LDS mark and a DPP max-scan), so a
 //     Gaussian covering hundreds of tiles does not serialise one lane;
-//  4. a stable onesweep sort of the pairs on the tile id bits only
+//  4. a stable LSD radix sort of the pairs on the tile id bits only
 //     (2 passes at 1080p/16x16 instead of 6 passes over 64-bit keys);
 //  5. tile_bins_kernel: [start, end) edges from the sorted tile ids, empty
 //     tiles (0, 0).

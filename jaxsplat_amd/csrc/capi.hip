@@ -431,6 +431,7 @@ struct jaxsplat_batch {
     long long num_pixels = 0;
     float glob_scale = 1.f, clip_thresh = 0.f;
     int block_width = 16;
+    int fwd_depth = 0, bwd_depth = 0;     // Gaussians per LDS batch (0: workgroup size)
     jsa::BatchBinPlan bin{};
     std::vector<jsa::ViewRec> views;
     std::vector<jsa::SceneRec> scenes;
@@ -451,6 +452,8 @@ jsa::RasterGrid batch_grid(const jaxsplat_batch* b) {
     g.total_tiles = b->total_tiles;
     g.block_width = b->block_width;
     g.n_items = b->n_items;
+    g.fwd_depth = b->fwd_depth;
+    g.bwd_depth = b->bwd_depth;
     return g;
 }
 
@@ -598,6 +601,18 @@ void jaxsplat_batch_destroy(jaxsplat_batch* b) {
 }
 
 size_t jaxsplat_batch_workspace_size(const jaxsplat_batch* b) { return b ? b->ws_bytes : 0; }
+
+int jaxsplat_batch_set_depths(jaxsplat_batch* b, int32_t fwd_depth, int32_t bwd_depth) {
+    g_err.clear();
+    if (!b) return fail(JAXSPLAT_ERR_INVALID, "batch is NULL");
+    auto ok = [](int d) { return d == 0 || d == 64 || d == 128 || d == 256; };
+    if (!ok(fwd_depth) || !ok(bwd_depth))
+        return fail(JAXSPLAT_ERR_INVALID, "depths must be 0, 64, 128 or 256 (got %d, %d)",
+                    fwd_depth, bwd_depth);
+    b->fwd_depth = fwd_depth;
+    b->bwd_depth = bwd_depth;
+    return JAXSPLAT_OK;
+}
 int64_t jaxsplat_batch_num_pixels(const jaxsplat_batch* b) { return b ? b->num_pixels : 0; }
 int64_t jaxsplat_batch_num_gaussians(const jaxsplat_batch* b) { return b ? b->n_gauss : 0; }
 int64_t jaxsplat_batch_pixel_offset(const jaxsplat_batch* b, int32_t view) {
@@ -675,6 +690,12 @@ int jaxsplat_batch_backward(const jaxsplat_batch* b, const float* means3d, const
                                          v_scales, v_quats, cons, s),
                   "batch_backward: project");
     return JAXSPLAT_OK;
+}
+
+const uint32_t* jaxsplat_batch_status_word(const jaxsplat_batch* b, const void* workspace) {
+    if (!b || !workspace) return nullptr;
+    return jsa::batch_bin_error_word(
+        b->bin, static_cast<const uint8_t*>(workspace) + b->bin_scratch);
 }
 
 int jaxsplat_batch_check(const jaxsplat_batch* b, const void* workspace,

@@ -1,6 +1,6 @@
 // Shared device utilities for the gfx950 (CDNA4) Gaussian-splatting kernels:
 // wave64 reductions on DPP, ballot helpers, the decoupled look-back status
-// words used by the single-pass scans and the onesweep radix sort, and the
+// words used by the single-pass scans (projection and emission), and the
 // internal host-side launch interface shared by the .hip translation units.
 #pragma once
 
@@ -221,6 +221,8 @@ struct RasterGrid {
     int total_tiles;
     int block_width;
     int n_items;          // sorted entries name items in [0, n_items)
+    int fwd_depth;        // Gaussians staged per batch: 64, 128 or 256 (0: workgroup size)
+    int bwd_depth;
 };
 
 ViewRec single_view(const Camera& c);
@@ -246,7 +248,7 @@ hipError_t project_bwd(const Camera& c, const float* means3d, const float* scale
                        float* v_quat, float* v_cov2d, float* v_cov3d, int consistent,
                        hipStream_t s);
 
-// ---- onesweep radix sort (uint32 keys, uint32 values) ----
+// ---- LSD radix sort (uint32 keys, uint32 values) ----
 struct SortPlan {
     int n_cap;          // capacity (upper bound on n) used for launch shapes
     int begin_bit, end_bit;

@@ -1,0 +1,199 @@
+// Device side of the image-fitting loop for gfx950: the training step of the
+// reference's examples/single_image.py (render of activated parameters, L2
+// loss, Adam) as three fused kernels around the render forward/backward.
+//
+//  * fit_activate_kernel: quats / |quats|, sigmoid(colors), sigmoid(opacities)
+//    (single_image.py:143-166); means and scales are used raw;
+//  * fit_l2_kernel: v_img = 2 (img - target) / numel and the loss sum
+//    (single_image.py:26-29: mean of squares), one atomic per workgroup;
+//  * fit_adam_kernel: per Gaussian, the activation VJP (sigmoid' = s (1 - s);
+//    d(q/|q|) = (I - q^ q^T) / |q|) then Adam with bias correction
+//    (optax.adam defaults b1 = 0.9, b2 = 0.999, eps = 1e-8 added to sqrt(v^));
+//    the step counter lives on the device so a captured graph can be
+//    replayed, and a non-zero skip word (the render's capacity-overflow flag)
+//    turns the step into a no-op, so an overflowing render never updates the
+//    parameters;
+//  * fit_finish_kernel advances the step counter unless the step was skipped.
+// All stores are plain vector stores; the loss accumulator is cleared by a
+// kernel, so the whole step may be captured.
+#include "../../include/jaxsplat_amd.h"
+#include "common.hpp"
+
+namespace jsa {
+namespace {
+
+__device__ __forceinline__ float sigmoidf_(float x) { return 1.f / (1.f + __expf(-x)); }
+
+__global__ __launch_bounds__(256) void fit_activate_kernel(
+    int n, const float4* __restrict__ quats_raw, const float* __restrict__ colors_raw,
+    const float* __restrict__ opac_raw, float4* __restrict__ quats, float* __restrict__ colors,
+    float* __restrict__ opac) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float4 q = quats_raw[i];
+    const float inv = rsqrtf(q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w);
+    quats[i] = make_float4(q.x * inv, q.y * inv, q.z * inv, q.w * inv);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) colors[3 * i + k] = sigmoidf_(colors_raw[3 * i + k]);
+    opac[i] = sigmoidf_(opac_raw[i]);
+}
+
+__global__ __launch_bounds__(256) void fit_l2_kernel(size_t n, const float* __restrict__ img,
+                                                      const float* __restrict__ target,
+                                                      float scale, float* __restrict__ v_img,
+                                                      float* __restrict__ loss_sum) {
+    __shared__ float s_part[4];
+    float acc = 0.f;
+    for (size_t i = blockIdx.x * (size_t)256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+        const float d = img[i] - target[i];
+        v_img[i] = scale * d;
+        acc += d * d;
+    }
+    acc = wave_sum_to_lane63(acc);
+    const int lane = lane_id(), wave = threadIdx.x >> 6;
+    if (lane == 63) s_part[wave] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(loss_sum, (s_part[0] + s_part[1]) + (s_part[2] + s_part[3]));
+}
+
+__global__ void fit_begin_kernel(float* __restrict__ loss_sum) { *loss_sum = 0.f; }
+
+__device__ __forceinline__ float adam_one(float& p, float& m, float& v, float g, float lr,
+                                          float b1, float b2, float eps, float c1, float c2) {
+    m = b1 * m + (1.f - b1) * g;
+    v = b2 * v + (1.f - b2) * g * g;
+    const float mh = m / c1, vh = v / c2;
+    p = p - lr * mh / (sqrtf(vh) + eps);
+    return p;
+}
+
+// Parameter groups per Gaussian, in the order of the moment rows:
+// means 3 | scales 3 | quats 4 | colors 3 | opacity 1  (14 floats).
+__global__ __launch_bounds__(256) void fit_adam_kernel(
+    int n, float* __restrict__ means, float* __restrict__ scales, float* __restrict__ quats_raw,
+    float* __restrict__ colors_raw, float* __restrict__ opac_raw,
+    const float* __restrict__ v_means, const float* __restrict__ v_scales,
+    const float* __restrict__ v_quats, const float* __restrict__ v_colors,
+    const float* __restrict__ v_opac, float* __restrict__ m1, float* __restrict__ m2,
+    const int* __restrict__ step, const uint32_t* __restrict__ skip, float lr, float b1,
+    float b2, float eps) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    if (skip && *skip) return;
+    const float t = (float)(*step + 1);
+    const float c1 = 1.f - powf(b1, t), c2 = 1.f - powf(b2, t);
+    float g[14], p[14];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        p[k] = means[3 * i + k];
+        g[k] = v_means[3 * i + k];
+        p[3 + k] = scales[3 * i + k];
+        g[3 + k] = v_scales[3 * i + k];
+    }
+    // quaternion normalisation VJP: (g - q^ (q^ . g)) / |q|
+    {
+        float q[4], gq[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            q[k] = quats_raw[4 * i + k];
+            gq[k] = v_quats[4 * i + k];
+        }
+        const float nrm = sqrtf(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+        const float inv = 1.f / nrm;
+        float dot = 0.f;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) dot += q[k] * inv * gq[k];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            p[6 + k] = q[k];
+            g[6 + k] = (gq[k] - q[k] * inv * dot) * inv;
+        }
+    }
+    // sigmoid VJP: g s (1 - s)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float x = colors_raw[3 * i + k];
+        const float s = sigmoidf_(x);
+        p[10 + k] = x;
+        g[10 + k] = v_colors[3 * i + k] * s * (1.f - s);
+    }
+    {
+        const float x = opac_raw[i];
+        const float s = sigmoidf_(x);
+        p[13] = x;
+        g[13] = v_opac[i] * s * (1.f - s);
+    }
+    float mm[14], vv[14];
+#pragma unroll
+    for (int k = 0; k < 14; ++k) {
+        mm[k] = m1[14 * (size_t)i + k];
+        vv[k] = m2[14 * (size_t)i + k];
+        adam_one(p[k], mm[k], vv[k], g[k], lr, b1, b2, eps, c1, c2);
+        m1[14 * (size_t)i + k] = mm[k];
+        m2[14 * (size_t)i + k] = vv[k];
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        means[3 * i + k] = p[k];
+        scales[3 * i + k] = p[3 + k];
+        colors_raw[3 * i + k] = p[10 + k];
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) quats_raw[4 * i + k] = p[6 + k];
+    opac_raw[i] = p[13];
+}
+
+__global__ void fit_finish_kernel(int* __restrict__ step, const uint32_t* __restrict__ skip) {
+    if (!(skip && *skip)) *step += 1;
+}
+
+}  // namespace
+}  // namespace jsa
+
+extern "C" {
+
+int jaxsplat_fit_activate(int64_t n, const float* quats_raw, const float* colors_raw,
+                          const float* opacities_raw, float* quats, float* colors,
+                          float* opacities, void* stream) {
+    if (n < 0 || n > (1LL << 30)) return JAXSPLAT_ERR_INVALID;
+    if (n == 0) return JAXSPLAT_OK;
+    if (!quats_raw || !colors_raw || !opacities_raw || !quats || !colors || !opacities)
+        return JAXSPLAT_ERR_INVALID;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    jsa::fit_activate_kernel<<<(int)((n + 255) / 256), 256, 0, s>>>(
+        (int)n, reinterpret_cast<const float4*>(quats_raw), colors_raw, opacities_raw,
+        reinterpret_cast<float4*>(quats), colors, opacities);
+    return hipGetLastError() == hipSuccess ? JAXSPLAT_OK : JAXSPLAT_ERR_HIP;
+}
+
+int jaxsplat_fit_l2(int64_t n, const float* img, const float* target, float* v_img,
+                    float* loss_sum, void* stream) {
+    if (n <= 0 || !img || !target || !v_img || !loss_sum) return JAXSPLAT_ERR_INVALID;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    jsa::fit_begin_kernel<<<1, 1, 0, s>>>(loss_sum);
+    const int64_t blocks = (n + 255) / 256;
+    jsa::fit_l2_kernel<<<(int)(blocks < 2048 ? blocks : 2048), 256, 0, s>>>(
+        (size_t)n, img, target, 2.f / (float)n, v_img, loss_sum);
+    return hipGetLastError() == hipSuccess ? JAXSPLAT_OK : JAXSPLAT_ERR_HIP;
+}
+
+int jaxsplat_fit_adam(int64_t n, float* means3d, float* scales, float* quats_raw,
+                      float* colors_raw, float* opacities_raw, const float* v_means3d,
+                      const float* v_scales, const float* v_quats, const float* v_colors,
+                      const float* v_opacities, float* moment1, float* moment2, int32_t* step,
+                      const uint32_t* skip, float lr, float beta1, float beta2, float eps,
+                      void* stream) {
+    if (n < 0 || n > (1LL << 30)) return JAXSPLAT_ERR_INVALID;
+    if (n == 0) return JAXSPLAT_OK;
+    if (!means3d || !scales || !quats_raw || !colors_raw || !opacities_raw || !v_means3d ||
+        !v_scales || !v_quats || !v_colors || !v_opacities || !moment1 || !moment2 || !step)
+        return JAXSPLAT_ERR_INVALID;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    jsa::fit_adam_kernel<<<(int)((n + 255) / 256), 256, 0, s>>>(
+        (int)n, means3d, scales, quats_raw, colors_raw, opacities_raw, v_means3d, v_scales,
+        v_quats, v_colors, v_opacities, moment1, moment2, step, skip, lr, beta1, beta2, eps);
+    jsa::fit_finish_kernel<<<1, 1, 0, s>>>(step, skip);
+    return hipGetLastError() == hipSuccess ? JAXSPLAT_OK : JAXSPLAT_ERR_HIP;
+}
+
+}  // extern "C"

@@ -141,16 +141,18 @@ __device__ __forceinline__ bool group_hit(const CullInfo& k, float4 r) {
 // ---------------------------------------------------------------------------
 // Forward: one pixel per lane, one 64-pixel group per wave.
 // ---------------------------------------------------------------------------
+template <int DEPTH>
 __global__ __launch_bounds__(256) void raster_fwd_kernel(
     RasterGrid rg, const int* __restrict__ gids, const int2* __restrict__ bins,
     const float2* __restrict__ xys, const float* __restrict__ conics,
     const float* __restrict__ colors, const float* __restrict__ opac,
     const float* __restrict__ bgs, float* __restrict__ final_Ts, int* __restrict__ final_idx,
     float* __restrict__ out_img) {
-    __shared__ float4 s_a[kMaxTileThreads];   // x, y, opacity, id
-    __shared__ float4 s_b[kMaxTileThreads];   // conic a, b, c
-    __shared__ float4 s_c[kMaxTileThreads];   // colour
-    __shared__ uint64_t s_mask[4][4];         // [group][64-entry word]
+    constexpr int NW = DEPTH / 64;            // 64-entry words per batch
+    __shared__ float4 s_a[DEPTH];             // x, y, opacity, id
+    __shared__ float4 s_b[DEPTH];             // conic a, b, c
+    __shared__ float4 s_c[DEPTH];             // colour
+    __shared__ uint64_t s_mask[4][NW];        // [group][64-entry word]
     __shared__ float4 s_rect[4];
     __shared__ int s_rok[4];
     __shared__ int s_cnt[4];
@@ -186,36 +188,42 @@ __global__ __launch_bounds__(256) void raster_fwd_kernel(
     int cur = 0;
     bool done = !inside;
 
-    for (int bstart = range.x; bstart < range.y; bstart += nthr) {
+    for (int bstart = range.x; bstart < range.y; bstart += DEPTH) {
         const uint64_t dmask = ballot(done);
         if (lane == 0) s_cnt[wave] = __popcll(dmask);
         __syncthreads();
         int ndone = 0;
         for (int w = 0; w < nwaves; ++w) ndone += s_cnt[w];
         if (ndone == nthr) break;
-        const int idx = bstart + tid;
-        uint32_t bits = 0;
-        const int it = idx < range.y ? gids[idx] : -1;
-        if ((unsigned)(it - vr.item_off) < (unsigned)vr.n) {    // an item of this view
-            const int g = it + gshift;
-            const float2 xy = xys[it];
-            const float o = opac[g];
-            const float ca = conics[3 * it + 0], cb_ = conics[3 * it + 1],
-                        cc = conics[3 * it + 2];
-            s_a[tid] = make_float4(xy.x, xy.y, o, __int_as_float(it));
-            s_b[tid] = make_float4(ca, cb_, cc, 0.f);
-            s_c[tid] = make_float4(colors[3 * g + 0], colors[3 * g + 1], colors[3 * g + 2], 0.f);
-            const CullInfo ci = cull_info(xy.x, xy.y, ca, cb_, cc, o);
-            for (int G = 0; G < nwaves; ++G)
-                if (s_rok[G] && group_hit(ci, s_rect[G])) bits |= 1u << G;
-        }
-        for (int G = 0; G < nwaves; ++G) {
-            const uint64_t m = ballot((bits >> G) & 1u);
-            if (lane == 0) s_mask[G][wave] = m;
+        // stage entries e = e0 + tid of the batch (several per lane when DEPTH > nthr)
+        for (int e0 = 0; e0 < DEPTH; e0 += nthr) {
+            const int e = e0 + tid;
+            const int idx = bstart + e;
+            uint32_t bits = 0;
+            const int it = (e < DEPTH && idx < range.y) ? gids[idx] : -1;
+            if ((unsigned)(it - vr.item_off) < (unsigned)vr.n) {    // an item of this view
+                const int g = it + gshift;
+                const float2 xy = xys[it];
+                const float o = opac[g];
+                const float ca = conics[3 * it + 0], cb_ = conics[3 * it + 1],
+                            cc = conics[3 * it + 2];
+                s_a[e] = make_float4(xy.x, xy.y, o, __int_as_float(it));
+                s_b[e] = make_float4(ca, cb_, cc, 0.f);
+                s_c[e] = make_float4(colors[3 * g + 0], colors[3 * g + 1], colors[3 * g + 2],
+                                     0.f);
+                const CullInfo ci = cull_info(xy.x, xy.y, ca, cb_, cc, o);
+                for (int G = 0; G < nwaves; ++G)
+                    if (s_rok[G] && group_hit(ci, s_rect[G])) bits |= 1u << G;
+            }
+            const int word = (e0 >> 6) + wave;
+            for (int G = 0; G < nwaves; ++G) {
+                const uint64_t m = ballot((bits >> G) & 1u);
+                if (lane == 0 && word < NW) s_mask[G][word] = m;
+            }
         }
         __syncthreads();
         bool live = true;
-        for (int q = 0; q < nwaves && live; ++q) {
+        for (int q = 0; q < NW && live; ++q) {
             uint64_t m = uniform64(s_mask[wave][q]);
             while (m) {
                 if (ballot(!done) == 0) {
@@ -307,7 +315,7 @@ __device__ __forceinline__ bool bwd_pixel(PixState& s, const float4 a, const flo
     return true;
 }
 
-template <bool CONSISTENT, bool ABS>
+template <bool CONSISTENT, bool ABS, int DEPTH>
 __global__ __launch_bounds__(128) void raster_bwd_kernel(
     RasterGrid rg, const int* __restrict__ gids, const int2* __restrict__ bins,
     const float2* __restrict__ xys, const float* __restrict__ conics,
@@ -318,13 +326,13 @@ __global__ __launch_bounds__(128) void raster_bwd_kernel(
     float* __restrict__ v_opacity, float* __restrict__ v_xy, float* __restrict__ v_xy_abs,
     float* __restrict__ v_conic) {
     constexpr int NV = ABS ? 11 : 9;     // rgb3, conic3, xy2, [xy_abs2], opacity
-    constexpr int NB = 128;              // batch entries (one per lane)
-    __shared__ float4 s_a[NB];           // x, y, opacity, id
-    __shared__ float4 s_b[NB];           // conic a, b, c
-    __shared__ float4 s_c[NB];           // colour
-    __shared__ float s_g[NB][NV][4];     // per 16-lane row partial sums
-    __shared__ int s_flag[NB];
-    __shared__ uint64_t s_mask[4][2];    // [group = slot * nwaves + wave][64-entry word]
+    constexpr int NW = DEPTH / 64;       // 64-entry words per batch
+    __shared__ float4 s_a[DEPTH];        // x, y, opacity, id
+    __shared__ float4 s_b[DEPTH];        // conic a, b, c
+    __shared__ float4 s_c[DEPTH];        // colour
+    __shared__ float s_g[DEPTH][NV][4];  // per 16-lane row partial sums
+    __shared__ int s_flag[DEPTH];
+    __shared__ uint64_t s_mask[4][NW];   // [group = slot * nwaves + wave][64-entry word]
     __shared__ float4 s_rect[4];
     __shared__ int s_rok[4];
     __shared__ int s_wmax[2];
@@ -378,10 +386,12 @@ __global__ __launch_bounds__(128) void raster_bwd_kernel(
         s_rect[tid] = group_rect(sl * nthr + w * 64, bw, x_org, y_org, vr.img_w, vr.img_h, ok);
         s_rok[tid] = ok;
     }
+    for (int e = tid; e < DEPTH; e += nthr) {
 #pragma unroll
-    for (int v = 0; v < NV; ++v)
-        *reinterpret_cast<float4*>(&s_g[tid][v][0]) = make_float4(0.f, 0.f, 0.f, 0.f);
-    s_flag[tid] = 0;
+        for (int v = 0; v < NV; ++v)
+            *reinterpret_cast<float4*>(&s_g[e][v][0]) = make_float4(0.f, 0.f, 0.f, 0.f);
+        s_flag[e] = 0;
+    }
     const int wave_fin = uniform_i(wave_max_i(my_fin));
     if (lane == 0) s_wmax[wave] = wave_fin;
     __syncthreads();
@@ -390,30 +400,35 @@ __global__ __launch_bounds__(128) void raster_bwd_kernel(
 
     const int2 range = bins[vr.bin_off + tile];
     const int last = min(range.y - 1, block_fin);
-    for (int bend = last; bend >= range.x; bend -= nthr) {
-        // load entry e = tid (sorted index bend - e, back to front)
-        const int idx = bend - tid;
-        uint32_t bits = 0;
-        const int it = idx >= range.x ? gids[idx] : -1;
-        if ((unsigned)(it - vr.item_off) < (unsigned)vr.n) {    // an item of this view
-            const int g = it + gshift;
-            const float2 xy = xys[it];
-            const float o = opac[g];
-            const float ca = conics[3 * it + 0], cb_ = conics[3 * it + 1],
-                        cc = conics[3 * it + 2];
-            s_a[tid] = make_float4(xy.x, xy.y, o, __int_as_float(it));
-            s_b[tid] = make_float4(ca, cb_, cc, 0.f);
-            s_c[tid] = make_float4(colors[3 * g + 0], colors[3 * g + 1], colors[3 * g + 2], 0.f);
-            const CullInfo ci = cull_info(xy.x, xy.y, ca, cb_, cc, o);
-            for (int G = 0; G < ngroups; ++G)
-                if (s_rok[G] && group_hit(ci, s_rect[G])) bits |= 1u << G;
-        }
-        for (int G = 0; G < ngroups; ++G) {
-            const uint64_t m = ballot((bits >> G) & 1u);
-            if (lane == 0) s_mask[G][wave] = m;
+    for (int bend = last; bend >= range.x; bend -= DEPTH) {
+        // stage entries e = e0 + tid (sorted index bend - e, back to front)
+        for (int e0 = 0; e0 < DEPTH; e0 += nthr) {
+            const int e = e0 + tid;
+            const int idx = bend - e;
+            uint32_t bits = 0;
+            const int it = (e < DEPTH && idx >= range.x) ? gids[idx] : -1;
+            if ((unsigned)(it - vr.item_off) < (unsigned)vr.n) {    // an item of this view
+                const int g = it + gshift;
+                const float2 xy = xys[it];
+                const float o = opac[g];
+                const float ca = conics[3 * it + 0], cb_ = conics[3 * it + 1],
+                            cc = conics[3 * it + 2];
+                s_a[e] = make_float4(xy.x, xy.y, o, __int_as_float(it));
+                s_b[e] = make_float4(ca, cb_, cc, 0.f);
+                s_c[e] = make_float4(colors[3 * g + 0], colors[3 * g + 1], colors[3 * g + 2],
+                                     0.f);
+                const CullInfo ci = cull_info(xy.x, xy.y, ca, cb_, cc, o);
+                for (int G = 0; G < ngroups; ++G)
+                    if (s_rok[G] && group_hit(ci, s_rect[G])) bits |= 1u << G;
+            }
+            const int word = (e0 >> 6) + wave;
+            for (int G = 0; G < ngroups; ++G) {
+                const uint64_t m = ballot((bits >> G) & 1u);
+                if (lane == 0 && word < NW) s_mask[G][word] = m;
+            }
         }
         __syncthreads();
-        const int nb = uniform_i(min(nthr, bend - range.x + 1));
+        const int nb = uniform_i(min(DEPTH, bend - range.x + 1));
         const int t0 = uniform_i(max(0, bend - wave_fin));   // entries t < t0 lie after every fin of this wave
         const int nwords = (nb + 63) >> 6;
         for (int q = t0 >> 6; q < nwords; ++q) {
@@ -452,16 +467,17 @@ __global__ __launch_bounds__(128) void raster_bwd_kernel(
             }
         }
         __syncthreads();
-        if (tid < nb && s_flag[tid]) {
+        for (int e = tid; e < nb; e += nthr) {
+            if (!s_flag[e]) continue;
             float r[NV];
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
-                const float4 p4 = *reinterpret_cast<const float4*>(&s_g[tid][v][0]);
+                const float4 p4 = *reinterpret_cast<const float4*>(&s_g[e][v][0]);
                 r[v] = (p4.x + p4.y) + (p4.z + p4.w);
-                *reinterpret_cast<float4*>(&s_g[tid][v][0]) = make_float4(0.f, 0.f, 0.f, 0.f);
+                *reinterpret_cast<float4*>(&s_g[e][v][0]) = make_float4(0.f, 0.f, 0.f, 0.f);
             }
-            s_flag[tid] = 0;
-            const int it = __float_as_int(s_a[tid].w);
+            s_flag[e] = 0;
+            const int it = __float_as_int(s_a[e].w);
             const int gid = it + gshift;
             unsafeAtomicAdd(v_rgb + 3 * gid + 0, r[0]);
             unsafeAtomicAdd(v_rgb + 3 * gid + 1, r[1]);
@@ -538,6 +554,8 @@ RasterGrid single_grid(const Camera& c) {
     g.total_tiles = c.tiles_x * c.tiles_y;
     g.block_width = c.block_width;
     g.n_items = c.num_points;
+    g.fwd_depth = 0;
+    g.bwd_depth = 0;
     return g;
 }
 
@@ -570,10 +588,21 @@ hipError_t raster_fwd_grid(const RasterGrid& g, const int* items_sorted, const i
                            int* final_idx, float* out_img, hipStream_t s) {
     if (g.total_tiles <= 0) return hipSuccess;
     const int nthr = ((g.block_width * g.block_width + 63) / 64) * 64;
-    raster_fwd_kernel<<<tile_grid(g.total_tiles), nthr, 0, s>>>(
-        g, items_sorted, reinterpret_cast<const int2*>(tile_bins),
-        reinterpret_cast<const float2*>(xys), conics, colors, opacities, backgrounds, final_Ts,
-        final_idx, out_img);
+    const int depth = g.fwd_depth > 0 ? g.fwd_depth : nthr;
+    const int grid = tile_grid(g.total_tiles);
+    const int2* b2 = reinterpret_cast<const int2*>(tile_bins);
+    const float2* x2 = reinterpret_cast<const float2*>(xys);
+#define JSA_LAUNCH(D)                                                                        \
+    raster_fwd_kernel<D><<<grid, nthr, 0, s>>>(g, items_sorted, b2, x2, conics, colors,     \
+                                               opacities, backgrounds, final_Ts, final_idx,  \
+                                               out_img)
+    switch (depth) {
+        case 64: JSA_LAUNCH(64); break;
+        case 128: JSA_LAUNCH(128); break;
+        case 256: JSA_LAUNCH(256); break;
+        default: return hipErrorInvalidValue;
+    }
+#undef JSA_LAUNCH
     return hipGetLastError();
 }
 
@@ -597,19 +626,28 @@ hipError_t raster_bwd_grid(const RasterGrid& g, const int* items_sorted, const i
     const int half = (g.block_width * g.block_width + 1) / 2;
     const int nthr = ((half + 63) / 64) * 64;
     const int grid = tile_grid(g.total_tiles);
+    const int depth = g.bwd_depth > 0 ? g.bwd_depth : nthr;
+    if (depth != 64 && depth != 128 && depth != 256) return hipErrorInvalidValue;
     const int* gids_sorted = items_sorted;
     const int2* b2 = reinterpret_cast<const int2*>(tile_bins);
     const float2* x2 = reinterpret_cast<const float2*>(xys);
-#define JSA_LAUNCH(CONS, ABS)                                                                 \
-    raster_bwd_kernel<CONS, ABS><<<grid, nthr, 0, s>>>(                                      \
+#define JSA_LAUNCH_D(CONS, ABS, D)                                                           \
+    raster_bwd_kernel<CONS, ABS, D><<<grid, nthr, 0, s>>>(                                  \
         g, gids_sorted, b2, x2, conics, colors, opacities, background, final_Ts, final_idx,  \
         v_out, v_out_alpha, v_colors, v_opacity, v_xy, v_xy_abs, v_conic)
+#define JSA_LAUNCH(CONS, ABS)                                                                \
+    do {                                                                                     \
+        if (depth == 64) JSA_LAUNCH_D(CONS, ABS, 64);                                        \
+        else if (depth == 128) JSA_LAUNCH_D(CONS, ABS, 128);                                 \
+        else JSA_LAUNCH_D(CONS, ABS, 256);                                                   \
+    } while (0)
     if (consistent) {
         if (v_xy_abs) JSA_LAUNCH(true, true); else JSA_LAUNCH(true, false);
     } else {
         if (v_xy_abs) JSA_LAUNCH(false, true); else JSA_LAUNCH(false, false);
     }
 #undef JSA_LAUNCH
+#undef JSA_LAUNCH_D
     return hipGetLastError();
 }
 

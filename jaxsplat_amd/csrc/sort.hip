@@ -1,4 +1,4 @@
-// Onesweep LSD radix sort of (uint32 key, uint32 value) pairs for gfx950.
+// Reduce-then-scan LSD radix sort of (uint32 key, uint32 value) pairs for gfx950.
 //
 // Replaces the reference's CUB DeviceRadixSort::SortPairs call
 // (lib/ops.cu:722-760): a stable least-significant-digit sort over a bit range.

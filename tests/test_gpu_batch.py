@@ -277,3 +277,28 @@ def test_batch_hip_graph_replays_match_eager():
         with t.no_grad():
             render_batch(*d, vmd, bgd, plan=_plan(sizes, cams))
         t.cuda.synchronize()
+
+
+@pytest.mark.parametrize("bw,fd,bd", [(16, 64, 256), (16, 128, 64), (16, 256, 128),
+                                      (8, 256, 64), (8, 128, 256)])
+def test_batch_lds_depths_do_not_change_results(bw, fd, bd):
+    """Batch depth is a pure performance knob: the image is bit-identical and
+    the gradients equal within the atomics tolerance."""
+    t = H.torch()
+    params, sizes, cams = _layout()
+    vm, bg = _views_arrays(cams)
+    rng = np.random.default_rng(31)
+    outs = []
+    for depths in ((0, 0), (fd, bd)):
+        plan = _plan(sizes, cams, bw)
+        plan.set_depths(*depths)
+        w = H.dev(rng.normal(size=(plan.num_pixels, 3)).astype(np.float32)) if not outs \
+            else outs[0][2]
+        d = _dev_params(params, requires_grad=True)
+        flat = render_batch(*d, H.dev(vm), H.dev(bg), plan=plan)
+        (flat * w).sum().backward()
+        outs.append((H.host(flat), [H.host(x.grad) for x in d], w))
+    assert np.array_equal(outs[0][0], outs[1][0])
+    for a, b in zip(outs[0][1], outs[1][1]):
+        ok, worst = H.grad_close(b, a, rtol=1e-4, scale_frac=1e-5)
+        assert ok, worst
