@@ -226,9 +226,9 @@ int jaxsplat_batch_create(jaxsplat_batch** out, const jaxsplat_scene* scenes, in
 void jaxsplat_batch_destroy(jaxsplat_batch* b);
 size_t jaxsplat_batch_workspace_size(const jaxsplat_batch* b);
 /* Gaussians staged in LDS per batch by the rasterize forward / backward:
- * 64, 128 or 256, or 0 for the workgroup size (bw^2 rounded up to whole
- * waves; half of it in the backward).  The results do not depend on it; the
- * autotuner of the Python layer picks the fastest. */
+ * 64, 128 or 256, or 0 for the defaults (forward: the workgroup size, bw^2
+ * rounded up to whole waves; backward: 64).  The results do not depend on
+ * it; the autotuner of the Python layer picks the fastest. */
 int jaxsplat_batch_set_depths(jaxsplat_batch* b, int32_t fwd_depth, int32_t bwd_depth);
 int64_t jaxsplat_batch_num_pixels(const jaxsplat_batch* b);
 int64_t jaxsplat_batch_num_gaussians(const jaxsplat_batch* b);

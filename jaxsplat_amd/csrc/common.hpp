@@ -222,7 +222,7 @@ struct RasterGrid {
     int block_width;
     int n_items;          // sorted entries name items in [0, n_items)
     int fwd_depth;        // Gaussians staged per batch: 64, 128 or 256 (0: workgroup size)
-    int bwd_depth;
+    int bwd_depth;        // likewise for the backward (0: 64)
 };
 
 ViewRec single_view(const Camera& c);

@@ -275,9 +275,21 @@ struct PixState {
     bool inside;
 };
 
+// One pixel's contribution to the per-Gaussian partial sums, branch-free
+// (an invalid pair contributes zeros and leaves the pixel state unchanged).
+// Only pixel-dependent sums are accumulated; the per-Gaussian factors are
+// applied once per (Gaussian, tile) by finish_grads:
+//   g[0..2] = sum fac * v_out                       (v_rgb)
+//   g[3..5] = sum vs dx^2, vs dx dy, vs dy^2         (v_conic, x 0.5 / 1 / 0.5)
+//   g[6..7] = sum vs dx, vs dy                       (v_xy = conic * (sum vs d))
+//   [g[8..9] = sum |v_xy per pixel|]                 (v_xy_abs)
+//   g[NV-1] = sum vs                                 (v_opacity = -sum vs / o)
+// with vs = dL/dsigma = -o * vis * v_alpha (times the clamp's zero slope in
+// consistent mode).
 template <bool CONSISTENT, int NV>
 __device__ __forceinline__ bool bwd_pixel(PixState& s, const float4 a, const float4 b,
                                           const float4 col, int k, float* g) {
+#define JSA_ACC(i, v) (g[i] += (v))
     constexpr float kClamp = CONSISTENT ? 0.999f : 0.99f;
     const float dx = a.x - s.px, dy = a.y - s.py;
     const float sigma = 0.5f * (b.x * dx * dx + b.z * dy * dy) + b.y * dx * dy;
@@ -285,34 +297,47 @@ __device__ __forceinline__ bool bwd_pixel(PixState& s, const float4 a, const flo
     const float raw = a.z * vis;
     const float alpha = fminf(kClamp, raw);
     const bool valid = s.inside && (k <= s.fin) && sigma >= 0.f && alpha >= kAlphaMin;
-    if (!valid) return false;
     const float ra = __builtin_amdgcn_rcpf(1.f - alpha);
-    s.T *= ra;
-    const float fac = alpha * s.T;
-    g[0] += fac * s.vo_r;
-    g[1] += fac * s.vo_g;
-    g[2] += fac * s.vo_b;
-    const float v_alpha = (col.x * s.T - s.buf_r * ra) * s.vo_r +
-                          (col.y * s.T - s.buf_g * ra) * s.vo_g +
-                          (col.z * s.T - s.buf_b * ra) * s.vo_b + ra * s.bgdot;
+    const float T = valid ? s.T * ra : s.T;      // transmittance in front of this Gaussian
+    s.T = T;
+    const float fac = valid ? alpha * T : 0.f;
+    JSA_ACC(0, fac * s.vo_r);
+    JSA_ACC(1, fac * s.vo_g);
+    JSA_ACC(2, fac * s.vo_b);
+    const float v_alpha = (col.x * T - s.buf_r * ra) * s.vo_r +
+                          (col.y * T - s.buf_g * ra) * s.vo_g +
+                          (col.z * T - s.buf_b * ra) * s.vo_b + ra * s.bgdot;
     s.buf_r += col.x * fac;
     s.buf_g += col.y * fac;
     s.buf_b += col.z * fac;
-    const float slope = (CONSISTENT && raw >= kClamp) ? 0.f : 1.f;
-    const float v_sigma = -a.z * vis * v_alpha * slope;
-    g[3] += 0.5f * v_sigma * dx * dx;
-    g[4] += v_sigma * dx * dy;
-    g[5] += 0.5f * v_sigma * dy * dy;
-    const float gx = v_sigma * (b.x * dx + b.y * dy);
-    const float gy = v_sigma * (b.y * dx + b.z * dy);
-    g[6] += gx;
-    g[7] += gy;
+    const bool live = valid && !(CONSISTENT && raw >= kClamp);
+    const float vs = live ? -raw * v_alpha : 0.f;
+    const float p = vs * dx, q = vs * dy;
+    JSA_ACC(3, p * dx);
+    JSA_ACC(4, p * dy);
+    JSA_ACC(5, q * dy);
+    JSA_ACC(6, p);
+    JSA_ACC(7, q);
     if (NV == 11) {
-        g[8] += fabsf(gx);
-        g[9] += fabsf(gy);
+        JSA_ACC(8, fabsf(b.x * p + b.y * q));
+        JSA_ACC(9, fabsf(b.y * p + b.z * q));
     }
-    g[NV - 1] += vis * v_alpha * slope;
-    return true;
+    JSA_ACC(NV - 1, vs);
+#undef JSA_ACC
+    return valid;
+}
+
+// Per-Gaussian factors of the partial sums of bwd_pixel (conic (a, b, c) and
+// opacity o of the Gaussian): r[] becomes (v_rgb 3, v_conic 3, v_xy 2,
+// [v_xy_abs 2], v_opacity).  o > 0 whenever a pair was valid (alpha >= 1/255).
+template <int NV>
+__device__ __forceinline__ void finish_grads(float* r, float4 conic, float o) {
+    r[3] *= 0.5f;
+    r[5] *= 0.5f;
+    const float sp = r[6], sq = r[7];
+    r[6] = conic.x * sp + conic.y * sq;
+    r[7] = conic.y * sp + conic.z * sq;
+    r[NV - 1] = o > 0.f ? -r[NV - 1] / o : 0.f;
 }
 
 template <bool CONSISTENT, bool ABS, int DEPTH>
@@ -477,6 +502,7 @@ __global__ __launch_bounds__(128) void raster_bwd_kernel(
                 *reinterpret_cast<float4*>(&s_g[e][v][0]) = make_float4(0.f, 0.f, 0.f, 0.f);
             }
             s_flag[e] = 0;
+            finish_grads<NV>(r, s_b[e], s_a[e].z);
             const int it = __float_as_int(s_a[e].w);
             const int gid = it + gshift;
             unsafeAtomicAdd(v_rgb + 3 * gid + 0, r[0]);
@@ -626,7 +652,10 @@ hipError_t raster_bwd_grid(const RasterGrid& g, const int* items_sorted, const i
     const int half = (g.block_width * g.block_width + 1) / 2;
     const int nthr = ((half + 63) / 64) * 64;
     const int grid = tile_grid(g.total_tiles);
-    const int depth = g.bwd_depth > 0 ? g.bwd_depth : nthr;
+    // 64 Gaussians per batch by default: the LDS partial sums (depth x 9..11 x 4
+    // floats) otherwise limit the workgroups per CU (measured fastest on the
+    // 1080p, 800x800 and mixed-size workloads)
+    const int depth = g.bwd_depth > 0 ? g.bwd_depth : 64;
     if (depth != 64 && depth != 128 && depth != 256) return hipErrorInvalidValue;
     const int* gids_sorted = items_sorted;
     const int2* b2 = reinterpret_cast<const int2*>(tile_bins);
