@@ -182,6 +182,31 @@ int jaxsplat_rasterize_backward_binned(
     float* v_colors, float* v_opacity, float* v_xy, float* v_xy_abs, float* v_conic,
     int adjoint, void* stream);
 
+/*
+ * N-channel rasterization (colors (N, C), background (C), out_img (H, W, C),
+ * v_out_img (H, W, C), v_colors (N, C)); replaces nd_rasterize_forward
+ * (lib/kernels/forward.cu:213-337) and nd_rasterize_backward_kernel
+ * (lib/kernels/backward.cu:27-151), which the reference builds but never
+ * calls.  The blend and its adjoint are those of the 3-channel entry points
+ * for every channel, accumulated in fp32 (the reference's dead N-channel code
+ * accumulates in fp16 and its backward skips the last contributor); C = 3
+ * gives exactly the 3-channel results.  Channels run in passes of up to
+ * three.  v_xy_abs must be NULL for C > 3 (it does not add over passes).
+ */
+int jaxsplat_rasterize_forward_nd(const jaxsplat_descriptor* d,
+                                  const int32_t* gaussian_ids_sorted, const int32_t* tile_bins,
+                                  uint32_t num_intersects, const float* xys, const float* conics,
+                                  const float* colors, const float* opacities,
+                                  const float* background, uint32_t channels, float* final_Ts,
+                                  int32_t* final_idx, float* out_img, void* stream);
+int jaxsplat_rasterize_backward_nd(
+    const jaxsplat_descriptor* d, const int32_t* gaussian_ids_sorted, const int32_t* tile_bins,
+    uint32_t num_intersects, const float* xys, const float* conics, const float* colors,
+    const float* opacities, const float* background, uint32_t channels, const float* final_Ts,
+    const int32_t* final_idx, const float* v_out_img, const float* v_out_img_alpha,
+    float* v_colors, float* v_opacity, float* v_xy, float* v_xy_abs, float* v_conic,
+    int adjoint, void* stream);
+
 /* ------------------------------------------------------------------------
  * Batched rendering: many views of one or more scenes in one pass.
  *

@@ -28,7 +28,8 @@ def render(means3d, scales, quats, colors, opacities, *, viewmat, background, im
     (N,3), opacities (N,1); viewmat (4,4) row-major world-to-camera;
     background (3,); img_shape (H, W); f (fx, fy); c (cx, cy); glob_scale;
     clip_thresh; block_size in [1, 16] (tile side).  Differentiable in means3d,
-    scales, quats, colors and opacities.
+    scales, quats, colors and opacities.  colors (N, C) with background (C,)
+    renders C channels (features, depth, ...) with the same blend.
 
     ``antialiased=True`` (not in the reference API) blends with
     ``opacities * compensation``, the compensation that the reference's

@@ -27,6 +27,7 @@ EXPORTS = (
     "jaxsplat_project_forward", "jaxsplat_project_backward",
     "jaxsplat_bin_workspace_size", "jaxsplat_bin_and_sort", "jaxsplat_bin_check",
     "jaxsplat_rasterize_forward_binned", "jaxsplat_rasterize_backward_binned",
+    "jaxsplat_rasterize_forward_nd", "jaxsplat_rasterize_backward_nd",
     "jaxsplat_batch_create", "jaxsplat_batch_destroy", "jaxsplat_batch_workspace_size",
     "jaxsplat_batch_set_depths",
     "jaxsplat_batch_num_pixels", "jaxsplat_batch_num_gaussians", "jaxsplat_batch_pixel_offset",
@@ -104,6 +105,9 @@ _SIGS = {
     "jaxsplat_bin_check": (I32, [DP, U32, P, P]),
     "jaxsplat_rasterize_forward_binned": (I32, [DP, P, P, U32] + [P] * 8 + [P]),
     "jaxsplat_rasterize_backward_binned": (I32, [DP, P, P, U32] + [P] * 14 + [I32, P]),
+    "jaxsplat_rasterize_forward_nd": (I32, [DP, P, P, U32] + [P] * 5 + [U32] + [P] * 3 + [P]),
+    "jaxsplat_rasterize_backward_nd": (I32, [DP, P, P, U32] + [P] * 5 + [U32] + [P] * 9
+                                       + [I32, P]),
     "jaxsplat_batch_create": (I32, [ctypes.POINTER(P), ctypes.POINTER(Scene), I32,
                                     ctypes.POINTER(View), I32, F32, F32, U32, ctypes.c_int64]),
     "jaxsplat_batch_destroy": (None, [P]),

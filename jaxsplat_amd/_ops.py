@@ -169,7 +169,7 @@ class _Rasterize(torch.autograd.Function):
     def forward(ctx, cfg, colors, opacities, background, xys, depths, radii, conics,
                 cum_tiles_hit):
         img_shape, block_width, adjoint = cfg
-        N = colors.shape[0]
+        N, nch = colors.shape
         dev = colors.device
         H, W = img_shape
         d = _lib.make_descriptor(N, img_shape, (1.0, 1.0), (0.0, 0.0), 1.0, 0.0, block_width)
@@ -177,12 +177,12 @@ class _Rasterize(torch.autograd.Function):
         gids, bins, _ = _bin(d, xys, depths, radii, isect, dev)
         final_Ts = torch.empty((H, W, 1), dtype=torch.float32, device=dev)
         final_idx = torch.empty((H, W, 1), dtype=torch.int32, device=dev)
-        img = torch.empty((H, W, 3), dtype=torch.float32, device=dev)
+        img = torch.empty((H, W, nch), dtype=torch.float32, device=dev)
         L = _lib.lib()
         with torch.cuda.device(dev):
-            _lib.check(L.jaxsplat_rasterize_forward_binned(
+            _lib.check(L.jaxsplat_rasterize_forward_nd(
                 ctypes.byref(d), _p(gids), _p(bins), isect, _p(xys), _p(conics), _p(colors),
-                _p(opacities), _p(background), _p(final_Ts), _p(final_idx), _p(img),
+                _p(opacities), _p(background), nch, _p(final_Ts), _p(final_idx), _p(img),
                 _stream(dev)))
         ctx.d = d
         ctx.isect = isect
@@ -203,11 +203,11 @@ class _Rasterize(torch.autograd.Function):
         v_conic = torch.empty_like(conics)
         L = _lib.lib()
         with torch.cuda.device(dev):
-            _lib.check(L.jaxsplat_rasterize_backward_binned(
+            _lib.check(L.jaxsplat_rasterize_backward_nd(
                 ctypes.byref(ctx.d), _p(gids), _p(bins), ctx.isect, _p(xys), _p(conics),
-                _p(colors), _p(opacities), _p(background), _p(final_Ts), _p(final_idx),
-                _p(v_img), None, _p(v_colors), _p(v_opacity), _p(v_xy), None, _p(v_conic),
-                ctx.adjoint, _stream(dev)))
+                _p(colors), _p(opacities), _p(background), colors.shape[1], _p(final_Ts),
+                _p(final_idx), _p(v_img), None, _p(v_colors), _p(v_opacity), _p(v_xy), None,
+                _p(v_conic), ctx.adjoint, _stream(dev)))
         return None, v_colors, v_opacity, None, v_xy, None, None, v_conic, None
 
 
@@ -215,13 +215,19 @@ def rasterize(colors, opacities, background, xys, depths, radii, conics, cum_til
               img_shape, block_width, adjoint="reference"):
     """Rasterize projected Gaussians (reference ``_rasterize/__init__.py:40-68``).
 
-    Returns the (H, W, 3) image.
+    Returns the (H, W, C) image for colors (N, C) and background (C,).  The
+    reference API has C = 3; other channel counts use the N-channel
+    rasterizer (``lib/kernels/forward.cu:213-337``, never called by the
+    reference), with the same blend per channel in fp32.
     """
     adj = _adjoint(adjoint)
     N = colors.shape[0] if isinstance(colors, torch.Tensor) else -1
-    _expect("colors", colors, (N, 3), torch.float32)
+    nch = colors.shape[1] if isinstance(colors, torch.Tensor) and colors.dim() == 2 else 3
+    if nch < 1:
+        raise ShapeDtypeError("colors: need at least one channel")
+    _expect("colors", colors, (N, nch), torch.float32)
     _expect("opacities", opacities, (N, 1), torch.float32)
-    _expect("background", background, (3,), torch.float32)
+    _expect("background", background, (nch,), torch.float32)
     _expect("xys", xys, (N, 2), torch.float32)
     _expect("depths", depths, (N, 1), torch.float32)
     _expect("radii", radii, (N, 1), torch.int32)

@@ -17,7 +17,7 @@
 
 namespace jsa {
 hipError_t raster_empty(const Camera& c, const float* background, float* final_Ts,
-                        int* final_idx, float* out_img, hipStream_t s);
+                        int* final_idx, float* out_img, hipStream_t s, int nch = 3);
 hipError_t zero_grads(float* const* ptrs, const size_t* counts, int k, hipStream_t s);
 }  // namespace jsa
 
@@ -334,6 +334,88 @@ int jaxsplat_rasterize_backward_binned(
     return JAXSPLAT_OK;
 }
 
+int jaxsplat_rasterize_forward_nd(const jaxsplat_descriptor* d,
+                                  const int32_t* gaussian_ids_sorted, const int32_t* tile_bins,
+                                  uint32_t num_intersects, const float* xys, const float* conics,
+                                  const float* colors, const float* opacities,
+                                  const float* background, uint32_t channels, float* final_Ts,
+                                  int32_t* final_idx, float* out_img, void* stream) {
+    g_err.clear();
+    if (int st = validate(d)) return st;
+    if (channels < 1 || channels > 4096)
+        return fail(JAXSPLAT_ERR_INVALID, "channels must be in [1, 4096], got %u", channels);
+    if (!background || !final_Ts || !final_idx || !out_img)
+        return fail(JAXSPLAT_ERR_INVALID, "rasterize_forward_nd: NULL required pointer");
+    const jsa::Camera c = camera_of(d);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (num_intersects == 0) {
+        JSA_CHECK_HIP(jsa::raster_empty(c, background, final_Ts, final_idx, out_img, s,
+                                        (int)channels),
+                      "rasterize_forward_nd");
+        return JAXSPLAT_OK;
+    }
+    if (!gaussian_ids_sorted || !tile_bins || !xys || !conics || !colors || !opacities)
+        return fail(JAXSPLAT_ERR_INVALID, "rasterize_forward_nd: NULL required pointer");
+    jsa::RasterGrid g = jsa::single_grid(c);
+    g.ch_stride = (int)channels;
+    // channels in passes of up to three: the blend decisions are the same in every pass
+    for (int off = 0; off < (int)channels; off += 3) {
+        g.ch_off = off;
+        g.ch_n = std::min(3, (int)channels - off);
+        JSA_CHECK_HIP(jsa::raster_fwd_grid(g, gaussian_ids_sorted, tile_bins, xys, conics, colors,
+                                           opacities, background, final_Ts, final_idx, out_img,
+                                           s),
+                      "rasterize_forward_nd");
+    }
+    return JAXSPLAT_OK;
+}
+
+int jaxsplat_rasterize_backward_nd(
+    const jaxsplat_descriptor* d, const int32_t* gaussian_ids_sorted, const int32_t* tile_bins,
+    uint32_t num_intersects, const float* xys, const float* conics, const float* colors,
+    const float* opacities, const float* background, uint32_t channels, const float* final_Ts,
+    const int32_t* final_idx, const float* v_out_img, const float* v_out_img_alpha,
+    float* v_colors, float* v_opacity, float* v_xy, float* v_xy_abs, float* v_conic,
+    int adjoint, void* stream) {
+    g_err.clear();
+    if (int st = validate(d)) return st;
+    if (channels < 1 || channels > 4096)
+        return fail(JAXSPLAT_ERR_INVALID, "channels must be in [1, 4096], got %u", channels);
+    if (v_xy_abs && channels > 3)
+        return fail(JAXSPLAT_ERR_INVALID,
+                    "v_xy_abs is only available for up to 3 channels (it is not additive over "
+                    "channel passes)");
+    if (adjoint != JAXSPLAT_ADJOINT_REFERENCE && adjoint != JAXSPLAT_ADJOINT_CONSISTENT)
+        return fail(JAXSPLAT_ERR_INVALID, "unknown adjoint mode %d", adjoint);
+    if (!v_colors || !v_opacity || !v_xy || !v_conic)
+        return fail(JAXSPLAT_ERR_INVALID, "rasterize_backward_nd: NULL gradient pointer");
+    const jsa::Camera c = camera_of(d);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const size_t n = d->num_points;
+    float* ptrs[5] = {v_colors, v_opacity, v_xy, v_conic, v_xy_abs};
+    const size_t cnt[5] = {(size_t)channels * n, n, 2 * n, 3 * n, v_xy_abs ? 2 * n : 0};
+    JSA_CHECK_HIP(jsa::zero_grads(ptrs, cnt, 5, s), "rasterize_backward_nd");
+    if (num_intersects == 0 || n == 0) return JAXSPLAT_OK;
+    if (!gaussian_ids_sorted || !tile_bins || !xys || !conics || !colors || !opacities ||
+        !background || !final_Ts || !final_idx || !v_out_img)
+        return fail(JAXSPLAT_ERR_INVALID, "rasterize_backward_nd: NULL required pointer");
+    jsa::RasterGrid g = jsa::single_grid(c);
+    g.ch_stride = (int)channels;
+    // geometry gradients are linear in v_alpha, which sums over channels: the
+    // passes add into the same arrays; the alpha cotangent enters once
+    for (int off = 0; off < (int)channels; off += 3) {
+        g.ch_off = off;
+        g.ch_n = std::min(3, (int)channels - off);
+        JSA_CHECK_HIP(jsa::raster_bwd_grid(g, gaussian_ids_sorted, tile_bins, xys, conics, colors,
+                                           opacities, background, final_Ts, final_idx, v_out_img,
+                                           off == 0 ? v_out_img_alpha : nullptr, v_colors,
+                                           v_opacity, v_xy, v_xy_abs, v_conic,
+                                           adjoint == JAXSPLAT_ADJOINT_CONSISTENT, s),
+                      "rasterize_backward_nd");
+    }
+    return JAXSPLAT_OK;
+}
+
 // ---------------------------------------------------------------------------
 // Reference-compatible custom-call targets
 // ---------------------------------------------------------------------------
@@ -454,6 +536,9 @@ jsa::RasterGrid batch_grid(const jaxsplat_batch* b) {
     g.n_items = b->n_items;
     g.fwd_depth = b->fwd_depth;
     g.bwd_depth = b->bwd_depth;
+    g.ch_stride = 3;
+    g.ch_off = 0;
+    g.ch_n = 3;
     return g;
 }
 

@@ -223,6 +223,9 @@ struct RasterGrid {
     int n_items;          // sorted entries name items in [0, n_items)
     int fwd_depth;        // Gaussians staged per batch: 64, 128 or 256 (0: workgroup size)
     int bwd_depth;        // likewise for the backward (0: 64)
+    // channel pass: channels [ch_off, ch_off + ch_n) (ch_n <= 3) of colors,
+    // backgrounds, images and v_colors stored with ch_stride channels per row
+    int ch_stride, ch_off, ch_n;
 };
 
 ViewRec single_view(const Camera& c);

@@ -62,6 +62,14 @@ __device__ __forceinline__ ViewRec resolve_view(const RasterGrid& g, int gt) {
     return g.views[lo];
 }
 
+// Up to three channels (rg.ch_off .. + rg.ch_n) of row g of a (rows, ch_stride)
+// array; absent channels read as 0.  The default pass is all of an (N, 3) array.
+__device__ __forceinline__ float4 load_color(const RasterGrid& rg, const float* __restrict__ a,
+                                             size_t g) {
+    const float* p = a + (size_t)rg.ch_stride * g + rg.ch_off;
+    return make_float4(p[0], rg.ch_n > 1 ? p[1] : 0.f, rg.ch_n > 2 ? p[2] : 0.f, 0.f);
+}
+
 // Pixel-centre box of the 64 consecutive tile pixels p0..p0+63 (row-major in
 // the bw x bw tile), clipped to the image; ok = false if none is inside.
 __device__ __forceinline__ float4 group_rect(int p0, int bw, int x_org, int y_org, int W, int H,
@@ -209,8 +217,7 @@ __global__ __launch_bounds__(256) void raster_fwd_kernel(
                             cc = conics[3 * it + 2];
                 s_a[e] = make_float4(xy.x, xy.y, o, __int_as_float(it));
                 s_b[e] = make_float4(ca, cb_, cc, 0.f);
-                s_c[e] = make_float4(colors[3 * g + 0], colors[3 * g + 1], colors[3 * g + 2],
-                                     0.f);
+                s_c[e] = load_color(rg, colors, g);
                 const CullInfo ci = cull_info(xy.x, xy.y, ca, cb_, cc, o);
                 for (int G = 0; G < nwaves; ++G)
                     if (s_rok[G] && group_hit(ci, s_rect[G])) bits |= 1u << G;
@@ -256,12 +263,15 @@ __global__ __launch_bounds__(256) void raster_fwd_kernel(
     }
     if (inside) {
         const size_t pix = (size_t)vr.pix_off + (size_t)i * vr.img_w + j;
-        const float* bg = bgs + 3 * vr.view;
-        final_Ts[pix] = T;
-        final_idx[pix] = cur;
-        out_img[3 * pix + 0] = cr + T * bg[0];
-        out_img[3 * pix + 1] = cg + T * bg[1];
-        out_img[3 * pix + 2] = cb + T * bg[2];
+        const float4 bg = load_color(rg, bgs, vr.view);
+        if (rg.ch_off == 0) {            // identical for every channel pass
+            final_Ts[pix] = T;
+            final_idx[pix] = cur;
+        }
+        float* o = out_img + (size_t)rg.ch_stride * pix + rg.ch_off;
+        o[0] = cr + T * bg.x;
+        if (rg.ch_n > 1) o[1] = cg + T * bg.y;
+        if (rg.ch_n > 2) o[2] = cb + T * bg.z;
     }
 }
 
@@ -377,8 +387,8 @@ __global__ __launch_bounds__(128) void raster_bwd_kernel(
     const int ngroups = 2 * nwaves;
     const int ty = tile / vr.tiles_x, tx = tile - ty * vr.tiles_x;
     const int x_org = tx * bw, y_org = ty * bw;
-    const float* bg = bgs + 3 * vr.view;
-    const float bgr = bg[0], bgg = bg[1], bgb = bg[2];
+    const float4 bg4 = load_color(rg, bgs, vr.view);
+    const float bgr = bg4.x, bgg = bg4.y, bgb = bg4.z;
 
     PixState ps[2];
     int my_fin = -1;
@@ -396,9 +406,10 @@ __global__ __launch_bounds__(128) void raster_bwd_kernel(
         s.T_final = s.inside ? final_Ts[pix] : 0.f;
         s.T = s.T_final;
         s.fin = s.inside ? final_idx[pix] : -1;
-        s.vo_r = s.inside ? v_out_img[3 * pix + 0] : 0.f;
-        s.vo_g = s.inside ? v_out_img[3 * pix + 1] : 0.f;
-        s.vo_b = s.inside ? v_out_img[3 * pix + 2] : 0.f;
+        const float4 vo = s.inside ? load_color(rg, v_out_img, pix) : make_float4(0, 0, 0, 0);
+        s.vo_r = vo.x;
+        s.vo_g = vo.y;
+        s.vo_b = vo.z;
         const float vo_a = (s.inside && v_out_alpha) ? v_out_alpha[pix] : 0.f;
         // background term of v_alpha: T_final * ra * (v_out_alpha - bg . v_out)
         s.bgdot = s.T_final * (vo_a - (bgr * s.vo_r + bgg * s.vo_g + bgb * s.vo_b));
@@ -440,8 +451,7 @@ __global__ __launch_bounds__(128) void raster_bwd_kernel(
                             cc = conics[3 * it + 2];
                 s_a[e] = make_float4(xy.x, xy.y, o, __int_as_float(it));
                 s_b[e] = make_float4(ca, cb_, cc, 0.f);
-                s_c[e] = make_float4(colors[3 * g + 0], colors[3 * g + 1], colors[3 * g + 2],
-                                     0.f);
+                s_c[e] = load_color(rg, colors, g);
                 const CullInfo ci = cull_info(xy.x, xy.y, ca, cb_, cc, o);
                 for (int G = 0; G < ngroups; ++G)
                     if (s_rok[G] && group_hit(ci, s_rect[G])) bits |= 1u << G;
@@ -505,9 +515,10 @@ __global__ __launch_bounds__(128) void raster_bwd_kernel(
             finish_grads<NV>(r, s_b[e], s_a[e].z);
             const int it = __float_as_int(s_a[e].w);
             const int gid = it + gshift;
-            unsafeAtomicAdd(v_rgb + 3 * gid + 0, r[0]);
-            unsafeAtomicAdd(v_rgb + 3 * gid + 1, r[1]);
-            unsafeAtomicAdd(v_rgb + 3 * gid + 2, r[2]);
+            float* vc = v_rgb + (size_t)rg.ch_stride * gid + rg.ch_off;
+            unsafeAtomicAdd(vc + 0, r[0]);
+            if (rg.ch_n > 1) unsafeAtomicAdd(vc + 1, r[1]);
+            if (rg.ch_n > 2) unsafeAtomicAdd(vc + 2, r[2]);
             unsafeAtomicAdd(v_conic + 3 * it + 0, r[3]);
             unsafeAtomicAdd(v_conic + 3 * it + 1, r[4]);
             unsafeAtomicAdd(v_conic + 3 * it + 2, r[5]);
@@ -538,17 +549,14 @@ __global__ void zero_many_kernel(ZeroList z) {
     }
 }
 
-__global__ void raster_empty_kernel(size_t n_pix, const float* __restrict__ bg,
+__global__ void raster_empty_kernel(size_t n_pix, int nch, const float* __restrict__ bg,
                                     float* __restrict__ final_Ts, int* __restrict__ final_idx,
                                     float* __restrict__ out_img) {
-    const float r = bg[0], g = bg[1], b = bg[2];
     for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < n_pix;
          p += (size_t)gridDim.x * blockDim.x) {
         final_Ts[p] = 0.f;
         final_idx[p] = 0;
-        out_img[3 * p + 0] = r;
-        out_img[3 * p + 1] = g;
-        out_img[3 * p + 2] = b;
+        for (int k = 0; k < nch; ++k) out_img[(size_t)nch * p + k] = bg[k];
     }
 }
 
@@ -582,15 +590,18 @@ RasterGrid single_grid(const Camera& c) {
     g.n_items = c.num_points;
     g.fwd_depth = 0;
     g.bwd_depth = 0;
+    g.ch_stride = 3;
+    g.ch_off = 0;
+    g.ch_n = 3;
     return g;
 }
 
 hipError_t raster_empty(const Camera& c, const float* background, float* final_Ts,
-                        int* final_idx, float* out_img, hipStream_t s) {
+                        int* final_idx, float* out_img, hipStream_t s, int nch) {
     const size_t n = (size_t)c.img_w * c.img_h;
     if (n == 0) return hipSuccess;
     const int blocks = (int)((n + 255) / 256 < 4096 ? (n + 255) / 256 : 4096);
-    raster_empty_kernel<<<blocks, 256, 0, s>>>(n, background, final_Ts, final_idx, out_img);
+    raster_empty_kernel<<<blocks, 256, 0, s>>>(n, nch, background, final_Ts, final_idx, out_img);
     return hipGetLastError();
 }
 

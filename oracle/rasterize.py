@@ -56,23 +56,27 @@ def _pair_terms(xy, conic, px, py):
 def rasterize_fwd(gids_sorted, tile_bins, xys, conics, colors, opacities,
                   background, *, img_shape, block_width, num_intersects=None,
                   tiles=None, chunk=2048):
-    """Vectorised forward.  Returns dict img (H,W,3), final_Ts (H,W),
-    final_idx (H,W) int32, ambiguous (H,W) bool.  ``tiles`` restricts the
-    work to a subset of tile ids (other pixels stay zero / unflagged)."""
+    """Vectorised forward.  Returns dict img (H,W,C), final_Ts (H,W),
+    final_idx (H,W) int32, ambiguous (H,W) bool, for colors (N,C) and
+    background (C,) (C = 3 in the reference API; any C for the N-channel
+    rasterizer, whose blend is the same per channel).  ``tiles`` restricts
+    the work to a subset of tile ids (other pixels stay zero / unflagged)."""
     H, W = img_shape
     bw = block_width
     tiles_x = (W + bw - 1) // bw
     tiles_y = (H + bw - 1) // bw
     xys = np.asarray(xys, np.float64).reshape(-1, 2)
     conics = np.asarray(conics, np.float64).reshape(-1, 3)
-    colors = np.asarray(colors, np.float64).reshape(-1, 3)
+    colors = np.asarray(colors, np.float64)
+    colors = colors.reshape(colors.shape[0], -1)
+    nch = colors.shape[1]
     opac = np.asarray(opacities, np.float64).reshape(-1)
-    bg = np.asarray(background, np.float64).reshape(3)
+    bg = np.asarray(background, np.float64).reshape(nch)
     gids_sorted = np.asarray(gids_sorted).reshape(-1)
     tile_bins = np.asarray(tile_bins).reshape(-1, 2)
     if num_intersects is None:
         num_intersects = gids_sorted.size
-    img = np.zeros((H, W, 3))
+    img = np.zeros((H, W, nch))
     final_T = np.zeros((H, W))
     final_idx = np.zeros((H, W), np.int32)
     amb = np.zeros((H, W), bool)
@@ -85,7 +89,7 @@ def rasterize_fwd(gids_sorted, tile_bins, xys, conics, colors, opacities,
         px, py = jj + 0.5, ii + 0.5
         P = ii.size
         T = np.ones(P)
-        C = np.zeros((P, 3))
+        C = np.zeros((P, nch))
         last = np.zeros(P, np.int64)
         done = np.zeros(P, bool)
         ab = np.zeros(P, bool)
@@ -139,8 +143,8 @@ def rasterize_bwd(gids_sorted, tile_bins, xys, conics, colors, opacities,
                   background, final_Ts, final_idx, v_img, v_img_alpha, *,
                   img_shape, block_width, mode="reference", num_intersects=None,
                   tiles=None, pix_chunk=64):
-    """Vectorised backward.  Returns dict v_colors (N,3), v_opacity (N,),
-    v_xy (N,2), v_xy_abs (N,2), v_conic (N,3)."""
+    """Vectorised backward.  Returns dict v_colors (N,C), v_opacity (N,),
+    v_xy (N,2), v_xy_abs (N,2), v_conic (N,3) for colors (N,C)."""
     if mode not in ("reference", "consistent"):
         raise ValueError(mode)
     clamp = 0.99 if mode == "reference" else 0.999
@@ -151,22 +155,24 @@ def rasterize_bwd(gids_sorted, tile_bins, xys, conics, colors, opacities,
     xys = np.asarray(xys, np.float64).reshape(-1, 2)
     N = xys.shape[0]
     conics = np.asarray(conics, np.float64).reshape(-1, 3)
-    colors = np.asarray(colors, np.float64).reshape(-1, 3)
+    colors = np.asarray(colors, np.float64)
+    colors = colors.reshape(colors.shape[0], -1)
+    nch = colors.shape[1]
     opac = np.asarray(opacities, np.float64).reshape(-1)
-    bg = np.asarray(background, np.float64).reshape(3)
+    bg = np.asarray(background, np.float64).reshape(nch)
     fT = np.asarray(final_Ts, np.float64).reshape(H, W)
     fI = np.asarray(final_idx, np.int64).reshape(H, W)
-    vimg = np.asarray(v_img, np.float64).reshape(H, W, 3)
+    vimg = np.asarray(v_img, np.float64).reshape(H, W, nch)
     valpha = np.asarray(v_img_alpha, np.float64).reshape(H, W)
     gids_sorted = np.asarray(gids_sorted).reshape(-1)
     tile_bins = np.asarray(tile_bins).reshape(-1, 2)
     if num_intersects is None:
         num_intersects = gids_sorted.size
-    out = dict(v_colors=np.zeros((N, 3)), v_opacity=np.zeros(N), v_xy=np.zeros((N, 2)),
+    out = dict(v_colors=np.zeros((N, nch)), v_opacity=np.zeros(N), v_xy=np.zeros((N, 2)),
                v_xy_abs=np.zeros((N, 2)), v_conic=np.zeros((N, 3)))
     if num_intersects == 0:                      # ops.cu:184-186
         return out
-    acc = np.zeros((N, 11))
+    acc = np.zeros((N, nch + 8))
     tile_list = range(tiles_x * tiles_y) if tiles is None else tiles
     for tile in tile_list:
         s, e = int(tile_bins[tile, 0]), int(tile_bins[tile, 1])
@@ -192,7 +198,7 @@ def rasterize_bwd(gids_sorted, tile_bins, xys, conics, colors, opacities,
             Prev = np.cumprod(om[:, ::-1], axis=1)[:, ::-1]     # prod_{j>=k}
             Tk = Tf[:, None] / Prev
             fac = np.where(m, alpha * Tk, 0.0)
-            contrib = rgb[None, :, :] * fac[:, :, None]           # (P,K,3)
+            contrib = rgb[None, :, :] * fac[:, :, None]           # (P,K,C)
             csum = np.cumsum(contrib[:, ::-1, :], axis=1)[:, ::-1, :]
             Sbuf = csum - contrib                                 # sum over j > k
             ra = 1.0 / (1.0 - alpha)
@@ -210,11 +216,11 @@ def rasterize_bwd(gids_sorted, tile_bins, xys, conics, colors, opacities,
             per = np.concatenate([vrgb, vcon, vxy, np.abs(vxy), vop[:, :, None]], -1)
             per = np.where(m[:, :, None], per, 0.0)
             np.add.at(acc, ids, per.sum(axis=0))
-    out["v_colors"] = acc[:, 0:3]
-    out["v_conic"] = acc[:, 3:6]
-    out["v_xy"] = acc[:, 6:8]
-    out["v_xy_abs"] = acc[:, 8:10]
-    out["v_opacity"] = acc[:, 10]
+    out["v_colors"] = acc[:, 0:nch]
+    out["v_conic"] = acc[:, nch:nch + 3]
+    out["v_xy"] = acc[:, nch + 3:nch + 5]
+    out["v_xy_abs"] = acc[:, nch + 5:nch + 7]
+    out["v_opacity"] = acc[:, nch + 7]
     return out
 
 
@@ -229,14 +235,15 @@ def rasterize_fwd_bruteforce(gids_sorted, tile_bins, xys, conics, colors,
     bw = block_width
     tiles_x = (W + bw - 1) // bw
     opacities = np.asarray(opacities, np.float64).reshape(-1)
-    img = np.zeros((H, W, 3))
+    nch = np.asarray(colors).reshape(len(opacities), -1).shape[1]
+    img = np.zeros((H, W, nch))
     fT = np.zeros((H, W))
     fI = np.zeros((H, W), np.int32)
     for i in range(H):
         for j in range(W):
             tile = (i // bw) * tiles_x + (j // bw)
             s, e = int(tile_bins[tile][0]), int(tile_bins[tile][1])
-            T, C, cur = 1.0, [0.0, 0.0, 0.0], 0
+            T, C, cur = 1.0, [0.0] * nch, 0
             for k in range(s, e):
                 g = int(gids_sorted[k])
                 dx = float(xys[g][0]) - (j + 0.5)
@@ -249,11 +256,11 @@ def rasterize_fwd_bruteforce(gids_sorted, tile_bins, xys, conics, colors,
                 nT = T * (1.0 - alpha)
                 if nT <= T_EPS:
                     break
-                for ch in range(3):
+                for ch in range(nch):
                     C[ch] += float(colors[g][ch]) * alpha * T
                 T = nT
                 cur = k
-            for ch in range(3):
+            for ch in range(nch):
                 img[i, j, ch] = C[ch] + T * float(background[ch])
             fT[i, j] = T
             fI[i, j] = cur
@@ -271,7 +278,8 @@ def rasterize_bwd_bruteforce(gids_sorted, tile_bins, xys, conics, colors,
     tiles_x = (W + bw - 1) // bw
     N = len(xys)
     opacities = np.asarray(opacities, np.float64).reshape(-1)
-    vc = np.zeros((N, 3))
+    nch = np.asarray(colors).reshape(N, -1).shape[1]
+    vc = np.zeros((N, nch))
     vo = np.zeros(N)
     vxy = np.zeros((N, 2))
     vxya = np.zeros((N, 2))
@@ -282,7 +290,7 @@ def rasterize_bwd_bruteforce(gids_sorted, tile_bins, xys, conics, colors,
             s, e = int(tile_bins[tile][0]), int(tile_bins[tile][1])
             Tf = float(final_Ts[i][j])
             T = Tf
-            buf = [0.0, 0.0, 0.0]
+            buf = [0.0] * nch
             fin = int(final_idx[i][j])
             v = [float(x) for x in v_img[i][j]]
             va_out = float(v_img_alpha[i][j])
@@ -304,13 +312,13 @@ def rasterize_bwd_bruteforce(gids_sorted, tile_bins, xys, conics, colors,
                 fac = alpha * T
                 rgb = [float(x) for x in colors[g]]
                 v_alpha = 0.0
-                for ch in range(3):
+                for ch in range(nch):
                     vc[g, ch] += fac * v[ch]
                     v_alpha += (rgb[ch] * T - buf[ch] * ra) * v[ch]
                 v_alpha += Tf * ra * va_out
-                for ch in range(3):
+                for ch in range(nch):
                     v_alpha += -Tf * ra * float(background[ch]) * v[ch]
-                for ch in range(3):
+                for ch in range(nch):
                     buf[ch] += rgb[ch] * fac
                 slope = 1.0 if (mode == "reference" or raw < clamp) else 0.0
                 v_sigma = -float(opacities[g]) * vis * v_alpha * slope

@@ -107,6 +107,62 @@ def test_raster_bwd_matches_bruteforce(mode):
         np.testing.assert_allclose(bv[k], bb[k], rtol=1e-10, atol=1e-12)
 
 
+@pytest.mark.parametrize("nch", [1, 5])
+@pytest.mark.parametrize("mode", ["reference", "consistent"])
+def test_nchannel_matches_bruteforce(nch, mode):
+    """N-channel blend: vectorised forward and backward vs the sequential loops."""
+    sc, kw, pr, ks, gs, bins, op = _tiny(scale=12.0, opac=lambda o: np.minimum(o * 1.9, 0.999))
+    rng = np.random.default_rng(nch)
+    col = rng.uniform(size=(sc.n, nch))
+    bg = rng.uniform(size=nch)
+    fv = oracle.rasterize.rasterize_fwd(gs, bins, pr["xys"], pr["conics"], col, op, bg,
+                                        img_shape=(40, 48), block_width=8, chunk=7)
+    fb = oracle.rasterize.rasterize_fwd_bruteforce(gs, bins, pr["xys"], pr["conics"], col, op, bg,
+                                                   img_shape=(40, 48), block_width=8)
+    assert fv["img"].shape == (40, 48, nch)
+    np.testing.assert_allclose(fv["img"], fb["img"], atol=1e-13)
+    vi, va = rng.normal(size=(40, 48, nch)), rng.normal(size=(40, 48))
+    bv = oracle.rasterize.rasterize_bwd(gs, bins, pr["xys"], pr["conics"], col, op, bg,
+                                        fb["final_Ts"], fb["final_idx"], vi, va, img_shape=(40, 48),
+                                        block_width=8, mode=mode, pix_chunk=5)
+    bb = oracle.rasterize.rasterize_bwd_bruteforce(gs, bins, pr["xys"], pr["conics"], col, op, bg,
+                                                   fb["final_Ts"], fb["final_idx"], vi, va,
+                                                   img_shape=(40, 48), block_width=8, mode=mode)
+    for k in bv:
+        np.testing.assert_allclose(bv[k], bb[k], rtol=1e-10, atol=1e-12)
+
+
+def test_nchannel_separable_and_alpha_closed_form():
+    """Colours do not take part in the blend's decisions, so a 5-channel render
+    equals its 2 + 3 channel split, and the geometry gradients add up over the
+    split (the alpha cotangent counted once).  One channel of colour 1 on a
+    zero background renders the accumulated alpha 1 - T."""
+    sc, kw, pr, ks, gs, bins, op = _tiny(scale=12.0, opac=lambda o: np.minimum(o * 1.9, 0.999))
+    rng = np.random.default_rng(7)
+    col = rng.uniform(size=(sc.n, 5))
+    bg = rng.uniform(size=5)
+    args = (gs, bins, pr["xys"], pr["conics"])
+    kw2 = dict(img_shape=(40, 48), block_width=8)
+    f5 = oracle.rasterize.rasterize_fwd(*args, col, op, bg, **kw2)
+    fa = oracle.rasterize.rasterize_fwd(*args, col[:, :2], op, bg[:2], **kw2)
+    fb = oracle.rasterize.rasterize_fwd(*args, col[:, 2:], op, bg[2:], **kw2)
+    np.testing.assert_allclose(f5["img"], np.concatenate([fa["img"], fb["img"]], -1), atol=1e-14)
+    vi, va = rng.normal(size=(40, 48, 5)), rng.normal(size=(40, 48))
+    g5 = oracle.rasterize.rasterize_bwd(*args, col, op, bg, f5["final_Ts"], f5["final_idx"], vi,
+                                        va, mode="consistent", **kw2)
+    ga = oracle.rasterize.rasterize_bwd(*args, col[:, :2], op, bg[:2], f5["final_Ts"],
+                                        f5["final_idx"], vi[..., :2], va, mode="consistent", **kw2)
+    gb = oracle.rasterize.rasterize_bwd(*args, col[:, 2:], op, bg[2:], f5["final_Ts"],
+                                        f5["final_idx"], vi[..., 2:], np.zeros_like(va),
+                                        mode="consistent", **kw2)
+    np.testing.assert_allclose(g5["v_colors"], np.concatenate([ga["v_colors"], gb["v_colors"]], 1),
+                               rtol=1e-12, atol=1e-14)
+    for k in ("v_opacity", "v_xy", "v_conic"):
+        np.testing.assert_allclose(g5[k], ga[k] + gb[k], rtol=1e-9, atol=1e-12)
+    f1 = oracle.rasterize.rasterize_fwd(*args, np.ones((sc.n, 1)), op, np.zeros(1), **kw2)
+    np.testing.assert_allclose(f1["img"][..., 0], 1.0 - f1["final_Ts"], atol=1e-14)
+
+
 def test_single_gaussian_closed_form():
     """One Gaussian: pixel = c*alpha + bg*(1-alpha) where alpha >= 1/255, else bg."""
     xys = np.array([[10.3, 7.7]])
