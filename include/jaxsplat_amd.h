@@ -148,8 +148,9 @@ int jaxsplat_bin_and_sort(const jaxsplat_descriptor* d, const float* xys, const 
                           void* stream);
 
 /*
- * Synchronise `stream` and report a look-back timeout recorded in the
- * workspace of the last jaxsplat_bin_and_sort (JAXSPLAT_ERR_CAPACITY).
+ * Synchronise `stream` and report an error recorded in the workspace of the
+ * last jaxsplat_bin_and_sort (JAXSPLAT_ERR_CAPACITY): more intersections than
+ * num_intersects, or a tile key out of range.
  */
 int jaxsplat_bin_check(const jaxsplat_descriptor* d, uint32_t num_intersects,
                        const void* workspace, void* stream);

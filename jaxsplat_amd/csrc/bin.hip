@@ -14,11 +14,12 @@
 //     Gaussians with radius 0, which emit nothing);
 //  2. a stable LSD radix sort of the N Gaussians by depth key (values = index),
 //     giving the (depth bits, gaussian id) order;
-//  3. emit: a single-pass kernel walks the Gaussians in that order, scans
-//     their tile counts with a decoupled look-back and writes
-//     (tile id, gaussian id) pairs; each wave spreads its slots evenly over
-//     its lanes (owner lookup by an LDS mark and a DPP max-scan), so a
-//     Gaussian covering hundreds of tiles does not serialise one lane;
+//  3. emission in that order: a reduce-then-scan of the Gaussians' tile
+//     counts gives each Gaussian its first output slot, then every wave
+//     writes whole 64-slot chunks of (tile id, gaussian id) pairs (owner
+//     lookup by an LDS mark and a DPP max-scan), so neither a Gaussian
+//     covering hundreds of tiles nor a run of large near Gaussians serialises
+//     a lane or a wave;
 //  4. a stable LSD radix sort of the pairs on the tile id bits only
 //     (2 passes at 1080p/16x16 instead of 6 passes over 64-bit keys);
 //  5. tile_bins_kernel: [start, end) edges from the sorted tile ids, empty
@@ -30,13 +31,6 @@
 namespace jsa {
 namespace {
 
-__global__ void depth_keys_kernel(int n, const float* __restrict__ depths,
-                                  const int* __restrict__ radii, uint32_t* __restrict__ keys) {
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= n) return;
-    keys[g] = radii[g] > 0 ? __float_as_uint(depths[g]) : 0xFFFFFFFFu;
-}
-
 // forward.cu:152-161 / helpers.h:9-32, recomputed from xys and the integer radius.
 __device__ __forceinline__ void tile_rect(int block_width, int tiles_x, int tiles_y, float2 xy,
                                           int rad, int& x0, int& x1, int& y0, int& y1) {
@@ -46,6 +40,23 @@ __device__ __forceinline__ void tile_rect(int block_width, int tiles_x, int tile
     x1 = min(max(0, (int)(tcx + tr + 1.f)), tiles_x);
     y0 = min(max(0, (int)(tcy - tr)), tiles_y);
     y1 = min(max(0, (int)(tcy + tr + 1.f)), tiles_y);
+}
+
+// Per Gaussian (id order): the depth sort key and the packed tile rectangle.
+__global__ void depth_keys_kernel(Camera c, const float* __restrict__ depths,
+                                  const int* __restrict__ radii, const float2* __restrict__ xys,
+                                  uint32_t* __restrict__ keys, uint2* __restrict__ rects) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= c.num_points) return;
+    const int rad = radii[g];
+    uint2 rc = make_uint2(0u, 0u);
+    if (rad > 0) {
+        int x0, x1, y0, y1;
+        tile_rect(c.block_width, c.tiles_x, c.tiles_y, xys[g], rad, x0, x1, y0, y1);
+        rc = pack_rect(x0, x1, y0, y1);
+    }
+    keys[g] = rad > 0 ? __float_as_uint(depths[g]) : 0xFFFFFFFFu;
+    rects[g] = rc;
 }
 
 // View of each depth-sorted item (largest v with item_off[v] <= item), the
@@ -70,143 +81,275 @@ __global__ __launch_bounds__(256) void view_keys_kernel(const uint32_t* __restri
     vkeys[r] = (uint32_t)lo;
 }
 
-constexpr int kEmitPer = 4;                  // Gaussians per lane
-constexpr int kEmitPart = 256 * kEmitPer;    // Gaussians per scan partition
+// ---------------------------------------------------------------------------
+// Emission: reduce-then-scan over the depth order, then slot-balanced expansion
+// ---------------------------------------------------------------------------
+// Depth order puts the large, near Gaussians together, so splitting the work
+// by Gaussians leaves a few waves with several times the average number of
+// slots (at 1080p with 1M Gaussians the busiest wave expanded 231 chunks of 64
+// slots against a mean of 35).  Here every wave of the expansion owns whole
+// 64-slot chunks of the output instead:
+//  1. emit_count_kernel: each depth-ordered entry's tile rectangle, packed
+//     (x0 | y0 << 16, w | h << 16), and per-partition totals of slots and of
+//     entries that emit (w * h > 0);
+//  2. emit_scan_kernel (one workgroup): partition offsets of both, the total,
+//     the number of slots written, min(total, capacity), and of emitting
+//     entries;
+//  3. emit_compact_kernel: the emitting entries, compacted in order, with
+//     their first slot (clamped to the slots written), rectangle, id and (for
+//     a batch) key base and tiles_x; and for every 64-slot chunk the compacted
+//     entry that owns its first slot.  Compaction matters for a batch, whose
+//     order (view, depth) leaves a run of culled items at the end of every
+//     view;
+//  4. emit_expand_kernel: every wave writes an equal, contiguous range of
+//     chunks, sliding a window of 64 compacted entries from the owner of its
+//     first slot; an LDS mark per entry start and a DPP max-scan give every
+//     slot of a chunk its entry, and the 64 slots are stored contiguously.
+// BATCH: the entries are items of a batch's views (vkeys gives each entry's
+// view) and the key written is the view's bin_off + tile.
+constexpr int kEmitPer = 4;                  // entries per lane in passes 1 and 3
+constexpr int kEmitPart = 256 * kEmitPer;    // entries per scan partition
+constexpr int kExpandWaves = 8192;           // waves of the expansion (a full chip)
 
-// Emission: the workgroup owns 1024 consecutive Gaussians of the depth order
-// (4 groups of 64 per wave).  A single-pass decoupled look-back scan gives the
-// workgroup's first output slot; each wave then expands its groups 64 slots
-// at a time: the lane whose tile run starts inside the chunk marks that
-// position in a per-wave LDS row, an inclusive max-scan (DPP) gives every
-// slot its owning lane, and the owner's (id, rectangle, offset) is read from
-// LDS.  Stores of one chunk are 64 consecutive slots.
-// BATCH: the sorted entries are items of a batch's views (vkeys gives each
-// entry's view); the key written is the view's bin_off + tile, and the last
-// partition publishes the total (clamped count for the sorts, exact total).
-template <bool BATCH>
-__global__ __launch_bounds__(256) void emit_kernel(
-    Camera c, const ViewRec* __restrict__ views, const uint32_t* __restrict__ vkeys,
-    const uint32_t* __restrict__ perm, const float2* __restrict__ xys,
-    const int* __restrict__ radii, int isect_cap, uint32_t* __restrict__ tile_out,
-    uint32_t* __restrict__ gid_out, uint64_t* __restrict__ status, uint32_t* __restrict__ counter,
-    uint32_t* __restrict__ err, int* __restrict__ count_out,
-    unsigned long long* __restrict__ total_out) {
-    __shared__ uint32_t s_part;
-    __shared__ uint32_t s_wave_tot[kEmitPer][4];
-    __shared__ uint64_t s_excl;
-    __shared__ int s_mark[4][64];
-    __shared__ int4 s_own[4][64];        // (gid, x0 | y0 << 16, width, group offset)
-    __shared__ int2 s_key[4][64];        // (key base, tiles_x) of the owner's view
+// Pass 1 over the entries perm[0..n): gathers each entry's packed rectangle
+// (one 8-byte read at a random id) into emission order and sums slots and
+// emitting entries per partition.  The lane's loads are issued stage by stage
+// so that they overlap.
+__global__ __launch_bounds__(256) void emit_count_kernel(
+    int n, const uint32_t* __restrict__ perm, const uint2* __restrict__ rect_by_id,
+    uint2* __restrict__ rects, uint32_t* __restrict__ part_tot, uint32_t* __restrict__ part_nz) {
+    __shared__ uint32_t s_w[4], s_z[4];
+    const int tid = threadIdx.x;
+    const int lane = lane_id();
+    const int r0 = blockIdx.x * kEmitPart + tid;
+    uint32_t g[kEmitPer];
+    uint2 rc[kEmitPer];
+#pragma unroll
+    for (int k = 0; k < kEmitPer; ++k) g[k] = r0 + k * 256 < n ? perm[r0 + k * 256] : 0xFFFFFFFFu;
+#pragma unroll
+    for (int k = 0; k < kEmitPer; ++k)
+        rc[k] = g[k] != 0xFFFFFFFFu ? rect_by_id[g[k]] : make_uint2(0u, 0u);
+    uint32_t sum = 0, nz = 0;
+#pragma unroll
+    for (int k = 0; k < kEmitPer; ++k) {
+        if (r0 + k * 256 < n) rects[r0 + k * 256] = rc[k];
+        const uint32_t cnt = rect_count(rc[k]);
+        sum += cnt;
+        nz += cnt > 0 ? 1u : 0u;
+    }
+    const uint32_t ws = wave_inclusive_sum(sum);
+    const uint32_t wz = wave_inclusive_sum(nz);
+    if (lane == 63) {
+        s_w[tid >> 6] = ws;
+        s_z[tid >> 6] = wz;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        part_tot[blockIdx.x] = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+        part_nz[blockIdx.x] = s_z[0] + s_z[1] + s_z[2] + s_z[3];
+    }
+}
+
+// One workgroup of 1024 threads; thread t scans a contiguous run of partitions.
+// info[0] = slots written (min(total, cap)), info[1] = emitting entries.
+__global__ __launch_bounds__(1024) void emit_scan_kernel(
+    const uint32_t* __restrict__ part_tot, const uint32_t* __restrict__ part_nz, int nparts,
+    uint64_t* __restrict__ part_off, uint32_t* __restrict__ part_nzoff, int cap,
+    int* __restrict__ info, unsigned long long* __restrict__ total_out,
+    uint32_t* __restrict__ err) {
+    __shared__ uint64_t s_w[16];
+    __shared__ uint32_t s_z[16];
     const int tid = threadIdx.x;
     const int lane = lane_id();
     const int wave = tid >> 6;
-    if (tid == 0) s_part = atomicAdd(counter, 1u);
-    __syncthreads();
-    const int part = (int)s_part;
-
-    // element e = k*256 + tid of the partition (depth rank part*kEmitPart + e)
-    int cnt[kEmitPer], incl[kEmitPer], rx0[kEmitPer], ry0[kEmitPer], rw[kEmitPer];
-    int kbase[kEmitPer], ktx[kEmitPer];
-    uint32_t gg[kEmitPer];
-#pragma unroll
-    for (int k = 0; k < kEmitPer; ++k) {
-        const int r = part * kEmitPart + k * 256 + tid;
-        int count = 0, x0 = 0, y0 = 0, w = 1, base = 0, tiles_x = c.tiles_x;
-        uint32_t g = 0;
-        if (r < c.num_points) {
-            g = perm[r];
-            const int rad = radii[g];
-            if (rad > 0) {
-                int tiles_y = c.tiles_y;
-                if (BATCH) {
-                    const ViewRec& vr = views[vkeys[r]];
-                    tiles_x = vr.tiles_x;
-                    tiles_y = vr.tiles_y;
-                    base = vr.bin_off;
-                }
-                int x1, y1;
-                tile_rect(c.block_width, tiles_x, tiles_y, xys[g], rad, x0, x1, y0, y1);
-                w = max(x1 - x0, 1);
-                count = (x1 - x0) * (y1 - y0);
-            }
-        }
-        cnt[k] = count;
-        rx0[k] = x0;
-        ry0[k] = y0;
-        rw[k] = w;
-        gg[k] = g;
-        kbase[k] = base;
-        ktx[k] = tiles_x;
-        incl[k] = wave_inclusive_sum(count);
-        if (lane == 63) s_wave_tot[k][wave] = (uint32_t)incl[k];
+    const int per = (nparts + 1023) / 1024;
+    const int b0 = min(tid * per, nparts), b1 = min(b0 + per, nparts);
+    uint64_t sum = 0;
+    uint32_t nz = 0;
+    for (int i = b0; i < b1; ++i) {
+        sum += part_tot[i];
+        nz += part_nz[i];
+    }
+    const uint64_t incl = wave_inclusive_sum(sum);
+    const uint32_t zincl = wave_inclusive_sum(nz);
+    if (lane == 63) {
+        s_w[wave] = incl;
+        s_z[wave] = zincl;
     }
     __syncthreads();
-    uint32_t total = 0, grp_off[kEmitPer];
+    uint64_t run = incl - sum;
+    uint32_t zrun = zincl - nz;
+    for (int w = 0; w < wave; ++w) {
+        run += s_w[w];
+        zrun += s_z[w];
+    }
+    for (int i = b0; i < b1; ++i) {
+        part_off[i] = run;
+        part_nzoff[i] = zrun;
+        run += part_tot[i];
+        zrun += part_nz[i];
+    }
+    if (tid == 1023) {                   // run and zrun are now the grand totals
+        if (total_out) *total_out = (unsigned long long)run;
+        info[0] = run > (uint64_t)cap ? cap : (int)run;
+        info[1] = (int)zrun;
+        if (run > (uint64_t)cap) atomicOr(err, 8u);
+    }
+}
+
+// Compacted emitting entries (see the overview above).
+struct EmitEntries {
+    uint32_t* off;      // first slot, clamped to the slots written
+    uint2* rect;        // packed rectangle
+    uint32_t* gid;      // gaussian / item id
+    int2* key;          // BATCH: (bin_off, tiles_x) of the entry's view
+    uint32_t* chunk_owner;
+};
+
+template <bool BATCH>
+__global__ __launch_bounds__(256) void emit_compact_kernel(
+    int n, const ViewRec* __restrict__ views, const uint32_t* __restrict__ vkeys,
+    const uint32_t* __restrict__ perm, const uint2* __restrict__ rects,
+    const uint64_t* __restrict__ part_off, const uint32_t* __restrict__ part_nzoff,
+    const int* __restrict__ info, EmitEntries E) {
+    __shared__ uint32_t s_w[kEmitPer][4], s_z[kEmitPer][4];
+    const int tid = threadIdx.x;
+    const int lane = lane_id();
+    const int wave = tid >> 6;
+    const uint64_t below = (1ull << lane) - 1ull;
+    uint2 rc[kEmitPer];
+    uint32_t cnt[kEmitPer], incl[kEmitPer];
+    uint64_t nzb[kEmitPer];
 #pragma unroll
     for (int k = 0; k < kEmitPer; ++k) {
-        uint32_t w_off = 0, slab = 0;
+        const int r = blockIdx.x * kEmitPart + k * 256 + tid;
+        rc[k] = r < n ? rects[r] : make_uint2(0u, 0u);
+        cnt[k] = rect_count(rc[k]);
+        incl[k] = wave_inclusive_sum(cnt[k]);
+        nzb[k] = ballot(cnt[k] > 0);
+        if (lane == 63) {
+            s_w[k][wave] = incl[k];
+            s_z[k][wave] = (uint32_t)__popcll(nzb[k]);
+        }
+    }
+    __syncthreads();
+    const uint64_t limit = (uint64_t)info[0];
+    uint64_t base = part_off[blockIdx.x];
+    uint32_t zbase = part_nzoff[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < kEmitPer; ++k) {
+        uint64_t w_off = 0, slab = 0;
+        uint32_t z_off = 0, zslab = 0;
 #pragma unroll
         for (int w = 0; w < 4; ++w) {
-            const uint32_t v = s_wave_tot[k][w];
+            const uint32_t v = s_w[k][w], z = s_z[k][w];
             w_off += (w < wave) ? v : 0u;
+            z_off += (w < wave) ? z : 0u;
             slab += v;
+            zslab += z;
         }
-        grp_off[k] = total + w_off;
-        total += slab;
-    }
-    if (tid == 0) {
-        store_agent(status + part, (part == 0 ? kFlagPre : kFlagAgg) | (uint64_t)total);
-        if (part == 0) s_excl = 0;
-    }
-    if (wave == 0 && part > 0) {
-        const uint64_t excl = lookback64(status, part, err);
-        if (lane == 0) {
-            store_agent(status + part, kFlagPre | (excl + total));
-            s_excl = excl;
+        const int r = blockIdx.x * kEmitPart + k * 256 + tid;
+        const uint64_t off = base + w_off + (incl[k] - cnt[k]);
+        const uint32_t j = zbase + z_off + (uint32_t)__popcll(nzb[k] & below);
+        base += slab;
+        zbase += zslab;
+        if (cnt[k] == 0) continue;          // also every r >= n
+        E.off[j] = (uint32_t)(off < limit ? off : limit);
+        E.rect[j] = rc[k];
+        E.gid[j] = perm[r];
+        if (BATCH) {
+            const ViewRec& vr = views[vkeys[r]];
+            E.key[j] = make_int2(vr.bin_off, vr.tiles_x);
+        }
+        if (off < limit) {
+            // chunks whose first slot lies in [off, min(off + cnt, limit))
+            const uint64_t end = off + cnt[k] < limit ? off + cnt[k] : limit;
+            for (uint64_t ch = (off + 63) >> 6; (ch << 6) < end; ++ch)
+                E.chunk_owner[ch] = j;
         }
     }
-    __syncthreads();
-    const uint64_t gbase = s_excl;
-    if (BATCH && tid == 0 && part == (int)gridDim.x - 1) {
-        const uint64_t all = gbase + total;
-        *total_out = (unsigned long long)all;
-        *count_out = all > (uint64_t)isect_cap ? isect_cap : (int)all;
-        if (all > (uint64_t)isect_cap) atomicOr(err, 8u);
-    }
+}
 
-#pragma unroll
-    for (int k = 0; k < kEmitPer; ++k) {
-        const int gtotal = uniform_i(__shfl(incl[k], 63));
-        const int ex = incl[k] - cnt[k];
-        s_own[wave][lane] = make_int4((int)gg[k], rx0[k] | (ry0[k] << 16), rw[k], ex);
-        if (BATCH) s_key[wave][lane] = make_int2(kbase[k], ktx[k]);
-        wave_lds_sync();
-        const uint64_t obase = gbase + uniform_i((int)grp_off[k]);
-        int carry = 0;
-        for (int chunk = 0; chunk < gtotal; chunk += 64) {
-            s_mark[wave][lane] = 0;
-            wave_lds_sync();
-            if (cnt[k] > 0 && ex >= chunk && ex < chunk + 64) s_mark[wave][ex - chunk] = lane + 1;
-            wave_lds_sync();
-            int v = wave_max_scan_nonneg(s_mark[wave][lane]);
-            const int owner = v > 0 ? v - 1 : carry;
-            carry = __builtin_amdgcn_readlane(owner, 63);
-            const int slot = chunk + lane;
-            if (slot < gtotal) {
-                const int4 od = s_own[wave][owner];
-                const int2 kb = BATCH ? s_key[wave][owner] : make_int2(0, c.tiles_x);
-                const int kk = slot - od.w;
-                const int q = kk / od.z;
-                const int ty = (od.y >> 16) + q;
-                const int tx = (od.y & 0xFFFF) + (kk - q * od.z);
-                const uint64_t pos = obase + (uint64_t)slot;
-                if (pos < (uint64_t)isect_cap) {
-                    tile_out[pos] = (uint32_t)(kb.x + ty * kb.y + tx);
-                    gid_out[pos] = (uint32_t)od.x;
-                }
-            }
-            wave_lds_sync();
+// Wave gw expands the chunks [c0, c0 + per) of the output, an equal share of
+// the slots.  It slides a window of 64 consecutive compacted entries (one per
+// lane, loaded once and staged in LDS) from the owner of its first slot on;
+// inside a window each 64-slot chunk gets its owners from an LDS mark per
+// entry start and a DPP max-scan, slots before the chunk's first mark
+// belonging to the owner of the previous chunk's last slot.  A mark is
+// (step << 7) | (lane + 1) with a step number that grows with every chunk the
+// wave processes, so older marks never win the max-scan and the mark row is
+// cleared only once.  Every entry emits at least one slot, so a window covers
+// at least 64 slots.
+template <bool BATCH>
+__global__ __launch_bounds__(256) void emit_expand_kernel(
+    int tiles_x_plain, const EmitEntries E, const int* __restrict__ info,
+    uint32_t* __restrict__ tile_out, uint32_t* __restrict__ gid_out) {
+    __shared__ int s_mark[4][64];
+    __shared__ uint4 s_ent[4][64];      // (gid, x0 | y0 << 16, w, first slot)
+    __shared__ int2 s_key[4][64];       // BATCH: (bin_off, tiles_x)
+    const int lane = lane_id();
+    const int wave = threadIdx.x >> 6;
+    const uint32_t limit = (uint32_t)info[0];
+    const uint32_t nnz = (uint32_t)info[1];
+    const uint32_t nchunks = (limit + 63u) >> 6;
+    const uint32_t nw = gridDim.x * 4u;
+    const uint32_t per = (nchunks + nw - 1u) / nw;
+    const uint32_t c0 = (blockIdx.x * 4u + (uint32_t)wave) * per;
+    if (c0 >= nchunks) return;
+    const uint32_t end = min((c0 + per) << 6, limit);   // limit < 2^31
+    uint32_t pos = c0 << 6;                               // first slot not yet written
+    uint32_t cur = (uint32_t)uniform_i((int)E.chunk_owner[c0]);   // owns slot pos
+    int step = 0;       // < 2^24: at most per + windows steps, per <= 2^25 / 64
+    s_mark[wave][lane] = 0;
+    while (pos < end) {
+        const uint32_t j = cur + (uint32_t)lane;
+        uint32_t off = limit, gid = 0u;
+        uint2 rc = make_uint2(0u, 0u);
+        if (j < nnz) {
+            off = E.off[j];
+            rc = E.rect[j];
+            gid = E.gid[j];
+            if (BATCH) s_key[wave][lane] = E.key[j];
         }
+        const uint32_t cnt = rect_count(rc);
+        s_ent[wave][lane] = make_uint4(gid, rc.x, rc.y & 0xFFFFu, off);
+        // the window's slots end where its last entry's run ends
+        const uint32_t run_end = cnt > limit - off ? limit : off + cnt;
+        const uint32_t seg = min((uint32_t)__builtin_amdgcn_readlane((int)run_end, 63), end);
+        int carry = 0;      // lane 0 owns pos in the first window; later windows start at pos
+        for (uint32_t s0 = pos & ~63u; s0 < seg; s0 += 64u) {
+            ++step;
+            const int tag = step << 7;
+            if (cnt > 0 && off >= s0 && off < s0 + 64u && off < seg)
+                s_mark[wave][off - s0] = tag | (lane + 1);
+            wave_lds_sync();
+            const int v = wave_max_scan_nonneg(s_mark[wave][lane]);
+            const int owner = v >= tag ? (v & 127) - 1 : carry;
+            carry = __builtin_amdgcn_readlane(owner, 63);
+            const uint4 od = s_ent[wave][owner];
+            const uint32_t slot = s0 + (uint32_t)lane;
+            if (slot >= pos && slot < seg) {
+                // q = kk / w from a float estimate (|error| < 1 for q < 2^16) and one correction
+                const uint32_t kk = slot - od.w;
+                uint32_t q = (uint32_t)((float)kk * __builtin_amdgcn_rcpf((float)od.z));
+                int rem = (int)(kk - q * od.z);
+                if (rem < 0) {
+                    --q;
+                    rem += (int)od.z;
+                } else if (rem >= (int)od.z) {
+                    ++q;
+                    rem -= (int)od.z;
+                }
+                const uint32_t ty = (od.y >> 16) + q;
+                const uint32_t tx = (od.y & 0xFFFFu) + (uint32_t)rem;
+                const int2 kb = BATCH ? s_key[wave][owner] : make_int2(0, tiles_x_plain);
+                tile_out[slot] = (uint32_t)kb.x + ty * (uint32_t)kb.y + tx;
+                gid_out[slot] = od.x;
+            }
+        }
+        pos = seg > pos ? seg : pos;
+        cur += 64u;
+        wave_lds_sync();    // the next window overwrites s_ent / s_key
     }
 }
 
@@ -235,26 +378,91 @@ __global__ void tile_bins_kernel(const uint32_t* __restrict__ tiles, int n_host,
 
 constexpr size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
+// Emission buffers (passes 1-4 above) for n entries and a capacity of cap slots.
+struct EmitLayout {
+    size_t part_tot, part_nz, part_off, part_nzoff, info, rects, off, rect, gid, key, chunk_owner;
+};
+
+size_t emit_layout(size_t o, int n, int cap, bool batch, EmitLayout& E) {
+    const size_t parts = (size_t)((n + kEmitPart - 1) / kEmitPart) + 1;
+    const size_t nn = (size_t)(n > 0 ? n : 1);
+    const size_t chunks = (size_t)(cap > 0 ? cap : 1) / 64 + 1;
+    E.part_tot = o; o = align256(o + 4 * parts);
+    E.part_nz = o; o = align256(o + 4 * parts);
+    E.part_off = o; o = align256(o + 8 * parts);
+    E.part_nzoff = o; o = align256(o + 4 * parts);
+    E.info = o; o = align256(o + 16);
+    E.rects = o; o = align256(o + 8 * nn);
+    E.off = o; o = align256(o + 4 * nn);
+    E.rect = o; o = align256(o + 8 * nn);
+    E.gid = o; o = align256(o + 4 * nn);
+    E.key = o; o = align256(o + (batch ? 8 * nn : 0));
+    E.chunk_owner = o; o = align256(o + 4 * chunks);
+    return o;
+}
+
+// Passes 1-4 over the n = c.num_points entries perm[0..n) in emission order.
+// Writes the slots written (min(total, cap)) to info[0] and the number of
+// emitting entries to info[1] (info: the layout's slot if null), the exact
+// total to *total if total is not null, and bit 8 of err when the total
+// exceeds cap.
+template <bool BATCH>
+hipError_t emit(int n, int tiles_x_plain, const ViewRec* views, const uint32_t* vkeys,
+                const uint32_t* perm, const uint2* rect_by_id, int cap, uint8_t* b,
+                const EmitLayout& L, int* info, unsigned long long* total, uint32_t* err,
+                uint32_t* tile_out, uint32_t* gid_out, hipStream_t s) {
+    const int parts = (n + kEmitPart - 1) / kEmitPart;
+    auto P = [&](size_t off) { return b + off; };
+    uint32_t* part_tot = reinterpret_cast<uint32_t*>(P(L.part_tot));
+    uint32_t* part_nz = reinterpret_cast<uint32_t*>(P(L.part_nz));
+    uint64_t* part_off = reinterpret_cast<uint64_t*>(P(L.part_off));
+    uint32_t* part_nzoff = reinterpret_cast<uint32_t*>(P(L.part_nzoff));
+    if (!info) info = reinterpret_cast<int*>(P(L.info));
+    uint2* rects = reinterpret_cast<uint2*>(P(L.rects));
+    EmitEntries E;
+    E.off = reinterpret_cast<uint32_t*>(P(L.off));
+    E.rect = reinterpret_cast<uint2*>(P(L.rect));
+    E.gid = reinterpret_cast<uint32_t*>(P(L.gid));
+    E.key = BATCH ? reinterpret_cast<int2*>(P(L.key)) : nullptr;
+    E.chunk_owner = reinterpret_cast<uint32_t*>(P(L.chunk_owner));
+    emit_count_kernel<<<parts, 256, 0, s>>>(n, perm, rect_by_id, rects, part_tot, part_nz);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    emit_scan_kernel<<<1, 1024, 0, s>>>(part_tot, part_nz, parts, part_off, part_nzoff, cap,
+                                        info, total, err);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    emit_compact_kernel<BATCH><<<parts, 256, 0, s>>>(n, views, vkeys, perm, rects, part_off,
+                                                     part_nzoff, info, E);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    const long long chunks = ((long long)cap + 63) / 64;
+    const int waves = (int)(chunks < kExpandWaves ? chunks : kExpandWaves);
+    emit_expand_kernel<BATCH><<<(waves + 3) / 4, 256, 0, s>>>(tiles_x_plain, E, info, tile_out,
+                                                              gid_out);
+    return hipGetLastError();
+}
+
 struct BinLayout {
     // zeroed region
-    size_t err, emit_counter, emit_status, zero_end;
+    size_t err, zero_end;
     // plain region
-    size_t depth_sort, tile_sort, dkeys, dtmp_k, dtmp_v, dsorted_k, perm, tiles, gids, ttmp_k,
-        ttmp_v, tiles_sorted, end;
+    size_t depth_sort, tile_sort, dkeys, rect_by_id, dtmp_k, dtmp_v, dsorted_k, perm, tiles,
+        gids, ttmp_k, ttmp_v, tiles_sorted, end;
+    EmitLayout emit;
 };
 
 BinLayout layout(const BinPlan& p) {
     BinLayout L;
     size_t o = 0;
     L.err = o; o += 16;
-    L.emit_counter = o; o += 16;
-    L.emit_status = o; o = align256(o + 8ull * ((p.n + kEmitPart - 1) / kEmitPart + 1));
     L.zero_end = o;
+    o = align256(o);
     L.depth_sort = o; o = align256(o + p.depth_sort.scratch_bytes);
     L.tile_sort = o; o = align256(o + p.tile_sort.scratch_bytes);
+    o = emit_layout(o, p.n, p.isect_cap, false, L.emit);
     const size_t n4 = align256(4ull * (p.n > 0 ? p.n : 1));
     const size_t i4 = align256(4ull * (p.isect_cap > 0 ? p.isect_cap : 1));
     L.dkeys = o; o += n4;
+    L.rect_by_id = o; o += 2 * n4;
     L.dtmp_k = o; o += n4;
     L.dtmp_v = o; o += n4;
     L.dsorted_k = o; o += n4;
@@ -302,16 +510,16 @@ hipError_t bin_and_sort(const BinPlan& p, const Camera& c, const float* xys, con
     uint32_t* err = reinterpret_cast<uint32_t*>(b + L.err);
     auto U = [&](size_t off) { return reinterpret_cast<uint32_t*>(b + off); };
 
-    depth_keys_kernel<<<(p.n + 255) / 256, 256, 0, s>>>(p.n, depths, radii, U(L.dkeys));
+    uint2* rect_by_id = reinterpret_cast<uint2*>(b + L.rect_by_id);
+    depth_keys_kernel<<<(p.n + 255) / 256, 256, 0, s>>>(
+        c, depths, radii, reinterpret_cast<const float2*>(xys), U(L.dkeys), rect_by_id);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     e = sort_pairs(p.depth_sort, U(L.dkeys), nullptr, U(L.dsorted_k), U(L.perm), U(L.dtmp_k),
                    U(L.dtmp_v), p.n, nullptr, b + L.depth_sort, err, s);
     if (e != hipSuccess) return e;
-    emit_kernel<false><<<(p.n + kEmitPart - 1) / kEmitPart, 256, 0, s>>>(
-        c, nullptr, nullptr, U(L.perm), reinterpret_cast<const float2*>(xys), radii, p.isect_cap,
-        U(L.tiles), U(L.gids), reinterpret_cast<uint64_t*>(b + L.emit_status), U(L.emit_counter),
-        err, nullptr, nullptr);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
+    e = emit<false>(p.n, c.tiles_x, nullptr, nullptr, U(L.perm), rect_by_id, p.isect_cap, b,
+                    L.emit, nullptr, nullptr, err, U(L.tiles), U(L.gids), s);
+    if (e != hipSuccess) return e;
     uint32_t* tiles_out = tiles_sorted ? reinterpret_cast<uint32_t*>(tiles_sorted) : U(L.tiles_sorted);
     e = sort_pairs(p.tile_sort, U(L.tiles), U(L.gids), tiles_out,
                    reinterpret_cast<uint32_t*>(gids_sorted), U(L.ttmp_k), U(L.ttmp_v),
@@ -330,8 +538,8 @@ hipError_t bin_and_sort(const BinPlan& p, const Camera& c, const float* xys, con
 //  1. stable sort of the items by depth key (4 passes): (depth bits, item);
 //  2. view_keys + one stable pass on the view bits: (view, depth bits, item),
 //     the emission order;
-//  3. emit_kernel<true>: keys view << tile_bits | tile, values = item ids,
-//     with the count published by the last partition;
+//  3. emission (BATCH): keys view << tile_bits | tile, values = item ids,
+//     with the total and the clamped count published on the device;
 //  4. stable sort on the tile bits only (the view bits ride along): order
 //     (tile, view, depth bits, item), so every (view, tile) run is contiguous
 //     and in the reference's per-tile order;
@@ -339,9 +547,10 @@ hipError_t bin_and_sort(const BinPlan& p, const Camera& c, const float* xys, con
 namespace {
 
 struct BatchLayout {
-    size_t err, count, total, emit_counter, emit_status, zero_end;
+    size_t err, count, total, zero_end;
     size_t depth_sort, view_sort, tile_sort, dsorted_k, perm, dtmp_k, dtmp_v, vkeys, vsorted,
         perm2, keys, vals, ttmp_k, ttmp_v, keys_sorted, end;
+    EmitLayout emit;
 };
 
 BatchLayout batch_layout(const BatchBinPlan& p) {
@@ -350,12 +559,12 @@ BatchLayout batch_layout(const BatchBinPlan& p) {
     L.err = o; o += 16;
     L.count = o; o += 16;
     L.total = o; o += 16;
-    L.emit_counter = o; o += 16;
-    L.emit_status = o; o = align256(o + 8ull * ((p.n_items + kEmitPart - 1) / kEmitPart + 1));
     L.zero_end = o;
+    o = align256(o);
     L.depth_sort = o; o = align256(o + p.depth_sort.scratch_bytes);
     L.view_sort = o; o = align256(o + p.view_sort.scratch_bytes);
     L.tile_sort = o; o = align256(o + p.tile_sort.scratch_bytes);
+    o = emit_layout(o, p.n_items, p.isect_cap, true, L.emit);
     const size_t n4 = align256(4ull * (p.n_items > 0 ? p.n_items : 1));
     const size_t i4 = align256(4ull * (p.isect_cap > 0 ? p.isect_cap : 1));
     L.dsorted_k = o; o += n4;
@@ -406,9 +615,9 @@ const unsigned long long* batch_bin_total(const BatchBinPlan& p, const void* scr
                                                        batch_layout(p).total);
 }
 
-hipError_t batch_bin(const BatchBinPlan& p, const ViewRec* views, int block_width,
-                     const float* xys, const int* radii, const uint32_t* depth_keys,
-                     int* items_sorted, int* tile_bins, void* scratch, hipStream_t s) {
+hipError_t batch_bin(const BatchBinPlan& p, const ViewRec* views, const uint2* item_rects,
+                     const uint32_t* depth_keys, int* items_sorted, int* tile_bins,
+                     void* scratch, hipStream_t s) {
     const BatchLayout L = batch_layout(p);
     uint8_t* b = static_cast<uint8_t*>(scratch);
     hipError_t e = fill_zero(b, L.zero_end, s);
@@ -436,14 +645,11 @@ hipError_t batch_bin(const BatchBinPlan& p, const ViewRec* views, int block_widt
         order = U(L.perm2);
         order_views = U(L.vsorted);
     }
-    Camera c{};
-    c.num_points = p.n_items;
-    c.block_width = block_width;
-    emit_kernel<true><<<(p.n_items + kEmitPart - 1) / kEmitPart, 256, 0, s>>>(
-        c, views, order_views, order, reinterpret_cast<const float2*>(xys), radii, p.isect_cap,
-        U(L.keys), U(L.vals), reinterpret_cast<uint64_t*>(b + L.emit_status), U(L.emit_counter),
-        err, count, reinterpret_cast<unsigned long long*>(b + L.total));
-    if ((e = hipGetLastError()) != hipSuccess) return e;
+    e = emit<true>(p.n_items, 0, views, order_views, order, item_rects, p.isect_cap, b, L.emit,
+                   count,
+                   reinterpret_cast<unsigned long long*>(b + L.total), err, U(L.keys),
+                   U(L.vals), s);
+    if (e != hipSuccess) return e;
     e = sort_pairs(p.tile_sort, U(L.keys), U(L.vals), U(L.keys_sorted),
                    reinterpret_cast<uint32_t*>(items_sorted), U(L.ttmp_k), U(L.ttmp_v), 0, count,
                    b + L.tile_sort, err, s);

@@ -273,7 +273,7 @@ int jaxsplat_bin_check(const jaxsplat_descriptor* d, uint32_t num_intersects,
                                  hipMemcpyDeviceToHost, s),
                   "bin_check");
     JSA_CHECK_HIP(hipStreamSynchronize(s), "bin_check");
-    if (err) return fail(JAXSPLAT_ERR_CAPACITY, "look-back wait limit exceeded (code %u)", err);
+    if (err) return fail(JAXSPLAT_ERR_CAPACITY, "binning error (code %u)", err);
     return JAXSPLAT_OK;
 }
 
@@ -520,8 +520,8 @@ struct jaxsplat_batch {
     jsa::ViewRec* d_views = nullptr;
     jsa::SceneRec* d_scenes = nullptr;
     // workspace layout
-    size_t xys = 0, conics = 0, radii = 0, dkeys = 0, v_xy = 0, v_conic = 0, items = 0,
-           bins = 0, bin_scratch = 0, ws_bytes = 0;
+    size_t xys = 0, conics = 0, radii = 0, dkeys = 0, rects = 0, v_xy = 0, v_conic = 0,
+           items = 0, bins = 0, bin_scratch = 0, ws_bytes = 0;
 };
 
 namespace {
@@ -654,6 +654,7 @@ int jaxsplat_batch_create(jaxsplat_batch** out, const jaxsplat_scene* scenes, in
     b->conics = o; o = align256(o + 12 * n);
     b->radii = o; o = align256(o + 4 * n);
     b->dkeys = o; o = align256(o + 4 * n);
+    b->rects = o; o = align256(o + 8 * n);
     b->v_xy = o; o = align256(o + 8 * n);
     b->v_conic = o; o = align256(o + 12 * n);
     b->items = o; o = align256(o + 4 * (size_t)isect_capacity);
@@ -721,14 +722,15 @@ int jaxsplat_batch_forward(const jaxsplat_batch* b, const float* means3d, const 
     float* conics = reinterpret_cast<float*>(w + b->conics);
     int* radii = reinterpret_cast<int*>(w + b->radii);
     uint32_t* dkeys = reinterpret_cast<uint32_t*>(w + b->dkeys);
+    uint2* rects = reinterpret_cast<uint2*>(w + b->rects);
     int* items = reinterpret_cast<int*>(w + b->items);
     int* bins = reinterpret_cast<int*>(w + b->bins);
     JSA_CHECK_HIP(jsa::project_batch_fwd(b->d_views, b->num_views, b->max_view_n, b->glob_scale,
                                          b->clip_thresh, b->block_width, means3d, scales, quats,
-                                         viewmats, xys, conics, radii, dkeys, s),
+                                         viewmats, xys, conics, radii, dkeys, rects, s),
                   "batch_forward: project");
-    JSA_CHECK_HIP(jsa::batch_bin(b->bin, b->d_views, b->block_width, xys, radii, dkeys, items,
-                                 bins, w + b->bin_scratch, s),
+    JSA_CHECK_HIP(jsa::batch_bin(b->bin, b->d_views, rects, dkeys, items, bins,
+                                 w + b->bin_scratch, s),
                   "batch_forward: bin");
     JSA_CHECK_HIP(jsa::raster_fwd_grid(batch_grid(b), items, bins, xys, conics, colors, opacities,
                                        backgrounds, final_Ts, final_idx, out_img, s),

@@ -123,6 +123,18 @@ __device__ __forceinline__ T wave_inclusive_sum(T v) {
     return v;
 }
 
+// Tile rectangle [x0, x1) x [y0, y1) packed as (x0 | y0 << 16, w | h << 16),
+// w = x1 - x0, h = y1 - y0; it covers w * h tiles (tile counts below 2^16 per
+// axis).
+__device__ __forceinline__ uint2 pack_rect(int x0, int x1, int y0, int y1) {
+    return make_uint2((uint32_t)x0 | ((uint32_t)y0 << 16),
+                      (uint32_t)(x1 - x0) | ((uint32_t)(y1 - y0) << 16));
+}
+
+__device__ __forceinline__ uint32_t rect_count(uint2 rc) {
+    return (rc.y & 0xFFFFu) * (rc.y >> 16);
+}
+
 // ---------------------------------------------------------------------------
 // Device-scope coherent loads/stores for look-back status words
 // ---------------------------------------------------------------------------
@@ -285,17 +297,19 @@ hipError_t bin_and_sort(const BinPlan& p, const Camera& c, const float* xys,
                         const float* depths, const int* radii, int num_isect_host,
                         const int* num_isect_dev, int* gids_sorted, int* tiles_sorted,
                         int* tile_bins, void* scratch, hipStream_t s);
-// status word of the last bin_and_sort using `scratch` (non-zero = look-back timeout)
+// status word of the last bin_and_sort using `scratch`: bit 4 bad tile key,
+// bit 8 more intersections than the capacity
 const uint32_t* bin_error_word(const BinPlan& p, const void* scratch);
 
 // ---- batched projection (views of scenes, see ViewRec) ----
-// Forward: items of every view; writes xys, conics, radii and the depth sort
-// key (depth bits, 0xFFFFFFFF for items that emit nothing).
+// Forward: items of every view; writes xys, conics, radii, the depth sort
+// key (depth bits, 0xFFFFFFFF for items that emit nothing) and the packed tile
+// rectangle (pack_rect; 0 for items that emit nothing).
 hipError_t project_batch_fwd(const ViewRec* views, int num_views, int max_view_n,
                              float glob_scale, float clip_thresh, int block_width,
                              const float* means3d, const float* scales, const float* quats,
                              const float* viewmats, float* xys, float* conics, int* radii,
-                             uint32_t* depth_keys, hipStream_t s);
+                             uint32_t* depth_keys, uint2* item_rects, hipStream_t s);
 // Backward: one lane per Gaussian, summing the contributions of its scene's
 // views; writes v_mean3d, v_scale, v_quat for every Gaussian of every scene.
 hipError_t project_batch_bwd(const SceneRec* scenes, int num_scenes, int max_scene_n,
@@ -319,10 +333,10 @@ BatchBinPlan batch_bin_plan(int n_items, int num_views, int max_view_tiles, int 
 // Sorted item ids of every (view, tile) and bins[view << tile_bits | tile].
 // The scratch keeps the intersection count clamped to the capacity (the
 // length the sorts and bins use), the exact total (64 bits) and an error word:
-// bit 1 look-back wait limit, bit 4 bad tile key, bit 8 capacity overflow.
-hipError_t batch_bin(const BatchBinPlan& p, const ViewRec* views, int block_width,
-                     const float* xys, const int* radii, const uint32_t* depth_keys,
-                     int* items_sorted, int* tile_bins, void* scratch, hipStream_t s);
+// bit 4 bad tile key, bit 8 capacity overflow.
+hipError_t batch_bin(const BatchBinPlan& p, const ViewRec* views, const uint2* item_rects,
+                     const uint32_t* depth_keys, int* items_sorted, int* tile_bins,
+                     void* scratch, hipStream_t s);
 const uint32_t* batch_bin_error_word(const BatchBinPlan& p, const void* scratch);
 const int* batch_bin_count(const BatchBinPlan& p, const void* scratch);
 const unsigned long long* batch_bin_total(const BatchBinPlan& p, const void* scratch);

@@ -138,6 +138,7 @@ struct Proj {
     float cov[6], conic[3];
     float cx, cy, depth, comp;
     int rad, count;
+    uint2 rect;         // pack_rect of the tile rectangle, 0 if count == 0
 };
 
 // Projection of Gaussian idx (forward.cu:13-131).
@@ -154,6 +155,7 @@ __device__ __forceinline__ Proj project_core(const Camera& c, int idx, const M3&
     float conic[3] = {0.f, 0.f, 0.f};
     float cx = 0.f, cy = 0.f, depth = 0.f, comp = 0.f;
     int rad_i = 0, count = 0;
+    uint2 rect = make_uint2(0u, 0u);
     if (!(t[2] <= c.clip_thresh)) {
         const float s[3] = {scales[3 * idx + 0], scales[3 * idx + 1], scales[3 * idx + 2]};
         const float4 q4 = reinterpret_cast<const float4*>(quats)[idx];
@@ -199,6 +201,7 @@ __device__ __forceinline__ Proj project_core(const Camera& c, int idx, const M3&
             const int area = (x1 - x0) * (y1 - y0);
             if (area > 0) {
                 count = area;
+                rect = pack_rect(x0, x1, y0, y1);
                 depth = t[2];
                 rad_i = (int)radius;
                 cx = px;
@@ -219,6 +222,7 @@ __device__ __forceinline__ Proj project_core(const Camera& c, int idx, const M3&
     r.comp = comp;
     r.rad = rad_i;
     r.count = count;
+    r.rect = rect;
     return r;
 }
 
@@ -279,7 +283,7 @@ __global__ __launch_bounds__(256) void project_batch_fwd_kernel(
     const float* __restrict__ means3d, const float* __restrict__ scales,
     const float* __restrict__ quats, const float* __restrict__ viewmats,
     float* __restrict__ xys, float* __restrict__ conics, int* __restrict__ radii,
-    uint32_t* __restrict__ dkeys) {
+    uint32_t* __restrict__ dkeys, uint2* __restrict__ rects) {
     const ViewRec vr = views[blockIdx.y];
     const int first = blockIdx.x * kProjPart;
     if (first >= vr.n) return;
@@ -302,6 +306,7 @@ __global__ __launch_bounds__(256) void project_batch_fwd_kernel(
         conics[3 * it + 2] = r.conic[2];
         radii[it] = r.rad;
         dkeys[it] = r.rad > 0 ? __float_as_uint(r.depth) : 0xFFFFFFFFu;
+        rects[it] = r.rect;
     }
 }
 
@@ -675,12 +680,12 @@ hipError_t project_batch_fwd(const ViewRec* views, int num_views, int max_view_n
                              float glob_scale, float clip_thresh, int block_width,
                              const float* means3d, const float* scales, const float* quats,
                              const float* viewmats, float* xys, float* conics, int* radii,
-                             uint32_t* depth_keys, hipStream_t s) {
+                             uint32_t* depth_keys, uint2* item_rects, hipStream_t s) {
     if (num_views <= 0 || max_view_n <= 0) return hipSuccess;
     const dim3 grid((max_view_n + kProjPart - 1) / kProjPart, num_views);
     project_batch_fwd_kernel<<<grid, 256, 0, s>>>(views, glob_scale, clip_thresh, block_width,
                                                   means3d, scales, quats, viewmats, xys, conics,
-                                                  radii, depth_keys);
+                                                  radii, depth_keys, item_rects);
     return hipGetLastError();
 }
 
