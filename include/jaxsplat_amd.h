@@ -29,7 +29,7 @@ typedef enum jaxsplat_status {
     JAXSPLAT_OK = 0,
     JAXSPLAT_ERR_INVALID = 1,   /* bad descriptor / argument */
     JAXSPLAT_ERR_HIP = 2,       /* HIP runtime error */
-    JAXSPLAT_ERR_CAPACITY = 3,  /* look-back timeout or intersection overflow */
+    JAXSPLAT_ERR_CAPACITY = 3,  /* intersection overflow or a binning error */
     JAXSPLAT_ERR_WORKSPACE = 4  /* workspace missing or too small */
 } jaxsplat_status;
 
@@ -148,9 +148,29 @@ int jaxsplat_bin_and_sort(const jaxsplat_descriptor* d, const float* xys, const 
                           void* stream);
 
 /*
+ * The same binning in two stages, so that a caller that reads num_intersects
+ * back from cum_tiles_hit can queue the first stage before it waits for the
+ * count (the reference reads it between projection and sort_and_bin,
+ * lib/ops.cu:645-660).  jaxsplat_bin_depth_sort needs no count: depth keys and
+ * the stable depth sort of the Gaussians, into a workspace that depends on
+ * num_points only.  jaxsplat_bin_emit_sort then emits, sorts and bins from
+ * that workspace, with the outputs of jaxsplat_bin_and_sort; its workspace
+ * (num_intersects dependent) is the one jaxsplat_bin_check reads.
+ */
+size_t jaxsplat_bin_depth_workspace_size(const jaxsplat_descriptor* d);
+int jaxsplat_bin_depth_sort(const jaxsplat_descriptor* d, const float* xys, const float* depths,
+                            const int32_t* radii, void* depth_workspace,
+                            size_t depth_workspace_bytes, void* stream);
+size_t jaxsplat_bin_emit_workspace_size(const jaxsplat_descriptor* d, uint32_t num_intersects);
+int jaxsplat_bin_emit_sort(const jaxsplat_descriptor* d, uint32_t num_intersects,
+                           const void* depth_workspace, int32_t* gaussian_ids_sorted,
+                           int32_t* tile_ids_sorted, int32_t* tile_bins, void* workspace,
+                           size_t workspace_bytes, void* stream);
+
+/*
  * Synchronise `stream` and report an error recorded in the workspace of the
- * last jaxsplat_bin_and_sort (JAXSPLAT_ERR_CAPACITY): more intersections than
- * num_intersects, or a tile key out of range.
+ * last jaxsplat_bin_and_sort or jaxsplat_bin_emit_sort (JAXSPLAT_ERR_CAPACITY):
+ * more intersections than num_intersects, or a tile key out of range.
  */
 int jaxsplat_bin_check(const jaxsplat_descriptor* d, uint32_t num_intersects,
                        const void* workspace, void* stream);
@@ -281,7 +301,7 @@ int jaxsplat_batch_backward(const jaxsplat_batch* b, const float* means3d, const
 
 /* Synchronous status of the last forward in `workspace`: the exact number of
  * intersections, and JAXSPLAT_ERR_CAPACITY if it exceeded isect_capacity (the
- * forward's outputs are then invalid) or a look-back wait timed out. */
+ * forward's outputs are then invalid) or a tile key was out of range. */
 int jaxsplat_batch_check(const jaxsplat_batch* b, const void* workspace,
                          int64_t* num_intersects, void* stream);
 

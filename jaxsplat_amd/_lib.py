@@ -26,6 +26,8 @@ EXPORTS = (
     "jaxsplat_rasterize_bwd", "jaxsplat_project_workspace_size",
     "jaxsplat_project_forward", "jaxsplat_project_backward",
     "jaxsplat_bin_workspace_size", "jaxsplat_bin_and_sort", "jaxsplat_bin_check",
+    "jaxsplat_bin_depth_workspace_size", "jaxsplat_bin_depth_sort",
+    "jaxsplat_bin_emit_workspace_size", "jaxsplat_bin_emit_sort",
     "jaxsplat_rasterize_forward_binned", "jaxsplat_rasterize_backward_binned",
     "jaxsplat_rasterize_forward_nd", "jaxsplat_rasterize_backward_nd",
     "jaxsplat_batch_create", "jaxsplat_batch_destroy", "jaxsplat_batch_workspace_size",
@@ -103,6 +105,10 @@ _SIGS = {
     "jaxsplat_bin_workspace_size": (SZ, [DP, U32]),
     "jaxsplat_bin_and_sort": (I32, [DP, P, P, P, U32, P, P, P, P, SZ, P]),
     "jaxsplat_bin_check": (I32, [DP, U32, P, P]),
+    "jaxsplat_bin_depth_workspace_size": (SZ, [DP]),
+    "jaxsplat_bin_depth_sort": (I32, [DP, P, P, P, P, SZ, P]),
+    "jaxsplat_bin_emit_workspace_size": (SZ, [DP, U32]),
+    "jaxsplat_bin_emit_sort": (I32, [DP, U32, P, P, P, P, P, SZ, P]),
     "jaxsplat_rasterize_forward_binned": (I32, [DP, P, P, U32] + [P] * 8 + [P]),
     "jaxsplat_rasterize_backward_binned": (I32, [DP, P, P, U32] + [P] * 14 + [I32, P]),
     "jaxsplat_rasterize_forward_nd": (I32, [DP, P, P, U32] + [P] * 5 + [U32] + [P] * 3 + [P]),

@@ -164,6 +164,36 @@ def _bin(d, xys, depths, radii, isect, dev, return_tiles=False):
     return out + (isect,)
 
 
+def _bin_overlapped(d, xys, depths, radii, cum_tiles_hit, dev):
+    """Binning with the host read of num_intersects = cum_tiles_hit[N-1] (which
+    sizes the outputs, as in the reference) overlapped with the depth stage:
+    the count is copied to pinned memory and the depth sort queued behind it
+    before the host waits.  Returns (gaussian_ids_sorted, tile_bins, isect)."""
+    N = xys.shape[0]
+    if N == 0:
+        return _bin(d, xys, depths, radii, 0, dev)
+    L = _lib.lib()
+    with torch.cuda.device(dev):
+        count = torch.empty(1, dtype=torch.int32, pin_memory=True)
+        count.copy_(cum_tiles_hit.reshape(-1)[-1:], non_blocking=True)
+        ready = torch.cuda.Event()
+        ready.record()
+        dws_bytes = L.jaxsplat_bin_depth_workspace_size(ctypes.byref(d))
+        dws = torch.empty(max(dws_bytes, 1), dtype=torch.uint8, device=dev)
+        _lib.check(L.jaxsplat_bin_depth_sort(ctypes.byref(d), _p(xys), _p(depths), _p(radii),
+                                             _p(dws), dws_bytes, _stream(dev)))
+        ready.synchronize()
+        isect = int(count[0])
+        ntiles = d.grid_dim_2d.x * d.grid_dim_2d.y
+        gids = torch.empty(max(isect, 1), dtype=torch.int32, device=dev)
+        bins = torch.empty((ntiles, 2), dtype=torch.int32, device=dev)
+        ws_bytes = L.jaxsplat_bin_emit_workspace_size(ctypes.byref(d), isect)
+        ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=dev)
+        _lib.check(L.jaxsplat_bin_emit_sort(ctypes.byref(d), isect, _p(dws), _p(gids), None,
+                                            _p(bins), _p(ws), ws_bytes, _stream(dev)))
+    return gids[:isect], bins, isect
+
+
 class _Rasterize(torch.autograd.Function):
     @staticmethod
     def forward(ctx, cfg, colors, opacities, background, xys, depths, radii, conics,
@@ -173,8 +203,7 @@ class _Rasterize(torch.autograd.Function):
         dev = colors.device
         H, W = img_shape
         d = _lib.make_descriptor(N, img_shape, (1.0, 1.0), (0.0, 0.0), 1.0, 0.0, block_width)
-        isect = int(cum_tiles_hit[-1, 0].item()) if N > 0 else 0
-        gids, bins, _ = _bin(d, xys, depths, radii, isect, dev)
+        gids, bins, isect = _bin_overlapped(d, xys, depths, radii, cum_tiles_hit, dev)
         final_Ts = torch.empty((H, W, 1), dtype=torch.float32, device=dev)
         final_idx = torch.empty((H, W, 1), dtype=torch.int32, device=dev)
         img = torch.empty((H, W, nch), dtype=torch.float32, device=dev)

@@ -441,12 +441,31 @@ hipError_t emit(int n, int tiles_x_plain, const ViewRec* views, const uint32_t* 
     return hipGetLastError();
 }
 
+// Depth stage: needs no intersection count, so a caller can run it while it
+// reads the count back.  Its workspace depends on the Gaussian count only.
+struct DepthLayout {
+    size_t depth_sort, dkeys, rect_by_id, dtmp_k, dtmp_v, dsorted_k, perm, end;
+};
+
+DepthLayout depth_layout(const BinPlan& p) {
+    DepthLayout L;
+    size_t o = 0;
+    L.depth_sort = o; o = align256(o + p.depth_sort.scratch_bytes);
+    const size_t n4 = align256(4ull * (p.n > 0 ? p.n : 1));
+    L.dkeys = o; o += n4;
+    L.rect_by_id = o; o += 2 * n4;
+    L.dtmp_k = o; o += n4;
+    L.dtmp_v = o; o += n4;
+    L.dsorted_k = o; o += n4;
+    L.perm = o; o += n4;
+    L.end = o;
+    return L;
+}
+
+// Emission stage (emission, tile sort, bin edges); the error word comes first.
 struct BinLayout {
-    // zeroed region
     size_t err, zero_end;
-    // plain region
-    size_t depth_sort, tile_sort, dkeys, rect_by_id, dtmp_k, dtmp_v, dsorted_k, perm, tiles,
-        gids, ttmp_k, ttmp_v, tiles_sorted, end;
+    size_t tile_sort, tiles, gids, ttmp_k, ttmp_v, tiles_sorted, end;
     EmitLayout emit;
 };
 
@@ -456,17 +475,9 @@ BinLayout layout(const BinPlan& p) {
     L.err = o; o += 16;
     L.zero_end = o;
     o = align256(o);
-    L.depth_sort = o; o = align256(o + p.depth_sort.scratch_bytes);
     L.tile_sort = o; o = align256(o + p.tile_sort.scratch_bytes);
     o = emit_layout(o, p.n, p.isect_cap, false, L.emit);
-    const size_t n4 = align256(4ull * (p.n > 0 ? p.n : 1));
     const size_t i4 = align256(4ull * (p.isect_cap > 0 ? p.isect_cap : 1));
-    L.dkeys = o; o += n4;
-    L.rect_by_id = o; o += 2 * n4;
-    L.dtmp_k = o; o += n4;
-    L.dtmp_v = o; o += n4;
-    L.dsorted_k = o; o += n4;
-    L.perm = o; o += n4;
     L.tiles = o; o += i4;
     L.gids = o; o += i4;
     L.ttmp_k = o; o += i4;
@@ -488,7 +499,9 @@ BinPlan bin_plan(const Camera& c, int isect_cap) {
     p.tile_bits = bitlen((uint32_t)(p.num_tiles > 1 ? p.num_tiles - 1 : 1));
     p.depth_sort = sort_plan(p.n, 0, 32);
     p.tile_sort = sort_plan(isect_cap, 0, p.tile_bits);
-    p.scratch_bytes = layout(p).end;
+    p.depth_bytes = align256(depth_layout(p).end);
+    p.emit_bytes = align256(layout(p).end);
+    p.scratch_bytes = p.emit_bytes + p.depth_bytes;
     return p;
 }
 
@@ -496,10 +509,24 @@ const uint32_t* bin_error_word(const BinPlan& p, const void* scratch) {
     return reinterpret_cast<const uint32_t*>(static_cast<const uint8_t*>(scratch) + layout(p).err);
 }
 
-hipError_t bin_and_sort(const BinPlan& p, const Camera& c, const float* xys, const float* depths,
-                        const int* radii, int num_isect_host, const int* num_isect_dev,
-                        int* gids_sorted, int* tiles_sorted, int* tile_bins, void* scratch,
-                        hipStream_t s) {
+hipError_t bin_depth(const BinPlan& p, const Camera& c, const float* xys, const float* depths,
+                     const int* radii, void* depth_ws, hipStream_t s) {
+    if (p.n <= 0) return hipSuccess;
+    const DepthLayout D = depth_layout(p);
+    uint8_t* b = static_cast<uint8_t*>(depth_ws);
+    auto U = [&](size_t off) { return reinterpret_cast<uint32_t*>(b + off); };
+    depth_keys_kernel<<<(p.n + 255) / 256, 256, 0, s>>>(
+        c, depths, radii, reinterpret_cast<const float2*>(xys), U(D.dkeys),
+        reinterpret_cast<uint2*>(b + D.rect_by_id));
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return sort_pairs(p.depth_sort, U(D.dkeys), nullptr, U(D.dsorted_k), U(D.perm), U(D.dtmp_k),
+                      U(D.dtmp_v), p.n, nullptr, b + D.depth_sort, nullptr, s);
+}
+
+hipError_t bin_emit(const BinPlan& p, const Camera& c, int num_isect_host,
+                    const int* num_isect_dev, const void* depth_ws, int* gids_sorted,
+                    int* tiles_sorted, int* tile_bins, void* scratch, hipStream_t s) {
     const BinLayout L = layout(p);
     uint8_t* b = static_cast<uint8_t*>(scratch);
     hipError_t e = fill_zero(b, L.zero_end, s);
@@ -509,16 +536,12 @@ hipError_t bin_and_sort(const BinPlan& p, const Camera& c, const float* xys, con
     if (p.n <= 0) return hipSuccess;
     uint32_t* err = reinterpret_cast<uint32_t*>(b + L.err);
     auto U = [&](size_t off) { return reinterpret_cast<uint32_t*>(b + off); };
-
-    uint2* rect_by_id = reinterpret_cast<uint2*>(b + L.rect_by_id);
-    depth_keys_kernel<<<(p.n + 255) / 256, 256, 0, s>>>(
-        c, depths, radii, reinterpret_cast<const float2*>(xys), U(L.dkeys), rect_by_id);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    e = sort_pairs(p.depth_sort, U(L.dkeys), nullptr, U(L.dsorted_k), U(L.perm), U(L.dtmp_k),
-                   U(L.dtmp_v), p.n, nullptr, b + L.depth_sort, err, s);
-    if (e != hipSuccess) return e;
-    e = emit<false>(p.n, c.tiles_x, nullptr, nullptr, U(L.perm), rect_by_id, p.isect_cap, b,
-                    L.emit, nullptr, nullptr, err, U(L.tiles), U(L.gids), s);
+    const DepthLayout D = depth_layout(p);
+    const uint8_t* d = static_cast<const uint8_t*>(depth_ws);
+    const uint32_t* perm = reinterpret_cast<const uint32_t*>(d + D.perm);
+    const uint2* rect_by_id = reinterpret_cast<const uint2*>(d + D.rect_by_id);
+    e = emit<false>(p.n, c.tiles_x, nullptr, nullptr, perm, rect_by_id, p.isect_cap, b, L.emit,
+                    nullptr, nullptr, err, U(L.tiles), U(L.gids), s);
     if (e != hipSuccess) return e;
     uint32_t* tiles_out = tiles_sorted ? reinterpret_cast<uint32_t*>(tiles_sorted) : U(L.tiles_sorted);
     e = sort_pairs(p.tile_sort, U(L.tiles), U(L.gids), tiles_out,
@@ -530,6 +553,19 @@ hipError_t bin_and_sort(const BinPlan& p, const Camera& c, const float* xys, con
                                                    p.num_tiles,
                                                    reinterpret_cast<int2*>(tile_bins), err);
     return hipGetLastError();
+}
+
+// One workspace: the emission stage's region (error word first), then the
+// depth stage's.
+hipError_t bin_and_sort(const BinPlan& p, const Camera& c, const float* xys, const float* depths,
+                        const int* radii, int num_isect_host, const int* num_isect_dev,
+                        int* gids_sorted, int* tiles_sorted, int* tile_bins, void* scratch,
+                        hipStream_t s) {
+    uint8_t* depth_ws = static_cast<uint8_t*>(scratch) + p.emit_bytes;
+    hipError_t e = bin_depth(p, c, xys, depths, radii, depth_ws, s);
+    if (e != hipSuccess) return e;
+    return bin_emit(p, c, num_isect_host, num_isect_dev, depth_ws, gids_sorted, tiles_sorted,
+                    tile_bins, scratch, s);
 }
 
 // ---------------------------------------------------------------------------

@@ -261,6 +261,62 @@ int jaxsplat_bin_and_sort(const jaxsplat_descriptor* d, const float* xys, const 
     return JAXSPLAT_OK;
 }
 
+size_t jaxsplat_bin_depth_workspace_size(const jaxsplat_descriptor* d) {
+    if (!d || validate(d)) return 0;
+    return align256(jsa::bin_plan(camera_of(d), 0).depth_bytes);
+}
+
+int jaxsplat_bin_depth_sort(const jaxsplat_descriptor* d, const float* xys, const float* depths,
+                            const int32_t* radii, void* depth_workspace,
+                            size_t depth_workspace_bytes, void* stream) {
+    g_err.clear();
+    if (int st = validate(d)) return st;
+    if (d->num_points == 0) return JAXSPLAT_OK;
+    const jsa::Camera c = camera_of(d);
+    const jsa::BinPlan p = jsa::bin_plan(c, 0);
+    if (!xys || !depths || !radii)
+        return fail(JAXSPLAT_ERR_INVALID, "bin_depth_sort: NULL required pointer");
+    if (!depth_workspace || depth_workspace_bytes < p.depth_bytes)
+        return fail(JAXSPLAT_ERR_WORKSPACE, "bin_depth_sort: workspace too small (%zu < %zu)",
+                    depth_workspace_bytes, p.depth_bytes);
+    JSA_CHECK_HIP(jsa::bin_depth(p, c, xys, depths, radii, depth_workspace,
+                                 static_cast<hipStream_t>(stream)),
+                  "bin_depth_sort");
+    return JAXSPLAT_OK;
+}
+
+size_t jaxsplat_bin_emit_workspace_size(const jaxsplat_descriptor* d, uint32_t num_intersects) {
+    if (!d || validate(d)) return 0;
+    return align256(jsa::bin_plan(camera_of(d), (int)num_intersects).emit_bytes);
+}
+
+int jaxsplat_bin_emit_sort(const jaxsplat_descriptor* d, uint32_t num_intersects,
+                           const void* depth_workspace, int32_t* gaussian_ids_sorted,
+                           int32_t* tile_ids_sorted, int32_t* tile_bins, void* workspace,
+                           size_t workspace_bytes, void* stream) {
+    g_err.clear();
+    if (int st = validate(d)) return st;
+    if (num_intersects >= (1u << 30))
+        return fail(JAXSPLAT_ERR_CAPACITY, "num_intersects %u exceeds 2^30", num_intersects);
+    if (!tile_bins) return fail(JAXSPLAT_ERR_INVALID, "bin_emit_sort: tile_bins is NULL");
+    const jsa::Camera c = camera_of(d);
+    const jsa::BinPlan p = jsa::bin_plan(c, (int)num_intersects);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (num_intersects == 0 || d->num_points == 0) {
+        JSA_CHECK_HIP(jsa::fill_zero(tile_bins, 8ull * p.num_tiles, s), "bin_emit_sort");
+        return JAXSPLAT_OK;
+    }
+    if (!depth_workspace || !gaussian_ids_sorted)
+        return fail(JAXSPLAT_ERR_INVALID, "bin_emit_sort: NULL required pointer");
+    if (!workspace || workspace_bytes < p.emit_bytes)
+        return fail(JAXSPLAT_ERR_WORKSPACE, "bin_emit_sort: workspace too small (%zu < %zu)",
+                    workspace_bytes, p.emit_bytes);
+    JSA_CHECK_HIP(jsa::bin_emit(p, c, (int)num_intersects, nullptr, depth_workspace,
+                                gaussian_ids_sorted, tile_ids_sorted, tile_bins, workspace, s),
+                  "bin_emit_sort");
+    return JAXSPLAT_OK;
+}
+
 int jaxsplat_bin_check(const jaxsplat_descriptor* d, uint32_t num_intersects,
                        const void* workspace, void* stream) {
     g_err.clear();

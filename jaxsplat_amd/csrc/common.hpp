@@ -1,7 +1,7 @@
 // Shared device utilities for the gfx950 (CDNA4) Gaussian-splatting kernels:
-// wave64 reductions on DPP, ballot helpers, the decoupled look-back status
-// words used by the single-pass scans (projection and emission), and the
-// internal host-side launch interface shared by the .hip translation units.
+// wave64 reductions and scans on DPP, ballot helpers, the packed tile
+// rectangle, and the internal host-side launch interface shared by the .hip
+// translation units.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -13,7 +13,6 @@ namespace jsa {
 constexpr int kWave = 64;
 constexpr int kMaxBlockWidth = 16;                 // tile side, bw*bw <= 256 threads
 constexpr int kMaxTileThreads = kMaxBlockWidth * kMaxBlockWidth;
-constexpr uint32_t kSpinLimit = 1u << 22;          // bounded look-back waits
 
 // ---------------------------------------------------------------------------
 // Error codes shared with the C ABI
@@ -136,60 +135,6 @@ __device__ __forceinline__ uint32_t rect_count(uint2 rc) {
 }
 
 // ---------------------------------------------------------------------------
-// Device-scope coherent loads/stores for look-back status words
-// ---------------------------------------------------------------------------
-template <typename T>
-__device__ __forceinline__ T load_agent(const T* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-template <typename T>
-__device__ __forceinline__ void store_agent(T* p, T v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// 64-bit status word of a single-pass scan partition:
-// bits 63..62 flag (0 = not ready, 1 = aggregate, 2 = inclusive prefix),
-// bits 61..0 value.
-constexpr uint64_t kFlagAgg = 1ull << 62;
-constexpr uint64_t kFlagPre = 2ull << 62;
-constexpr uint64_t kValMask = (1ull << 62) - 1;
-
-// Wave-parallel decoupled look-back: returns the exclusive prefix of
-// partition `part` (> 0).  Executed by one full wave; every lane returns the
-// same value.  Waits are bounded by kSpinLimit; on overflow *err is set and
-// the partial prefix is returned (the result is then invalid but the kernel
-// terminates).
-__device__ inline uint64_t lookback64(const uint64_t* status, int part, uint32_t* err) {
-    const int lane = lane_id();
-    uint64_t excl = 0;
-    int j = part - 1;
-    uint32_t spins = 0;
-    while (true) {
-        const int idx = j - lane;
-        uint64_t s = idx >= 0 ? load_agent(status + idx) : kFlagPre;
-        const uint64_t flag = s >> 62;
-        const uint64_t not_ready = ballot(flag == 0);
-        const uint64_t pre = ballot(flag == 2);
-        const uint64_t need = pre ? ((pre & (~pre + 1)) << 1) - 1 : ~0ull;
-        if (not_ready & need) {
-            if (++spins > kSpinLimit) {
-                if (lane == 0) atomicOr(err, 1u);
-                return excl;
-            }
-            __builtin_amdgcn_s_sleep(1);
-            continue;
-        }
-        uint64_t v = ((1ull << lane) & need) ? (s & kValMask) : 0;
-#pragma unroll
-        for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
-        excl += v;
-        if (pre) break;
-        j -= kWave;
-    }
-    return excl;
-}
-
-// ---------------------------------------------------------------------------
 // Host-side internal interface (implemented in the .hip files)
 // ---------------------------------------------------------------------------
 struct Camera {
@@ -290,9 +235,18 @@ struct BinPlan {
     int tile_bits;
     SortPlan depth_sort;
     SortPlan tile_sort;
-    size_t scratch_bytes;
+    size_t depth_bytes;     // depth stage workspace (depends on n only)
+    size_t emit_bytes;      // emission stage workspace
+    size_t scratch_bytes;   // both, for bin_and_sort: emission region first
 };
 BinPlan bin_plan(const Camera& c, int isect_cap);
+// Depth stage: depth keys, packed tile rectangles and the stable depth sort.
+hipError_t bin_depth(const BinPlan& p, const Camera& c, const float* xys, const float* depths,
+                     const int* radii, void* depth_ws, hipStream_t s);
+// Emission stage over the depth stage's workspace: emission, tile sort, edges.
+hipError_t bin_emit(const BinPlan& p, const Camera& c, int num_isect_host,
+                    const int* num_isect_dev, const void* depth_ws, int* gids_sorted,
+                    int* tiles_sorted, int* tile_bins, void* scratch, hipStream_t s);
 hipError_t bin_and_sort(const BinPlan& p, const Camera& c, const float* xys,
                         const float* depths, const int* radii, int num_isect_host,
                         const int* num_isect_dev, int* gids_sorted, int* tiles_sorted,

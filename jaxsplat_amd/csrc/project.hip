@@ -17,10 +17,11 @@
 //  * forward: four Gaussians per lane in coalesced 256-wide slabs, 256-lane
 //    workgroups; outputs of every Gaussian are written explicitly (culled ones
 //    get zeros), so no output memsets are needed;
-//  * the inclusive scan of num_tiles_hit (cum_tiles_hit) is fused into the
-//    forward kernel as a single-pass decoupled look-back scan: partitions are
-//    taken from an atomic counter, a wave publishes its aggregate and looks
-//    back over 64 predecessors at a time;
+//  * the inclusive scan of num_tiles_hit (cum_tiles_hit) is a reduce-then-scan:
+//    the forward kernel writes partition-local sums and partition totals, one
+//    workgroup scans the totals and a fix-up pass adds the offsets (a
+//    single-pass look-back scan was latency bound: its status words cross the
+//    XCDs);
 //  * cov3d and compensation outputs are optional (nullptr skips the store);
 //    the backward (one lane per Gaussian) recomputes cov3d from scales/quats;
 //  * batched variants (views of scenes, a device table of ViewRec): the
@@ -316,16 +317,12 @@ __global__ __launch_bounds__(256) void project_fwd_kernel(
     float* __restrict__ cov3d_out, float* __restrict__ xys, float* __restrict__ depths,
     int* __restrict__ radii, float* __restrict__ conics, float* __restrict__ comp_out,
     int* __restrict__ num_tiles_hit, int* __restrict__ cum_tiles_hit,
-    uint64_t* __restrict__ status, uint32_t* __restrict__ counter, uint32_t* __restrict__ err) {
-    __shared__ uint32_t s_part;
+    uint32_t* __restrict__ part_tot) {
     __shared__ uint32_t s_wave_tot[kProjPer][4];
-    __shared__ uint64_t s_excl;
     const int tid = threadIdx.x;
     const int lane = lane_id();
     const int wave = tid >> 6;
-    if (tid == 0) s_part = atomicAdd(counter, 1u);
-    __syncthreads();
-    const int part = (int)s_part;
+    const int part = blockIdx.x;
     M3 W;
     float pt[3];
     load_view(viewmat, W, pt);
@@ -344,8 +341,9 @@ __global__ __launch_bounds__(256) void project_fwd_kernel(
         if (lane == 63) s_wave_tot[k][wave] = (uint32_t)incl[k];
     }
     __syncthreads();
-    // ---- single-pass inclusive scan of the counts (decoupled look-back) ----
-    uint32_t total = 0, off[kProjPer];
+    // partition-local inclusive sums; project_scan_fixup_kernel adds the
+    // partition's offset
+    uint32_t total = 0;
 #pragma unroll
     for (int k = 0; k < kProjPer; ++k) {
         uint32_t w_off = 0, slab = 0;
@@ -355,27 +353,43 @@ __global__ __launch_bounds__(256) void project_fwd_kernel(
             w_off += (w < wave) ? v : 0u;
             slab += v;
         }
-        off[k] = total + w_off;
+        const int idx = part * kProjPart + k * 256 + tid;
+        if (idx < c.num_points) cum_tiles_hit[idx] = (int)(total + w_off + (uint32_t)incl[k]);
         total += slab;
     }
-    if (tid == 0) {
-        store_agent(status + part, (part == 0 ? kFlagPre : kFlagAgg) | (uint64_t)total);
-        if (part == 0) s_excl = 0;
-    }
-    if (wave == 0 && part > 0) {
-        const uint64_t excl = lookback64(status, part, err);
-        if (lane == 0) {
-            store_agent(status + part, kFlagPre | (excl + total));
-            s_excl = excl;
-        }
-    }
+    if (tid == 0) part_tot[part] = total;
+}
+
+// Exclusive scan of the partition totals, one workgroup of 1024 threads
+// (thread t scans a contiguous run of partitions).
+__global__ __launch_bounds__(1024) void project_scan_kernel(const uint32_t* __restrict__ part_tot,
+                                                            int nparts,
+                                                            uint32_t* __restrict__ part_off) {
+    __shared__ uint32_t s_w[16];
+    const int tid = threadIdx.x;
+    const int lane = lane_id();
+    const int wave = tid >> 6;
+    const int per = (nparts + 1023) / 1024;
+    const int b0 = min(tid * per, nparts), b1 = min(b0 + per, nparts);
+    uint32_t sum = 0;
+    for (int i = b0; i < b1; ++i) sum += part_tot[i];
+    const uint32_t incl = wave_inclusive_sum(sum);
+    if (lane == 63) s_w[wave] = incl;
     __syncthreads();
-    const uint64_t base = s_excl;
-#pragma unroll
-    for (int k = 0; k < kProjPer; ++k) {
-        const int idx = part * kProjPart + k * 256 + tid;
-        if (idx < c.num_points) cum_tiles_hit[idx] = (int)(base + off[k] + (uint32_t)incl[k]);
+    uint32_t run = incl - sum;
+    for (int w = 0; w < wave; ++w) run += s_w[w];
+    for (int i = b0; i < b1; ++i) {
+        part_off[i] = run;
+        run += part_tot[i];
     }
+}
+
+// cum_tiles_hit += its partition's offset (partition 0 has none).
+__global__ __launch_bounds__(256) void project_scan_fixup_kernel(
+    int n, const uint32_t* __restrict__ part_off, int* __restrict__ cum_tiles_hit) {
+    const int idx = kProjPart + blockIdx.x * 256 + threadIdx.x;
+    if (idx >= n) return;
+    cum_tiles_hit[idx] += (int)part_off[idx / kProjPart];
 }
 
 // helpers.h:155-192 with vR[row][col]; the reference indexes v_R[col][row].
@@ -649,15 +663,18 @@ hipError_t project_fwd(const Camera& c, const float* means3d, const float* scale
                        int* num_tiles_hit, int* cum_tiles_hit, void* scratch, hipStream_t s) {
     if (c.num_points <= 0) return hipSuccess;
     const int parts = (c.num_points + kProjPart - 1) / kProjPart;
-    uint8_t* base = static_cast<uint8_t*>(scratch);
-    hipError_t e = fill_zero(base, project_fwd_scratch_bytes(c.num_points), s);
-    if (e != hipSuccess) return e;
-    uint32_t* counter = reinterpret_cast<uint32_t*>(base);
-    uint32_t* err = counter + 1;
-    uint64_t* status = reinterpret_cast<uint64_t*>(base + 16);
+    uint32_t* part_tot = static_cast<uint32_t*>(scratch);
+    uint32_t* part_off = part_tot + parts;
     project_fwd_kernel<<<parts, 256, 0, s>>>(c, means3d, scales, quats, viewmat, cov3d, xys,
                                              depths, radii, conics, compensation, num_tiles_hit,
-                                             cum_tiles_hit, status, counter, err);
+                                             cum_tiles_hit, part_tot);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || parts == 1) return e;
+    project_scan_kernel<<<1, 1024, 0, s>>>(part_tot, parts, part_off);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    const int rest = c.num_points - kProjPart;
+    project_scan_fixup_kernel<<<(rest + 255) / 256, 256, 0, s>>>(c.num_points, part_off,
+                                                                 cum_tiles_hit);
     return hipGetLastError();
 }
 

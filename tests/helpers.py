@@ -63,7 +63,9 @@ def gpu_project(means3d, scales, quats, viewmat, *, img_shape, f, c, glob_scale=
     return {k: host(v) for k, v in outs.items()}
 
 
-def gpu_bin(xys, depths, radii, cum_tiles_hit, *, img_shape, block_width=16):
+def gpu_bin(xys, depths, radii, cum_tiles_hit, *, img_shape, block_width=16, split=False):
+    """Binning through jaxsplat_bin_and_sort, or (split) through the depth
+    stage and the emission stage with separate workspaces."""
     t = torch()
     N = xys.shape[0]
     d = descriptor(N, img_shape, block_width=block_width)
@@ -73,13 +75,23 @@ def gpu_bin(xys, depths, radii, cum_tiles_hit, *, img_shape, block_width=16):
     gids = t.full((max(isect, 1),), -5, dtype=t.int32, device="cuda")
     tiles = t.full((max(isect, 1),), -5, dtype=t.int32, device="cuda")
     bins = t.full((ntiles, 2), -5, dtype=t.int32, device="cuda")
-    wsb = L.jaxsplat_bin_workspace_size(ctypes.byref(d), isect)
-    ws = t.empty(max(wsb, 1), dtype=t.uint8, device="cuda")
     x = dev(np.asarray(xys, np.float32).reshape(N, 2))
     dp = dev(np.asarray(depths, np.float32).reshape(N))
     r = dev(np.asarray(radii, np.int32).reshape(N))
-    _lib.check(L.jaxsplat_bin_and_sort(ctypes.byref(d), P(x), P(dp), P(r), isect, P(gids),
-                                       P(tiles), P(bins), P(ws), wsb, stream()))
+    if split:
+        dwb = L.jaxsplat_bin_depth_workspace_size(ctypes.byref(d))
+        dws = t.full((max(dwb, 1),), 0xA5, dtype=t.uint8, device="cuda")
+        _lib.check(L.jaxsplat_bin_depth_sort(ctypes.byref(d), P(x), P(dp), P(r), P(dws), dwb,
+                                             stream()))
+        wsb = L.jaxsplat_bin_emit_workspace_size(ctypes.byref(d), isect)
+        ws = t.full((max(wsb, 1),), 0xA5, dtype=t.uint8, device="cuda")
+        _lib.check(L.jaxsplat_bin_emit_sort(ctypes.byref(d), isect, P(dws), P(gids), P(tiles),
+                                            P(bins), P(ws), wsb, stream()))
+    else:
+        wsb = L.jaxsplat_bin_workspace_size(ctypes.byref(d), isect)
+        ws = t.empty(max(wsb, 1), dtype=t.uint8, device="cuda")
+        _lib.check(L.jaxsplat_bin_and_sort(ctypes.byref(d), P(x), P(dp), P(r), isect, P(gids),
+                                           P(tiles), P(bins), P(ws), wsb, stream()))
     _lib.check(L.jaxsplat_bin_check(ctypes.byref(d), isect, P(ws), stream()))
     return host(gids)[:isect], host(tiles)[:isect], host(bins), isect
 

@@ -60,13 +60,14 @@ def test_project_fwd_vs_float64():
     assert (g["radii"] != o64["radii"]).mean() < 1e-3
 
 
+@pytest.mark.parametrize("split", [False, True])
 @pytest.mark.parametrize("name,n,bw", [("small", None, 16), ("small", None, 8),
                                        ("small", 3000, 3), ("hd1m", 200_000, 16)])
-def test_bin_sort_bitexact(name, n, bw):
+def test_bin_sort_bitexact(name, n, bw, split):
     sc, cam, kw = _setup(name, n=n, block_width=bw)
     g = H.gpu_project(sc.means3d, sc.scales, sc.quats, cam.viewmat, **kw)
     gids, tiles, bins, isect = H.gpu_bin(g["xys"], g["depths"], g["radii"], g["cum_tiles_hit"],
-                                         img_shape=cam.img_shape, block_width=bw)
+                                         img_shape=cam.img_shape, block_width=bw, split=split)
     ks, gs, bs = oracle.binning.bin_and_sort(g["xys"], g["depths"], g["radii"],
                                              g["cum_tiles_hit"], img_shape=cam.img_shape,
                                              block_width=bw)
