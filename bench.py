@@ -181,6 +181,7 @@ def stage_breakdown(torch, sc, cam, params, target, iters):
     names = ["project_fwd", "bin_sort", "raster_fwd", "loss_grad", "raster_bwd", "project_bwd"]
     acc = {k: 0.0 for k in names}
     s = _stream(dev)
+    bws = None
     for it in range(iters + 2):
         ev = [torch.cuda.Event(enable_timing=True) for _ in range(len(names) + 1)]
         ev[0].record()
@@ -196,10 +197,15 @@ def stage_breakdown(torch, sc, cam, params, target, iters):
             _p(bg), _p(fT), _p(fI), _p(img), s))
         ev[3].record()
         v_img = (2.0 / img.numel()) * (img - target)
+        bwsb = L.jaxsplat_rasterize_backward_staged_workspace_size(ctypes.byref(d), isect)
+        if bws is None or bws.numel() < bwsb:
+            bws = torch.empty(bwsb, dtype=torch.uint8, device=dev)
         ev[4].record()
-        _lib.check(L.jaxsplat_rasterize_backward_binned(
+        # the render path's backward: staged records + per-Gaussian reduction
+        _lib.check(L.jaxsplat_rasterize_backward_staged(
             ctypes.byref(d), _p(gids), _p(bins), isect, _p(xys), _p(conics), _p(colors), _p(opac),
-            _p(bg), _p(fT), _p(fI), _p(v_img), None, _p(vc), _p(vo), _p(vxy), None, _p(vcn), 0, s))
+            _p(bg), 3, _p(radii), _p(cum), _p(fT), _p(fI), _p(v_img), None, _p(vc), _p(vo),
+            _p(vxy), None, _p(vcn), 0, _p(bws), bwsb, s))
         ev[5].record()
         _lib.check(L.jaxsplat_project_backward(
             ctypes.byref(d), _p(means3d), _p(scales), _p(quats), _p(viewmat), _p(radii),
@@ -228,7 +234,9 @@ def stage_breakdown(torch, sc, cam, params, target, iters):
     bytes_bwd = 40 * L_b + 20 * Himg * Wimg + 36 * V + 8 * T
     t_bwd = acc["raster_bwd"] * 1e-3
     achieved = bytes_bwd / t_bwd / 1e9
-    roof = {"kernel": "raster_bwd_kernel", "bound": "hbm", "achieved": round(achieved, 2),
+    roof = {"kernel": "raster_bwd_kernel (staged backward: flag fill + raster_bwd_kernel + "
+                      "raster_bwd_reduce_kernel)",
+            "bound": "hbm", "achieved": round(achieved, 2),
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
             "traffic": measured_traffic(sc.name), "avg_ms": round(acc["raster_bwd"], 4),
             "bytes_per_launch": bytes_bwd}

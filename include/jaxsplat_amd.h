@@ -228,6 +228,30 @@ int jaxsplat_rasterize_backward_nd(
     float* v_colors, float* v_opacity, float* v_xy, float* v_xy_abs, float* v_conic,
     int adjoint, void* stream);
 
+/*
+ * The backward of jaxsplat_rasterize_backward_nd for up to three channels
+ * without float atomics; same adjoint as rasterize_bwd
+ * (lib/kernels/backward.cu:153-343).  Each (Gaussian, tile) pair writes its
+ * partial sums to a record at its intersection slot in the reference's
+ * emission order (cum_tiles_hit[g] - num_tiles_hit[g] + the tile's row-major
+ * offset in the Gaussian's tile rectangle, lib/kernels/forward.cu:135-179), and
+ * one lane per Gaussian sums its records in slot order.  The result does not
+ * depend on scheduling (bitwise reproducible) and every gradient row is
+ * written.  radii and cum_tiles_hit are the projection's outputs for the same
+ * xys; the workspace holds 49 bytes per intersection
+ * (jaxsplat_rasterize_backward_staged_workspace_size).
+ */
+size_t jaxsplat_rasterize_backward_staged_workspace_size(const jaxsplat_descriptor* d,
+                                                         uint32_t num_intersects);
+int jaxsplat_rasterize_backward_staged(
+    const jaxsplat_descriptor* d, const int32_t* gaussian_ids_sorted, const int32_t* tile_bins,
+    uint32_t num_intersects, const float* xys, const float* conics, const float* colors,
+    const float* opacities, const float* background, uint32_t channels, const int32_t* radii,
+    const int32_t* cum_tiles_hit, const float* final_Ts, const int32_t* final_idx,
+    const float* v_out_img, const float* v_out_img_alpha, float* v_colors, float* v_opacity,
+    float* v_xy, float* v_xy_abs, float* v_conic, int adjoint, void* workspace,
+    size_t workspace_bytes, void* stream);
+
 /* ------------------------------------------------------------------------
  * Batched rendering: many views of one or more scenes in one pass.
  *

@@ -28,6 +28,7 @@ EXPORTS = (
     "jaxsplat_bin_workspace_size", "jaxsplat_bin_and_sort", "jaxsplat_bin_check",
     "jaxsplat_bin_depth_workspace_size", "jaxsplat_bin_depth_sort",
     "jaxsplat_bin_emit_workspace_size", "jaxsplat_bin_emit_sort",
+    "jaxsplat_rasterize_backward_staged_workspace_size", "jaxsplat_rasterize_backward_staged",
     "jaxsplat_rasterize_forward_binned", "jaxsplat_rasterize_backward_binned",
     "jaxsplat_rasterize_forward_nd", "jaxsplat_rasterize_backward_nd",
     "jaxsplat_batch_create", "jaxsplat_batch_destroy", "jaxsplat_batch_workspace_size",
@@ -109,6 +110,9 @@ _SIGS = {
     "jaxsplat_bin_depth_sort": (I32, [DP, P, P, P, P, SZ, P]),
     "jaxsplat_bin_emit_workspace_size": (SZ, [DP, U32]),
     "jaxsplat_bin_emit_sort": (I32, [DP, U32, P, P, P, P, P, SZ, P]),
+    "jaxsplat_rasterize_backward_staged_workspace_size": (SZ, [DP, U32]),
+    "jaxsplat_rasterize_backward_staged": (I32, [DP, P, P, U32, P, P, P, P, P, U32, P, P, P, P, P,
+                                                  P, P, P, P, P, P, I32, P, SZ, P]),
     "jaxsplat_rasterize_forward_binned": (I32, [DP, P, P, U32] + [P] * 8 + [P]),
     "jaxsplat_rasterize_backward_binned": (I32, [DP, P, P, U32] + [P] * 14 + [I32, P]),
     "jaxsplat_rasterize_forward_nd": (I32, [DP, P, P, U32] + [P] * 5 + [U32] + [P] * 3 + [P]),

@@ -15,15 +15,24 @@ Difference in residuals (same results): the sorted Gaussian ids and tile bins
 of the forward are kept for the backward instead of sorting twice
 (``lib/ops.cu:188-198``), and cov3d is recomputed in the backward instead of
 being stored.
+
+The rasterize backward of up to three channels sums each Gaussian's
+per-tile partial gradients in a fixed order (``jaxsplat_rasterize_backward_
+staged``) instead of with float atomics, so it is bitwise reproducible.
+``STAGED_BACKWARD = False`` (or ``JAXSPLAT_AMD_STAGED_BACKWARD=0`` in the
+environment) selects the atomic backward, which differs only in rounding.
 """
 
 from __future__ import annotations
 
 import ctypes
+import os
 
 import torch
 
 from . import _lib
+
+STAGED_BACKWARD = os.environ.get("JAXSPLAT_AMD_STAGED_BACKWARD", "1") != "0"
 
 
 class ShapeDtypeError(AssertionError, TypeError):
@@ -217,13 +226,13 @@ class _Rasterize(torch.autograd.Function):
         ctx.isect = isect
         ctx.adjoint = adjoint
         ctx.save_for_backward(colors, opacities, background, xys, conics, gids, bins, final_Ts,
-                              final_idx)
+                              final_idx, radii, cum_tiles_hit)
         return img
 
     @staticmethod
     def backward(ctx, v_img):
-        colors, opacities, background, xys, conics, gids, bins, final_Ts, final_idx = \
-            ctx.saved_tensors
+        colors, opacities, background, xys, conics, gids, bins, final_Ts, final_idx, radii, \
+            cum_tiles_hit = ctx.saved_tensors
         dev = colors.device
         v_img = v_img.contiguous()
         v_colors = torch.empty_like(colors)
@@ -231,6 +240,21 @@ class _Rasterize(torch.autograd.Function):
         v_xy = torch.empty_like(xys)
         v_conic = torch.empty_like(conics)
         L = _lib.lib()
+        nch = colors.shape[1]
+        if nch <= 3 and STAGED_BACKWARD:
+            # per-intersection records summed per Gaussian: no float atomics,
+            # bitwise reproducible
+            wsb = L.jaxsplat_rasterize_backward_staged_workspace_size(ctypes.byref(ctx.d),
+                                                                      ctx.isect)
+            ws = torch.empty(max(wsb, 1), dtype=torch.uint8, device=dev)
+            with torch.cuda.device(dev):
+                _lib.check(L.jaxsplat_rasterize_backward_staged(
+                    ctypes.byref(ctx.d), _p(gids), _p(bins), ctx.isect, _p(xys), _p(conics),
+                    _p(colors), _p(opacities), _p(background), nch, _p(radii),
+                    _p(cum_tiles_hit), _p(final_Ts), _p(final_idx), _p(v_img), None,
+                    _p(v_colors), _p(v_opacity), _p(v_xy), None, _p(v_conic), ctx.adjoint,
+                    _p(ws), wsb, _stream(dev)))
+            return None, v_colors, v_opacity, None, v_xy, None, None, v_conic, None
         with torch.cuda.device(dev):
             _lib.check(L.jaxsplat_rasterize_backward_nd(
                 ctypes.byref(ctx.d), _p(gids), _p(bins), ctx.isect, _p(xys), _p(conics),

@@ -31,17 +31,6 @@
 namespace jsa {
 namespace {
 
-// forward.cu:152-161 / helpers.h:9-32, recomputed from xys and the integer radius.
-__device__ __forceinline__ void tile_rect(int block_width, int tiles_x, int tiles_y, float2 xy,
-                                          int rad, int& x0, int& x1, int& y0, int& y1) {
-    const float bwf = (float)block_width;
-    const float tcx = xy.x / bwf, tcy = xy.y / bwf, tr = (float)rad / bwf;
-    x0 = min(max(0, (int)(tcx - tr)), tiles_x);
-    x1 = min(max(0, (int)(tcx + tr + 1.f)), tiles_x);
-    y0 = min(max(0, (int)(tcy - tr)), tiles_y);
-    y1 = min(max(0, (int)(tcy + tr + 1.f)), tiles_y);
-}
-
 // Per Gaussian (id order): the depth sort key and the packed tile rectangle.
 __global__ void depth_keys_kernel(Camera c, const float* __restrict__ depths,
                                   const int* __restrict__ radii, const float2* __restrict__ xys,

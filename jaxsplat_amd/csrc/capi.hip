@@ -472,6 +472,53 @@ int jaxsplat_rasterize_backward_nd(
     return JAXSPLAT_OK;
 }
 
+size_t jaxsplat_rasterize_backward_staged_workspace_size(const jaxsplat_descriptor* d,
+                                                         uint32_t num_intersects) {
+    if (!d || validate(d)) return 0;
+    return align256(jsa::raster_bwd_staged_bytes((int)num_intersects));
+}
+
+int jaxsplat_rasterize_backward_staged(
+    const jaxsplat_descriptor* d, const int32_t* gaussian_ids_sorted, const int32_t* tile_bins,
+    uint32_t num_intersects, const float* xys, const float* conics, const float* colors,
+    const float* opacities, const float* background, uint32_t channels, const int32_t* radii,
+    const int32_t* cum_tiles_hit, const float* final_Ts, const int32_t* final_idx,
+    const float* v_out_img, const float* v_out_img_alpha, float* v_colors, float* v_opacity,
+    float* v_xy, float* v_xy_abs, float* v_conic, int adjoint, void* workspace,
+    size_t workspace_bytes, void* stream) {
+    g_err.clear();
+    if (int st = validate(d)) return st;
+    if (channels < 1 || channels > 3)
+        return fail(JAXSPLAT_ERR_INVALID, "rasterize_backward_staged: channels must be in [1, 3], "
+                    "got %u", channels);
+    if (num_intersects >= (1u << 30))
+        return fail(JAXSPLAT_ERR_CAPACITY, "num_intersects %u exceeds 2^30", num_intersects);
+    if (adjoint != JAXSPLAT_ADJOINT_REFERENCE && adjoint != JAXSPLAT_ADJOINT_CONSISTENT)
+        return fail(JAXSPLAT_ERR_INVALID, "unknown adjoint mode %d", adjoint);
+    if (!v_colors || !v_opacity || !v_xy || !v_conic)
+        return fail(JAXSPLAT_ERR_INVALID, "rasterize_backward_staged: NULL gradient pointer");
+    if (d->num_points == 0) return JAXSPLAT_OK;
+    if (!xys || !radii || !cum_tiles_hit ||
+        (num_intersects > 0 && (!gaussian_ids_sorted || !tile_bins || !conics || !colors ||
+                                !opacities || !background || !final_Ts || !final_idx ||
+                                !v_out_img)))
+        return fail(JAXSPLAT_ERR_INVALID, "rasterize_backward_staged: NULL required pointer");
+    const size_t need = align256(jsa::raster_bwd_staged_bytes((int)num_intersects));
+    if (!workspace || workspace_bytes < need)
+        return fail(JAXSPLAT_ERR_WORKSPACE,
+                    "rasterize_backward_staged: workspace too small (%zu < %zu)",
+                    workspace_bytes, need);
+    JSA_CHECK_HIP(jsa::raster_bwd_staged(camera_of(d), (int)channels, gaussian_ids_sorted,
+                                         tile_bins, (int)num_intersects, xys, conics, colors,
+                                         opacities, background, final_Ts, final_idx, v_out_img,
+                                         v_out_img_alpha, radii, cum_tiles_hit, workspace,
+                                         v_colors, v_opacity, v_xy, v_xy_abs, v_conic,
+                                         adjoint == JAXSPLAT_ADJOINT_CONSISTENT,
+                                         static_cast<hipStream_t>(stream)),
+                  "rasterize_backward_staged");
+    return JAXSPLAT_OK;
+}
+
 // ---------------------------------------------------------------------------
 // Reference-compatible custom-call targets
 // ---------------------------------------------------------------------------

@@ -122,6 +122,19 @@ __device__ __forceinline__ T wave_inclusive_sum(T v) {
     return v;
 }
 
+// Tile rectangle [x0, x1) x [y0, y1) of a Gaussian from its centre and integer
+// radius: forward.cu:152-161 / helpers.h:9-32 (the binning and the staged
+// backward must agree bit for bit, so both use this one function).
+__device__ __forceinline__ void tile_rect(int block_width, int tiles_x, int tiles_y, float2 xy,
+                                          int rad, int& x0, int& x1, int& y0, int& y1) {
+    const float bwf = (float)block_width;
+    const float tcx = xy.x / bwf, tcy = xy.y / bwf, tr = (float)rad / bwf;
+    x0 = min(max(0, (int)(tcx - tr)), tiles_x);
+    x1 = min(max(0, (int)(tcx + tr + 1.f)), tiles_x);
+    y0 = min(max(0, (int)(tcy - tr)), tiles_y);
+    y1 = min(max(0, (int)(tcy + tr + 1.f)), tiles_y);
+}
+
 // Tile rectangle [x0, x1) x [y0, y1) packed as (x0 | y0 << 16, w | h << 16),
 // w = x1 - x0, h = y1 - y0; it covers w * h tiles (tile counts below 2^16 per
 // axis).
@@ -190,6 +203,8 @@ RasterGrid single_grid(const Camera& c);
 
 // Zero `bytes` (multiple of 4) starting at p with a fill kernel (never a
 // hipMemsetAsync, so the call is safe inside captured graphs).
+// Zeroes exactly `bytes` bytes at p (4-byte aligned) with a kernel (no memset
+// node when captured in a graph).
 hipError_t fill_zero(void* p, size_t bytes, hipStream_t s);
 hipError_t fill_value_f32x3(float* dst, size_t n_pixels, const float* rgb, hipStream_t s);
 
@@ -314,6 +329,18 @@ hipError_t raster_fwd_grid(const RasterGrid& g, const int* items_sorted, const i
                            const float* xys, const float* conics, const float* colors,
                            const float* opacities, const float* backgrounds, float* final_Ts,
                            int* final_idx, float* out_img, hipStream_t s);
+// Single-view backward with staged gradients instead of float atomics (see
+// StagedOut in raster.hip): nch <= 3 channels, workspace raster_bwd_staged_bytes;
+// radii and cum_tiles_hit are the projection's.  Writes every gradient row.
+size_t raster_bwd_staged_bytes(int num_isect);
+hipError_t raster_bwd_staged(const Camera& c, int nch, const int* gids_sorted,
+                             const int* tile_bins, int num_isect, const float* xys,
+                             const float* conics, const float* colors, const float* opacities,
+                             const float* background, const float* final_Ts,
+                             const int* final_idx, const float* v_out, const float* v_out_alpha,
+                             const int* radii, const int* cum_tiles_hit, void* ws,
+                             float* v_colors, float* v_opacity, float* v_xy, float* v_xy_abs,
+                             float* v_conic, int consistent, hipStream_t s);
 hipError_t raster_bwd_grid(const RasterGrid& g, const int* items_sorted, const int* tile_bins,
                            const float* xys, const float* conics, const float* colors,
                            const float* opacities, const float* backgrounds,

@@ -618,10 +618,13 @@ __global__ __launch_bounds__(256) void project_batch_bwd_kernel(
     reinterpret_cast<float4*>(v_quat)[g] = make_float4(vq[0], vq[1], vq[2], vq[3]);
 }
 
-__global__ void fill_zero_kernel(uint32_t* __restrict__ p, size_t n) {
+// n words, then tail (0..3) bytes after them.
+__global__ void fill_zero_kernel(uint32_t* __restrict__ p, size_t n, int tail) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
          i += (size_t)gridDim.x * blockDim.x)
         p[i] = 0u;
+    if (blockIdx.x == 0 && (int)threadIdx.x < tail)
+        reinterpret_cast<uint8_t*>(p + n)[threadIdx.x] = 0;
 }
 
 __global__ void fill_rgb_kernel(float* __restrict__ dst, size_t n_pix, const float* __restrict__ rgb) {
@@ -638,9 +641,11 @@ __global__ void fill_rgb_kernel(float* __restrict__ dst, size_t n_pix, const flo
 
 hipError_t fill_zero(void* p, size_t bytes, hipStream_t s) {
     const size_t n = bytes / 4;
-    if (n == 0) return hipSuccess;
-    const int blocks = (int)((n + 255) / 256 < 2048 ? (n + 255) / 256 : 2048);
-    fill_zero_kernel<<<blocks, 256, 0, s>>>(static_cast<uint32_t*>(p), n);
+    const int tail = (int)(bytes % 4);
+    if (bytes == 0) return hipSuccess;
+    const size_t nb = (n + 255) / 256;
+    const int blocks = (int)(nb == 0 ? 1 : (nb < 2048 ? nb : 2048));
+    fill_zero_kernel<<<blocks, 256, 0, s>>>(static_cast<uint32_t*>(p), n, tail);
     return hipGetLastError();
 }
 

@@ -38,6 +38,16 @@ namespace {
 
 constexpr float kAlphaMin = 1.f / 255.f;
 constexpr float kTEps = 1e-4f;
+// The batches hold the conic scaled to base 2: (a, b, c) * (log2(e)/2,
+// log2(e), log2(e)/2), so that sigma2 = sigma * log2(e) =
+// dx (a' dx + b' dy) + c' dy^2 and exp(-sigma) = exp2(-sigma2) (v_exp_f32).
+constexpr float kLog2e = 1.44269504088896341f;
+constexpr float kHalfLog2e = 0.5f * kLog2e;
+
+// sigma * log2(e) of the offset (dx, dy) under a base-2 scaled conic.
+__device__ __forceinline__ float sigma2(float4 b, float dx, float dy) {
+    return dx * (b.x * dx + b.y * dy) + b.z * dy * dy;
+}
 
 __device__ __forceinline__ int xcd_tile(int b, int num_tiles) {
     const int per = (num_tiles + 7) / 8;
@@ -247,7 +257,7 @@ __global__ __launch_bounds__(256) void raster_fwd_kernel(
                 const float ca = conics[3 * it + 0], cb_ = conics[3 * it + 1],
                             cc = conics[3 * it + 2];
                 s_a[e] = make_float4(xy.x, xy.y, o, __int_as_float(it));
-                s_b[e] = make_float4(ca, cb_, cc, 0.f);
+                s_b[e] = make_float4(ca * kHalfLog2e, cb_ * kLog2e, cc * kHalfLog2e, 0.f);
                 s_c[e] = load_color(rg, colors, g);
                 const CullInfo ci = cull_info(xy.x, xy.y, ca, cb_, cc, o);
                 for (int G = 0; G < nwaves; ++G)
@@ -273,9 +283,9 @@ __global__ __launch_bounds__(256) void raster_fwd_kernel(
                 const float4 a = s_a[t];
                 const float4 b = s_b[t];
                 const float dx = a.x - px, dy = a.y - py;
-                const float sigma = 0.5f * (b.x * dx * dx + b.z * dy * dy) + b.y * dx * dy;
-                const float alpha = fminf(0.999f, a.z * __expf(-sigma));
-                if (done || sigma < 0.f || alpha < kAlphaMin) continue;
+                const float s2 = sigma2(b, dx, dy);
+                const float alpha = fminf(0.999f, a.z * __builtin_amdgcn_exp2f(-s2));
+                if (done || s2 < 0.f || alpha < kAlphaMin) continue;
                 const float next_T = T * (1.f - alpha);
                 if (next_T <= kTEps) {
                     done = true;
@@ -311,7 +321,8 @@ __global__ __launch_bounds__(256) void raster_fwd_kernel(
 // covers 128 pixels; 64-pixel groups = (slot, wave).
 // ---------------------------------------------------------------------------
 struct PixState {
-    float px, py, T_final, T, vo_r, vo_g, vo_b, bgdot, buf_r, buf_g, buf_b;
+    float px, py, T_final, T, vo_r, vo_g, vo_b, bgdot;
+    float bv;       // sum over the Gaussians behind of colour * alpha * T, dotted with v_out
     int fin;
     bool inside;
 };
@@ -333,24 +344,23 @@ __device__ __forceinline__ bool bwd_pixel(PixState& s, const float4 a, const flo
 #define JSA_ACC(i, v) (g[i] += (v))
     constexpr float kClamp = CONSISTENT ? 0.999f : 0.99f;
     const float dx = a.x - s.px, dy = a.y - s.py;
-    const float sigma = 0.5f * (b.x * dx * dx + b.z * dy * dy) + b.y * dx * dy;
-    const float vis = __expf(-sigma);
-    const float raw = a.z * vis;
+    const float s2 = sigma2(b, dx, dy);
+    const float raw = a.z * __builtin_amdgcn_exp2f(-s2);
     const float alpha = fminf(kClamp, raw);
-    const bool valid = s.inside && (k <= s.fin) && sigma >= 0.f && alpha >= kAlphaMin;
-    const float ra = __builtin_amdgcn_rcpf(1.f - alpha);
-    const float T = valid ? s.T * ra : s.T;      // transmittance in front of this Gaussian
+    const bool valid = s.inside && (k <= s.fin) && s2 >= 0.f && alpha >= kAlphaMin;
+    // an invalid pair acts as alpha = 0: T unchanged (1 / (1 - 0) is exact), no weight
+    const float al = valid ? alpha : 0.f;
+    const float ra = __builtin_amdgcn_rcpf(1.f - al);
+    const float T = s.T * ra;                    // transmittance in front of this Gaussian
     s.T = T;
-    const float fac = valid ? alpha * T : 0.f;
+    const float fac = al * T;
     JSA_ACC(0, fac * s.vo_r);
     JSA_ACC(1, fac * s.vo_g);
     JSA_ACC(2, fac * s.vo_b);
-    const float v_alpha = (col.x * T - s.buf_r * ra) * s.vo_r +
-                          (col.y * T - s.buf_g * ra) * s.vo_g +
-                          (col.z * T - s.buf_b * ra) * s.vo_b + ra * s.bgdot;
-    s.buf_r += col.x * fac;
-    s.buf_g += col.y * fac;
-    s.buf_b += col.z * fac;
+    // v_alpha = sum_ch (col T - buf ra) v_out + ra bgdot with buf . v_out kept as bv
+    const float cv = col.x * s.vo_r + col.y * s.vo_g + col.z * s.vo_b;
+    const float v_alpha = T * cv - ra * (s.bv - s.bgdot);
+    s.bv += fac * cv;
     const bool live = valid && !(CONSISTENT && raw >= kClamp);
     const float vs = live ? -raw * v_alpha : 0.f;
     const float p = vs * dx, q = vs * dy;
@@ -359,29 +369,53 @@ __device__ __forceinline__ bool bwd_pixel(PixState& s, const float4 a, const flo
     JSA_ACC(5, q * dy);
     JSA_ACC(6, p);
     JSA_ACC(7, q);
-    if (NV == 11) {
-        JSA_ACC(8, fabsf(b.x * p + b.y * q));
-        JSA_ACC(9, fabsf(b.y * p + b.z * q));
+    if (NV == 11) {     // log2(e) |conic (p, q)|, rescaled by finish_grads
+        JSA_ACC(8, fabsf(2.f * b.x * p + b.y * q));
+        JSA_ACC(9, fabsf(b.y * p + 2.f * b.z * q));
     }
     JSA_ACC(NV - 1, vs);
 #undef JSA_ACC
     return valid;
 }
 
-// Per-Gaussian factors of the partial sums of bwd_pixel (conic (a, b, c) and
-// opacity o of the Gaussian): r[] becomes (v_rgb 3, v_conic 3, v_xy 2,
-// [v_xy_abs 2], v_opacity).  o > 0 whenever a pair was valid (alpha >= 1/255).
+// Per-Gaussian factors of the partial sums of bwd_pixel (the batch's base-2
+// scaled conic and opacity o of the Gaussian): r[] becomes (v_rgb 3, v_conic 3,
+// v_xy 2, [v_xy_abs 2], v_opacity).  o > 0 whenever a pair was valid
+// (alpha >= 1/255).
 template <int NV>
-__device__ __forceinline__ void finish_grads(float* r, float4 conic, float o) {
+__device__ __forceinline__ void finish_grads(float* r, float4 conic2, float o) {
+    constexpr float kInvLog2e = 1.f / kLog2e;
+    const float ca = conic2.x * (2.f * kInvLog2e), cb = conic2.y * kInvLog2e,
+                cc = conic2.z * (2.f * kInvLog2e);
     r[3] *= 0.5f;
     r[5] *= 0.5f;
     const float sp = r[6], sq = r[7];
-    r[6] = conic.x * sp + conic.y * sq;
-    r[7] = conic.y * sp + conic.z * sq;
+    r[6] = ca * sp + cb * sq;
+    r[7] = cb * sp + cc * sq;
+    if (NV == 11) {
+        r[8] *= kInvLog2e;
+        r[9] *= kInvLog2e;
+    }
     r[NV - 1] = o > 0.f ? -r[NV - 1] / o : 0.f;
 }
 
-template <bool CONSISTENT, bool ABS, int DEPTH>
+// Staged gradient output of a single-view backward (no float atomics).  The
+// flush of (Gaussian g, tile) writes its sums as one 48-byte record at the
+// pair's intersection slot in the reference's emission order,
+// cum_tiles_hit[g] - num_tiles_hit[g] + the tile's row-major offset in g's
+// tile rectangle, and sets its flag; raster_bwd_reduce_kernel then sums each
+// Gaussian's flagged records in slot order, so the result does not depend on
+// the order in which tiles finish.  Record: (rgb 3, conic a), (conic b, c,
+// xy 2), (opacity, xy_abs 2, -).
+struct StagedOut {
+    float4* rec;            // 3 float4 per slot
+    uint8_t* flag;          // one byte per slot, zero before the backward
+    const int* cum;         // cum_tiles_hit
+    const int* radii;
+    int n_slots;            // number of intersections; a slot outside is not written
+};
+
+template <bool CONSISTENT, bool ABS, int DEPTH, bool STAGED>
 __global__ __launch_bounds__(128) void raster_bwd_kernel(
     RasterGrid rg, const int* __restrict__ gids, const int2* __restrict__ bins,
     const float2* __restrict__ xys, const float* __restrict__ conics,
@@ -390,13 +424,13 @@ __global__ __launch_bounds__(128) void raster_bwd_kernel(
     const int* __restrict__ final_idx, const float* __restrict__ v_out_img,
     const float* __restrict__ v_out_alpha, float* __restrict__ v_rgb,
     float* __restrict__ v_opacity, float* __restrict__ v_xy, float* __restrict__ v_xy_abs,
-    float* __restrict__ v_conic) {
+    float* __restrict__ v_conic, StagedOut st) {
     constexpr int NV = ABS ? 11 : 9;     // rgb3, conic3, xy2, [xy_abs2], opacity
     constexpr int NW = DEPTH / 64;       // 64-entry words per batch
     __shared__ float4 s_a[DEPTH];        // x, y, opacity, id
     __shared__ float4 s_b[DEPTH];        // conic a, b, c
     __shared__ float4 s_c[DEPTH];        // colour
-    __shared__ float s_g[DEPTH][NV][4];  // per 16-lane row partial sums
+    __shared__ float s_g[DEPTH][2][NV];  // per-wave sums [entry][wave][value]
     __shared__ int s_flag[DEPTH];
     __shared__ uint64_t s_mask[4][NW];   // [group = slot * nwaves + wave][64-entry word]
     __shared__ float4 s_rect[4];
@@ -445,7 +479,7 @@ __global__ __launch_bounds__(128) void raster_bwd_kernel(
         const float vo_a = (s.inside && v_out_alpha) ? v_out_alpha[pix] : 0.f;
         // background term of v_alpha: T_final * ra * (v_out_alpha - bg . v_out)
         s.bgdot = s.T_final * (vo_a - (bgr * s.vo_r + bgg * s.vo_g + bgb * s.vo_b));
-        s.buf_r = s.buf_g = s.buf_b = 0.f;
+        s.bv = 0.f;
         slot_fin[sl] = uniform_i(wave_max_i(s.fin));
     }
     if (tid < ngroups) {
@@ -456,8 +490,10 @@ __global__ __launch_bounds__(128) void raster_bwd_kernel(
     }
     for (int e = tid; e < DEPTH; e += nthr) {
 #pragma unroll
-        for (int v = 0; v < NV; ++v)
-            *reinterpret_cast<float4*>(&s_g[e][v][0]) = make_float4(0.f, 0.f, 0.f, 0.f);
+        for (int v = 0; v < NV; ++v) {
+            s_g[e][0][v] = 0.f;
+            s_g[e][1][v] = 0.f;
+        }
         s_flag[e] = 0;
     }
     const int wave_fin = max(slot_fin[0], slot_fin[1]);
@@ -467,7 +503,9 @@ __global__ __launch_bounds__(128) void raster_bwd_kernel(
     for (int w = 0; w < nwaves; ++w) block_fin = max(block_fin, s_wmax[w]);
 
     const int2 range = bins[vr.bin_off + tile];
-    const int last = min(range.y - 1, block_fin);
+    // wave-uniform (block_fin comes from LDS), so that the batch bounds and the
+    // culling masks below stay in scalar registers
+    const int last = uniform_i(min(range.y - 1, block_fin));
     for (int bend = last; bend >= range.x; bend -= DEPTH) {
         // stage entries e = e0 + tid (sorted index bend - e, back to front)
         for (int e0 = 0; e0 < DEPTH; e0 += nthr) {
@@ -482,7 +520,19 @@ __global__ __launch_bounds__(128) void raster_bwd_kernel(
                 const float ca = conics[3 * it + 0], cb_ = conics[3 * it + 1],
                             cc = conics[3 * it + 2];
                 s_a[e] = make_float4(xy.x, xy.y, o, __int_as_float(it));
-                s_b[e] = make_float4(ca, cb_, cc, 0.f);
+                int slot = -1;
+                if (STAGED) {
+                    int x0, x1, y0, y1;
+                    tile_rect(bw, vr.tiles_x, vr.tiles_y, xy, st.radii[it], x0, x1, y0, y1);
+                    const int w = x1 - x0, lx = tx - x0, ly = ty - y0;
+                    const int sl = st.cum[it] - w * (y1 - y0) + ly * w + lx;
+                    // radii / cum_tiles_hit inconsistent with the binning: no slot
+                    if (lx >= 0 && lx < w && ly >= 0 && ly < y1 - y0 && sl >= 0 &&
+                        sl < st.n_slots)
+                        slot = sl;
+                }
+                s_b[e] = make_float4(ca * kHalfLog2e, cb_ * kLog2e, cc * kHalfLog2e,
+                                     __int_as_float(slot));
                 s_c[e] = load_color(rg, colors, g);
                 const CullInfo ci = cull_info(xy.x, xy.y, ca, cb_, cc, o);
                 for (int G = 0; G < ngroups; ++G)
@@ -521,12 +571,31 @@ __global__ __launch_bounds__(128) void raster_bwd_kernel(
                 if (ballot(any) == 0) continue;
                 // 16-lane row sums; each row leader adds into its own LDS slot
 #pragma unroll
-                for (int v = 0; v < NV; ++v) g[v] = row_sum16(g[v]);
-                if ((lane & 15) == 15) {
-#pragma unroll
-                    for (int v = 0; v < NV; ++v) atomicAdd(&s_g[t][v][row], g[v]);
-                    if (row == 3) s_flag[t] = 1;
+                for (int v = 0; v < NV; ++v) {
+                    g[v] = row_sum16(g[v]);
+                    // complete the sums in every lane here: otherwise the last
+                    // DPP add is sunk into the row leaders' branch below as a
+                    // zeroed move, a DPP move and an add
+                    asm volatile("" : "+v"(g[v]));
                 }
+                // lane j of every row takes value j, then the four rows are summed
+                // with the row swaps; lanes 0..NV-1 store the wave's sums
+                const int j = lane & 15;
+                float val = g[0];
+#pragma unroll
+                for (int v = 1; v < NV; ++v) val = (j == v) ? g[v] : val;
+                {
+                    const auto r = __builtin_amdgcn_permlane16_swap(
+                        __float_as_uint(val), __float_as_uint(val), false, false);
+                    val = __uint_as_float(r[0]) + __uint_as_float(r[1]);
+                }
+                {
+                    const auto r = __builtin_amdgcn_permlane32_swap(
+                        __float_as_uint(val), __float_as_uint(val), false, false);
+                    val = __uint_as_float(r[0]) + __uint_as_float(r[1]);
+                }
+                if (lane < NV) s_g[t][wave][lane] = val;
+                if (lane == 0) s_flag[t] = 1;
             }
         }
         __syncthreads();
@@ -535,12 +604,22 @@ __global__ __launch_bounds__(128) void raster_bwd_kernel(
             float r[NV];
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
-                const float4 p4 = *reinterpret_cast<const float4*>(&s_g[e][v][0]);
-                r[v] = (p4.x + p4.y) + (p4.z + p4.w);
-                *reinterpret_cast<float4*>(&s_g[e][v][0]) = make_float4(0.f, 0.f, 0.f, 0.f);
+                r[v] = s_g[e][0][v] + s_g[e][1][v];
+                s_g[e][0][v] = 0.f;
+                s_g[e][1][v] = 0.f;
             }
             s_flag[e] = 0;
             finish_grads<NV>(r, s_b[e], s_a[e].z);
+            if (STAGED) {
+                const int slot = __float_as_int(s_b[e].w);
+                if (slot < 0) continue;
+                st.rec[3 * (size_t)slot + 0] = make_float4(r[0], r[1], r[2], r[3]);
+                st.rec[3 * (size_t)slot + 1] = make_float4(r[4], r[5], r[6], r[7]);
+                st.rec[3 * (size_t)slot + 2] =
+                    make_float4(r[NV - 1], ABS ? r[8] : 0.f, ABS ? r[9] : 0.f, 0.f);
+                st.flag[slot] = 1;
+                continue;
+            }
             const int it = __float_as_int(s_a[e].w);
             const int gid = it + gshift;
             float* vc = v_rgb + (size_t)rg.ch_stride * gid + rg.ch_off;
@@ -559,6 +638,54 @@ __global__ __launch_bounds__(128) void raster_bwd_kernel(
             unsafeAtomicAdd(v_opacity + gid, r[NV - 1]);
         }
         __syncthreads();
+    }
+}
+
+// Per Gaussian: the sum of its flagged staged records, in slot order, as the
+// final gradients (every output row is written; a Gaussian with no flagged
+// record gets zeros).  num_isect bounds the slots (a radius or cum_tiles_hit
+// inconsistent with it yields zeros rather than an out-of-range read).
+template <bool ABS>
+__global__ __launch_bounds__(256) void raster_bwd_reduce_kernel(
+    Camera c, int num_isect, const float2* __restrict__ xys, const int* __restrict__ radii,
+    const int* __restrict__ cum, const float4* __restrict__ rec,
+    const uint8_t* __restrict__ flag, int nch, float* __restrict__ v_rgb,
+    float* __restrict__ v_opacity, float* __restrict__ v_xy, float* __restrict__ v_xy_abs,
+    float* __restrict__ v_conic) {
+    const int g = blockIdx.x * 256 + threadIdx.x;
+    if (g >= c.num_points) return;
+    float4 a0 = make_float4(0.f, 0.f, 0.f, 0.f), a1 = a0, a2 = a0;
+    const int rad = radii[g];
+    if (rad > 0) {
+        int x0, x1, y0, y1;
+        tile_rect(c.block_width, c.tiles_x, c.tiles_y, xys[g], rad, x0, x1, y0, y1);
+        const int cnt = (x1 - x0) * (y1 - y0);
+        const int end = cum[g];
+        const int off = end - cnt;
+        if (off >= 0 && end <= num_isect) {
+            for (int k = off; k < end; ++k) {
+                if (!flag[k]) continue;
+                const float4 r0 = rec[3 * (size_t)k + 0], r1 = rec[3 * (size_t)k + 1],
+                             r2 = rec[3 * (size_t)k + 2];
+                a0.x += r0.x; a0.y += r0.y; a0.z += r0.z; a0.w += r0.w;
+                a1.x += r1.x; a1.y += r1.y; a1.z += r1.z; a1.w += r1.w;
+                a2.x += r2.x; a2.y += r2.y; a2.z += r2.z;
+            }
+        }
+    }
+    float* vc = v_rgb + (size_t)nch * g;
+    vc[0] = a0.x;
+    if (nch > 1) vc[1] = a0.y;
+    if (nch > 2) vc[2] = a0.z;
+    v_conic[3 * (size_t)g + 0] = a0.w;
+    v_conic[3 * (size_t)g + 1] = a1.x;
+    v_conic[3 * (size_t)g + 2] = a1.y;
+    v_xy[2 * (size_t)g + 0] = a1.z;
+    v_xy[2 * (size_t)g + 1] = a1.w;
+    v_opacity[g] = a2.x;
+    if (ABS) {
+        v_xy_abs[2 * (size_t)g + 0] = a2.y;
+        v_xy_abs[2 * (size_t)g + 1] = a2.z;
     }
 }
 
@@ -700,9 +827,9 @@ hipError_t raster_bwd_grid(const RasterGrid& g, const int* items_sorted, const i
     const int2* b2 = reinterpret_cast<const int2*>(tile_bins);
     const float2* x2 = reinterpret_cast<const float2*>(xys);
 #define JSA_LAUNCH_D(CONS, ABS, D)                                                           \
-    raster_bwd_kernel<CONS, ABS, D><<<grid, nthr, 0, s>>>(                                  \
+    raster_bwd_kernel<CONS, ABS, D, false><<<grid, nthr, 0, s>>>(                           \
         g, gids_sorted, b2, x2, conics, colors, opacities, background, final_Ts, final_idx,  \
-        v_out, v_out_alpha, v_colors, v_opacity, v_xy, v_xy_abs, v_conic)
+        v_out, v_out_alpha, v_colors, v_opacity, v_xy, v_xy_abs, v_conic, StagedOut{})
 #define JSA_LAUNCH(CONS, ABS)                                                                \
     do {                                                                                     \
         if (depth == 64) JSA_LAUNCH_D(CONS, ABS, 64);                                        \
@@ -716,6 +843,65 @@ hipError_t raster_bwd_grid(const RasterGrid& g, const int* items_sorted, const i
     }
 #undef JSA_LAUNCH
 #undef JSA_LAUNCH_D
+    return hipGetLastError();
+}
+
+size_t raster_bwd_staged_bytes(int num_isect) {
+    const size_t n = (size_t)(num_isect > 0 ? num_isect : 1);
+    return ((48 * n + 255) & ~size_t(255)) + ((n + 255) & ~size_t(255));
+}
+
+hipError_t raster_bwd_staged(const Camera& c, int nch, const int* gids_sorted,
+                             const int* tile_bins, int num_isect, const float* xys,
+                             const float* conics, const float* colors, const float* opacities,
+                             const float* background, const float* final_Ts,
+                             const int* final_idx, const float* v_out, const float* v_out_alpha,
+                             const int* radii, const int* cum_tiles_hit, void* ws,
+                             float* v_colors, float* v_opacity, float* v_xy, float* v_xy_abs,
+                             float* v_conic, int consistent, hipStream_t s) {
+    if (c.num_points <= 0) return hipSuccess;
+    if (nch < 1 || nch > 3) return hipErrorInvalidValue;
+    uint8_t* w = static_cast<uint8_t*>(ws);
+    const size_t n = (size_t)(num_isect > 0 ? num_isect : 1);
+    StagedOut st;
+    st.rec = reinterpret_cast<float4*>(w);
+    st.flag = w + ((48 * n + 255) & ~size_t(255));
+    st.cum = cum_tiles_hit;
+    st.radii = radii;
+    st.n_slots = num_isect;
+    hipError_t e = hipSuccess;
+    if (num_isect > 0) {
+        if ((e = fill_zero(st.flag, (size_t)num_isect, s)) != hipSuccess) return e;
+        RasterGrid g = single_grid(c);
+        g.ch_stride = nch;
+        g.ch_n = nch;
+        const int half = (g.block_width * g.block_width + 1) / 2;
+        const int nthr = ((half + 63) / 64) * 64;
+        const int grid = tile_grid(g.total_tiles);
+        const int2* b2 = reinterpret_cast<const int2*>(tile_bins);
+        const float2* x2 = reinterpret_cast<const float2*>(xys);
+#define JSA_LAUNCH(CONS, ABS)                                                                \
+    raster_bwd_kernel<CONS, ABS, 64, true><<<grid, nthr, 0, s>>>(                           \
+        g, gids_sorted, b2, x2, conics, colors, opacities, background, final_Ts, final_idx,  \
+        v_out, v_out_alpha, nullptr, nullptr, nullptr, nullptr, nullptr, st)
+        if (consistent) {
+            if (v_xy_abs) JSA_LAUNCH(true, true); else JSA_LAUNCH(true, false);
+        } else {
+            if (v_xy_abs) JSA_LAUNCH(false, true); else JSA_LAUNCH(false, false);
+        }
+#undef JSA_LAUNCH
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    const int blocks = (c.num_points + 255) / 256;
+    const float2* x2 = reinterpret_cast<const float2*>(xys);
+    if (v_xy_abs)
+        raster_bwd_reduce_kernel<true><<<blocks, 256, 0, s>>>(
+            c, num_isect, x2, radii, cum_tiles_hit, st.rec, st.flag, nch, v_colors, v_opacity,
+            v_xy, v_xy_abs, v_conic);
+    else
+        raster_bwd_reduce_kernel<false><<<blocks, 256, 0, s>>>(
+            c, num_isect, x2, radii, cum_tiles_hit, st.rec, st.flag, nch, v_colors, v_opacity,
+            v_xy, nullptr, v_conic);
     return hipGetLastError();
 }
 
