@@ -238,7 +238,7 @@ int jaxsplat_rasterize_backward_nd(
  * one lane per Gaussian sums its records in slot order.  The result does not
  * depend on scheduling (bitwise reproducible) and every gradient row is
  * written.  radii and cum_tiles_hit are the projection's outputs for the same
- * xys; the workspace holds 49 bytes per intersection
+ * xys; the workspace holds a 48-byte record and one bit per intersection
  * (jaxsplat_rasterize_backward_staged_workspace_size).
  */
 size_t jaxsplat_rasterize_backward_staged_workspace_size(const jaxsplat_descriptor* d,

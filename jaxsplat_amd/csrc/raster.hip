@@ -403,13 +403,14 @@ __device__ __forceinline__ void finish_grads(float* r, float4 conic2, float o) {
 // flush of (Gaussian g, tile) writes its sums as one 48-byte record at the
 // pair's intersection slot in the reference's emission order,
 // cum_tiles_hit[g] - num_tiles_hit[g] + the tile's row-major offset in g's
-// tile rectangle, and sets its flag; raster_bwd_reduce_kernel then sums each
-// Gaussian's flagged records in slot order, so the result does not depend on
-// the order in which tiles finish.  Record: (rgb 3, conic a), (conic b, c,
+// tile rectangle, and sets the slot's bit in a bitmap (one 32-bit atomic OR
+// per flush, where the direct path adds nine floats atomically);
+// raster_bwd_reduce_kernel then sums each Gaussian's flagged records in slot
+// order, so the result does not depend on the order in which tiles finish.  Record: (rgb 3, conic a), (conic b, c,
 // xy 2), (opacity, xy_abs 2, -).
 struct StagedOut {
     float4* rec;            // 3 float4 per slot
-    uint8_t* flag;          // one byte per slot, zero before the backward
+    uint32_t* bits;         // one bit per slot, zero before the backward
     const int* cum;         // cum_tiles_hit
     const int* radii;
     int n_slots;            // number of intersections; a slot outside is not written
@@ -617,7 +618,7 @@ __global__ __launch_bounds__(128) void raster_bwd_kernel(
                 st.rec[3 * (size_t)slot + 1] = make_float4(r[4], r[5], r[6], r[7]);
                 st.rec[3 * (size_t)slot + 2] =
                     make_float4(r[NV - 1], ABS ? r[8] : 0.f, ABS ? r[9] : 0.f, 0.f);
-                st.flag[slot] = 1;
+                atomicOr(&st.bits[slot >> 5], 1u << (slot & 31));
                 continue;
             }
             const int it = __float_as_int(s_a[e].w);
@@ -649,30 +650,112 @@ template <bool ABS>
 __global__ __launch_bounds__(256) void raster_bwd_reduce_kernel(
     Camera c, int num_isect, const float2* __restrict__ xys, const int* __restrict__ radii,
     const int* __restrict__ cum, const float4* __restrict__ rec,
-    const uint8_t* __restrict__ flag, int nch, float* __restrict__ v_rgb,
+    const uint32_t* __restrict__ bits, int nch, float* __restrict__ v_rgb,
     float* __restrict__ v_opacity, float* __restrict__ v_xy, float* __restrict__ v_xy_abs,
     float* __restrict__ v_conic) {
+    // no early exit: the wave-wide pass below needs every lane
     const int g = blockIdx.x * 256 + threadIdx.x;
-    if (g >= c.num_points) return;
+    const int lane = lane_id();
+    const bool own = g < c.num_points;
     float4 a0 = make_float4(0.f, 0.f, 0.f, 0.f), a1 = a0, a2 = a0;
-    const int rad = radii[g];
-    if (rad > 0) {
+    int off = 0, end = 0;
+    if (own && radii[g] > 0) {
         int x0, x1, y0, y1;
-        tile_rect(c.block_width, c.tiles_x, c.tiles_y, xys[g], rad, x0, x1, y0, y1);
-        const int cnt = (x1 - x0) * (y1 - y0);
-        const int end = cum[g];
-        const int off = end - cnt;
-        if (off >= 0 && end <= num_isect) {
-            for (int k = off; k < end; ++k) {
-                if (!flag[k]) continue;
-                const float4 r0 = rec[3 * (size_t)k + 0], r1 = rec[3 * (size_t)k + 1],
-                             r2 = rec[3 * (size_t)k + 2];
-                a0.x += r0.x; a0.y += r0.y; a0.z += r0.z; a0.w += r0.w;
-                a1.x += r1.x; a1.y += r1.y; a1.z += r1.z; a1.w += r1.w;
-                a2.x += r2.x; a2.y += r2.y; a2.z += r2.z;
+        tile_rect(c.block_width, c.tiles_x, c.tiles_y, xys[g], radii[g], x0, x1, y0, y1);
+        end = cum[g];
+        off = end - (x1 - x0) * (y1 - y0);
+        if (off < 0 || end > num_isect) off = end = 0;     // inconsistent input: zeros
+    }
+    // more than kBig slots: summed by the whole wave below, so that one large
+    // Gaussian does not hold its wave for tens of rounds
+    constexpr int kBig = 32;
+    const bool big = end - off > kBig;
+    {
+        const int cnt = end - off;
+        if (cnt > 0 && !big) {
+            // the bitmap words covering [off, end), four independent loads per
+            // step (bits outside the range masked off); the set bits are taken
+            // in slot order, four records per round with their loads in flight
+            // together (a missing fourth reads record 0 and is discarded)
+            const int wfirst = off >> 5, wlast = (end - 1) >> 5;
+            const uint32_t lo_mask = ~0u << (off & 31);
+            const int hb = (end - 1) & 31;
+            const uint32_t hi_mask = hb == 31 ? ~0u : ((2u << hb) - 1u);
+            for (int w0 = wfirst; w0 <= wlast; w0 += 4) {
+                uint32_t W[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int w = w0 + q;
+                    uint32_t v = w <= wlast ? bits[w] : 0u;
+                    if (w == wfirst) v &= lo_mask;
+                    if (w == wlast) v &= hi_mask;
+                    W[q] = v;
+                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    uint32_t v = W[q];
+                    const int kb = (w0 + q) << 5;
+                    while (v) {
+                        int k[4];
+                        bool ok[4];
+#pragma unroll
+                        for (int i = 0; i < 4; ++i) {
+                            ok[i] = v != 0u;
+                            k[i] = ok[i] ? kb + __ffs(v) - 1 : 0;
+                            v &= v - 1u;
+                        }
+                        float4 r[4][3];
+#pragma unroll
+                        for (int i = 0; i < 4; ++i) {
+                            r[i][0] = rec[3 * (size_t)k[i] + 0];
+                            r[i][1] = rec[3 * (size_t)k[i] + 1];
+                            r[i][2] = rec[3 * (size_t)k[i] + 2];
+                        }
+#pragma unroll
+                        for (int i = 0; i < 4; ++i) {
+                            const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+                            const float4 q0 = ok[i] ? r[i][0] : z, q1 = ok[i] ? r[i][1] : z,
+                                         q2 = ok[i] ? r[i][2] : z;
+                            a0.x += q0.x; a0.y += q0.y; a0.z += q0.z; a0.w += q0.w;
+                            a1.x += q1.x; a1.y += q1.y; a1.z += q1.z; a1.w += q1.w;
+                            a2.x += q2.x; a2.y += q2.y; a2.z += q2.z;
+                        }
+                    }
+                }
             }
         }
     }
+    // large Gaussians, one at a time over the wave: lane l takes slots
+    // off + l, off + l + 64, ... and the eleven sums are combined in a fixed
+    // tree (results stay independent of scheduling)
+    uint64_t bigm = ballot(big);
+    while (bigm) {
+        const int src = __ffsll((unsigned long long)bigm) - 1;
+        bigm &= bigm - 1;
+        const int boff = __builtin_amdgcn_readlane(off, src);
+        const int bend = __builtin_amdgcn_readlane(end, src);
+        float4 b0 = make_float4(0.f, 0.f, 0.f, 0.f), b1 = b0, b2 = b0;
+        for (int k = boff + lane; k < bend; k += 64) {
+            if (!((bits[k >> 5] >> (k & 31)) & 1u)) continue;
+            const float4 r0 = rec[3 * (size_t)k + 0], r1 = rec[3 * (size_t)k + 1],
+                         r2 = rec[3 * (size_t)k + 2];
+            b0.x += r0.x; b0.y += r0.y; b0.z += r0.z; b0.w += r0.w;
+            b1.x += r1.x; b1.y += r1.y; b1.z += r1.z; b1.w += r1.w;
+            b2.x += r2.x; b2.y += r2.y; b2.z += r2.z;
+        }
+        float v[11] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w, b2.x, b2.y, b2.z};
+#pragma unroll
+        for (int i = 0; i < 11; ++i) {
+            if (!ABS && i >= 9) continue;    // xy_abs sums only with ABS
+            v[i] = wave_sum(v[i]);
+        }
+        if (lane == src) {
+            a0 = make_float4(v[0], v[1], v[2], v[3]);
+            a1 = make_float4(v[4], v[5], v[6], v[7]);
+            a2 = make_float4(v[8], ABS ? v[9] : 0.f, ABS ? v[10] : 0.f, 0.f);
+        }
+    }
+    if (!own) return;
     float* vc = v_rgb + (size_t)nch * g;
     vc[0] = a0.x;
     if (nch > 1) vc[1] = a0.y;
@@ -848,7 +931,7 @@ hipError_t raster_bwd_grid(const RasterGrid& g, const int* items_sorted, const i
 
 size_t raster_bwd_staged_bytes(int num_isect) {
     const size_t n = (size_t)(num_isect > 0 ? num_isect : 1);
-    return ((48 * n + 255) & ~size_t(255)) + ((n + 255) & ~size_t(255));
+    return ((48 * n + 255) & ~size_t(255)) + ((4 * ((n + 31) / 32) + 255) & ~size_t(255));
 }
 
 hipError_t raster_bwd_staged(const Camera& c, int nch, const int* gids_sorted,
@@ -865,13 +948,14 @@ hipError_t raster_bwd_staged(const Camera& c, int nch, const int* gids_sorted,
     const size_t n = (size_t)(num_isect > 0 ? num_isect : 1);
     StagedOut st;
     st.rec = reinterpret_cast<float4*>(w);
-    st.flag = w + ((48 * n + 255) & ~size_t(255));
+    st.bits = reinterpret_cast<uint32_t*>(w + ((48 * n + 255) & ~size_t(255)));
     st.cum = cum_tiles_hit;
     st.radii = radii;
     st.n_slots = num_isect;
     hipError_t e = hipSuccess;
     if (num_isect > 0) {
-        if ((e = fill_zero(st.flag, (size_t)num_isect, s)) != hipSuccess) return e;
+        const size_t words = ((size_t)num_isect + 31) / 32;
+        if ((e = fill_zero(st.bits, 4 * words, s)) != hipSuccess) return e;
         RasterGrid g = single_grid(c);
         g.ch_stride = nch;
         g.ch_n = nch;
@@ -896,11 +980,11 @@ hipError_t raster_bwd_staged(const Camera& c, int nch, const int* gids_sorted,
     const float2* x2 = reinterpret_cast<const float2*>(xys);
     if (v_xy_abs)
         raster_bwd_reduce_kernel<true><<<blocks, 256, 0, s>>>(
-            c, num_isect, x2, radii, cum_tiles_hit, st.rec, st.flag, nch, v_colors, v_opacity,
+            c, num_isect, x2, radii, cum_tiles_hit, st.rec, st.bits, nch, v_colors, v_opacity,
             v_xy, v_xy_abs, v_conic);
     else
         raster_bwd_reduce_kernel<false><<<blocks, 256, 0, s>>>(
-            c, num_isect, x2, radii, cum_tiles_hit, st.rec, st.flag, nch, v_colors, v_opacity,
+            c, num_isect, x2, radii, cum_tiles_hit, st.rec, st.bits, nch, v_colors, v_opacity,
             v_xy, nullptr, v_conic);
     return hipGetLastError();
 }
