@@ -270,14 +270,14 @@ __global__ __launch_bounds__(256) void raster_fwd_kernel(
             }
         }
         __syncthreads();
-        bool live = true;
+        // Branch-free per pixel: the decisions are lane masks, the colour is
+        // read and blended under one uniform branch when some lane adds, and
+        // the wave's exit test runs only after some lane terminated (keeps the
+        // scalar unit from issuing exec-mask updates for every evaluation).
+        bool live = ballot(!done) != 0;
         for (int q = 0; q < NW && live; ++q) {
             uint64_t m = uniform64(s_mask[wave][q]);
             while (m) {
-                if (ballot(!done) == 0) {
-                    live = false;
-                    break;
-                }
                 const int t = q * 64 + __ffsll((unsigned long long)m) - 1;
                 m &= m - 1;
                 const float4 a = s_a[t];
@@ -285,19 +285,26 @@ __global__ __launch_bounds__(256) void raster_fwd_kernel(
                 const float dx = a.x - px, dy = a.y - py;
                 const float s2 = sigma2(b, dx, dy);
                 const float alpha = fminf(0.999f, a.z * __builtin_amdgcn_exp2f(-s2));
-                if (done || s2 < 0.f || alpha < kAlphaMin) continue;
+                const bool ok = !done && s2 >= 0.f && alpha >= kAlphaMin;
                 const float next_T = T * (1.f - alpha);
-                if (next_T <= kTEps) {
-                    done = true;
-                    continue;
+                const bool fin = ok && next_T <= kTEps;
+                const bool add = ok && !fin;
+                if (ballot(add)) {
+                    const float4 col = s_c[t];
+                    const float vis = add ? alpha * T : 0.f;
+                    cr += col.x * vis;
+                    cg += col.y * vis;
+                    cb += col.z * vis;
+                    T = add ? next_T : T;
+                    cur = add ? bstart + t : cur;
                 }
-                const float vis = alpha * T;
-                const float4 col = s_c[t];
-                cr += col.x * vis;
-                cg += col.y * vis;
-                cb += col.z * vis;
-                T = next_T;
-                cur = bstart + t;
+                if (ballot(fin)) {
+                    done = done || fin;
+                    if (ballot(!done) == 0) {
+                        live = false;
+                        break;
+                    }
+                }
             }
         }
         __syncthreads();
