@@ -377,7 +377,7 @@ def batched_vs_plain(torch, name, steps, warmup=2):
 
 # the rasterize backward of the render path: its record kernel and per-Gaussian
 # reduction (the bitmap fill is a 1 MB memset at 1080p)
-STAGED_BWD = "raster_bwd_kernel<*, true> + raster_bwd_reduce_kernel"
+STAGED_BWD = "raster_bwd_kernel<*, *, *, true, *> + raster_bwd_reduce_kernel"
 
 
 def measured_traffic(workload):
