@@ -337,11 +337,52 @@ __global__ __launch_bounds__(256) void raster_fwd_kernel(
 // Backward: NSLOT pixels per lane (slots); 64-pixel groups = (slot, wave).
 // ---------------------------------------------------------------------------
 struct PixState {
-    float px, py, T_final, T, vo_r, vo_g, vo_b, bgdot;
-    float bv;       // sum over the Gaussians behind of colour * alpha * T, dotted with v_out
+    float px, py, T_final, T, vo_r, vo_g, vo_b;
+    // (sum over the Gaussians behind of colour * alpha * T, dotted with v_out)
+    // - T_final (v_out_alpha - background . v_out): the only form the adjoint uses
+    float bv;
     int fin;
     bool inside;
 };
+
+// Row (16-lane) sums of v[0..NV) by recursive halving.  At each DPP level
+// (partner lanes l^1, l^2, l+-4, l+-8 within the row: each flips one lane bit
+// and keeps the lower ones) the values are taken in pairs: a lane whose level
+// bit is set keeps the second value of a pair and passes the first, the other
+// lanes the reverse, and each lane adds what its partner passes.  A pair costs
+// two selects and one DPP add (two DPP adds as whole row sums, plus the
+// selects that pick lane j's value afterwards); an unpaired value is summed
+// whole.  Lane j of every row ends with the row sum of value j, j < NV <= 16.
+template <int L>
+__device__ __forceinline__ float dpp_level(float x) {
+    if constexpr (L == 0) return dpp_f<kDppQuadPerm1032>(x);
+    else if constexpr (L == 1) return dpp_f<kDppQuadPerm2301>(x);
+    else if constexpr (L == 2) return dpp_f<kDppRowRor4>(x);
+    else return dpp_f<kDppRowRor8>(x);
+}
+
+template <int L, int M>
+__device__ __forceinline__ void halve_level(float* v, int lane) {
+    const bool hi = (lane >> L) & 1;
+#pragma unroll
+    for (int i = 0; i < M / 2; ++i) {
+        const float a = v[2 * i], b = v[2 * i + 1];
+        const float keep = hi ? b : a, pass = hi ? a : b;
+        v[i] = keep + dpp_level<L>(pass);
+    }
+    if (M & 1) v[M / 2] = v[M - 1] + dpp_level<L>(v[M - 1]);
+}
+
+template <int NV>
+__device__ __forceinline__ float row_sums_split(float* v, int lane) {
+    constexpr int M1 = (NV + 1) / 2, M2 = (M1 + 1) / 2, M3 = (M2 + 1) / 2;
+    static_assert(NV <= 16 && (M3 + 1) / 2 == 1, "at most 16 values per row");
+    halve_level<0, NV>(v, lane);
+    halve_level<1, M1>(v, lane);
+    halve_level<2, M2>(v, lane);
+    halve_level<3, M3>(v, lane);
+    return v[0];
+}
 
 // One pixel's contribution to the per-Gaussian partial sums, branch-free
 // (an invalid pair contributes zeros and leaves the pixel state unchanged).
@@ -373,9 +414,9 @@ __device__ __forceinline__ bool bwd_pixel(PixState& s, const float4 a, const flo
     JSA_ACC(0, fac * s.vo_r);
     JSA_ACC(1, fac * s.vo_g);
     JSA_ACC(2, fac * s.vo_b);
-    // v_alpha = sum_ch (col T - buf ra) v_out + ra bgdot with buf . v_out kept as bv
+    // v_alpha = sum_ch (col T - buf ra) v_out + ra T_final (v_out_alpha - bg . v_out)
     const float cv = col.x * s.vo_r + col.y * s.vo_g + col.z * s.vo_b;
-    const float v_alpha = T * cv - ra * (s.bv - s.bgdot);
+    const float v_alpha = T * cv - ra * s.bv;
     s.bv += fac * cv;
     const bool live = valid && !(CONSISTENT && raw >= kClamp);
     const float vs = live ? -raw * v_alpha : 0.f;
@@ -496,8 +537,7 @@ __global__ __launch_bounds__(128) void raster_bwd_kernel(
         s.vo_b = vo.z;
         const float vo_a = (s.inside && v_out_alpha) ? v_out_alpha[pix] : 0.f;
         // background term of v_alpha: T_final * ra * (v_out_alpha - bg . v_out)
-        s.bgdot = s.T_final * (vo_a - (bgr * s.vo_r + bgg * s.vo_g + bgb * s.vo_b));
-        s.bv = 0.f;
+        s.bv = -s.T_final * (vo_a - (bgr * s.vo_r + bgg * s.vo_g + bgb * s.vo_b));
         slot_fin[sl] = uniform_i(wave_max_i(s.fin));
     }
     if (tid < ngroups) {
@@ -600,21 +640,10 @@ __global__ __launch_bounds__(128) void raster_bwd_kernel(
                     if ((ms[sl] >> b) & 1ull)
                         any |= bwd_pixel<CONSISTENT, NV>(ps[sl], a, bb, col, k, g);
                 if (ballot(any) == 0) continue;
-                // 16-lane row sums; each row leader adds into its own LDS slot
-#pragma unroll
-                for (int v = 0; v < NV; ++v) {
-                    g[v] = row_sum16(g[v]);
-                    // complete the sums in every lane here: otherwise the last
-                    // DPP add is sunk into the row leaders' branch below as a
-                    // zeroed move, a DPP move and an add
-                    asm volatile("" : "+v"(g[v]));
-                }
-                // lane j of every row takes value j, then the four rows are summed
-                // with the row swaps; lanes 0..NV-1 store the wave's sums
-                const int j = lane & 15;
-                float val = g[0];
-#pragma unroll
-                for (int v = 1; v < NV; ++v) val = (j == v) ? g[v] : val;
+                // lane j of every row gets the row sum of value j, then the four
+                // rows are summed with the row swaps; lanes 0..NV-1 store the
+                // wave's sums
+                float val = row_sums_split<NV>(g, lane);
                 {
                     const auto r = __builtin_amdgcn_permlane16_swap(
                         __float_as_uint(val), __float_as_uint(val), false, false);
