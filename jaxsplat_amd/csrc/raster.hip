@@ -46,7 +46,6 @@ inline int bwd_threads(int block_width, int slots) {
     return ((per + 63) / 64) * 64;
 }
 
-constexpr float kAlphaMin = 1.f / 255.f;
 constexpr float kTEps = 1e-4f;
 // The batches hold the conic scaled to base 2: (a, b, c) * (log2(e)/2,
 // log2(e), log2(e)/2), so that sigma2 = sigma * log2(e) =
@@ -57,6 +56,19 @@ constexpr float kHalfLog2e = 0.5f * kLog2e;
 // sigma * log2(e) of the offset (dx, dy) under a base-2 scaled conic.
 __device__ __forceinline__ float sigma2(float4 b, float dx, float dy) {
     return dx * (b.x * dx + b.y * dy) + b.z * dy * dy;
+}
+
+// Per-pixel acceptance of a (Gaussian, pixel) pair, one rule for the forward
+// and the backward (the backward recovers T by undoing the forward's blends,
+// so both must take the same pairs): o exp(-sigma) >= 1/255 with sigma >= 0 is
+// sigma2 in [0, lim2], lim2 = log2(255 o).  Gaussians with lim2 < 0 are culled
+// whole (CullInfo::any), so lim2 >= 0 here, and the range test is one unsigned
+// compare of the float bits (negative sigma2 and NaN have larger patterns).
+__device__ __forceinline__ float alpha_limit2(float o) {
+    return __builtin_amdgcn_logf(255.f * o);
+}
+__device__ __forceinline__ bool pair_accepted(float s2, float lim2) {
+    return __float_as_uint(s2) <= __float_as_uint(lim2);
 }
 
 __device__ __forceinline__ int xcd_tile(int b, int num_tiles) {
@@ -143,7 +155,8 @@ __device__ __forceinline__ float4 group_rect(int p0, int bw, int x_org, int y_or
 // reject anyway.  Conics that are not positive definite are never culled.
 struct CullInfo {
     float gx, gy, a, b, c, ra, rc, smax;
-    bool any;       // false: no pixel can pass (o <= ~1/255)
+    float lim2;     // alpha_limit2(o)
+    bool any;       // false: no pixel can pass (lim2 < 0, i.e. 255 o < 1)
     bool exact;     // false: degenerate conic, keep every group
 };
 
@@ -156,7 +169,8 @@ __device__ __forceinline__ CullInfo cull_info(float gx, float gy, float a, float
     k.b = b;
     k.c = c;
     k.smax = __logf(255.f * o) + 0.01f;
-    k.any = k.smax > 0.f;
+    k.lim2 = alpha_limit2(o);
+    k.any = k.lim2 >= 0.f;
     k.exact = a > 0.f && c > 0.f && a * c - b * b > 0.f;
     k.ra = __builtin_amdgcn_rcpf(a);
     k.rc = __builtin_amdgcn_rcpf(c);
@@ -208,7 +222,7 @@ __global__ __launch_bounds__(256) void raster_fwd_kernel(
     float* __restrict__ out_img) {
     constexpr int NW = DEPTH / 64;            // 64-entry words per batch
     __shared__ float4 s_a[DEPTH];             // x, y, opacity, id
-    __shared__ float4 s_b[DEPTH];             // conic a, b, c
+    __shared__ float4 s_b[DEPTH];             // base-2 conic a', b', c', alpha_limit2(opacity)
     __shared__ float4 s_c[DEPTH];             // colour
     __shared__ uint64_t s_mask[4][NW];        // [group][64-entry word]
     __shared__ float4 s_rect[4];
@@ -266,10 +280,10 @@ __global__ __launch_bounds__(256) void raster_fwd_kernel(
                 const float o = opac[g];
                 const float ca = conics[3 * it + 0], cb_ = conics[3 * it + 1],
                             cc = conics[3 * it + 2];
-                s_a[e] = make_float4(xy.x, xy.y, o, __int_as_float(it));
-                s_b[e] = make_float4(ca * kHalfLog2e, cb_ * kLog2e, cc * kHalfLog2e, 0.f);
-                s_c[e] = load_color(rg, colors, g);
                 const CullInfo ci = cull_info(xy.x, xy.y, ca, cb_, cc, o);
+                s_a[e] = make_float4(xy.x, xy.y, o, __int_as_float(it));
+                s_b[e] = make_float4(ca * kHalfLog2e, cb_ * kLog2e, cc * kHalfLog2e, ci.lim2);
+                s_c[e] = load_color(rg, colors, g);
                 for (int G = 0; G < nwaves; ++G)
                     if (s_rok[G] && group_hit(ci, s_rect[G])) bits |= 1u << G;
             }
@@ -295,7 +309,7 @@ __global__ __launch_bounds__(256) void raster_fwd_kernel(
                 const float dx = a.x - px, dy = a.y - py;
                 const float s2 = sigma2(b, dx, dy);
                 const float alpha = fminf(0.999f, a.z * __builtin_amdgcn_exp2f(-s2));
-                const bool ok = !done && s2 >= 0.f && alpha >= kAlphaMin;
+                const bool ok = !done && pair_accepted(s2, b.w);
                 const float next_T = T * (1.f - alpha);
                 const bool fin = ok && next_T <= kTEps;
                 const bool add = ok && !fin;
@@ -403,10 +417,10 @@ __device__ __forceinline__ bool bwd_pixel(PixState& s, const float4 a, const flo
     const float dx = a.x - s.px, dy = a.y - s.py;
     const float s2 = sigma2(b, dx, dy);
     const float raw = a.z * __builtin_amdgcn_exp2f(-s2);
-    const float alpha = fminf(kClamp, raw);
-    const bool valid = s.inside && (k <= s.fin) && s2 >= 0.f && alpha >= kAlphaMin;
+    const bool valid = s.inside && (k <= s.fin) && pair_accepted(s2, col.w);
     // an invalid pair acts as alpha = 0: T unchanged (1 / (1 - 0) is exact), no weight
-    const float al = valid ? alpha : 0.f;
+    const float rv = valid ? raw : 0.f;
+    const float al = fminf(kClamp, rv);
     const float ra = __builtin_amdgcn_rcpf(1.f - al);
     const float T = s.T * ra;                    // transmittance in front of this Gaussian
     s.T = T;
@@ -418,8 +432,8 @@ __device__ __forceinline__ bool bwd_pixel(PixState& s, const float4 a, const flo
     const float cv = col.x * s.vo_r + col.y * s.vo_g + col.z * s.vo_b;
     const float v_alpha = T * cv - ra * s.bv;
     s.bv += fac * cv;
-    const bool live = valid && !(CONSISTENT && raw >= kClamp);
-    const float vs = live ? -raw * v_alpha : 0.f;
+    // the clamp's zero slope applies in consistent mode only
+    const float vs = (CONSISTENT && raw >= kClamp) ? 0.f : -rv * v_alpha;
     const float p = vs * dx, q = vs * dy;
     JSA_ACC(3, p * dx);
     JSA_ACC(4, p * dy);
@@ -488,8 +502,8 @@ __global__ __launch_bounds__(128) void raster_bwd_kernel(
     constexpr int NV = ABS ? 11 : 9;     // rgb3, conic3, xy2, [xy_abs2], opacity
     constexpr int NW = DEPTH / 64;       // 64-entry words per batch
     __shared__ float4 s_a[DEPTH];        // x, y, opacity, id
-    __shared__ float4 s_b[DEPTH];        // conic a, b, c
-    __shared__ float4 s_c[DEPTH];        // colour
+    __shared__ float4 s_b[DEPTH];        // base-2 conic a', b', c', staged slot
+    __shared__ float4 s_c[DEPTH];        // colour, alpha_limit2(opacity)
     __shared__ float s_g[DEPTH][2][NV];  // per-wave sums [entry][wave][value]
     __shared__ int s_flag[DEPTH];
     __shared__ uint64_t s_mask[4][NW];   // [group = slot * nwaves + wave][64-entry word]
@@ -593,8 +607,10 @@ __global__ __launch_bounds__(128) void raster_bwd_kernel(
                 }
                 s_b[e] = make_float4(ca * kHalfLog2e, cb_ * kLog2e, cc * kHalfLog2e,
                                      __int_as_float(slot));
-                s_c[e] = load_color(rg, colors, g);
                 const CullInfo ci = cull_info(xy.x, xy.y, ca, cb_, cc, o);
+                float4 col = load_color(rg, colors, g);
+                col.w = ci.lim2;                 // colour, acceptance limit
+                s_c[e] = col;
                 for (int G = 0; G < ngroups; ++G)
                     if (s_rok[G] && group_hit(ci, s_rect[G])) bits |= 1u << G;
             }
