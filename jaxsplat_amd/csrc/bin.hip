@@ -193,6 +193,7 @@ struct EmitEntries {
     uint32_t* gid;      // gaussian / item id
     int2* key;          // BATCH: (bin_off, tiles_x) of the entry's view
     uint32_t* chunk_owner;
+    uint32_t* first_by_id;  // optional: first slot by id, for emitting entries
 };
 
 template <bool BATCH>
@@ -243,9 +244,11 @@ __global__ __launch_bounds__(256) void emit_compact_kernel(
         base += slab;
         zbase += zslab;
         if (cnt[k] == 0) continue;          // also every r >= n
+        const uint32_t id = perm[r];
         E.off[j] = (uint32_t)(off < limit ? off : limit);
         E.rect[j] = rc[k];
-        E.gid[j] = perm[r];
+        E.gid[j] = id;
+        if (E.first_by_id) E.first_by_id[id] = (uint32_t)(off < limit ? off : limit);
         if (BATCH) {
             const ViewRec& vr = views[vkeys[r]];
             E.key[j] = make_int2(vr.bin_off, vr.tiles_x);
@@ -394,12 +397,13 @@ size_t emit_layout(size_t o, int n, int cap, bool batch, EmitLayout& E) {
 // Writes the slots written (min(total, cap)) to info[0] and the number of
 // emitting entries to info[1] (info: the layout's slot if null), the exact
 // total to *total if total is not null, and bit 8 of err when the total
-// exceeds cap.
+// exceeds cap.  first_by_id (optional) receives the first slot of every
+// emitting id (clamped to the slots written).
 template <bool BATCH>
 hipError_t emit(int n, int tiles_x_plain, const ViewRec* views, const uint32_t* vkeys,
                 const uint32_t* perm, const uint2* rect_by_id, int cap, uint8_t* b,
                 const EmitLayout& L, int* info, unsigned long long* total, uint32_t* err,
-                uint32_t* tile_out, uint32_t* gid_out, hipStream_t s) {
+                uint32_t* tile_out, uint32_t* gid_out, uint32_t* first_by_id, hipStream_t s) {
     const int parts = (n + kEmitPart - 1) / kEmitPart;
     auto P = [&](size_t off) { return b + off; };
     uint32_t* part_tot = reinterpret_cast<uint32_t*>(P(L.part_tot));
@@ -414,6 +418,7 @@ hipError_t emit(int n, int tiles_x_plain, const ViewRec* views, const uint32_t* 
     E.gid = reinterpret_cast<uint32_t*>(P(L.gid));
     E.key = BATCH ? reinterpret_cast<int2*>(P(L.key)) : nullptr;
     E.chunk_owner = reinterpret_cast<uint32_t*>(P(L.chunk_owner));
+    E.first_by_id = first_by_id;
     emit_count_kernel<<<parts, 256, 0, s>>>(n, perm, rect_by_id, rects, part_tot, part_nz);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
@@ -530,7 +535,7 @@ hipError_t bin_emit(const BinPlan& p, const Camera& c, int num_isect_host,
     const uint32_t* perm = reinterpret_cast<const uint32_t*>(d + D.perm);
     const uint2* rect_by_id = reinterpret_cast<const uint2*>(d + D.rect_by_id);
     e = emit<false>(p.n, c.tiles_x, nullptr, nullptr, perm, rect_by_id, p.isect_cap, b, L.emit,
-                    nullptr, nullptr, err, U(L.tiles), U(L.gids), s);
+                    nullptr, nullptr, err, U(L.tiles), U(L.gids), nullptr, s);
     if (e != hipSuccess) return e;
     uint32_t* tiles_out = tiles_sorted ? reinterpret_cast<uint32_t*>(tiles_sorted) : U(L.tiles_sorted);
     e = sort_pairs(p.tile_sort, U(L.tiles), U(L.gids), tiles_out,
@@ -642,7 +647,7 @@ const unsigned long long* batch_bin_total(const BatchBinPlan& p, const void* scr
 
 hipError_t batch_bin(const BatchBinPlan& p, const ViewRec* views, const uint2* item_rects,
                      const uint32_t* depth_keys, int* items_sorted, int* tile_bins,
-                     void* scratch, hipStream_t s) {
+                     uint32_t* item_first, void* scratch, hipStream_t s) {
     const BatchLayout L = batch_layout(p);
     uint8_t* b = static_cast<uint8_t*>(scratch);
     hipError_t e = fill_zero(b, L.zero_end, s);
@@ -673,7 +678,7 @@ hipError_t batch_bin(const BatchBinPlan& p, const ViewRec* views, const uint2* i
     e = emit<true>(p.n_items, 0, views, order_views, order, item_rects, p.isect_cap, b, L.emit,
                    count,
                    reinterpret_cast<unsigned long long*>(b + L.total), err, U(L.keys),
-                   U(L.vals), s);
+                   U(L.vals), item_first, s);
     if (e != hipSuccess) return e;
     e = sort_pairs(p.tile_sort, U(L.keys), U(L.vals), U(L.keys_sorted),
                    reinterpret_cast<uint32_t*>(items_sorted), U(L.ttmp_k), U(L.ttmp_v), 0, count,

@@ -624,7 +624,9 @@ struct jaxsplat_batch {
     jsa::SceneRec* d_scenes = nullptr;
     // workspace layout
     size_t xys = 0, conics = 0, radii = 0, dkeys = 0, rects = 0, v_xy = 0, v_conic = 0,
-           items = 0, bins = 0, bin_scratch = 0, ws_bytes = 0;
+           items = 0, bins = 0, bin_scratch = 0, item_first = 0, v_rgbo = 0, staged = 0,
+           ws_bytes = 0;
+    int isect_cap = 0;
 };
 
 namespace {
@@ -750,7 +752,10 @@ int jaxsplat_batch_create(jaxsplat_batch** out, const jaxsplat_scene* scenes, in
         return fail(JAXSPLAT_ERR_INVALID, "views x tiles exceed 2^31 bins");
     }
     for (int v = 0; v < num_views; ++v) b->views[v].bin_off = v << b->bin.tile_bits;
-    // workspace: item rows, backward item gradients, sorted ids, bins, binning scratch
+    // workspace: item rows, backward item gradients, sorted ids, bins, binning
+    // scratch, each item's first intersection slot and the staged backward's
+    // records (48 bytes per intersection slot) and slot bitmap
+    b->isect_cap = (int)isect_capacity;
     size_t o = 0;
     const size_t n = (size_t)(b->n_items > 0 ? b->n_items : 1);
     b->xys = o; o = align256(o + 8 * n);
@@ -763,6 +768,9 @@ int jaxsplat_batch_create(jaxsplat_batch** out, const jaxsplat_scene* scenes, in
     b->items = o; o = align256(o + 4 * (size_t)isect_capacity);
     b->bins = o; o = align256(o + 8 * (size_t)b->bin.num_bins);
     b->bin_scratch = o; o = align256(o + b->bin.scratch_bytes);
+    b->item_first = o; o = align256(o + 4 * n);
+    b->v_rgbo = o; o = align256(o + 16 * n);
+    b->staged = o; o = align256(o + jsa::raster_bwd_staged_bytes((int)isect_capacity));
     b->ws_bytes = o;
     // device tables
     hipError_t e = hipGetDevice(&b->device);
@@ -833,6 +841,7 @@ int jaxsplat_batch_forward(const jaxsplat_batch* b, const float* means3d, const 
                                          viewmats, xys, conics, radii, dkeys, rects, s),
                   "batch_forward: project");
     JSA_CHECK_HIP(jsa::batch_bin(b->bin, b->d_views, rects, dkeys, items, bins,
+                                 reinterpret_cast<uint32_t*>(w + b->item_first),
                                  w + b->bin_scratch, s),
                   "batch_forward: bin");
     JSA_CHECK_HIP(jsa::raster_fwd_grid(batch_grid(b), items, bins, xys, conics, colors, opacities,
@@ -865,19 +874,23 @@ int jaxsplat_batch_backward(const jaxsplat_batch* b, const float* means3d, const
     float* v_conic = reinterpret_cast<float*>(w + b->v_conic);
     const int* items = reinterpret_cast<const int*>(w + b->items);
     const int* bins = reinterpret_cast<const int*>(w + b->bins);
-    const size_t G = (size_t)b->n_gauss, n = (size_t)b->n_items;
-    float* ptrs[5] = {v_colors, v_opacities, v_xy, v_conic, nullptr};
-    const size_t cnt[5] = {3 * G, G, 2 * n, 3 * n, 0};
-    JSA_CHECK_HIP(jsa::zero_grads(ptrs, cnt, 5, s), "batch_backward: zero");
+    const uint32_t* item_first = reinterpret_cast<const uint32_t*>(w + b->item_first);
+    float* v_rgbo = reinterpret_cast<float*>(w + b->v_rgbo);
     const int cons = adjoint == JAXSPLAT_ADJOINT_CONSISTENT;
-    JSA_CHECK_HIP(jsa::raster_bwd_grid(batch_grid(b), items, bins, xys, conics, colors, opacities,
-                                       backgrounds, final_Ts, final_idx, v_out_img, nullptr,
-                                       v_colors, v_opacities, v_xy, nullptr, v_conic, cons, s),
+    // staged: per-(item, tile) records summed per item (v_xy, v_conic, v_rgbo),
+    // then per Gaussian over the views in the projection backward; every
+    // gradient row is written, no float atomics
+    JSA_CHECK_HIP(jsa::raster_bwd_batch_staged(batch_grid(b), b->max_view_n, items, bins,
+                                               b->isect_cap, xys, conics, colors, opacities,
+                                               backgrounds, final_Ts, final_idx, v_out_img,
+                                               radii, item_first, w + b->staged, v_xy, v_conic,
+                                               v_rgbo, cons, s),
                   "batch_backward: rasterize");
     JSA_CHECK_HIP(jsa::project_batch_bwd(b->d_scenes, b->num_scenes, b->max_scene_n, b->d_views,
                                          b->glob_scale, b->clip_thresh, means3d, scales, quats,
-                                         viewmats, radii, conics, v_xy, v_conic, v_means3d,
-                                         v_scales, v_quats, cons, s),
+                                         viewmats, radii, conics, v_xy, v_conic, v_rgbo,
+                                         v_means3d, v_scales, v_quats, v_colors, v_opacities,
+                                         cons, s),
                   "batch_backward: project");
     return JAXSPLAT_OK;
 }

@@ -280,13 +280,16 @@ hipError_t project_batch_fwd(const ViewRec* views, int num_views, int max_view_n
                              const float* viewmats, float* xys, float* conics, int* radii,
                              uint32_t* depth_keys, uint2* item_rects, hipStream_t s);
 // Backward: one lane per Gaussian, summing the contributions of its scene's
-// views; writes v_mean3d, v_scale, v_quat for every Gaussian of every scene.
+// views; writes v_mean3d, v_scale, v_quat for every Gaussian of every scene,
+// and with v_rgbo (per item: rgb, opacity; optional) also v_colors and
+// v_opacity as the sums of the Gaussian's items over the views.
 hipError_t project_batch_bwd(const SceneRec* scenes, int num_scenes, int max_scene_n,
                              const ViewRec* views, float glob_scale, float clip_thresh,
                              const float* means3d, const float* scales, const float* quats,
                              const float* viewmats, const int* radii, const float* conics,
-                             const float* v_xy, const float* v_conic, float* v_mean3d,
-                             float* v_scale, float* v_quat, int consistent, hipStream_t s);
+                             const float* v_xy, const float* v_conic, const float* v_rgbo,
+                             float* v_mean3d, float* v_scale, float* v_quat, float* v_colors,
+                             float* v_opacity, int consistent, hipStream_t s);
 
 // ---- batched binning ----
 struct BatchBinPlan {
@@ -302,10 +305,12 @@ BatchBinPlan batch_bin_plan(int n_items, int num_views, int max_view_tiles, int 
 // Sorted item ids of every (view, tile) and bins[view << tile_bits | tile].
 // The scratch keeps the intersection count clamped to the capacity (the
 // length the sorts and bins use), the exact total (64 bits) and an error word:
-// bit 4 bad tile key, bit 8 capacity overflow.
+// bit 4 bad tile key, bit 8 capacity overflow.  item_first (optional, one per
+// item) receives the first intersection slot of every item with a non-empty
+// tile rectangle; its slots are the rectangle's tiles in row-major order.
 hipError_t batch_bin(const BatchBinPlan& p, const ViewRec* views, const uint2* item_rects,
                      const uint32_t* depth_keys, int* items_sorted, int* tile_bins,
-                     void* scratch, hipStream_t s);
+                     uint32_t* item_first, void* scratch, hipStream_t s);
 const uint32_t* batch_bin_error_word(const BatchBinPlan& p, const void* scratch);
 const int* batch_bin_count(const BatchBinPlan& p, const void* scratch);
 const unsigned long long* batch_bin_total(const BatchBinPlan& p, const void* scratch);
@@ -341,6 +346,20 @@ hipError_t raster_bwd_staged(const Camera& c, int nch, const int* gids_sorted,
                              const int* radii, const int* cum_tiles_hit, void* ws,
                              float* v_colors, float* v_opacity, float* v_xy, float* v_xy_abs,
                              float* v_conic, int consistent, hipStream_t s);
+// Batch backward with staged gradients (3 channels): records at the slots
+// item_first[item] + the tile's row-major offset in the item's rectangle
+// (workspace raster_bwd_staged_bytes(cap)), summed per item into v_xy, v_conic
+// and v_rgbo (4 floats per item: rgb, opacity); every item row is written.
+// g.views is the device view table; max_view_n the largest view's item count.
+hipError_t raster_bwd_batch_staged(const RasterGrid& g, int max_view_n, const int* items_sorted,
+                                   const int* tile_bins, int cap, const float* xys,
+                                   const float* conics, const float* colors,
+                                   const float* opacities, const float* backgrounds,
+                                   const float* final_Ts, const int* final_idx,
+                                   const float* v_out, const int* radii,
+                                   const uint32_t* item_first, void* ws, float* v_xy,
+                                   float* v_conic, float* v_rgbo, int consistent,
+                                   hipStream_t s);
 hipError_t raster_bwd_grid(const RasterGrid& g, const int* items_sorted, const int* tile_bins,
                            const float* xys, const float* conics, const float* colors,
                            const float* opacities, const float* backgrounds,

@@ -566,15 +566,17 @@ __global__ __launch_bounds__(256) void project_bwd_kernel(
 // Batched backward: blockIdx.y = scene, one lane per Gaussian; the views of
 // the scene are walked in order and their v_mean3d / v_cov3d contributions
 // summed in registers (the scale/rotation VJP is linear in v_cov3d, so it
-// runs once on the sum).  No atomics; every Gaussian of the scene is written.
+// runs once on the sum), and likewise the items' colour and opacity gradients
+// when v_rgbo is given.  No atomics; every Gaussian of the scene is written.
 __global__ __launch_bounds__(256) void project_batch_bwd_kernel(
     const SceneRec* __restrict__ scenes, const ViewRec* __restrict__ views, float glob_scale,
     float clip_thresh, const float* __restrict__ means3d, const float* __restrict__ scales,
     const float* __restrict__ quats, const float* __restrict__ viewmats,
     const int* __restrict__ radii, const float* __restrict__ conics,
     const float* __restrict__ v_xy, const float* __restrict__ v_conic,
-    float* __restrict__ v_mean3d, float* __restrict__ v_scale, float* __restrict__ v_quat,
-    int consistent) {
+    const float4* __restrict__ v_rgbo, float* __restrict__ v_mean3d,
+    float* __restrict__ v_scale, float* __restrict__ v_quat, float* __restrict__ v_colors,
+    float* __restrict__ v_opacity, int consistent) {
     const SceneRec sc = scenes[blockIdx.y];
     const int k = blockIdx.x * 256 + (int)threadIdx.x;
     if (k >= sc.n) return;
@@ -584,12 +586,20 @@ __global__ __launch_bounds__(256) void project_batch_bwd_kernel(
     const float4 q4 = reinterpret_cast<const float4*>(quats)[g];
     const float q[4] = {q4.x, q4.y, q4.z, q4.w};
     float vmean[3] = {0.f, 0.f, 0.f}, vc3[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    float4 rgbo = make_float4(0.f, 0.f, 0.f, 0.f);
     float cov[6];
     bool any = false;
     for (int v = sc.v_begin; v < sc.v_end; ++v) {
         const ViewRec vr = views[v];
         const size_t it = (size_t)vr.item_off + k;
         if (radii[it] <= 0) continue;
+        if (v_rgbo) {
+            const float4 c = v_rgbo[it];
+            rgbo.x += c.x;
+            rgbo.y += c.y;
+            rgbo.z += c.z;
+            rgbo.w += c.w;
+        }
         if (!any) {
             cov3d_of(s, q, glob_scale, cov);
             any = true;
@@ -616,6 +626,12 @@ __global__ __launch_bounds__(256) void project_batch_bwd_kernel(
         v_scale[3 * g + j] = vsc[j];
     }
     reinterpret_cast<float4*>(v_quat)[g] = make_float4(vq[0], vq[1], vq[2], vq[3]);
+    if (v_rgbo) {
+        v_colors[3 * g + 0] = rgbo.x;
+        v_colors[3 * g + 1] = rgbo.y;
+        v_colors[3 * g + 2] = rgbo.z;
+        v_opacity[g] = rgbo.w;
+    }
 }
 
 // n words, then tail (0..3) bytes after them.
@@ -715,14 +731,15 @@ hipError_t project_batch_bwd(const SceneRec* scenes, int num_scenes, int max_sce
                              const ViewRec* views, float glob_scale, float clip_thresh,
                              const float* means3d, const float* scales, const float* quats,
                              const float* viewmats, const int* radii, const float* conics,
-                             const float* v_xy, const float* v_conic, float* v_mean3d,
-                             float* v_scale, float* v_quat, int consistent, hipStream_t s) {
+                             const float* v_xy, const float* v_conic, const float* v_rgbo,
+                             float* v_mean3d, float* v_scale, float* v_quat, float* v_colors,
+                             float* v_opacity, int consistent, hipStream_t s) {
     if (num_scenes <= 0 || max_scene_n <= 0) return hipSuccess;
     const dim3 grid((max_scene_n + 255) / 256, num_scenes);
-    project_batch_bwd_kernel<<<grid, 256, 0, s>>>(scenes, views, glob_scale, clip_thresh,
-                                                  means3d, scales, quats, viewmats, radii,
-                                                  conics, v_xy, v_conic, v_mean3d, v_scale,
-                                                  v_quat, consistent);
+    project_batch_bwd_kernel<<<grid, 256, 0, s>>>(
+        scenes, views, glob_scale, clip_thresh, means3d, scales, quats, viewmats, radii, conics,
+        v_xy, v_conic, reinterpret_cast<const float4*>(v_rgbo), v_mean3d, v_scale, v_quat,
+        v_colors, v_opacity, consistent);
     return hipGetLastError();
 }
 

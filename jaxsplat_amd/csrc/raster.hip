@@ -482,7 +482,8 @@ __device__ __forceinline__ void finish_grads(float* r, float4 conic2, float o) {
 struct StagedOut {
     float4* rec;            // 3 float4 per slot
     uint32_t* bits;         // one bit per slot, zero before the backward
-    const int* cum;         // cum_tiles_hit
+    const int* cum;         // cum_tiles_hit (inclusive), used when first is null
+    const uint32_t* first;  // a batch's first slot per item (exclusive)
     const int* radii;
     int n_slots;            // number of intersections; a slot outside is not written
 };
@@ -599,7 +600,8 @@ __global__ __launch_bounds__(128) void raster_bwd_kernel(
                     int x0, x1, y0, y1;
                     tile_rect(bw, vr.tiles_x, vr.tiles_y, xy, st.radii[it], x0, x1, y0, y1);
                     const int w = x1 - x0, lx = tx - x0, ly = ty - y0;
-                    const int sl = st.cum[it] - w * (y1 - y0) + ly * w + lx;
+                    const int base = st.first ? (int)st.first[it] : st.cum[it] - w * (y1 - y0);
+                    const int sl = base + ly * w + lx;
                     // radii / cum_tiles_hit inconsistent with the binning: no slot
                     if (lx >= 0 && lx < w && ly >= 0 && ly < y1 - y0 && sl >= 0 &&
                         sl < st.n_slots)
@@ -717,92 +719,71 @@ __global__ __launch_bounds__(128) void raster_bwd_kernel(
     }
 }
 
-// Per Gaussian: the sum of its flagged staged records, in slot order, as the
-// final gradients (every output row is written; a Gaussian with no flagged
-// record gets zeros).  num_isect bounds the slots (a radius or cum_tiles_hit
-// inconsistent with it yields zeros rather than an out-of-range read).
+// Sum of the flagged staged records of slots [off, end) (empty: off == end),
+// in slot order, into (a0, a1, a2) of the calling lane.  Ranges of at most 32
+// slots are summed by their own lane: the bitmap words covering the range,
+// four independent loads per step (bits outside the range masked off), the set
+// bits taken four records at a time with their loads in flight together (a
+// missing fourth reads record 0 and is discarded).  Longer ranges are summed
+// one at a time by the whole wave -- lane l takes slots off + l, off + l + 64,
+// ... -- and combined in a fixed tree, so that one large Gaussian does not hold
+// its wave for tens of rounds; the results stay independent of scheduling.
+// Called by every lane of the wave.
 template <bool ABS>
-__global__ __launch_bounds__(256) void raster_bwd_reduce_kernel(
-    Camera c, int num_isect, const float2* __restrict__ xys, const int* __restrict__ radii,
-    const int* __restrict__ cum, const float4* __restrict__ rec,
-    const uint32_t* __restrict__ bits, int nch, float* __restrict__ v_rgb,
-    float* __restrict__ v_opacity, float* __restrict__ v_xy, float* __restrict__ v_xy_abs,
-    float* __restrict__ v_conic) {
-    // no early exit: the wave-wide pass below needs every lane
-    const int g = blockIdx.x * 256 + threadIdx.x;
-    const int lane = lane_id();
-    const bool own = g < c.num_points;
-    float4 a0 = make_float4(0.f, 0.f, 0.f, 0.f), a1 = a0, a2 = a0;
-    int off = 0, end = 0;
-    if (own && radii[g] > 0) {
-        int x0, x1, y0, y1;
-        tile_rect(c.block_width, c.tiles_x, c.tiles_y, xys[g], radii[g], x0, x1, y0, y1);
-        end = cum[g];
-        off = end - (x1 - x0) * (y1 - y0);
-        if (off < 0 || end > num_isect) off = end = 0;     // inconsistent input: zeros
-    }
-    // more than kBig slots: summed by the whole wave below, so that one large
-    // Gaussian does not hold its wave for tens of rounds
+__device__ __forceinline__ void sum_records(int off, int end, const float4* __restrict__ rec,
+                                            const uint32_t* __restrict__ bits, int lane,
+                                            float4& a0, float4& a1, float4& a2) {
     constexpr int kBig = 32;
     const bool big = end - off > kBig;
-    {
-        const int cnt = end - off;
-        if (cnt > 0 && !big) {
-            // the bitmap words covering [off, end), four independent loads per
-            // step (bits outside the range masked off); the set bits are taken
-            // in slot order, four records per round with their loads in flight
-            // together (a missing fourth reads record 0 and is discarded)
-            const int wfirst = off >> 5, wlast = (end - 1) >> 5;
-            const uint32_t lo_mask = ~0u << (off & 31);
-            const int hb = (end - 1) & 31;
-            const uint32_t hi_mask = hb == 31 ? ~0u : ((2u << hb) - 1u);
-            for (int w0 = wfirst; w0 <= wlast; w0 += 4) {
-                uint32_t W[4];
+    const int cnt = end - off;
+    if (cnt > 0 && !big) {
+        const int wfirst = off >> 5, wlast = (end - 1) >> 5;
+        const uint32_t lo_mask = ~0u << (off & 31);
+        const int hb = (end - 1) & 31;
+        const uint32_t hi_mask = hb == 31 ? ~0u : ((2u << hb) - 1u);
+        for (int w0 = wfirst; w0 <= wlast; w0 += 4) {
+            uint32_t W[4];
 #pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const int w = w0 + q;
-                    uint32_t v = w <= wlast ? bits[w] : 0u;
-                    if (w == wfirst) v &= lo_mask;
-                    if (w == wlast) v &= hi_mask;
-                    W[q] = v;
-                }
+            for (int q = 0; q < 4; ++q) {
+                const int w = w0 + q;
+                uint32_t v = w <= wlast ? bits[w] : 0u;
+                if (w == wfirst) v &= lo_mask;
+                if (w == wlast) v &= hi_mask;
+                W[q] = v;
+            }
 #pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    uint32_t v = W[q];
-                    const int kb = (w0 + q) << 5;
-                    while (v) {
-                        int k[4];
-                        bool ok[4];
+            for (int q = 0; q < 4; ++q) {
+                uint32_t v = W[q];
+                const int kb = (w0 + q) << 5;
+                while (v) {
+                    int k[4];
+                    bool ok[4];
 #pragma unroll
-                        for (int i = 0; i < 4; ++i) {
-                            ok[i] = v != 0u;
-                            k[i] = ok[i] ? kb + __ffs(v) - 1 : 0;
-                            v &= v - 1u;
-                        }
-                        float4 r[4][3];
+                    for (int i = 0; i < 4; ++i) {
+                        ok[i] = v != 0u;
+                        k[i] = ok[i] ? kb + __ffs(v) - 1 : 0;
+                        v &= v - 1u;
+                    }
+                    float4 r[4][3];
 #pragma unroll
-                        for (int i = 0; i < 4; ++i) {
-                            r[i][0] = rec[3 * (size_t)k[i] + 0];
-                            r[i][1] = rec[3 * (size_t)k[i] + 1];
-                            r[i][2] = rec[3 * (size_t)k[i] + 2];
-                        }
+                    for (int i = 0; i < 4; ++i) {
+                        r[i][0] = rec[3 * (size_t)k[i] + 0];
+                        r[i][1] = rec[3 * (size_t)k[i] + 1];
+                        r[i][2] = rec[3 * (size_t)k[i] + 2];
+                    }
 #pragma unroll
-                        for (int i = 0; i < 4; ++i) {
-                            const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
-                            const float4 q0 = ok[i] ? r[i][0] : z, q1 = ok[i] ? r[i][1] : z,
-                                         q2 = ok[i] ? r[i][2] : z;
-                            a0.x += q0.x; a0.y += q0.y; a0.z += q0.z; a0.w += q0.w;
-                            a1.x += q1.x; a1.y += q1.y; a1.z += q1.z; a1.w += q1.w;
-                            a2.x += q2.x; a2.y += q2.y; a2.z += q2.z;
-                        }
+                    for (int i = 0; i < 4; ++i) {
+                        const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+                        const float4 q0 = ok[i] ? r[i][0] : z, q1 = ok[i] ? r[i][1] : z,
+                                     q2 = ok[i] ? r[i][2] : z;
+                        a0.x += q0.x; a0.y += q0.y; a0.z += q0.z; a0.w += q0.w;
+                        a1.x += q1.x; a1.y += q1.y; a1.z += q1.z; a1.w += q1.w;
+                        a2.x += q2.x; a2.y += q2.y; a2.z += q2.z;
                     }
                 }
             }
         }
     }
-    // large Gaussians, one at a time over the wave: lane l takes slots
-    // off + l, off + l + 64, ... and the eleven sums are combined in a fixed
-    // tree (results stay independent of scheduling)
     uint64_t bigm = ballot(big);
     while (bigm) {
         const int src = __ffsll((unsigned long long)bigm) - 1;
@@ -830,6 +811,33 @@ __global__ __launch_bounds__(256) void raster_bwd_reduce_kernel(
             a2 = make_float4(v[8], ABS ? v[9] : 0.f, ABS ? v[10] : 0.f, 0.f);
         }
     }
+}
+
+// Per Gaussian: the sum of its flagged staged records, in slot order, as the
+// final gradients (every output row is written; a Gaussian with no flagged
+// record gets zeros).  num_isect bounds the slots (a radius or cum_tiles_hit
+// inconsistent with it yields zeros rather than an out-of-range read).
+template <bool ABS>
+__global__ __launch_bounds__(256) void raster_bwd_reduce_kernel(
+    Camera c, int num_isect, const float2* __restrict__ xys, const int* __restrict__ radii,
+    const int* __restrict__ cum, const float4* __restrict__ rec,
+    const uint32_t* __restrict__ bits, int nch, float* __restrict__ v_rgb,
+    float* __restrict__ v_opacity, float* __restrict__ v_xy, float* __restrict__ v_xy_abs,
+    float* __restrict__ v_conic) {
+    // no early exit: the wave-wide pass of sum_records needs every lane
+    const int g = blockIdx.x * 256 + threadIdx.x;
+    const int lane = lane_id();
+    const bool own = g < c.num_points;
+    float4 a0 = make_float4(0.f, 0.f, 0.f, 0.f), a1 = a0, a2 = a0;
+    int off = 0, end = 0;
+    if (own && radii[g] > 0) {
+        int x0, x1, y0, y1;
+        tile_rect(c.block_width, c.tiles_x, c.tiles_y, xys[g], radii[g], x0, x1, y0, y1);
+        end = cum[g];
+        off = end - (x1 - x0) * (y1 - y0);
+        if (off < 0 || end > num_isect) off = end = 0;     // inconsistent input: zeros
+    }
+    sum_records<ABS>(off, end, rec, bits, lane, a0, a1, a2);
     if (!own) return;
     float* vc = v_rgb + (size_t)nch * g;
     vc[0] = a0.x;
@@ -845,6 +853,46 @@ __global__ __launch_bounds__(256) void raster_bwd_reduce_kernel(
         v_xy_abs[2 * (size_t)g + 0] = a2.y;
         v_xy_abs[2 * (size_t)g + 1] = a2.z;
     }
+}
+
+// Per item of a batch (one grid row per view): the sum of the item's flagged
+// records, slots [item_first, item_first + its rectangle's tile count), as the
+// item's v_xy and v_conic and its colour and opacity gradients
+// v_rgbo = (rgb, opacity); every item row of every view is written (zeros for
+// items without records).  Slots beyond n_slots yield zeros.
+__global__ __launch_bounds__(256) void raster_bwd_batch_reduce_kernel(
+    const ViewRec* __restrict__ views, int bw, int n_slots, const float2* __restrict__ xys,
+    const int* __restrict__ radii, const uint32_t* __restrict__ item_first,
+    const float4* __restrict__ rec, const uint32_t* __restrict__ bits, float* __restrict__ v_xy,
+    float* __restrict__ v_conic, float4* __restrict__ v_rgbo) {
+    const ViewRec vr = views[blockIdx.y];
+    if ((int)blockIdx.x * 256 >= vr.n) return;        // uniform over the block
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    const int lane = lane_id();
+    const bool own = k < vr.n;
+    const size_t it = (size_t)vr.item_off + (own ? k : 0);
+    float4 a0 = make_float4(0.f, 0.f, 0.f, 0.f), a1 = a0, a2 = a0;
+    int off = 0, end = 0;
+    if (own && radii[it] > 0) {
+        int x0, x1, y0, y1;
+        tile_rect(bw, vr.tiles_x, vr.tiles_y, xys[it], radii[it], x0, x1, y0, y1);
+        const int cnt = (x1 - x0) * (y1 - y0);
+        if (cnt > 0) {
+            const long long f = item_first[it];
+            if (f + cnt <= (long long)n_slots) {
+                off = (int)f;
+                end = off + cnt;
+            }
+        }
+    }
+    sum_records<false>(off, end, rec, bits, lane, a0, a1, a2);
+    if (!own) return;
+    v_conic[3 * it + 0] = a0.w;
+    v_conic[3 * it + 1] = a1.x;
+    v_conic[3 * it + 2] = a1.y;
+    v_xy[2 * it + 0] = a1.z;
+    v_xy[2 * it + 1] = a1.w;
+    v_rgbo[it] = make_float4(a0.x, a0.y, a0.z, a2.x);
 }
 
 struct ZeroList {
@@ -1030,6 +1078,7 @@ hipError_t raster_bwd_staged(const Camera& c, int nch, const int* gids_sorted,
     st.rec = reinterpret_cast<float4*>(w);
     st.bits = reinterpret_cast<uint32_t*>(w + ((48 * n + 255) & ~size_t(255)));
     st.cum = cum_tiles_hit;
+    st.first = nullptr;
     st.radii = radii;
     st.n_slots = num_isect;
     hipError_t e = hipSuccess;
@@ -1072,6 +1121,62 @@ hipError_t raster_bwd_staged(const Camera& c, int nch, const int* gids_sorted,
         raster_bwd_reduce_kernel<false><<<blocks, 256, 0, s>>>(
             c, num_isect, x2, radii, cum_tiles_hit, st.rec, st.bits, nch, v_colors, v_opacity,
             v_xy, nullptr, v_conic);
+    return hipGetLastError();
+}
+
+hipError_t raster_bwd_batch_staged(const RasterGrid& g, int max_view_n, const int* items_sorted,
+                                   const int* tile_bins, int cap, const float* xys,
+                                   const float* conics, const float* colors,
+                                   const float* opacities, const float* backgrounds,
+                                   const float* final_Ts, const int* final_idx,
+                                   const float* v_out, const int* radii,
+                                   const uint32_t* item_first, void* ws, float* v_xy,
+                                   float* v_conic, float* v_rgbo, int consistent,
+                                   hipStream_t s) {
+    if (g.total_tiles <= 0 || g.num_views <= 0 || max_view_n <= 0) return hipSuccess;
+    if (g.ch_n != 3 || g.ch_stride != 3 || g.ch_off != 0) return hipErrorInvalidValue;
+    const int depth = g.bwd_depth > 0 ? g.bwd_depth : 64;
+    if (depth != 64 && depth != 128 && depth != 256) return hipErrorInvalidValue;
+    uint8_t* w = static_cast<uint8_t*>(ws);
+    const size_t n = (size_t)(cap > 0 ? cap : 1);
+    StagedOut st;
+    st.rec = reinterpret_cast<float4*>(w);
+    st.bits = reinterpret_cast<uint32_t*>(w + ((48 * n + 255) & ~size_t(255)));
+    st.cum = nullptr;
+    st.first = item_first;
+    st.radii = radii;
+    st.n_slots = cap;
+    hipError_t e = fill_zero(st.bits, 4 * ((n + 31) / 32), s);
+    if (e != hipSuccess) return e;
+    const int slots = bwd_slots(g.block_width);
+    const int nthr = bwd_threads(g.block_width, slots);
+    const int grid = tile_grid(g.total_tiles);
+    const int2* b2 = reinterpret_cast<const int2*>(tile_bins);
+    const float2* x2 = reinterpret_cast<const float2*>(xys);
+#define JSA_LAUNCH_S(CONS, D, S)                                                             \
+    raster_bwd_kernel<CONS, false, D, true, S><<<grid, nthr, 0, s>>>(                       \
+        g, items_sorted, b2, x2, conics, colors, opacities, backgrounds, final_Ts, final_idx, \
+        v_out, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, st)
+#define JSA_LAUNCH_D(CONS, D)                                                                \
+    do {                                                                                     \
+        if (slots == 4) JSA_LAUNCH_S(CONS, D, 4);                                            \
+        else JSA_LAUNCH_S(CONS, D, 2);                                                       \
+    } while (0)
+#define JSA_LAUNCH(CONS)                                                                     \
+    do {                                                                                     \
+        if (depth == 64) JSA_LAUNCH_D(CONS, 64);                                             \
+        else if (depth == 128) JSA_LAUNCH_D(CONS, 128);                                      \
+        else JSA_LAUNCH_D(CONS, 256);                                                        \
+    } while (0)
+    if (consistent) JSA_LAUNCH(true); else JSA_LAUNCH(false);
+#undef JSA_LAUNCH
+#undef JSA_LAUNCH_D
+#undef JSA_LAUNCH_S
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    const dim3 rgrid((max_view_n + 255) / 256, g.num_views);
+    raster_bwd_batch_reduce_kernel<<<rgrid, 256, 0, s>>>(
+        g.views, g.block_width, cap, x2, radii, item_first, st.rec, st.bits, v_xy, v_conic,
+        reinterpret_cast<float4*>(v_rgbo));
     return hipGetLastError();
 }
 

@@ -6,11 +6,13 @@ Bars (written in each test):
 * one view against the float64 oracle at the plain path's image bar
   (max-abs <= 1e-4 outside float32-ambiguous pixels);
 * gradients equal the sum over the scene's views of the plain per-view
-  gradients within |g - g_sum| <= 1e-3 |g_sum| + 1e-4 max|g_sum| (float
-  atomics and the per-view sum add in a different order);
+  gradients within |g - g_sum| <= 1e-3 |g_sum| + 1e-4 max|g_sum| (the batch
+  sums per item and then per Gaussian over the views, the plain path per
+  Gaussian, so the float additions differ in order);
 * a view that sees nothing renders the background with zero gradients;
 * a capacity overflow is reported, and BatchRenderer recovers from it;
-* a HIP graph of forward + backward replays to the eager results.
+* a HIP graph of forward + backward replays to the eager results bit for bit
+  (the staged backward sums in a fixed order and writes every gradient row).
 """
 
 import numpy as np
@@ -224,8 +226,8 @@ def test_batch_backward_after_second_forward_raises():
 def test_batch_hip_graph_replays_match_eager():
     """Forward + backward of the whole batch captured in one HIP graph;
     replayed three times with eager work and synchronisations in between,
-    every replay equals the eager run (image bit-exact, gradients within the
-    atomics tolerance)."""
+    every replay equals the eager run bit for bit, image and gradients (the
+    gradient buffers are refilled with garbage before each replay)."""
     t = H.torch()
     params, sizes, cams = _layout()
     plan = _plan(sizes, cams)
@@ -271,8 +273,8 @@ def test_batch_hip_graph_replays_match_eager():
         t.cuda.synchronize()
         assert np.array_equal(H.host(img), eager_img), f"replay {r} image"
         for k, g in enumerate(grads):
-            ok, worst = H.grad_close(H.host(g), eager_grads[k], rtol=1e-4, scale_frac=1e-5)
-            assert ok, (r, k, worst)
+            assert np.array_equal(H.host(g).view(np.uint32), eager_grads[k].view(np.uint32)), \
+                (r, k)
         # eager work between replays
         with t.no_grad():
             render_batch(*d, vmd, bgd, plan=_plan(sizes, cams))
