@@ -314,7 +314,11 @@ int jaxsplat_batch_forward(const jaxsplat_batch* b, const float* means3d, const 
                            size_t workspace_bytes, void* stream);
 
 /* Backward of every view for the cotangent v_out_img (P,3): writes the
- * gradients of all G Gaussians (summed over the views of their scene). */
+ * gradients of all G Gaussians (summed over the views of their scene).  No
+ * float atomics: the per-(view, tile) partial sums are staged in the
+ * workspace (48 bytes per intersection of capacity) and summed in a fixed
+ * order, so two calls on the same inputs give bit-identical gradients and
+ * the outputs need no zeroing. */
 int jaxsplat_batch_backward(const jaxsplat_batch* b, const float* means3d, const float* scales,
                             const float* quats, const float* colors, const float* opacities,
                             const float* viewmats, const float* backgrounds,
