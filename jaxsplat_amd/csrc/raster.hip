@@ -255,7 +255,9 @@ __global__ __launch_bounds__(256) void raster_fwd_kernel(
         s_rok[tid] = ok;
     }
 
-    const int2 range = bins[vr.bin_off + tile];
+    // wave-uniform, so that the batch bounds and sorted indices stay scalar
+    const int2 rb = bins[vr.bin_off + tile];
+    const int2 range = make_int2(uniform_i(rb.x), uniform_i(rb.y));
     float T = 1.f;
     float cr = 0.f, cg = 0.f, cb = 0.f;
     int cur = 0;
@@ -310,7 +312,7 @@ __global__ __launch_bounds__(256) void raster_fwd_kernel(
                 const float s2 = sigma2(b, dx, dy);
                 const float alpha = fminf(0.999f, a.z * __builtin_amdgcn_exp2f(-s2));
                 const bool ok = !done && pair_accepted(s2, b.w);
-                const float next_T = T * (1.f - alpha);
+                const float next_T = fmaf(-T, alpha, T);      // T (1 - alpha)
                 const bool fin = ok && next_T <= kTEps;
                 const bool add = ok && !fin;
                 if (ballot(add)) {
