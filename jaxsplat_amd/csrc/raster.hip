@@ -300,14 +300,26 @@ __global__ __launch_bounds__(256) void raster_fwd_kernel(
         // read and blended under one uniform branch when some lane adds, and
         // the wave's exit test runs only after some lane terminated (keeps the
         // scalar unit from issuing exec-mask updates for every evaluation).
+        // Software-pipelined over the set bits of each mask word: the next
+        // entry's LDS reads are issued before the current pair is evaluated,
+        // which takes their latency out of the dependent chain per pair.  Two
+        // register sets alternate (unrolled by two), so nothing is copied.
         bool live = ballot(!done) != 0;
         for (int q = 0; q < NW && live; ++q) {
             uint64_t m = uniform64(s_mask[wave][q]);
-            while (m) {
-                const int t = q * 64 + __ffsll((unsigned long long)m) - 1;
+            // branch-free: with no bit left it re-reads entry t (a valid index),
+            // so the LDS waits before each evaluation stay exact
+            auto fetch = [&](int& t, float4& a, float4& b, float4& c) -> bool {
+                const bool h = m != 0;
+                t = h ? q * 64 + __ffsll((unsigned long long)m) - 1 : t;
                 m &= m - 1;
-                const float4 a = s_a[t];
-                const float4 b = s_b[t];
+                a = s_a[t];
+                b = s_b[t];
+                c = s_c[t];
+                return h;
+            };
+            // one pair; true when every pixel of the wave is done
+            auto eval = [&](int t, const float4& a, const float4& b, const float4& col) -> bool {
                 const float dx = a.x - px, dy = a.y - py;
                 const float s2 = sigma2(b, dx, dy);
                 const float alpha = fminf(0.999f, a.z * __builtin_amdgcn_exp2f(-s2));
@@ -316,7 +328,6 @@ __global__ __launch_bounds__(256) void raster_fwd_kernel(
                 const bool fin = ok && next_T <= kTEps;
                 const bool add = ok && !fin;
                 if (ballot(add)) {
-                    const float4 col = s_c[t];
                     const float vis = add ? alpha * T : 0.f;
                     cr += col.x * vis;
                     cg += col.y * vis;
@@ -326,10 +337,24 @@ __global__ __launch_bounds__(256) void raster_fwd_kernel(
                 }
                 if (ballot(fin)) {
                     done = done || fin;
-                    if (ballot(!done) == 0) {
-                        live = false;
-                        break;
-                    }
+                    if (ballot(!done) == 0) return true;
+                }
+                return false;
+            };
+            int t0 = 0, t1 = 0;
+            float4 a0, b0, c0, a1, b1, c1;
+            bool h0 = fetch(t0, a0, b0, c0);
+            while (h0) {
+                const bool h1 = fetch(t1, a1, b1, c1);
+                if (eval(t0, a0, b0, c0)) {
+                    live = false;
+                    break;
+                }
+                if (!h1) break;
+                h0 = fetch(t0, a0, b0, c0);
+                if (eval(t1, a1, b1, c1)) {
+                    live = false;
+                    break;
                 }
             }
         }
