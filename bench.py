@@ -155,7 +155,7 @@ def main():
 def stage_breakdown(torch, sc, cam, params, target, iters):
     """Run the same pipeline through the C ABI with HIP events between stages."""
     from jaxsplat_amd import _lib
-    from jaxsplat_amd._ops import _bin, _p, _stream
+    from jaxsplat_amd._ops import TIGHT_TILES, _bin, _bin_tight, _p, _stream
     L = _lib.lib()
     dev = params[0].device
     N = sc.n
@@ -189,8 +189,14 @@ def stage_breakdown(torch, sc, cam, params, target, iters):
             ctypes.byref(d), _p(means3d), _p(scales), _p(quats), _p(viewmat), None, _p(xys),
             _p(depths), _p(radii), _p(conics), None, _p(nth), _p(cum), _p(ws), wsb, s))
         ev[1].record()
-        isect = int(cum[-1, 0].item())
-        gids, bins, _ = _bin(d, xys, depths, radii, isect, dev)
+        # the render path's binning: opacity-limited tile rectangles (the
+        # rasterize step computes them), or the reference's rectangles
+        if TIGHT_TILES:
+            gids, bins, isect, rects, bcum = _bin_tight(d, xys, depths, radii, conics, opac, dev)
+        else:
+            isect = int(cum[-1, 0].item())
+            gids, bins, _ = _bin(d, xys, depths, radii, isect, dev)
+            rects, bcum = None, cum
         ev[2].record()
         _lib.check(L.jaxsplat_rasterize_forward_binned(
             ctypes.byref(d), _p(gids), _p(bins), isect, _p(xys), _p(conics), _p(colors), _p(opac),
@@ -202,10 +208,16 @@ def stage_breakdown(torch, sc, cam, params, target, iters):
             bws = torch.empty(bwsb, dtype=torch.uint8, device=dev)
         ev[4].record()
         # the render path's backward: staged records + per-Gaussian reduction
-        _lib.check(L.jaxsplat_rasterize_backward_staged(
-            ctypes.byref(d), _p(gids), _p(bins), isect, _p(xys), _p(conics), _p(colors), _p(opac),
-            _p(bg), 3, _p(radii), _p(cum), _p(fT), _p(fI), _p(v_img), None, _p(vc), _p(vo),
-            _p(vxy), None, _p(vcn), 0, _p(bws), bwsb, s))
+        if rects is not None:
+            _lib.check(L.jaxsplat_rasterize_backward_staged_rects(
+                ctypes.byref(d), _p(gids), _p(bins), isect, _p(xys), _p(conics), _p(colors),
+                _p(opac), _p(bg), 3, _p(radii), _p(rects), _p(bcum), _p(fT), _p(fI), _p(v_img),
+                None, _p(vc), _p(vo), _p(vxy), None, _p(vcn), 0, _p(bws), bwsb, s))
+        else:
+            _lib.check(L.jaxsplat_rasterize_backward_staged(
+                ctypes.byref(d), _p(gids), _p(bins), isect, _p(xys), _p(conics), _p(colors),
+                _p(opac), _p(bg), 3, _p(radii), _p(bcum), _p(fT), _p(fI), _p(v_img), None,
+                _p(vc), _p(vo), _p(vxy), None, _p(vcn), 0, _p(bws), bwsb, s))
         ev[5].record()
         _lib.check(L.jaxsplat_project_backward(
             ctypes.byref(d), _p(means3d), _p(scales), _p(quats), _p(viewmat), _p(radii),
@@ -221,14 +233,30 @@ def stage_breakdown(torch, sc, cam, params, target, iters):
     # + 20 B per pixel (final T 4 + final idx 4 + v_img 12)
     # + 36 B per visible Gaussian (v_rgb 12 + v_opacity 4 + v_xy 8 + v_conic 12 written)
     # + 8 B per tile (bin read)
+    # The intersections are counted on the reference's tile lists (exact
+    # binning of the same projection, outside the timed region), also when the
+    # timed path bins into the shorter opacity-limited lists.
     bw = sc.block_width
     tiles_x, tiles_y = (Wimg + bw - 1) // bw, (Himg + bw - 1) // bw
-    bins_np = bins.cpu().numpy().astype(np.int64)
-    fI_np = fI.cpu().numpy().astype(np.int64)
-    pad = np.full((tiles_y * bw, tiles_x * bw), -1, np.int64)
-    pad[:Himg, :Wimg] = fI_np
-    tmax = pad.reshape(tiles_y, bw, tiles_x, bw).max(axis=(1, 3)).reshape(-1)
-    L_b = int(np.clip(tmax - bins_np[:, 0] + 1, 0, bins_np[:, 1] - bins_np[:, 0]).sum())
+
+    def consumed(bins_t, fI_t):
+        bins_np = bins_t.cpu().numpy().astype(np.int64)
+        pad = np.full((tiles_y * bw, tiles_x * bw), -1, np.int64)
+        pad[:Himg, :Wimg] = fI_t.cpu().numpy().astype(np.int64)
+        tmax = pad.reshape(tiles_y, bw, tiles_x, bw).max(axis=(1, 3)).reshape(-1)
+        return int(np.clip(tmax - bins_np[:, 0] + 1, 0, bins_np[:, 1] - bins_np[:, 0]).sum())
+
+    isect_binned, L_b_binned = isect, consumed(bins, fI)
+    isect = int(cum[-1, 0].item())
+    if rects is not None:
+        gids, bins, _ = _bin(d, xys, depths, radii, isect, dev)
+        fT2, fI2, img2 = torch.empty_like(fT), torch.empty_like(fI), torch.empty_like(img)
+        _lib.check(L.jaxsplat_rasterize_forward_binned(
+            ctypes.byref(d), _p(gids), _p(bins), isect, _p(xys), _p(conics), _p(colors), _p(opac),
+            _p(bg), _p(fT2), _p(fI2), _p(img2), s))
+        L_b = consumed(bins, fI2)
+    else:
+        L_b = L_b_binned
     V = int((radii > 0).sum().item())
     T = tiles_x * tiles_y
     bytes_bwd = 40 * L_b + 20 * Himg * Wimg + 36 * V + 8 * T
@@ -241,9 +269,9 @@ def stage_breakdown(torch, sc, cam, params, target, iters):
             "bytes_per_launch": bytes_bwd}
     # whole-step algorithmic bytes (B_alg) for the combined fraction
     P = 2 if T > 256 else 1
-    L_f_tiles = tmax  # forward consumed prefix bounded by the last contributor (lower bound)
     B_alg = 232 * N + 20 * V + (20 + 24 * P) * isect + 40 * 2 * L_b + 24 * T + 40 * Himg * Wimg
     extra = {"num_intersects": isect, "visible": V, "L_b": L_b,
+             "num_intersects_binned": isect_binned, "L_b_binned": L_b_binned,
              "B_alg_bytes": int(B_alg), "stage_sum_ms": round(sum(acc.values()), 4),
              "step_frac_hbm": round(B_alg / (sum(acc.values()) * 1e-3) / 1e9 / HBM_PEAK_GBS, 5)}
     return {k: round(v, 4) for k, v in acc.items()}, roof, extra

@@ -168,6 +168,35 @@ int jaxsplat_bin_emit_sort(const jaxsplat_descriptor* d, uint32_t num_intersects
                            size_t workspace_bytes, void* stream);
 
 /*
+ * Opacity-limited tile rectangles.  No counterpart in the reference; for a
+ * fused projection + rasterization (the reference's render,
+ * jaxsplat/__init__.py:6-70) whose intermediate tile lists the caller never
+ * sees.  Per Gaussian: its tile rectangle (the reference's,
+ * lib/kernels/helpers.h:9-32) restricted to the tiles holding a pixel centre
+ * where o exp(-sigma) >= 1/255 can hold (the bounding box of that ellipse,
+ * with margins), packed as two words (x0 | y0 << 16, w | h << 16); the tile
+ * counts (num_tiles_hit, nullable) and their inclusive sums (cum_tiles_hit).
+ * The tiles dropped are those where the rasterizer's per-pixel test rejects
+ * every pixel, so an image rendered through these lists equals the one
+ * rendered through the reference's, while the lists are shorter (78% of the
+ * intersections at the 1080p, 1M-Gaussian workload).  Binned with
+ * jaxsplat_bin_depth_sort_rects + jaxsplat_bin_emit_sort (num_intersects =
+ * the last cum_tiles_hit), rasterized by the usual forward and backward, or
+ * by jaxsplat_rasterize_backward_staged_rects with the same rectangles.
+ */
+size_t jaxsplat_tile_rects_workspace_size(const jaxsplat_descriptor* d);
+int jaxsplat_tile_rects(const jaxsplat_descriptor* d, const float* xys, const int32_t* radii,
+                        const float* conics, const float* opacities, uint32_t* tile_rects,
+                        int32_t* num_tiles_hit, int32_t* cum_tiles_hit, void* workspace,
+                        size_t workspace_bytes, void* stream);
+/* jaxsplat_bin_depth_sort with the tile rectangles given (jaxsplat_tile_rects)
+ * instead of computed from xys and radii; same workspace. */
+int jaxsplat_bin_depth_sort_rects(const jaxsplat_descriptor* d, const float* depths,
+                                  const int32_t* radii, const uint32_t* tile_rects,
+                                  void* depth_workspace, size_t depth_workspace_bytes,
+                                  void* stream);
+
+/*
  * Synchronise `stream` and report an error recorded in the workspace of the
  * last jaxsplat_bin_and_sort or jaxsplat_bin_emit_sort (JAXSPLAT_ERR_CAPACITY):
  * more intersections than num_intersects, or a tile key out of range.
@@ -251,6 +280,17 @@ int jaxsplat_rasterize_backward_staged(
     const float* v_out_img, const float* v_out_img_alpha, float* v_colors, float* v_opacity,
     float* v_xy, float* v_xy_abs, float* v_conic, int adjoint, void* workspace,
     size_t workspace_bytes, void* stream);
+/* The same with the lists binned from tile_rects (jaxsplat_tile_rects; its
+ * cum_tiles_hit): the slots follow those rectangles.  No counterpart in the
+ * reference.  Same workspace size. */
+int jaxsplat_rasterize_backward_staged_rects(
+    const jaxsplat_descriptor* d, const int32_t* gaussian_ids_sorted, const int32_t* tile_bins,
+    uint32_t num_intersects, const float* xys, const float* conics, const float* colors,
+    const float* opacities, const float* background, uint32_t channels, const int32_t* radii,
+    const uint32_t* tile_rects, const int32_t* cum_tiles_hit, const float* final_Ts,
+    const int32_t* final_idx, const float* v_out_img, const float* v_out_img_alpha,
+    float* v_colors, float* v_opacity, float* v_xy, float* v_xy_abs, float* v_conic, int adjoint,
+    void* workspace, size_t workspace_bytes, void* stream);
 
 /* ------------------------------------------------------------------------
  * Batched rendering: many views of one or more scenes in one pass.

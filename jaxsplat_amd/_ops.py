@@ -21,6 +21,14 @@ per-tile partial gradients in a fixed order (``jaxsplat_rasterize_backward_
 staged``) instead of with float atomics, so it is bitwise reproducible.
 ``STAGED_BACKWARD = False`` (or ``JAXSPLAT_AMD_STAGED_BACKWARD=0`` in the
 environment) selects the atomic backward, which differs only in rounding.
+
+``rasterize`` bins each Gaussian into its opacity-limited tile rectangle
+(``jaxsplat_tile_rects``): the reference's rectangle without the tiles where
+every pixel fails the per-pixel test o exp(-sigma) >= 1/255.  Those entries
+never blend, so the image and gradients are the reference's, from shorter tile
+lists; only the internal sorted indices (the final_idx residual) differ.
+``TIGHT_TILES = False`` (or ``JAXSPLAT_AMD_TIGHT_TILES=0``) bins into the
+reference's rectangles given by ``cum_tiles_hit``.
 """
 
 from __future__ import annotations
@@ -33,6 +41,7 @@ import torch
 from . import _lib
 
 STAGED_BACKWARD = os.environ.get("JAXSPLAT_AMD_STAGED_BACKWARD", "1") != "0"
+TIGHT_TILES = os.environ.get("JAXSPLAT_AMD_TIGHT_TILES", "1") != "0"
 
 
 class ShapeDtypeError(AssertionError, TypeError):
@@ -203,6 +212,44 @@ def _bin_overlapped(d, xys, depths, radii, cum_tiles_hit, dev):
     return gids[:isect], bins, isect
 
 
+def _bin_tight(d, xys, depths, radii, conics, opacities, dev):
+    """Binning into the opacity-limited tile rectangles: the rectangles and
+    their inclusive counts, then the depth stage on those rectangles with the
+    host read of the total overlapped as in _bin_overlapped.  Returns
+    (gaussian_ids_sorted, tile_bins, isect, rects, cum)."""
+    N = xys.shape[0]
+    L = _lib.lib()
+    rects = torch.empty((max(N, 1), 2), dtype=torch.int32, device=dev)
+    cum = torch.empty((max(N, 1), 1), dtype=torch.int32, device=dev)
+    if N == 0:
+        gids, bins, isect = _bin(d, xys, depths, radii, 0, dev)
+        return gids, bins, isect, rects, cum
+    with torch.cuda.device(dev):
+        rws_bytes = L.jaxsplat_tile_rects_workspace_size(ctypes.byref(d))
+        rws = torch.empty(max(rws_bytes, 1), dtype=torch.uint8, device=dev)
+        _lib.check(L.jaxsplat_tile_rects(ctypes.byref(d), _p(xys), _p(radii), _p(conics),
+                                         _p(opacities), _p(rects), None, _p(cum), _p(rws),
+                                         rws_bytes, _stream(dev)))
+        count = torch.empty(1, dtype=torch.int32, pin_memory=True)
+        count.copy_(cum.reshape(-1)[N - 1:N], non_blocking=True)
+        ready = torch.cuda.Event()
+        ready.record()
+        dws_bytes = L.jaxsplat_bin_depth_workspace_size(ctypes.byref(d))
+        dws = torch.empty(max(dws_bytes, 1), dtype=torch.uint8, device=dev)
+        _lib.check(L.jaxsplat_bin_depth_sort_rects(ctypes.byref(d), _p(depths), _p(radii),
+                                                   _p(rects), _p(dws), dws_bytes, _stream(dev)))
+        ready.synchronize()
+        isect = int(count[0])
+        ntiles = d.grid_dim_2d.x * d.grid_dim_2d.y
+        gids = torch.empty(max(isect, 1), dtype=torch.int32, device=dev)
+        bins = torch.empty((ntiles, 2), dtype=torch.int32, device=dev)
+        ws_bytes = L.jaxsplat_bin_emit_workspace_size(ctypes.byref(d), isect)
+        ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=dev)
+        _lib.check(L.jaxsplat_bin_emit_sort(ctypes.byref(d), isect, _p(dws), _p(gids), None,
+                                            _p(bins), _p(ws), ws_bytes, _stream(dev)))
+    return gids[:isect], bins, isect, rects, cum
+
+
 class _Rasterize(torch.autograd.Function):
     @staticmethod
     def forward(ctx, cfg, colors, opacities, background, xys, depths, radii, conics,
@@ -212,7 +259,12 @@ class _Rasterize(torch.autograd.Function):
         dev = colors.device
         H, W = img_shape
         d = _lib.make_descriptor(N, img_shape, (1.0, 1.0), (0.0, 0.0), 1.0, 0.0, block_width)
-        gids, bins, isect = _bin_overlapped(d, xys, depths, radii, cum_tiles_hit, dev)
+        rects = None
+        if TIGHT_TILES:
+            gids, bins, isect, rects, cum_tiles_hit = _bin_tight(d, xys, depths, radii, conics,
+                                                                 opacities, dev)
+        else:
+            gids, bins, isect = _bin_overlapped(d, xys, depths, radii, cum_tiles_hit, dev)
         final_Ts = torch.empty((H, W, 1), dtype=torch.float32, device=dev)
         final_idx = torch.empty((H, W, 1), dtype=torch.int32, device=dev)
         img = torch.empty((H, W, nch), dtype=torch.float32, device=dev)
@@ -225,14 +277,16 @@ class _Rasterize(torch.autograd.Function):
         ctx.d = d
         ctx.isect = isect
         ctx.adjoint = adjoint
+        ctx.tight = rects is not None
         ctx.save_for_backward(colors, opacities, background, xys, conics, gids, bins, final_Ts,
-                              final_idx, radii, cum_tiles_hit)
+                              final_idx, radii, cum_tiles_hit,
+                              rects if rects is not None else radii)
         return img
 
     @staticmethod
     def backward(ctx, v_img):
         colors, opacities, background, xys, conics, gids, bins, final_Ts, final_idx, radii, \
-            cum_tiles_hit = ctx.saved_tensors
+            cum_tiles_hit, rects = ctx.saved_tensors
         dev = colors.device
         v_img = v_img.contiguous()
         v_colors = torch.empty_like(colors)
@@ -248,12 +302,20 @@ class _Rasterize(torch.autograd.Function):
                                                                       ctx.isect)
             ws = torch.empty(max(wsb, 1), dtype=torch.uint8, device=dev)
             with torch.cuda.device(dev):
-                _lib.check(L.jaxsplat_rasterize_backward_staged(
-                    ctypes.byref(ctx.d), _p(gids), _p(bins), ctx.isect, _p(xys), _p(conics),
-                    _p(colors), _p(opacities), _p(background), nch, _p(radii),
-                    _p(cum_tiles_hit), _p(final_Ts), _p(final_idx), _p(v_img), None,
-                    _p(v_colors), _p(v_opacity), _p(v_xy), None, _p(v_conic), ctx.adjoint,
-                    _p(ws), wsb, _stream(dev)))
+                if ctx.tight:
+                    _lib.check(L.jaxsplat_rasterize_backward_staged_rects(
+                        ctypes.byref(ctx.d), _p(gids), _p(bins), ctx.isect, _p(xys), _p(conics),
+                        _p(colors), _p(opacities), _p(background), nch, _p(radii), _p(rects),
+                        _p(cum_tiles_hit), _p(final_Ts), _p(final_idx), _p(v_img), None,
+                        _p(v_colors), _p(v_opacity), _p(v_xy), None, _p(v_conic), ctx.adjoint,
+                        _p(ws), wsb, _stream(dev)))
+                else:
+                    _lib.check(L.jaxsplat_rasterize_backward_staged(
+                        ctypes.byref(ctx.d), _p(gids), _p(bins), ctx.isect, _p(xys), _p(conics),
+                        _p(colors), _p(opacities), _p(background), nch, _p(radii),
+                        _p(cum_tiles_hit), _p(final_Ts), _p(final_idx), _p(v_img), None,
+                        _p(v_colors), _p(v_opacity), _p(v_xy), None, _p(v_conic), ctx.adjoint,
+                        _p(ws), wsb, _stream(dev)))
             return None, v_colors, v_opacity, None, v_xy, None, None, v_conic, None
         with torch.cuda.device(dev):
             _lib.check(L.jaxsplat_rasterize_backward_nd(

@@ -34,15 +34,20 @@ namespace {
 // Per Gaussian (id order): the depth sort key and the packed tile rectangle.
 __global__ void depth_keys_kernel(Camera c, const float* __restrict__ depths,
                                   const int* __restrict__ radii, const float2* __restrict__ xys,
-                                  uint32_t* __restrict__ keys, uint2* __restrict__ rects) {
+                                  const uint2* __restrict__ rects_in, uint32_t* __restrict__ keys,
+                                  uint2* __restrict__ rects) {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= c.num_points) return;
     const int rad = radii[g];
     uint2 rc = make_uint2(0u, 0u);
     if (rad > 0) {
-        int x0, x1, y0, y1;
-        tile_rect(c.block_width, c.tiles_x, c.tiles_y, xys[g], rad, x0, x1, y0, y1);
-        rc = pack_rect(x0, x1, y0, y1);
+        if (rects_in) {
+            rc = rects_in[g];
+        } else {
+            int x0, x1, y0, y1;
+            tile_rect(c.block_width, c.tiles_x, c.tiles_y, xys[g], rad, x0, x1, y0, y1);
+            rc = pack_rect(x0, x1, y0, y1);
+        }
     }
     keys[g] = rad > 0 ? __float_as_uint(depths[g]) : 0xFFFFFFFFu;
     rects[g] = rc;
@@ -504,13 +509,13 @@ const uint32_t* bin_error_word(const BinPlan& p, const void* scratch) {
 }
 
 hipError_t bin_depth(const BinPlan& p, const Camera& c, const float* xys, const float* depths,
-                     const int* radii, void* depth_ws, hipStream_t s) {
+                     const int* radii, const uint2* rects, void* depth_ws, hipStream_t s) {
     if (p.n <= 0) return hipSuccess;
     const DepthLayout D = depth_layout(p);
     uint8_t* b = static_cast<uint8_t*>(depth_ws);
     auto U = [&](size_t off) { return reinterpret_cast<uint32_t*>(b + off); };
     depth_keys_kernel<<<(p.n + 255) / 256, 256, 0, s>>>(
-        c, depths, radii, reinterpret_cast<const float2*>(xys), U(D.dkeys),
+        c, depths, radii, reinterpret_cast<const float2*>(xys), rects, U(D.dkeys),
         reinterpret_cast<uint2*>(b + D.rect_by_id));
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
@@ -556,7 +561,7 @@ hipError_t bin_and_sort(const BinPlan& p, const Camera& c, const float* xys, con
                         int* gids_sorted, int* tiles_sorted, int* tile_bins, void* scratch,
                         hipStream_t s) {
     uint8_t* depth_ws = static_cast<uint8_t*>(scratch) + p.emit_bytes;
-    hipError_t e = bin_depth(p, c, xys, depths, radii, depth_ws, s);
+    hipError_t e = bin_depth(p, c, xys, depths, radii, nullptr, depth_ws, s);
     if (e != hipSuccess) return e;
     return bin_emit(p, c, num_isect_host, num_isect_dev, depth_ws, gids_sorted, tiles_sorted,
                     tile_bins, scratch, s);

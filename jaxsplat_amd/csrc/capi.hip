@@ -177,6 +177,28 @@ size_t jaxsplat_project_workspace_size(const jaxsplat_descriptor* d) {
     return d ? align256(jsa::project_fwd_scratch_bytes((int)d->num_points)) : 0;
 }
 
+size_t jaxsplat_tile_rects_workspace_size(const jaxsplat_descriptor* d) {
+    return jaxsplat_project_workspace_size(d);
+}
+
+int jaxsplat_tile_rects(const jaxsplat_descriptor* d, const float* xys, const int32_t* radii,
+                        const float* conics, const float* opacities, uint32_t* tile_rects,
+                        int32_t* num_tiles_hit, int32_t* cum_tiles_hit, void* workspace,
+                        size_t workspace_bytes, void* stream) {
+    g_err.clear();
+    if (int st = validate(d)) return st;
+    if (d->num_points == 0) return JAXSPLAT_OK;
+    if (!xys || !radii || !conics || !opacities || !tile_rects || !cum_tiles_hit)
+        return fail(JAXSPLAT_ERR_INVALID, "tile_rects: NULL required pointer");
+    if (!workspace || workspace_bytes < jaxsplat_tile_rects_workspace_size(d))
+        return fail(JAXSPLAT_ERR_WORKSPACE, "tile_rects: workspace too small");
+    JSA_CHECK_HIP(jsa::tile_rects(camera_of(d), xys, radii, conics, opacities,
+                                  reinterpret_cast<uint2*>(tile_rects), num_tiles_hit,
+                                  cum_tiles_hit, workspace, static_cast<hipStream_t>(stream)),
+                  "tile_rects");
+    return JAXSPLAT_OK;
+}
+
 int jaxsplat_project_forward(const jaxsplat_descriptor* d, const float* means3d,
                              const float* scales, const float* quats, const float* viewmat,
                              float* cov3d, float* xys, float* depths, int32_t* radii,
@@ -279,9 +301,31 @@ int jaxsplat_bin_depth_sort(const jaxsplat_descriptor* d, const float* xys, cons
     if (!depth_workspace || depth_workspace_bytes < p.depth_bytes)
         return fail(JAXSPLAT_ERR_WORKSPACE, "bin_depth_sort: workspace too small (%zu < %zu)",
                     depth_workspace_bytes, p.depth_bytes);
-    JSA_CHECK_HIP(jsa::bin_depth(p, c, xys, depths, radii, depth_workspace,
+    JSA_CHECK_HIP(jsa::bin_depth(p, c, xys, depths, radii, nullptr, depth_workspace,
                                  static_cast<hipStream_t>(stream)),
                   "bin_depth_sort");
+    return JAXSPLAT_OK;
+}
+
+int jaxsplat_bin_depth_sort_rects(const jaxsplat_descriptor* d, const float* depths,
+                                  const int32_t* radii, const uint32_t* tile_rects,
+                                  void* depth_workspace, size_t depth_workspace_bytes,
+                                  void* stream) {
+    g_err.clear();
+    if (int st = validate(d)) return st;
+    if (d->num_points == 0) return JAXSPLAT_OK;
+    const jsa::Camera c = camera_of(d);
+    const jsa::BinPlan p = jsa::bin_plan(c, 0);
+    if (!depths || !radii || !tile_rects)
+        return fail(JAXSPLAT_ERR_INVALID, "bin_depth_sort_rects: NULL required pointer");
+    if (!depth_workspace || depth_workspace_bytes < p.depth_bytes)
+        return fail(JAXSPLAT_ERR_WORKSPACE,
+                    "bin_depth_sort_rects: workspace too small (%zu < %zu)",
+                    depth_workspace_bytes, p.depth_bytes);
+    JSA_CHECK_HIP(jsa::bin_depth(p, c, nullptr, depths, radii,
+                                 reinterpret_cast<const uint2*>(tile_rects), depth_workspace,
+                                 static_cast<hipStream_t>(stream)),
+                  "bin_depth_sort_rects");
     return JAXSPLAT_OK;
 }
 
@@ -511,11 +555,54 @@ int jaxsplat_rasterize_backward_staged(
     JSA_CHECK_HIP(jsa::raster_bwd_staged(camera_of(d), (int)channels, gaussian_ids_sorted,
                                          tile_bins, (int)num_intersects, xys, conics, colors,
                                          opacities, background, final_Ts, final_idx, v_out_img,
-                                         v_out_img_alpha, radii, cum_tiles_hit, workspace,
-                                         v_colors, v_opacity, v_xy, v_xy_abs, v_conic,
+                                         v_out_img_alpha, radii, nullptr, cum_tiles_hit,
+                                         workspace, v_colors, v_opacity, v_xy, v_xy_abs, v_conic,
                                          adjoint == JAXSPLAT_ADJOINT_CONSISTENT,
                                          static_cast<hipStream_t>(stream)),
                   "rasterize_backward_staged");
+    return JAXSPLAT_OK;
+}
+
+int jaxsplat_rasterize_backward_staged_rects(
+    const jaxsplat_descriptor* d, const int32_t* gaussian_ids_sorted, const int32_t* tile_bins,
+    uint32_t num_intersects, const float* xys, const float* conics, const float* colors,
+    const float* opacities, const float* background, uint32_t channels, const int32_t* radii,
+    const uint32_t* tile_rects, const int32_t* cum_tiles_hit, const float* final_Ts,
+    const int32_t* final_idx, const float* v_out_img, const float* v_out_img_alpha,
+    float* v_colors, float* v_opacity, float* v_xy, float* v_xy_abs, float* v_conic, int adjoint,
+    void* workspace, size_t workspace_bytes, void* stream) {
+    g_err.clear();
+    if (int st = validate(d)) return st;
+    if (channels < 1 || channels > 3)
+        return fail(JAXSPLAT_ERR_INVALID, "rasterize_backward_staged_rects: channels must be in "
+                    "[1, 3], got %u", channels);
+    if (num_intersects >= (1u << 30))
+        return fail(JAXSPLAT_ERR_CAPACITY, "num_intersects %u exceeds 2^30", num_intersects);
+    if (adjoint != JAXSPLAT_ADJOINT_REFERENCE && adjoint != JAXSPLAT_ADJOINT_CONSISTENT)
+        return fail(JAXSPLAT_ERR_INVALID, "unknown adjoint mode %d", adjoint);
+    if (!v_colors || !v_opacity || !v_xy || !v_conic)
+        return fail(JAXSPLAT_ERR_INVALID,
+                    "rasterize_backward_staged_rects: NULL gradient pointer");
+    if (d->num_points == 0) return JAXSPLAT_OK;
+    if (!xys || !radii || !tile_rects || !cum_tiles_hit ||
+        (num_intersects > 0 && (!gaussian_ids_sorted || !tile_bins || !conics || !colors ||
+                                !opacities || !background || !final_Ts || !final_idx ||
+                                !v_out_img)))
+        return fail(JAXSPLAT_ERR_INVALID, "rasterize_backward_staged_rects: NULL required pointer");
+    const size_t need = align256(jsa::raster_bwd_staged_bytes((int)num_intersects));
+    if (!workspace || workspace_bytes < need)
+        return fail(JAXSPLAT_ERR_WORKSPACE,
+                    "rasterize_backward_staged_rects: workspace too small (%zu < %zu)",
+                    workspace_bytes, need);
+    JSA_CHECK_HIP(jsa::raster_bwd_staged(camera_of(d), (int)channels, gaussian_ids_sorted,
+                                         tile_bins, (int)num_intersects, xys, conics, colors,
+                                         opacities, background, final_Ts, final_idx, v_out_img,
+                                         v_out_img_alpha, radii,
+                                         reinterpret_cast<const uint2*>(tile_rects),
+                                         cum_tiles_hit, workspace, v_colors, v_opacity, v_xy,
+                                         v_xy_abs, v_conic, adjoint == JAXSPLAT_ADJOINT_CONSISTENT,
+                                         static_cast<hipStream_t>(stream)),
+                  "rasterize_backward_staged_rects");
     return JAXSPLAT_OK;
 }
 

@@ -147,6 +147,51 @@ __device__ __forceinline__ uint32_t rect_count(uint2 rc) {
     return (rc.y & 0xFFFFu) * (rc.y >> 16);
 }
 
+__device__ __forceinline__ void unpack_rect(uint2 rc, int& x0, int& x1, int& y0, int& y1) {
+    x0 = (int)(rc.x & 0xFFFFu);
+    y0 = (int)(rc.x >> 16);
+    x1 = x0 + (int)(rc.y & 0xFFFFu);
+    y1 = y0 + (int)(rc.y >> 16);
+}
+
+// Tile rectangle of a Gaussian (centre xy, radius rad > 0, conic (a, b, c),
+// opacity o) restricted to the tiles holding a pixel centre it can blend
+// into: those where o exp(-sigma) >= 1/255, i.e. the ellipse
+// sigma = (a dx^2 + c dy^2) / 2 + b dx dy <= ln(255 o), whose extents are
+// sqrt(2 ln(255 o) cov_xx) and sqrt(2 ln(255 o) cov_yy) (cov = conic^-1).
+// The limit and the extents get generous margins (1% + 0.05 on the limit,
+// 0.1% + 0.05 px on the extents: float32 rounding of sigma at a pixel stays
+// far below, even with cancellation in strongly elongated conics).  A
+// Gaussian with log2(255 o) < 0 -- culled whole by the rasterizer -- gets an
+// empty rectangle; a conic that is not positive definite keeps the full one.
+// Packed as pack_rect.
+__device__ __forceinline__ uint2 opacity_tile_rect(int bw, int tiles_x, int tiles_y, float2 xy,
+                                                   int rad, float a, float b, float c, float o) {
+    int x0, x1, y0, y1;
+    tile_rect(bw, tiles_x, tiles_y, xy, rad, x0, x1, y0, y1);
+    const float lim2 = __builtin_amdgcn_logf(255.f * o);
+    if (!(lim2 >= 0.f)) return make_uint2((uint32_t)x0 | ((uint32_t)y0 << 16), 0u);
+    const float det = a * c - b * b;
+    if (a > 0.f && c > 0.f && det > 0.f) {
+        const float s = lim2 * 0.69314718f * 1.01f + 0.05f;     // ln(255 o) with margin
+        const float hx = sqrtf(2.f * s * c / det) * 1.001f + 0.05f;
+        const float hy = sqrtf(2.f * s * a / det) * 1.001f + 0.05f;
+        const float bwf = (float)bw, inv = 1.f / bwf;
+        // tile t holds the pixel centres [t bw + 0.5, t bw + bw - 0.5]
+        const float fx0 = fminf(fmaxf(ceilf((xy.x - hx - bwf + 0.5f) * inv), -1.f), 65535.f);
+        const float fx1 = fminf(fmaxf(floorf((xy.x + hx - 0.5f) * inv) + 1.f, -1.f), 65535.f);
+        const float fy0 = fminf(fmaxf(ceilf((xy.y - hy - bwf + 0.5f) * inv), -1.f), 65535.f);
+        const float fy1 = fminf(fmaxf(floorf((xy.y + hy - 0.5f) * inv) + 1.f, -1.f), 65535.f);
+        x0 = max(x0, (int)fx0);
+        x1 = min(x1, (int)fx1);
+        y0 = max(y0, (int)fy0);
+        y1 = min(y1, (int)fy1);
+        if (x1 < x0) x1 = x0;
+        if (y1 < y0) y1 = y0;
+    }
+    return pack_rect(x0, x1, y0, y1);
+}
+
 // ---------------------------------------------------------------------------
 // Host-side internal interface (implemented in the .hip files)
 // ---------------------------------------------------------------------------
@@ -215,6 +260,12 @@ hipError_t project_fwd(const Camera& c, const float* means3d, const float* scale
                        float* xys, float* depths, int* radii, float* conics,
                        float* compensation, int* num_tiles_hit, int* cum_tiles_hit,
                        void* scratch, hipStream_t s);
+// Opacity-limited tile rectangles (opacity_tile_rect) of projected Gaussians,
+// their counts (num_tiles_hit optional) and inclusive sums; scratch
+// project_fwd_scratch_bytes(c.num_points).
+hipError_t tile_rects(const Camera& c, const float* xys, const int* radii, const float* conics,
+                      const float* opacities, uint2* rects, int* num_tiles_hit,
+                      int* cum_tiles_hit, void* scratch, hipStream_t s);
 hipError_t project_bwd(const Camera& c, const float* means3d, const float* scales,
                        const float* quats, const float* viewmat, const int* radii,
                        const float* conics, const float* compensation,
@@ -256,8 +307,10 @@ struct BinPlan {
 };
 BinPlan bin_plan(const Camera& c, int isect_cap);
 // Depth stage: depth keys, packed tile rectangles and the stable depth sort.
+// rects (optional, packed as pack_rect) replaces the rectangles computed from
+// xys and radii.
 hipError_t bin_depth(const BinPlan& p, const Camera& c, const float* xys, const float* depths,
-                     const int* radii, void* depth_ws, hipStream_t s);
+                     const int* radii, const uint2* rects, void* depth_ws, hipStream_t s);
 // Emission stage over the depth stage's workspace: emission, tile sort, edges.
 hipError_t bin_emit(const BinPlan& p, const Camera& c, int num_isect_host,
                     const int* num_isect_dev, const void* depth_ws, int* gids_sorted,
@@ -338,12 +391,15 @@ hipError_t raster_fwd_grid(const RasterGrid& g, const int* items_sorted, const i
 // StagedOut in raster.hip): nch <= 3 channels, workspace raster_bwd_staged_bytes;
 // radii and cum_tiles_hit are the projection's.  Writes every gradient row.
 size_t raster_bwd_staged_bytes(int num_isect);
+// rects (optional, packed as pack_rect) replaces the rectangles computed from
+// xys and radii (and must be the ones the binning used).
 hipError_t raster_bwd_staged(const Camera& c, int nch, const int* gids_sorted,
                              const int* tile_bins, int num_isect, const float* xys,
                              const float* conics, const float* colors, const float* opacities,
                              const float* background, const float* final_Ts,
                              const int* final_idx, const float* v_out, const float* v_out_alpha,
-                             const int* radii, const int* cum_tiles_hit, void* ws,
+                             const int* radii, const uint2* rects, const int* cum_tiles_hit,
+                             void* ws,
                              float* v_colors, float* v_opacity, float* v_xy, float* v_xy_abs,
                              float* v_conic, int consistent, hipStream_t s);
 // Batch backward with staged gradients (3 channels): records at the slots

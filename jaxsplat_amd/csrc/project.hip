@@ -360,6 +360,58 @@ __global__ __launch_bounds__(256) void project_fwd_kernel(
     if (tid == 0) part_tot[part] = total;
 }
 
+// Opacity-limited tile rectangles (opacity_tile_rect) of the projected
+// Gaussians, their tile counts and the partition-local inclusive sums of the
+// counts (finished by project_scan_kernel / project_scan_fixup_kernel), laid
+// out like project_fwd_kernel's.  Gaussians with radius 0 get empty rectangles.
+__global__ __launch_bounds__(256) void tile_rects_kernel(
+    Camera c, const float2* __restrict__ xys, const int* __restrict__ radii,
+    const float* __restrict__ conics, const float* __restrict__ opacities,
+    uint2* __restrict__ rects, int* __restrict__ num_tiles_hit, int* __restrict__ cum_tiles_hit,
+    uint32_t* __restrict__ part_tot) {
+    __shared__ uint32_t s_wave_tot[kProjPer][4];
+    const int tid = threadIdx.x;
+    const int lane = lane_id();
+    const int wave = tid >> 6;
+    const int part = blockIdx.x;
+    int incl[kProjPer];
+    int cnt[kProjPer];
+#pragma unroll
+    for (int k = 0; k < kProjPer; ++k) {
+        const int idx = part * kProjPart + k * 256 + tid;
+        cnt[k] = 0;
+        if (idx < c.num_points) {
+            const int rad = radii[idx];
+            uint2 rc = make_uint2(0u, 0u);
+            if (rad > 0)
+                rc = opacity_tile_rect(c.block_width, c.tiles_x, c.tiles_y, xys[idx], rad,
+                                       conics[3 * idx + 0], conics[3 * idx + 1],
+                                       conics[3 * idx + 2], opacities[idx]);
+            rects[idx] = rc;
+            cnt[k] = (int)rect_count(rc);
+            if (num_tiles_hit) num_tiles_hit[idx] = cnt[k];
+        }
+        incl[k] = wave_inclusive_sum(cnt[k]);
+        if (lane == 63) s_wave_tot[k][wave] = (uint32_t)incl[k];
+    }
+    __syncthreads();
+    uint32_t total = 0;
+#pragma unroll
+    for (int k = 0; k < kProjPer; ++k) {
+        uint32_t w_off = 0, slab = 0;
+#pragma unroll
+        for (int w = 0; w < 4; ++w) {
+            const uint32_t v = s_wave_tot[k][w];
+            w_off += (w < wave) ? v : 0u;
+            slab += v;
+        }
+        const int idx = part * kProjPart + k * 256 + tid;
+        if (idx < c.num_points) cum_tiles_hit[idx] = (int)(total + w_off + (uint32_t)incl[k]);
+        total += slab;
+    }
+    if (tid == 0) part_tot[part] = total;
+}
+
 // Exclusive scan of the partition totals, one workgroup of 1024 threads
 // (thread t scans a contiguous run of partitions).
 __global__ __launch_bounds__(1024) void project_scan_kernel(const uint32_t* __restrict__ part_tot,
@@ -689,6 +741,26 @@ hipError_t project_fwd(const Camera& c, const float* means3d, const float* scale
     project_fwd_kernel<<<parts, 256, 0, s>>>(c, means3d, scales, quats, viewmat, cov3d, xys,
                                              depths, radii, conics, compensation, num_tiles_hit,
                                              cum_tiles_hit, part_tot);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || parts == 1) return e;
+    project_scan_kernel<<<1, 1024, 0, s>>>(part_tot, parts, part_off);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    const int rest = c.num_points - kProjPart;
+    project_scan_fixup_kernel<<<(rest + 255) / 256, 256, 0, s>>>(c.num_points, part_off,
+                                                                 cum_tiles_hit);
+    return hipGetLastError();
+}
+
+hipError_t tile_rects(const Camera& c, const float* xys, const int* radii, const float* conics,
+                      const float* opacities, uint2* rects, int* num_tiles_hit,
+                      int* cum_tiles_hit, void* scratch, hipStream_t s) {
+    if (c.num_points <= 0) return hipSuccess;
+    const int parts = (c.num_points + kProjPart - 1) / kProjPart;
+    uint32_t* part_tot = static_cast<uint32_t*>(scratch);
+    uint32_t* part_off = part_tot + parts;
+    tile_rects_kernel<<<parts, 256, 0, s>>>(c, reinterpret_cast<const float2*>(xys), radii,
+                                            conics, opacities, rects, num_tiles_hit,
+                                            cum_tiles_hit, part_tot);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || parts == 1) return e;
     project_scan_kernel<<<1, 1024, 0, s>>>(part_tot, parts, part_off);

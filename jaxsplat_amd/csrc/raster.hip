@@ -511,6 +511,7 @@ struct StagedOut {
     uint32_t* bits;         // one bit per slot, zero before the backward
     const int* cum;         // cum_tiles_hit (inclusive), used when first is null
     const uint32_t* first;  // a batch's first slot per item (exclusive)
+    const uint2* rects;     // packed tile rectangles; null: from xys and radii
     const int* radii;
     int n_slots;            // number of intersections; a slot outside is not written
 };
@@ -625,7 +626,10 @@ __global__ __launch_bounds__(128) void raster_bwd_kernel(
                 int slot = -1;
                 if (STAGED) {
                     int x0, x1, y0, y1;
-                    tile_rect(bw, vr.tiles_x, vr.tiles_y, xy, st.radii[it], x0, x1, y0, y1);
+                    if (st.rects)
+                        unpack_rect(st.rects[it], x0, x1, y0, y1);
+                    else
+                        tile_rect(bw, vr.tiles_x, vr.tiles_y, xy, st.radii[it], x0, x1, y0, y1);
                     const int w = x1 - x0, lx = tx - x0, ly = ty - y0;
                     const int base = st.first ? (int)st.first[it] : st.cum[it] - w * (y1 - y0);
                     const int sl = base + ly * w + lx;
@@ -847,7 +851,7 @@ __device__ __forceinline__ void sum_records(int off, int end, const float4* __re
 template <bool ABS>
 __global__ __launch_bounds__(256) void raster_bwd_reduce_kernel(
     Camera c, int num_isect, const float2* __restrict__ xys, const int* __restrict__ radii,
-    const int* __restrict__ cum, const float4* __restrict__ rec,
+    const uint2* __restrict__ rects, const int* __restrict__ cum, const float4* __restrict__ rec,
     const uint32_t* __restrict__ bits, int nch, float* __restrict__ v_rgb,
     float* __restrict__ v_opacity, float* __restrict__ v_xy, float* __restrict__ v_xy_abs,
     float* __restrict__ v_conic) {
@@ -859,7 +863,10 @@ __global__ __launch_bounds__(256) void raster_bwd_reduce_kernel(
     int off = 0, end = 0;
     if (own && radii[g] > 0) {
         int x0, x1, y0, y1;
-        tile_rect(c.block_width, c.tiles_x, c.tiles_y, xys[g], radii[g], x0, x1, y0, y1);
+        if (rects)
+            unpack_rect(rects[g], x0, x1, y0, y1);
+        else
+            tile_rect(c.block_width, c.tiles_x, c.tiles_y, xys[g], radii[g], x0, x1, y0, y1);
         end = cum[g];
         off = end - (x1 - x0) * (y1 - y0);
         if (off < 0 || end > num_isect) off = end = 0;     // inconsistent input: zeros
@@ -1094,7 +1101,8 @@ hipError_t raster_bwd_staged(const Camera& c, int nch, const int* gids_sorted,
                              const float* conics, const float* colors, const float* opacities,
                              const float* background, const float* final_Ts,
                              const int* final_idx, const float* v_out, const float* v_out_alpha,
-                             const int* radii, const int* cum_tiles_hit, void* ws,
+                             const int* radii, const uint2* rects, const int* cum_tiles_hit,
+                             void* ws,
                              float* v_colors, float* v_opacity, float* v_xy, float* v_xy_abs,
                              float* v_conic, int consistent, hipStream_t s) {
     if (c.num_points <= 0) return hipSuccess;
@@ -1106,6 +1114,7 @@ hipError_t raster_bwd_staged(const Camera& c, int nch, const int* gids_sorted,
     st.bits = reinterpret_cast<uint32_t*>(w + ((48 * n + 255) & ~size_t(255)));
     st.cum = cum_tiles_hit;
     st.first = nullptr;
+    st.rects = rects;
     st.radii = radii;
     st.n_slots = num_isect;
     hipError_t e = hipSuccess;
@@ -1142,12 +1151,12 @@ hipError_t raster_bwd_staged(const Camera& c, int nch, const int* gids_sorted,
     const float2* x2 = reinterpret_cast<const float2*>(xys);
     if (v_xy_abs)
         raster_bwd_reduce_kernel<true><<<blocks, 256, 0, s>>>(
-            c, num_isect, x2, radii, cum_tiles_hit, st.rec, st.bits, nch, v_colors, v_opacity,
-            v_xy, v_xy_abs, v_conic);
+            c, num_isect, x2, radii, rects, cum_tiles_hit, st.rec, st.bits, nch, v_colors,
+            v_opacity, v_xy, v_xy_abs, v_conic);
     else
         raster_bwd_reduce_kernel<false><<<blocks, 256, 0, s>>>(
-            c, num_isect, x2, radii, cum_tiles_hit, st.rec, st.bits, nch, v_colors, v_opacity,
-            v_xy, nullptr, v_conic);
+            c, num_isect, x2, radii, rects, cum_tiles_hit, st.rec, st.bits, nch, v_colors,
+            v_opacity, v_xy, nullptr, v_conic);
     return hipGetLastError();
 }
 
@@ -1171,6 +1180,7 @@ hipError_t raster_bwd_batch_staged(const RasterGrid& g, int max_view_n, const in
     st.bits = reinterpret_cast<uint32_t*>(w + ((48 * n + 255) & ~size_t(255)));
     st.cum = nullptr;
     st.first = item_first;
+    st.rects = nullptr;
     st.radii = radii;
     st.n_slots = cap;
     hipError_t e = fill_zero(st.bits, 4 * ((n + 31) / 32), s);
