@@ -13,7 +13,9 @@ Per view the image equals the plain ``render`` of that view with the same
 Gaussians are summed over its views.
 
 Intersection capacity: the workspace is sized for ``isect_capacity``
-(view, tile) intersections.  ``BatchPlan.check()`` (a host synchronisation)
+(view, tile) intersections, counted on the opacity-limited tile rectangles
+that ``rasterize`` bins into (the reference's rectangles without the tiles
+where every pixel fails the alpha test).  ``BatchPlan.check()`` (a host synchronisation)
 reports the exact count and raises ``JaxsplatError`` with code
 ``ERR_CAPACITY`` when a forward overflowed; ``BatchRenderer`` sizes the
 capacity on its first call and grows it when a check finds an overflow.

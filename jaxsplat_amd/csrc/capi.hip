@@ -925,7 +925,8 @@ int jaxsplat_batch_forward(const jaxsplat_batch* b, const float* means3d, const 
     int* bins = reinterpret_cast<int*>(w + b->bins);
     JSA_CHECK_HIP(jsa::project_batch_fwd(b->d_views, b->num_views, b->max_view_n, b->glob_scale,
                                          b->clip_thresh, b->block_width, means3d, scales, quats,
-                                         viewmats, xys, conics, radii, dkeys, rects, s),
+                                         viewmats, opacities, xys, conics, radii, dkeys, rects,
+                                         s),
                   "batch_forward: project");
     JSA_CHECK_HIP(jsa::batch_bin(b->bin, b->d_views, rects, dkeys, items, bins,
                                  reinterpret_cast<uint32_t*>(w + b->item_first),
@@ -970,8 +971,9 @@ int jaxsplat_batch_backward(const jaxsplat_batch* b, const float* means3d, const
     JSA_CHECK_HIP(jsa::raster_bwd_batch_staged(batch_grid(b), b->max_view_n, items, bins,
                                                b->isect_cap, xys, conics, colors, opacities,
                                                backgrounds, final_Ts, final_idx, v_out_img,
-                                               radii, item_first, w + b->staged, v_xy, v_conic,
-                                               v_rgbo, cons, s),
+                                               radii, reinterpret_cast<const uint2*>(w + b->rects),
+                                               item_first, w + b->staged, v_xy, v_conic, v_rgbo,
+                                               cons, s),
                   "batch_backward: rasterize");
     JSA_CHECK_HIP(jsa::project_batch_bwd(b->d_scenes, b->num_scenes, b->max_scene_n, b->d_views,
                                          b->glob_scale, b->clip_thresh, means3d, scales, quats,

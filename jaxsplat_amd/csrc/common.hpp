@@ -326,12 +326,15 @@ const uint32_t* bin_error_word(const BinPlan& p, const void* scratch);
 // ---- batched projection (views of scenes, see ViewRec) ----
 // Forward: items of every view; writes xys, conics, radii, the depth sort
 // key (depth bits, 0xFFFFFFFF for items that emit nothing) and the packed tile
-// rectangle (pack_rect; 0 for items that emit nothing).
+// rectangle (pack_rect; 0 for items that emit nothing): with opacities
+// (optional) the opacity-limited one (opacity_tile_rect), as the single-view
+// rasterize path bins.
 hipError_t project_batch_fwd(const ViewRec* views, int num_views, int max_view_n,
                              float glob_scale, float clip_thresh, int block_width,
                              const float* means3d, const float* scales, const float* quats,
-                             const float* viewmats, float* xys, float* conics, int* radii,
-                             uint32_t* depth_keys, uint2* item_rects, hipStream_t s);
+                             const float* viewmats, const float* opacities, float* xys,
+                             float* conics, int* radii, uint32_t* depth_keys, uint2* item_rects,
+                             hipStream_t s);
 // Backward: one lane per Gaussian, summing the contributions of its scene's
 // views; writes v_mean3d, v_scale, v_quat for every Gaussian of every scene,
 // and with v_rgbo (per item: rgb, opacity; optional) also v_colors and
@@ -404,15 +407,16 @@ hipError_t raster_bwd_staged(const Camera& c, int nch, const int* gids_sorted,
                              float* v_conic, int consistent, hipStream_t s);
 // Batch backward with staged gradients (3 channels): records at the slots
 // item_first[item] + the tile's row-major offset in the item's rectangle
-// (workspace raster_bwd_staged_bytes(cap)), summed per item into v_xy, v_conic
-// and v_rgbo (4 floats per item: rgb, opacity); every item row is written.
-// g.views is the device view table; max_view_n the largest view's item count.
+// (item_rects, the ones the binning used; workspace raster_bwd_staged_bytes(cap)),
+// summed per item into v_xy, v_conic and v_rgbo (4 floats per item: rgb,
+// opacity); every item row is written.  g.views is the device view table;
+// max_view_n the largest view's item count.
 hipError_t raster_bwd_batch_staged(const RasterGrid& g, int max_view_n, const int* items_sorted,
                                    const int* tile_bins, int cap, const float* xys,
                                    const float* conics, const float* colors,
                                    const float* opacities, const float* backgrounds,
                                    const float* final_Ts, const int* final_idx,
-                                   const float* v_out, const int* radii,
+                                   const float* v_out, const int* radii, const uint2* item_rects,
                                    const uint32_t* item_first, void* ws, float* v_xy,
                                    float* v_conic, float* v_rgbo, int consistent,
                                    hipStream_t s);

@@ -283,8 +283,8 @@ __global__ __launch_bounds__(256) void project_batch_fwd_kernel(
     const ViewRec* __restrict__ views, float glob_scale, float clip_thresh, int block_width,
     const float* __restrict__ means3d, const float* __restrict__ scales,
     const float* __restrict__ quats, const float* __restrict__ viewmats,
-    float* __restrict__ xys, float* __restrict__ conics, int* __restrict__ radii,
-    uint32_t* __restrict__ dkeys, uint2* __restrict__ rects) {
+    const float* __restrict__ opacities, float* __restrict__ xys, float* __restrict__ conics,
+    int* __restrict__ radii, uint32_t* __restrict__ dkeys, uint2* __restrict__ rects) {
     const ViewRec vr = views[blockIdx.y];
     const int first = blockIdx.x * kProjPart;
     if (first >= vr.n) return;
@@ -307,7 +307,13 @@ __global__ __launch_bounds__(256) void project_batch_fwd_kernel(
         conics[3 * it + 2] = r.conic[2];
         radii[it] = r.rad;
         dkeys[it] = r.rad > 0 ? __float_as_uint(r.depth) : 0xFFFFFFFFu;
-        rects[it] = r.rect;
+        // opacity-limited: the tiles where the per-pixel alpha test can pass
+        rects[it] = (opacities && r.rad > 0)
+                        ? opacity_tile_rect(block_width, c.tiles_x, c.tiles_y,
+                                            make_float2(r.cx, r.cy), r.rad, r.conic[0],
+                                            r.conic[1], r.conic[2],
+                                            opacities[(size_t)vr.g0 + local])
+                        : r.rect;
     }
 }
 
@@ -789,13 +795,14 @@ hipError_t project_bwd(const Camera& c, const float* means3d, const float* scale
 hipError_t project_batch_fwd(const ViewRec* views, int num_views, int max_view_n,
                              float glob_scale, float clip_thresh, int block_width,
                              const float* means3d, const float* scales, const float* quats,
-                             const float* viewmats, float* xys, float* conics, int* radii,
-                             uint32_t* depth_keys, uint2* item_rects, hipStream_t s) {
+                             const float* viewmats, const float* opacities, float* xys,
+                             float* conics, int* radii, uint32_t* depth_keys, uint2* item_rects,
+                             hipStream_t s) {
     if (num_views <= 0 || max_view_n <= 0) return hipSuccess;
     const dim3 grid((max_view_n + kProjPart - 1) / kProjPart, num_views);
     project_batch_fwd_kernel<<<grid, 256, 0, s>>>(views, glob_scale, clip_thresh, block_width,
-                                                  means3d, scales, quats, viewmats, xys, conics,
-                                                  radii, depth_keys, item_rects);
+                                                  means3d, scales, quats, viewmats, opacities,
+                                                  xys, conics, radii, depth_keys, item_rects);
     return hipGetLastError();
 }
 

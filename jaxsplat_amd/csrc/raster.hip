@@ -895,8 +895,8 @@ __global__ __launch_bounds__(256) void raster_bwd_reduce_kernel(
 // v_rgbo = (rgb, opacity); every item row of every view is written (zeros for
 // items without records).  Slots beyond n_slots yield zeros.
 __global__ __launch_bounds__(256) void raster_bwd_batch_reduce_kernel(
-    const ViewRec* __restrict__ views, int bw, int n_slots, const float2* __restrict__ xys,
-    const int* __restrict__ radii, const uint32_t* __restrict__ item_first,
+    const ViewRec* __restrict__ views, int n_slots, const int* __restrict__ radii,
+    const uint2* __restrict__ item_rects, const uint32_t* __restrict__ item_first,
     const float4* __restrict__ rec, const uint32_t* __restrict__ bits, float* __restrict__ v_xy,
     float* __restrict__ v_conic, float4* __restrict__ v_rgbo) {
     const ViewRec vr = views[blockIdx.y];
@@ -908,9 +908,7 @@ __global__ __launch_bounds__(256) void raster_bwd_batch_reduce_kernel(
     float4 a0 = make_float4(0.f, 0.f, 0.f, 0.f), a1 = a0, a2 = a0;
     int off = 0, end = 0;
     if (own && radii[it] > 0) {
-        int x0, x1, y0, y1;
-        tile_rect(bw, vr.tiles_x, vr.tiles_y, xys[it], radii[it], x0, x1, y0, y1);
-        const int cnt = (x1 - x0) * (y1 - y0);
+        const int cnt = (int)rect_count(item_rects[it]);
         if (cnt > 0) {
             const long long f = item_first[it];
             if (f + cnt <= (long long)n_slots) {
@@ -1165,7 +1163,7 @@ hipError_t raster_bwd_batch_staged(const RasterGrid& g, int max_view_n, const in
                                    const float* conics, const float* colors,
                                    const float* opacities, const float* backgrounds,
                                    const float* final_Ts, const int* final_idx,
-                                   const float* v_out, const int* radii,
+                                   const float* v_out, const int* radii, const uint2* item_rects,
                                    const uint32_t* item_first, void* ws, float* v_xy,
                                    float* v_conic, float* v_rgbo, int consistent,
                                    hipStream_t s) {
@@ -1180,7 +1178,7 @@ hipError_t raster_bwd_batch_staged(const RasterGrid& g, int max_view_n, const in
     st.bits = reinterpret_cast<uint32_t*>(w + ((48 * n + 255) & ~size_t(255)));
     st.cum = nullptr;
     st.first = item_first;
-    st.rects = nullptr;
+    st.rects = item_rects;
     st.radii = radii;
     st.n_slots = cap;
     hipError_t e = fill_zero(st.bits, 4 * ((n + 31) / 32), s);
@@ -1212,7 +1210,7 @@ hipError_t raster_bwd_batch_staged(const RasterGrid& g, int max_view_n, const in
     if ((e = hipGetLastError()) != hipSuccess) return e;
     const dim3 rgrid((max_view_n + 255) / 256, g.num_views);
     raster_bwd_batch_reduce_kernel<<<rgrid, 256, 0, s>>>(
-        g.views, g.block_width, cap, x2, radii, item_first, st.rec, st.bits, v_xy, v_conic,
+        g.views, cap, radii, item_rects, item_first, st.rec, st.bits, v_xy, v_conic,
         reinterpret_cast<float4*>(v_rgbo));
     return hipGetLastError();
 }

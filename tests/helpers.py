@@ -96,6 +96,26 @@ def gpu_bin(xys, depths, radii, cum_tiles_hit, *, img_shape, block_width=16, spl
     return host(gids)[:isect], host(tiles)[:isect], host(bins), isect
 
 
+def gpu_tile_rects(xys, radii, conics, opacities, *, img_shape, block_width=16):
+    """jaxsplat_tile_rects: opacity-limited rectangles (N, 2) uint32 and the
+    inclusive sums of their tile counts."""
+    t = torch()
+    N = xys.shape[0]
+    d = descriptor(N, img_shape, block_width=block_width)
+    L = _lib.lib()
+    rects = t.empty((max(N, 1), 2), dtype=t.int32, device="cuda")
+    cum = t.empty((max(N, 1),), dtype=t.int32, device="cuda")
+    wsb = L.jaxsplat_tile_rects_workspace_size(ctypes.byref(d))
+    ws = t.empty(max(wsb, 1), dtype=t.uint8, device="cuda")
+    ins = [dev(np.asarray(xys, np.float32).reshape(N, 2)),
+           dev(np.asarray(radii, np.int32).reshape(N)),
+           dev(np.asarray(conics, np.float32).reshape(N, 3)),
+           dev(np.asarray(opacities, np.float32).reshape(N))]
+    _lib.check(L.jaxsplat_tile_rects(ctypes.byref(d), *[P(x) for x in ins], P(rects), None,
+                                     P(cum), P(ws), wsb, stream()))
+    return host(rects)[:N].view(np.uint32), host(cum)[:N]
+
+
 def gpu_raster_fwd(gids, bins, isect, xys, conics, colors, opacities, background, *, img_shape,
                    block_width=16):
     t = torch()

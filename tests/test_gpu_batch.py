@@ -98,7 +98,11 @@ def test_batch_forward_equals_plain_per_view(bw):
     total = 0
     for v, cam in enumerate(cams):
         out, pr, _, _ = _plain_view(params, sizes, cam, bg[v], bw)
-        total += int(pr["cum_tiles_hit"][-1])
+        # the batch bins into the opacity-limited rectangles, as rasterize does
+        _, tcum = H.gpu_tile_rects(pr["xys"], pr["radii"], pr["conics"],
+                                   params[4][_scene_rows(sizes, cam[0])], img_shape=cam[2],
+                                   block_width=bw)
+        total += int(tcum[-1]) if tcum.size else 0
         assert np.array_equal(imgs[v], out["img"]), f"view {v} image"
     assert n == total                      # the exact count is the sum of the views' counts
     # the view looking away sees nothing: background only
