@@ -164,3 +164,27 @@ def test_tight_lists_render_the_same_on_the_small_workload():
     (img_t, g_t), (img_r, g_r) = _raster_both(sc.params(), cam, seed=9)
     assert np.array_equal(img_t.view(np.uint32), img_r.view(np.uint32))
     _grads_match(g_t, g_r)
+
+
+@pytest.mark.parametrize("case", ["behind_camera", "transparent", "one_gaussian"])
+def test_tight_lists_edge_cases(case):
+    """No intersection at all (every Gaussian behind the camera), none in the
+    opacity-limited lists while the reference's lists are not empty (every
+    opacity below 1/255), and a single Gaussian: the image is the same as
+    through the reference's lists, and the gradients too (zero when nothing
+    blends)."""
+    params, cam = _stress_scene(11, n=400)
+    if case == "behind_camera":
+        params[0] = params[0].copy()
+        params[0][:, 2] = -5.0
+    elif case == "transparent":
+        params[4] = np.full_like(params[4], 0.5 / 255.0)
+    else:
+        params = [p[:1].copy() for p in params]
+        params[0][0] = (0.0, 0.0, 4.0)
+        params[4][0] = 0.9
+    (img_t, g_t), (img_r, g_r) = _raster_both(params, cam, seed=12)
+    assert np.array_equal(img_t.view(np.uint32), img_r.view(np.uint32))
+    _grads_match(g_t, g_r)
+    if case != "one_gaussian":
+        assert all(not g.any() for g in g_t)
