@@ -155,7 +155,7 @@ def main():
 def stage_breakdown(torch, sc, cam, params, target, iters):
     """Run the same pipeline through the C ABI with HIP events between stages."""
     from jaxsplat_amd import _lib
-    from jaxsplat_amd._ops import TIGHT_TILES, _bin, _bin_tight, _p, _stream
+    from jaxsplat_amd._ops import TIGHT_TILES, _bin, _bin_rects, _p, _stream
     L = _lib.lib()
     dev = params[0].device
     N = sc.n
@@ -171,6 +171,10 @@ def stage_breakdown(torch, sc, cam, params, target, iters):
         torch.empty((N, 3), **f32)
     radii, nth, cum = torch.empty((N, 1), **i32), torch.empty((N, 1), **i32), \
         torch.empty((N, 1), **i32)
+    # the render path's projection also writes the opacity-limited tile
+    # rectangles and counts them (reference rectangles and counts below)
+    rects, tnth, tcum = torch.empty((N, 2), **i32), torch.empty((N, 1), **i32), \
+        torch.empty((N, 1), **i32)
     wsb = L.jaxsplat_project_workspace_size(ctypes.byref(d))
     ws = torch.empty(wsb, dtype=torch.uint8, device=dev)
     fT, fI = torch.empty((Himg, Wimg), **f32), torch.empty((Himg, Wimg), **i32)
@@ -185,18 +189,25 @@ def stage_breakdown(torch, sc, cam, params, target, iters):
     for it in range(iters + 2):
         ev = [torch.cuda.Event(enable_timing=True) for _ in range(len(names) + 1)]
         ev[0].record()
-        _lib.check(L.jaxsplat_project_forward(
-            ctypes.byref(d), _p(means3d), _p(scales), _p(quats), _p(viewmat), None, _p(xys),
-            _p(depths), _p(radii), _p(conics), None, _p(nth), _p(cum), _p(ws), wsb, s))
-        ev[1].record()
-        # the render path's binning: opacity-limited tile rectangles (the
-        # rasterize step computes them), or the reference's rectangles
         if TIGHT_TILES:
-            gids, bins, isect, rects, bcum = _bin_tight(d, xys, depths, radii, conics, opac, dev)
+            _lib.check(L.jaxsplat_project_forward_rects(
+                ctypes.byref(d), _p(means3d), _p(scales), _p(quats), _p(viewmat), _p(opac),
+                None, _p(xys), _p(depths), _p(radii), _p(conics), None, _p(tnth), _p(tcum),
+                _p(rects), _p(ws), wsb, s))
+        else:
+            _lib.check(L.jaxsplat_project_forward(
+                ctypes.byref(d), _p(means3d), _p(scales), _p(quats), _p(viewmat), None, _p(xys),
+                _p(depths), _p(radii), _p(conics), None, _p(nth), _p(cum), _p(ws), wsb, s))
+        ev[1].record()
+        # the render path's binning: into the opacity-limited rectangles, or
+        # into the reference's
+        if TIGHT_TILES:
+            gids, bins, isect = _bin_rects(d, depths, radii, rects, tcum, dev)
+            brects, bcum = rects, tcum
         else:
             isect = int(cum[-1, 0].item())
             gids, bins, _ = _bin(d, xys, depths, radii, isect, dev)
-            rects, bcum = None, cum
+            brects, bcum = None, cum
         ev[2].record()
         _lib.check(L.jaxsplat_rasterize_forward_binned(
             ctypes.byref(d), _p(gids), _p(bins), isect, _p(xys), _p(conics), _p(colors), _p(opac),
@@ -208,10 +219,10 @@ def stage_breakdown(torch, sc, cam, params, target, iters):
             bws = torch.empty(bwsb, dtype=torch.uint8, device=dev)
         ev[4].record()
         # the render path's backward: staged records + per-Gaussian reduction
-        if rects is not None:
+        if brects is not None:
             _lib.check(L.jaxsplat_rasterize_backward_staged_rects(
                 ctypes.byref(d), _p(gids), _p(bins), isect, _p(xys), _p(conics), _p(colors),
-                _p(opac), _p(bg), 3, _p(radii), _p(rects), _p(bcum), _p(fT), _p(fI), _p(v_img),
+                _p(opac), _p(bg), 3, _p(radii), _p(brects), _p(bcum), _p(fT), _p(fI), _p(v_img),
                 None, _p(vc), _p(vo), _p(vxy), None, _p(vcn), 0, _p(bws), bwsb, s))
         else:
             _lib.check(L.jaxsplat_rasterize_backward_staged(
@@ -247,8 +258,13 @@ def stage_breakdown(torch, sc, cam, params, target, iters):
         return int(np.clip(tmax - bins_np[:, 0] + 1, 0, bins_np[:, 1] - bins_np[:, 0]).sum())
 
     isect_binned, L_b_binned = isect, consumed(bins, fI)
+    if brects is not None:
+        # the reference's counts of the same projection
+        _lib.check(L.jaxsplat_project_forward(
+            ctypes.byref(d), _p(means3d), _p(scales), _p(quats), _p(viewmat), None, _p(xys),
+            _p(depths), _p(radii), _p(conics), None, _p(nth), _p(cum), _p(ws), wsb, s))
     isect = int(cum[-1, 0].item())
-    if rects is not None:
+    if brects is not None:
         gids, bins, _ = _bin(d, xys, depths, radii, isect, dev)
         fT2, fI2, img2 = torch.empty_like(fT), torch.empty_like(fI), torch.empty_like(img)
         _lib.check(L.jaxsplat_rasterize_forward_binned(

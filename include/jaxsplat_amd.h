@@ -189,6 +189,17 @@ int jaxsplat_tile_rects(const jaxsplat_descriptor* d, const float* xys, const in
                         const float* conics, const float* opacities, uint32_t* tile_rects,
                         int32_t* num_tiles_hit, int32_t* cum_tiles_hit, void* workspace,
                         size_t workspace_bytes, void* stream);
+/* jaxsplat_project_forward whose num_tiles_hit and cum_tiles_hit count the
+ * opacity-limited rectangles of the opacities (N), written to tile_rects: the
+ * projection and jaxsplat_tile_rects in one pass, for a fused render.  No
+ * counterpart in the reference.  Same workspace as jaxsplat_project_forward. */
+int jaxsplat_project_forward_rects(const jaxsplat_descriptor* d, const float* means3d,
+                                   const float* scales, const float* quats,
+                                   const float* viewmat, const float* opacities, float* cov3d,
+                                   float* xys, float* depths, int32_t* radii, float* conics,
+                                   float* compensation, int32_t* num_tiles_hit,
+                                   int32_t* cum_tiles_hit, uint32_t* tile_rects,
+                                   void* workspace, size_t workspace_bytes, void* stream);
 /* jaxsplat_bin_depth_sort with the tile rectangles given (jaxsplat_tile_rects)
  * instead of computed from xys and radii; same workspace. */
 int jaxsplat_bin_depth_sort_rects(const jaxsplat_descriptor* d, const float* depths,

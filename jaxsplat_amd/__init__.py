@@ -12,6 +12,7 @@ including its fwd/bwd alpha-clamp mismatch and its unclamped EWA Jacobian;
 
 from __future__ import annotations
 
+from . import _ops
 from ._ops import ShapeDtypeError, bin_and_sort, project, rasterize
 from .batch import BatchPlan, BatchRenderer, ViewSpec, render_batch
 
@@ -36,6 +37,18 @@ def render(means3d, scales, quats, colors, opacities, *, viewmat, background, im
     projection computes and drops (``forward.cu:537-543``); gradients flow
     through both factors.
     """
+    if _ops.TIGHT_TILES:
+        # one projection pass also yields the opacity-limited tile rectangles
+        # (of the raw opacities: compensation <= 1 only shrinks the region)
+        xys, depths, radii, conics, cum, comp, rects = _ops.project_for_render(
+            means3d, scales, quats, opacities, viewmat, img_shape=img_shape, f=f, c=c,
+            glob_scale=glob_scale, clip_thresh=clip_thresh, block_width=block_size,
+            adjoint=adjoint)
+        if antialiased:
+            opacities = opacities * comp
+        return rasterize(colors, opacities, background, xys, depths, radii, conics, cum,
+                         img_shape=img_shape, block_width=block_size, adjoint=adjoint,
+                         tile_rects=rects)
     out = project(means3d, scales, quats, viewmat, img_shape=img_shape, f=f, c=c,
                   glob_scale=glob_scale, clip_thresh=clip_thresh, block_width=block_size,
                   adjoint=adjoint, return_compensation=antialiased)

@@ -30,6 +30,7 @@ EXPORTS = (
     "jaxsplat_bin_emit_workspace_size", "jaxsplat_bin_emit_sort",
     "jaxsplat_rasterize_backward_staged_workspace_size", "jaxsplat_rasterize_backward_staged",
     "jaxsplat_tile_rects_workspace_size", "jaxsplat_tile_rects", "jaxsplat_bin_depth_sort_rects",
+    "jaxsplat_project_forward_rects",
     "jaxsplat_rasterize_backward_staged_rects",
     "jaxsplat_rasterize_forward_binned", "jaxsplat_rasterize_backward_binned",
     "jaxsplat_rasterize_forward_nd", "jaxsplat_rasterize_backward_nd",
@@ -115,6 +116,7 @@ _SIGS = {
     "jaxsplat_rasterize_backward_staged_workspace_size": (SZ, [DP, U32]),
     "jaxsplat_tile_rects_workspace_size": (SZ, [DP]),
     "jaxsplat_tile_rects": (I32, [DP] + [P] * 7 + [P, SZ, P]),
+    "jaxsplat_project_forward_rects": (I32, [DP] + [P] * 14 + [P, SZ, P]),
     "jaxsplat_bin_depth_sort_rects": (I32, [DP, P, P, P, P, SZ, P]),
     "jaxsplat_rasterize_backward_staged_rects": (I32, [DP, P, P, U32, P, P, P, P, P, U32, P, P,
                                                         P, P, P, P, P, P, P, P, P, P, I32, P, SZ,

@@ -75,7 +75,9 @@ def _adjoint(mode):
 
 class _Project(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, cfg, means3d, scales, quats, viewmat):
+    def forward(ctx, cfg, means3d, scales, quats, viewmat, opacities=None):
+        # with opacities (the fused render): the counts are those of the
+        # opacity-limited tile rectangles, returned as an eighth output
         img_shape, f, c, glob_scale, clip_thresh, block_width, adjoint = cfg
         N = means3d.shape[0]
         dev = means3d.device
@@ -88,22 +90,30 @@ class _Project(torch.autograd.Function):
         nth = torch.empty((N, 1), dtype=torch.int32, device=dev)
         cum = torch.empty((N, 1), dtype=torch.int32, device=dev)
         comp = torch.empty((N, 1), dtype=torch.float32, device=dev)
+        rects = torch.empty((N if opacities is not None else 0, 2), dtype=torch.int32,
+                            device=dev)
         ws_bytes = L.jaxsplat_project_workspace_size(ctypes.byref(d))
         ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=dev)
         with torch.cuda.device(dev):
-            _lib.check(L.jaxsplat_project_forward(
-                ctypes.byref(d), _p(means3d), _p(scales), _p(quats), _p(viewmat), None,
-                _p(xys), _p(depths), _p(radii), _p(conics), _p(comp), _p(nth), _p(cum),
-                _p(ws), ws_bytes, _stream(dev)))
+            if opacities is not None:
+                _lib.check(L.jaxsplat_project_forward_rects(
+                    ctypes.byref(d), _p(means3d), _p(scales), _p(quats), _p(viewmat),
+                    _p(opacities), None, _p(xys), _p(depths), _p(radii), _p(conics), _p(comp),
+                    _p(nth), _p(cum), _p(rects), _p(ws), ws_bytes, _stream(dev)))
+            else:
+                _lib.check(L.jaxsplat_project_forward(
+                    ctypes.byref(d), _p(means3d), _p(scales), _p(quats), _p(viewmat), None,
+                    _p(xys), _p(depths), _p(radii), _p(conics), _p(comp), _p(nth), _p(cum),
+                    _p(ws), ws_bytes, _stream(dev)))
         ctx.d = d
         ctx.adjoint = adjoint
         ctx.save_for_backward(means3d, scales, quats, viewmat, radii, conics, comp)
-        ctx.mark_non_differentiable(radii, nth, cum)
+        ctx.mark_non_differentiable(radii, nth, cum, rects)
         ctx.set_materialize_grads(False)
-        return xys, depths, radii, conics, nth, cum, comp
+        return xys, depths, radii, conics, nth, cum, comp, rects
 
     @staticmethod
-    def backward(ctx, v_xys, v_depths, _v_radii, v_conics, _v_nth, _v_cum, v_comp):
+    def backward(ctx, v_xys, v_depths, _v_radii, v_conics, _v_nth, _v_cum, v_comp, _v_rects):
         means3d, scales, quats, viewmat, radii, conics, comp = ctx.saved_tensors
         N = means3d.shape[0]
         dev = means3d.device
@@ -123,7 +133,7 @@ class _Project(torch.autograd.Function):
                 _p(radii), _p(conics), _p(comp), _p(v_xys), _p(v_depths), _p(v_conics),
                 _p(v_comp), _p(v_mean3d), _p(v_scale), _p(v_quat), None, None, ctx.adjoint,
                 _stream(dev)))
-        return None, v_mean3d, v_scale, v_quat, None
+        return None, v_mean3d, v_scale, v_quat, None, None
 
 
 def project(means3d, scales, quats, viewmat, *, img_shape, f, c, glob_scale, clip_thresh,
@@ -149,8 +159,30 @@ def project(means3d, scales, quats, viewmat, *, img_shape, f, c, glob_scale, cli
     cfg = (tuple(img_shape), tuple(f), tuple(c), float(glob_scale), float(clip_thresh),
            int(block_width), adj)
     out = _Project.apply(cfg, means3d.contiguous(), scales.contiguous(), quats.contiguous(),
-                         viewmat.contiguous())
-    return out if return_compensation else out[:6]
+                         viewmat.contiguous(), None)
+    return out[:7] if return_compensation else out[:6]
+
+
+def project_for_render(means3d, scales, quats, opacities, viewmat, *, img_shape, f, c,
+                       glob_scale, clip_thresh, block_width, adjoint="reference"):
+    """The projection of the fused render: ``project`` whose tile counts are
+    those of the opacity-limited rectangles of ``opacities`` (no gradient
+    flows to them through this op).  Returns ``(xys, depths, radii, conics,
+    cum_tiles_hit, compensation, tile_rects)`` for ``rasterize(...,
+    tile_rects=...)``."""
+    adj = _adjoint(adjoint)
+    N = means3d.shape[0] if isinstance(means3d, torch.Tensor) else -1
+    _expect("means3d", means3d, (N, 3), torch.float32)
+    _expect("scales", scales, (N, 3), torch.float32)
+    _expect("quats", quats, (N, 4), torch.float32)
+    _expect("opacities", opacities, (N, 1), torch.float32)
+    _expect("viewmat", viewmat, (4, 4), torch.float32)
+    cfg = (tuple(img_shape), tuple(f), tuple(c), float(glob_scale), float(clip_thresh),
+           int(block_width), adj)
+    xys, depths, radii, conics, _nth, cum, comp, rects = _Project.apply(
+        cfg, means3d.contiguous(), scales.contiguous(), quats.contiguous(), viewmat.contiguous(),
+        opacities.detach().contiguous())
+    return xys, depths, radii, conics, cum, comp, rects
 
 
 def bin_and_sort(xys, depths, radii, cum_tiles_hit, *, img_shape, block_width,
@@ -250,17 +282,51 @@ def _bin_tight(d, xys, depths, radii, conics, opacities, dev):
     return gids[:isect], bins, isect, rects, cum
 
 
+def _bin_rects(d, depths, radii, rects, cum, dev):
+    """Binning into given tile rectangles (with their inclusive counts cum):
+    the depth stage on those rectangles, the host read of the total overlapped
+    with it, then emission.  Returns (gaussian_ids_sorted, tile_bins, isect)."""
+    N = depths.shape[0]
+    L = _lib.lib()
+    with torch.cuda.device(dev):
+        count = torch.empty(1, dtype=torch.int32, pin_memory=True)
+        count.copy_(cum.reshape(-1)[N - 1:N], non_blocking=True)
+        ready = torch.cuda.Event()
+        ready.record()
+        dws_bytes = L.jaxsplat_bin_depth_workspace_size(ctypes.byref(d))
+        dws = torch.empty(max(dws_bytes, 1), dtype=torch.uint8, device=dev)
+        _lib.check(L.jaxsplat_bin_depth_sort_rects(ctypes.byref(d), _p(depths), _p(radii),
+                                                   _p(rects), _p(dws), dws_bytes, _stream(dev)))
+        ready.synchronize()
+        isect = int(count[0])
+        ntiles = d.grid_dim_2d.x * d.grid_dim_2d.y
+        gids = torch.empty(max(isect, 1), dtype=torch.int32, device=dev)
+        bins = torch.empty((ntiles, 2), dtype=torch.int32, device=dev)
+        ws_bytes = L.jaxsplat_bin_emit_workspace_size(ctypes.byref(d), isect)
+        ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=dev)
+        _lib.check(L.jaxsplat_bin_emit_sort(ctypes.byref(d), isect, _p(dws), _p(gids), None,
+                                            _p(bins), _p(ws), ws_bytes, _stream(dev)))
+    return gids[:isect], bins, isect
+
+
 class _Rasterize(torch.autograd.Function):
     @staticmethod
     def forward(ctx, cfg, colors, opacities, background, xys, depths, radii, conics,
-                cum_tiles_hit):
+                cum_tiles_hit, tile_rects=None):
         img_shape, block_width, adjoint = cfg
         N, nch = colors.shape
         dev = colors.device
         H, W = img_shape
         d = _lib.make_descriptor(N, img_shape, (1.0, 1.0), (0.0, 0.0), 1.0, 0.0, block_width)
         rects = None
-        if TIGHT_TILES:
+        if tile_rects is not None:
+            # rectangles and counts from project_for_render
+            rects = tile_rects
+            if N == 0:
+                gids, bins, isect = _bin(d, xys, depths, radii, 0, dev)
+            else:
+                gids, bins, isect = _bin_rects(d, depths, radii, rects, cum_tiles_hit, dev)
+        elif TIGHT_TILES:
             gids, bins, isect, rects, cum_tiles_hit = _bin_tight(d, xys, depths, radii, conics,
                                                                  opacities, dev)
         else:
@@ -316,24 +382,28 @@ class _Rasterize(torch.autograd.Function):
                         _p(cum_tiles_hit), _p(final_Ts), _p(final_idx), _p(v_img), None,
                         _p(v_colors), _p(v_opacity), _p(v_xy), None, _p(v_conic), ctx.adjoint,
                         _p(ws), wsb, _stream(dev)))
-            return None, v_colors, v_opacity, None, v_xy, None, None, v_conic, None
+            return None, v_colors, v_opacity, None, v_xy, None, None, v_conic, None, None
         with torch.cuda.device(dev):
             _lib.check(L.jaxsplat_rasterize_backward_nd(
                 ctypes.byref(ctx.d), _p(gids), _p(bins), ctx.isect, _p(xys), _p(conics),
                 _p(colors), _p(opacities), _p(background), colors.shape[1], _p(final_Ts),
                 _p(final_idx), _p(v_img), None, _p(v_colors), _p(v_opacity), _p(v_xy), None,
                 _p(v_conic), ctx.adjoint, _stream(dev)))
-        return None, v_colors, v_opacity, None, v_xy, None, None, v_conic, None
+        return None, v_colors, v_opacity, None, v_xy, None, None, v_conic, None, None
 
 
 def rasterize(colors, opacities, background, xys, depths, radii, conics, cum_tiles_hit, *,
-              img_shape, block_width, adjoint="reference"):
+              img_shape, block_width, adjoint="reference", tile_rects=None):
     """Rasterize projected Gaussians (reference ``_rasterize/__init__.py:40-68``).
 
     Returns the (H, W, C) image for colors (N, C) and background (C,).  The
     reference API has C = 3; other channel counts use the N-channel
     rasterizer (``lib/kernels/forward.cu:213-337``, never called by the
     reference), with the same blend per channel in fp32.
+
+    ``tile_rects`` (N, 2) int32 (not in the reference API): the
+    opacity-limited tile rectangles of ``project_for_render``, whose counts
+    ``cum_tiles_hit`` then are.
     """
     adj = _adjoint(adjoint)
     N = colors.shape[0] if isinstance(colors, torch.Tensor) else -1
@@ -348,7 +418,11 @@ def rasterize(colors, opacities, background, xys, depths, radii, conics, cum_til
     _expect("radii", radii, (N, 1), torch.int32)
     _expect("conics", conics, (N, 3), torch.float32)
     _expect("cum_tiles_hit", cum_tiles_hit, (N, 1), torch.int32)
+    if tile_rects is not None:
+        _expect("tile_rects", tile_rects, (N, 2), torch.int32)
+        tile_rects = tile_rects.contiguous()
     cfg = (tuple(img_shape), int(block_width), adj)
     return _Rasterize.apply(cfg, colors.contiguous(), opacities.contiguous(),
                             background.contiguous(), xys.contiguous(), depths.contiguous(),
-                            radii.contiguous(), conics.contiguous(), cum_tiles_hit.contiguous())
+                            radii.contiguous(), conics.contiguous(), cum_tiles_hit.contiguous(),
+                            tile_rects)

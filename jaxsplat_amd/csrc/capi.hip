@@ -215,9 +215,33 @@ int jaxsplat_project_forward(const jaxsplat_descriptor* d, const float* means3d,
         return fail(JAXSPLAT_ERR_WORKSPACE, "project_forward: workspace too small");
     const jsa::Camera c = camera_of(d);
     JSA_CHECK_HIP(jsa::project_fwd(c, means3d, scales, quats, viewmat, cov3d, xys, depths, radii,
-                                   conics, compensation, num_tiles_hit, cum_tiles_hit, workspace,
-                                   static_cast<hipStream_t>(stream)),
+                                   conics, compensation, num_tiles_hit, cum_tiles_hit, nullptr,
+                                   nullptr, workspace, static_cast<hipStream_t>(stream)),
                   "project_forward");
+    return JAXSPLAT_OK;
+}
+
+int jaxsplat_project_forward_rects(const jaxsplat_descriptor* d, const float* means3d,
+                                   const float* scales, const float* quats,
+                                   const float* viewmat, const float* opacities, float* cov3d,
+                                   float* xys, float* depths, int32_t* radii, float* conics,
+                                   float* compensation, int32_t* num_tiles_hit,
+                                   int32_t* cum_tiles_hit, uint32_t* tile_rects,
+                                   void* workspace, size_t workspace_bytes, void* stream) {
+    g_err.clear();
+    if (int st = validate(d)) return st;
+    if (d->num_points == 0) return JAXSPLAT_OK;
+    if (!means3d || !scales || !quats || !viewmat || !opacities || !xys || !depths || !radii ||
+        !conics || !num_tiles_hit || !cum_tiles_hit || !tile_rects)
+        return fail(JAXSPLAT_ERR_INVALID, "project_forward_rects: NULL required pointer");
+    if (!workspace || workspace_bytes < jaxsplat_project_workspace_size(d))
+        return fail(JAXSPLAT_ERR_WORKSPACE, "project_forward_rects: workspace too small");
+    const jsa::Camera c = camera_of(d);
+    JSA_CHECK_HIP(jsa::project_fwd(c, means3d, scales, quats, viewmat, cov3d, xys, depths, radii,
+                                   conics, compensation, num_tiles_hit, cum_tiles_hit, opacities,
+                                   reinterpret_cast<uint2*>(tile_rects), workspace,
+                                   static_cast<hipStream_t>(stream)),
+                  "project_forward_rects");
     return JAXSPLAT_OK;
 }
 

@@ -255,11 +255,13 @@ hipError_t fill_value_f32x3(float* dst, size_t n_pixels, const float* rgb, hipSt
 
 // ---- projection ----
 size_t project_fwd_scratch_bytes(int n);
+// With opacities and rects (both optional), the tile counts and their sums are
+// those of the opacity-limited rectangles (opacity_tile_rect), written to rects.
 hipError_t project_fwd(const Camera& c, const float* means3d, const float* scales,
                        const float* quats, const float* viewmat, float* cov3d,
                        float* xys, float* depths, int* radii, float* conics,
                        float* compensation, int* num_tiles_hit, int* cum_tiles_hit,
-                       void* scratch, hipStream_t s);
+                       const float* opacities, uint2* rects, void* scratch, hipStream_t s);
 // Opacity-limited tile rectangles (opacity_tile_rect) of projected Gaussians,
 // their counts (num_tiles_hit optional) and inclusive sums; scratch
 // project_fwd_scratch_bytes(c.num_points).

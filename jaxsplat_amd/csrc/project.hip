@@ -234,7 +234,8 @@ __device__ __forceinline__ int project_one(
     const float* __restrict__ scales, const float* __restrict__ quats,
     float* __restrict__ cov3d_out, float* __restrict__ xys, float* __restrict__ depths,
     int* __restrict__ radii, float* __restrict__ conics, float* __restrict__ comp_out,
-    int* __restrict__ num_tiles_hit) {
+    int* __restrict__ num_tiles_hit, const float* __restrict__ opacities,
+    uint2* __restrict__ rects) {
     const Proj r = project_core(c, idx, W, pt, means3d, scales, quats);
     const float* cov = r.cov;
     const float* conic = r.conic;
@@ -251,8 +252,17 @@ __device__ __forceinline__ int project_one(
     conics[3 * idx + 1] = conic[1];
     conics[3 * idx + 2] = conic[2];
     if (comp_out) comp_out[idx] = r.comp;
-    num_tiles_hit[idx] = r.count;
-    return r.count;
+    int count = r.count;
+    if (rects) {        // opacity-limited rectangle and its count (tile_rects_kernel's)
+        const uint2 rc = r.rad > 0 ? opacity_tile_rect(c.block_width, c.tiles_x, c.tiles_y,
+                                                       make_float2(r.cx, r.cy), r.rad, conic[0],
+                                                       conic[1], conic[2], opacities[idx])
+                                   : make_uint2(0u, 0u);
+        rects[idx] = rc;
+        count = (int)rect_count(rc);
+    }
+    num_tiles_hit[idx] = count;
+    return count;
 }
 
 // Camera of one view of a batch (the block width and scalars are shared).
@@ -323,6 +333,7 @@ __global__ __launch_bounds__(256) void project_fwd_kernel(
     float* __restrict__ cov3d_out, float* __restrict__ xys, float* __restrict__ depths,
     int* __restrict__ radii, float* __restrict__ conics, float* __restrict__ comp_out,
     int* __restrict__ num_tiles_hit, int* __restrict__ cum_tiles_hit,
+    const float* __restrict__ opacities, uint2* __restrict__ rects,
     uint32_t* __restrict__ part_tot) {
     __shared__ uint32_t s_wave_tot[kProjPer][4];
     const int tid = threadIdx.x;
@@ -341,7 +352,8 @@ __global__ __launch_bounds__(256) void project_fwd_kernel(
         const int idx = part * kProjPart + k * 256 + tid;
         cnt[k] = idx < c.num_points
                      ? project_one(c, idx, W, pt, means3d, scales, quats, cov3d_out, xys,
-                                   depths, radii, conics, comp_out, num_tiles_hit)
+                                   depths, radii, conics, comp_out, num_tiles_hit, opacities,
+                                   rects)
                      : 0;
         incl[k] = wave_inclusive_sum(cnt[k]);
         if (lane == 63) s_wave_tot[k][wave] = (uint32_t)incl[k];
@@ -739,14 +751,15 @@ size_t project_fwd_scratch_bytes(int n) {
 hipError_t project_fwd(const Camera& c, const float* means3d, const float* scales,
                        const float* quats, const float* viewmat, float* cov3d, float* xys,
                        float* depths, int* radii, float* conics, float* compensation,
-                       int* num_tiles_hit, int* cum_tiles_hit, void* scratch, hipStream_t s) {
+                       int* num_tiles_hit, int* cum_tiles_hit, const float* opacities,
+                       uint2* rects, void* scratch, hipStream_t s) {
     if (c.num_points <= 0) return hipSuccess;
     const int parts = (c.num_points + kProjPart - 1) / kProjPart;
     uint32_t* part_tot = static_cast<uint32_t*>(scratch);
     uint32_t* part_off = part_tot + parts;
     project_fwd_kernel<<<parts, 256, 0, s>>>(c, means3d, scales, quats, viewmat, cov3d, xys,
                                              depths, radii, conics, compensation, num_tiles_hit,
-                                             cum_tiles_hit, part_tot);
+                                             cum_tiles_hit, opacities, rects, part_tot);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || parts == 1) return e;
     project_scan_kernel<<<1, 1024, 0, s>>>(part_tot, parts, part_off);
