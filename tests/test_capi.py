@@ -106,3 +106,49 @@ def test_batch_create_rejects(kw, msg):
     assert code == _lib.ERR_INVALID
     assert msg in L.jaxsplat_last_error()
     assert not h.value
+
+
+def _fake(n):
+    """Host addresses that are never dereferenced: the calls below fail their
+    argument checks before any GPU work."""
+    return ctypes.c_void_p(0x1000 * n)
+
+
+def test_opacity_limited_entry_points_reject_bad_arguments():
+    L = _lib.lib()
+    d = _lib.make_descriptor(1000, (256, 256), (256, 256), (128, 128), 1.0, 0.01, 16)
+    D = ctypes.byref(d)
+    P = _fake
+    wsb = L.jaxsplat_tile_rects_workspace_size(D)
+    assert wsb == L.jaxsplat_project_workspace_size(D) > 0
+    # NULL required pointers
+    assert L.jaxsplat_tile_rects(D, None, P(1), P(2), P(3), P(4), None, P(5), P(6), wsb,
+                                 None) == _lib.ERR_INVALID
+    assert L.jaxsplat_tile_rects(D, P(1), P(2), P(3), P(4), None, None, P(5), P(6), wsb,
+                                 None) == _lib.ERR_INVALID
+    # short workspace
+    assert L.jaxsplat_tile_rects(D, P(1), P(2), P(3), P(4), P(5), None, P(6), P(7), wsb - 1,
+                                 None) == _lib.ERR_WORKSPACE
+    assert L.jaxsplat_project_forward_rects(D, P(1), P(2), P(3), P(4), None, None, P(6), P(7),
+                                            P(8), P(9), None, P(10), P(11), P(12), P(13), wsb,
+                                            None) == _lib.ERR_INVALID
+    assert L.jaxsplat_project_forward_rects(D, P(1), P(2), P(3), P(4), P(5), None, P(6), P(7),
+                                            P(8), P(9), None, P(10), P(11), P(12), P(13),
+                                            wsb - 1, None) == _lib.ERR_WORKSPACE
+    dwsb = L.jaxsplat_bin_depth_workspace_size(D)
+    assert L.jaxsplat_bin_depth_sort_rects(D, P(1), P(2), None, P(3), dwsb,
+                                           None) == _lib.ERR_INVALID
+    assert L.jaxsplat_bin_depth_sort_rects(D, P(1), P(2), P(3), P(4), dwsb - 1,
+                                           None) == _lib.ERR_WORKSPACE
+    # staged backward with rectangles: channels, adjoint, NULL rectangles, workspace
+    bwsb = L.jaxsplat_rasterize_backward_staged_workspace_size(D, 100)
+
+    def staged(ch=3, rects=P(20), adj=0, ws=bwsb):
+        return L.jaxsplat_rasterize_backward_staged_rects(
+            D, P(1), P(2), 100, P(3), P(4), P(5), P(6), P(7), ch, P(8), rects, P(9), P(10),
+            P(11), P(12), None, P(13), P(14), P(15), None, P(16), adj, P(17), ws, None)
+    assert staged(ch=4) == _lib.ERR_INVALID
+    assert staged(adj=7) == _lib.ERR_INVALID
+    assert staged(rects=None) == _lib.ERR_INVALID
+    assert staged(ws=bwsb - 1) == _lib.ERR_WORKSPACE
+    assert b"workspace too small" in L.jaxsplat_last_error()
