@@ -100,7 +100,9 @@ def main():
         for p in params:
             p.grad = None
         img = jaxsplat_amd.render(*params, **kw)
-        loss = ((img - target) ** 2).mean()
+        # mean((img - target)^2) as in the reference example's loss, as
+        # torch's MSE (one reduction kernel forward, one elementwise backward)
+        loss = torch.nn.functional.mse_loss(img, target)
         loss.backward()
         return loss
 
@@ -394,7 +396,7 @@ def batched_vs_plain(torch, name, steps, warmup=2):
             out = jaxsplat_amd.render(*[x[a:b] for x in pp], viewmat=vms[v], background=bgs[v],
                                       img_shape=c.img_shape, f=c.f, c=c.c, glob_scale=glob,
                                       clip_thresh=clip, block_size=tuning.block_width)
-            ((out - targets[v]) ** 2).mean().backward()
+            torch.nn.functional.mse_loss(out, targets[v]).backward()
 
     for _ in range(warmup):
         plain_step()
