@@ -350,26 +350,45 @@ __global__ __launch_bounds__(256) void emit_expand_kernel(
     }
 }
 
-__global__ void tile_bins_kernel(const uint32_t* __restrict__ tiles, int n_host,
-                                 const int* __restrict__ n_dev, int num_tiles,
-                                 int2* __restrict__ bins, uint32_t* __restrict__ err) {
+// kTileBinsPer consecutive sorted ids per lane (one 16-byte load when
+// aligned); an edge between ids i - 1 and i closes bin tiles[i - 1] and opens
+// bin tiles[i].
+constexpr int kTileBinsPer = 4;
+__global__ __launch_bounds__(256) void tile_bins_kernel(const uint32_t* __restrict__ tiles,
+                                                        int n_host, const int* __restrict__ n_dev,
+                                                        int num_tiles, int2* __restrict__ bins,
+                                                        uint32_t* __restrict__ err) {
     const int n = n_dev ? *n_dev : n_host;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t t = tiles[i];
-    const uint32_t p_ = i > 0 ? tiles[i - 1] : 0u;
-    if (t >= (uint32_t)num_tiles || p_ >= (uint32_t)num_tiles) {   // never for valid input
-        atomicOr(err, 4u);
-        return;
+    const int i0 = (blockIdx.x * blockDim.x + threadIdx.x) * kTileBinsPer;
+    if (i0 >= n) return;
+    uint32_t k[kTileBinsPer];
+    if (i0 + kTileBinsPer <= n && (reinterpret_cast<uintptr_t>(tiles) & 15u) == 0) {
+        const uint4 v = *reinterpret_cast<const uint4*>(tiles + i0);
+        k[0] = v.x;
+        k[1] = v.y;
+        k[2] = v.z;
+        k[3] = v.w;
+    } else {
+#pragma unroll
+        for (int j = 0; j < kTileBinsPer; ++j) k[j] = i0 + j < n ? tiles[i0 + j] : 0u;
     }
-    if (i == 0) bins[t].x = 0;
-    if (i == n - 1) bins[t].y = n;
-    if (i > 0) {
-        const uint32_t p = tiles[i - 1];
-        if (p != t) {
-            bins[p].y = i;
+    uint32_t prev = i0 > 0 ? tiles[i0 - 1] : 0u;
+#pragma unroll
+    for (int j = 0; j < kTileBinsPer; ++j) {
+        const int i = i0 + j;
+        if (i >= n) break;
+        const uint32_t t = k[j];
+        if (t >= (uint32_t)num_tiles || prev >= (uint32_t)num_tiles) {   // never for valid input
+            atomicOr(err, 4u);
+            return;
+        }
+        if (i == 0) bins[t].x = 0;
+        if (i == n - 1) bins[t].y = n;
+        if (i > 0 && prev != t) {
+            bins[prev].y = i;
             bins[t].x = i;
         }
+        prev = t;
     }
 }
 
@@ -547,7 +566,7 @@ hipError_t bin_emit(const BinPlan& p, const Camera& c, int num_isect_host,
                    reinterpret_cast<uint32_t*>(gids_sorted), U(L.ttmp_k), U(L.ttmp_v),
                    num_isect_host, num_isect_dev, b + L.tile_sort, err, s);
     if (e != hipSuccess) return e;
-    const int nb = (p.isect_cap + 255) / 256;
+    const int nb = (int)(((long long)p.isect_cap + 256 * kTileBinsPer - 1) / (256 * kTileBinsPer));
     tile_bins_kernel<<<nb > 0 ? nb : 1, 256, 0, s>>>(tiles_out, num_isect_host, num_isect_dev,
                                                    p.num_tiles,
                                                    reinterpret_cast<int2*>(tile_bins), err);
@@ -689,7 +708,7 @@ hipError_t batch_bin(const BatchBinPlan& p, const ViewRec* views, const uint2* i
                    reinterpret_cast<uint32_t*>(items_sorted), U(L.ttmp_k), U(L.ttmp_v), 0, count,
                    b + L.tile_sort, err, s);
     if (e != hipSuccess) return e;
-    const int nb = (p.isect_cap + 255) / 256;
+    const int nb = (int)(((long long)p.isect_cap + 256 * kTileBinsPer - 1) / (256 * kTileBinsPer));
     tile_bins_kernel<<<nb > 0 ? nb : 1, 256, 0, s>>>(U(L.keys_sorted), 0, count, p.num_bins,
                                                    reinterpret_cast<int2*>(tile_bins), err);
     return hipGetLastError();
