@@ -168,8 +168,8 @@ __device__ __forceinline__ CullInfo cull_info(float gx, float gy, float a, float
     k.a = a;
     k.b = b;
     k.c = c;
-    k.smax = __logf(255.f * o) + 0.01f;
     k.lim2 = alpha_limit2(o);
+    k.smax = k.lim2 * 0.69314718f + 0.01f;        // ln(255 o) + margin
     k.any = k.lim2 >= 0.f;
     k.exact = a > 0.f && c > 0.f && a * c - b * b > 0.f;
     k.ra = __builtin_amdgcn_rcpf(a);
