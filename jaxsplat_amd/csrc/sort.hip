@@ -76,7 +76,10 @@ __global__ __launch_bounds__(kSortThreads) void radix_upsweep(
 }
 
 // In-place exclusive scan of one digit's column counts[d * parts + 0..parts)
-// (one workgroup per digit); totals[d] = column sum.
+// (one workgroup per digit); totals[d] = column sum.  Up to kColPer * 256
+// partitions each thread scans a run of consecutive entries and one block
+// scan joins the runs (two barriers); longer columns go in 256-entry rounds.
+constexpr int kColPer = 16;
 __global__ __launch_bounds__(kSortThreads) void radix_colscan(uint32_t* __restrict__ counts,
                                                              int parts,
                                                              uint32_t* __restrict__ totals) {
@@ -86,6 +89,33 @@ __global__ __launch_bounds__(kSortThreads) void radix_colscan(uint32_t* __restri
     const int lane = lane_id();
     const int wave = tid >> 6;
     uint32_t* col = counts + (size_t)blockIdx.x * parts;
+    const int per = (parts + kSortThreads - 1) / kSortThreads;
+    if (per <= kColPer) {
+        const int b0 = min(tid * per, parts), b1 = min(b0 + per, parts);
+        uint32_t v[kColPer];
+        uint32_t sum = 0;
+#pragma unroll
+        for (int k = 0; k < kColPer; ++k) {
+            v[k] = (k < per && b0 + k < b1) ? col[b0 + k] : 0u;
+            sum += v[k];
+        }
+        const uint32_t incl = wave_inclusive_sum(sum);
+        if (lane == 63) s_tmp[wave] = incl;
+        __syncthreads();
+        uint32_t run = incl - sum, tot = 0;
+#pragma unroll
+        for (int w = 0; w < kSortWaves; ++w) {
+            run += (w < wave) ? s_tmp[w] : 0u;
+            tot += s_tmp[w];
+        }
+#pragma unroll
+        for (int k = 0; k < kColPer; ++k) {
+            if (k < per && b0 + k < b1) col[b0 + k] = run;
+            run += v[k];
+        }
+        if (tid == 0) totals[blockIdx.x] = tot;
+        return;
+    }
     if (tid == 0) s_carry = 0;
     __syncthreads();
     for (int b = 0; b < parts; b += kSortThreads) {
