@@ -438,7 +438,8 @@ int jaxsplat_rasterize_backward_binned(
     if (int st = validate(d)) return st;
     if (adjoint != JAXSPLAT_ADJOINT_REFERENCE && adjoint != JAXSPLAT_ADJOINT_CONSISTENT)
         return fail(JAXSPLAT_ERR_INVALID, "unknown adjoint mode %d", adjoint);
-    if (!v_colors || !v_opacity || !v_xy || !v_conic)
+    // (an empty gradient array, N = 0, may come as NULL)
+    if (d->num_points > 0 && (!v_colors || !v_opacity || !v_xy || !v_conic))
         return fail(JAXSPLAT_ERR_INVALID, "rasterize_backward: NULL gradient pointer");
     const jsa::Camera c = camera_of(d);
     hipStream_t s = static_cast<hipStream_t>(stream);
@@ -511,7 +512,7 @@ int jaxsplat_rasterize_backward_nd(
                     "channel passes)");
     if (adjoint != JAXSPLAT_ADJOINT_REFERENCE && adjoint != JAXSPLAT_ADJOINT_CONSISTENT)
         return fail(JAXSPLAT_ERR_INVALID, "unknown adjoint mode %d", adjoint);
-    if (!v_colors || !v_opacity || !v_xy || !v_conic)
+    if (d->num_points > 0 && (!v_colors || !v_opacity || !v_xy || !v_conic))
         return fail(JAXSPLAT_ERR_INVALID, "rasterize_backward_nd: NULL gradient pointer");
     const jsa::Camera c = camera_of(d);
     hipStream_t s = static_cast<hipStream_t>(stream);
@@ -563,9 +564,9 @@ int jaxsplat_rasterize_backward_staged(
         return fail(JAXSPLAT_ERR_CAPACITY, "num_intersects %u exceeds 2^30", num_intersects);
     if (adjoint != JAXSPLAT_ADJOINT_REFERENCE && adjoint != JAXSPLAT_ADJOINT_CONSISTENT)
         return fail(JAXSPLAT_ERR_INVALID, "unknown adjoint mode %d", adjoint);
+    if (d->num_points == 0) return JAXSPLAT_OK;
     if (!v_colors || !v_opacity || !v_xy || !v_conic)
         return fail(JAXSPLAT_ERR_INVALID, "rasterize_backward_staged: NULL gradient pointer");
-    if (d->num_points == 0) return JAXSPLAT_OK;
     if (!xys || !radii || !cum_tiles_hit ||
         (num_intersects > 0 && (!gaussian_ids_sorted || !tile_bins || !conics || !colors ||
                                 !opacities || !background || !final_Ts || !final_idx ||
@@ -604,10 +605,10 @@ int jaxsplat_rasterize_backward_staged_rects(
         return fail(JAXSPLAT_ERR_CAPACITY, "num_intersects %u exceeds 2^30", num_intersects);
     if (adjoint != JAXSPLAT_ADJOINT_REFERENCE && adjoint != JAXSPLAT_ADJOINT_CONSISTENT)
         return fail(JAXSPLAT_ERR_INVALID, "unknown adjoint mode %d", adjoint);
+    if (d->num_points == 0) return JAXSPLAT_OK;
     if (!v_colors || !v_opacity || !v_xy || !v_conic)
         return fail(JAXSPLAT_ERR_INVALID,
                     "rasterize_backward_staged_rects: NULL gradient pointer");
-    if (d->num_points == 0) return JAXSPLAT_OK;
     if (!xys || !radii || !tile_rects || !cum_tiles_hit ||
         (num_intersects > 0 && (!gaussian_ids_sorted || !tile_bins || !conics || !colors ||
                                 !opacities || !background || !final_Ts || !final_idx ||

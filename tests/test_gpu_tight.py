@@ -218,3 +218,24 @@ def test_fused_render_matches_reference_lists(antialiased):
     assert np.array_equal(res[0][0].view(np.uint32), res[1][0].view(np.uint32))
     if not antialiased:
         _grads_match(res[0][1], res[1][1])
+
+
+def test_render_without_gaussians():
+    """N = 0 through the fused path and through the reference's lists: the
+    background everywhere, and empty gradients."""
+    t = H.torch()
+    cam = dict(viewmat=scenes.IDENTITY.copy(), img_shape=(40, 56), f=(56.0, 56.0), c=(28.0, 20.0))
+    bg = np.array([0.2, 0.4, 0.6], np.float32)
+    for tight in (True, False):
+        _ops.TIGHT_TILES = tight
+        try:
+            d = [H.dev(np.zeros(s, np.float32)).requires_grad_(True)
+                 for s in ((0, 3), (0, 3), (0, 4), (0, 3), (0, 1))]
+            img = render(*d, viewmat=H.dev(cam["viewmat"]), background=H.dev(bg),
+                         img_shape=cam["img_shape"], f=cam["f"], c=cam["c"], glob_scale=1.0,
+                         clip_thresh=0.01, block_size=16)
+            assert np.array_equal(H.host(img), np.broadcast_to(bg, (40, 56, 3)))
+            img.sum().backward()
+            assert all(x.grad is not None and x.grad.shape[0] == 0 for x in d)
+        finally:
+            _ops.TIGHT_TILES = True
