@@ -246,8 +246,7 @@ def _bin_overlapped(d, xys, depths, radii, cum_tiles_hit, dev):
 
 def _bin_tight(d, xys, depths, radii, conics, opacities, dev):
     """Binning into the opacity-limited tile rectangles: the rectangles and
-    their inclusive counts, then the depth stage on those rectangles with the
-    host read of the total overlapped as in _bin_overlapped.  Returns
+    their inclusive counts (jaxsplat_tile_rects), then _bin_rects.  Returns
     (gaussian_ids_sorted, tile_bins, isect, rects, cum)."""
     N = xys.shape[0]
     L = _lib.lib()
@@ -262,24 +261,8 @@ def _bin_tight(d, xys, depths, radii, conics, opacities, dev):
         _lib.check(L.jaxsplat_tile_rects(ctypes.byref(d), _p(xys), _p(radii), _p(conics),
                                          _p(opacities), _p(rects), None, _p(cum), _p(rws),
                                          rws_bytes, _stream(dev)))
-        count = torch.empty(1, dtype=torch.int32, pin_memory=True)
-        count.copy_(cum.reshape(-1)[N - 1:N], non_blocking=True)
-        ready = torch.cuda.Event()
-        ready.record()
-        dws_bytes = L.jaxsplat_bin_depth_workspace_size(ctypes.byref(d))
-        dws = torch.empty(max(dws_bytes, 1), dtype=torch.uint8, device=dev)
-        _lib.check(L.jaxsplat_bin_depth_sort_rects(ctypes.byref(d), _p(depths), _p(radii),
-                                                   _p(rects), _p(dws), dws_bytes, _stream(dev)))
-        ready.synchronize()
-        isect = int(count[0])
-        ntiles = d.grid_dim_2d.x * d.grid_dim_2d.y
-        gids = torch.empty(max(isect, 1), dtype=torch.int32, device=dev)
-        bins = torch.empty((ntiles, 2), dtype=torch.int32, device=dev)
-        ws_bytes = L.jaxsplat_bin_emit_workspace_size(ctypes.byref(d), isect)
-        ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=dev)
-        _lib.check(L.jaxsplat_bin_emit_sort(ctypes.byref(d), isect, _p(dws), _p(gids), None,
-                                            _p(bins), _p(ws), ws_bytes, _stream(dev)))
-    return gids[:isect], bins, isect, rects, cum
+    gids, bins, isect = _bin_rects(d, depths, radii, rects, cum, dev)
+    return gids, bins, isect, rects, cum
 
 
 def _bin_rects(d, depths, radii, rects, cum, dev):
