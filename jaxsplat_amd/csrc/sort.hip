@@ -250,9 +250,10 @@ __global__ __launch_bounds__(kSortThreads) void radix_downsweep(
 
 }  // namespace
 
-// Partition size: 16 keys per lane for large arrays, 4 for small ones (more
-// workgroups in flight when n is only a few hundred thousand).
-static int items_for(int n_cap) { return n_cap >= (1 << 22) ? 16 : 4; }
+// Partition size: 16 keys per lane for large arrays (the tile sort of millions
+// of pairs), 8 from 512K keys (the depth sort of 1M Gaussians: measured
+// faster than 4 or 16), 4 for small ones (more workgroups in flight).
+static int items_for(int n_cap) { return n_cap >= (1 << 22) ? 16 : (n_cap >= (1 << 19) ? 8 : 4); }
 
 SortPlan sort_plan(int n_cap, int begin_bit, int end_bit) {
     SortPlan p;
@@ -292,6 +293,9 @@ hipError_t sort_pairs(const SortPlan& p, const uint32_t* keys_in, const uint32_t
         if (items == 16)
             radix_upsweep<16><<<p.parts, kSortThreads, 0, s>>>(ki, n_host, n_dev, shift, bits,
                                                                 p.parts, counts);
+        else if (items == 8)
+            radix_upsweep<8><<<p.parts, kSortThreads, 0, s>>>(ki, n_host, n_dev, shift, bits,
+                                                               p.parts, counts);
         else
             radix_upsweep<4><<<p.parts, kSortThreads, 0, s>>>(ki, n_host, n_dev, shift, bits,
                                                                p.parts, counts);
@@ -302,6 +306,10 @@ hipError_t sort_pairs(const SortPlan& p, const uint32_t* keys_in, const uint32_t
             radix_downsweep<16><<<p.parts, kSortThreads, 0, s>>>(ki, vi, ko, vo, n_host, n_dev,
                                                                   shift, bits, p.parts, counts,
                                                                   totals);
+        else if (items == 8)
+            radix_downsweep<8><<<p.parts, kSortThreads, 0, s>>>(ki, vi, ko, vo, n_host, n_dev,
+                                                                 shift, bits, p.parts, counts,
+                                                                 totals);
         else
             radix_downsweep<4><<<p.parts, kSortThreads, 0, s>>>(ki, vi, ko, vo, n_host, n_dev,
                                                                  shift, bits, p.parts, counts,
