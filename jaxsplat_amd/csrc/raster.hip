@@ -261,10 +261,13 @@ __global__ __launch_bounds__(256) void raster_fwd_kernel(
     float T = 1.f;
     float cr = 0.f, cg = 0.f, cb = 0.f;
     int cur = 0;
-    bool done = !inside;
+    // 0 while the pixel blends, the float sign bit once it is done: OR-ed
+    // into the acceptance compare, so that done pixels accept nothing and
+    // every ballot below is taken on a plain compare
+    uint32_t dbits = inside ? 0u : 0x80000000u;
 
     for (int bstart = range.x; bstart < range.y; bstart += DEPTH) {
-        const uint64_t dmask = ballot(done);
+        const uint64_t dmask = ballot(dbits != 0u);
         if (lane == 0) s_cnt[wave] = __popcll(dmask);
         __syncthreads();
         int ndone = 0;
@@ -304,7 +307,7 @@ __global__ __launch_bounds__(256) void raster_fwd_kernel(
         // entry's LDS reads are issued before the current pair is evaluated,
         // which takes their latency out of the dependent chain per pair.  Two
         // register sets alternate (unrolled by two), so nothing is copied.
-        bool live = ballot(!done) != 0;
+        bool live = ballot(dbits == 0u) != 0;
         for (int q = 0; q < NW && live; ++q) {
             uint64_t m = uniform64(s_mask[wave][q]);
             // branch-free: with no bit left it re-reads entry t (a valid index),
@@ -323,11 +326,15 @@ __global__ __launch_bounds__(256) void raster_fwd_kernel(
                 const float dx = a.x - px, dy = a.y - py;
                 const float s2 = sigma2(b, dx, dy);
                 const float alpha = fminf(0.999f, a.z * __builtin_amdgcn_exp2f(-s2));
-                const bool ok = !done && pair_accepted(s2, b.w);
+                // pair_accepted(s2, lim2) for a blending pixel; lim2 >= 0, so
+                // the sign bit of a done pixel fails the compare
+                const bool ok = (__float_as_uint(s2) | dbits) <= __float_as_uint(b.w);
                 const float next_T = fmaf(-T, alpha, T);      // T (1 - alpha)
-                const bool fin = ok && next_T <= kTEps;
-                const bool add = ok && !fin;
-                if (ballot(add)) {
+                const bool term = next_T <= kTEps;
+                const uint64_t okm = ballot(ok), tm = ballot(term);
+                const bool fin = ok && term;
+                const bool add = ok && !term;
+                if (okm & ~tm) {
                     const float vis = add ? alpha * T : 0.f;
                     cr += col.x * vis;
                     cg += col.y * vis;
@@ -335,9 +342,9 @@ __global__ __launch_bounds__(256) void raster_fwd_kernel(
                     T = add ? next_T : T;
                     cur = add ? bstart + t : cur;
                 }
-                if (ballot(fin)) {
-                    done = done || fin;
-                    if (ballot(!done) == 0) return true;
+                if (okm & tm) {
+                    dbits = fin ? 0x80000000u : dbits;
+                    if (ballot(dbits == 0u) == 0) return true;
                 }
                 return false;
             };
