@@ -61,7 +61,8 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
-              "(the path shards by view/scene with one gradient all-reduce; see DESIGN.md)",
+              "(the path would shard by view or scene, each GPU holding all Gaussians, with one "
+              "all-reduce of the per-Gaussian gradients per step; only the one-GPU share is built)",
               file=sys.stderr)
         sys.exit(2)
     import torch
