@@ -280,6 +280,10 @@ hipError_t sort_pairs(const SortPlan& p, const uint32_t* keys_in, const uint32_t
     uint32_t* totals = static_cast<uint32_t*>(scratch);
     uint32_t* counts = totals + kBins;
     const int items = items_for(p.n_cap);
+    // the key bits split evenly over the passes (13 bits: 7 + 6), so that no
+    // pass scatters into more digits than needed: longer runs per digit and
+    // partition, fewer partly written lines
+    const int per = (p.end_bit - p.begin_bit + p.passes - 1) / p.passes;
     hipError_t e;
     // ping-pong so that the last pass lands in (keys_out, vals_out)
     const uint32_t* ki = keys_in;
@@ -288,8 +292,8 @@ hipError_t sort_pairs(const SortPlan& p, const uint32_t* keys_in, const uint32_t
         const bool to_out = ((p.passes - 1 - q) % 2) == 0;
         uint32_t* ko = to_out ? keys_out : tmp_keys;
         uint32_t* vo = to_out ? vals_out : tmp_vals;
-        const int shift = p.begin_bit + q * kDigitBits;
-        const int bits = min(kDigitBits, p.end_bit - shift);
+        const int shift = p.begin_bit + q * per;
+        const int bits = min(per, p.end_bit - shift);
         if (items == 16)
             radix_upsweep<16><<<p.parts, kSortThreads, 0, s>>>(ki, n_host, n_dev, shift, bits,
                                                                 p.parts, counts);
