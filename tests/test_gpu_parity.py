@@ -117,6 +117,52 @@ def test_raster_fwd_deep_termination():
     assert np.array_equal(out["final_idx"][ok], ref["final_idx"][ok])
 
 
+def test_raster_partial_tiles_vs_oracle():
+    """Image sides that are not multiples of the 16-pixel tile (250 x 243):
+    the last tile row and column are partial, so lanes outside the image
+    start done.  Every other Gaussian is near opaque, so pixels also
+    terminate inside those tiles.  Forward and backward vs the oracle."""
+    import dataclasses
+    sc = scenes.make("small", n=6000)
+    sc.scales = (sc.scales * 3.0).astype(np.float32)
+    cam = dataclasses.replace(sc.camera, img_shape=(250, 243), c=(121.5, 125.0))
+    bw = 16
+    g = H.gpu_project(sc.means3d, sc.scales, sc.quats, cam.viewmat, img_shape=cam.img_shape,
+                      f=cam.f, c=cam.c, glob_scale=sc.glob_scale, clip_thresh=sc.clip_thresh,
+                      block_width=bw)
+    gids, _, bins, isect = H.gpu_bin(g["xys"], g["depths"], g["radii"], g["cum_tiles_hit"],
+                                     img_shape=cam.img_shape, block_width=bw)
+    assert isect > 0
+    op = np.where(np.arange(sc.n)[:, None] % 2 == 0, 0.97, sc.opacities).astype(np.float32)
+    bgv = np.asarray((0.3, 0.1, 0.6), np.float32)
+    Hh, Ww = cam.img_shape
+    out = H.gpu_raster_fwd(gids, bins, isect, g["xys"], g["conics"], sc.colors, op, bgv,
+                           img_shape=cam.img_shape, block_width=bw)
+    ref = oracle.rasterize.rasterize_fwd(gids, bins, g["xys"], g["conics"], sc.colors, op, bgv,
+                                         img_shape=cam.img_shape, block_width=bw)
+    ok = ~ref["ambiguous"]
+    assert ref["ambiguous"].mean() < 5e-3
+    assert (ref["final_Ts"] < 1e-3).mean() > 0.3           # most pixels terminate
+    assert np.abs(out["img"][ok] - ref["img"][ok]).max() <= 1e-4
+    assert np.array_equal(out["final_idx"][ok], ref["final_idx"][ok])
+    assert np.abs(out["final_Ts"][ok] - ref["final_Ts"][ok]).max() <= 1e-4
+    rng = np.random.default_rng(17)
+    v_img = rng.normal(size=(Hh, Ww, 3)).astype(np.float32)
+    v_alpha = rng.normal(size=(Hh, Ww)).astype(np.float32)
+    v_img[ref["ambiguous"]] = 0
+    v_alpha[ref["ambiguous"]] = 0
+    got = H.gpu_raster_bwd(gids, bins, isect, g["xys"], g["conics"], sc.colors, op, bgv,
+                           out["final_Ts"], out["final_idx"], v_img, v_alpha,
+                           img_shape=cam.img_shape, block_width=bw, adjoint=0)
+    exp = oracle.rasterize.rasterize_bwd(gids, bins, g["xys"], g["conics"], sc.colors, op, bgv,
+                                         ref["final_Ts"], ref["final_idx"], v_img, v_alpha,
+                                         img_shape=cam.img_shape, block_width=bw,
+                                         mode="reference")
+    for k in ("v_colors", "v_opacity", "v_xy", "v_xy_abs", "v_conic"):
+        okg, worst = H.grad_close(got[k].reshape(exp[k].shape), exp[k])
+        assert okg, (k, worst)
+
+
 @pytest.mark.parametrize("adjoint,mode", [(0, "reference"), (1, "consistent")])
 @pytest.mark.parametrize("bw", [16, 8])
 def test_raster_bwd_vs_oracle(adjoint, mode, bw):
