@@ -391,6 +391,7 @@ struct PixState {
     float bv;
     int fin;
     bool inside;
+    uint64_t inside_m;  // ballot(inside), fixed for the kernel
 };
 
 // Row (16-lane) sums of v[0..NV) by recursive halving.  At each DPP level
@@ -444,14 +445,16 @@ __device__ __forceinline__ float row_sums_split(float* v, int lane) {
 // with vs = dL/dsigma = -o * vis * v_alpha (times the clamp's zero slope in
 // consistent mode).
 template <bool CONSISTENT, int NV>
-__device__ __forceinline__ bool bwd_pixel(PixState& s, const float4 a, const float4 b,
+__device__ __forceinline__ uint64_t bwd_pixel(PixState& s, const float4 a, const float4 b,
                                           const float4 col, int k, float* g) {
 #define JSA_ACC(i, v) (g[i] += (v))
     constexpr float kClamp = CONSISTENT ? 0.999f : 0.99f;
     const float dx = a.x - s.px, dy = a.y - s.py;
     const float s2 = sigma2(b, dx, dy);
     const float raw = a.z * __builtin_amdgcn_exp2f(-s2);
-    const bool valid = s.inside && (k <= s.fin) && pair_accepted(s2, col.w);
+    const bool live = k <= s.fin;
+    const bool acc = pair_accepted(s2, col.w);
+    const bool valid = s.inside && live && acc;
     // an invalid pair acts as alpha = 0: T unchanged (1 / (1 - 0) is exact), no weight
     const float rv = valid ? raw : 0.f;
     const float al = fminf(kClamp, rv);
@@ -480,7 +483,8 @@ __device__ __forceinline__ bool bwd_pixel(PixState& s, const float4 a, const flo
     }
     JSA_ACC(NV - 1, vs);
 #undef JSA_ACC
-    return valid;
+    // the lanes with a valid pair, from ballots of the plain compares
+    return ballot(live) & ballot(acc) & s.inside_m;
 }
 
 // Per-Gaussian factors of the partial sums of bwd_pixel (the batch's base-2
@@ -575,6 +579,7 @@ __global__ __launch_bounds__(128) void raster_bwd_kernel(
         const int j = x_org + rc.y;
         PixState& s = ps[sl];
         s.inside = pixel && i < vr.img_h && j < vr.img_w;
+        s.inside_m = ballot(s.inside);
         s.px = (float)j + 0.5f;
         s.py = (float)i + 0.5f;
         const size_t pix = s.inside ? (size_t)vr.pix_off + (size_t)i * vr.img_w + j : 0;
@@ -689,13 +694,14 @@ __global__ __launch_bounds__(128) void raster_bwd_kernel(
                 const float4 col = s_c[t];
                 float g[NV];
 #pragma unroll
-                for (int v = 0; v < NV; ++v) g[v] = 0.f;
-                bool any = false;
+                // -0 + x is x exactly, so the first slot's sums need no add
+                for (int v = 0; v < NV; ++v) g[v] = -0.f;
+                uint64_t any = 0;
 #pragma unroll
                 for (int sl = 0; sl < NSLOT; ++sl)
                     if ((ms[sl] >> b) & 1ull)
                         any |= bwd_pixel<CONSISTENT, NV>(ps[sl], a, bb, col, k, g);
-                if (ballot(any) == 0) continue;
+                if (any == 0) continue;
                 // lane j of every row gets the row sum of value j, then the four
                 // rows are summed with the row swaps; lanes 0..NV-1 store the
                 // wave's sums
