@@ -84,8 +84,14 @@ int jaxsplat_make_descriptor(jaxsplat_descriptor* out, uint32_t num_points, uint
 /* as "project_fwd", "project_bwd", "rasterize_fwd", "rasterize_bwd"         */
 /* (lib/ffi.cu:4-13).  `buffers` holds the operands then the results in the   */
 /* reference lowering order (lib/ops.cu:246-258, 336-354, 408-419,           */
-/* 502-519); `opaque` is a jaxsplat_descriptor.  Errors are reported through */
-/* jaxsplat_last_error() (the signature has no status).                      */
+/* 502-519); `opaque` is a jaxsplat_descriptor.  The signature has no        */
+/* status: on any error the message is kept in jaxsplat_last_error() and     */
+/* every result buffer is filled with 0x7FC00000 on `stream` (a quiet NaN in */
+/* the float results, an out-of-range value in the integer ones), so that no */
+/* stale memory passes as a result; only a descriptor of the wrong length,   */
+/* which cannot size the results, leaves them untouched.  The two rasterize  */
+/* targets also report a binning overflow (cum_tiles_hit inconsistent with   */
+/* xys and radii) this way, after a synchronisation of the stream.           */
 /* ------------------------------------------------------------------------ */
 void jaxsplat_project_fwd(void* stream, void** buffers, const char* opaque, size_t opaque_len);
 void jaxsplat_project_bwd(void* stream, void** buffers, const char* opaque, size_t opaque_len);

@@ -5,9 +5,17 @@ Drop-in for the reference API of yklcs/jaxsplat (``jaxsplat/__init__.py:6-73``):
 outputs and cotangent routing, on torch GPU tensors, backed by hand-written
 HIP kernels for gfx950 reached through the C ABI of ``libjaxsplat_amd.so``.
 
-``adjoint="reference"`` (default) reproduces the reference backward exactly,
+``adjoint="reference"`` (default) follows the reference backward's formulas,
 including its fwd/bwd alpha-clamp mismatch and its unclamped EWA Jacobian;
-``adjoint="consistent"`` gives the exact adjoint of the forward.
+``adjoint="consistent"`` gives the exact adjoint of the forward.  Either
+matches its float64 restatement up to float32 rounding, which includes the
+borderline discrete decisions: the forward updates T as fma(-T, alpha, T)
+and accepts a pair by comparing sigma * log2(e) with log2(255 o) on a conic
+pre-scaled to base 2, so a pair with alpha within rounding of 1/255, or a T
+within rounding of 1e-4, may be decided the other way than the reference's
+o * exp(-sigma) >= 1/255 and T (1 - alpha) <= 1e-4 would decide it.  The
+parity tests exclude (and count) the pixels where such a decision is
+ambiguous in float32.
 """
 
 from __future__ import annotations

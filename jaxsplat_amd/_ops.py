@@ -292,6 +292,17 @@ def _bin_rects(d, depths, radii, rects, cum, dev):
     return gids[:isect], bins, isect
 
 
+STAGED_MAX_INTERSECTIONS = (1 << 30) - 1
+
+
+def staged_backward_applies(nch, isect):
+    """Whether the rasterize backward takes the staged entry point: up to
+    three channels and fewer than 2^30 intersections (its record slots are
+    31-bit); otherwise the atomic N-channel backward, which has neither limit
+    and differs only in the order of the float additions."""
+    return STAGED_BACKWARD and 1 <= nch <= 3 and 0 <= isect <= STAGED_MAX_INTERSECTIONS
+
+
 class _Rasterize(torch.autograd.Function):
     @staticmethod
     def forward(ctx, cfg, colors, opacities, background, xys, depths, radii, conics,
@@ -344,12 +355,17 @@ class _Rasterize(torch.autograd.Function):
         v_conic = torch.empty_like(conics)
         L = _lib.lib()
         nch = colors.shape[1]
-        if nch <= 3 and STAGED_BACKWARD:
+        ws = None
+        if staged_backward_applies(nch, ctx.isect):
             # per-intersection records summed per Gaussian: no float atomics,
             # bitwise reproducible
             wsb = L.jaxsplat_rasterize_backward_staged_workspace_size(ctypes.byref(ctx.d),
                                                                       ctx.isect)
-            ws = torch.empty(max(wsb, 1), dtype=torch.uint8, device=dev)
+            try:
+                ws = torch.empty(max(wsb, 1), dtype=torch.uint8, device=dev)
+            except torch.OutOfMemoryError:
+                ws = None       # 52 bytes per intersection do not fit: the atomic backward
+        if ws is not None:
             with torch.cuda.device(dev):
                 if ctx.tight:
                     _lib.check(L.jaxsplat_rasterize_backward_staged_rects(

@@ -216,13 +216,18 @@ class BatchRenderer:
 
     The first call renders, reads the exact intersection count and, if it
     overflowed the initial guess, re-plans with ``headroom`` times the count
-    and renders again.  Later calls check the count only when
-    ``check_every`` > 0 (every that many calls; a host synchronisation), and
-    re-plan on overflow.
+    and renders again.  Later calls check the count every ``check_every``
+    calls (a host synchronisation; default every call) and re-plan and render
+    again on overflow, so a returned image is never one cut off at the
+    capacity.  ``check_every=0`` skips the checks after the first call: an
+    overflow then leaves that call's images and gradients invalid, and the
+    caller must read ``plan.check()`` (or the device status word,
+    ``jaxsplat_batch_status_word``) itself -- the form for loops captured in
+    a HIP graph, where no host synchronisation may run.
     """
 
     def __init__(self, scenes, views, *, glob_scale=1.0, clip_thresh=0.01, block_width=16,
-                 headroom=1.25, check_every=0, device=None, initial_capacity=None):
+                 headroom=1.25, check_every=1, device=None, initial_capacity=None):
         self.scenes = tuple(scenes)
         self.views = tuple(views)
         self.kw = dict(glob_scale=glob_scale, clip_thresh=clip_thresh, block_width=block_width,

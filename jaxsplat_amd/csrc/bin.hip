@@ -543,7 +543,7 @@ hipError_t bin_depth(const BinPlan& p, const Camera& c, const float* xys, const 
 }
 
 hipError_t bin_emit(const BinPlan& p, const Camera& c, int num_isect_host,
-                    const int* num_isect_dev, const void* depth_ws, int* gids_sorted,
+                    const void* depth_ws, int* gids_sorted,
                     int* tiles_sorted, int* tile_bins, void* scratch, hipStream_t s) {
     const BinLayout L = layout(p);
     uint8_t* b = static_cast<uint8_t*>(scratch);
@@ -558,16 +558,21 @@ hipError_t bin_emit(const BinPlan& p, const Camera& c, int num_isect_host,
     const uint8_t* d = static_cast<const uint8_t*>(depth_ws);
     const uint32_t* perm = reinterpret_cast<const uint32_t*>(d + D.perm);
     const uint2* rect_by_id = reinterpret_cast<const uint2*>(d + D.rect_by_id);
+    // the sort and the edges cover the slots the emission wrote, min(total,
+    // capacity), read on the device: a count given too large (an inconsistent
+    // cum_tiles_hit) then never sorts unwritten slots, and one too small is
+    // an overflow recorded in the error word
+    int* written = reinterpret_cast<int*>(b + L.emit.info);
     e = emit<false>(p.n, c.tiles_x, nullptr, nullptr, perm, rect_by_id, p.isect_cap, b, L.emit,
-                    nullptr, nullptr, err, U(L.tiles), U(L.gids), nullptr, s);
+                    written, nullptr, err, U(L.tiles), U(L.gids), nullptr, s);
     if (e != hipSuccess) return e;
     uint32_t* tiles_out = tiles_sorted ? reinterpret_cast<uint32_t*>(tiles_sorted) : U(L.tiles_sorted);
     e = sort_pairs(p.tile_sort, U(L.tiles), U(L.gids), tiles_out,
                    reinterpret_cast<uint32_t*>(gids_sorted), U(L.ttmp_k), U(L.ttmp_v),
-                   num_isect_host, num_isect_dev, b + L.tile_sort, err, s);
+                   num_isect_host, written, b + L.tile_sort, err, s);
     if (e != hipSuccess) return e;
     const int nb = (int)(((long long)p.isect_cap + 256 * kTileBinsPer - 1) / (256 * kTileBinsPer));
-    tile_bins_kernel<<<nb > 0 ? nb : 1, 256, 0, s>>>(tiles_out, num_isect_host, num_isect_dev,
+    tile_bins_kernel<<<nb > 0 ? nb : 1, 256, 0, s>>>(tiles_out, num_isect_host, written,
                                                    p.num_tiles,
                                                    reinterpret_cast<int2*>(tile_bins), err);
     return hipGetLastError();
@@ -576,13 +581,13 @@ hipError_t bin_emit(const BinPlan& p, const Camera& c, int num_isect_host,
 // One workspace: the emission stage's region (error word first), then the
 // depth stage's.
 hipError_t bin_and_sort(const BinPlan& p, const Camera& c, const float* xys, const float* depths,
-                        const int* radii, int num_isect_host, const int* num_isect_dev,
+                        const int* radii, int num_isect_host,
                         int* gids_sorted, int* tiles_sorted, int* tile_bins, void* scratch,
                         hipStream_t s) {
     uint8_t* depth_ws = static_cast<uint8_t*>(scratch) + p.emit_bytes;
     hipError_t e = bin_depth(p, c, xys, depths, radii, nullptr, depth_ws, s);
     if (e != hipSuccess) return e;
-    return bin_emit(p, c, num_isect_host, num_isect_dev, depth_ws, gids_sorted, tiles_sorted,
+    return bin_emit(p, c, num_isect_host, depth_ws, gids_sorted, tiles_sorted,
                     tile_bins, scratch, s);
 }
 

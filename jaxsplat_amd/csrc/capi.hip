@@ -301,7 +301,7 @@ int jaxsplat_bin_and_sort(const jaxsplat_descriptor* d, const float* xys, const 
     if (!workspace || workspace_bytes < align256(p.scratch_bytes))
         return fail(JAXSPLAT_ERR_WORKSPACE, "bin_and_sort: workspace too small (%zu < %zu)",
                     workspace_bytes, align256(p.scratch_bytes));
-    JSA_CHECK_HIP(jsa::bin_and_sort(p, c, xys, depths, radii, (int)num_intersects, nullptr,
+    JSA_CHECK_HIP(jsa::bin_and_sort(p, c, xys, depths, radii, (int)num_intersects,
                                     gaussian_ids_sorted, tile_ids_sorted, tile_bins, workspace, s),
                   "bin_and_sort");
     return JAXSPLAT_OK;
@@ -379,7 +379,7 @@ int jaxsplat_bin_emit_sort(const jaxsplat_descriptor* d, uint32_t num_intersects
     if (!workspace || workspace_bytes < p.emit_bytes)
         return fail(JAXSPLAT_ERR_WORKSPACE, "bin_emit_sort: workspace too small (%zu < %zu)",
                     workspace_bytes, p.emit_bytes);
-    JSA_CHECK_HIP(jsa::bin_emit(p, c, (int)num_intersects, nullptr, depth_workspace,
+    JSA_CHECK_HIP(jsa::bin_emit(p, c, (int)num_intersects, depth_workspace,
                                 gaussian_ids_sorted, tile_ids_sorted, tile_bins, workspace, s),
                   "bin_emit_sort");
     return JAXSPLAT_OK;
@@ -634,38 +634,43 @@ int jaxsplat_rasterize_backward_staged_rects(
 // ---------------------------------------------------------------------------
 // Reference-compatible custom-call targets
 // ---------------------------------------------------------------------------
+// The legacy signature carries no status, so a failing call must not leave
+// its results holding whatever the buffers held before (XLA would consume
+// them as valid): every result buffer is filled with kPoison, the bits of a
+// quiet NaN (an out-of-range value in the integer results), and the message
+// is kept in jaxsplat_last_error().  A descriptor of the wrong length cannot
+// size the results; such a call only sets the message.
 
-void jaxsplat_project_fwd(void* stream, void** buffers, const char* opaque, size_t opaque_len) {
-    g_err.clear();
-    const jaxsplat_descriptor* d = unpack(opaque, opaque_len);
-    if (!d) return;
-    int st = 0;
-    void* ws = cached_workspace(stream, jaxsplat_project_workspace_size(d), &st);
-    if (!ws) return;
-    // operands: means3d, scales, quats, viewmat; results: cov3d, xys, depths,
-    // radii, conics, compensation, num_tiles_hit, cum_tiles_hit (ops.cu:246-258)
-    auto F = [&](int i) { return static_cast<float*>(buffers[i]); };
-    auto I = [&](int i) { return static_cast<int32_t*>(buffers[i]); };
-    jaxsplat_project_forward(d, F(0), F(1), F(2), F(3), F(4), F(5), F(6), I(7), F(8), F(9), I(10),
-                             I(11), ws, jaxsplat_project_workspace_size(d), stream);
+}  // extern "C"
+
+namespace {
+
+constexpr uint32_t kPoison = 0x7FC00000u;
+
+struct Result {
+    int index;          // position in `buffers`
+    size_t words;       // 32-bit words
+};
+
+// Results can be sized from the descriptor unless its sizes are out of range
+// (a descriptor that valid buffers could not have come with).
+bool sizes_ok(const jaxsplat_descriptor* d) {
+    return d->num_points <= (1u << 30) && d->img_shape.x <= (1u << 16) &&
+           d->img_shape.y <= (1u << 16);
 }
 
-void jaxsplat_project_bwd(void* stream, void** buffers, const char* opaque, size_t opaque_len) {
-    g_err.clear();
-    const jaxsplat_descriptor* d = unpack(opaque, opaque_len);
-    if (!d) return;
-    // operands: means3d, scales, quats, viewmat, cov3ds, xys, radii, conics,
-    // compensation, v_compensation, v_xy, v_depth, v_conic; results: v_mean3d,
-    // v_scale, v_quat, v_cov2d, v_cov3d (ops.cu:336-354)
-    auto F = [&](int i) { return static_cast<float*>(buffers[i]); };
-    jaxsplat_project_backward(d, F(0), F(1), F(2), F(3), static_cast<int32_t*>(buffers[6]), F(7),
-                              F(8), F(10), F(11), F(12), F(9), F(13), F(14), F(15), F(16), F(17),
-                              JAXSPLAT_ADJOINT_REFERENCE, stream);
+void poison(void* stream, void** buffers, const Result* res, int k,
+            const jaxsplat_descriptor* d) {
+    if (!sizes_ok(d)) return;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    for (int i = 0; i < k; ++i)
+        if (buffers[res[i].index] && res[i].words)
+            (void)jsa::fill_words(buffers[res[i].index], res[i].words, kPoison, s);
 }
 
-static int legacy_bin(const jaxsplat_descriptor* d, void* stream, const float* xys,
-                      const float* depths, const int32_t* radii, uint32_t isect,
-                      int32_t** gids, int32_t** bins) {
+int legacy_bin(const jaxsplat_descriptor* d, void* stream, const float* xys,
+               const float* depths, const int32_t* radii, uint32_t isect, int32_t** gids,
+               int32_t** bins) {
     const jsa::Camera c = camera_of(d);
     const size_t ntiles = (size_t)c.tiles_x * c.tiles_y;
     const size_t bin_ws = jaxsplat_bin_workspace_size(d, isect);
@@ -676,42 +681,115 @@ static int legacy_bin(const jaxsplat_descriptor* d, void* stream, const float* x
     if (!ws) return st;
     *gids = reinterpret_cast<int32_t*>(ws + bin_ws);
     *bins = reinterpret_cast<int32_t*>(ws + bin_ws + gid_bytes);
-    return jaxsplat_bin_and_sort(d, xys, depths, radii, isect, *gids, nullptr, *bins, ws, bin_ws,
-                                 stream);
+    if ((st = jaxsplat_bin_and_sort(d, xys, depths, radii, isect, *gids, nullptr, *bins, ws,
+                                    bin_ws, stream)))
+        return st;
+    // an overflow (cum_tiles_hit inconsistent with xys / radii) is an error here:
+    // the caller has no other way to learn of it
+    return jaxsplat_bin_check(d, isect, ws, stream);
+}
+
+// operands: means3d, scales, quats, viewmat; results: cov3d, xys, depths,
+// radii, conics, compensation, num_tiles_hit, cum_tiles_hit (ops.cu:246-258)
+int legacy_project_fwd(const jaxsplat_descriptor* d, void* stream, void** buffers) {
+    if (int st = validate(d)) return st;
+    int st = 0;
+    void* ws = cached_workspace(stream, jaxsplat_project_workspace_size(d), &st);
+    if (!ws) return st;
+    auto F = [&](int i) { return static_cast<float*>(buffers[i]); };
+    auto I = [&](int i) { return static_cast<int32_t*>(buffers[i]); };
+    return jaxsplat_project_forward(d, F(0), F(1), F(2), F(3), F(4), F(5), F(6), I(7), F(8),
+                                    F(9), I(10), I(11), ws, jaxsplat_project_workspace_size(d),
+                                    stream);
+}
+
+// operands: colors, opacities, background, xys, depths, radii, conics,
+// cum_tiles_hit; results: final_Ts, final_idx, out_img (ops.cu:408-419)
+int legacy_rasterize_fwd(const jaxsplat_descriptor* d, void* stream, void** buffers) {
+    if (int st = validate(d)) return st;
+    auto F = [&](int i) { return static_cast<float*>(buffers[i]); };
+    auto I = [&](int i) { return static_cast<int32_t*>(buffers[i]); };
+    uint32_t isect = 0;
+    if (int st = read_num_intersects(I(7), d->num_points, stream, &isect)) return st;
+    int32_t *gids = nullptr, *bins = nullptr;
+    if (isect > 0) {
+        if (int st = legacy_bin(d, stream, F(3), F(4), I(5), isect, &gids, &bins)) return st;
+    }
+    return jaxsplat_rasterize_forward_binned(d, gids, bins, isect, F(3), F(6), F(0), F(1), F(2),
+                                             F(8), I(9), F(10), stream);
+}
+
+// operands: colors, opacities, background, xys, depths, radii, conics,
+// cum_tiles_hit, final_Ts, final_idx, v_out_img, v_out_img_alpha; results:
+// v_colors, v_opacity, v_xy, v_xy_abs, v_conic (ops.cu:502-519)
+int legacy_rasterize_bwd(const jaxsplat_descriptor* d, void* stream, void** buffers) {
+    if (int st = validate(d)) return st;
+    auto F = [&](int i) { return static_cast<float*>(buffers[i]); };
+    auto I = [&](int i) { return static_cast<int32_t*>(buffers[i]); };
+    uint32_t isect = 0;
+    if (int st = read_num_intersects(I(7), d->num_points, stream, &isect)) return st;
+    int32_t *gids = nullptr, *bins = nullptr;
+    if (isect > 0) {
+        if (int st = legacy_bin(d, stream, F(3), F(4), I(5), isect, &gids, &bins)) return st;
+    }
+    return jaxsplat_rasterize_backward_binned(d, gids, bins, isect, F(3), F(6), F(0), F(1), F(2),
+                                              F(8), I(9), F(10), F(11), F(12), F(13), F(14),
+                                              F(15), F(16), JAXSPLAT_ADJOINT_REFERENCE, stream);
+}
+
+}  // namespace
+
+extern "C" {
+
+void jaxsplat_project_fwd(void* stream, void** buffers, const char* opaque, size_t opaque_len) {
+    g_err.clear();
+    const jaxsplat_descriptor* d = unpack(opaque, opaque_len);
+    if (!d) return;
+    if (legacy_project_fwd(d, stream, buffers)) {
+        const size_t n = d->num_points;
+        const Result res[] = {{4, 6 * n}, {5, 2 * n}, {6, n},     {7, n},
+                              {8, 3 * n}, {9, n},     {10, n},    {11, n}};
+        poison(stream, buffers, res, 8, d);
+    }
+}
+
+void jaxsplat_project_bwd(void* stream, void** buffers, const char* opaque, size_t opaque_len) {
+    g_err.clear();
+    const jaxsplat_descriptor* d = unpack(opaque, opaque_len);
+    if (!d) return;
+    // operands: means3d, scales, quats, viewmat, cov3ds, xys, radii, conics,
+    // compensation, v_compensation, v_xy, v_depth, v_conic; results: v_mean3d,
+    // v_scale, v_quat, v_cov2d, v_cov3d (ops.cu:336-354)
+    auto F = [&](int i) { return static_cast<float*>(buffers[i]); };
+    if (jaxsplat_project_backward(d, F(0), F(1), F(2), F(3), static_cast<int32_t*>(buffers[6]),
+                                  F(7), F(8), F(10), F(11), F(12), F(9), F(13), F(14), F(15),
+                                  F(16), F(17), JAXSPLAT_ADJOINT_REFERENCE, stream)) {
+        const size_t n = d->num_points;
+        const Result res[] = {{13, 3 * n}, {14, 3 * n}, {15, 4 * n}, {16, 3 * n}, {17, 6 * n}};
+        poison(stream, buffers, res, 5, d);
+    }
 }
 
 void jaxsplat_rasterize_fwd(void* stream, void** buffers, const char* opaque, size_t opaque_len) {
     g_err.clear();
     const jaxsplat_descriptor* d = unpack(opaque, opaque_len);
-    if (!d || validate(d)) return;
-    // operands: colors, opacities, background, xys, depths, radii, conics,
-    // cum_tiles_hit; results: final_Ts, final_idx, out_img (ops.cu:408-419)
-    auto F = [&](int i) { return static_cast<float*>(buffers[i]); };
-    auto I = [&](int i) { return static_cast<int32_t*>(buffers[i]); };
-    uint32_t isect = 0;
-    if (read_num_intersects(I(7), d->num_points, stream, &isect)) return;
-    int32_t *gids = nullptr, *bins = nullptr;
-    if (isect > 0 && legacy_bin(d, stream, F(3), F(4), I(5), isect, &gids, &bins)) return;
-    jaxsplat_rasterize_forward_binned(d, gids, bins, isect, F(3), F(6), F(0), F(1), F(2), F(8),
-                                      I(9), F(10), stream);
+    if (!d) return;
+    if (legacy_rasterize_fwd(d, stream, buffers)) {
+        const size_t px = (size_t)d->img_shape.x * d->img_shape.y;
+        const Result res[] = {{8, px}, {9, px}, {10, 3 * px}};
+        poison(stream, buffers, res, 3, d);
+    }
 }
 
 void jaxsplat_rasterize_bwd(void* stream, void** buffers, const char* opaque, size_t opaque_len) {
     g_err.clear();
     const jaxsplat_descriptor* d = unpack(opaque, opaque_len);
-    if (!d || validate(d)) return;
-    // operands: colors, opacities, background, xys, depths, radii, conics,
-    // cum_tiles_hit, final_Ts, final_idx, v_out_img, v_out_img_alpha; results:
-    // v_colors, v_opacity, v_xy, v_xy_abs, v_conic (ops.cu:502-519)
-    auto F = [&](int i) { return static_cast<float*>(buffers[i]); };
-    auto I = [&](int i) { return static_cast<int32_t*>(buffers[i]); };
-    uint32_t isect = 0;
-    if (read_num_intersects(I(7), d->num_points, stream, &isect)) return;
-    int32_t *gids = nullptr, *bins = nullptr;
-    if (isect > 0 && legacy_bin(d, stream, F(3), F(4), I(5), isect, &gids, &bins)) return;
-    jaxsplat_rasterize_backward_binned(d, gids, bins, isect, F(3), F(6), F(0), F(1), F(2), F(8),
-                                       I(9), F(10), F(11), F(12), F(13), F(14), F(15), F(16),
-                                       JAXSPLAT_ADJOINT_REFERENCE, stream);
+    if (!d) return;
+    if (legacy_rasterize_bwd(d, stream, buffers)) {
+        const size_t n = d->num_points;
+        const Result res[] = {{12, 3 * n}, {13, n}, {14, 2 * n}, {15, 2 * n}, {16, 3 * n}};
+        poison(stream, buffers, res, 5, d);
+    }
 }
 
 }  // extern "C"

@@ -246,11 +246,11 @@ struct RasterGrid {
 ViewRec single_view(const Camera& c);
 RasterGrid single_grid(const Camera& c);
 
-// Zero `bytes` (multiple of 4) starting at p with a fill kernel (never a
-// hipMemsetAsync, so the call is safe inside captured graphs).
 // Zeroes exactly `bytes` bytes at p (4-byte aligned) with a kernel (no memset
 // node when captured in a graph).
 hipError_t fill_zero(void* p, size_t bytes, hipStream_t s);
+// Sets n_words 32-bit words at p to value, with a kernel.
+hipError_t fill_words(void* p, size_t n_words, uint32_t value, hipStream_t s);
 hipError_t fill_value_f32x3(float* dst, size_t n_pixels, const float* rgb, hipStream_t s);
 
 // ---- projection ----
@@ -314,13 +314,16 @@ BinPlan bin_plan(const Camera& c, int isect_cap);
 hipError_t bin_depth(const BinPlan& p, const Camera& c, const float* xys, const float* depths,
                      const int* radii, const uint2* rects, void* depth_ws, hipStream_t s);
 // Emission stage over the depth stage's workspace: emission, tile sort, edges.
+// p.isect_cap bounds the slots written; the sort and the edges cover the slots
+// actually written, min(total, p.isect_cap), read on the device, so that a
+// num_isect_host larger than the total never sorts unwritten slots.
 hipError_t bin_emit(const BinPlan& p, const Camera& c, int num_isect_host,
-                    const int* num_isect_dev, const void* depth_ws, int* gids_sorted,
-                    int* tiles_sorted, int* tile_bins, void* scratch, hipStream_t s);
+                    const void* depth_ws, int* gids_sorted, int* tiles_sorted, int* tile_bins,
+                    void* scratch, hipStream_t s);
 hipError_t bin_and_sort(const BinPlan& p, const Camera& c, const float* xys,
                         const float* depths, const int* radii, int num_isect_host,
-                        const int* num_isect_dev, int* gids_sorted, int* tiles_sorted,
-                        int* tile_bins, void* scratch, hipStream_t s);
+                        int* gids_sorted, int* tiles_sorted, int* tile_bins, void* scratch,
+                        hipStream_t s);
 // status word of the last bin_and_sort using `scratch`: bit 4 bad tile key,
 // bit 8 more intersections than the capacity
 const uint32_t* bin_error_word(const BinPlan& p, const void* scratch);

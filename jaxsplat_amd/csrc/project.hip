@@ -704,11 +704,11 @@ __global__ __launch_bounds__(256) void project_batch_bwd_kernel(
     }
 }
 
-// n words, then tail (0..3) bytes after them.
-__global__ void fill_zero_kernel(uint32_t* __restrict__ p, size_t n, int tail) {
+// n words set to v, then tail (0..3) zero bytes after them.
+__global__ void fill_word_kernel(uint32_t* __restrict__ p, size_t n, int tail, uint32_t v) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
          i += (size_t)gridDim.x * blockDim.x)
-        p[i] = 0u;
+        p[i] = v;
     if (blockIdx.x == 0 && (int)threadIdx.x < tail)
         reinterpret_cast<uint8_t*>(p + n)[threadIdx.x] = 0;
 }
@@ -731,7 +731,15 @@ hipError_t fill_zero(void* p, size_t bytes, hipStream_t s) {
     if (bytes == 0) return hipSuccess;
     const size_t nb = (n + 255) / 256;
     const int blocks = (int)(nb == 0 ? 1 : (nb < 2048 ? nb : 2048));
-    fill_zero_kernel<<<blocks, 256, 0, s>>>(static_cast<uint32_t*>(p), n, tail);
+    fill_word_kernel<<<blocks, 256, 0, s>>>(static_cast<uint32_t*>(p), n, tail, 0u);
+    return hipGetLastError();
+}
+
+hipError_t fill_words(void* p, size_t n_words, uint32_t value, hipStream_t s) {
+    if (n_words == 0) return hipSuccess;
+    const size_t nb = (n_words + 255) / 256;
+    const int blocks = (int)(nb < 2048 ? nb : 2048);
+    fill_word_kernel<<<blocks, 256, 0, s>>>(static_cast<uint32_t*>(p), n_words, 0, value);
     return hipGetLastError();
 }
 

@@ -443,7 +443,11 @@ __device__ __forceinline__ float row_sums_split(float* v, int lane) {
 //   [g[8..9] = sum |v_xy per pixel|]                 (v_xy_abs)
 //   g[NV-1] = sum vs                                 (v_opacity = -sum vs / o)
 // with vs = dL/dsigma = -o * vis * v_alpha (times the clamp's zero slope in
-// consistent mode).
+// consistent mode).  The returned mask of lanes with a valid pair comes from
+// ballots taken inside, so every call must run in wave-uniform control flow
+// with all 64 lanes active (the call sites branch on scalar mask bits only):
+// a lane masked off around the call would be missing from the ballots, and
+// its pairs from the per-Gaussian sums.
 template <bool CONSISTENT, int NV>
 __device__ __forceinline__ uint64_t bwd_pixel(PixState& s, const float4 a, const float4 b,
                                           const float4 col, int k, float* g) {

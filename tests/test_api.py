@@ -55,3 +55,19 @@ def test_rasterize_shapes():
 def test_render_rejects_unknown_adjoint():
     with pytest.raises(ValueError):
         jaxsplat_amd.render(**_args(), **KW, adjoint="exact")
+
+
+@pytest.mark.parametrize("nch,isect,staged", [(3, 0, True), (3, 8_000_000, True),
+                                              (1, 2**30 - 1, True), (3, 2**30, False),
+                                              (3, 2**31 - 1, False), (4, 1000, False),
+                                              (7, 1000, False)])
+def test_rasterize_backward_dispatch(nch, isect, staged, monkeypatch):
+    """The rasterize backward takes the staged entry point for up to three
+    channels and fewer than 2^30 intersections (its record slots are 31-bit
+    and it rejects more); everything else goes to the atomic N-channel
+    backward, which has neither limit."""
+    from jaxsplat_amd import _ops
+    monkeypatch.setattr(_ops, "STAGED_BACKWARD", True)
+    assert _ops.staged_backward_applies(nch, isect) is staged
+    monkeypatch.setattr(_ops, "STAGED_BACKWARD", False)
+    assert _ops.staged_backward_applies(nch, isect) is False

@@ -108,6 +108,24 @@ def test_batch_create_rejects(kw, msg):
     assert not h.value
 
 
+@pytest.mark.parametrize("field,value,msg", [("block_width", 0, b"block_width"),
+                                             ("block_width", 17, b"block_width"),
+                                             ("clip_thresh", -0.5, b"clip_thresh")])
+@pytest.mark.parametrize("target", ["jaxsplat_project_fwd", "jaxsplat_project_bwd",
+                                    "jaxsplat_rasterize_fwd", "jaxsplat_rasterize_bwd"])
+def test_legacy_targets_reject_invalid_descriptor(target, field, value, msg):
+    """An 80-byte descriptor with an invalid field (one make_descriptor would
+    refuse) is rejected by every custom-call target before any device work;
+    the message names the field.  (With NULL result buffers there is nothing
+    to fill; the GPU test of the fill is in test_gpu_legacy.)"""
+    L = _lib.lib()
+    d = _lib.make_descriptor(64, (32, 32), (32.0, 32.0), (16.0, 16.0), 1.0, 0.01, 16)
+    setattr(d, field, value)
+    arr = (ctypes.c_void_p * 18)()
+    getattr(L, target)(None, arr, ctypes.string_at(ctypes.addressof(d), 80), 80)
+    assert msg in L.jaxsplat_last_error()
+
+
 def _fake(n):
     """Host addresses that are never dereferenced: the calls below fail their
     argument checks before any GPU work."""

@@ -213,6 +213,19 @@ def test_batch_capacity_overflow_and_renderer():
         got = H.host(r(*_dev_params(params), H.dev(vm), H.dev(bg)))
     assert r.capacity >= need
     assert np.array_equal(got, ref)
+    # a later call that needs more intersections than the first one sized the
+    # capacity for (most opacities below 1/255 at first) is checked and
+    # rendered again, not cut off at the capacity
+    faint = list(params)
+    faint[4] = (params[4] * np.where(np.arange(params[4].shape[0]) % 8 == 0, 1.0, 1e-3)
+                ).astype(np.float32)
+    r2 = BatchRenderer(sizes, views, initial_capacity=100)
+    with t.no_grad():
+        r2(*_dev_params(faint), H.dev(vm), H.dev(bg))
+        assert r2.capacity < need
+        got2 = H.host(r2(*_dev_params(params), H.dev(vm), H.dev(bg)))
+    assert r2.capacity >= need
+    assert np.array_equal(got2, ref)
 
 
 def test_batch_backward_after_second_forward_raises():

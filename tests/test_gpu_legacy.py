@@ -95,6 +95,80 @@ def test_legacy_targets_match_typed_path_and_oracle():
         assert ok_, (k, worst)
 
 
+POISON = 0x7FC00000
+
+
+def _call_failing(name, bufs, d, msg):
+    L = _lib.lib()
+    arr = (ctypes.c_void_p * len(bufs))(*[b.data_ptr() for b in bufs])
+    getattr(L, name)(H.stream(), arr, ctypes.string_at(ctypes.addressof(d), 80), 80)
+    H.torch().cuda.synchronize()
+    assert msg in L.jaxsplat_last_error(), L.jaxsplat_last_error()
+
+
+def _poisoned(t):
+    a = H.host(t).reshape(-1)
+    return a.size > 0 and bool((a.view(np.uint32) == POISON).all())
+
+
+@pytest.mark.parametrize("case", ["count_too_small", "count_negative"])
+def test_legacy_rasterize_failure_poisons_results(case):
+    """An inconsistent cum_tiles_hit (its last entry below the intersections
+    that xys and radii give, or negative) is an error of the rasterize
+    targets: every result buffer is filled with NaN bits (0x7FC00000 in the
+    integer final_idx) instead of keeping its previous contents."""
+    t = H.torch()
+    sc = scenes.make("small", n=3000)
+    cam = sc.camera
+    N = sc.n
+    Hh, Ww = cam.img_shape
+    d = _lib.make_descriptor(N, cam.img_shape, cam.f, cam.c, 1.0, 0.01, 16)
+    f32 = dict(dtype=t.float32, device="cuda")
+    i32 = dict(dtype=t.int32, device="cuda")
+    g = H.gpu_project(sc.means3d, sc.scales, sc.quats, cam.viewmat, img_shape=cam.img_shape,
+                      f=cam.f, c=cam.c)
+    cum = g["cum_tiles_hit"].reshape(N, 1).copy()
+    assert cum[-1, 0] > 100
+    cum[-1, 0] = cum[-1, 0] // 2 if case == "count_too_small" else -5
+    msg = b"binning error" if case == "count_too_small" else b"negative"
+    colors, opac = H.dev(sc.colors), H.dev(sc.opacities)
+    bg = H.dev(np.zeros(3, np.float32))
+    xys, depths = H.dev(g["xys"]), H.dev(g["depths"].reshape(N, 1))
+    radii, conics = H.dev(g["radii"].reshape(N, 1)), H.dev(g["conics"])
+    dcum = H.dev(cum)
+    outs = [t.zeros((Hh, Ww, 1), **f32), t.zeros((Hh, Ww, 1), **i32),
+            t.zeros((Hh, Ww, 3), **f32)]
+    _call_failing("jaxsplat_rasterize_fwd",
+                  [colors, opac, bg, xys, depths, radii, conics, dcum] + outs, d, msg)
+    assert all(_poisoned(o) for o in outs)
+    fT, fI = t.ones((Hh, Ww, 1), **f32), t.zeros((Hh, Ww, 1), **i32)
+    vi, va = t.ones((Hh, Ww, 3), **f32), t.zeros((Hh, Ww, 1), **f32)
+    grads = [t.zeros((N, 3), **f32), t.zeros((N, 1), **f32), t.zeros((N, 2), **f32),
+             t.zeros((N, 2), **f32), t.zeros((N, 3), **f32)]
+    _call_failing("jaxsplat_rasterize_bwd",
+                  [colors, opac, bg, xys, depths, radii, conics, dcum, fT, fI, vi, va] + grads,
+                  d, msg)
+    assert all(_poisoned(o) for o in grads)
+
+
+def test_legacy_project_failure_poisons_results():
+    """A descriptor the typed entry points reject (block_width 0 written into
+    the 80 bytes directly) leaves every projection result NaN-filled."""
+    t = H.torch()
+    N = 500
+    d = _lib.make_descriptor(N, (64, 64), (64.0, 64.0), (32.0, 32.0), 1.0, 0.01, 16)
+    d.block_width = 0
+    f32 = dict(dtype=t.float32, device="cuda")
+    i32 = dict(dtype=t.int32, device="cuda")
+    ins = [t.zeros((N, 3), **f32), t.ones((N, 3), **f32), t.ones((N, 4), **f32),
+           t.eye(4, **f32)]
+    outs = [t.zeros((N, 6), **f32), t.zeros((N, 2), **f32), t.zeros((N, 1), **f32),
+            t.zeros((N, 1), **i32), t.zeros((N, 3), **f32), t.zeros((N, 1), **f32),
+            t.zeros((N, 1), **i32), t.zeros((N, 1), **i32)]
+    _call_failing("jaxsplat_project_fwd", ins + outs, d, b"block_width")
+    assert all(_poisoned(o) for o in outs)
+
+
 def test_legacy_rejects_bad_descriptor():
     L = _lib.lib()
     arr = (ctypes.c_void_p * 12)()

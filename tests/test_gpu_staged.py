@@ -6,9 +6,12 @@ Gaussian, instead of adding them with float atomics.  Bars: every gradient
 within |g - g_a| <= 1e-5 |g_a| + 1e-6 max|g_a| of the atomic backward on the
 same inputs (the two differ only in the order of float additions); two runs
 bitwise identical; every output row written (rows of Gaussians that no pixel
-reaches are exactly zero, also where the output buffer held garbage); the
-float64 oracle comparison of the render path, which uses this backward, is in
-test_gpu_parity / test_gpu_fullsize.
+reaches are exactly zero, also where the output buffer held garbage).  The
+float64 oracle comparison of the render path, which uses this backward
+through torch autograd, is in test_gpu_modes (small scenes, every gradient,
+both adjoints) and test_gpu_render_fullsize (the 1M- and 3M-Gaussian
+workloads, tile subsets); test_gpu_parity and test_gpu_fullsize check the
+atomic backward.
 """
 
 import ctypes
