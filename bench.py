@@ -8,12 +8,23 @@ adjoint) through the public API ``jaxsplat_amd.render`` and torch autograd.
 Inputs are resident in HBM before the timed region.
 
 Prints ONE JSON line.  Besides the contract fields it carries:
-* ``roofline``: the dominant kernel (rasterize backward) -- algorithmic bytes
-  per launch / its average duration measured with HIP events on the launch
-  stream, against the 8 TB/s HBM peak;
-* ``cpu_baseline``: the float64 NumPy oracle timed on a bounded sample of the
-  same workload on this host (1 core);
-* ``stages``: per-stage milliseconds of the same pipeline;
+* ``roofline``: the dominant kernel (the rasterize backward: its record kernel
+  and per-Gaussian reduction) -- algorithmic bytes per launch / its average
+  duration measured with HIP events on the launch stream, against the 8 TB/s
+  HBM peak -- plus the VALU-issue figures of both rasterize kernels from the
+  committed rocprofv3 counters (``profiles/hd1m_counters.json``, used only
+  while its source hash matches the kernels built here);
+* ``stages``: per-stage milliseconds of the same pipeline (HIP events, and the
+  library's stage marks inside the binning and the backward), each stage's
+  algorithmic bytes and its fraction of the HBM peak;
+* ``stats``: the whole-step fraction of the HBM peak with the algorithmic byte
+  count B_alg of the reference algorithm (reference tile lists, 64-bit key
+  sort), with L_f and L_b measured, and with this implementation's own
+  per-stage byte model;
+* ``graph``: the same step captured in one HIP graph (``render`` with an
+  ``IsectCapacity``: no host synchronisation) and replayed;
+* ``cpu_baseline``: the float64 NumPy oracle on a bounded sample of the same
+  workload, on a process pool over the host's cores;
 * ``batched``: the multi-view workload ``orbit64`` (64 cameras around one
   500,000-Gaussian scene, 800x800 each) and the uneven multi-scene workload
   ``multi32`` (32 scenes of 10k-2M Gaussians, one camera each, mixed image
@@ -27,14 +38,18 @@ from __future__ import annotations
 
 import argparse
 import ctypes
+import hashlib
 import json
+import math
 import os
 import sys
 import time
+from pathlib import Path
 
 import numpy as np
 
 HBM_PEAK_GBS = 8000.0
+ROOT = Path(__file__).resolve().parent
 
 
 def parse():
@@ -45,8 +60,12 @@ def parse():
     ap.add_argument("--workload", default="hd1m", choices=["small", "hd1m", "dense3m"])
     ap.add_argument("--cpu-baseline", dest="cpu_baseline", action="store_true", default=True)
     ap.add_argument("--no-cpu-baseline", dest="cpu_baseline", action="store_false")
-    ap.add_argument("--cpu-tiles", type=int, default=200)
+    ap.add_argument("--cpu-tiles-per-worker", type=int, default=60)
+    ap.add_argument("--cpu-workers", type=int, default=0,
+                    help="CPU-baseline processes (0: the host's cores, at most 16)")
     ap.add_argument("--stage-iters", type=int, default=10)
+    ap.add_argument("--graph", dest="graph", action="store_true", default=True)
+    ap.add_argument("--no-graph", dest="graph", action="store_false")
     ap.add_argument("--batched", dest="batched", action="store_true", default=True)
     ap.add_argument("--no-batched", dest="batched", action="store_false")
     ap.add_argument("--batched-steps", type=int, default=5)
@@ -65,11 +84,17 @@ def main():
               "all-reduce of the per-Gaussian gradients per step; only the one-GPU share is built)",
               file=sys.stderr)
         sys.exit(2)
+    from jaxsplat_amd import scenes
+    sc = scenes.make(args.workload)
+    cam = sc.camera
+    tgt_seed = {"small": 2000, "hd1m": 2001, "dense3m": 2002}[args.workload]
+    # the CPU baseline runs first, in forked worker processes, before this
+    # process touches the GPU
+    cpu = cpu_baseline(sc, cam, tgt_seed, args) if args.cpu_baseline else None
+
     import torch
     import jaxsplat_amd
-    from jaxsplat_amd import _lib, scenes
-    from jaxsplat_amd._ops import _bin, _p, _stream
-
+    from jaxsplat_amd import _lib
     if not torch.cuda.is_available():
         print("bench.py needs a GPU", file=sys.stderr)
         sys.exit(1)
@@ -86,10 +111,7 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
-    sc = scenes.make(args.workload)
-    cam = sc.camera
     Himg, Wimg = cam.img_shape
-    tgt_seed = {"small": 2000, "hd1m": 2001, "dense3m": 2002}[args.workload]
     target = torch.from_numpy(scenes.target_image(cam.img_shape, tgt_seed)).to(dev)
     params = [torch.from_numpy(a).to(dev).requires_grad_(True) for a in sc.params()]
     viewmat = torch.from_numpy(cam.viewmat).to(dev)
@@ -122,11 +144,12 @@ def main():
     mpix = Himg * Wimg / (ms * 1e-3) / 1e6
 
     stages, roof, extra = stage_breakdown(torch, sc, cam, params, target, args.stage_iters)
+    graph = graph_leg(torch, sc, cam, params, kw, target, args.steps, args.warmup,
+                      extra["num_intersects_binned"]) if args.graph else None
+    step_fractions(extra, ms, graph)
 
     batched = [batched_vs_plain(torch, w, args.batched_steps)
                for w in args.batched_workloads.split(",") if w] if args.batched else None
-
-    cpu = cpu_baseline(sc, cam, tgt_seed, args.cpu_tiles) if args.cpu_baseline else None
 
     rec = {
         "metric": "fwd+bwd Mpix/s at 1M Gaussians 1080p, 1 GPU; % HBM roofline; batched vs plain",
@@ -143,159 +166,406 @@ def main():
         "data": "synthetic (seeded scene and U(0,1) target)",
         "config": {"workload": args.workload, "num_gaussians": sc.n, "img_shape": [Himg, Wimg],
                    "block_width": sc.block_width, "f": list(cam.f), "c": list(cam.c),
-                   "path": "plain single view (jaxsplat_amd.render + torch autograd)",
+                   "path": "plain single view (jaxsplat_amd.render + torch autograd, eager)",
                    "parallelism": "none (1 GPU)"},
         "roofline": roof,
         "cpu_baseline": cpu,
-        "stages_ms": stages,
+        "stages": stages,
         "stats": extra,
+        "graph": graph,
         "batched": batched,
         "wall_s": round(wall, 3),
     }
     print(json.dumps(rec), flush=True)
 
 
+# ---------------------------------------------------------------------------
+# Committed counter summaries (rocprofv3) and their validity
+# ---------------------------------------------------------------------------
+STAGED_BWD = "raster_bwd_kernel<*, *, *, true, *> + raster_bwd_reduce_kernel"
+
+
+def source_hash():
+    """sha256 (16 hex digits) of the HIP sources the library is built from."""
+    h = hashlib.sha256()
+    for p in sorted((ROOT / "jaxsplat_amd" / "csrc").glob("*")):
+        if p.suffix in (".hip", ".hpp", ".h"):
+            h.update(p.name.encode())
+            h.update(p.read_bytes())
+    h.update((ROOT / "include" / "jaxsplat_amd.h").read_bytes())
+    return h.hexdigest()[:16]
+
+
+def committed_counters(workload):
+    """The committed rocprofv3 counter summary of this workload, if it was
+    measured on the kernels built from the current sources (else None and a
+    reason)."""
+    p = ROOT / "profiles" / f"{workload}_counters.json"
+    try:
+        rec = json.loads(p.read_text())
+    except (OSError, ValueError):
+        return None, f"no {p.name}"
+    if rec.get("source_sha16") != source_hash():
+        return None, f"{p.name} was measured on other kernel sources (stale)"
+    return rec, None
+
+
+def _kernel_entry(rec, prefix):
+    for name, v in rec.get("kernels", {}).items():
+        if name.startswith(prefix):
+            return v
+    return None
+
+
+# ---------------------------------------------------------------------------
+# Per-stage timing and byte accounting
+# ---------------------------------------------------------------------------
+def consumed_lengths(torch, gids, bins, xys, conics, opac, fT, fI, img_shape, bw, chunk=1 << 17):
+    """(L_f, L_b) of one render on the given tile lists: L_b = sum over tiles
+    of (largest final index - start + 1); L_f = sum over tiles of the
+    consumed prefix of the forward: (largest termination index - start + 1)
+    when every pixel of the tile terminated, else the whole list.  A pixel
+    terminated iff some pair after its last contributor passes the alpha test
+    (the first such pair is where it stopped); a pixel with T = 1 had no
+    contributor and never terminated.  Pairs are evaluated on the GPU in
+    float32 with the reference's formula; borderline pairs may be decided
+    differently from the kernel, which moves L_f by a few entries at most."""
+    H, W = img_shape
+    tx_n, ty_n = (W + bw - 1) // bw, (H + bw - 1) // bw
+    T = tx_n * ty_n
+    P = bw * bw
+    dev = gids.device
+    b = bins.to(torch.int64)
+    lens = (b[:, 1] - b[:, 0]).clamp_min(0)
+    fT = fT.reshape(-1)
+    fI = fI.reshape(-1).to(torch.int64)
+    # per tile pixel: image index and inside mask
+    py = torch.arange(P, device=dev) // bw
+    px = torch.arange(P, device=dev) % bw
+    tiles = torch.arange(T, device=dev)
+    gy = (tiles // tx_n)[:, None] * bw + py[None, :]
+    gx = (tiles % tx_n)[:, None] * bw + px[None, :]
+    inside = (gy < H) & (gx < W)
+    lin = torch.where(inside, gy * W + gx, torch.zeros_like(gy))
+    fidx_t = torch.where(inside, fI[lin], torch.full_like(lin, -1))
+    # L_b
+    tmax = fidx_t.max(dim=1).values
+    L_b = int(torch.clamp(tmax - b[:, 0] + 1, min=torch.zeros_like(lens), max=lens).sum())
+    # L_f
+    big = torch.iinfo(torch.int64).max
+    contrib = inside & (fT[lin] < 1.0)
+    term = torch.full((T * P,), big, dtype=torch.int64, device=dev)
+    n = int(lens.sum())
+    tile_of = torch.repeat_interleave(tiles, lens)
+    ks = torch.repeat_interleave(b[:, 0], lens) + \
+        (torch.arange(n, device=dev) - torch.repeat_interleave(torch.cumsum(lens, 0) - lens, lens))
+    xy = xys.reshape(-1, 2)
+    cn = conics.reshape(-1, 3)
+    op = opac.reshape(-1)
+    for k0 in range(0, n, chunk):
+        t = tile_of[k0:k0 + chunk]
+        k = ks[k0:k0 + chunk]
+        g = gids[k].to(torch.int64)
+        dx = xy[g, 0][:, None] - (gx[t].to(torch.float32) + 0.5)
+        dy = xy[g, 1][:, None] - (gy[t].to(torch.float32) + 0.5)
+        sig = 0.5 * (cn[g, 0][:, None] * dx * dx + cn[g, 2][:, None] * dy * dy) + \
+            cn[g, 1][:, None] * dx * dy
+        alpha = torch.clamp(op[g][:, None] * torch.exp(-sig), max=0.999)
+        ok = (sig >= 0) & (alpha >= 1.0 / 255.0) & contrib[t] & (k[:, None] > fidx_t[t])
+        cand = torch.where(ok, k[:, None].expand_as(ok), torch.full_like(ok, big, dtype=torch.int64))
+        dest = t[:, None] * P + torch.arange(P, device=dev)[None, :]
+        term.scatter_reduce_(0, dest.reshape(-1), cand.reshape(-1), reduce="amin")
+    term = term.reshape(T, P)
+    done = (term < big) | ~inside
+    all_done = done.all(dim=1) & (lens > 0)
+    tmax_term = torch.where(inside, term, torch.full_like(term, -1)).max(dim=1).values
+    lf_tile = torch.where(all_done, tmax_term - b[:, 0] + 1, lens)
+    L_f = int(lf_tile.sum())
+    return L_f, L_b, int(all_done.sum())
+
+
+def _popcount32(torch, words):
+    w = words.to(torch.int64) & 0xFFFFFFFF
+    c = torch.zeros_like(w)
+    for s in range(32):
+        c += (w >> s) & 1
+    return int(c.sum())
+
+
 def stage_breakdown(torch, sc, cam, params, target, iters):
-    """Run the same pipeline through the C ABI with HIP events between stages."""
+    """The render path through the C ABI with HIP events between stages and
+    the library's stage marks inside the binning and the staged backward."""
     from jaxsplat_amd import _lib
-    from jaxsplat_amd._ops import TIGHT_TILES, _bin, _bin_rects, _p, _stream
+    from jaxsplat_amd._ops import _bin, _p, _stream
     L = _lib.lib()
     dev = params[0].device
     N = sc.n
     Himg, Wimg = cam.img_shape
+    bw = sc.block_width
     means3d, scales, quats, colors, opac = [p.detach() for p in params]
     viewmat = torch.from_numpy(cam.viewmat).to(dev)
     bg = torch.from_numpy(sc.background).to(dev)
-    d = _lib.make_descriptor(N, cam.img_shape, cam.f, cam.c, sc.glob_scale, sc.clip_thresh,
-                             sc.block_width)
+    d = _lib.make_descriptor(N, cam.img_shape, cam.f, cam.c, sc.glob_scale, sc.clip_thresh, bw)
+    D = ctypes.byref(d)
     f32 = dict(dtype=torch.float32, device=dev)
     i32 = dict(dtype=torch.int32, device=dev)
     xys, depths, conics = torch.empty((N, 2), **f32), torch.empty((N, 1), **f32), \
         torch.empty((N, 3), **f32)
+    comp = torch.empty((N, 1), **f32)
     radii, nth, cum = torch.empty((N, 1), **i32), torch.empty((N, 1), **i32), \
         torch.empty((N, 1), **i32)
-    # the render path's projection also writes the opacity-limited tile
-    # rectangles and counts them (reference rectangles and counts below)
+    # the render path's projection writes the opacity-limited tile rectangles
+    # and counts them (reference rectangles and counts further below)
     rects, tnth, tcum = torch.empty((N, 2), **i32), torch.empty((N, 1), **i32), \
         torch.empty((N, 1), **i32)
-    wsb = L.jaxsplat_project_workspace_size(ctypes.byref(d))
+    wsb = L.jaxsplat_project_workspace_size(D)
     ws = torch.empty(wsb, dtype=torch.uint8, device=dev)
+    s = _stream(dev)
+
+    def project(opacity_limited):
+        if opacity_limited:
+            _lib.check(L.jaxsplat_project_forward_rects(
+                D, _p(means3d), _p(scales), _p(quats), _p(viewmat), _p(opac), None, _p(xys),
+                _p(depths), _p(radii), _p(conics), _p(comp), _p(tnth), _p(tcum), _p(rects),
+                _p(ws), wsb, s))
+        else:
+            _lib.check(L.jaxsplat_project_forward(
+                D, _p(means3d), _p(scales), _p(quats), _p(viewmat), None, _p(xys), _p(depths),
+                _p(radii), _p(conics), _p(comp), _p(nth), _p(cum), _p(ws), wsb, s))
+
+    project(True)
+    isect = int(tcum[-1, 0].item())                 # read once, outside the timed loop
+    ntiles = d.grid_dim_2d.x * d.grid_dim_2d.y
+    dwsb = L.jaxsplat_bin_depth_workspace_size(D)
+    dws = torch.empty(dwsb, dtype=torch.uint8, device=dev)
+    ewsb = L.jaxsplat_bin_emit_workspace_size(D, isect)
+    ews = torch.empty(ewsb, dtype=torch.uint8, device=dev)
+    gids = torch.empty(max(isect, 1), **i32)
+    bins = torch.empty((ntiles, 2), **i32)
     fT, fI = torch.empty((Himg, Wimg), **f32), torch.empty((Himg, Wimg), **i32)
     img = torch.empty((Himg, Wimg, 3), **f32)
+    v_img = torch.empty_like(img)
     vc, vo, vxy, vcn = (torch.empty((N, 3), **f32), torch.empty((N, 1), **f32),
                         torch.empty((N, 2), **f32), torch.empty((N, 3), **f32))
     vm, vs, vq = torch.empty((N, 3), **f32), torch.empty((N, 3), **f32), torch.empty((N, 4), **f32)
-    names = ["project_fwd", "bin_sort", "raster_fwd", "loss_grad", "raster_bwd", "project_bwd"]
+    bwsb = L.jaxsplat_rasterize_backward_staged_workspace_size(D, isect)
+    bws = torch.empty(bwsb, dtype=torch.uint8, device=dev)
+    names = ["project_fwd", "depth_sort", "emission", "tile_sort", "bin_edges", "raster_fwd",
+             "loss_grad", "raster_bwd_records", "raster_bwd_reduce", "project_bwd"]
     acc = {k: 0.0 for k in names}
-    s = _stream(dev)
-    bws = None
-    for it in range(iters + 2):
-        ev = [torch.cuda.Event(enable_timing=True) for _ in range(len(names) + 1)]
-        ev[0].record()
-        if TIGHT_TILES:
-            _lib.check(L.jaxsplat_project_forward_rects(
-                ctypes.byref(d), _p(means3d), _p(scales), _p(quats), _p(viewmat), _p(opac),
-                None, _p(xys), _p(depths), _p(radii), _p(conics), None, _p(tnth), _p(tcum),
-                _p(rects), _p(ws), wsb, s))
-        else:
-            _lib.check(L.jaxsplat_project_forward(
-                ctypes.byref(d), _p(means3d), _p(scales), _p(quats), _p(viewmat), None, _p(xys),
-                _p(depths), _p(radii), _p(conics), None, _p(nth), _p(cum), _p(ws), wsb, s))
-        ev[1].record()
-        # the render path's binning: into the opacity-limited rectangles, or
-        # into the reference's
-        if TIGHT_TILES:
-            gids, bins, isect = _bin_rects(d, depths, radii, rects, tcum, dev)
-            brects, bcum = rects, tcum
-        else:
-            isect = int(cum[-1, 0].item())
-            gids, bins, _ = _bin(d, xys, depths, radii, isect, dev)
-            brects, bcum = None, cum
-        ev[2].record()
-        _lib.check(L.jaxsplat_rasterize_forward_binned(
-            ctypes.byref(d), _p(gids), _p(bins), isect, _p(xys), _p(conics), _p(colors), _p(opac),
-            _p(bg), _p(fT), _p(fI), _p(img), s))
-        ev[3].record()
-        v_img = (2.0 / img.numel()) * (img - target)
-        bwsb = L.jaxsplat_rasterize_backward_staged_workspace_size(ctypes.byref(d), isect)
-        if bws is None or bws.numel() < bwsb:
-            bws = torch.empty(bwsb, dtype=torch.uint8, device=dev)
-        ev[4].record()
-        # the render path's backward: staged records + per-Gaussian reduction
-        if brects is not None:
+    marks = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    for m in marks:
+        m.record()                                  # creates the event
+    torch.cuda.synchronize()
+    handles = (ctypes.c_void_p * 3)(*[m.cuda_event for m in marks])
+    L.jaxsplat_set_stage_marks(handles, 3)
+    try:
+        for it in range(iters + 2):
+            ev = [torch.cuda.Event(enable_timing=True) for _ in range(8)]
+            ev[0].record()
+            project(True)
+            ev[1].record()
+            _lib.check(L.jaxsplat_bin_depth_sort_rects(D, _p(depths), _p(radii), _p(rects),
+                                                       _p(dws), dwsb, s))
+            ev[2].record()
+            _lib.check(L.jaxsplat_bin_emit_sort(D, isect, _p(dws), _p(gids), None, _p(bins),
+                                                _p(ews), ewsb, s))
+            ev[3].record()
+            _lib.check(L.jaxsplat_rasterize_forward_binned(
+                D, _p(gids), _p(bins), isect, _p(xys), _p(conics), _p(colors), _p(opac), _p(bg),
+                _p(fT), _p(fI), _p(img), s))
+            ev[4].record()
+            torch.mul(torch.sub(img, target), 2.0 / img.numel(), out=v_img)
+            ev[5].record()
             _lib.check(L.jaxsplat_rasterize_backward_staged_rects(
-                ctypes.byref(d), _p(gids), _p(bins), isect, _p(xys), _p(conics), _p(colors),
-                _p(opac), _p(bg), 3, _p(radii), _p(brects), _p(bcum), _p(fT), _p(fI), _p(v_img),
-                None, _p(vc), _p(vo), _p(vxy), None, _p(vcn), 0, _p(bws), bwsb, s))
-        else:
-            _lib.check(L.jaxsplat_rasterize_backward_staged(
-                ctypes.byref(d), _p(gids), _p(bins), isect, _p(xys), _p(conics), _p(colors),
-                _p(opac), _p(bg), 3, _p(radii), _p(bcum), _p(fT), _p(fI), _p(v_img), None,
-                _p(vc), _p(vo), _p(vxy), None, _p(vcn), 0, _p(bws), bwsb, s))
-        ev[5].record()
-        _lib.check(L.jaxsplat_project_backward(
-            ctypes.byref(d), _p(means3d), _p(scales), _p(quats), _p(viewmat), _p(radii),
-            _p(conics), None, _p(vxy), None, _p(vcn), None, _p(vm), _p(vs), _p(vq), None, None,
-            0, s))
-        ev[6].record()
-        torch.cuda.synchronize()
-        if it >= 2:
-            for k, n in enumerate(names):
-                acc[n] += ev[k].elapsed_time(ev[k + 1]) / iters
-    # algorithmic bytes of the dominant kernel (rasterize backward), per launch:
-    #   40 B per consumed intersection (id 4 + xy 8 + opacity 4 + conic 12 + color 12)
-    # + 20 B per pixel (final T 4 + final idx 4 + v_img 12)
-    # + 36 B per visible Gaussian (v_rgb 12 + v_opacity 4 + v_xy 8 + v_conic 12 written)
-    # + 8 B per tile (bin read)
-    # The intersections are counted on the reference's tile lists (exact
-    # binning of the same projection, outside the timed region), also when the
-    # timed path bins into the shorter opacity-limited lists.
-    bw = sc.block_width
-    tiles_x, tiles_y = (Wimg + bw - 1) // bw, (Himg + bw - 1) // bw
+                D, _p(gids), _p(bins), isect, _p(xys), _p(conics), _p(colors), _p(opac), _p(bg),
+                3, _p(radii), _p(rects), _p(tcum), _p(fT), _p(fI), _p(v_img), None, _p(vc),
+                _p(vo), _p(vxy), None, _p(vcn), 0, _p(bws), bwsb, s))
+            ev[6].record()
+            _lib.check(L.jaxsplat_project_backward(
+                D, _p(means3d), _p(scales), _p(quats), _p(viewmat), _p(radii), _p(conics),
+                None, _p(vxy), None, _p(vcn), None, _p(vm), _p(vs), _p(vq), None, None, 0, s))
+            ev[7].record()
+            torch.cuda.synchronize()
+            if it >= 2:
+                t = [ev[0].elapsed_time(ev[1]), ev[1].elapsed_time(ev[2]),
+                     ev[2].elapsed_time(marks[0]), marks[0].elapsed_time(marks[1]),
+                     marks[1].elapsed_time(ev[3]), ev[3].elapsed_time(ev[4]),
+                     ev[4].elapsed_time(ev[5]), ev[5].elapsed_time(marks[2]),
+                     marks[2].elapsed_time(ev[6]), ev[6].elapsed_time(ev[7])]
+                for k, n in enumerate(names):
+                    acc[n] += t[k] / iters
+    finally:
+        L.jaxsplat_set_stage_marks(None, 0)
+    _lib.check(L.jaxsplat_bin_check(D, isect, _p(ews), s))
 
-    def consumed(bins_t, fI_t):
-        bins_np = bins_t.cpu().numpy().astype(np.int64)
-        pad = np.full((tiles_y * bw, tiles_x * bw), -1, np.int64)
-        pad[:Himg, :Wimg] = fI_t.cpu().numpy().astype(np.int64)
-        tmax = pad.reshape(tiles_y, bw, tiles_x, bw).max(axis=(1, 3)).reshape(-1)
-        return int(np.clip(tmax - bins_np[:, 0] + 1, 0, bins_np[:, 1] - bins_np[:, 0]).sum())
-
-    isect_binned, L_b_binned = isect, consumed(bins, fI)
-    if brects is not None:
-        # the reference's counts of the same projection
-        _lib.check(L.jaxsplat_project_forward(
-            ctypes.byref(d), _p(means3d), _p(scales), _p(quats), _p(viewmat), None, _p(xys),
-            _p(depths), _p(radii), _p(conics), None, _p(nth), _p(cum), _p(ws), wsb, s))
-    isect = int(cum[-1, 0].item())
-    if brects is not None:
-        gids, bins, _ = _bin(d, xys, depths, radii, isect, dev)
-        fT2, fI2, img2 = torch.empty_like(fT), torch.empty_like(fI), torch.empty_like(img)
-        _lib.check(L.jaxsplat_rasterize_forward_binned(
-            ctypes.byref(d), _p(gids), _p(bins), isect, _p(xys), _p(conics), _p(colors), _p(opac),
-            _p(bg), _p(fT2), _p(fI2), _p(img2), s))
-        L_b = consumed(bins, fI2)
-    else:
-        L_b = L_b_binned
+    # ---- counts on the render path's (opacity-limited) lists ----
     V = int((radii > 0).sum().item())
-    T = tiles_x * tiles_y
-    bytes_bwd = 40 * L_b + 20 * Himg * Wimg + 36 * V + 8 * T
-    t_bwd = acc["raster_bwd"] * 1e-3
-    achieved = bytes_bwd / t_bwd / 1e9
-    roof = {"kernel": STAGED_BWD,
-            "bound": "hbm", "achieved": round(achieved, 2),
+    E = int((tnth > 0).sum().item())
+    R = _popcount32(torch, bws[((48 * max(isect, 1) + 255) // 256) * 256:].view(torch.int32)
+                    [:(isect + 31) // 32])
+    L_ft, L_bt, _ = consumed_lengths(torch, gids[:isect], bins, xys, conics, opac, fT, fI,
+                                     cam.img_shape, bw)
+    # ---- the reference algorithm's counts: its tile lists of the same projection ----
+    project(False)
+    I_ref = int(cum[-1, 0].item())
+    rgids, rbins, _ = _bin(d, xys, depths, radii, I_ref, dev)
+    fT2, fI2, img2 = torch.empty_like(fT), torch.empty_like(fI), torch.empty_like(img)
+    _lib.check(L.jaxsplat_rasterize_forward_binned(
+        D, _p(rgids), _p(rbins), I_ref, _p(xys), _p(conics), _p(colors), _p(opac), _p(bg),
+        _p(fT2), _p(fI2), _p(img2), s))
+    L_f, L_b, tiles_done = consumed_lengths(torch, rgids, rbins, xys, conics, opac, fT2, fI2,
+                                            cam.img_shape, bw)
+    same_img = bool(torch.equal(img2.view(torch.int32), img.view(torch.int32)))
+    T = ntiles
+    Px = Himg * Wimg
+
+    # ---- algorithmic bytes of each stage of this implementation ----
+    # (documented term by term in the design notes; N Gaussians, V visible,
+    # E with a non-empty opacity-limited rectangle, I = isect pairs binned,
+    # L_ft / L_bt consumed by the forward / backward, R records flushed)
+    stage_bytes = {
+        "project_fwd": 100 * N,
+        "depth_sort": 104 * N,
+        "emission": 32 * N + 32 * E + 8 * isect,
+        "tile_sort": 40 * isect,
+        "bin_edges": 4 * isect + 8 * T,
+        "raster_fwd": 40 * L_ft + 8 * T + 20 * Px,
+        "loss_grad": 36 * Px,
+        "raster_bwd_records": 40 * L_bt + 8 * T + 20 * Px + 48 * R + (isect + 7) // 8,
+        "raster_bwd_reduce": 48 * R + (isect + 7) // 8 + 16 * N + 36 * N,
+        "project_bwd": 76 * V + 4 * (N - V) + 40 * N,
+    }
+    valu_bound = {"raster_fwd", "raster_bwd_records"}
+    stages = {}
+    for k in names:
+        ms_k = acc[k]
+        gbs = stage_bytes[k] / (ms_k * 1e-3) / 1e9 if ms_k > 0 else 0.0
+        stages[k] = {"ms": round(ms_k, 4), "alg_bytes": int(stage_bytes[k]),
+                     "GBs": round(gbs, 1), "frac_hbm": round(gbs / HBM_PEAK_GBS, 4),
+                     "bound": "valu" if k in valu_bound else "hbm"}
+
+    # ---- roofline of the dominant kernel: the rasterize backward ----
+    # algorithmic bytes per launch, counted on the reference's lists of the
+    # same projection (L_b): 40 B per consumed intersection (id 4 + xy 8 +
+    # opacity 4 + conic 12 + colour 12) + 20 B per pixel (final T 4 + final
+    # index 4 + v_img 12) + 36 B per visible Gaussian (v_rgb 12 + v_opacity 4
+    # + v_xy 8 + v_conic 12 written) + 8 B per tile (bin read)
+    bytes_bwd = 40 * L_b + 20 * Px + 36 * V + 8 * T
+    ms_bwd = acc["raster_bwd_records"] + acc["raster_bwd_reduce"]
+    achieved = bytes_bwd / (ms_bwd * 1e-3) / 1e9
+    counters, why = committed_counters(sc.name)
+    roof = {"kernel": STAGED_BWD, "bound": "hbm", "achieved": round(achieved, 2),
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-            "traffic": measured_traffic(sc.name), "avg_ms": round(acc["raster_bwd"], 4),
-            "bytes_per_launch": bytes_bwd}
-    # whole-step algorithmic bytes (B_alg) for the combined fraction
-    P = 2 if T > 256 else 1
-    B_alg = 232 * N + 20 * V + (20 + 24 * P) * isect + 40 * 2 * L_b + 24 * T + 40 * Himg * Wimg
-    extra = {"num_intersects": isect, "visible": V, "L_b": L_b,
-             "num_intersects_binned": isect_binned, "L_b_binned": L_b_binned,
-             "B_alg_bytes": int(B_alg), "stage_sum_ms": round(sum(acc.values()), 4),
-             "step_frac_hbm": round(B_alg / (sum(acc.values()) * 1e-3) / 1e9 / HBM_PEAK_GBS, 5)}
-    return {k: round(v, 4) for k, v in acc.items()}, roof, extra
+            "traffic": None, "avg_ms": round(ms_bwd, 4), "bytes_per_launch": int(bytes_bwd)}
+    if counters is not None:
+        kr = _kernel_entry(counters, "jsa::raster_bwd_kernel<false, false, 64, true, 4>")
+        kd = _kernel_entry(counters, "jsa::raster_bwd_reduce_kernel<false>")
+        kf = _kernel_entry(counters, "jsa::raster_fwd_kernel<256>")
+        if kr and kd and kr.get("hbm_bytes") and kd.get("hbm_bytes"):
+            roof["traffic"] = int(kr["hbm_bytes"] + kd["hbm_bytes"])
+            roof["traffic_over_alg"] = round(roof["traffic"] / bytes_bwd, 3)
+        valu = {}
+        for key, k_ in (("raster_bwd_records", kr), ("raster_fwd", kf)):
+            if k_ and k_.get("valu_busy") is not None:
+                ent = L_bt if key == "raster_bwd_records" else L_ft
+                valu[key] = {"valu_busy": k_["valu_busy"],
+                             "valu_insts_per_launch": k_.get("valu_insts"),
+                             "valu_insts_per_entry": round(k_["valu_insts"] / ent, 1)
+                             if k_.get("valu_insts") else None,
+                             "lds_bank_conflict_frac": k_.get("lds_bank_conflict_frac")}
+        roof["valu"] = valu
+        roof["counters_source"] = f"profiles/{sc.name}_counters.json"
+    else:
+        roof["counters_note"] = why
+
+    # ---- whole-step fractions of the HBM peak ----
+    # B_alg of the reference algorithm (one sort per step, 64-bit keys on
+    # 32 + ceil(log2 T) bits in 8-bit LSD passes)
+    P = math.ceil((32 + math.ceil(math.log2(T))) / 8)
+    B_alg = 232 * N + 20 * V + (20 + 24 * P) * I_ref + 40 * (L_f + L_b) + 24 * T + 40 * Px
+    step_ms = sum(acc.values())
+    B_impl = sum(stage_bytes.values())
+    extra = {"num_intersects": I_ref, "visible": V, "emitting": E, "L_f": L_f, "L_b": L_b,
+             "tiles_all_terminated": tiles_done, "sort_passes_P": P,
+             "num_intersects_binned": isect, "L_f_binned": L_ft, "L_b_binned": L_bt,
+             "records_flushed": R, "images_equal_on_both_lists": same_img,
+             "B_alg_bytes": int(B_alg), "stage_sum_ms": round(step_ms, 4),
+             "impl_bytes_per_step": int(B_impl)}
+    return stages, roof, extra
 
 
+def step_fractions(extra, ms, graph):
+    """Whole-step fractions of the HBM peak: B_alg (the reference algorithm's
+    bytes) and this implementation's per-stage byte model, over the eager
+    step time and over the graph replay's."""
+    def frac(b, t_ms):
+        return round(b / (t_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5)
+    extra["step_frac_hbm"] = frac(extra["B_alg_bytes"], ms)
+    extra["impl_step_frac_hbm"] = frac(extra["impl_bytes_per_step"], ms)
+    if graph:
+        extra["step_frac_hbm_graph"] = frac(extra["B_alg_bytes"], graph["ms_per_step"])
+
+
+# ---------------------------------------------------------------------------
+# The same step as one HIP graph replay
+# ---------------------------------------------------------------------------
+def graph_leg(torch, sc, cam, params, kw, target, steps, warmup, isect_hint):
+    """render() with an IsectCapacity (no host synchronisation) + the fused
+    L2 loss and gradient (jaxsplat_fit_l2: loss sum and v_img in one kernel) +
+    autograd backward, captured once in a HIP graph and replayed."""
+    import jaxsplat_amd
+    from jaxsplat_amd import _lib
+    from jaxsplat_amd._ops import _p, _stream
+    L = _lib.lib()
+    dev = params[0].device
+    cap = jaxsplat_amd.IsectCapacity(int(isect_hint * 1.1) + 4096, device=dev)
+    static = [p.detach().clone().requires_grad_(True) for p in params]
+    v_img = torch.empty_like(target)
+    loss_sum = torch.zeros(1, dtype=torch.float32, device=dev)
+
+    def gstep():
+        img = jaxsplat_amd.render(*static, **kw, isect_capacity=cap)
+        _lib.check(L.jaxsplat_fit_l2(img.numel(), _p(img), _p(target), _p(v_img), _p(loss_sum),
+                                     _stream(dev)))
+        img.backward(v_img)
+
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        for _ in range(2):
+            for p in static:
+                p.grad = None
+            gstep()
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    for p in static:
+        p.grad = None
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        gstep()
+    for _ in range(warmup):
+        graph.replay()
+    torch.cuda.synchronize()
+    cap.check()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        graph.replay()
+    e1.record()
+    torch.cuda.synchronize()
+    total = cap.check()                    # raises if a replay overflowed the capacity
+    ms = e0.elapsed_time(e1) / steps
+    H, W = cam.img_shape
+    return {"path": "render(isect_capacity=...) + fused L2 + backward, one HIP graph replay",
+            "ms_per_step": round(ms, 4), "mpix_s": round(H * W / (ms * 1e-3) / 1e6, 1),
+            "capacity": cap.capacity, "num_intersects": total,
+            "loss": float(loss_sum.item()) / target.numel()}
+
+
+# ---------------------------------------------------------------------------
+# Batched workloads
+# ---------------------------------------------------------------------------
 def _batched_workload(torch, name, dev):
     """Scenes, views, parameters and targets of a batched workload."""
     from jaxsplat_amd import ViewSpec, scenes
@@ -326,6 +596,13 @@ def _batched_workload(torch, name, dev):
     return sizes, views, cams, params, vms, bgs, targets, meta, glob, clip
 
 
+# tile sides the autotuner may pick per workload: the tile side is part of
+# the reference's semantics (a Gaussian joins the lists of the tiles its
+# 3-sigma rectangle touches), so the plain calls it is compared with use the
+# same side; the uneven multi-scene workload allows 8 and 16
+BLOCK_WIDTHS = {"orbit64": (16,), "multi32": (8, 16)}
+
+
 def batched_vs_plain(torch, name, steps, warmup=2):
     """All views of a batched workload per step: the batched path (autotuned,
     one HIP graph replay of forward + L2 gradient + backward) against one
@@ -338,7 +615,8 @@ def batched_vs_plain(torch, name, steps, warmup=2):
     sizes, views, cams, params, vms, bgs, targets, meta, glob, clip = \
         _batched_workload(torch, name, dev)
     V = len(views)
-    tuning = autotune(sizes, views, *params, vms, bgs, glob_scale=glob, clip_thresh=clip)
+    tuning = autotune(sizes, views, *params, vms, bgs, glob_scale=glob, clip_thresh=clip,
+                      block_widths=BLOCK_WIDTHS[name])
     plan = tuned_plan(tuning, sizes, views, glob_scale=glob, clip_thresh=clip, device=dev)
     P = plan.num_pixels
     L = _lib.lib()
@@ -412,7 +690,8 @@ def batched_vs_plain(torch, name, steps, warmup=2):
     gp = torch.cat([x.grad.reshape(-1) for x in pp])
     rel = float((gb - gp).abs().max() / gp.abs().max().clamp_min(1e-30))
     return dict(workload=name, views=V, **meta, num_intersects=isect,
-                tuning={"block_width": tuning.block_width, "fwd_depth": tuning.fwd_depth,
+                tuning={"block_widths_allowed": list(BLOCK_WIDTHS[name]),
+                        "block_width": tuning.block_width, "fwd_depth": tuning.fwd_depth,
                         "bwd_depth": tuning.bwd_depth,
                         "candidates_ms": {f"bw{k[0]}_{k[1]}{k[2]}": round(v, 3)
                                           for k, v in tuning.table.items()}},
@@ -422,33 +701,50 @@ def batched_vs_plain(torch, name, steps, warmup=2):
                 speedup=round(ms_p / ms_b, 3), grad_max_rel_diff=rel)
 
 
-# the rasterize backward of the render path: its record kernel and per-Gaussian
-# reduction (the bitmap fill is a 1 MB memset at 1080p)
-STAGED_BWD = "raster_bwd_kernel<*, *, *, true, *> + raster_bwd_reduce_kernel"
+# ---------------------------------------------------------------------------
+# CPU baseline: the float64 oracle on a process pool
+# ---------------------------------------------------------------------------
+_CPU = {}
 
 
-def measured_traffic(workload):
-    """HBM bytes per launch of the dominant kernel from the committed
-    rocprofv3 FETCH_SIZE/WRITE_SIZE runs (scaled by a copy-kernel
-    calibration), or None when no measurement for this workload exists."""
-    p = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
-                     f"{workload}_traffic.json")
+def _cpu_worker_init():
     try:
-        with open(p) as fh:
-            rec = json.load(fh)
-        return rec["bytes_per_launch"] if rec.get("kernel") == STAGED_BWD else None
-    except (OSError, ValueError, KeyError):
-        return None
+        from threadpoolctl import threadpool_limits
+        threadpool_limits(1)
+    except ImportError:
+        pass
 
 
-def cpu_baseline(sc, cam, tgt_seed, n_tiles):
-    """Oracle (float64 NumPy, 1 core) on a bounded sample of the same workload."""
+def _cpu_tiles(tiles):
+    """Rasterize forward + backward of the given tiles (one worker)."""
+    import oracle
+    c = _CPU
+    t0 = time.perf_counter()
+    f = oracle.rasterize.rasterize_fwd(c["gs"], c["bins"], c["xys"], c["conics"], c["colors"],
+                                       c["opac"], c["bg"], img_shape=c["img_shape"],
+                                       block_width=c["bw"], tiles=tiles)
+    v_img = 2.0 * (f["img"] - c["target"]) / c["target"].size
+    oracle.rasterize.rasterize_bwd(c["gs"], c["bins"], c["xys"], c["conics"], c["colors"],
+                                   c["opac"], c["bg"], f["final_Ts"], f["final_idx"], v_img,
+                                   np.zeros(c["img_shape"]), img_shape=c["img_shape"],
+                                   block_width=c["bw"], tiles=tiles)
+    return time.perf_counter() - t0
+
+
+def cpu_baseline(sc, cam, tgt_seed, args):
+    """The float64 NumPy oracle (``kind: port``) on a bounded sample of the
+    same workload: projection and binning of all N Gaussians and the
+    projection backward of all N (one process each, prorated to the sampled
+    tiles' share of the frame and spread over the pool's cores), plus the
+    rasterize forward + backward of seeded random non-empty tiles, split
+    over a pool of forked worker processes (BLAS threads pinned to 1)."""
+    import multiprocessing as mp
     import oracle
     from jaxsplat_amd import scenes
-    Himg, Wimg = cam.img_shape
     bw = sc.block_width
     kw = dict(img_shape=cam.img_shape, f=cam.f, c=cam.c, glob_scale=sc.glob_scale,
               clip_thresh=sc.clip_thresh, block_width=bw)
+    workers = args.cpu_workers or min(16, os.cpu_count() or 1)
     t0 = time.perf_counter()
     pr = oracle.project.project_fwd(sc.means3d, sc.scales, sc.quats, cam.viewmat, **kw)
     ks, gs, bins = oracle.binning.bin_and_sort(pr["xys"].astype(np.float32),
@@ -456,37 +752,40 @@ def cpu_baseline(sc, cam, tgt_seed, n_tiles):
                                                pr["cum_tiles_hit"], img_shape=cam.img_shape,
                                                block_width=bw)
     t_pre = time.perf_counter() - t0
-    T = bins.shape[0]
     rng = np.random.default_rng(0)
-    nonempty = np.nonzero(bins[:, 1] > bins[:, 0])[0]
-    tiles = np.sort(rng.choice(nonempty, size=min(n_tiles, nonempty.size), replace=False))
-    target = scenes.target_image(cam.img_shape, tgt_seed)
+    vxy = rng.normal(size=(sc.n, 2)) * 1e-6
+    vcn = rng.normal(size=(sc.n, 3)) * 1e-6
     t1 = time.perf_counter()
-    f = oracle.rasterize.rasterize_fwd(gs, bins, pr["xys"], pr["conics"], sc.colors,
-                                       sc.opacities, sc.background, img_shape=cam.img_shape,
-                                       block_width=bw, tiles=tiles)
-    v_img = 2.0 * (f["img"] - target) / target.size
-    b = oracle.rasterize.rasterize_bwd(gs, bins, pr["xys"], pr["conics"], sc.colors, sc.opacities,
-                                       sc.background, f["final_Ts"], f["final_idx"], v_img,
-                                       np.zeros(cam.img_shape), img_shape=cam.img_shape,
-                                       block_width=bw, tiles=tiles)
-    vproj = oracle.project.project_bwd(sc.means3d, sc.scales, sc.quats, cam.viewmat, pr["radii"],
-                                       pr["conics"], pr["compensation"], b["v_xy"],
-                                       np.zeros(sc.n), b["v_conic"], np.zeros(sc.n),
-                                       img_shape=cam.img_shape, f=cam.f, glob_scale=sc.glob_scale)
-    t_tiles = time.perf_counter() - t1
-    px = tiles.size * bw * bw
-    t_eff = t_tiles + t_pre * tiles.size / T
-    return {"value": round(px / t_eff / 1e6, 5), "unit": "Mpix/s", "cores": 1, "kind": "port",
-            "sample": (f"{args_workload(sc)}: float64 oracle; projection + binning of all "
-                       f"{sc.n} Gaussians ({t_pre:.1f} s, prorated {tiles.size}/{T} tiles) + "
-                       f"rasterize fwd+bwd of {tiles.size} seeded random non-empty tiles "
-                       f"({t_tiles:.1f} s) + projection bwd"),
+    oracle.project.project_bwd(sc.means3d, sc.scales, sc.quats, cam.viewmat, pr["radii"],
+                               pr["conics"], pr["compensation"], vxy, np.zeros(sc.n), vcn,
+                               np.zeros(sc.n), img_shape=cam.img_shape, f=cam.f,
+                               glob_scale=sc.glob_scale)
+    t_pbwd = time.perf_counter() - t1
+    T = bins.shape[0]
+    nonempty = np.nonzero(bins[:, 1] > bins[:, 0])[0]
+    n_tiles = min(workers * args.cpu_tiles_per_worker, nonempty.size)
+    tiles = np.sort(rng.choice(nonempty, size=n_tiles, replace=False))
+    _CPU.update(gs=gs, bins=bins, xys=pr["xys"], conics=pr["conics"], colors=sc.colors,
+                opac=sc.opacities, bg=sc.background, img_shape=cam.img_shape, bw=bw,
+                target=scenes.target_image(cam.img_shape, tgt_seed))
+    chunks = [c for c in np.array_split(tiles, workers) if c.size]
+    t2 = time.perf_counter()
+    with mp.get_context("fork").Pool(len(chunks), initializer=_cpu_worker_init) as pool:
+        per = pool.map(_cpu_tiles, chunks)
+    t_tiles = time.perf_counter() - t2
+    _CPU.clear()
+    px = n_tiles * bw * bw
+    # the serial stages' share of the sampled tiles, spread over the cores
+    t_eff = t_tiles + (t_pre + t_pbwd) * n_tiles / T / workers
+    return {"value": round(px / t_eff / 1e6, 5), "unit": "Mpix/s", "cores": workers,
+            "kind": "port",
+            "sample": (f"{sc.name}: float64 oracle; rasterize fwd+bwd of {n_tiles} seeded random "
+                       f"non-empty tiles of {T} on {len(chunks)} worker processes "
+                       f"({t_tiles:.1f} s wall, {sum(per):.1f} s CPU); projection + binning "
+                       f"({t_pre:.1f} s) and projection bwd ({t_pbwd:.1f} s) of all {sc.n} "
+                       f"Gaussians on one core, prorated to {n_tiles}/{T} tiles and divided "
+                       f"over the {workers} cores"),
             "host_cpu": cpu_model(), "host_logical_cpus": os.cpu_count()}
-
-
-def args_workload(sc):
-    return sc.name
 
 
 def cpu_model():

@@ -68,6 +68,18 @@ const char* jaxsplat_version(void);
 const char* jaxsplat_last_error(void);
 
 /*
+ * Profiling marks (no counterpart in the reference, which has no tracing).
+ * events: up to 3 hipEvent_t (NULL entries allowed) that later calls from
+ * this thread record, on their stream, at internal stage boundaries:
+ * [0] in jaxsplat_bin_emit_sort / jaxsplat_bin_and_sort after the emission,
+ * [1] there after the tile sort (before the bin edges), [2] in the staged
+ * rasterize backward after the per-intersection records (before the
+ * per-Gaussian reduction).  events == NULL (or count 0) clears them.  The
+ * caller keeps the events alive while they are set.
+ */
+void jaxsplat_set_stage_marks(void* const* events, int count);
+
+/*
  * Fill *out; replaces make_descriptor (lib/ffi.cu:15-52).  img_h/img_w are the
  * reference's img_shape (H, W).  Rejects block_width outside [1, 16] (the
  * reference's LDS arrays hold 256 entries, lib/common.h:9) and a negative
@@ -162,6 +174,17 @@ int jaxsplat_bin_and_sort(const jaxsplat_descriptor* d, const float* xys, const 
  * num_points only.  jaxsplat_bin_emit_sort then emits, sorts and bins from
  * that workspace, with the outputs of jaxsplat_bin_and_sort; its workspace
  * (num_intersects dependent) is the one jaxsplat_bin_check reads.
+ *
+ * In both binning calls num_intersects is a capacity: the length of
+ * gaussian_ids_sorted / tile_ids_sorted and what the workspace is sized for.
+ * The kernels sort and bin the intersections actually emitted,
+ * min(total, num_intersects), counted on the device, so a caller may pass a
+ * capacity instead of the exact cum_tiles_hit[N-1] and skip reading the count
+ * back (no host synchronisation).  A total above the capacity is an overflow:
+ * only the first num_intersects in emission order are binned, and bit 8 of
+ * the error word is set.  After the call (in stream order) the workspace
+ * holds the error word in bytes [0, 4) and the exact total, a uint64, in
+ * bytes [8, 16), for callers that check them on the device or later.
  */
 size_t jaxsplat_bin_depth_workspace_size(const jaxsplat_descriptor* d);
 int jaxsplat_bin_depth_sort(const jaxsplat_descriptor* d, const float* xys, const float* depths,

@@ -21,15 +21,16 @@ ambiguous in float32.
 from __future__ import annotations
 
 from . import _ops
-from ._ops import ShapeDtypeError, bin_and_sort, project, rasterize
+from ._ops import IsectCapacity, ShapeDtypeError, bin_and_sort, project, rasterize
 from .batch import BatchPlan, BatchRenderer, ViewSpec, render_batch
 
-__all__ = ["render", "project", "rasterize", "bin_and_sort", "ShapeDtypeError", "BatchPlan",
-           "BatchRenderer", "ViewSpec", "render_batch"]
+__all__ = ["render", "project", "rasterize", "bin_and_sort", "ShapeDtypeError", "IsectCapacity",
+           "BatchPlan", "BatchRenderer", "ViewSpec", "render_batch"]
 
 
 def render(means3d, scales, quats, colors, opacities, *, viewmat, background, img_shape, f, c,
-           glob_scale, clip_thresh, block_size=16, adjoint="reference", antialiased=False):
+           glob_scale, clip_thresh, block_size=16, adjoint="reference", antialiased=False,
+           isect_capacity=None):
     """Render 3D Gaussians to an (H, W, 3) image, differentiably.
 
     Same contract as the reference ``render`` (``jaxsplat/__init__.py:6-70``):
@@ -44,6 +45,12 @@ def render(means3d, scales, quats, colors, opacities, *, viewmat, background, im
     ``opacities * compensation``, the compensation that the reference's
     projection computes and drops (``forward.cu:537-543``); gradients flow
     through both factors.
+
+    ``isect_capacity`` (an ``IsectCapacity``, not in the reference API): no
+    host synchronisation -- the binning uses buffers of that fixed capacity
+    and counts on the device, so a whole fwd+bwd step is kernel launches and
+    may be captured in a HIP graph; ``isect_capacity.check()`` reports an
+    overflow afterwards.
     """
     if _ops.TIGHT_TILES:
         # one projection pass also yields the opacity-limited tile rectangles
@@ -56,7 +63,7 @@ def render(means3d, scales, quats, colors, opacities, *, viewmat, background, im
             opacities = opacities * comp
         return rasterize(colors, opacities, background, xys, depths, radii, conics, cum,
                          img_shape=img_shape, block_width=block_size, adjoint=adjoint,
-                         tile_rects=rects)
+                         tile_rects=rects, isect_capacity=isect_capacity)
     out = project(means3d, scales, quats, viewmat, img_shape=img_shape, f=f, c=c,
                   glob_scale=glob_scale, clip_thresh=clip_thresh, block_width=block_size,
                   adjoint=adjoint, return_compensation=antialiased)
@@ -64,4 +71,5 @@ def render(means3d, scales, quats, colors, opacities, *, viewmat, background, im
     if antialiased:
         opacities = opacities * out[6]
     return rasterize(colors, opacities, background, xys, depths, radii, conics, cum_tiles_hit,
-                     img_shape=img_shape, block_width=block_size, adjoint=adjoint)
+                     img_shape=img_shape, block_width=block_size, adjoint=adjoint,
+                     isect_capacity=isect_capacity)

@@ -21,7 +21,8 @@ ADJOINT = {"reference": 0, "consistent": 1}
 
 # Every extern "C" symbol declared in include/jaxsplat_amd.h.
 EXPORTS = (
-    "jaxsplat_version", "jaxsplat_last_error", "jaxsplat_make_descriptor",
+    "jaxsplat_version", "jaxsplat_last_error", "jaxsplat_set_stage_marks",
+    "jaxsplat_make_descriptor",
     "jaxsplat_project_fwd", "jaxsplat_project_bwd", "jaxsplat_rasterize_fwd",
     "jaxsplat_rasterize_bwd", "jaxsplat_project_workspace_size",
     "jaxsplat_project_forward", "jaxsplat_project_backward",
@@ -98,6 +99,7 @@ DP = ctypes.POINTER(Descriptor)
 _SIGS = {
     "jaxsplat_version": (ctypes.c_char_p, []),
     "jaxsplat_last_error": (ctypes.c_char_p, []),
+    "jaxsplat_set_stage_marks": (None, [ctypes.POINTER(P), I32]),
     "jaxsplat_make_descriptor": (I32, [DP, U32, U32, U32, F32, F32, F32, F32, F32, F32, U32]),
     "jaxsplat_project_fwd": (None, [P, ctypes.POINTER(P), ctypes.c_char_p, SZ]),
     "jaxsplat_project_bwd": (None, [P, ctypes.POINTER(P), ctypes.c_char_p, SZ]),

@@ -42,6 +42,8 @@ from . import _lib
 
 STAGED_BACKWARD = os.environ.get("JAXSPLAT_AMD_STAGED_BACKWARD", "1") != "0"
 TIGHT_TILES = os.environ.get("JAXSPLAT_AMD_TIGHT_TILES", "1") != "0"
+# the staged backward's record slots, and the capacity of one binning, are 31-bit
+STAGED_MAX_INTERSECTIONS = (1 << 30) - 1
 
 
 class ShapeDtypeError(AssertionError, TypeError):
@@ -214,37 +216,117 @@ def _bin(d, xys, depths, radii, isect, dev, return_tiles=False):
     return out + (isect,)
 
 
-def _bin_overlapped(d, xys, depths, radii, cum_tiles_hit, dev):
-    """Binning with the host read of num_intersects = cum_tiles_hit[N-1] (which
-    sizes the outputs, as in the reference) overlapped with the depth stage:
-    the count is copied to pinned memory and the depth sort queued behind it
-    before the host waits.  Returns (gaussian_ids_sorted, tile_bins, isect)."""
+class IsectCapacity:
+    """A fixed intersection capacity for ``render`` / ``rasterize`` calls
+    that must not synchronise with the host (not in the reference API).
+
+    By default the binning reads the intersection count back to the host to
+    size its buffers, as the reference does (``lib/ops.cu:425-431``); that
+    host read keeps a fwd+bwd step out of a HIP graph.  With
+    ``isect_capacity=cap`` the buffers are sized for ``cap.capacity``
+    intersections instead and the kernels count on the device, so a step is
+    kernel launches only.  Each binning records on the device whether it
+    overflowed the capacity and its exact total; ``check()`` reads them (a
+    host synchronisation) and raises ``JaxsplatError`` (``ERR_CAPACITY``)
+    after an overflow -- whose images and gradients miss the intersections
+    beyond the capacity -- having grown ``capacity`` to ``headroom`` times the
+    largest total seen (re-run, and re-capture a graph, after that).
+
+    Results equal the default path bit for bit, except that a render with no
+    intersections at all keeps final_Ts = 1 internally (the reference writes 0
+    there); its image (the background) and gradients (zero) are the same.
+    """
+
+    def __init__(self, capacity, device=None, headroom=1.25):
+        capacity = int(capacity)
+        if not 1 <= capacity <= STAGED_MAX_INTERSECTIONS:
+            raise ValueError(f"capacity must be in [1, 2^30 - 1], got {capacity}")
+        self.capacity = capacity
+        self.headroom = float(headroom)
+        self.device = torch.device(device if device is not None else "cuda")
+        self._err = torch.zeros(1, dtype=torch.int32, device=self.device)
+        self._total = torch.zeros(1, dtype=torch.int64, device=self.device)
+        self.last_total = 0
+
+    def _record(self, ws):
+        # emission workspace header: error word at byte 0, exact total (uint64) at byte 8
+        torch.bitwise_or(self._err, ws[0:4].view(torch.int32), out=self._err)
+        torch.maximum(self._total, ws[8:16].view(torch.int64), out=self._total)
+
+    def check(self):
+        """Largest intersection total since the last check (host sync);
+        raises ``JaxsplatError`` after an overflow or a binning error."""
+        err, total = int(self._err.item()), int(self._total.item())
+        self._err.zero_()
+        self._total.zero_()
+        self.last_total = total
+        if err & 8:
+            old = self.capacity
+            self.capacity = min(max(int(total * self.headroom), total + 1),
+                                STAGED_MAX_INTERSECTIONS)
+            raise _lib.JaxsplatError(
+                _lib.ERR_CAPACITY, f"{total} intersections exceeded the capacity {old}; "
+                f"capacity grown to {self.capacity}")
+        if err:
+            raise _lib.JaxsplatError(_lib.ERR_CAPACITY, f"binning error (code {err})")
+        return total
+
+
+def _emit(d, dws, isect, dev, capacity):
+    """Emission, tile sort and bin edges over a depth-stage workspace, for
+    isect intersections (the exact count, or capacity.capacity).  Returns
+    (gaussian_ids_sorted, tile_bins, isect)."""
+    L = _lib.lib()
+    ntiles = d.grid_dim_2d.x * d.grid_dim_2d.y
+    gids = torch.empty(max(isect, 1), dtype=torch.int32, device=dev)
+    bins = torch.empty((ntiles, 2), dtype=torch.int32, device=dev)
+    ws_bytes = L.jaxsplat_bin_emit_workspace_size(ctypes.byref(d), isect)
+    ws = torch.empty(max(ws_bytes, 16), dtype=torch.uint8, device=dev)
+    _lib.check(L.jaxsplat_bin_emit_sort(ctypes.byref(d), isect, _p(dws), _p(gids), None,
+                                        _p(bins), _p(ws), ws_bytes, _stream(dev)))
+    if capacity is not None:
+        capacity._record(ws)
+    return gids[:isect], bins, isect
+
+
+def _read_count(cum, dev):
+    """Queue a copy of the last inclusive count to pinned memory; returns
+    (host tensor, event) -- the caller waits on the event after queueing the
+    depth stage, so that the read overlaps it."""
+    count = torch.empty(1, dtype=torch.int32, pin_memory=True)
+    count.copy_(cum.reshape(-1)[-1:], non_blocking=True)
+    ready = torch.cuda.Event()
+    ready.record()
+    return count, ready
+
+
+def _bin_overlapped(d, xys, depths, radii, cum_tiles_hit, dev, capacity=None):
+    """Binning into the reference's tile rectangles.  By default the host read
+    of num_intersects = cum_tiles_hit[N-1] (which sizes the outputs, as in the
+    reference) overlaps the depth stage: the count is copied to pinned memory
+    and the depth sort queued behind it before the host waits.  With a
+    capacity (IsectCapacity) nothing is read back.  Returns
+    (gaussian_ids_sorted, tile_bins, isect)."""
     N = xys.shape[0]
     if N == 0:
         return _bin(d, xys, depths, radii, 0, dev)
     L = _lib.lib()
     with torch.cuda.device(dev):
-        count = torch.empty(1, dtype=torch.int32, pin_memory=True)
-        count.copy_(cum_tiles_hit.reshape(-1)[-1:], non_blocking=True)
-        ready = torch.cuda.Event()
-        ready.record()
+        if capacity is None:
+            count, ready = _read_count(cum_tiles_hit, dev)
         dws_bytes = L.jaxsplat_bin_depth_workspace_size(ctypes.byref(d))
         dws = torch.empty(max(dws_bytes, 1), dtype=torch.uint8, device=dev)
         _lib.check(L.jaxsplat_bin_depth_sort(ctypes.byref(d), _p(xys), _p(depths), _p(radii),
                                              _p(dws), dws_bytes, _stream(dev)))
-        ready.synchronize()
-        isect = int(count[0])
-        ntiles = d.grid_dim_2d.x * d.grid_dim_2d.y
-        gids = torch.empty(max(isect, 1), dtype=torch.int32, device=dev)
-        bins = torch.empty((ntiles, 2), dtype=torch.int32, device=dev)
-        ws_bytes = L.jaxsplat_bin_emit_workspace_size(ctypes.byref(d), isect)
-        ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=dev)
-        _lib.check(L.jaxsplat_bin_emit_sort(ctypes.byref(d), isect, _p(dws), _p(gids), None,
-                                            _p(bins), _p(ws), ws_bytes, _stream(dev)))
-    return gids[:isect], bins, isect
+        if capacity is None:
+            ready.synchronize()
+            isect = int(count[0])
+        else:
+            isect = capacity.capacity
+        return _emit(d, dws, isect, dev, capacity)
 
 
-def _bin_tight(d, xys, depths, radii, conics, opacities, dev):
+def _bin_tight(d, xys, depths, radii, conics, opacities, dev, capacity=None):
     """Binning into the opacity-limited tile rectangles: the rectangles and
     their inclusive counts (jaxsplat_tile_rects), then _bin_rects.  Returns
     (gaussian_ids_sorted, tile_bins, isect, rects, cum)."""
@@ -261,39 +343,29 @@ def _bin_tight(d, xys, depths, radii, conics, opacities, dev):
         _lib.check(L.jaxsplat_tile_rects(ctypes.byref(d), _p(xys), _p(radii), _p(conics),
                                          _p(opacities), _p(rects), None, _p(cum), _p(rws),
                                          rws_bytes, _stream(dev)))
-    gids, bins, isect = _bin_rects(d, depths, radii, rects, cum, dev)
+    gids, bins, isect = _bin_rects(d, depths, radii, rects, cum, dev, capacity)
     return gids, bins, isect, rects, cum
 
 
-def _bin_rects(d, depths, radii, rects, cum, dev):
+def _bin_rects(d, depths, radii, rects, cum, dev, capacity=None):
     """Binning into given tile rectangles (with their inclusive counts cum):
-    the depth stage on those rectangles, the host read of the total overlapped
-    with it, then emission.  Returns (gaussian_ids_sorted, tile_bins, isect)."""
-    N = depths.shape[0]
+    the depth stage on those rectangles, then emission -- with the host read
+    of the total overlapped with the depth stage, or (with a capacity) no
+    read.  Returns (gaussian_ids_sorted, tile_bins, isect)."""
     L = _lib.lib()
     with torch.cuda.device(dev):
-        count = torch.empty(1, dtype=torch.int32, pin_memory=True)
-        count.copy_(cum.reshape(-1)[N - 1:N], non_blocking=True)
-        ready = torch.cuda.Event()
-        ready.record()
+        if capacity is None:
+            count, ready = _read_count(cum, dev)
         dws_bytes = L.jaxsplat_bin_depth_workspace_size(ctypes.byref(d))
         dws = torch.empty(max(dws_bytes, 1), dtype=torch.uint8, device=dev)
         _lib.check(L.jaxsplat_bin_depth_sort_rects(ctypes.byref(d), _p(depths), _p(radii),
                                                    _p(rects), _p(dws), dws_bytes, _stream(dev)))
-        ready.synchronize()
-        isect = int(count[0])
-        ntiles = d.grid_dim_2d.x * d.grid_dim_2d.y
-        gids = torch.empty(max(isect, 1), dtype=torch.int32, device=dev)
-        bins = torch.empty((ntiles, 2), dtype=torch.int32, device=dev)
-        ws_bytes = L.jaxsplat_bin_emit_workspace_size(ctypes.byref(d), isect)
-        ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=dev)
-        _lib.check(L.jaxsplat_bin_emit_sort(ctypes.byref(d), isect, _p(dws), _p(gids), None,
-                                            _p(bins), _p(ws), ws_bytes, _stream(dev)))
-    return gids[:isect], bins, isect
-
-
-STAGED_MAX_INTERSECTIONS = (1 << 30) - 1
-
+        if capacity is None:
+            ready.synchronize()
+            isect = int(count[0])
+        else:
+            isect = capacity.capacity
+        return _emit(d, dws, isect, dev, capacity)
 
 def staged_backward_applies(nch, isect):
     """Whether the rasterize backward takes the staged entry point: up to
@@ -306,7 +378,7 @@ def staged_backward_applies(nch, isect):
 class _Rasterize(torch.autograd.Function):
     @staticmethod
     def forward(ctx, cfg, colors, opacities, background, xys, depths, radii, conics,
-                cum_tiles_hit, tile_rects=None):
+                cum_tiles_hit, tile_rects=None, capacity=None):
         img_shape, block_width, adjoint = cfg
         N, nch = colors.shape
         dev = colors.device
@@ -319,12 +391,14 @@ class _Rasterize(torch.autograd.Function):
             if N == 0:
                 gids, bins, isect = _bin(d, xys, depths, radii, 0, dev)
             else:
-                gids, bins, isect = _bin_rects(d, depths, radii, rects, cum_tiles_hit, dev)
+                gids, bins, isect = _bin_rects(d, depths, radii, rects, cum_tiles_hit, dev,
+                                               capacity)
         elif TIGHT_TILES:
             gids, bins, isect, rects, cum_tiles_hit = _bin_tight(d, xys, depths, radii, conics,
-                                                                 opacities, dev)
+                                                                 opacities, dev, capacity)
         else:
-            gids, bins, isect = _bin_overlapped(d, xys, depths, radii, cum_tiles_hit, dev)
+            gids, bins, isect = _bin_overlapped(d, xys, depths, radii, cum_tiles_hit, dev,
+                                                capacity)
         final_Ts = torch.empty((H, W, 1), dtype=torch.float32, device=dev)
         final_idx = torch.empty((H, W, 1), dtype=torch.int32, device=dev)
         img = torch.empty((H, W, nch), dtype=torch.float32, device=dev)
@@ -381,18 +455,19 @@ class _Rasterize(torch.autograd.Function):
                         _p(cum_tiles_hit), _p(final_Ts), _p(final_idx), _p(v_img), None,
                         _p(v_colors), _p(v_opacity), _p(v_xy), None, _p(v_conic), ctx.adjoint,
                         _p(ws), wsb, _stream(dev)))
-            return None, v_colors, v_opacity, None, v_xy, None, None, v_conic, None, None
+            return None, v_colors, v_opacity, None, v_xy, None, None, v_conic, None, None, None
         with torch.cuda.device(dev):
             _lib.check(L.jaxsplat_rasterize_backward_nd(
                 ctypes.byref(ctx.d), _p(gids), _p(bins), ctx.isect, _p(xys), _p(conics),
                 _p(colors), _p(opacities), _p(background), colors.shape[1], _p(final_Ts),
                 _p(final_idx), _p(v_img), None, _p(v_colors), _p(v_opacity), _p(v_xy), None,
                 _p(v_conic), ctx.adjoint, _stream(dev)))
-        return None, v_colors, v_opacity, None, v_xy, None, None, v_conic, None, None
+        return None, v_colors, v_opacity, None, v_xy, None, None, v_conic, None, None, None
 
 
 def rasterize(colors, opacities, background, xys, depths, radii, conics, cum_tiles_hit, *,
-              img_shape, block_width, adjoint="reference", tile_rects=None):
+              img_shape, block_width, adjoint="reference", tile_rects=None,
+              isect_capacity=None):
     """Rasterize projected Gaussians (reference ``_rasterize/__init__.py:40-68``).
 
     Returns the (H, W, C) image for colors (N, C) and background (C,).  The
@@ -403,6 +478,11 @@ def rasterize(colors, opacities, background, xys, depths, radii, conics, cum_til
     ``tile_rects`` (N, 2) int32 (not in the reference API): the
     opacity-limited tile rectangles of ``project_for_render``, whose counts
     ``cum_tiles_hit`` then are.
+
+    ``isect_capacity`` (an ``IsectCapacity``, not in the reference API): bin
+    into buffers of that fixed capacity, counting on the device, instead of
+    reading the intersection count back to the host -- no host
+    synchronisation, so the call may be captured in a HIP graph.
     """
     adj = _adjoint(adjoint)
     N = colors.shape[0] if isinstance(colors, torch.Tensor) else -1
@@ -420,8 +500,10 @@ def rasterize(colors, opacities, background, xys, depths, radii, conics, cum_til
     if tile_rects is not None:
         _expect("tile_rects", tile_rects, (N, 2), torch.int32)
         tile_rects = tile_rects.contiguous()
+    if isect_capacity is not None and not isinstance(isect_capacity, IsectCapacity):
+        raise TypeError("isect_capacity must be an IsectCapacity")
     cfg = (tuple(img_shape), int(block_width), adj)
     return _Rasterize.apply(cfg, colors.contiguous(), opacities.contiguous(),
                             background.contiguous(), xys.contiguous(), depths.contiguous(),
                             radii.contiguous(), conics.contiguous(), cum_tiles_hit.contiguous(),
-                            tile_rects)
+                            tile_rects, isect_capacity)

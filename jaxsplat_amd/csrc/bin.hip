@@ -481,8 +481,10 @@ DepthLayout depth_layout(const BinPlan& p) {
 }
 
 // Emission stage (emission, tile sort, bin edges); the error word comes first.
+// The zeroed header is part of the C ABI (jaxsplat_bin_emit_sort): the error
+// word at byte 0 and the exact intersection total (uint64) at byte 8.
 struct BinLayout {
-    size_t err, zero_end;
+    size_t err, total, zero_end;
     size_t tile_sort, tiles, gids, ttmp_k, ttmp_v, tiles_sorted, end;
     EmitLayout emit;
 };
@@ -490,7 +492,9 @@ struct BinLayout {
 BinLayout layout(const BinPlan& p) {
     BinLayout L;
     size_t o = 0;
-    L.err = o; o += 16;
+    L.err = o;
+    L.total = o + 8;
+    o += 16;
     L.zero_end = o;
     o = align256(o);
     L.tile_sort = o; o = align256(o + p.tile_sort.scratch_bytes);
@@ -564,13 +568,16 @@ hipError_t bin_emit(const BinPlan& p, const Camera& c, int num_isect_host,
     // an overflow recorded in the error word
     int* written = reinterpret_cast<int*>(b + L.emit.info);
     e = emit<false>(p.n, c.tiles_x, nullptr, nullptr, perm, rect_by_id, p.isect_cap, b, L.emit,
-                    written, nullptr, err, U(L.tiles), U(L.gids), nullptr, s);
+                    written, reinterpret_cast<unsigned long long*>(b + L.total), err,
+                    U(L.tiles), U(L.gids), nullptr, s);
     if (e != hipSuccess) return e;
+    stage_mark(kMarkEmitted, s);
     uint32_t* tiles_out = tiles_sorted ? reinterpret_cast<uint32_t*>(tiles_sorted) : U(L.tiles_sorted);
     e = sort_pairs(p.tile_sort, U(L.tiles), U(L.gids), tiles_out,
                    reinterpret_cast<uint32_t*>(gids_sorted), U(L.ttmp_k), U(L.ttmp_v),
                    num_isect_host, written, b + L.tile_sort, err, s);
     if (e != hipSuccess) return e;
+    stage_mark(kMarkTileSorted, s);
     const int nb = (int)(((long long)p.isect_cap + 256 * kTileBinsPer - 1) / (256 * kTileBinsPer));
     tile_bins_kernel<<<nb > 0 ? nb : 1, 256, 0, s>>>(tiles_out, num_isect_host, written,
                                                    p.num_tiles,

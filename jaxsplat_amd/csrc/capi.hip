@@ -29,6 +29,25 @@ namespace {
 
 thread_local std::string g_err;
 
+}  // namespace
+
+namespace jsa {
+// The thread's jaxsplat_last_error() message, for entry points in other files.
+void set_last_error(const std::string& msg) { g_err = msg; }
+
+// Stage marks (jaxsplat_set_stage_marks): events of this thread recorded at
+// internal stage boundaries.
+namespace {
+thread_local hipEvent_t g_marks[kNumStageMarks] = {};
+}  // namespace
+
+void stage_mark(int k, hipStream_t s) {
+    if (k >= 0 && k < kNumStageMarks && g_marks[k]) (void)hipEventRecord(g_marks[k], s);
+}
+}  // namespace jsa
+
+namespace {
+
 int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
 int fail(int code, const char* fmt, ...) {
     char buf[512];
@@ -149,6 +168,11 @@ extern "C" {
 const char* jaxsplat_version(void) { return "jaxsplat_amd 0.1.0 (gfx950)"; }
 
 const char* jaxsplat_last_error(void) { return g_err.c_str(); }
+
+void jaxsplat_set_stage_marks(void* const* events, int count) {
+    for (int k = 0; k < jsa::kNumStageMarks; ++k)
+        jsa::g_marks[k] = (events && k < count) ? static_cast<hipEvent_t>(events[k]) : nullptr;
+}
 
 int jaxsplat_make_descriptor(jaxsplat_descriptor* out, uint32_t num_points, uint32_t img_h,
                              uint32_t img_w, float fx, float fy, float cx, float cy,

@@ -246,6 +246,16 @@ struct RasterGrid {
 ViewRec single_view(const Camera& c);
 RasterGrid single_grid(const Camera& c);
 
+// Profiling marks (jaxsplat_set_stage_marks): stage_mark(k, s) records the
+// calling thread's event k, if one is set, on stream s.
+enum StageMark : int {
+    kMarkEmitted = 0,       // bin_emit: emission done, tile sort next
+    kMarkTileSorted = 1,    // bin_emit: tile sort done, bin edges next
+    kMarkRecords = 2,       // staged rasterize backward: records done, reduction next
+    kNumStageMarks = 3,
+};
+void stage_mark(int k, hipStream_t s);
+
 // Zeroes exactly `bytes` bytes at p (4-byte aligned) with a kernel (no memset
 // node when captured in a graph).
 hipError_t fill_zero(void* p, size_t bytes, hipStream_t s);

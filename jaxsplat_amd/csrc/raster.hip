@@ -1162,6 +1162,7 @@ hipError_t raster_bwd_staged(const Camera& c, int nch, const int* gids_sorted,
 #undef JSA_LAUNCH_S
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
+    stage_mark(kMarkRecords, s);
     const int blocks = (c.num_points + 255) / 256;
     const float2* x2 = reinterpret_cast<const float2*>(xys);
     if (v_xy_abs)
