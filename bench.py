@@ -692,7 +692,7 @@ def batched_vs_plain(torch, name, steps, warmup=2):
     return dict(workload=name, views=V, **meta, num_intersects=isect,
                 tuning={"block_widths_allowed": list(BLOCK_WIDTHS[name]),
                         "block_width": tuning.block_width, "fwd_depth": tuning.fwd_depth,
-                        "bwd_depth": tuning.bwd_depth,
+                        "bwd_depth": tuning.bwd_depth, "schedule": tuning.schedule,
                         "candidates_ms": {f"bw{k[0]}_{k[1]}{k[2]}": round(v, 3)
                                           for k, v in tuning.table.items()}},
                 ms_batched_graph=round(ms_b, 3), ms_plain_per_view_calls=round(ms_p, 3),

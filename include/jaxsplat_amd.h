@@ -380,6 +380,19 @@ size_t jaxsplat_batch_workspace_size(const jaxsplat_batch* b);
  * rounded up to whole waves; backward: 64).  The results do not depend on
  * it; the autotuner of the Python layer picks the fastest. */
 int jaxsplat_batch_set_depths(jaxsplat_batch* b, int32_t fwd_depth, int32_t bwd_depth);
+
+/* How the batch rasterizer (forward and backward) assigns its (view, tile)
+ * items to workgroups: JAXSPLAT_SCHEDULE_GRID (default) launches one
+ * workgroup per tile; JAXSPLAT_SCHEDULE_QUEUE orders the tiles longest list
+ * first after each binning (one stable radix pass over the tiles) and runs a
+ * persistent grid that takes them from a device counter, which evens out
+ * batches of uneven scenes.  The results do not depend on it (every tile runs
+ * the same code on the same list; the backward sums in a fixed order). */
+typedef enum jaxsplat_schedule {
+    JAXSPLAT_SCHEDULE_GRID = 0,
+    JAXSPLAT_SCHEDULE_QUEUE = 1
+} jaxsplat_schedule;
+int jaxsplat_batch_set_schedule(jaxsplat_batch* b, int32_t schedule);
 int64_t jaxsplat_batch_num_pixels(const jaxsplat_batch* b);
 int64_t jaxsplat_batch_num_gaussians(const jaxsplat_batch* b);
 int64_t jaxsplat_batch_pixel_offset(const jaxsplat_batch* b, int32_t view);

@@ -18,6 +18,7 @@ ERR_HIP = 2
 ERR_CAPACITY = 3
 ERR_WORKSPACE = 4
 ADJOINT = {"reference": 0, "consistent": 1}
+SCHEDULE = {"grid": 0, "queue": 1}
 
 # Every extern "C" symbol declared in include/jaxsplat_amd.h.
 EXPORTS = (
@@ -36,7 +37,7 @@ EXPORTS = (
     "jaxsplat_rasterize_forward_binned", "jaxsplat_rasterize_backward_binned",
     "jaxsplat_rasterize_forward_nd", "jaxsplat_rasterize_backward_nd",
     "jaxsplat_batch_create", "jaxsplat_batch_destroy", "jaxsplat_batch_workspace_size",
-    "jaxsplat_batch_set_depths",
+    "jaxsplat_batch_set_depths", "jaxsplat_batch_set_schedule",
     "jaxsplat_batch_num_pixels", "jaxsplat_batch_num_gaussians", "jaxsplat_batch_pixel_offset",
     "jaxsplat_batch_forward", "jaxsplat_batch_backward", "jaxsplat_batch_check",
     "jaxsplat_batch_status_word", "jaxsplat_fit_activate", "jaxsplat_fit_l2",
@@ -135,6 +136,7 @@ _SIGS = {
     "jaxsplat_batch_destroy": (None, [P]),
     "jaxsplat_batch_workspace_size": (SZ, [P]),
     "jaxsplat_batch_set_depths": (I32, [P, I32, I32]),
+    "jaxsplat_batch_set_schedule": (I32, [P, I32]),
     "jaxsplat_batch_num_pixels": (ctypes.c_int64, [P]),
     "jaxsplat_batch_num_gaussians": (ctypes.c_int64, [P]),
     "jaxsplat_batch_pixel_offset": (ctypes.c_int64, [P, I32]),

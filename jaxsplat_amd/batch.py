@@ -93,6 +93,7 @@ class BatchPlan:
         self.workspace = torch.empty(self.workspace_bytes, dtype=torch.uint8, device=self.device)
         self.generation = 0          # bumped by every forward; the backward checks it
         self.depths = (0, 0)
+        self.schedule = "grid"
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -112,6 +113,19 @@ class BatchPlan:
         (64, 128 or 256; 0 = workgroup size).  Results do not depend on it."""
         _lib.check(_lib.lib().jaxsplat_batch_set_depths(self._h, int(fwd_depth), int(bwd_depth)))
         self.depths = (int(fwd_depth), int(bwd_depth))
+
+    def set_schedule(self, schedule="grid"):
+        """How the rasterizer hands (view, tile) items to workgroups: "grid"
+        (one workgroup per tile) or "queue" (tiles ordered longest list
+        first, taken by a persistent grid from a device counter).  Results do
+        not depend on it."""
+        try:
+            code = _lib.SCHEDULE[schedule]
+        except KeyError:
+            raise ValueError(f"schedule must be one of {sorted(_lib.SCHEDULE)}, "
+                             f"got {schedule!r}") from None
+        _lib.check(_lib.lib().jaxsplat_batch_set_schedule(self._h, code))
+        self.schedule = schedule
 
     def split(self, flat):
         """Per-view (H, W, C) views of a (num_pixels, C) buffer."""

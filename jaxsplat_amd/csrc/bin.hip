@@ -11,9 +11,12 @@
 //
 // This implementation produces exactly that order with far less sort traffic:
 //  1. depth_keys: one 32-bit key per Gaussian (depth bits, or 0xFFFFFFFF for
-//     Gaussians with radius 0, which emit nothing);
-//  2. a stable LSD radix sort of the N Gaussians by depth key (values = index),
-//     giving the (depth bits, gaussian id) order;
+//     Gaussians with an empty tile rectangle, which emit nothing);
+//  2. a stable LSD radix sort of the Gaussians by depth key (values = index),
+//     giving the (depth bits, gaussian id) order; its first pass drops the
+//     0xFFFFFFFF keys, so the later passes and the emission see only the
+//     Gaussians that emit (71% of them at 1080p with 1M Gaussians), counted
+//     on the device;
 //  3. emission in that order: a reduce-then-scan of the Gaussians' tile
 //     counts gives each Gaussian its first output slot, then every wave
 //     writes whole 64-slot chunks of (tile id, gaussian id) pairs (owner
@@ -49,20 +52,23 @@ __global__ void depth_keys_kernel(Camera c, const float* __restrict__ depths,
             rc = pack_rect(x0, x1, y0, y1);
         }
     }
-    keys[g] = rad > 0 ? __float_as_uint(depths[g]) : 0xFFFFFFFFu;
+    // Gaussians that emit nothing get the key the depth sort drops
+    keys[g] = rect_count(rc) > 0 ? __float_as_uint(depths[g]) : 0xFFFFFFFFu;
     rects[g] = rc;
 }
 
 // View of each depth-sorted item (largest v with item_off[v] <= item), the
 // key of the stable pass that groups a batch's items by view.
 constexpr int kMaxBatchViews = 4096;
-__global__ __launch_bounds__(256) void view_keys_kernel(const uint32_t* __restrict__ perm, int n,
+__global__ __launch_bounds__(256) void view_keys_kernel(const uint32_t* __restrict__ perm,
+                                                        const int* __restrict__ n_dev,
                                                         const ViewRec* __restrict__ views,
                                                         int num_views,
                                                         uint32_t* __restrict__ vkeys) {
     __shared__ int s_off[kMaxBatchViews];
     for (int v = threadIdx.x; v < num_views; v += blockDim.x) s_off[v] = views[v].item_off;
     __syncthreads();
+    const int n = *n_dev;
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n) return;
     const int it = (int)perm[r];
@@ -110,9 +116,11 @@ constexpr int kExpandWaves = 8192;           // waves of the expansion (a full c
 // emitting entries per partition.  The lane's loads are issued stage by stage
 // so that they overlap.
 __global__ __launch_bounds__(256) void emit_count_kernel(
-    int n, const uint32_t* __restrict__ perm, const uint2* __restrict__ rect_by_id,
-    uint2* __restrict__ rects, uint32_t* __restrict__ part_tot, uint32_t* __restrict__ part_nz) {
+    int n_host, const int* __restrict__ n_dev, const uint32_t* __restrict__ perm,
+    const uint2* __restrict__ rect_by_id, uint2* __restrict__ rects,
+    uint32_t* __restrict__ part_tot, uint32_t* __restrict__ part_nz) {
     __shared__ uint32_t s_w[4], s_z[4];
+    const int n = n_dev ? *n_dev : n_host;
     const int tid = threadIdx.x;
     const int lane = lane_id();
     const int r0 = blockIdx.x * kEmitPart + tid;
@@ -203,11 +211,12 @@ struct EmitEntries {
 
 template <bool BATCH>
 __global__ __launch_bounds__(256) void emit_compact_kernel(
-    int n, const ViewRec* __restrict__ views, const uint32_t* __restrict__ vkeys,
-    const uint32_t* __restrict__ perm, const uint2* __restrict__ rects,
-    const uint64_t* __restrict__ part_off, const uint32_t* __restrict__ part_nzoff,
-    const int* __restrict__ info, EmitEntries E) {
+    int n_host, const int* __restrict__ n_dev, const ViewRec* __restrict__ views,
+    const uint32_t* __restrict__ vkeys, const uint32_t* __restrict__ perm,
+    const uint2* __restrict__ rects, const uint64_t* __restrict__ part_off,
+    const uint32_t* __restrict__ part_nzoff, const int* __restrict__ info, EmitEntries E) {
     __shared__ uint32_t s_w[kEmitPer][4], s_z[kEmitPer][4];
+    const int n = n_dev ? *n_dev : n_host;
     const int tid = threadIdx.x;
     const int lane = lane_id();
     const int wave = tid >> 6;
@@ -417,14 +426,16 @@ size_t emit_layout(size_t o, int n, int cap, bool batch, EmitLayout& E) {
     return o;
 }
 
-// Passes 1-4 over the n = c.num_points entries perm[0..n) in emission order.
-// Writes the slots written (min(total, cap)) to info[0] and the number of
-// emitting entries to info[1] (info: the layout's slot if null), the exact
-// total to *total if total is not null, and bit 8 of err when the total
-// exceeds cap.  first_by_id (optional) receives the first slot of every
-// emitting id (clamped to the slots written).
+// Passes 1-4 over the entries perm[0..n) in emission order, n = *n_dev when
+// given (the depth sort's kept keys), else n_host; launch shapes follow
+// n_host, an upper bound.  Writes the slots written (min(total, cap)) to
+// info[0] and the number of emitting entries to info[1] (info: the layout's
+// slot if null), the exact total to *total if total is not null, and bit 8 of
+// err when the total exceeds cap.  first_by_id (optional) receives the first
+// slot of every emitting id (clamped to the slots written).
 template <bool BATCH>
-hipError_t emit(int n, int tiles_x_plain, const ViewRec* views, const uint32_t* vkeys,
+hipError_t emit(int n, const int* n_dev, int tiles_x_plain, const ViewRec* views,
+                const uint32_t* vkeys,
                 const uint32_t* perm, const uint2* rect_by_id, int cap, uint8_t* b,
                 const EmitLayout& L, int* info, unsigned long long* total, uint32_t* err,
                 uint32_t* tile_out, uint32_t* gid_out, uint32_t* first_by_id, hipStream_t s) {
@@ -443,14 +454,15 @@ hipError_t emit(int n, int tiles_x_plain, const ViewRec* views, const uint32_t* 
     E.key = BATCH ? reinterpret_cast<int2*>(P(L.key)) : nullptr;
     E.chunk_owner = reinterpret_cast<uint32_t*>(P(L.chunk_owner));
     E.first_by_id = first_by_id;
-    emit_count_kernel<<<parts, 256, 0, s>>>(n, perm, rect_by_id, rects, part_tot, part_nz);
+    emit_count_kernel<<<parts, 256, 0, s>>>(n, n_dev, perm, rect_by_id, rects, part_tot,
+                                            part_nz);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     emit_scan_kernel<<<1, 1024, 0, s>>>(part_tot, part_nz, parts, part_off, part_nzoff, cap,
                                         info, total, err);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    emit_compact_kernel<BATCH><<<parts, 256, 0, s>>>(n, views, vkeys, perm, rects, part_off,
-                                                     part_nzoff, info, E);
+    emit_compact_kernel<BATCH><<<parts, 256, 0, s>>>(n, n_dev, views, vkeys, perm, rects,
+                                                     part_off, part_nzoff, info, E);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     const long long chunks = ((long long)cap + 63) / 64;
     const int waves = (int)(chunks < kExpandWaves ? chunks : kExpandWaves);
@@ -462,12 +474,13 @@ hipError_t emit(int n, int tiles_x_plain, const ViewRec* views, const uint32_t* 
 // Depth stage: needs no intersection count, so a caller can run it while it
 // reads the count back.  Its workspace depends on the Gaussian count only.
 struct DepthLayout {
-    size_t depth_sort, dkeys, rect_by_id, dtmp_k, dtmp_v, dsorted_k, perm, end;
+    size_t n_kept, depth_sort, dkeys, rect_by_id, dtmp_k, dtmp_v, dsorted_k, perm, end;
 };
 
 DepthLayout depth_layout(const BinPlan& p) {
     DepthLayout L;
     size_t o = 0;
+    L.n_kept = o; o = align256(o + 16);     // Gaussians that emit (kept by the depth sort)
     L.depth_sort = o; o = align256(o + p.depth_sort.scratch_bytes);
     const size_t n4 = align256(4ull * (p.n > 0 ? p.n : 1));
     L.dkeys = o; o += n4;
@@ -542,8 +555,10 @@ hipError_t bin_depth(const BinPlan& p, const Camera& c, const float* xys, const 
         reinterpret_cast<uint2*>(b + D.rect_by_id));
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
+    // only the Gaussians that emit are sorted: the first pass drops the others
     return sort_pairs(p.depth_sort, U(D.dkeys), nullptr, U(D.dsorted_k), U(D.perm), U(D.dtmp_k),
-                      U(D.dtmp_v), p.n, nullptr, b + D.depth_sort, nullptr, s);
+                      U(D.dtmp_v), p.n, nullptr, b + D.depth_sort, nullptr, s,
+                      reinterpret_cast<int*>(b + D.n_kept));
 }
 
 hipError_t bin_emit(const BinPlan& p, const Camera& c, int num_isect_host,
@@ -567,9 +582,10 @@ hipError_t bin_emit(const BinPlan& p, const Camera& c, int num_isect_host,
     // cum_tiles_hit) then never sorts unwritten slots, and one too small is
     // an overflow recorded in the error word
     int* written = reinterpret_cast<int*>(b + L.emit.info);
-    e = emit<false>(p.n, c.tiles_x, nullptr, nullptr, perm, rect_by_id, p.isect_cap, b, L.emit,
-                    written, reinterpret_cast<unsigned long long*>(b + L.total), err,
-                    U(L.tiles), U(L.gids), nullptr, s);
+    e = emit<false>(p.n, reinterpret_cast<const int*>(d + D.n_kept), c.tiles_x, nullptr,
+                    nullptr, perm, rect_by_id, p.isect_cap, b, L.emit, written,
+                    reinterpret_cast<unsigned long long*>(b + L.total), err, U(L.tiles),
+                    U(L.gids), nullptr, s);
     if (e != hipSuccess) return e;
     stage_mark(kMarkEmitted, s);
     uint32_t* tiles_out = tiles_sorted ? reinterpret_cast<uint32_t*>(tiles_sorted) : U(L.tiles_sorted);
@@ -613,7 +629,7 @@ hipError_t bin_and_sort(const BinPlan& p, const Camera& c, const float* xys, con
 namespace {
 
 struct BatchLayout {
-    size_t err, count, total, zero_end;
+    size_t err, count, total, kept, zero_end;
     size_t depth_sort, view_sort, tile_sort, dsorted_k, perm, dtmp_k, dtmp_v, vkeys, vsorted,
         perm2, keys, vals, ttmp_k, ttmp_v, keys_sorted, end;
     EmitLayout emit;
@@ -625,6 +641,7 @@ BatchLayout batch_layout(const BatchBinPlan& p) {
     L.err = o; o += 16;
     L.count = o; o += 16;
     L.total = o; o += 16;
+    L.kept = o; o += 16;
     L.zero_end = o;
     o = align256(o);
     L.depth_sort = o; o = align256(o + p.depth_sort.scratch_bytes);
@@ -696,23 +713,26 @@ hipError_t batch_bin(const BatchBinPlan& p, const ViewRec* views, const uint2* i
     int* count = reinterpret_cast<int*>(b + L.count);
     auto U = [&](size_t off) { return reinterpret_cast<uint32_t*>(b + off); };
 
+    // only the items that emit are sorted: the first depth pass drops the
+    // others (key 0xFFFFFFFF) and counts the rest in `kept`
+    int* kept = reinterpret_cast<int*>(b + L.kept);
     e = sort_pairs(p.depth_sort, depth_keys, nullptr, U(L.dsorted_k), U(L.perm), U(L.dtmp_k),
-                   U(L.dtmp_v), p.n_items, nullptr, b + L.depth_sort, err, s);
+                   U(L.dtmp_v), p.n_items, nullptr, b + L.depth_sort, err, s, kept);
     if (e != hipSuccess) return e;
-    view_keys_kernel<<<(p.n_items + 255) / 256, 256, 0, s>>>(U(L.perm), p.n_items, views,
+    view_keys_kernel<<<(p.n_items + 255) / 256, 256, 0, s>>>(U(L.perm), kept, views,
                                                               p.num_views, U(L.vkeys));
     if ((e = hipGetLastError()) != hipSuccess) return e;
     const uint32_t* order = U(L.perm);
     const uint32_t* order_views = U(L.vkeys);
     if (p.view_bits > 0) {
         e = sort_pairs(p.view_sort, U(L.vkeys), U(L.perm), U(L.vsorted), U(L.perm2),
-                       U(L.dtmp_k), U(L.dtmp_v), p.n_items, nullptr, b + L.view_sort, err, s);
+                       U(L.dtmp_k), U(L.dtmp_v), p.n_items, kept, b + L.view_sort, err, s);
         if (e != hipSuccess) return e;
         order = U(L.perm2);
         order_views = U(L.vsorted);
     }
-    e = emit<true>(p.n_items, 0, views, order_views, order, item_rects, p.isect_cap, b, L.emit,
-                   count,
+    e = emit<true>(p.n_items, kept, 0, views, order_views, order, item_rects, p.isect_cap, b,
+                   L.emit, count,
                    reinterpret_cast<unsigned long long*>(b + L.total), err, U(L.keys),
                    U(L.vals), item_first, s);
     if (e != hipSuccess) return e;

@@ -831,6 +831,7 @@ struct jaxsplat_batch {
     float glob_scale = 1.f, clip_thresh = 0.f;
     int block_width = 16;
     int fwd_depth = 0, bwd_depth = 0;     // Gaussians per LDS batch (0: workgroup size)
+    int schedule = JAXSPLAT_SCHEDULE_GRID;
     jsa::BatchBinPlan bin{};
     std::vector<jsa::ViewRec> views;
     std::vector<jsa::SceneRec> scenes;
@@ -839,13 +840,16 @@ struct jaxsplat_batch {
     // workspace layout
     size_t xys = 0, conics = 0, radii = 0, dkeys = 0, rects = 0, v_xy = 0, v_conic = 0,
            items = 0, bins = 0, bin_scratch = 0, item_first = 0, v_rgbo = 0, staged = 0,
+           order = 0, okeys = 0, okeys_out = 0, otmp_k = 0, otmp_v = 0, osort = 0, qcount = 0,
            ws_bytes = 0;
     int isect_cap = 0;
 };
 
 namespace {
 
-jsa::RasterGrid batch_grid(const jaxsplat_batch* b) {
+// The batch's rasterizer grid; with the work-queue schedule, the tile order
+// and counters live in the workspace w.
+jsa::RasterGrid batch_grid(const jaxsplat_batch* b, uint8_t* w) {
     jsa::RasterGrid g{};
     g.v0 = b->views[0];
     g.views = b->d_views;
@@ -858,6 +862,10 @@ jsa::RasterGrid batch_grid(const jaxsplat_batch* b) {
     g.ch_stride = 3;
     g.ch_off = 0;
     g.ch_n = 3;
+    if (b->schedule == JAXSPLAT_SCHEDULE_QUEUE) {
+        g.order = reinterpret_cast<const int*>(w + b->order);
+        g.queue_counters = reinterpret_cast<int*>(w + b->qcount);
+    }
     return g;
 }
 
@@ -985,6 +993,15 @@ int jaxsplat_batch_create(jaxsplat_batch** out, const jaxsplat_scene* scenes, in
     b->item_first = o; o = align256(o + 4 * n);
     b->v_rgbo = o; o = align256(o + 16 * n);
     b->staged = o; o = align256(o + jsa::raster_bwd_staged_bytes((int)isect_capacity));
+    // work-queue schedule: the tile order, its sort buffers and two counters
+    const size_t t4 = 4 * (size_t)(b->total_tiles > 0 ? b->total_tiles : 1);
+    b->order = o; o = align256(o + t4);
+    b->okeys = o; o = align256(o + t4);
+    b->okeys_out = o; o = align256(o + t4);
+    b->otmp_k = o; o = align256(o + t4);
+    b->otmp_v = o; o = align256(o + t4);
+    b->osort = o; o = align256(o + jsa::order_sort_plan(b->total_tiles).scratch_bytes);
+    b->qcount = o; o = align256(o + 16);
     b->ws_bytes = o;
     // device tables
     hipError_t e = hipGetDevice(&b->device);
@@ -1024,6 +1041,14 @@ int jaxsplat_batch_set_depths(jaxsplat_batch* b, int32_t fwd_depth, int32_t bwd_
     b->bwd_depth = bwd_depth;
     return JAXSPLAT_OK;
 }
+int jaxsplat_batch_set_schedule(jaxsplat_batch* b, int32_t schedule) {
+    g_err.clear();
+    if (!b) return fail(JAXSPLAT_ERR_INVALID, "batch is NULL");
+    if (schedule != JAXSPLAT_SCHEDULE_GRID && schedule != JAXSPLAT_SCHEDULE_QUEUE)
+        return fail(JAXSPLAT_ERR_INVALID, "unknown schedule %d", schedule);
+    b->schedule = schedule;
+    return JAXSPLAT_OK;
+}
 int64_t jaxsplat_batch_num_pixels(const jaxsplat_batch* b) { return b ? b->num_pixels : 0; }
 int64_t jaxsplat_batch_num_gaussians(const jaxsplat_batch* b) { return b ? b->n_gauss : 0; }
 int64_t jaxsplat_batch_pixel_offset(const jaxsplat_batch* b, int32_t view) {
@@ -1059,7 +1084,16 @@ int jaxsplat_batch_forward(const jaxsplat_batch* b, const float* means3d, const 
                                  reinterpret_cast<uint32_t*>(w + b->item_first),
                                  w + b->bin_scratch, s),
                   "batch_forward: bin");
-    JSA_CHECK_HIP(jsa::raster_fwd_grid(batch_grid(b), items, bins, xys, conics, colors, opacities,
+    const jsa::RasterGrid g = batch_grid(b, w);
+    if (g.order) {
+        // the work queue's tile order, longest lists first (the backward reuses it)
+        auto U = [&](size_t off) { return reinterpret_cast<uint32_t*>(w + off); };
+        JSA_CHECK_HIP(jsa::tile_order(g, bins, U(b->okeys), U(b->okeys_out), U(b->otmp_k),
+                                      U(b->otmp_v), reinterpret_cast<int*>(w + b->order),
+                                      w + b->osort, s),
+                      "batch_forward: tile order");
+    }
+    JSA_CHECK_HIP(jsa::raster_fwd_grid(g, items, bins, xys, conics, colors, opacities,
                                        backgrounds, final_Ts, final_idx, out_img, s),
                   "batch_forward: rasterize");
     return JAXSPLAT_OK;
@@ -1095,7 +1129,7 @@ int jaxsplat_batch_backward(const jaxsplat_batch* b, const float* means3d, const
     // staged: per-(item, tile) records summed per item (v_xy, v_conic, v_rgbo),
     // then per Gaussian over the views in the projection backward; every
     // gradient row is written, no float atomics
-    JSA_CHECK_HIP(jsa::raster_bwd_batch_staged(batch_grid(b), b->max_view_n, items, bins,
+    JSA_CHECK_HIP(jsa::raster_bwd_batch_staged(batch_grid(b, w), b->max_view_n, items, bins,
                                                b->isect_cap, xys, conics, colors, opacities,
                                                backgrounds, final_Ts, final_idx, v_out_img,
                                                radii, reinterpret_cast<const uint2*>(w + b->rects),

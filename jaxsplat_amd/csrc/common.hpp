@@ -241,6 +241,11 @@ struct RasterGrid {
     // channel pass: channels [ch_off, ch_off + ch_n) (ch_n <= 3) of colors,
     // backgrounds, images and v_colors stored with ch_stride channels per row
     int ch_stride, ch_off, ch_n;
+    // work queue (optional): a permutation of the tiles in the order a
+    // persistent grid takes them, and two device counters (forward,
+    // backward), zeroed by the launch; null: one workgroup per tile
+    const int* order;
+    int* queue_counters;
 };
 
 ViewRec single_view(const Camera& c);
@@ -300,10 +305,13 @@ SortPlan sort_plan(int n_cap, int begin_bit, int end_bit);
 // tmp_keys/tmp_vals must hold n_cap entries (ping-pong buffers).  The scratch
 // area must be zeroed by the caller (sort_zero_bytes of it) before the call.
 size_t sort_zero_bytes(const SortPlan& p);
+// With n_kept (a device int), keys equal to 0xFFFFFFFF are dropped: the first
+// pass leaves them out and writes the number of keys kept to *n_kept, and the
+// output holds the kept keys only (the later passes read their count there).
 hipError_t sort_pairs(const SortPlan& p, const uint32_t* keys_in, const uint32_t* vals_in,
                       uint32_t* keys_out, uint32_t* vals_out, uint32_t* tmp_keys,
                       uint32_t* tmp_vals, int n_host, const int* n_dev, void* scratch,
-                      uint32_t* err, hipStream_t s);
+                      uint32_t* err, hipStream_t s, int* n_kept = nullptr);
 
 // ---- binning ----
 struct BinPlan {
@@ -442,5 +450,14 @@ hipError_t raster_bwd_grid(const RasterGrid& g, const int* items_sorted, const i
                            const float* v_out_alpha, float* v_colors, float* v_opacity,
                            float* v_xy, float* v_xy_abs, float* v_conic, int consistent,
                            hipStream_t s);
+
+// Work-queue order of a batch's tiles, longest tile lists first: one stable
+// radix pass over the tiles keyed by the bit length of their list length
+// (the tile numbering order within a key).  keys/keys_out/tmp_* hold
+// total_tiles entries; sort_scratch order_sort_plan(total_tiles).scratch_bytes.
+SortPlan order_sort_plan(int total_tiles);
+hipError_t tile_order(const RasterGrid& g, const int* tile_bins, uint32_t* keys,
+                      uint32_t* keys_out, uint32_t* tmp_k, uint32_t* tmp_v, int* order,
+                      void* sort_scratch, hipStream_t s);
 
 }  // namespace jsa

@@ -316,14 +316,16 @@ __global__ __launch_bounds__(256) void project_batch_fwd_kernel(
         conics[3 * it + 1] = r.conic[1];
         conics[3 * it + 2] = r.conic[2];
         radii[it] = r.rad;
-        dkeys[it] = r.rad > 0 ? __float_as_uint(r.depth) : 0xFFFFFFFFu;
         // opacity-limited: the tiles where the per-pixel alpha test can pass
-        rects[it] = (opacities && r.rad > 0)
-                        ? opacity_tile_rect(block_width, c.tiles_x, c.tiles_y,
-                                            make_float2(r.cx, r.cy), r.rad, r.conic[0],
-                                            r.conic[1], r.conic[2],
-                                            opacities[(size_t)vr.g0 + local])
-                        : r.rect;
+        const uint2 rc = (opacities && r.rad > 0)
+                             ? opacity_tile_rect(block_width, c.tiles_x, c.tiles_y,
+                                                 make_float2(r.cx, r.cy), r.rad, r.conic[0],
+                                                 r.conic[1], r.conic[2],
+                                                 opacities[(size_t)vr.g0 + local])
+                             : r.rect;
+        rects[it] = rc;
+        // items that emit nothing get the key the depth sort drops
+        dkeys[it] = (r.rad > 0 && rect_count(rc) > 0) ? __float_as_uint(r.depth) : 0xFFFFFFFFu;
     }
 }
 

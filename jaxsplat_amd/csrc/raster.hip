@@ -53,9 +53,15 @@ constexpr float kTEps = 1e-4f;
 constexpr float kLog2e = 1.44269504088896341f;
 constexpr float kHalfLog2e = 0.5f * kLog2e;
 
-// sigma * log2(e) of the offset (dx, dy) under a base-2 scaled conic.
+// sigma * log2(e) of the offset (dx, dy) under a base-2 scaled conic, in one
+// fixed order of roundings, fma(dy, c' dy, dx * fma(a', dx, b' dy)): the
+// forward and the backward must evaluate it bit for bit alike.
+__device__ __forceinline__ float sigma2_terms(float ax, float dx, float bdy, float cdy,
+                                             float dy) {
+    return __builtin_fmaf(dy, cdy, dx * __builtin_fmaf(ax, dx, bdy));
+}
 __device__ __forceinline__ float sigma2(float4 b, float dx, float dy) {
-    return dx * (b.x * dx + b.y * dy) + b.z * dy * dy;
+    return sigma2_terms(b.x, dx, b.y * dy, b.z * dy, dy);
 }
 
 // Per-pixel acceptance of a (Gaussian, pixel) pair, one rule for the forward
@@ -213,9 +219,12 @@ __device__ __forceinline__ bool group_hit(const CullInfo& k, float4 r) {
 // ---------------------------------------------------------------------------
 // Forward: one pixel per lane, one 64-pixel group per wave.
 // ---------------------------------------------------------------------------
+// Tile gt (< rg.total_tiles) of the grid's tile numbering, rendered by the
+// whole workgroup; the flat-grid kernel and the work-queue kernel below both
+// run this body.
 template <int DEPTH>
-__global__ __launch_bounds__(256) void raster_fwd_kernel(
-    RasterGrid rg, const int* __restrict__ gids, const int2* __restrict__ bins,
+__device__ __forceinline__ void raster_fwd_tile(
+    const RasterGrid& rg, int gt, const int* __restrict__ gids, const int2* __restrict__ bins,
     const float2* __restrict__ xys, const float* __restrict__ conics,
     const float* __restrict__ colors, const float* __restrict__ opac,
     const float* __restrict__ bgs, float* __restrict__ final_Ts, int* __restrict__ final_idx,
@@ -229,8 +238,6 @@ __global__ __launch_bounds__(256) void raster_fwd_kernel(
     __shared__ int s_rok[4];
     __shared__ int s_cnt[4];
 
-    const int gt = xcd_tile(blockIdx.x, rg.total_tiles);
-    if (gt >= rg.total_tiles) return;
     const ViewRec vr = resolve_view(rg, gt);
     const int tile = gt - vr.tile_off;
     const int gshift = vr.g0 - vr.item_off;      // gaussian = item + gshift
@@ -381,6 +388,52 @@ __global__ __launch_bounds__(256) void raster_fwd_kernel(
     }
 }
 
+// Flat grid: one workgroup per tile, XCD-aware tile order.
+template <int DEPTH>
+__global__ __launch_bounds__(256) void raster_fwd_kernel(
+    RasterGrid rg, const int* __restrict__ gids, const int2* __restrict__ bins,
+    const float2* __restrict__ xys, const float* __restrict__ conics,
+    const float* __restrict__ colors, const float* __restrict__ opac,
+    const float* __restrict__ bgs, float* __restrict__ final_Ts, int* __restrict__ final_idx,
+    float* __restrict__ out_img) {
+    const int gt = xcd_tile(blockIdx.x, rg.total_tiles);
+    if (gt >= rg.total_tiles) return;
+    raster_fwd_tile<DEPTH>(rg, gt, gids, bins, xys, conics, colors, opac, bgs, final_Ts,
+                           final_idx, out_img);
+}
+
+// Next item of a work queue, taken by one workgroup: order[i] for the i-th
+// atomic increment of *counter, -1 once all n items are taken.  Every
+// workgroup ends after at most one item past n, so the grid always drains.
+__device__ __forceinline__ int next_queue_item(int* __restrict__ counter,
+                                               const int* __restrict__ order, int n) {
+    __shared__ int s_item;
+    __syncthreads();                      // the previous item's LDS reads are done
+    if (threadIdx.x == 0) s_item = atomicAdd(counter, 1);
+    __syncthreads();
+    const int i = s_item;
+    return i < n ? order[i] : -1;
+}
+
+// Work queue: a persistent grid whose workgroups take tiles in the given order
+// (longest tile lists first) until none is left.
+template <int DEPTH>
+__global__ __launch_bounds__(256) void raster_fwd_queue_kernel(
+    RasterGrid rg, const int* __restrict__ order, int* __restrict__ counter,
+    const int* __restrict__ gids, const int2* __restrict__ bins,
+    const float2* __restrict__ xys, const float* __restrict__ conics,
+    const float* __restrict__ colors, const float* __restrict__ opac,
+    const float* __restrict__ bgs, float* __restrict__ final_Ts, int* __restrict__ final_idx,
+    float* __restrict__ out_img) {
+    for (;;) {
+        const int gt = next_queue_item(counter, order, rg.total_tiles);
+        if (gt == -1) return;
+        if ((unsigned)gt >= (unsigned)rg.total_tiles) continue;   // never for a valid order
+        raster_fwd_tile<DEPTH>(rg, gt, gids, bins, xys, conics, colors, opac, bgs, final_Ts,
+                               final_idx, out_img);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Backward: NSLOT pixels per lane (slots); 64-pixel groups = (slot, wave).
 // ---------------------------------------------------------------------------
@@ -449,12 +502,11 @@ __device__ __forceinline__ float row_sums_split(float* v, int lane) {
 // a lane masked off around the call would be missing from the ballots, and
 // its pairs from the per-Gaussian sums.
 template <bool CONSISTENT, int NV>
-__device__ __forceinline__ uint64_t bwd_pixel(PixState& s, const float4 a, const float4 b,
-                                          const float4 col, int k, float* g) {
+__device__ __forceinline__ uint64_t bwd_pixel(PixState& s, float dx, float dy, float s2,
+                                              const float4 a, const float4 b, const float4 col,
+                                              int k, float* g) {
 #define JSA_ACC(i, v) (g[i] += (v))
     constexpr float kClamp = CONSISTENT ? 0.999f : 0.99f;
-    const float dx = a.x - s.px, dy = a.y - s.py;
-    const float s2 = sigma2(b, dx, dy);
     const float raw = a.z * __builtin_amdgcn_exp2f(-s2);
     const bool live = k <= s.fin;
     const bool acc = pair_accepted(s2, col.w);
@@ -534,15 +586,15 @@ struct StagedOut {
 // NSLOT pixels per lane (slots), so each (Gaussian, wave) reduction covers
 // 64 * NSLOT pixels; 64-pixel groups = (slot, wave).
 template <bool CONSISTENT, bool ABS, int DEPTH, bool STAGED, int NSLOT>
-__global__ __launch_bounds__(128) void raster_bwd_kernel(
-    RasterGrid rg, const int* __restrict__ gids, const int2* __restrict__ bins,
+__device__ __forceinline__ void raster_bwd_tile(
+    const RasterGrid& rg, int gt, const int* __restrict__ gids, const int2* __restrict__ bins,
     const float2* __restrict__ xys, const float* __restrict__ conics,
     const float* __restrict__ colors, const float* __restrict__ opac,
     const float* __restrict__ bgs, const float* __restrict__ final_Ts,
     const int* __restrict__ final_idx, const float* __restrict__ v_out_img,
     const float* __restrict__ v_out_alpha, float* __restrict__ v_rgb,
     float* __restrict__ v_opacity, float* __restrict__ v_xy, float* __restrict__ v_xy_abs,
-    float* __restrict__ v_conic, StagedOut st) {
+    float* __restrict__ v_conic, const StagedOut& st) {
     constexpr int NV = ABS ? 11 : 9;     // rgb3, conic3, xy2, [xy_abs2], opacity
     constexpr int NW = DEPTH / 64;       // 64-entry words per batch
     __shared__ float4 s_a[DEPTH];        // x, y, opacity, id
@@ -555,8 +607,6 @@ __global__ __launch_bounds__(128) void raster_bwd_kernel(
     __shared__ int s_rok[4];
     __shared__ int s_wmax[2];
 
-    const int gt = xcd_tile(blockIdx.x, rg.total_tiles);
-    if (gt >= rg.total_tiles) return;
     const ViewRec vr = resolve_view(rg, gt);
     const int tile = gt - vr.tile_off;
     const int gshift = vr.g0 - vr.item_off;      // gaussian = item + gshift
@@ -567,6 +617,9 @@ __global__ __launch_bounds__(128) void raster_bwd_kernel(
     const int wave = tid >> 6;
     const int nwaves = nthr >> 6;
     const int ngroups = NSLOT * nwaves;  // <= 4
+    // slot sl of lane l is pixel sl * 64 + l: for 16 x 16 tiles (one wave) the
+    // four 8 x 8 quadrants at the same offset l
+    const bool quad = bw == 16;
     const int ty = tile / vr.tiles_x, tx = tile - ty * vr.tiles_x;
     const int x_org = tx * bw, y_org = ty * bw;
     const float4 bg4 = load_color(rg, bgs, vr.view);
@@ -701,10 +754,42 @@ __global__ __launch_bounds__(128) void raster_bwd_kernel(
                 // -0 + x is x exactly, so the first slot's sums need no add
                 for (int v = 0; v < NV; ++v) g[v] = -0.f;
                 uint64_t any = 0;
+                bool shared = false;
+                if constexpr (NSLOT == 4) {
+                    if (quad) {
+                        // 16 x 16 quadrant slots: slots 0/2 and 1/3 share a
+                        // pixel column, 0/1 and 2/3 a row, so dx, dy and the
+                        // row terms b' dy, c' dy of sigma are computed once per
+                        // column / row (the same roundings as sigma2 per pixel)
+                        shared = true;
+                        const float dxa = a.x - ps[0].px, dxb = a.x - ps[1].px;
 #pragma unroll
-                for (int sl = 0; sl < NSLOT; ++sl)
-                    if ((ms[sl] >> b) & 1ull)
-                        any |= bwd_pixel<CONSISTENT, NV>(ps[sl], a, bb, col, k, g);
+                        for (int r = 0; r < 2; ++r) {
+                            const int s0 = 2 * r;
+                            const bool h0 = (ms[s0] >> b) & 1ull, h1 = (ms[s0 + 1] >> b) & 1ull;
+                            if (!(h0 || h1)) continue;
+                            const float dy = a.y - ps[s0].py;
+                            const float bdy = bb.y * dy, cdy = bb.z * dy;
+                            if (h0)
+                                any |= bwd_pixel<CONSISTENT, NV>(
+                                    ps[s0], dxa, dy, sigma2_terms(bb.x, dxa, bdy, cdy, dy), a,
+                                    bb, col, k, g);
+                            if (h1)
+                                any |= bwd_pixel<CONSISTENT, NV>(
+                                    ps[s0 + 1], dxb, dy, sigma2_terms(bb.x, dxb, bdy, cdy, dy),
+                                    a, bb, col, k, g);
+                        }
+                    }
+                }
+                if (!shared) {
+#pragma unroll
+                    for (int sl = 0; sl < NSLOT; ++sl)
+                        if ((ms[sl] >> b) & 1ull) {
+                            const float dx = a.x - ps[sl].px, dy = a.y - ps[sl].py;
+                            any |= bwd_pixel<CONSISTENT, NV>(ps[sl], dx, dy, sigma2(bb, dx, dy),
+                                                             a, bb, col, k, g);
+                        }
+                }
                 if (any == 0) continue;
                 // lane j of every row gets the row sum of value j, then the four
                 // rows are summed with the row swaps; lanes 0..NV-1 store the
@@ -764,6 +849,42 @@ __global__ __launch_bounds__(128) void raster_bwd_kernel(
             unsafeAtomicAdd(v_opacity + gid, r[NV - 1]);
         }
         __syncthreads();
+    }
+}
+
+template <bool CONSISTENT, bool ABS, int DEPTH, bool STAGED, int NSLOT>
+__global__ __launch_bounds__(128) void raster_bwd_kernel(
+    RasterGrid rg, const int* __restrict__ gids, const int2* __restrict__ bins,
+    const float2* __restrict__ xys, const float* __restrict__ conics,
+    const float* __restrict__ colors, const float* __restrict__ opac,
+    const float* __restrict__ bgs, const float* __restrict__ final_Ts,
+    const int* __restrict__ final_idx, const float* __restrict__ v_out_img,
+    const float* __restrict__ v_out_alpha, float* __restrict__ v_rgb,
+    float* __restrict__ v_opacity, float* __restrict__ v_xy, float* __restrict__ v_xy_abs,
+    float* __restrict__ v_conic, StagedOut st) {
+    const int gt = xcd_tile(blockIdx.x, rg.total_tiles);
+    if (gt >= rg.total_tiles) return;
+    raster_bwd_tile<CONSISTENT, ABS, DEPTH, STAGED, NSLOT>(
+        rg, gt, gids, bins, xys, conics, colors, opac, bgs, final_Ts, final_idx, v_out_img,
+        v_out_alpha, v_rgb, v_opacity, v_xy, v_xy_abs, v_conic, st);
+}
+
+// Work-queue form of the staged batch backward (see raster_fwd_queue_kernel).
+template <bool CONSISTENT, int DEPTH, int NSLOT>
+__global__ __launch_bounds__(128) void raster_bwd_queue_kernel(
+    RasterGrid rg, const int* __restrict__ order, int* __restrict__ counter,
+    const int* __restrict__ gids, const int2* __restrict__ bins,
+    const float2* __restrict__ xys, const float* __restrict__ conics,
+    const float* __restrict__ colors, const float* __restrict__ opac,
+    const float* __restrict__ bgs, const float* __restrict__ final_Ts,
+    const int* __restrict__ final_idx, const float* __restrict__ v_out_img, StagedOut st) {
+    for (;;) {
+        const int gt = next_queue_item(counter, order, rg.total_tiles);
+        if (gt == -1) return;
+        if ((unsigned)gt >= (unsigned)rg.total_tiles) continue;   // never for a valid order
+        raster_bwd_tile<CONSISTENT, false, DEPTH, true, NSLOT>(
+            rg, gt, gids, bins, xys, conics, colors, opac, bgs, final_Ts, final_idx, v_out_img,
+            nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, st);
     }
 }
 
@@ -972,7 +1093,51 @@ __global__ void raster_empty_kernel(size_t n_pix, int nch, const float* __restri
 
 int tile_grid(int total_tiles) { return 8 * ((total_tiles + 7) / 8); }
 
+// Workgroups of a persistent work-queue grid: the CUs times the workgroups of
+// `kernel` one CU holds at once (at most one per tile).
+template <typename K>
+hipError_t queue_grid(K kernel, int threads, int total_tiles, int* grid) {
+    int dev = 0, cus = 0, per_cu = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e == hipSuccess)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0);
+    if (e != hipSuccess) return e;
+    const long long n = (long long)(cus > 0 ? cus : 1) * (per_cu > 0 ? per_cu : 1);
+    *grid = (int)(n < total_tiles ? n : total_tiles);
+    return hipSuccess;
+}
+
+// Stable-pass key of tile gt: the leading zeros of its list length (31 for
+// an empty list), so that a stable ascending pass puts the longest lists
+// first; 5 bits.
+__global__ __launch_bounds__(256) void tile_order_keys_kernel(RasterGrid g,
+                                                              const int2* __restrict__ bins,
+                                                              uint32_t* __restrict__ keys) {
+    const int gt = blockIdx.x * 256 + threadIdx.x;
+    if (gt >= g.total_tiles) return;
+    const ViewRec vr = resolve_view(g, gt);
+    const int2 b = bins[vr.bin_off + gt - vr.tile_off];
+    const int len = b.y > b.x ? b.y - b.x : 0;
+    keys[gt] = len > 0 ? (uint32_t)__clz(len) : 31u;
+}
+
 }  // namespace
+
+SortPlan order_sort_plan(int total_tiles) { return sort_plan(total_tiles, 0, 5); }
+
+hipError_t tile_order(const RasterGrid& g, const int* tile_bins, uint32_t* keys,
+                      uint32_t* keys_out, uint32_t* tmp_k, uint32_t* tmp_v, int* order,
+                      void* sort_scratch, hipStream_t s) {
+    if (g.total_tiles <= 0) return hipSuccess;
+    tile_order_keys_kernel<<<(g.total_tiles + 255) / 256, 256, 0, s>>>(
+        g, reinterpret_cast<const int2*>(tile_bins), keys);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return sort_pairs(order_sort_plan(g.total_tiles), keys, nullptr, keys_out,
+                      reinterpret_cast<uint32_t*>(order), tmp_k, tmp_v, g.total_tiles, nullptr,
+                      sort_scratch, nullptr, s);
+}
 
 ViewRec single_view(const Camera& c) {
     ViewRec v{};
@@ -1039,6 +1204,29 @@ hipError_t raster_fwd_grid(const RasterGrid& g, const int* items_sorted, const i
     const int grid = tile_grid(g.total_tiles);
     const int2* b2 = reinterpret_cast<const int2*>(tile_bins);
     const float2* x2 = reinterpret_cast<const float2*>(xys);
+    if (g.order) {
+        // work queue: a persistent grid takes the tiles in g.order
+        hipError_t e = fill_zero(g.queue_counters, 4, s);
+        if (e != hipSuccess) return e;
+        int qgrid = 0;
+#define JSA_QUEUE(D)                                                                         \
+    do {                                                                                     \
+        if ((e = queue_grid(raster_fwd_queue_kernel<D>, nthr, g.total_tiles, &qgrid)) !=     \
+            hipSuccess)                                                                      \
+            return e;                                                                        \
+        raster_fwd_queue_kernel<D><<<qgrid, nthr, 0, s>>>(                                   \
+            g, g.order, g.queue_counters, items_sorted, b2, x2, conics, colors, opacities,   \
+            backgrounds, final_Ts, final_idx, out_img);                                      \
+    } while (0)
+        switch (depth) {
+            case 64: JSA_QUEUE(64); break;
+            case 128: JSA_QUEUE(128); break;
+            case 256: JSA_QUEUE(256); break;
+            default: return hipErrorInvalidValue;
+        }
+#undef JSA_QUEUE
+        return hipGetLastError();
+    }
 #define JSA_LAUNCH(D)                                                                        \
     raster_fwd_kernel<D><<<grid, nthr, 0, s>>>(g, items_sorted, b2, x2, conics, colors,     \
                                                opacities, backgrounds, final_Ts, final_idx,  \
@@ -1206,10 +1394,24 @@ hipError_t raster_bwd_batch_staged(const RasterGrid& g, int max_view_n, const in
     const int grid = tile_grid(g.total_tiles);
     const int2* b2 = reinterpret_cast<const int2*>(tile_bins);
     const float2* x2 = reinterpret_cast<const float2*>(xys);
+    // work queue (g.order): a persistent grid takes the tiles in that order
+    if (g.order && (e = fill_zero(g.queue_counters + 1, 4, s)) != hipSuccess) return e;
 #define JSA_LAUNCH_S(CONS, D, S)                                                             \
-    raster_bwd_kernel<CONS, false, D, true, S><<<grid, nthr, 0, s>>>(                       \
-        g, items_sorted, b2, x2, conics, colors, opacities, backgrounds, final_Ts, final_idx, \
-        v_out, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, st)
+    do {                                                                                     \
+        if (g.order) {                                                                       \
+            int qgrid = 0;                                                                   \
+            if ((e = queue_grid(raster_bwd_queue_kernel<CONS, D, S>, nthr, g.total_tiles,    \
+                                &qgrid)) != hipSuccess)                                      \
+                return e;                                                                    \
+            raster_bwd_queue_kernel<CONS, D, S><<<qgrid, nthr, 0, s>>>(                      \
+                g, g.order, g.queue_counters + 1, items_sorted, b2, x2, conics, colors,      \
+                opacities, backgrounds, final_Ts, final_idx, v_out, st);                     \
+        } else {                                                                             \
+            raster_bwd_kernel<CONS, false, D, true, S><<<grid, nthr, 0, s>>>(                \
+                g, items_sorted, b2, x2, conics, colors, opacities, backgrounds, final_Ts,   \
+                final_idx, v_out, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, st); \
+        }                                                                                    \
+    } while (0)
 #define JSA_LAUNCH_D(CONS, D)                                                                \
     do {                                                                                     \
         if (slots == 4) JSA_LAUNCH_S(CONS, D, 4);                                            \

@@ -28,6 +28,7 @@ constexpr int kSortWaves = kSortThreads / kWave;
 constexpr int kDigitBits = 8;
 constexpr int kBins = 1 << kDigitBits;
 constexpr int kMaxPasses = 4;
+constexpr uint32_t kDropKey = 0xFFFFFFFFu;      // the key the first pass may drop
 
 __device__ __forceinline__ int read_n(int n_host, const int* n_dev) {
     return n_dev ? *n_dev : n_host;
@@ -48,10 +49,11 @@ __device__ __forceinline__ uint64_t match_digit(uint32_t d, int bits, uint64_t v
 // Per-partition digit counts of one pass, stored digit-major:
 // counts[d * parts + part].  Per-wave LDS histograms keep same-digit runs
 // (frequent in emission order) from piling onto one LDS word.
+// With drop, keys equal to kDropKey are left out (counted nowhere).
 template <int ITEMS>
 __global__ __launch_bounds__(kSortThreads) void radix_upsweep(
     const uint32_t* __restrict__ keys, int n_host, const int* __restrict__ n_dev, int shift,
-    int bits, int parts, uint32_t* __restrict__ counts) {
+    int bits, int parts, uint32_t* __restrict__ counts, int drop) {
     constexpr int TILE = kSortThreads * ITEMS;
     __shared__ uint32_t s_hist[kSortWaves][kBins];
     const int tid = threadIdx.x;
@@ -66,7 +68,10 @@ __global__ __launch_bounds__(kSortThreads) void radix_upsweep(
 #pragma unroll
     for (int r = 0; r < ITEMS; ++r) {
         const int p = base + r * kSortThreads + tid;
-        if (p < n) atomicAdd(&s_hist[wave][(keys[p] >> shift) & mask], 1u);
+        if (p < n) {
+            const uint32_t k = keys[p];
+            if (!(drop && k == kDropKey)) atomicAdd(&s_hist[wave][(k >> shift) & mask], 1u);
+        }
     }
     __syncthreads();
     uint32_t v = 0;
@@ -153,14 +158,16 @@ __device__ __forceinline__ uint32_t block_exclusive_scan256(uint32_t v, uint32_t
     return off + incl - v;
 }
 
-// Stable ranking + scatter of one partition with known global offsets.
+// Stable ranking + scatter of one partition with known global offsets.  With
+// drop, keys equal to kDropKey are left out, so the output is the kept keys
+// only; partition 0 then also writes their number to *n_kept.
 template <int ITEMS>
 __global__ __launch_bounds__(kSortThreads) void radix_downsweep(
     const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
     uint32_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out, int n_host,
     const int* __restrict__ n_dev, int shift, int bits, int parts,
     const uint32_t* __restrict__ counts /* scanned, digit-major */,
-    const uint32_t* __restrict__ totals /* per digit */) {
+    const uint32_t* __restrict__ totals /* per digit */, int drop, int* __restrict__ n_kept) {
     constexpr int TILE = kSortThreads * ITEMS;
     __shared__ uint32_t s_keys[TILE];
     __shared__ uint32_t s_vals[TILE];
@@ -168,36 +175,42 @@ __global__ __launch_bounds__(kSortThreads) void radix_downsweep(
     __shared__ uint32_t s_lstart[kBins];
     __shared__ uint32_t s_gbase[kBins];
     __shared__ uint32_t s_tmp[kSortWaves];
+    __shared__ uint32_t s_valid;
 
     const int tid = threadIdx.x;
     const int lane = lane_id();
     const int wave = tid >> 6;
     const uint32_t digit_mask = (1u << bits) - 1u;
+    const int part = blockIdx.x;
+    if (n_kept && part == 0) {       // the kept keys: the sum of the digit totals
+        const uint32_t ex = block_exclusive_scan256(totals[tid], s_tmp);
+        if (tid == kSortThreads - 1) *n_kept = (int)(ex + totals[tid]);
+    }
 #pragma unroll
     for (int w = 0; w < kSortWaves; ++w) s_whist[w][tid] = 0u;
     __syncthreads();
     const int n = read_n(n_host, n_dev);
-    const int part = blockIdx.x;
     const int base = part * TILE;
     if (base >= n) return;  // uniform over the block
 
     // ---- coalesced loads of this wave's ITEMS x 64 consecutive keys ----
     uint32_t key[ITEMS], val[ITEMS];
     uint16_t rank[ITEMS];
+    bool keep[ITEMS];
     const uint64_t lt = __lanemask_lt();
     const int wbase = base + wave * (kWave * ITEMS);
 #pragma unroll
     for (int r = 0; r < ITEMS; ++r) {
         const int p = wbase + r * kWave + lane;
-        const bool valid = p < n;
-        key[r] = valid ? keys_in[p] : 0u;
-        val[r] = valid ? (vals_in ? vals_in[p] : (uint32_t)p) : 0u;
+        const bool in = p < n;
+        key[r] = in ? keys_in[p] : 0u;
+        val[r] = in ? (vals_in ? vals_in[p] : (uint32_t)p) : 0u;
+        keep[r] = in && !(drop && key[r] == kDropKey);
     }
     // ---- stable wave-local ranking with 64-lane ballots ----
 #pragma unroll
     for (int r = 0; r < ITEMS; ++r) {
-        const int p = wbase + r * kWave + lane;
-        const bool valid = p < n;
+        const bool valid = keep[r];
         const uint32_t d = (key[r] >> shift) & digit_mask;
         const uint64_t peers = match_digit(d, bits, ballot(valid));
         uint32_t prior = 0;
@@ -222,13 +235,13 @@ __global__ __launch_bounds__(kSortThreads) void radix_downsweep(
     const uint32_t lstart = block_exclusive_scan256(total, s_tmp);
     s_lstart[d] = lstart;
     s_gbase[d] = dstart + counts[(size_t)d * parts + part];
+    if (d == kBins - 1) s_valid = lstart + total;      // keys kept in this partition
     __syncthreads();
 
     // ---- scatter into LDS in digit order ----
 #pragma unroll
     for (int r = 0; r < ITEMS; ++r) {
-        const int p = wbase + r * kWave + lane;
-        if (p < n) {
+        if (keep[r]) {
             const uint32_t dd = (key[r] >> shift) & digit_mask;
             const uint32_t lp = s_lstart[dd] + s_whist[wave][dd] + rank[r];
             s_keys[lp] = key[r];
@@ -238,7 +251,7 @@ __global__ __launch_bounds__(kSortThreads) void radix_downsweep(
     __syncthreads();
 
     // ---- runs of equal digits go to contiguous global positions ----
-    const int cnt = min(TILE, n - base);
+    const int cnt = (int)s_valid;
     for (int i = tid; i < cnt; i += kSortThreads) {
         const uint32_t k = s_keys[i];
         const uint32_t dd = (k >> shift) & digit_mask;
@@ -274,7 +287,7 @@ size_t sort_zero_bytes(const SortPlan& p) { return 0; }
 hipError_t sort_pairs(const SortPlan& p, const uint32_t* keys_in, const uint32_t* vals_in,
                       uint32_t* keys_out, uint32_t* vals_out, uint32_t* tmp_keys,
                       uint32_t* tmp_vals, int n_host, const int* n_dev, void* scratch,
-                      uint32_t* err, hipStream_t s) {
+                      uint32_t* err, hipStream_t s, int* n_kept) {
     (void)err;
     if (p.passes <= 0 || p.passes > kMaxPasses) return hipErrorInvalidValue;
     uint32_t* totals = static_cast<uint32_t*>(scratch);
@@ -294,31 +307,27 @@ hipError_t sort_pairs(const SortPlan& p, const uint32_t* keys_in, const uint32_t
         uint32_t* vo = to_out ? vals_out : tmp_vals;
         const int shift = p.begin_bit + q * per;
         const int bits = min(per, p.end_bit - shift);
-        if (items == 16)
-            radix_upsweep<16><<<p.parts, kSortThreads, 0, s>>>(ki, n_host, n_dev, shift, bits,
-                                                                p.parts, counts);
-        else if (items == 8)
-            radix_upsweep<8><<<p.parts, kSortThreads, 0, s>>>(ki, n_host, n_dev, shift, bits,
-                                                               p.parts, counts);
-        else
-            radix_upsweep<4><<<p.parts, kSortThreads, 0, s>>>(ki, n_host, n_dev, shift, bits,
-                                                               p.parts, counts);
-        if ((e = hipGetLastError()) != hipSuccess) return e;
-        radix_colscan<<<kBins, kSortThreads, 0, s>>>(counts, p.parts, totals);
-        if ((e = hipGetLastError()) != hipSuccess) return e;
-        if (items == 16)
-            radix_downsweep<16><<<p.parts, kSortThreads, 0, s>>>(ki, vi, ko, vo, n_host, n_dev,
-                                                                  shift, bits, p.parts, counts,
-                                                                  totals);
-        else if (items == 8)
-            radix_downsweep<8><<<p.parts, kSortThreads, 0, s>>>(ki, vi, ko, vo, n_host, n_dev,
-                                                                 shift, bits, p.parts, counts,
-                                                                 totals);
-        else
-            radix_downsweep<4><<<p.parts, kSortThreads, 0, s>>>(ki, vi, ko, vo, n_host, n_dev,
-                                                                 shift, bits, p.parts, counts,
-                                                                 totals);
-        if ((e = hipGetLastError()) != hipSuccess) return e;
+        // with n_kept the first pass drops the kDropKey keys; the later passes
+        // sort the kept ones, counted on the device
+        const int drop = (n_kept && q == 0) ? 1 : 0;
+        const int* nd = (n_kept && q > 0) ? n_kept : n_dev;
+        int* nk = drop ? n_kept : nullptr;
+#define JSA_PASS(IT)                                                                         \
+    do {                                                                                     \
+        radix_upsweep<IT><<<p.parts, kSortThreads, 0, s>>>(ki, n_host, nd, shift, bits,      \
+                                                            p.parts, counts, drop);          \
+        if ((e = hipGetLastError()) != hipSuccess) return e;                                 \
+        radix_colscan<<<kBins, kSortThreads, 0, s>>>(counts, p.parts, totals);               \
+        if ((e = hipGetLastError()) != hipSuccess) return e;                                 \
+        radix_downsweep<IT><<<p.parts, kSortThreads, 0, s>>>(ki, vi, ko, vo, n_host, nd,     \
+                                                              shift, bits, p.parts, counts,  \
+                                                              totals, drop, nk);             \
+        if ((e = hipGetLastError()) != hipSuccess) return e;                                 \
+    } while (0)
+        if (items == 16) JSA_PASS(16);
+        else if (items == 8) JSA_PASS(8);
+        else JSA_PASS(4);
+#undef JSA_PASS
         ki = ko;
         vi = vo;
     }

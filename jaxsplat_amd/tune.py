@@ -6,6 +6,9 @@ Gaussians, cameras and image sizes):
 * ``fwd_depth`` and ``bwd_depth`` in {64, 128, 256}: Gaussians staged in LDS
   per batch by the rasterize forward and backward.  Results do not depend on
   them (every pixel walks the same entries in the same order).
+* ``schedule``: the rasterizer's one workgroup per tile ("grid"), or a
+  persistent grid taking the tiles longest list first from a device counter
+  ("queue"), timed at the chosen depths.  Results do not depend on it.
 * ``block_width`` over the candidates the caller allows (8 and/or 16).  The
   tile side is part of the reference's semantics: a Gaussian joins the lists
   of the tiles its 3-sigma rectangle touches, so a different tile size can
@@ -42,7 +45,8 @@ class Tuning:
     bwd_depth: int
     capacity: int
     ms: float                       # forward + backward at the chosen setting
-    table: dict = field(default_factory=dict)   # (bw, "fwd"|"bwd", depth) -> ms
+    table: dict = field(default_factory=dict)   # (bw, "fwd"|"bwd"[_queue], depth) -> ms
+    schedule: str = "grid"
 
 
 def _p(t):
@@ -69,11 +73,11 @@ def _median_ms(fn, iters, stream):
 
 
 def autotune(scenes, views, means3d, scales, quats, colors, opacities, viewmats, backgrounds, *,
-             block_widths=(16,), depths=DEPTHS, iters=3, glob_scale=1.0, clip_thresh=0.01,
-             headroom=1.1, device=None, cache=True):
+             block_widths=(16,), depths=DEPTHS, schedules=("grid", "queue"), iters=3,
+             glob_scale=1.0, clip_thresh=0.01, headroom=1.1, device=None, cache=True):
     """Pick (block_width, fwd_depth, bwd_depth) for a batch; see the module
     docstring.  Arguments are those of ``BatchPlan`` and ``render_batch``."""
-    key = layout_key(scenes, views, glob_scale, clip_thresh, block_widths)
+    key = layout_key(scenes, views, glob_scale, clip_thresh, block_widths) + (tuple(schedules),)
     if cache and key in _CACHE:
         return _CACHE[key]
     dev = torch.device(device) if device is not None else means3d.device
@@ -126,8 +130,20 @@ def autotune(scenes, views, means3d, scales, quats, colors, opacities, viewmats,
             table[(bw, "bwd", d)] = _median_ms(bwd, iters, stream)
         bd = min(depths, key=lambda d: table[(bw, "bwd", d)])
         ms = table[(bw, "fwd", fd)] + table[(bw, "bwd", bd)]
+        sched = "grid"
+        if "queue" in schedules:
+            # the work queue at the chosen depths
+            plan.set_depths(fd, bd)
+            plan.set_schedule("queue")
+            fwd()
+            table[(bw, "fwd_queue", fd)] = _median_ms(fwd, iters, stream)
+            bwd()
+            table[(bw, "bwd_queue", bd)] = _median_ms(bwd, iters, stream)
+            qms = table[(bw, "fwd_queue", fd)] + table[(bw, "bwd_queue", bd)]
+            if qms < ms:
+                ms, sched = qms, "queue"
         if best is None or ms < best.ms:
-            best = Tuning(bw, fd, bd, plan.isect_capacity, ms)
+            best = Tuning(bw, fd, bd, plan.isect_capacity, ms, schedule=sched)
         del plan
     best.table = table
     if cache:
@@ -141,4 +157,5 @@ def tuned_plan(tuning, scenes, views, *, glob_scale=1.0, clip_thresh=0.01, devic
                      block_width=tuning.block_width, isect_capacity=tuning.capacity,
                      device=device)
     plan.set_depths(tuning.fwd_depth, tuning.bwd_depth)
+    plan.set_schedule(tuning.schedule)
     return plan

@@ -170,3 +170,9 @@ def test_opacity_limited_entry_points_reject_bad_arguments():
     assert staged(rects=None) == _lib.ERR_INVALID
     assert staged(ws=bwsb - 1) == _lib.ERR_WORKSPACE
     assert b"workspace too small" in L.jaxsplat_last_error()
+
+
+def test_batch_set_schedule_rejects():
+    L = _lib.lib()
+    assert L.jaxsplat_batch_set_schedule(None, 0) == _lib.ERR_INVALID
+    assert set(_lib.SCHEDULE.values()) == {0, 1}

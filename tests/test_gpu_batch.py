@@ -217,8 +217,8 @@ def test_batch_capacity_overflow_and_renderer():
     # capacity for (most opacities below 1/255 at first) is checked and
     # rendered again, not cut off at the capacity
     faint = list(params)
-    faint[4] = (params[4] * np.where(np.arange(params[4].shape[0]) % 8 == 0, 1.0, 1e-3)
-                ).astype(np.float32)
+    keep = np.arange(params[4].shape[0]) % 8 == 0
+    faint[4] = (params[4] * np.where(keep, 1.0, 1e-3)[:, None]).astype(np.float32)
     r2 = BatchRenderer(sizes, views, initial_capacity=100)
     with t.no_grad():
         r2(*_dev_params(faint), H.dev(vm), H.dev(bg))
@@ -321,3 +321,31 @@ def test_batch_lds_depths_do_not_change_results(bw, fd, bd):
     for a, b in zip(outs[0][1], outs[1][1]):
         ok, worst = H.grad_close(b, a, rtol=1e-4, scale_frac=1e-5)
         assert ok, worst
+
+
+@pytest.mark.parametrize("bw", [16, 8])
+def test_batch_queue_schedule_equals_grid(bw):
+    """The work-queue schedule (tiles ordered longest list first, taken by a
+    persistent grid) gives the flat grid's images, final_Ts and gradients
+    bit for bit, on two calls in a row (the queue counters start over)."""
+    t = H.torch()
+    params, sizes, cams = _layout()
+    vm, bg = _views_arrays(cams)
+    out = {}
+    for sched in ("grid", "queue", "queue"):
+        plan = _plan(sizes, cams, bw)
+        plan.set_schedule(sched)
+        d = _dev_params(params, requires_grad=True)
+        flat = render_batch(*d, H.dev(vm), H.dev(bg), plan=plan)
+        w = t.from_numpy(np.random.default_rng(5).normal(
+            size=tuple(flat.shape)).astype(np.float32)).cuda()
+        (flat * w).sum().backward()
+        plan.check()
+        res = [H.host(flat.detach())] + [H.host(x.grad) for x in d]
+        if sched in out:
+            res2 = res
+        else:
+            out[sched] = res
+    for a, b, c in zip(out["grid"], out["queue"], res2):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+        assert np.array_equal(a.view(np.uint32), c.view(np.uint32))
