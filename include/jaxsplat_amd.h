@@ -134,6 +134,24 @@ int jaxsplat_project_forward(const jaxsplat_descriptor* d, const float* means3d,
                              void* stream);
 
 /*
+ * Projection forward that allocates nothing and takes no workspace.  No
+ * counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Same arguments and results as jaxsplat_project_forward
+ * without the workspace; the scan's scratch is a library-held device array
+ * of 512 KiB per GPU, so num_points is at most 2^26 (JAXSPLAT_ERR_CAPACITY
+ * above).  All calls on one GPU share that array: calls that overlap on one
+ * GPU (other streams, other threads) race and must be serialised by the
+ * caller.
+ */
+int jaxsplat_project_forward_noalloc(const jaxsplat_descriptor* d, const float* means3d,
+                                     const float* scales, const float* quats,
+                                     const float* viewmat, float* cov3d, float* xys,
+                                     float* depths, int32_t* radii, float* conics,
+                                     float* compensation, int32_t* num_tiles_hit,
+                                     int32_t* cum_tiles_hit, void* stream);
+
+/*
  * Projection backward; replaces ops::project::bwd::xla (lib/ops.cu:63-105) and
  * project_gaussians_bwd (lib/kernels/backward.cu:345-418).  v_depth,
  * v_compensation (with compensation), v_cov2d and v_cov3d are nullable.

@@ -26,7 +26,7 @@ EXPORTS = (
     "jaxsplat_make_descriptor",
     "jaxsplat_project_fwd", "jaxsplat_project_bwd", "jaxsplat_rasterize_fwd",
     "jaxsplat_rasterize_bwd", "jaxsplat_project_workspace_size",
-    "jaxsplat_project_forward", "jaxsplat_project_backward",
+    "jaxsplat_project_forward", "jaxsplat_project_forward_noalloc", "jaxsplat_project_backward",
     "jaxsplat_bin_workspace_size", "jaxsplat_bin_and_sort", "jaxsplat_bin_check",
     "jaxsplat_bin_depth_workspace_size", "jaxsplat_bin_depth_sort",
     "jaxsplat_bin_emit_workspace_size", "jaxsplat_bin_emit_sort",
@@ -108,6 +108,7 @@ _SIGS = {
     "jaxsplat_rasterize_bwd": (None, [P, ctypes.POINTER(P), ctypes.c_char_p, SZ]),
     "jaxsplat_project_workspace_size": (SZ, [DP]),
     "jaxsplat_project_forward": (I32, [DP] + [P] * 12 + [P, SZ, P]),
+    "jaxsplat_project_forward_noalloc": (I32, [DP] + [P] * 12 + [P]),
     "jaxsplat_project_backward": (I32, [DP] + [P] * 16 + [I32, P]),
     "jaxsplat_bin_workspace_size": (SZ, [DP, U32]),
     "jaxsplat_bin_and_sort": (I32, [DP, P, P, P, U32, P, P, P, P, SZ, P]),

@@ -245,6 +245,33 @@ int jaxsplat_project_forward(const jaxsplat_descriptor* d, const float* means3d,
     return JAXSPLAT_OK;
 }
 
+int jaxsplat_project_forward_noalloc(const jaxsplat_descriptor* d, const float* means3d,
+                                     const float* scales, const float* quats,
+                                     const float* viewmat, float* cov3d, float* xys,
+                                     float* depths, int32_t* radii, float* conics,
+                                     float* compensation, int32_t* num_tiles_hit,
+                                     int32_t* cum_tiles_hit, void* stream) {
+    g_err.clear();
+    if (int st = validate(d)) return st;
+    if (d->num_points > (uint32_t)jsa::kStaticProjectPoints)
+        return fail(JAXSPLAT_ERR_CAPACITY,
+                    "project_forward_noalloc: %u Gaussians, at most %d with the library's "
+                    "own scratch (use jaxsplat_project_forward with a workspace)",
+                    d->num_points, jsa::kStaticProjectPoints);
+    if (d->num_points == 0) return JAXSPLAT_OK;
+    if (!means3d || !scales || !quats || !viewmat || !xys || !depths || !radii || !conics ||
+        !num_tiles_hit || !cum_tiles_hit)
+        return fail(JAXSPLAT_ERR_INVALID, "project_forward_noalloc: NULL required pointer");
+    void* scratch = nullptr;
+    JSA_CHECK_HIP(jsa::project_static_scratch(&scratch), "project_forward_noalloc: scratch");
+    JSA_CHECK_HIP(jsa::project_fwd(camera_of(d), means3d, scales, quats, viewmat, cov3d, xys,
+                                   depths, radii, conics, compensation, num_tiles_hit,
+                                   cum_tiles_hit, nullptr, nullptr, scratch,
+                                   static_cast<hipStream_t>(stream)),
+                  "project_forward_noalloc");
+    return JAXSPLAT_OK;
+}
+
 int jaxsplat_project_forward_rects(const jaxsplat_descriptor* d, const float* means3d,
                                    const float* scales, const float* quats,
                                    const float* viewmat, const float* opacities, float* cov3d,

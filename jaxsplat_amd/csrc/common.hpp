@@ -270,6 +270,12 @@ hipError_t fill_value_f32x3(float* dst, size_t n_pixels, const float* rgb, hipSt
 
 // ---- projection ----
 size_t project_fwd_scratch_bytes(int n);
+// Largest num_points of the projection forward with the library-held scratch.
+constexpr int kStaticProjectPoints = 1 << 26;
+// *out = the current device's address of the library-held projection scratch
+// (project_fwd_scratch_bytes(kStaticProjectPoints) bytes, looked up once per
+// device).  Every caller on a device shares it.
+hipError_t project_static_scratch(void** out);
 // With opacities and rects (both optional), the tile counts and their sums are
 // those of the opacity-limited rectangles (opacity_tile_rect), written to rects.
 hipError_t project_fwd(const Camera& c, const float* means3d, const float* scales,

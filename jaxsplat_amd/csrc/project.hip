@@ -29,6 +29,8 @@
 //    writes the depth sort key instead of a scan; the backward runs one lane
 //    per Gaussian over all views of its scene, summing the per-view
 //    contributions in registers.
+#include <mutex>
+
 #include "common.hpp"
 
 namespace jsa {
@@ -756,6 +758,41 @@ hipError_t fill_value_f32x3(float* dst, size_t n_pixels, const float* rgb, hipSt
 size_t project_fwd_scratch_bytes(int n) {
     const size_t parts = ((size_t)n + kProjPart - 1) / kProjPart;
     return 16 + 8 * (parts > 0 ? parts : 1);
+}
+
+// Projection scratch held by the library, for callers that cannot allocate
+// (jaxsplat_project_forward_noalloc): the fused scan's partition totals and
+// offsets for up to kStaticProjectPoints Gaussians, 512 KiB + 256 B.  Each
+// device holds its own copy until the process ends.  In host code the array's
+// name is not a device address: kernels get the address that
+// hipGetSymbolAddress returns for the current device.  Nothing in it needs
+// to start at zero (the forward kernel writes every partition total before
+// the scan reads it); calls that overlap on one device race on it.
+constexpr size_t kStaticProjectScratch =
+    (16 + 8 * (size_t)(kStaticProjectPoints / kProjPart) + 255) & ~(size_t)255;
+static_assert(kStaticProjectPoints % kProjPart == 0, "whole partitions");
+static __device__ __align__(256) uint8_t scratch[kStaticProjectScratch];
+
+hipError_t project_static_scratch(void** out) {
+    constexpr int kMaxDevices = 64;
+    static std::mutex mu;
+    static void* ptr[kMaxDevices] = {};
+    *out = nullptr;
+    if (project_fwd_scratch_bytes(kStaticProjectPoints) > kStaticProjectScratch)
+        return hipErrorInvalidValue;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(mu);
+    if (!ptr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        ptr[dev] = p;
+    }
+    *out = ptr[dev];
+    return hipSuccess;
 }
 
 hipError_t project_fwd(const Camera& c, const float* means3d, const float* scales,
