@@ -73,11 +73,20 @@ def parse():
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="run a device-to-device copy of this many MiB first (HBM counter "
                          "calibration under rocprofv3 --pmc)")
+    ap.add_argument("--summarize-profiles", metavar="DIR", default=None,
+                    help="no GPU work: summarise the rocprofv3 outputs under DIR (trace/, fetch/, "
+                         "write/, sq/, lds/ of the profiling commands) into profiles/")
+    ap.add_argument("--profile-command", default="",
+                    help="with --summarize-profiles: the profiled command, quoted in the summaries")
     return ap.parse_args()
 
 
 def main():
     args = parse()
+    if args.summarize_profiles:
+        summarize_profiles(Path(args.summarize_profiles), args.workload, args.copy_calibration_mb,
+                           args.profile_command)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
               "(the path would shard by view or scene, each GPU holding all Gaussians, with one "
@@ -210,9 +219,15 @@ def committed_counters(workload):
     return rec, None
 
 
+def _plain_name(name):
+    """A kernel name without its return type and anonymous namespaces."""
+    name = name.replace("(anonymous namespace)::", "")
+    return name[5:] if name.startswith("void ") else name
+
+
 def _kernel_entry(rec, prefix):
     for name, v in rec.get("kernels", {}).items():
-        if name.startswith(prefix):
+        if _plain_name(name).startswith(prefix):
             return v
     return None
 
@@ -473,7 +488,8 @@ def stage_breakdown(torch, sc, cam, params, target, iters):
                              "valu_insts_per_launch": k_.get("valu_insts"),
                              "valu_insts_per_entry": round(k_["valu_insts"] / ent, 1)
                              if k_.get("valu_insts") else None,
-                             "lds_bank_conflict_frac": k_.get("lds_bank_conflict_frac")}
+                             "lds_bank_conflict_frac": k_.get("lds_bank_conflict_frac"),
+                             "l2_atomics_per_launch": k_.get("l2_atomics")}
         roof["valu"] = valu
         roof["counters_source"] = f"profiles/{sc.name}_counters.json"
     else:
@@ -786,6 +802,177 @@ def cpu_baseline(sc, cam, tgt_seed, args):
                        f"Gaussians on one core, prorated to {n_tiles}/{T} tiles and divided "
                        f"over the {workers} cores"),
             "host_cpu": cpu_model(), "host_logical_cpus": os.cpu_count()}
+
+
+# ---------------------------------------------------------------------------
+# rocprofv3 output -> committed summaries (no GPU work)
+# ---------------------------------------------------------------------------
+# the profiling runs, each into DIR/<sub> with -o run --output-format csv
+PROFILE_PASSES = {
+    "trace": "--kernel-trace --stats",
+    "fetch": "--pmc FETCH_SIZE",
+    "write": "--pmc WRITE_SIZE",
+    "sq": "--pmc SQ_WAVES SQ_INSTS_VALU SQ_INSTS_SALU SQ_INSTS_LDS SQ_ACTIVE_INST_VALU "
+          "SQ_WAVE_CYCLES SQ_BUSY_CYCLES GRBM_GUI_ACTIVE",
+    "lds": "--pmc SQ_LDS_BANK_CONFLICT SQ_LDS_ADDR_CONFLICT SQ_LDS_IDX_ACTIVE SQ_ACTIVE_INST_LDS "
+           "SQ_WAIT_INST_LDS GRBM_GUI_ACTIVE TCC_ATOMIC_sum",
+}
+NUM_CU = 256
+NUM_XCD = 8
+
+
+def _read_csv_rows(paths):
+    import csv
+    rows = []
+    for p in paths:
+        with open(p, newline="") as fh:
+            rows.extend(csv.DictReader(fh))
+    return rows
+
+
+def _counter_table(sub_dir):
+    """{kernel name: {counter: [value per dispatch]}} from a --pmc run."""
+    out = {}
+    rows = _read_csv_rows(sorted(sub_dir.rglob("*counter_collection.csv")))
+    per = {}
+    for r in rows:
+        key = (r["Dispatch_Id"], r["Counter_Name"])
+        per[key] = per.get(key, 0.0) + float(r["Counter_Value"])
+        out.setdefault(r["Kernel_Name"], {}).setdefault(r["Counter_Name"], {})
+        out[r["Kernel_Name"]][r["Counter_Name"]][r["Dispatch_Id"]] = per[key]
+    return {k: {c: list(d.values()) for c, d in v.items()} for k, v in out.items()}
+
+
+def _short(name, n=72):
+    return name if len(name) <= n else name[:n]
+
+
+def summarize_profiles(prof, workload, calib_mb, command):
+    """Kernel statistics, HBM traffic (FETCH_SIZE / WRITE_SIZE scaled by the
+    copy calibration), VALU issue and LDS / atomic counters per kernel, written
+    as profiles/<workload>_counters.json (read back by the bench line while
+    its source hash matches the kernels built here) and two Markdown tables."""
+    prof = Path(prof)
+    rec = {"workload": workload, "source_sha16": source_hash(),
+           "command": command or "(not given)", "passes": {}, "kernels": {}}
+    kern = rec["kernels"]
+    stats = _read_csv_rows(sorted((prof / "trace").rglob("*kernel_stats.csv")))
+    for r in stats:
+        kern.setdefault(r["Name"], {}).update(
+            calls=int(r["Calls"]), avg_us=float(r["AverageNs"]) / 1e3,
+            min_us=float(r["MinNs"]) / 1e3, max_us=float(r["MaxNs"]) / 1e3,
+            pct=float(r["Percentage"]))
+    rec["passes"]["trace"] = PROFILE_PASSES["trace"]
+    # copy calibration: the calib_mb MiB device copies are the copy kernel's
+    # dispatches with the largest counts; bytes moved each way = calib_mb MiB
+    scale = {}
+    for sub, ctr in (("fetch", "FETCH_SIZE"), ("write", "WRITE_SIZE")):
+        if not (prof / sub).is_dir():
+            continue
+        rec["passes"][sub] = PROFILE_PASSES[sub]
+        tab = _counter_table(prof / sub)
+        if calib_mb:
+            copies = [v for k, c in tab.items() if "copyBuffer" in k for v in c.get(ctr, [])]
+            if copies:
+                top = sorted(copies)[-3:]
+                scale[sub] = calib_mb * 1024.0 / (sum(top) / len(top))   # KiB per KiB counted
+        for k, c in tab.items():
+            if ctr in c:
+                vals = c[ctr]
+                kern.setdefault(k, {})[ctr.lower() + "_kib"] = sum(vals) / len(vals)
+    rec["fetch_scale"] = round(scale.get("fetch", 1.0), 4)
+    rec["write_scale"] = round(scale.get("write", 1.0), 4)
+    for k, v in kern.items():
+        if "fetch_size_kib" in v and "write_size_kib" in v:
+            v["hbm_bytes"] = int(1024 * (v["fetch_size_kib"] * rec["fetch_scale"] +
+                                         v["write_size_kib"] * rec["write_scale"]))
+            if v.get("avg_us"):
+                v["hbm_GBs"] = round(v["hbm_bytes"] / (v["avg_us"] * 1e-6) / 1e9, 1)
+    for sub in ("sq", "lds"):
+        if not (prof / sub).is_dir():
+            continue
+        rec["passes"][sub] = PROFILE_PASSES[sub]
+        for k, c in _counter_table(prof / sub).items():
+            v = kern.setdefault(k, {})
+            mean = {n: sum(x) / len(x) for n, x in c.items()}
+            cyc = mean.get("GRBM_GUI_ACTIVE", 0.0) / NUM_XCD     # per XCD
+            if sub == "sq":
+                v["waves"] = mean.get("SQ_WAVES")
+                v["valu_insts"] = mean.get("SQ_INSTS_VALU")
+                v["salu_insts"] = mean.get("SQ_INSTS_SALU")
+                v["lds_insts"] = mean.get("SQ_INSTS_LDS")
+                v["cycles"] = cyc
+                if cyc and "SQ_ACTIVE_INST_VALU" in mean:
+                    v["valu_busy"] = round(100.0 * mean["SQ_ACTIVE_INST_VALU"] / NUM_CU / cyc, 1)
+            else:
+                v["lds_bank_conflict_cycles"] = mean.get("SQ_LDS_BANK_CONFLICT")
+                v["lds_addr_conflict_cycles"] = mean.get("SQ_LDS_ADDR_CONFLICT")
+                v["lds_active_cycles"] = mean.get("SQ_LDS_IDX_ACTIVE")
+                v["lds_active_inst_cycles"] = mean.get("SQ_ACTIVE_INST_LDS")
+                v["lds_wait_inst_cycles"] = mean.get("SQ_WAIT_INST_LDS")
+                v["l2_atomics"] = mean.get("TCC_ATOMIC_sum", mean.get("TCC_ATOMIC"))
+                if cyc and mean.get("SQ_LDS_BANK_CONFLICT") is not None:
+                    v["lds_bank_conflict_frac"] = round(
+                        mean["SQ_LDS_BANK_CONFLICT"] / NUM_CU / cyc, 4)
+                if mean.get("SQ_LDS_IDX_ACTIVE"):
+                    v["lds_bank_conflict_per_active"] = round(
+                        mean.get("SQ_LDS_BANK_CONFLICT", 0.0) / mean["SQ_LDS_IDX_ACTIVE"], 4)
+    out_dir = ROOT / "profiles"
+    (out_dir / f"{workload}_counters.json").write_text(json.dumps(rec, indent=1) + "\n")
+
+    # the library's kernels and the calibration copy; the bench's own torch
+    # analysis kernels (consumed lengths, outside the timed regions) are left out
+    order = sorted((k for k in kern if "avg_us" in kern[k] and
+                    (_plain_name(k).startswith("jsa::") or "copyBuffer" in k)),
+                   key=lambda k: -kern[k]["avg_us"] * kern[k]["calls"])
+    cmds = "\n".join(f"rocprofv3 {PROFILE_PASSES[s]} -d DIR/{s} -o run --output-format csv -- "
+                     f"{command}" for s in rec["passes"])
+    lines = [f"# rocprofv3 kernel statistics and HBM traffic: {workload} fwd+bwd step", "",
+             "Commands (repository root, MI355X, ROCm 7.2; DIR = gpurun_out/prof), then "
+             f"`python bench.py --summarize-profiles DIR --workload {workload} "
+             f"--copy-calibration-mb {calib_mb}`:", "", "```", cmds, "```", "",
+             f"Source hash of the profiled kernels: `{rec['source_sha16']}`.  HBM bytes = "
+             f"FETCH_SIZE x {rec['fetch_scale']} + WRITE_SIZE x {rec['write_scale']} (KiB per "
+             f"dispatch; scales from the {calib_mb} MiB copy calibration).  Library kernels "
+             "only: the torch kernels of the bench's consumed-length analysis (outside the "
+             "timed regions) are left out, so % GPU time is of the whole profiled run.", "",
+             "| kernel | calls | avg us | min us | max us | % GPU time | HBM MB / dispatch | GB/s |",
+             "|---|---:|---:|---:|---:|---:|---:|---:|"]
+    for k in order:
+        v = kern[k]
+        hb = v.get("hbm_bytes")
+        lines.append(f"| `{_short(_plain_name(k))}` | {v['calls']} | {v['avg_us']:.1f} | {v['min_us']:.1f} | "
+                     f"{v['max_us']:.1f} | {v['pct']:.2f} | "
+                     f"{hb / 1e6:.1f} | {v.get('hbm_GBs', 0):.0f} |" if hb else
+                     f"| `{_short(_plain_name(k))}` | {v['calls']} | {v['avg_us']:.1f} | {v['min_us']:.1f} | "
+                     f"{v['max_us']:.1f} | {v['pct']:.2f} | - | - |")
+    (out_dir / f"{workload}_step_kernels.md").write_text("\n".join(lines) + "\n")
+
+    lines = [f"# SQ, LDS and L2-atomic counters: {workload} fwd+bwd step", "",
+             "Same commands as the kernel statistics.  Counters are summed over the 8 XCDs "
+             "by rocprofv3 and averaged over dispatches.  cycles = GRBM_GUI_ACTIVE / 8; "
+             "VALUBusy % = 100 SQ_ACTIVE_INST_VALU / 256 CUs / cycles; LDS bank-conflict "
+             "fraction = SQ_LDS_BANK_CONFLICT / 256 CUs / cycles (share of CU cycles the LDS "
+             "is stalled by bank conflicts); per active = SQ_LDS_BANK_CONFLICT / "
+             "SQ_LDS_IDX_ACTIVE.", "",
+             "| kernel | VALU insts | VALU / wave | VALUBusy % | LDS insts | LDS bank-conflict frac "
+             "| conflict / LDS active | L2 atomics |",
+             "|---|---:|---:|---:|---:|---:|---:|---:|"]
+
+    def f(x, fmt):
+        return format(x, fmt) if x is not None else "-"
+    for k in order:
+        v = kern[k]
+        if "valu_insts" not in v and "lds_bank_conflict_cycles" not in v:
+            continue
+        vpw = v["valu_insts"] / v["waves"] if v.get("valu_insts") and v.get("waves") else None
+        lines.append(f"| `{_short(_plain_name(k))}` | {f(v.get('valu_insts'), '.3g')} | {f(vpw, '.0f')} | "
+                     f"{f(v.get('valu_busy'), '.1f')} | {f(v.get('lds_insts'), '.3g')} | "
+                     f"{f(v.get('lds_bank_conflict_frac'), '.4f')} | "
+                     f"{f(v.get('lds_bank_conflict_per_active'), '.4f')} | "
+                     f"{f(v.get('l2_atomics'), '.3g')} |")
+    (out_dir / f"{workload}_sq_counters.md").write_text("\n".join(lines) + "\n")
+    print(f"wrote profiles/{workload}_counters.json, _step_kernels.md, _sq_counters.md")
 
 
 def cpu_model():
