@@ -5,11 +5,16 @@ c=(960,540), block 16, seeded synthetic scene (``jaxsplat_amd.scenes``), L2
 loss against a seeded U(0,1) target.  One step = render forward (projection,
 binning/sort, rasterize) + loss + backward (rasterize adjoint, projection
 adjoint) through the public API ``jaxsplat_amd.render`` and torch autograd.
-Inputs are resident in HBM before the timed region.
+The headline (``value``, ``ms_per_step``) times that step captured once in a
+HIP graph and replayed (``render`` with an ``IsectCapacity``, so nothing
+synchronises with the host, and the fused L2 loss kernel); ``eager`` times the
+same step launched eagerly with torch's ``mse_loss``.  Inputs are resident in
+HBM before the timed region.
 
 Prints ONE JSON line.  Besides the contract fields it carries:
-* ``roofline``: the dominant kernel (the rasterize backward: its record kernel
-  and per-Gaussian reduction) -- algorithmic bytes per launch / its average
+* ``roofline``: the dominant kernels (the render backward: the rasterize
+  adjoint's record kernel and the per-Gaussian tail that sums the records and
+  runs the projection adjoint) -- algorithmic bytes per launch / their average
   duration measured with HIP events on the launch stream, against the 8 TB/s
   HBM peak -- plus the VALU-issue figures of both rasterize kernels from the
   committed rocprofv3 counters (``profiles/hd1m_counters.json``, used only
@@ -21,8 +26,7 @@ Prints ONE JSON line.  Besides the contract fields it carries:
   count B_alg of the reference algorithm (reference tile lists, 64-bit key
   sort), with L_f and L_b measured, and with this implementation's own
   per-stage byte model;
-* ``graph``: the same step captured in one HIP graph (``render`` with an
-  ``IsectCapacity``: no host synchronisation) and replayed;
+* ``graph``: the headline step's details (capacity, intersections, loss);
 * ``cpu_baseline``: the float64 NumPy oracle on a bounded sample of the same
   workload, on a process pool over the host's cores;
 * ``batched``: the multi-view workload ``orbit64`` (64 cameras around one
@@ -149,13 +153,21 @@ def main():
     e1.record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    ms = e0.elapsed_time(e1) / args.steps
-    mpix = Himg * Wimg / (ms * 1e-3) / 1e6
+    ms_eager = e0.elapsed_time(e1) / args.steps
+    mpix_eager = Himg * Wimg / (ms_eager * 1e-3) / 1e6
 
     stages, roof, extra = stage_breakdown(torch, sc, cam, params, target, args.stage_iters)
     graph = graph_leg(torch, sc, cam, params, kw, target, args.steps, args.warmup,
                       extra["num_intersects_binned"]) if args.graph else None
-    step_fractions(extra, ms, graph)
+    # the headline step is the HIP-graph replay of the same step (the way a
+    # training loop runs it: no host synchronisation, no per-launch host
+    # work), with the eager figure beside it
+    if graph is not None:
+        ms, mpix, path = graph["ms_per_step"], graph["mpix_s"], graph["path"]
+    else:
+        ms, mpix = ms_eager, mpix_eager
+        path = "jaxsplat_amd.render + torch mse_loss + autograd backward, eager"
+    step_fractions(extra, ms, ms_eager)
 
     batched = [batched_vs_plain(torch, w, args.batched_steps)
                for w in args.batched_workloads.split(",") if w] if args.batched else None
@@ -175,8 +187,11 @@ def main():
         "data": "synthetic (seeded scene and U(0,1) target)",
         "config": {"workload": args.workload, "num_gaussians": sc.n, "img_shape": [Himg, Wimg],
                    "block_width": sc.block_width, "f": list(cam.f), "c": list(cam.c),
-                   "path": "plain single view (jaxsplat_amd.render + torch autograd, eager)",
+                   "path": "plain single view: " + path,
                    "parallelism": "none (1 GPU)"},
+        "eager": {"path": "jaxsplat_amd.render + torch mse_loss + autograd backward, eager "
+                          "(the binning reads the intersection count back to the host)",
+                  "ms_per_step": round(ms_eager, 4), "mpix_s": round(mpix_eager, 1)},
         "roofline": roof,
         "cpu_baseline": cpu,
         "stages": stages,
@@ -191,7 +206,7 @@ def main():
 # ---------------------------------------------------------------------------
 # Committed counter summaries (rocprofv3) and their validity
 # ---------------------------------------------------------------------------
-STAGED_BWD = "raster_bwd_kernel<*, *, *, true, *> + raster_bwd_reduce_kernel"
+RENDER_BWD = "raster_bwd_kernel<*, *, *, true, *> + render_bwd_tail_kernel"
 
 
 def source_hash():
@@ -366,7 +381,7 @@ def stage_breakdown(torch, sc, cam, params, target, iters):
     bwsb = L.jaxsplat_rasterize_backward_staged_workspace_size(D, isect)
     bws = torch.empty(bwsb, dtype=torch.uint8, device=dev)
     names = ["project_fwd", "depth_sort", "emission", "tile_sort", "bin_edges", "raster_fwd",
-             "loss_grad", "raster_bwd_records", "raster_bwd_reduce", "project_bwd"]
+             "loss_grad", "raster_bwd_records", "bwd_tail"]
     acc = {k: 0.0 for k in names}
     marks = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
     for m in marks:
@@ -376,7 +391,7 @@ def stage_breakdown(torch, sc, cam, params, target, iters):
     L.jaxsplat_set_stage_marks(handles, 3)
     try:
         for it in range(iters + 2):
-            ev = [torch.cuda.Event(enable_timing=True) for _ in range(8)]
+            ev = [torch.cuda.Event(enable_timing=True) for _ in range(7)]
             ev[0].record()
             project(True)
             ev[1].record()
@@ -392,22 +407,21 @@ def stage_breakdown(torch, sc, cam, params, target, iters):
             ev[4].record()
             torch.mul(torch.sub(img, target), 2.0 / img.numel(), out=v_img)
             ev[5].record()
-            _lib.check(L.jaxsplat_rasterize_backward_staged_rects(
-                D, _p(gids), _p(bins), isect, _p(xys), _p(conics), _p(colors), _p(opac), _p(bg),
-                3, _p(radii), _p(rects), _p(tcum), _p(fT), _p(fI), _p(v_img), None, _p(vc),
-                _p(vo), _p(vxy), None, _p(vcn), 0, _p(bws), bwsb, s))
+            # the render backward: the staged records, then the per-Gaussian
+            # tail (record sums + projection adjoint)
+            _lib.check(L.jaxsplat_render_backward(
+                D, _p(gids), _p(bins), isect, _p(means3d), _p(scales), _p(quats), _p(viewmat),
+                _p(xys), _p(conics), _p(colors), _p(opac), _p(bg), 3, _p(radii), _p(rects),
+                _p(tcum), _p(fT), _p(fI), _p(v_img), _p(vc), _p(vo), _p(vm), _p(vs), _p(vq), 0,
+                _p(bws), bwsb, s))
             ev[6].record()
-            _lib.check(L.jaxsplat_project_backward(
-                D, _p(means3d), _p(scales), _p(quats), _p(viewmat), _p(radii), _p(conics),
-                None, _p(vxy), None, _p(vcn), None, _p(vm), _p(vs), _p(vq), None, None, 0, s))
-            ev[7].record()
             torch.cuda.synchronize()
             if it >= 2:
                 t = [ev[0].elapsed_time(ev[1]), ev[1].elapsed_time(ev[2]),
                      ev[2].elapsed_time(marks[0]), marks[0].elapsed_time(marks[1]),
                      marks[1].elapsed_time(ev[3]), ev[3].elapsed_time(ev[4]),
                      ev[4].elapsed_time(ev[5]), ev[5].elapsed_time(marks[2]),
-                     marks[2].elapsed_time(ev[6]), ev[6].elapsed_time(ev[7])]
+                     marks[2].elapsed_time(ev[6])]
                 for k, n in enumerate(names):
                     acc[n] += t[k] / iters
     finally:
@@ -448,8 +462,7 @@ def stage_breakdown(torch, sc, cam, params, target, iters):
         "raster_fwd": 40 * L_ft + 8 * T + 20 * Px,
         "loss_grad": 36 * Px,
         "raster_bwd_records": 40 * L_bt + 8 * T + 20 * Px + 48 * R + (isect + 7) // 8,
-        "raster_bwd_reduce": 48 * R + (isect + 7) // 8 + 16 * N + 36 * N,
-        "project_bwd": 76 * V + 4 * (N - V) + 40 * N,
+        "bwd_tail": 48 * R + (isect + 7) // 8 + 16 * N + 56 * V + 4 * (N - V) + 40 * N,
     }
     valu_bound = {"raster_fwd", "raster_bwd_records"}
     stages = {}
@@ -460,22 +473,26 @@ def stage_breakdown(torch, sc, cam, params, target, iters):
                      "GBs": round(gbs, 1), "frac_hbm": round(gbs / HBM_PEAK_GBS, 4),
                      "bound": "valu" if k in valu_bound else "hbm"}
 
-    # ---- roofline of the dominant kernel: the rasterize backward ----
-    # algorithmic bytes per launch, counted on the reference's lists of the
-    # same projection (L_b): 40 B per consumed intersection (id 4 + xy 8 +
-    # opacity 4 + conic 12 + colour 12) + 20 B per pixel (final T 4 + final
-    # index 4 + v_img 12) + 36 B per visible Gaussian (v_rgb 12 + v_opacity 4
-    # + v_xy 8 + v_conic 12 written) + 8 B per tile (bin read)
-    bytes_bwd = 40 * L_b + 20 * Px + 36 * V + 8 * T
-    ms_bwd = acc["raster_bwd_records"] + acc["raster_bwd_reduce"]
+    # ---- roofline of the dominant kernels: the render backward ----
+    # (the rasterize adjoint's record kernel and the per-Gaussian tail that
+    # sums the records and runs the projection adjoint).  Algorithmic bytes
+    # per launch, counted on the reference's lists of the same projection
+    # (L_b): 40 B per consumed intersection (id 4 + xy 8 + opacity 4 + conic
+    # 12 + colour 12) + 20 B per pixel (final T 4 + final index 4 + v_img 12)
+    # + 8 B per tile (bin read) + 16 B per Gaussian (v_rgb 12 + v_opacity 4
+    # written) + the projection adjoint's 56 B per visible Gaussian read
+    # (means, scales, quats 40, radius 4, conic 12) + 4 B per culled one
+    # (radius) + 40 B per Gaussian written (v_mean3d, v_scale, v_quat)
+    bytes_bwd = 40 * L_b + 20 * Px + 8 * T + 16 * N + 56 * V + 4 * (N - V) + 40 * N
+    ms_bwd = acc["raster_bwd_records"] + acc["bwd_tail"]
     achieved = bytes_bwd / (ms_bwd * 1e-3) / 1e9
     counters, why = committed_counters(sc.name)
-    roof = {"kernel": STAGED_BWD, "bound": "hbm", "achieved": round(achieved, 2),
+    roof = {"kernel": RENDER_BWD, "bound": "hbm", "achieved": round(achieved, 2),
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
             "traffic": None, "avg_ms": round(ms_bwd, 4), "bytes_per_launch": int(bytes_bwd)}
     if counters is not None:
         kr = _kernel_entry(counters, "jsa::raster_bwd_kernel<false, false, 64, true, 4>")
-        kd = _kernel_entry(counters, "jsa::raster_bwd_reduce_kernel<false>")
+        kd = _kernel_entry(counters, "jsa::render_bwd_tail_kernel")
         kf = _kernel_entry(counters, "jsa::raster_fwd_kernel<256>")
         if kr and kd and kr.get("hbm_bytes") and kd.get("hbm_bytes"):
             roof["traffic"] = int(kr["hbm_bytes"] + kd["hbm_bytes"])
@@ -511,16 +528,15 @@ def stage_breakdown(torch, sc, cam, params, target, iters):
     return stages, roof, extra
 
 
-def step_fractions(extra, ms, graph):
+def step_fractions(extra, ms, ms_eager):
     """Whole-step fractions of the HBM peak: B_alg (the reference algorithm's
-    bytes) and this implementation's per-stage byte model, over the eager
-    step time and over the graph replay's."""
+    bytes) and this implementation's per-stage byte model, over the headline
+    step time and over the eager step's."""
     def frac(b, t_ms):
         return round(b / (t_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5)
     extra["step_frac_hbm"] = frac(extra["B_alg_bytes"], ms)
     extra["impl_step_frac_hbm"] = frac(extra["impl_bytes_per_step"], ms)
-    if graph:
-        extra["step_frac_hbm_graph"] = frac(extra["B_alg_bytes"], graph["ms_per_step"])
+    extra["step_frac_hbm_eager"] = frac(extra["B_alg_bytes"], ms_eager)
 
 
 # ---------------------------------------------------------------------------
@@ -573,7 +589,9 @@ def graph_leg(torch, sc, cam, params, kw, target, steps, warmup, isect_hint):
     total = cap.check()                    # raises if a replay overflowed the capacity
     ms = e0.elapsed_time(e1) / steps
     H, W = cam.img_shape
-    return {"path": "render(isect_capacity=...) + fused L2 + backward, one HIP graph replay",
+    return {"path": "jaxsplat_amd.render(isect_capacity=...) + fused L2 loss and gradient "
+                    "(jaxsplat_fit_l2) + autograd backward, captured once, one HIP graph "
+                    "replay per step",
             "ms_per_step": round(ms, 4), "mpix_s": round(H * W / (ms * 1e-3) / 1e6, 1),
             "capacity": cap.capacity, "num_intersects": total,
             "loss": float(loss_sum.item()) / target.numel()}

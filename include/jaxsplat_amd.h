@@ -350,6 +350,30 @@ int jaxsplat_rasterize_backward_staged_rects(
     float* v_colors, float* v_opacity, float* v_xy, float* v_xy_abs, float* v_conic, int adjoint,
     void* workspace, size_t workspace_bytes, void* stream);
 
+/*
+ * Backward of the fused render (jaxsplat_project_forward_rects, binning into
+ * those opacity-limited rectangles, jaxsplat_rasterize_forward_binned); for
+ * the reference's render (jaxsplat/__init__.py:6-70), whose backward is the
+ * rasterize adjoint (lib/kernels/backward.cu:153-343) followed by the
+ * projection adjoint (backward.cu:345-570).  The rasterize adjoint writes the
+ * staged records of jaxsplat_rasterize_backward_staged_rects into the
+ * workspace; one per-Gaussian pass then sums them and runs the projection
+ * VJP in registers, writing v_colors (N, channels), v_opacity (N),
+ * v_mean3d (N,3), v_scale (N,3) and v_quat (N,4), every row.  Results equal
+ * jaxsplat_rasterize_backward_staged_rects followed by
+ * jaxsplat_project_backward bit for bit.  d: the projection's descriptor
+ * (its intrinsics); workspace: jaxsplat_rasterize_backward_staged_workspace_size.
+ */
+int jaxsplat_render_backward(
+    const jaxsplat_descriptor* d, const int32_t* gaussian_ids_sorted, const int32_t* tile_bins,
+    uint32_t num_intersects, const float* means3d, const float* scales, const float* quats,
+    const float* viewmat, const float* xys, const float* conics, const float* colors,
+    const float* opacities, const float* background, uint32_t channels, const int32_t* radii,
+    const uint32_t* tile_rects, const int32_t* cum_tiles_hit, const float* final_Ts,
+    const int32_t* final_idx, const float* v_out_img, float* v_colors, float* v_opacity,
+    float* v_mean3d, float* v_scale, float* v_quat, int adjoint, void* workspace,
+    size_t workspace_bytes, void* stream);
+
 /* ------------------------------------------------------------------------
  * Batched rendering: many views of one or more scenes in one pass.
  *

@@ -52,6 +52,15 @@ def render(means3d, scales, quats, colors, opacities, *, viewmat, background, im
     may be captured in a HIP graph; ``isect_capacity.check()`` reports an
     overflow afterwards.
     """
+    if _ops.TIGHT_TILES and _ops.FUSED_RENDER and not antialiased:
+        # one autograd node: projection with the opacity-limited rectangles,
+        # binning, rasterization; its backward sums the rasterize records and
+        # runs the projection adjoint in one per-Gaussian pass
+        return _ops.render_fused(means3d, scales, quats, colors, opacities, viewmat=viewmat,
+                                 background=background, img_shape=img_shape, f=f, c=c,
+                                 glob_scale=glob_scale, clip_thresh=clip_thresh,
+                                 block_width=block_size, adjoint=adjoint,
+                                 isect_capacity=isect_capacity)
     if _ops.TIGHT_TILES:
         # one projection pass also yields the opacity-limited tile rectangles
         # (of the raw opacities: compensation <= 1 only shrinks the region)

@@ -33,7 +33,7 @@ EXPORTS = (
     "jaxsplat_rasterize_backward_staged_workspace_size", "jaxsplat_rasterize_backward_staged",
     "jaxsplat_tile_rects_workspace_size", "jaxsplat_tile_rects", "jaxsplat_bin_depth_sort_rects",
     "jaxsplat_project_forward_rects",
-    "jaxsplat_rasterize_backward_staged_rects",
+    "jaxsplat_rasterize_backward_staged_rects", "jaxsplat_render_backward",
     "jaxsplat_rasterize_forward_binned", "jaxsplat_rasterize_backward_binned",
     "jaxsplat_rasterize_forward_nd", "jaxsplat_rasterize_backward_nd",
     "jaxsplat_batch_create", "jaxsplat_batch_destroy", "jaxsplat_batch_workspace_size",
@@ -125,6 +125,8 @@ _SIGS = {
     "jaxsplat_rasterize_backward_staged_rects": (I32, [DP, P, P, U32, P, P, P, P, P, U32, P, P,
                                                         P, P, P, P, P, P, P, P, P, P, I32, P, SZ,
                                                         P]),
+    "jaxsplat_render_backward": (I32, [DP, P, P, U32] + [P] * 9 + [U32] + [P] * 11
+                                 + [I32, P, SZ, P]),
     "jaxsplat_rasterize_backward_staged": (I32, [DP, P, P, U32, P, P, P, P, P, U32, P, P, P, P, P,
                                                   P, P, P, P, P, P, I32, P, SZ, P]),
     "jaxsplat_rasterize_forward_binned": (I32, [DP, P, P, U32] + [P] * 8 + [P]),

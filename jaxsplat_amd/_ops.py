@@ -42,6 +42,7 @@ from . import _lib
 
 STAGED_BACKWARD = os.environ.get("JAXSPLAT_AMD_STAGED_BACKWARD", "1") != "0"
 TIGHT_TILES = os.environ.get("JAXSPLAT_AMD_TIGHT_TILES", "1") != "0"
+FUSED_RENDER = os.environ.get("JAXSPLAT_AMD_FUSED_RENDER", "1") != "0"
 # the staged backward's record slots, and the capacity of one binning, are 31-bit
 STAGED_MAX_INTERSECTIONS = (1 << 30) - 1
 
@@ -75,6 +76,63 @@ def _adjoint(mode):
         raise ValueError(f"adjoint must be one of {sorted(_lib.ADJOINT)}, got {mode!r}") from None
 
 
+def _project_forward(d, means3d, scales, quats, viewmat, opacities=None):
+    """Projection through the C ABI.  With opacities (the fused render) the
+    tile counts are those of the opacity-limited rectangles, also returned.
+    Returns (xys, depths, radii, conics, num_tiles_hit, cum_tiles_hit,
+    compensation, rects)."""
+    N = means3d.shape[0]
+    dev = means3d.device
+    L = _lib.lib()
+    xys = torch.empty((N, 2), dtype=torch.float32, device=dev)
+    depths = torch.empty((N, 1), dtype=torch.float32, device=dev)
+    radii = torch.empty((N, 1), dtype=torch.int32, device=dev)
+    conics = torch.empty((N, 3), dtype=torch.float32, device=dev)
+    nth = torch.empty((N, 1), dtype=torch.int32, device=dev)
+    cum = torch.empty((N, 1), dtype=torch.int32, device=dev)
+    comp = torch.empty((N, 1), dtype=torch.float32, device=dev)
+    rects = torch.empty((N if opacities is not None else 0, 2), dtype=torch.int32, device=dev)
+    ws_bytes = L.jaxsplat_project_workspace_size(ctypes.byref(d))
+    ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=dev)
+    with torch.cuda.device(dev):
+        if opacities is not None:
+            _lib.check(L.jaxsplat_project_forward_rects(
+                ctypes.byref(d), _p(means3d), _p(scales), _p(quats), _p(viewmat),
+                _p(opacities), None, _p(xys), _p(depths), _p(radii), _p(conics), _p(comp),
+                _p(nth), _p(cum), _p(rects), _p(ws), ws_bytes, _stream(dev)))
+        else:
+            _lib.check(L.jaxsplat_project_forward(
+                ctypes.byref(d), _p(means3d), _p(scales), _p(quats), _p(viewmat), None,
+                _p(xys), _p(depths), _p(radii), _p(conics), _p(comp), _p(nth), _p(cum),
+                _p(ws), ws_bytes, _stream(dev)))
+    return xys, depths, radii, conics, nth, cum, comp, rects
+
+
+def _project_backward(d, adjoint, means3d, scales, quats, viewmat, radii, conics, comp, v_xys,
+                      v_depths, v_conics, v_comp):
+    """Projection adjoint through the C ABI; None cotangents are zero.
+    Returns (v_mean3d, v_scale, v_quat)."""
+    N = means3d.shape[0]
+    dev = means3d.device
+    v_xys = torch.zeros((N, 2), dtype=torch.float32, device=dev) if v_xys is None \
+        else v_xys.contiguous()
+    v_conics = torch.zeros((N, 3), dtype=torch.float32, device=dev) if v_conics is None \
+        else v_conics.contiguous()
+    v_depths = v_depths.contiguous() if v_depths is not None else None
+    v_comp = v_comp.contiguous() if v_comp is not None else None
+    v_mean3d = torch.empty_like(means3d)
+    v_scale = torch.empty_like(scales)
+    v_quat = torch.empty_like(quats)
+    L = _lib.lib()
+    with torch.cuda.device(dev):
+        _lib.check(L.jaxsplat_project_backward(
+            ctypes.byref(d), _p(means3d), _p(scales), _p(quats), _p(viewmat),
+            _p(radii), _p(conics), _p(comp), _p(v_xys), _p(v_depths), _p(v_conics),
+            _p(v_comp), _p(v_mean3d), _p(v_scale), _p(v_quat), None, None, adjoint,
+            _stream(dev)))
+    return v_mean3d, v_scale, v_quat
+
+
 class _Project(torch.autograd.Function):
     @staticmethod
     def forward(ctx, cfg, means3d, scales, quats, viewmat, opacities=None):
@@ -82,31 +140,9 @@ class _Project(torch.autograd.Function):
         # opacity-limited tile rectangles, returned as an eighth output
         img_shape, f, c, glob_scale, clip_thresh, block_width, adjoint = cfg
         N = means3d.shape[0]
-        dev = means3d.device
         d = _lib.make_descriptor(N, img_shape, f, c, glob_scale, clip_thresh, block_width)
-        L = _lib.lib()
-        xys = torch.empty((N, 2), dtype=torch.float32, device=dev)
-        depths = torch.empty((N, 1), dtype=torch.float32, device=dev)
-        radii = torch.empty((N, 1), dtype=torch.int32, device=dev)
-        conics = torch.empty((N, 3), dtype=torch.float32, device=dev)
-        nth = torch.empty((N, 1), dtype=torch.int32, device=dev)
-        cum = torch.empty((N, 1), dtype=torch.int32, device=dev)
-        comp = torch.empty((N, 1), dtype=torch.float32, device=dev)
-        rects = torch.empty((N if opacities is not None else 0, 2), dtype=torch.int32,
-                            device=dev)
-        ws_bytes = L.jaxsplat_project_workspace_size(ctypes.byref(d))
-        ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=dev)
-        with torch.cuda.device(dev):
-            if opacities is not None:
-                _lib.check(L.jaxsplat_project_forward_rects(
-                    ctypes.byref(d), _p(means3d), _p(scales), _p(quats), _p(viewmat),
-                    _p(opacities), None, _p(xys), _p(depths), _p(radii), _p(conics), _p(comp),
-                    _p(nth), _p(cum), _p(rects), _p(ws), ws_bytes, _stream(dev)))
-            else:
-                _lib.check(L.jaxsplat_project_forward(
-                    ctypes.byref(d), _p(means3d), _p(scales), _p(quats), _p(viewmat), None,
-                    _p(xys), _p(depths), _p(radii), _p(conics), _p(comp), _p(nth), _p(cum),
-                    _p(ws), ws_bytes, _stream(dev)))
+        xys, depths, radii, conics, nth, cum, comp, rects = _project_forward(
+            d, means3d, scales, quats, viewmat, opacities)
         ctx.d = d
         ctx.adjoint = adjoint
         ctx.save_for_backward(means3d, scales, quats, viewmat, radii, conics, comp)
@@ -117,24 +153,9 @@ class _Project(torch.autograd.Function):
     @staticmethod
     def backward(ctx, v_xys, v_depths, _v_radii, v_conics, _v_nth, _v_cum, v_comp, _v_rects):
         means3d, scales, quats, viewmat, radii, conics, comp = ctx.saved_tensors
-        N = means3d.shape[0]
-        dev = means3d.device
-        v_xys = torch.zeros((N, 2), dtype=torch.float32, device=dev) if v_xys is None \
-            else v_xys.contiguous()
-        v_conics = torch.zeros((N, 3), dtype=torch.float32, device=dev) if v_conics is None \
-            else v_conics.contiguous()
-        v_depths = v_depths.contiguous() if v_depths is not None else None
-        v_comp = v_comp.contiguous() if v_comp is not None else None
-        v_mean3d = torch.empty_like(means3d)
-        v_scale = torch.empty_like(scales)
-        v_quat = torch.empty_like(quats)
-        L = _lib.lib()
-        with torch.cuda.device(dev):
-            _lib.check(L.jaxsplat_project_backward(
-                ctypes.byref(ctx.d), _p(means3d), _p(scales), _p(quats), _p(viewmat),
-                _p(radii), _p(conics), _p(comp), _p(v_xys), _p(v_depths), _p(v_conics),
-                _p(v_comp), _p(v_mean3d), _p(v_scale), _p(v_quat), None, None, ctx.adjoint,
-                _stream(dev)))
+        v_mean3d, v_scale, v_quat = _project_backward(
+            ctx.d, ctx.adjoint, means3d, scales, quats, viewmat, radii, conics, comp, v_xys,
+            v_depths, v_conics, v_comp)
         return None, v_mean3d, v_scale, v_quat, None, None
 
 
@@ -375,6 +396,77 @@ def staged_backward_applies(nch, isect):
     return STAGED_BACKWARD and 1 <= nch <= 3 and 0 <= isect <= STAGED_MAX_INTERSECTIONS
 
 
+def _rasterize_forward(d, gids, bins, isect, xys, conics, colors, opacities, background,
+                       img_shape):
+    """Rasterize forward of binned lists.  Returns (img (H,W,C), final_Ts
+    (H,W,1), final_idx (H,W,1))."""
+    N, nch = colors.shape
+    dev = colors.device
+    H, W = img_shape
+    final_Ts = torch.empty((H, W, 1), dtype=torch.float32, device=dev)
+    final_idx = torch.empty((H, W, 1), dtype=torch.int32, device=dev)
+    img = torch.empty((H, W, nch), dtype=torch.float32, device=dev)
+    L = _lib.lib()
+    with torch.cuda.device(dev):
+        _lib.check(L.jaxsplat_rasterize_forward_nd(
+            ctypes.byref(d), _p(gids), _p(bins), isect, _p(xys), _p(conics), _p(colors),
+            _p(opacities), _p(background), nch, _p(final_Ts), _p(final_idx), _p(img),
+            _stream(dev)))
+    return img, final_Ts, final_idx
+
+
+def _staged_workspace(d, isect, nch, dev):
+    """Workspace of the staged rasterize backward (records and bitmap), or
+    None where that backward does not apply or its 52 bytes per intersection
+    do not fit (the atomic backward is then used).  Returns (ws, bytes)."""
+    if not staged_backward_applies(nch, isect):
+        return None, 0
+    L = _lib.lib()
+    wsb = L.jaxsplat_rasterize_backward_staged_workspace_size(ctypes.byref(d), isect)
+    try:
+        return torch.empty(max(wsb, 1), dtype=torch.uint8, device=dev), wsb
+    except torch.OutOfMemoryError:
+        return None, 0
+
+
+def _rasterize_backward(d, isect, adjoint, colors, opacities, background, xys, conics, gids,
+                        bins, final_Ts, final_idx, radii, cum_tiles_hit, rects, v_img):
+    """Rasterize adjoint: the staged backward (per-intersection records summed
+    per Gaussian: no float atomics, bitwise reproducible) when it applies,
+    else the atomic N-channel backward.  rects: the opacity-limited tile
+    rectangles the binning used, or None for the reference's.  Returns
+    (v_colors, v_opacity, v_xy, v_conic)."""
+    dev = colors.device
+    v_img = v_img.contiguous()
+    v_colors = torch.empty_like(colors)
+    v_opacity = torch.empty_like(opacities)
+    v_xy = torch.empty_like(xys)
+    v_conic = torch.empty_like(conics)
+    L = _lib.lib()
+    nch = colors.shape[1]
+    ws, wsb = _staged_workspace(d, isect, nch, dev)
+    with torch.cuda.device(dev):
+        if ws is not None and rects is not None:
+            _lib.check(L.jaxsplat_rasterize_backward_staged_rects(
+                ctypes.byref(d), _p(gids), _p(bins), isect, _p(xys), _p(conics), _p(colors),
+                _p(opacities), _p(background), nch, _p(radii), _p(rects), _p(cum_tiles_hit),
+                _p(final_Ts), _p(final_idx), _p(v_img), None, _p(v_colors), _p(v_opacity),
+                _p(v_xy), None, _p(v_conic), adjoint, _p(ws), wsb, _stream(dev)))
+        elif ws is not None:
+            _lib.check(L.jaxsplat_rasterize_backward_staged(
+                ctypes.byref(d), _p(gids), _p(bins), isect, _p(xys), _p(conics), _p(colors),
+                _p(opacities), _p(background), nch, _p(radii), _p(cum_tiles_hit),
+                _p(final_Ts), _p(final_idx), _p(v_img), None, _p(v_colors), _p(v_opacity),
+                _p(v_xy), None, _p(v_conic), adjoint, _p(ws), wsb, _stream(dev)))
+        else:
+            _lib.check(L.jaxsplat_rasterize_backward_nd(
+                ctypes.byref(d), _p(gids), _p(bins), isect, _p(xys), _p(conics), _p(colors),
+                _p(opacities), _p(background), nch, _p(final_Ts), _p(final_idx), _p(v_img),
+                None, _p(v_colors), _p(v_opacity), _p(v_xy), None, _p(v_conic), adjoint,
+                _stream(dev)))
+    return v_colors, v_opacity, v_xy, v_conic
+
+
 class _Rasterize(torch.autograd.Function):
     @staticmethod
     def forward(ctx, cfg, colors, opacities, background, xys, depths, radii, conics,
@@ -382,7 +474,6 @@ class _Rasterize(torch.autograd.Function):
         img_shape, block_width, adjoint = cfg
         N, nch = colors.shape
         dev = colors.device
-        H, W = img_shape
         d = _lib.make_descriptor(N, img_shape, (1.0, 1.0), (0.0, 0.0), 1.0, 0.0, block_width)
         rects = None
         if tile_rects is not None:
@@ -399,15 +490,8 @@ class _Rasterize(torch.autograd.Function):
         else:
             gids, bins, isect = _bin_overlapped(d, xys, depths, radii, cum_tiles_hit, dev,
                                                 capacity)
-        final_Ts = torch.empty((H, W, 1), dtype=torch.float32, device=dev)
-        final_idx = torch.empty((H, W, 1), dtype=torch.int32, device=dev)
-        img = torch.empty((H, W, nch), dtype=torch.float32, device=dev)
-        L = _lib.lib()
-        with torch.cuda.device(dev):
-            _lib.check(L.jaxsplat_rasterize_forward_nd(
-                ctypes.byref(d), _p(gids), _p(bins), isect, _p(xys), _p(conics), _p(colors),
-                _p(opacities), _p(background), nch, _p(final_Ts), _p(final_idx), _p(img),
-                _stream(dev)))
+        img, final_Ts, final_idx = _rasterize_forward(d, gids, bins, isect, xys, conics, colors,
+                                                      opacities, background, img_shape)
         ctx.d = d
         ctx.isect = isect
         ctx.adjoint = adjoint
@@ -421,48 +505,101 @@ class _Rasterize(torch.autograd.Function):
     def backward(ctx, v_img):
         colors, opacities, background, xys, conics, gids, bins, final_Ts, final_idx, radii, \
             cum_tiles_hit, rects = ctx.saved_tensors
-        dev = colors.device
+        v_colors, v_opacity, v_xy, v_conic = _rasterize_backward(
+            ctx.d, ctx.isect, ctx.adjoint, colors, opacities, background, xys, conics, gids, bins,
+            final_Ts, final_idx, radii, cum_tiles_hit, rects if ctx.tight else None, v_img)
+        return None, v_colors, v_opacity, None, v_xy, None, None, v_conic, None, None, None
+
+
+class _Render(torch.autograd.Function):
+    """The fused render as one autograd node: the projection with the
+    opacity-limited rectangles, the binning into them and the rasterize
+    forward; its backward is ``jaxsplat_render_backward`` (the staged
+    rasterize records, then one per-Gaussian pass that sums them and runs the
+    projection VJP in registers: bit for bit the rasterize adjoint followed by
+    the projection adjoint, without the v_xy / v_conic rows in between), or,
+    where the staged backward does not apply, those two adjoints."""
+
+    @staticmethod
+    def forward(ctx, cfg, means3d, scales, quats, colors, opacities, viewmat, background,
+                capacity):
+        img_shape, f, c, glob_scale, clip_thresh, block_width, adjoint = cfg
+        N = means3d.shape[0]
+        dev = means3d.device
+        d = _lib.make_descriptor(N, img_shape, f, c, glob_scale, clip_thresh, block_width)
+        xys, depths, radii, conics, _nth, cum, comp, rects = _project_forward(
+            d, means3d, scales, quats, viewmat, opacities)
+        if N == 0:
+            gids, bins, isect = _bin(d, xys, depths, radii, 0, dev)
+        else:
+            gids, bins, isect = _bin_rects(d, depths, radii, rects, cum, dev, capacity)
+        img, final_Ts, final_idx = _rasterize_forward(d, gids, bins, isect, xys, conics, colors,
+                                                      opacities, background, img_shape)
+        ctx.d = d
+        ctx.isect = isect
+        ctx.adjoint = adjoint
+        ctx.save_for_backward(means3d, scales, quats, viewmat, colors, opacities, background,
+                              xys, conics, radii, comp, cum, rects, gids, bins, final_Ts,
+                              final_idx)
+        return img
+
+    @staticmethod
+    def backward(ctx, v_img):
+        means3d, scales, quats, viewmat, colors, opacities, background, xys, conics, radii, \
+            comp, cum, rects, gids, bins, final_Ts, final_idx = ctx.saved_tensors
+        dev = means3d.device
+        nch = colors.shape[1]
+        ws, wsb = _staged_workspace(ctx.d, ctx.isect, nch, dev)
+        if ws is None:
+            v_colors, v_opacity, v_xy, v_conic = _rasterize_backward(
+                ctx.d, ctx.isect, ctx.adjoint, colors, opacities, background, xys, conics, gids,
+                bins, final_Ts, final_idx, radii, cum, rects, v_img)
+            v_mean3d, v_scale, v_quat = _project_backward(
+                ctx.d, ctx.adjoint, means3d, scales, quats, viewmat, radii, conics, comp, v_xy,
+                None, v_conic, None)
+            return None, v_mean3d, v_scale, v_quat, v_colors, v_opacity, None, None, None
         v_img = v_img.contiguous()
         v_colors = torch.empty_like(colors)
         v_opacity = torch.empty_like(opacities)
-        v_xy = torch.empty_like(xys)
-        v_conic = torch.empty_like(conics)
+        v_mean3d = torch.empty_like(means3d)
+        v_scale = torch.empty_like(scales)
+        v_quat = torch.empty_like(quats)
         L = _lib.lib()
-        nch = colors.shape[1]
-        ws = None
-        if staged_backward_applies(nch, ctx.isect):
-            # per-intersection records summed per Gaussian: no float atomics,
-            # bitwise reproducible
-            wsb = L.jaxsplat_rasterize_backward_staged_workspace_size(ctypes.byref(ctx.d),
-                                                                      ctx.isect)
-            try:
-                ws = torch.empty(max(wsb, 1), dtype=torch.uint8, device=dev)
-            except torch.OutOfMemoryError:
-                ws = None       # 52 bytes per intersection do not fit: the atomic backward
-        if ws is not None:
-            with torch.cuda.device(dev):
-                if ctx.tight:
-                    _lib.check(L.jaxsplat_rasterize_backward_staged_rects(
-                        ctypes.byref(ctx.d), _p(gids), _p(bins), ctx.isect, _p(xys), _p(conics),
-                        _p(colors), _p(opacities), _p(background), nch, _p(radii), _p(rects),
-                        _p(cum_tiles_hit), _p(final_Ts), _p(final_idx), _p(v_img), None,
-                        _p(v_colors), _p(v_opacity), _p(v_xy), None, _p(v_conic), ctx.adjoint,
-                        _p(ws), wsb, _stream(dev)))
-                else:
-                    _lib.check(L.jaxsplat_rasterize_backward_staged(
-                        ctypes.byref(ctx.d), _p(gids), _p(bins), ctx.isect, _p(xys), _p(conics),
-                        _p(colors), _p(opacities), _p(background), nch, _p(radii),
-                        _p(cum_tiles_hit), _p(final_Ts), _p(final_idx), _p(v_img), None,
-                        _p(v_colors), _p(v_opacity), _p(v_xy), None, _p(v_conic), ctx.adjoint,
-                        _p(ws), wsb, _stream(dev)))
-            return None, v_colors, v_opacity, None, v_xy, None, None, v_conic, None, None, None
         with torch.cuda.device(dev):
-            _lib.check(L.jaxsplat_rasterize_backward_nd(
-                ctypes.byref(ctx.d), _p(gids), _p(bins), ctx.isect, _p(xys), _p(conics),
-                _p(colors), _p(opacities), _p(background), colors.shape[1], _p(final_Ts),
-                _p(final_idx), _p(v_img), None, _p(v_colors), _p(v_opacity), _p(v_xy), None,
-                _p(v_conic), ctx.adjoint, _stream(dev)))
-        return None, v_colors, v_opacity, None, v_xy, None, None, v_conic, None, None, None
+            _lib.check(L.jaxsplat_render_backward(
+                ctypes.byref(ctx.d), _p(gids), _p(bins), ctx.isect, _p(means3d), _p(scales),
+                _p(quats), _p(viewmat), _p(xys), _p(conics), _p(colors), _p(opacities),
+                _p(background), nch, _p(radii), _p(rects), _p(cum), _p(final_Ts),
+                _p(final_idx), _p(v_img), _p(v_colors), _p(v_opacity), _p(v_mean3d),
+                _p(v_scale), _p(v_quat), ctx.adjoint, _p(ws), wsb, _stream(dev)))
+        return None, v_mean3d, v_scale, v_quat, v_colors, v_opacity, None, None, None
+
+
+def render_fused(means3d, scales, quats, colors, opacities, *, viewmat, background, img_shape, f,
+                 c, glob_scale, clip_thresh, block_width, adjoint="reference",
+                 isect_capacity=None):
+    """``render`` as one autograd node (``_Render``): the opacity-limited
+    binning and the fused backward tail.  Same arguments and result as
+    ``render`` without the antialiased mode."""
+    adj = _adjoint(adjoint)
+    N = means3d.shape[0] if isinstance(means3d, torch.Tensor) else -1
+    nch = colors.shape[1] if isinstance(colors, torch.Tensor) and colors.dim() == 2 else 3
+    if nch < 1:
+        raise ShapeDtypeError("colors: need at least one channel")
+    _expect("means3d", means3d, (N, 3), torch.float32)
+    _expect("scales", scales, (N, 3), torch.float32)
+    _expect("quats", quats, (N, 4), torch.float32)
+    _expect("colors", colors, (N, nch), torch.float32)
+    _expect("opacities", opacities, (N, 1), torch.float32)
+    _expect("viewmat", viewmat, (4, 4), torch.float32)
+    _expect("background", background, (nch,), torch.float32)
+    if isect_capacity is not None and not isinstance(isect_capacity, IsectCapacity):
+        raise TypeError("isect_capacity must be an IsectCapacity")
+    cfg = (tuple(img_shape), tuple(f), tuple(c), float(glob_scale), float(clip_thresh),
+           int(block_width), adj)
+    return _Render.apply(cfg, means3d.contiguous(), scales.contiguous(), quats.contiguous(),
+                         colors.contiguous(), opacities.contiguous(), viewmat.contiguous(),
+                         background.contiguous(), isect_capacity)
 
 
 def rasterize(colors, opacities, background, xys, depths, radii, conics, cum_tiles_hit, *,

@@ -209,6 +209,27 @@ struct EmitEntries {
     uint32_t* first_by_id;  // optional: first slot by id, for emitting entries
 };
 
+// The tile sort's first-pass digit counts, produced by the expansion (see
+// sort_first_counts): with counts set, workgroup b of the expansion writes
+// exactly the slots of sort partition b (tile slots) and stores the counts of
+// digit (key >> shift) & (2^bits - 1) of the keys it wrote at
+// counts[d * parts + b] for all 256 digits, as the sort's counting kernel
+// would.  The grid is then parts workgroups.
+struct FirstCounts {
+    uint32_t* counts;
+    int tile, parts, shift, bits;
+};
+
+FirstCounts first_counts(const SortPlan& sp, uint8_t* sort_scratch) {
+    FirstCounts fc;
+    fc.counts = sp.passes > 0 ? sort_first_counts(sp, sort_scratch) : nullptr;
+    fc.tile = sp.tile;
+    fc.parts = sp.parts;
+    fc.shift = sp.first_shift;
+    fc.bits = sp.first_bits;
+    return fc;
+}
+
 template <bool BATCH>
 __global__ __launch_bounds__(256) void emit_compact_kernel(
     int n_host, const int* __restrict__ n_dev, const ViewRec* __restrict__ views,
@@ -289,73 +310,93 @@ __global__ __launch_bounds__(256) void emit_compact_kernel(
 template <bool BATCH>
 __global__ __launch_bounds__(256) void emit_expand_kernel(
     int tiles_x_plain, const EmitEntries E, const int* __restrict__ info,
-    uint32_t* __restrict__ tile_out, uint32_t* __restrict__ gid_out) {
+    uint32_t* __restrict__ tile_out, uint32_t* __restrict__ gid_out, FirstCounts fc) {
     __shared__ int s_mark[4][64];
     __shared__ uint4 s_ent[4][64];      // (gid, x0 | y0 << 16, w, first slot)
     __shared__ int2 s_key[4][64];       // BATCH: (bin_off, tiles_x)
+    __shared__ uint32_t s_hist[4][256]; // fc: per-wave digit counts of the keys written
     const int lane = lane_id();
     const int wave = threadIdx.x >> 6;
     const uint32_t limit = (uint32_t)info[0];
     const uint32_t nnz = (uint32_t)info[1];
     const uint32_t nchunks = (limit + 63u) >> 6;
-    const uint32_t nw = gridDim.x * 4u;
-    const uint32_t per = (nchunks + nw - 1u) / nw;
-    const uint32_t c0 = (blockIdx.x * 4u + (uint32_t)wave) * per;
-    if (c0 >= nchunks) return;
-    const uint32_t end = min((c0 + per) << 6, limit);   // limit < 2^31
-    uint32_t pos = c0 << 6;                               // first slot not yet written
-    uint32_t cur = (uint32_t)uniform_i((int)E.chunk_owner[c0]);   // owns slot pos
-    int step = 0;       // < 2^24: at most per + windows steps, per <= 2^25 / 64
-    s_mark[wave][lane] = 0;
-    while (pos < end) {
-        const uint32_t j = cur + (uint32_t)lane;
-        uint32_t off = limit, gid = 0u;
-        uint2 rc = make_uint2(0u, 0u);
-        if (j < nnz) {
-            off = E.off[j];
-            rc = E.rect[j];
-            gid = E.gid[j];
-            if (BATCH) s_key[wave][lane] = E.key[j];
-        }
-        const uint32_t cnt = rect_count(rc);
-        s_ent[wave][lane] = make_uint4(gid, rc.x, rc.y & 0xFFFFu, off);
-        // the window's slots end where its last entry's run ends
-        const uint32_t run_end = cnt > limit - off ? limit : off + cnt;
-        const uint32_t seg = min((uint32_t)__builtin_amdgcn_readlane((int)run_end, 63), end);
-        int carry = 0;      // lane 0 owns pos in the first window; later windows start at pos
-        for (uint32_t s0 = pos & ~63u; s0 < seg; s0 += 64u) {
-            ++step;
-            const int tag = step << 7;
-            if (cnt > 0 && off >= s0 && off < s0 + 64u && off < seg)
-                s_mark[wave][off - s0] = tag | (lane + 1);
-            wave_lds_sync();
-            const int v = wave_max_scan_nonneg(s_mark[wave][lane]);
-            const int owner = v >= tag ? (v & 127) - 1 : carry;
-            carry = __builtin_amdgcn_readlane(owner, 63);
-            const uint4 od = s_ent[wave][owner];
-            const uint32_t slot = s0 + (uint32_t)lane;
-            if (slot >= pos && slot < seg) {
-                // q = kk / w from a float estimate (|error| < 1 for q < 2^16) and one correction
-                const uint32_t kk = slot - od.w;
-                uint32_t q = (uint32_t)((float)kk * __builtin_amdgcn_rcpf((float)od.z));
-                int rem = (int)(kk - q * od.z);
-                if (rem < 0) {
-                    --q;
-                    rem += (int)od.z;
-                } else if (rem >= (int)od.z) {
-                    ++q;
-                    rem -= (int)od.z;
-                }
-                const uint32_t ty = (od.y >> 16) + q;
-                const uint32_t tx = (od.y & 0xFFFFu) + (uint32_t)rem;
-                const int2 kb = BATCH ? s_key[wave][owner] : make_int2(0, tiles_x_plain);
-                tile_out[slot] = (uint32_t)kb.x + ty * (uint32_t)kb.y + tx;
-                gid_out[slot] = od.x;
+    uint32_t per, c0;
+    if (fc.counts) {                    // workgroup = sort partition of fc.tile slots
+        per = (uint32_t)fc.tile >> 8;
+        c0 = blockIdx.x * ((uint32_t)fc.tile >> 6) + (uint32_t)wave * per;
+#pragma unroll
+        for (int w = 0; w < 4; ++w) s_hist[w][threadIdx.x] = 0u;
+        __syncthreads();
+    } else {
+        const uint32_t nw = gridDim.x * 4u;
+        per = (nchunks + nw - 1u) / nw;
+        c0 = (blockIdx.x * 4u + (uint32_t)wave) * per;
+    }
+    const uint32_t dmask = (1u << fc.bits) - 1u;
+    if (c0 < nchunks) {
+        const uint32_t end = min((c0 + per) << 6, limit);   // limit < 2^31
+        uint32_t pos = c0 << 6;                               // first slot not yet written
+        uint32_t cur = (uint32_t)uniform_i((int)E.chunk_owner[c0]);   // owns slot pos
+        int step = 0;       // < 2^24: at most per + windows steps, per <= 2^25 / 64
+        s_mark[wave][lane] = 0;
+        while (pos < end) {
+            const uint32_t j = cur + (uint32_t)lane;
+            uint32_t off = limit, gid = 0u;
+            uint2 rc = make_uint2(0u, 0u);
+            if (j < nnz) {
+                off = E.off[j];
+                rc = E.rect[j];
+                gid = E.gid[j];
+                if (BATCH) s_key[wave][lane] = E.key[j];
             }
+            const uint32_t cnt = rect_count(rc);
+            s_ent[wave][lane] = make_uint4(gid, rc.x, rc.y & 0xFFFFu, off);
+            // the window's slots end where its last entry's run ends
+            const uint32_t run_end = cnt > limit - off ? limit : off + cnt;
+            const uint32_t seg = min((uint32_t)__builtin_amdgcn_readlane((int)run_end, 63), end);
+            int carry = 0;      // lane 0 owns pos in the first window; later windows start at pos
+            for (uint32_t s0 = pos & ~63u; s0 < seg; s0 += 64u) {
+                ++step;
+                const int tag = step << 7;
+                if (cnt > 0 && off >= s0 && off < s0 + 64u && off < seg)
+                    s_mark[wave][off - s0] = tag | (lane + 1);
+                wave_lds_sync();
+                const int v = wave_max_scan_nonneg(s_mark[wave][lane]);
+                const int owner = v >= tag ? (v & 127) - 1 : carry;
+                carry = __builtin_amdgcn_readlane(owner, 63);
+                const uint4 od = s_ent[wave][owner];
+                const uint32_t slot = s0 + (uint32_t)lane;
+                if (slot >= pos && slot < seg) {
+                    // q = kk / w from a float estimate (|error| < 1 for q < 2^16) and one correction
+                    const uint32_t kk = slot - od.w;
+                    uint32_t q = (uint32_t)((float)kk * __builtin_amdgcn_rcpf((float)od.z));
+                    int rem = (int)(kk - q * od.z);
+                    if (rem < 0) {
+                        --q;
+                        rem += (int)od.z;
+                    } else if (rem >= (int)od.z) {
+                        ++q;
+                        rem -= (int)od.z;
+                    }
+                    const uint32_t ty = (od.y >> 16) + q;
+                    const uint32_t tx = (od.y & 0xFFFFu) + (uint32_t)rem;
+                    const int2 kb = BATCH ? s_key[wave][owner] : make_int2(0, tiles_x_plain);
+                    const uint32_t key = (uint32_t)kb.x + ty * (uint32_t)kb.y + tx;
+                    tile_out[slot] = key;
+                    gid_out[slot] = od.x;
+                    if (fc.counts) atomicAdd(&s_hist[wave][(key >> fc.shift) & dmask], 1u);
+                }
+            }
+            pos = seg > pos ? seg : pos;
+            cur += 64u;
+            wave_lds_sync();    // the next window overwrites s_ent / s_key
         }
-        pos = seg > pos ? seg : pos;
-        cur += 64u;
-        wave_lds_sync();    // the next window overwrites s_ent / s_key
+    }
+    if (fc.counts) {
+        __syncthreads();
+        const int d = threadIdx.x;
+        fc.counts[(size_t)d * fc.parts + blockIdx.x] =
+            (s_hist[0][d] + s_hist[1][d]) + (s_hist[2][d] + s_hist[3][d]);
     }
 }
 
@@ -438,7 +479,8 @@ hipError_t emit(int n, const int* n_dev, int tiles_x_plain, const ViewRec* views
                 const uint32_t* vkeys,
                 const uint32_t* perm, const uint2* rect_by_id, int cap, uint8_t* b,
                 const EmitLayout& L, int* info, unsigned long long* total, uint32_t* err,
-                uint32_t* tile_out, uint32_t* gid_out, uint32_t* first_by_id, hipStream_t s) {
+                uint32_t* tile_out, uint32_t* gid_out, uint32_t* first_by_id,
+                const FirstCounts& fc, hipStream_t s) {
     const int parts = (n + kEmitPart - 1) / kEmitPart;
     auto P = [&](size_t off) { return b + off; };
     uint32_t* part_tot = reinterpret_cast<uint32_t*>(P(L.part_tot));
@@ -466,8 +508,9 @@ hipError_t emit(int n, const int* n_dev, int tiles_x_plain, const ViewRec* views
     if ((e = hipGetLastError()) != hipSuccess) return e;
     const long long chunks = ((long long)cap + 63) / 64;
     const int waves = (int)(chunks < kExpandWaves ? chunks : kExpandWaves);
-    emit_expand_kernel<BATCH><<<(waves + 3) / 4, 256, 0, s>>>(tiles_x_plain, E, info, tile_out,
-                                                              gid_out);
+    const int blocks = fc.counts ? fc.parts : (waves + 3) / 4;
+    emit_expand_kernel<BATCH><<<blocks > 0 ? blocks : 1, 256, 0, s>>>(tiles_x_plain, E, info,
+                                                                      tile_out, gid_out, fc);
     return hipGetLastError();
 }
 
@@ -582,16 +625,18 @@ hipError_t bin_emit(const BinPlan& p, const Camera& c, int num_isect_host,
     // cum_tiles_hit) then never sorts unwritten slots, and one too small is
     // an overflow recorded in the error word
     int* written = reinterpret_cast<int*>(b + L.emit.info);
+    // the expansion also counts the tile sort's first digit per sort partition
+    const FirstCounts fc = first_counts(p.tile_sort, b + L.tile_sort);
     e = emit<false>(p.n, reinterpret_cast<const int*>(d + D.n_kept), c.tiles_x, nullptr,
                     nullptr, perm, rect_by_id, p.isect_cap, b, L.emit, written,
                     reinterpret_cast<unsigned long long*>(b + L.total), err, U(L.tiles),
-                    U(L.gids), nullptr, s);
+                    U(L.gids), nullptr, fc, s);
     if (e != hipSuccess) return e;
     stage_mark(kMarkEmitted, s);
     uint32_t* tiles_out = tiles_sorted ? reinterpret_cast<uint32_t*>(tiles_sorted) : U(L.tiles_sorted);
     e = sort_pairs(p.tile_sort, U(L.tiles), U(L.gids), tiles_out,
                    reinterpret_cast<uint32_t*>(gids_sorted), U(L.ttmp_k), U(L.ttmp_v),
-                   num_isect_host, written, b + L.tile_sort, err, s);
+                   num_isect_host, written, b + L.tile_sort, err, s, nullptr, fc.counts != nullptr);
     if (e != hipSuccess) return e;
     stage_mark(kMarkTileSorted, s);
     const int nb = (int)(((long long)p.isect_cap + 256 * kTileBinsPer - 1) / (256 * kTileBinsPer));
@@ -731,14 +776,15 @@ hipError_t batch_bin(const BatchBinPlan& p, const ViewRec* views, const uint2* i
         order = U(L.perm2);
         order_views = U(L.vsorted);
     }
+    const FirstCounts fc = first_counts(p.tile_sort, b + L.tile_sort);
     e = emit<true>(p.n_items, kept, 0, views, order_views, order, item_rects, p.isect_cap, b,
                    L.emit, count,
                    reinterpret_cast<unsigned long long*>(b + L.total), err, U(L.keys),
-                   U(L.vals), item_first, s);
+                   U(L.vals), item_first, fc, s);
     if (e != hipSuccess) return e;
     e = sort_pairs(p.tile_sort, U(L.keys), U(L.vals), U(L.keys_sorted),
                    reinterpret_cast<uint32_t*>(items_sorted), U(L.ttmp_k), U(L.ttmp_v), 0, count,
-                   b + L.tile_sort, err, s);
+                   b + L.tile_sort, err, s, nullptr, fc.counts != nullptr);
     if (e != hipSuccess) return e;
     const int nb = (int)(((long long)p.isect_cap + 256 * kTileBinsPer - 1) / (256 * kTileBinsPer));
     tile_bins_kernel<<<nb > 0 ? nb : 1, 256, 0, s>>>(U(L.keys_sorted), 0, count, p.num_bins,

@@ -639,6 +639,54 @@ int jaxsplat_rasterize_backward_staged(
     return JAXSPLAT_OK;
 }
 
+int jaxsplat_render_backward(
+    const jaxsplat_descriptor* d, const int32_t* gaussian_ids_sorted, const int32_t* tile_bins,
+    uint32_t num_intersects, const float* means3d, const float* scales, const float* quats,
+    const float* viewmat, const float* xys, const float* conics, const float* colors,
+    const float* opacities, const float* background, uint32_t channels, const int32_t* radii,
+    const uint32_t* tile_rects, const int32_t* cum_tiles_hit, const float* final_Ts,
+    const int32_t* final_idx, const float* v_out_img, float* v_colors, float* v_opacity,
+    float* v_mean3d, float* v_scale, float* v_quat, int adjoint, void* workspace,
+    size_t workspace_bytes, void* stream) {
+    g_err.clear();
+    if (int st = validate(d)) return st;
+    if (channels < 1 || channels > 3)
+        return fail(JAXSPLAT_ERR_INVALID, "render_backward: channels must be in [1, 3], got %u",
+                    channels);
+    if (num_intersects >= (1u << 30))
+        return fail(JAXSPLAT_ERR_CAPACITY, "num_intersects %u exceeds 2^30", num_intersects);
+    if (adjoint != JAXSPLAT_ADJOINT_REFERENCE && adjoint != JAXSPLAT_ADJOINT_CONSISTENT)
+        return fail(JAXSPLAT_ERR_INVALID, "unknown adjoint mode %d", adjoint);
+    if (d->num_points == 0) return JAXSPLAT_OK;
+    if (!v_colors || !v_opacity || !v_mean3d || !v_scale || !v_quat)
+        return fail(JAXSPLAT_ERR_INVALID, "render_backward: NULL gradient pointer");
+    if (!means3d || !scales || !quats || !viewmat || !xys || !radii || !tile_rects ||
+        !cum_tiles_hit || !conics ||
+        (num_intersects > 0 && (!gaussian_ids_sorted || !tile_bins || !colors || !opacities ||
+                                !background || !final_Ts || !final_idx || !v_out_img)))
+        return fail(JAXSPLAT_ERR_INVALID, "render_backward: NULL required pointer");
+    const size_t need = align256(jsa::raster_bwd_staged_bytes((int)num_intersects));
+    if (!workspace || workspace_bytes < need)
+        return fail(JAXSPLAT_ERR_WORKSPACE, "render_backward: workspace too small (%zu < %zu)",
+                    workspace_bytes, need);
+    const jsa::Camera c = camera_of(d);
+    const uint2* rects = reinterpret_cast<const uint2*>(tile_rects);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const int consistent = adjoint == JAXSPLAT_ADJOINT_CONSISTENT;
+    JSA_CHECK_HIP(jsa::raster_bwd_staged(c, (int)channels, gaussian_ids_sorted, tile_bins,
+                                         (int)num_intersects, xys, conics, colors, opacities,
+                                         background, final_Ts, final_idx, v_out_img, nullptr,
+                                         radii, rects, cum_tiles_hit, workspace, nullptr, nullptr,
+                                         nullptr, nullptr, nullptr, consistent, s, true),
+                  "render_backward (records)");
+    JSA_CHECK_HIP(jsa::render_bwd_tail(c, (int)channels, (int)num_intersects, means3d, scales,
+                                       quats, viewmat, radii, rects, cum_tiles_hit, conics,
+                                       workspace, v_colors, v_opacity, v_mean3d, v_scale, v_quat,
+                                       consistent, s),
+                  "render_backward (tail)");
+    return JAXSPLAT_OK;
+}
+
 int jaxsplat_rasterize_backward_staged_rects(
     const jaxsplat_descriptor* d, const int32_t* gaussian_ids_sorted, const int32_t* tile_bins,
     uint32_t num_intersects, const float* xys, const float* conics, const float* colors,

@@ -193,6 +193,108 @@ __device__ __forceinline__ uint2 opacity_tile_rect(int bw, int tiles_x, int tile
 }
 
 // ---------------------------------------------------------------------------
+// Staged rasterize-backward records (raster.hip writes them; the reduction
+// kernels in raster.hip and project.hip sum them)
+// ---------------------------------------------------------------------------
+// A record is three float4 per intersection slot: (rgb 3, conic a),
+// (conic b, c, xy 2), (opacity, xy_abs 2, -); bits holds one bit per slot,
+// set when the slot's record was written.
+// Sum of the flagged staged records of slots [off, end) (empty: off == end),
+// in slot order, into (a0, a1, a2) of the calling lane.  Ranges of at most 32
+// slots are summed by their own lane: the bitmap words covering the range,
+// four independent loads per step (bits outside the range masked off), the set
+// bits taken four records at a time with their loads in flight together (a
+// missing fourth reads record 0 and is discarded).  Longer ranges are summed
+// one at a time by the whole wave -- lane l takes slots off + l, off + l + 64,
+// ... -- and combined in a fixed tree, so that one large Gaussian does not hold
+// its wave for tens of rounds; the results stay independent of scheduling.
+// Called by every lane of the wave.
+template <bool ABS>
+__device__ __forceinline__ void sum_records(int off, int end, const float4* __restrict__ rec,
+                                            const uint32_t* __restrict__ bits, int lane,
+                                            float4& a0, float4& a1, float4& a2) {
+    constexpr int kBig = 32;
+    const bool big = end - off > kBig;
+    const int cnt = end - off;
+    if (cnt > 0 && !big) {
+        const int wfirst = off >> 5, wlast = (end - 1) >> 5;
+        const uint32_t lo_mask = ~0u << (off & 31);
+        const int hb = (end - 1) & 31;
+        const uint32_t hi_mask = hb == 31 ? ~0u : ((2u << hb) - 1u);
+        for (int w0 = wfirst; w0 <= wlast; w0 += 4) {
+            uint32_t W[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int w = w0 + q;
+                uint32_t v = w <= wlast ? bits[w] : 0u;
+                if (w == wfirst) v &= lo_mask;
+                if (w == wlast) v &= hi_mask;
+                W[q] = v;
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                uint32_t v = W[q];
+                const int kb = (w0 + q) << 5;
+                while (v) {
+                    int k[4];
+                    bool ok[4];
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        ok[i] = v != 0u;
+                        k[i] = ok[i] ? kb + __ffs(v) - 1 : 0;
+                        v &= v - 1u;
+                    }
+                    float4 r[4][3];
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        r[i][0] = rec[3 * (size_t)k[i] + 0];
+                        r[i][1] = rec[3 * (size_t)k[i] + 1];
+                        r[i][2] = rec[3 * (size_t)k[i] + 2];
+                    }
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+                        const float4 q0 = ok[i] ? r[i][0] : z, q1 = ok[i] ? r[i][1] : z,
+                                     q2 = ok[i] ? r[i][2] : z;
+                        a0.x += q0.x; a0.y += q0.y; a0.z += q0.z; a0.w += q0.w;
+                        a1.x += q1.x; a1.y += q1.y; a1.z += q1.z; a1.w += q1.w;
+                        a2.x += q2.x; a2.y += q2.y; a2.z += q2.z;
+                    }
+                }
+            }
+        }
+    }
+    uint64_t bigm = ballot(big);
+    while (bigm) {
+        const int src = __ffsll((unsigned long long)bigm) - 1;
+        bigm &= bigm - 1;
+        const int boff = __builtin_amdgcn_readlane(off, src);
+        const int bend = __builtin_amdgcn_readlane(end, src);
+        float4 b0 = make_float4(0.f, 0.f, 0.f, 0.f), b1 = b0, b2 = b0;
+        for (int k = boff + lane; k < bend; k += 64) {
+            if (!((bits[k >> 5] >> (k & 31)) & 1u)) continue;
+            const float4 r0 = rec[3 * (size_t)k + 0], r1 = rec[3 * (size_t)k + 1],
+                         r2 = rec[3 * (size_t)k + 2];
+            b0.x += r0.x; b0.y += r0.y; b0.z += r0.z; b0.w += r0.w;
+            b1.x += r1.x; b1.y += r1.y; b1.z += r1.z; b1.w += r1.w;
+            b2.x += r2.x; b2.y += r2.y; b2.z += r2.z;
+        }
+        float v[11] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w, b2.x, b2.y, b2.z};
+#pragma unroll
+        for (int i = 0; i < 11; ++i) {
+            if (!ABS && i >= 9) continue;    // xy_abs sums only with ABS
+            v[i] = wave_sum(v[i]);
+        }
+        if (lane == src) {
+            a0 = make_float4(v[0], v[1], v[2], v[3]);
+            a1 = make_float4(v[4], v[5], v[6], v[7]);
+            a2 = make_float4(v[8], ABS ? v[9] : 0.f, ABS ? v[10] : 0.f, 0.f);
+        }
+    }
+}
+
+
+// ---------------------------------------------------------------------------
 // Host-side internal interface (implemented in the .hip files)
 // ---------------------------------------------------------------------------
 struct Camera {
@@ -297,15 +399,35 @@ hipError_t project_bwd(const Camera& c, const float* means3d, const float* scale
                        float* v_quat, float* v_cov2d, float* v_cov3d, int consistent,
                        hipStream_t s);
 
+// Backward tail of the fused render (projection, binning into the
+// opacity-limited rectangles, rasterization): per Gaussian, the sum of its
+// staged rasterize records (ws of raster_bwd_staged with records_only, over
+// num_isect slots of the rectangles rects / inclusive counts cum) gives
+// v_colors (nch channels), v_opacity, and, in registers, v_xy and v_conic,
+// whose projection VJP gives v_mean3d, v_scale and v_quat; every row written.
+hipError_t render_bwd_tail(const Camera& c, int nch, int num_isect, const float* means3d,
+                           const float* scales, const float* quats, const float* viewmat,
+                           const int* radii, const uint2* rects, const int* cum,
+                           const float* conics, const void* ws, float* v_colors,
+                           float* v_opacity, float* v_mean3d, float* v_scale, float* v_quat,
+                           int consistent, hipStream_t s);
+
 // ---- LSD radix sort (uint32 keys, uint32 values) ----
 struct SortPlan {
     int n_cap;          // capacity (upper bound on n) used for launch shapes
     int begin_bit, end_bit;
     int passes;
     int parts;          // partitions per pass at capacity
+    int tile;           // keys per partition
+    int first_shift, first_bits;   // digit of the first pass
     size_t scratch_bytes;
 };
 SortPlan sort_plan(int n_cap, int begin_bit, int end_bit);
+// The first pass's per-partition digit counts, digit-major (counts[d * parts +
+// part], 256 digits), inside the plan's scratch: a producer of the keys may
+// fill them itself and pass counted = true to sort_pairs, which then skips
+// that pass's counting kernel.
+uint32_t* sort_first_counts(const SortPlan& p, void* scratch);
 // Sorts (keys_in, vals_in) -> (keys_out, vals_out) stably on [begin, end).
 // vals_in == nullptr means values = index.  n is read from n_dev when non-null.
 // tmp_keys/tmp_vals must hold n_cap entries (ping-pong buffers).  The scratch
@@ -317,7 +439,8 @@ size_t sort_zero_bytes(const SortPlan& p);
 hipError_t sort_pairs(const SortPlan& p, const uint32_t* keys_in, const uint32_t* vals_in,
                       uint32_t* keys_out, uint32_t* vals_out, uint32_t* tmp_keys,
                       uint32_t* tmp_vals, int n_host, const int* n_dev, void* scratch,
-                      uint32_t* err, hipStream_t s, int* n_kept = nullptr);
+                      uint32_t* err, hipStream_t s, int* n_kept = nullptr,
+                      bool counted = false);
 
 // ---- binning ----
 struct BinPlan {
@@ -424,7 +547,8 @@ hipError_t raster_fwd_grid(const RasterGrid& g, const int* items_sorted, const i
 // radii and cum_tiles_hit are the projection's.  Writes every gradient row.
 size_t raster_bwd_staged_bytes(int num_isect);
 // rects (optional, packed as pack_rect) replaces the rectangles computed from
-// xys and radii (and must be the ones the binning used).
+// xys and radii (and must be the ones the binning used).  records_only: stop
+// after the records and their bitmap (in ws); render_bwd_tail sums them.
 hipError_t raster_bwd_staged(const Camera& c, int nch, const int* gids_sorted,
                              const int* tile_bins, int num_isect, const float* xys,
                              const float* conics, const float* colors, const float* opacities,
@@ -433,7 +557,8 @@ hipError_t raster_bwd_staged(const Camera& c, int nch, const int* gids_sorted,
                              const int* radii, const uint2* rects, const int* cum_tiles_hit,
                              void* ws,
                              float* v_colors, float* v_opacity, float* v_xy, float* v_xy_abs,
-                             float* v_conic, int consistent, hipStream_t s);
+                             float* v_conic, int consistent, hipStream_t s,
+                             bool records_only = false);
 // Batch backward with staged gradients (3 channels): records at the slots
 // item_first[item] + the tile's row-major offset in the item's rectangle
 // (item_rects, the ones the binning used; workspace raster_bwd_staged_bytes(cap)),

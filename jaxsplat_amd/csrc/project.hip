@@ -637,6 +637,67 @@ __global__ __launch_bounds__(256) void project_bwd_kernel(
     }
 }
 
+// Backward tail of the fused render: per Gaussian, the sum of its staged
+// rasterize records in slot order (raster_bwd_reduce_kernel's sum, over the
+// slots [cum - count, cum) of its opacity-limited rectangle) gives v_colors and
+// v_opacity, written, and v_xy and v_conic, kept in registers for the
+// projection VJP of project_bwd_kernel (no depth or compensation cotangent).
+// Saves writing the per-Gaussian v_xy / v_conic rows and reading them back
+// in a second launch.  No early exit: the wave-wide pass of sum_records needs
+// every lane.
+__global__ __launch_bounds__(256) void render_bwd_tail_kernel(
+    Camera c, int num_isect, const float* __restrict__ means3d, const float* __restrict__ scales,
+    const float* __restrict__ quats, const float* __restrict__ viewmat,
+    const int* __restrict__ radii, const uint2* __restrict__ rects, const int* __restrict__ cum,
+    const float* __restrict__ conics, const float4* __restrict__ rec,
+    const uint32_t* __restrict__ bits, int nch, float* __restrict__ v_colors,
+    float* __restrict__ v_opacity, float* __restrict__ v_mean3d, float* __restrict__ v_scale,
+    float* __restrict__ v_quat, int consistent) {
+    const int idx = blockIdx.x * 256 + threadIdx.x;
+    const int lane = lane_id();
+    const bool own = idx < c.num_points;
+    const bool vis = own && radii[idx] > 0;
+    float4 a0 = make_float4(0.f, 0.f, 0.f, 0.f), a1 = a0, a2 = a0;
+    int off = 0, end = 0;
+    if (vis) {
+        int x0, x1, y0, y1;
+        unpack_rect(rects[idx], x0, x1, y0, y1);
+        end = cum[idx];
+        off = end - (x1 - x0) * (y1 - y0);
+        if (off < 0 || end > num_isect) off = end = 0;     // inconsistent input: zeros
+    }
+    sum_records<false>(off, end, rec, bits, lane, a0, a1, a2);
+    if (!own) return;
+    float* vcol = v_colors + (size_t)nch * idx;
+    vcol[0] = a0.x;
+    if (nch > 1) vcol[1] = a0.y;
+    if (nch > 2) vcol[2] = a0.z;
+    v_opacity[idx] = a2.x;
+    float vmean[3] = {0.f, 0.f, 0.f}, vsc[3] = {0.f, 0.f, 0.f}, vq[4] = {0.f, 0.f, 0.f, 0.f};
+    if (vis) {                                                  // backward.cu:369-371
+        M3 W;
+        float pt[3];
+        load_view(viewmat, W, pt);
+        const float m[3] = {means3d[3 * idx + 0], means3d[3 * idx + 1], means3d[3 * idx + 2]};
+        const float sc[3] = {scales[3 * idx + 0], scales[3 * idx + 1], scales[3 * idx + 2]};
+        const float4 q4 = reinterpret_cast<const float4*>(quats)[idx];
+        const float q[4] = {q4.x, q4.y, q4.z, q4.w};
+        float cov[6];
+        cov3d_of(sc, q, c.glob_scale, cov);
+        float vc2[3], vc3[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        view_vjp(c, W, pt, m, cov, conics[3 * idx + 0], conics[3 * idx + 1], conics[3 * idx + 2],
+                 make_float2(a1.z, a1.w), 0.f, a0.w, a1.x, a1.y, false, 0.f, 0.f, consistent,
+                 vmean, vc2, vc3);
+        cov3d_vjp(sc, q, c.glob_scale, vc3, vsc, vq);
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        v_mean3d[3 * idx + k] = vmean[k];
+        v_scale[3 * idx + k] = vsc[k];
+    }
+    reinterpret_cast<float4*>(v_quat)[idx] = make_float4(vq[0], vq[1], vq[2], vq[3]);
+}
+
 // Batched backward: blockIdx.y = scene, one lane per Gaussian; the views of
 // the scene are walked in order and their v_mean3d / v_cov3d contributions
 // summed in registers (the scale/rotation VJP is linear in v_cov3d, so it
@@ -849,6 +910,27 @@ hipError_t project_bwd(const Camera& c, const float* means3d, const float* scale
                                               compensation, v_xy, v_depth, v_conic,
                                               v_compensation, v_mean3d, v_scale, v_quat, v_cov2d,
                                               v_cov3d, consistent);
+    return hipGetLastError();
+}
+
+hipError_t render_bwd_tail(const Camera& c, int nch, int num_isect, const float* means3d,
+                           const float* scales, const float* quats, const float* viewmat,
+                           const int* radii, const uint2* rects, const int* cum,
+                           const float* conics, const void* ws, float* v_colors,
+                           float* v_opacity, float* v_mean3d, float* v_scale, float* v_quat,
+                           int consistent, hipStream_t s) {
+    if (c.num_points <= 0) return hipSuccess;
+    if (nch < 1 || nch > 3) return hipErrorInvalidValue;
+    // the layout of raster_bwd_staged's workspace: records, then the bitmap
+    const uint8_t* w = static_cast<const uint8_t*>(ws);
+    const size_t n = (size_t)(num_isect > 0 ? num_isect : 1);
+    const float4* rec = reinterpret_cast<const float4*>(w);
+    const uint32_t* bits = reinterpret_cast<const uint32_t*>(w + ((48 * n + 255) & ~size_t(255)));
+    const int blocks = (c.num_points + 255) / 256;
+    render_bwd_tail_kernel<<<blocks, 256, 0, s>>>(c, num_isect, means3d, scales, quats, viewmat,
+                                                  radii, rects, cum, conics, rec, bits, nch,
+                                                  v_colors, v_opacity, v_mean3d, v_scale,
+                                                  v_quat, consistent);
     return hipGetLastError();
 }
 

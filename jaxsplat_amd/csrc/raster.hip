@@ -888,100 +888,6 @@ __global__ __launch_bounds__(128) void raster_bwd_queue_kernel(
     }
 }
 
-// Sum of the flagged staged records of slots [off, end) (empty: off == end),
-// in slot order, into (a0, a1, a2) of the calling lane.  Ranges of at most 32
-// slots are summed by their own lane: the bitmap words covering the range,
-// four independent loads per step (bits outside the range masked off), the set
-// bits taken four records at a time with their loads in flight together (a
-// missing fourth reads record 0 and is discarded).  Longer ranges are summed
-// one at a time by the whole wave -- lane l takes slots off + l, off + l + 64,
-// ... -- and combined in a fixed tree, so that one large Gaussian does not hold
-// its wave for tens of rounds; the results stay independent of scheduling.
-// Called by every lane of the wave.
-template <bool ABS>
-__device__ __forceinline__ void sum_records(int off, int end, const float4* __restrict__ rec,
-                                            const uint32_t* __restrict__ bits, int lane,
-                                            float4& a0, float4& a1, float4& a2) {
-    constexpr int kBig = 32;
-    const bool big = end - off > kBig;
-    const int cnt = end - off;
-    if (cnt > 0 && !big) {
-        const int wfirst = off >> 5, wlast = (end - 1) >> 5;
-        const uint32_t lo_mask = ~0u << (off & 31);
-        const int hb = (end - 1) & 31;
-        const uint32_t hi_mask = hb == 31 ? ~0u : ((2u << hb) - 1u);
-        for (int w0 = wfirst; w0 <= wlast; w0 += 4) {
-            uint32_t W[4];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int w = w0 + q;
-                uint32_t v = w <= wlast ? bits[w] : 0u;
-                if (w == wfirst) v &= lo_mask;
-                if (w == wlast) v &= hi_mask;
-                W[q] = v;
-            }
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                uint32_t v = W[q];
-                const int kb = (w0 + q) << 5;
-                while (v) {
-                    int k[4];
-                    bool ok[4];
-#pragma unroll
-                    for (int i = 0; i < 4; ++i) {
-                        ok[i] = v != 0u;
-                        k[i] = ok[i] ? kb + __ffs(v) - 1 : 0;
-                        v &= v - 1u;
-                    }
-                    float4 r[4][3];
-#pragma unroll
-                    for (int i = 0; i < 4; ++i) {
-                        r[i][0] = rec[3 * (size_t)k[i] + 0];
-                        r[i][1] = rec[3 * (size_t)k[i] + 1];
-                        r[i][2] = rec[3 * (size_t)k[i] + 2];
-                    }
-#pragma unroll
-                    for (int i = 0; i < 4; ++i) {
-                        const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
-                        const float4 q0 = ok[i] ? r[i][0] : z, q1 = ok[i] ? r[i][1] : z,
-                                     q2 = ok[i] ? r[i][2] : z;
-                        a0.x += q0.x; a0.y += q0.y; a0.z += q0.z; a0.w += q0.w;
-                        a1.x += q1.x; a1.y += q1.y; a1.z += q1.z; a1.w += q1.w;
-                        a2.x += q2.x; a2.y += q2.y; a2.z += q2.z;
-                    }
-                }
-            }
-        }
-    }
-    uint64_t bigm = ballot(big);
-    while (bigm) {
-        const int src = __ffsll((unsigned long long)bigm) - 1;
-        bigm &= bigm - 1;
-        const int boff = __builtin_amdgcn_readlane(off, src);
-        const int bend = __builtin_amdgcn_readlane(end, src);
-        float4 b0 = make_float4(0.f, 0.f, 0.f, 0.f), b1 = b0, b2 = b0;
-        for (int k = boff + lane; k < bend; k += 64) {
-            if (!((bits[k >> 5] >> (k & 31)) & 1u)) continue;
-            const float4 r0 = rec[3 * (size_t)k + 0], r1 = rec[3 * (size_t)k + 1],
-                         r2 = rec[3 * (size_t)k + 2];
-            b0.x += r0.x; b0.y += r0.y; b0.z += r0.z; b0.w += r0.w;
-            b1.x += r1.x; b1.y += r1.y; b1.z += r1.z; b1.w += r1.w;
-            b2.x += r2.x; b2.y += r2.y; b2.z += r2.z;
-        }
-        float v[11] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w, b2.x, b2.y, b2.z};
-#pragma unroll
-        for (int i = 0; i < 11; ++i) {
-            if (!ABS && i >= 9) continue;    // xy_abs sums only with ABS
-            v[i] = wave_sum(v[i]);
-        }
-        if (lane == src) {
-            a0 = make_float4(v[0], v[1], v[2], v[3]);
-            a1 = make_float4(v[4], v[5], v[6], v[7]);
-            a2 = make_float4(v[8], ABS ? v[9] : 0.f, ABS ? v[10] : 0.f, 0.f);
-        }
-    }
-}
-
 // Per Gaussian: the sum of its flagged staged records, in slot order, as the
 // final gradients (every output row is written; a Gaussian with no flagged
 // record gets zeros).  num_isect bounds the slots (a radius or cum_tiles_hit
@@ -1307,7 +1213,7 @@ hipError_t raster_bwd_staged(const Camera& c, int nch, const int* gids_sorted,
                              const int* radii, const uint2* rects, const int* cum_tiles_hit,
                              void* ws,
                              float* v_colors, float* v_opacity, float* v_xy, float* v_xy_abs,
-                             float* v_conic, int consistent, hipStream_t s) {
+                             float* v_conic, int consistent, hipStream_t s, bool records_only) {
     if (c.num_points <= 0) return hipSuccess;
     if (nch < 1 || nch > 3) return hipErrorInvalidValue;
     uint8_t* w = static_cast<uint8_t*>(ws);
@@ -1351,6 +1257,7 @@ hipError_t raster_bwd_staged(const Camera& c, int nch, const int* gids_sorted,
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     stage_mark(kMarkRecords, s);
+    if (records_only) return hipSuccess;
     const int blocks = (c.num_points + 255) / 256;
     const float2* x2 = reinterpret_cast<const float2*>(xys);
     if (v_xy_abs)

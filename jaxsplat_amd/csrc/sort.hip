@@ -275,7 +275,12 @@ SortPlan sort_plan(int n_cap, int begin_bit, int end_bit) {
     p.end_bit = end_bit > begin_bit ? end_bit : begin_bit;
     p.passes = (p.end_bit - p.begin_bit + kDigitBits - 1) / kDigitBits;
     const int tile = kSortThreads * items_for(p.n_cap);
+    p.tile = tile;
     p.parts = (p.n_cap + tile - 1) / tile;
+    // the key bits split evenly over the passes (see sort_pairs)
+    const int per = p.passes > 0 ? (p.end_bit - p.begin_bit + p.passes - 1) / p.passes : 0;
+    p.first_shift = p.begin_bit;
+    p.first_bits = min(per, p.end_bit - p.begin_bit);
     // [totals 256][counts 256 * parts]
     p.scratch_bytes = 4ull * kBins + 4ull * kBins * (size_t)p.parts;
     p.scratch_bytes = (p.scratch_bytes + 255) & ~size_t(255);
@@ -284,10 +289,14 @@ SortPlan sort_plan(int n_cap, int begin_bit, int end_bit) {
 
 size_t sort_zero_bytes(const SortPlan& p) { return 0; }
 
+uint32_t* sort_first_counts(const SortPlan& p, void* scratch) {
+    return static_cast<uint32_t*>(scratch) + kBins;
+}
+
 hipError_t sort_pairs(const SortPlan& p, const uint32_t* keys_in, const uint32_t* vals_in,
                       uint32_t* keys_out, uint32_t* vals_out, uint32_t* tmp_keys,
                       uint32_t* tmp_vals, int n_host, const int* n_dev, void* scratch,
-                      uint32_t* err, hipStream_t s, int* n_kept) {
+                      uint32_t* err, hipStream_t s, int* n_kept, bool counted) {
     (void)err;
     if (p.passes <= 0 || p.passes > kMaxPasses) return hipErrorInvalidValue;
     uint32_t* totals = static_cast<uint32_t*>(scratch);
@@ -312,11 +321,15 @@ hipError_t sort_pairs(const SortPlan& p, const uint32_t* keys_in, const uint32_t
         const int drop = (n_kept && q == 0) ? 1 : 0;
         const int* nd = (n_kept && q > 0) ? n_kept : n_dev;
         int* nk = drop ? n_kept : nullptr;
+        // the first pass's counts may come from the keys' producer
+        const bool count_here = !(counted && q == 0 && !drop);
 #define JSA_PASS(IT)                                                                         \
     do {                                                                                     \
-        radix_upsweep<IT><<<p.parts, kSortThreads, 0, s>>>(ki, n_host, nd, shift, bits,      \
-                                                            p.parts, counts, drop);          \
-        if ((e = hipGetLastError()) != hipSuccess) return e;                                 \
+        if (count_here) {                                                                    \
+            radix_upsweep<IT><<<p.parts, kSortThreads, 0, s>>>(ki, n_host, nd, shift, bits,  \
+                                                                p.parts, counts, drop);      \
+            if ((e = hipGetLastError()) != hipSuccess) return e;                             \
+        }                                                                                    \
         radix_colscan<<<kBins, kSortThreads, 0, s>>>(counts, p.parts, totals);               \
         if ((e = hipGetLastError()) != hipSuccess) return e;                                 \
         radix_downsweep<IT><<<p.parts, kSortThreads, 0, s>>>(ki, vi, ko, vo, n_host, nd,     \
