@@ -1,13 +1,13 @@
-"""Depth sort of the binning on the GPU: key ranges that need 1, 2, 3 and 4 passes.
+"""Binning on the GPU over depth-key distributions of different spreads.
 
-The depth stage sorts only the key bits that vary among the Gaussians that
-emit, in digits of at most 9 bits, with the number of passes decided on the
-device from the keys themselves.  These cases craft depth keys whose varying
-bits span each pass count (and the degenerate cases: all keys equal, nothing
-emits, heavy ties), bin them through ``jaxsplat_bin_and_sort`` and the split
-depth + emission entry points, and require the sorted ids, the reconstructed
-64-bit keys and the tile bins to be bit-identical to the oracle's stable
-(tile, depth bits, id) order (``oracle.binning``, reference
+The depth stage sorts the Gaussians that emit by their 32-bit depth keys in
+four 8-bit LSD passes, dropping the rest in the first pass.  These cases craft
+depth keys that vary in 0, 2, 9, 17, 25 and 31 bits (all keys equal, heavy
+ties broken by id, a few octaves of depth, depths over five decades) plus the
+case where nothing emits, bin them through ``jaxsplat_bin_and_sort`` and the
+split depth + emission entry points, and require the sorted ids, the
+reconstructed 64-bit keys and the tile bins to be bit-identical to the
+oracle's stable (tile, depth bits, id) order (``oracle.binning``, reference
 ``lib/ops.cu:722-760``).
 """
 
@@ -30,17 +30,17 @@ def _case(kind, n=20_000, seed=7):
     xys = xys.astype(np.float32)
     radii = rng.integers(1, 20, n).astype(np.int32)
     base = np.array(3.0, np.float32).view(np.uint32)
-    if kind == "equal":                 # no varying bit: the first pass alone
+    if kind == "equal":                 # no varying bit
         bits = np.full(n, base, np.uint32)
     elif kind == "ties":                # few distinct keys, many ties broken by id
         bits = base + rng.integers(0, 4, n).astype(np.uint32)
-    elif kind == "bits9":               # varies in the low 9 bits: one pass
+    elif kind == "bits9":               # varies in the low 9 bits
         bits = base + rng.integers(0, 1 << 9, n).astype(np.uint32)
-    elif kind == "bits17":              # 17 bits: two passes
+    elif kind == "bits17":              # 17 bits
         bits = base + rng.integers(0, 1 << 17, n).astype(np.uint32)
-    elif kind == "octaves":             # depths in [2, 10): 25 bits, three passes
+    elif kind == "octaves":             # depths in [2, 10): 25 bits
         bits = rng.uniform(2.0, 10.0, n).astype(np.float32).view(np.uint32)
-    elif kind == "wide":                # [0.02, 5000): 31 bits, four passes
+    elif kind == "wide":                # [0.02, 5000): 31 bits
         bits = np.exp(rng.uniform(np.log(0.02), np.log(5000.0), n)).astype(
             np.float32).view(np.uint32)
     elif kind == "none":                # nothing emits
@@ -62,7 +62,7 @@ def _case(kind, n=20_000, seed=7):
 
 @pytest.mark.parametrize("split", [False, True])
 @pytest.mark.parametrize("kind", ["equal", "ties", "bits9", "bits17", "octaves", "wide"])
-def test_ranged_depth_sort_bitexact(kind, split):
+def test_depth_key_spreads_bitexact(kind, split):
     xys, depths, radii, cum = _case(kind)
     gids, tiles, bins, isect = H.gpu_bin(xys, depths, radii, cum, img_shape=IMG, block_width=BW,
                                          split=split)
@@ -75,9 +75,9 @@ def test_ranged_depth_sort_bitexact(kind, split):
     assert np.array_equal(bins, bs), kind
 
 
-def test_ranged_depth_sort_nothing_emits():
-    # the split entry points run the depth stage whatever the count: an empty
-    # set of kept keys
+def test_depth_stage_nothing_emits():
+    # the split entry points run the depth stage whatever the count: no key
+    # is kept
     xys, depths, radii, cum = _case("none")
     assert int(cum[-1]) == 0
     gids, tiles, bins, isect = H.gpu_bin(xys, depths, radii, cum, img_shape=IMG, block_width=BW,
