@@ -352,12 +352,15 @@ def stage_breakdown(torch, sc, cam, params, target, iters):
     ws = torch.empty(wsb, dtype=torch.uint8, device=dev)
     s = _stream(dev)
 
+    splats = torch.empty((N, 12), **f32)
+
     def project(opacity_limited):
         if opacity_limited:
-            _lib.check(L.jaxsplat_project_forward_rects(
-                D, _p(means3d), _p(scales), _p(quats), _p(viewmat), _p(opac), None, _p(xys),
-                _p(depths), _p(radii), _p(conics), _p(comp), _p(tnth), _p(tcum), _p(rects),
-                _p(ws), wsb, s))
+            # the render path: opacity-limited rectangles and splat records
+            _lib.check(L.jaxsplat_project_forward_splats(
+                D, _p(means3d), _p(scales), _p(quats), _p(viewmat), _p(opac), _p(colors), 3,
+                _p(xys), _p(depths), _p(radii), _p(conics), None, _p(tnth), _p(tcum),
+                _p(rects), _p(splats), _p(ws), wsb, s))
         else:
             _lib.check(L.jaxsplat_project_forward(
                 D, _p(means3d), _p(scales), _p(quats), _p(viewmat), None, _p(xys), _p(depths),
@@ -401,9 +404,9 @@ def stage_breakdown(torch, sc, cam, params, target, iters):
             _lib.check(L.jaxsplat_bin_emit_sort(D, isect, _p(dws), _p(gids), None, _p(bins),
                                                 _p(ews), ewsb, s))
             ev[3].record()
-            _lib.check(L.jaxsplat_rasterize_forward_binned(
-                D, _p(gids), _p(bins), isect, _p(xys), _p(conics), _p(colors), _p(opac), _p(bg),
-                _p(fT), _p(fI), _p(img), s))
+            _lib.check(L.jaxsplat_rasterize_forward_splats(
+                D, _p(gids), _p(bins), isect, _p(splats), _p(bg), 3, _p(fT), _p(fI), _p(img),
+                s))
             ev[4].record()
             torch.mul(torch.sub(img, target), 2.0 / img.numel(), out=v_img)
             ev[5].record()
@@ -411,9 +414,9 @@ def stage_breakdown(torch, sc, cam, params, target, iters):
             # tail (record sums + projection adjoint)
             _lib.check(L.jaxsplat_render_backward(
                 D, _p(gids), _p(bins), isect, _p(means3d), _p(scales), _p(quats), _p(viewmat),
-                _p(xys), _p(conics), _p(colors), _p(opac), _p(bg), 3, _p(radii), _p(rects),
-                _p(tcum), _p(fT), _p(fI), _p(v_img), _p(vc), _p(vo), _p(vm), _p(vs), _p(vq), 0,
-                _p(bws), bwsb, s))
+                _p(splats), _p(conics), _p(bg), 3, _p(radii), _p(rects), _p(tcum), _p(fT),
+                _p(fI), _p(v_img), _p(vc), _p(vo), _p(vm), _p(vs), _p(vq), 0, _p(bws), bwsb,
+                s))
             ev[6].record()
             torch.cuda.synchronize()
             if it >= 2:
@@ -452,17 +455,22 @@ def stage_breakdown(torch, sc, cam, params, target, iters):
     # ---- algorithmic bytes of each stage of this implementation ----
     # (documented term by term in the design notes; N Gaussians, V visible,
     # E with a non-empty opacity-limited rectangle, I = isect pairs binned,
-    # L_ft / L_bt consumed by the forward / backward, R records flushed)
+    # L_ft / L_bt consumed by the forward / backward, R records flushed).
+    # The projection reads 56 B per Gaussian (means, scales, quats, opacity,
+    # colour) and writes 100 (xys, depth, radius, conic, count, inclusive sum,
+    # rectangle, 48-byte splat record) + 8 of scan; a staged list entry costs
+    # its id and its splat record (52 B), in the backward also its inclusive
+    # sum; a flushed record is 36 B.
     stage_bytes = {
-        "project_fwd": 100 * N,
+        "project_fwd": 164 * N,
         "depth_sort": 104 * N,
         "emission": 32 * N + 32 * E + 8 * isect,
         "tile_sort": 40 * isect,
         "bin_edges": 4 * isect + 8 * T,
-        "raster_fwd": 40 * L_ft + 8 * T + 20 * Px,
+        "raster_fwd": 52 * L_ft + 8 * T + 20 * Px,
         "loss_grad": 36 * Px,
-        "raster_bwd_records": 40 * L_bt + 8 * T + 20 * Px + 48 * R + (isect + 7) // 8,
-        "bwd_tail": 48 * R + (isect + 7) // 8 + 16 * N + 56 * V + 4 * (N - V) + 40 * N,
+        "raster_bwd_records": 56 * L_bt + 8 * T + 20 * Px + 36 * R + (isect + 7) // 8,
+        "bwd_tail": 36 * R + (isect + 7) // 8 + 16 * N + 56 * V + 4 * (N - V) + 40 * N,
     }
     valu_bound = {"raster_fwd", "raster_bwd_records"}
     stages = {}

@@ -325,7 +325,7 @@ int jaxsplat_rasterize_backward_nd(
  * one lane per Gaussian sums its records in slot order.  The result does not
  * depend on scheduling (bitwise reproducible) and every gradient row is
  * written.  radii and cum_tiles_hit are the projection's outputs for the same
- * xys; the workspace holds a 48-byte record and one bit per intersection
+ * xys; the workspace holds 48 bytes of record and one bit per intersection
  * (jaxsplat_rasterize_backward_staged_workspace_size).
  */
 size_t jaxsplat_rasterize_backward_staged_workspace_size(const jaxsplat_descriptor* d,
@@ -351,28 +351,52 @@ int jaxsplat_rasterize_backward_staged_rects(
     void* workspace, size_t workspace_bytes, void* stream);
 
 /*
- * Backward of the fused render (jaxsplat_project_forward_rects, binning into
- * those opacity-limited rectangles, jaxsplat_rasterize_forward_binned); for
- * the reference's render (jaxsplat/__init__.py:6-70), whose backward is the
- * rasterize adjoint (lib/kernels/backward.cu:153-343) followed by the
- * projection adjoint (backward.cu:345-570).  The rasterize adjoint writes the
- * staged records of jaxsplat_rasterize_backward_staged_rects into the
- * workspace; one per-Gaussian pass then sums them and runs the projection
- * VJP in registers, writing v_colors (N, channels), v_opacity (N),
- * v_mean3d (N,3), v_scale (N,3) and v_quat (N,4), every row.  Results equal
- * jaxsplat_rasterize_backward_staged_rects followed by
- * jaxsplat_project_backward bit for bit.  d: the projection's descriptor
- * (its intrinsics); workspace: jaxsplat_rasterize_backward_staged_workspace_size.
+ * The fused render (no counterpart as one call in the reference, whose render,
+ * jaxsplat/__init__.py:6-70, is project then rasterize): the projection with
+ * the opacity-limited rectangles also writes one 48-byte splat record per
+ * Gaussian (N, 12 floats): (x, y, opacity, log2(255 opacity)), (conic a, b, c,
+ * rectangle word 0), (colour, zero-padded to 3 channels, rectangle word 1);
+ * the rasterize forward and backward stage their list entries from these
+ * records instead of gathering xys, conics, colours, opacities and
+ * rectangles separately.  Results are those of jaxsplat_project_forward_rects
+ * + jaxsplat_rasterize_forward_binned / the backward below, bit for bit.
+ */
+int jaxsplat_project_forward_splats(const jaxsplat_descriptor* d, const float* means3d,
+                                    const float* scales, const float* quats,
+                                    const float* viewmat, const float* opacities,
+                                    const float* colors, uint32_t channels, float* xys,
+                                    float* depths, int32_t* radii, float* conics,
+                                    float* compensation, int32_t* num_tiles_hit,
+                                    int32_t* cum_tiles_hit, uint32_t* tile_rects, float* splats,
+                                    void* workspace, size_t workspace_bytes, void* stream);
+/* Rasterize forward of the fused render; replaces rasterize_fwd
+ * (lib/kernels/forward.cu:339-462) for channels in [1, 3]. */
+int jaxsplat_rasterize_forward_splats(const jaxsplat_descriptor* d,
+                                      const int32_t* gaussian_ids_sorted,
+                                      const int32_t* tile_bins, uint32_t num_intersects,
+                                      const float* splats, const float* background,
+                                      uint32_t channels, float* final_Ts, int32_t* final_idx,
+                                      float* out_img, void* stream);
+/*
+ * Backward of the fused render: the rasterize adjoint (lib/kernels/
+ * backward.cu:153-343) writes the staged records of
+ * jaxsplat_rasterize_backward_staged_rects into the workspace, staging from
+ * the splat records; one per-Gaussian pass then sums them and runs the
+ * projection VJP (backward.cu:345-570) in registers, writing v_colors
+ * (N, channels), v_opacity (N), v_mean3d (N,3), v_scale (N,3) and v_quat
+ * (N,4), every row.  Results equal jaxsplat_rasterize_backward_staged_rects
+ * followed by jaxsplat_project_backward bit for bit.  d: the projection's
+ * descriptor (its intrinsics); workspace:
+ * jaxsplat_rasterize_backward_staged_workspace_size.
  */
 int jaxsplat_render_backward(
     const jaxsplat_descriptor* d, const int32_t* gaussian_ids_sorted, const int32_t* tile_bins,
     uint32_t num_intersects, const float* means3d, const float* scales, const float* quats,
-    const float* viewmat, const float* xys, const float* conics, const float* colors,
-    const float* opacities, const float* background, uint32_t channels, const int32_t* radii,
-    const uint32_t* tile_rects, const int32_t* cum_tiles_hit, const float* final_Ts,
-    const int32_t* final_idx, const float* v_out_img, float* v_colors, float* v_opacity,
-    float* v_mean3d, float* v_scale, float* v_quat, int adjoint, void* workspace,
-    size_t workspace_bytes, void* stream);
+    const float* viewmat, const float* splats, const float* conics, const float* background,
+    uint32_t channels, const int32_t* radii, const uint32_t* tile_rects,
+    const int32_t* cum_tiles_hit, const float* final_Ts, const int32_t* final_idx,
+    const float* v_out_img, float* v_colors, float* v_opacity, float* v_mean3d, float* v_scale,
+    float* v_quat, int adjoint, void* workspace, size_t workspace_bytes, void* stream);
 
 /* ------------------------------------------------------------------------
  * Batched rendering: many views of one or more scenes in one pass.

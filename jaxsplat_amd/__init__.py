@@ -52,10 +52,13 @@ def render(means3d, scales, quats, colors, opacities, *, viewmat, background, im
     may be captured in a HIP graph; ``isect_capacity.check()`` reports an
     overflow afterwards.
     """
-    if _ops.TIGHT_TILES and _ops.FUSED_RENDER and not antialiased:
-        # one autograd node: projection with the opacity-limited rectangles,
-        # binning, rasterization; its backward sums the rasterize records and
-        # runs the projection adjoint in one per-Gaussian pass
+    fusable = (isinstance(colors, _ops.torch.Tensor) and colors.dim() == 2 and
+               1 <= colors.shape[1] <= 3)
+    if _ops.TIGHT_TILES and _ops.FUSED_RENDER and not antialiased and fusable:
+        # one autograd node (up to 3 channels): projection with the
+        # opacity-limited rectangles and splat records, binning,
+        # rasterization; its backward sums the rasterize records and runs the
+        # projection adjoint in one per-Gaussian pass
         return _ops.render_fused(means3d, scales, quats, colors, opacities, viewmat=viewmat,
                                  background=background, img_shape=img_shape, f=f, c=c,
                                  glob_scale=glob_scale, clip_thresh=clip_thresh,

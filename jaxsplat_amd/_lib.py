@@ -7,10 +7,14 @@ loudly when it is missing: there is no CPU fallback for any entry point.
 from __future__ import annotations
 
 import ctypes
+import os
 import threading
 from pathlib import Path
 
-LIB_PATH = Path(__file__).resolve().parent / "libjaxsplat_amd.so"
+# JAXSPLAT_AMD_LIB selects another build of the library (e.g. for an A/B
+# comparison of two kernel variants); by default the in-tree build.
+LIB_PATH = Path(os.environ.get("JAXSPLAT_AMD_LIB") or
+                Path(__file__).resolve().parent / "libjaxsplat_amd.so")
 
 OK = 0
 ERR_INVALID = 1
@@ -34,6 +38,7 @@ EXPORTS = (
     "jaxsplat_tile_rects_workspace_size", "jaxsplat_tile_rects", "jaxsplat_bin_depth_sort_rects",
     "jaxsplat_project_forward_rects",
     "jaxsplat_rasterize_backward_staged_rects", "jaxsplat_render_backward",
+    "jaxsplat_project_forward_splats", "jaxsplat_rasterize_forward_splats",
     "jaxsplat_rasterize_forward_binned", "jaxsplat_rasterize_backward_binned",
     "jaxsplat_rasterize_forward_nd", "jaxsplat_rasterize_backward_nd",
     "jaxsplat_batch_create", "jaxsplat_batch_destroy", "jaxsplat_batch_workspace_size",
@@ -125,8 +130,10 @@ _SIGS = {
     "jaxsplat_rasterize_backward_staged_rects": (I32, [DP, P, P, U32, P, P, P, P, P, U32, P, P,
                                                         P, P, P, P, P, P, P, P, P, P, I32, P, SZ,
                                                         P]),
-    "jaxsplat_render_backward": (I32, [DP, P, P, U32] + [P] * 9 + [U32] + [P] * 11
+    "jaxsplat_render_backward": (I32, [DP, P, P, U32] + [P] * 7 + [U32] + [P] * 11
                                  + [I32, P, SZ, P]),
+    "jaxsplat_project_forward_splats": (I32, [DP] + [P] * 6 + [U32] + [P] * 9 + [P, SZ, P]),
+    "jaxsplat_rasterize_forward_splats": (I32, [DP, P, P, U32, P, P, U32, P, P, P, P]),
     "jaxsplat_rasterize_backward_staged": (I32, [DP, P, P, U32, P, P, P, P, P, U32, P, P, P, P, P,
                                                   P, P, P, P, P, P, I32, P, SZ, P]),
     "jaxsplat_rasterize_forward_binned": (I32, [DP, P, P, U32] + [P] * 8 + [P]),

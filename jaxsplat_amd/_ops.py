@@ -513,40 +513,64 @@ class _Rasterize(torch.autograd.Function):
 
 class _Render(torch.autograd.Function):
     """The fused render as one autograd node: the projection with the
-    opacity-limited rectangles, the binning into them and the rasterize
-    forward; its backward is ``jaxsplat_render_backward`` (the staged
-    rasterize records, then one per-Gaussian pass that sums them and runs the
-    projection VJP in registers: bit for bit the rasterize adjoint followed by
-    the projection adjoint, without the v_xy / v_conic rows in between), or,
-    where the staged backward does not apply, those two adjoints."""
+    opacity-limited rectangles, which also writes one 48-byte splat record per
+    Gaussian (position, opacity and its alpha limit, conic, colour,
+    rectangle), the binning into those rectangles and the rasterize forward,
+    staging its entries from the records; the backward is
+    ``jaxsplat_render_backward`` (the staged rasterize records, staged from
+    the splat records, then one per-Gaussian pass that sums them and runs the
+    projection VJP in registers), or, where the staged backward does not
+    apply, the rasterize adjoint followed by the projection adjoint.  Results
+    are those of ``project_for_render`` + ``rasterize`` bit for bit."""
 
     @staticmethod
     def forward(ctx, cfg, means3d, scales, quats, colors, opacities, viewmat, background,
                 capacity):
         img_shape, f, c, glob_scale, clip_thresh, block_width, adjoint = cfg
-        N = means3d.shape[0]
+        N, nch = colors.shape
         dev = means3d.device
+        H, W = img_shape
         d = _lib.make_descriptor(N, img_shape, f, c, glob_scale, clip_thresh, block_width)
-        xys, depths, radii, conics, _nth, cum, comp, rects = _project_forward(
-            d, means3d, scales, quats, viewmat, opacities)
+        L = _lib.lib()
+        xys = torch.empty((N, 2), dtype=torch.float32, device=dev)
+        depths = torch.empty((N, 1), dtype=torch.float32, device=dev)
+        radii = torch.empty((N, 1), dtype=torch.int32, device=dev)
+        conics = torch.empty((N, 3), dtype=torch.float32, device=dev)
+        nth = torch.empty((N, 1), dtype=torch.int32, device=dev)
+        cum = torch.empty((N, 1), dtype=torch.int32, device=dev)
+        rects = torch.empty((N, 2), dtype=torch.int32, device=dev)
+        splats = torch.empty((N, 12), dtype=torch.float32, device=dev)
+        ws_bytes = L.jaxsplat_project_workspace_size(ctypes.byref(d))
+        ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=dev)
+        with torch.cuda.device(dev):
+            _lib.check(L.jaxsplat_project_forward_splats(
+                ctypes.byref(d), _p(means3d), _p(scales), _p(quats), _p(viewmat),
+                _p(opacities), _p(colors), nch, _p(xys), _p(depths), _p(radii), _p(conics),
+                None, _p(nth), _p(cum), _p(rects), _p(splats), _p(ws), ws_bytes,
+                _stream(dev)))
         if N == 0:
             gids, bins, isect = _bin(d, xys, depths, radii, 0, dev)
         else:
             gids, bins, isect = _bin_rects(d, depths, radii, rects, cum, dev, capacity)
-        img, final_Ts, final_idx = _rasterize_forward(d, gids, bins, isect, xys, conics, colors,
-                                                      opacities, background, img_shape)
+        final_Ts = torch.empty((H, W, 1), dtype=torch.float32, device=dev)
+        final_idx = torch.empty((H, W, 1), dtype=torch.int32, device=dev)
+        img = torch.empty((H, W, nch), dtype=torch.float32, device=dev)
+        with torch.cuda.device(dev):
+            _lib.check(L.jaxsplat_rasterize_forward_splats(
+                ctypes.byref(d), _p(gids), _p(bins), isect, _p(splats), _p(background), nch,
+                _p(final_Ts), _p(final_idx), _p(img), _stream(dev)))
         ctx.d = d
         ctx.isect = isect
         ctx.adjoint = adjoint
         ctx.save_for_backward(means3d, scales, quats, viewmat, colors, opacities, background,
-                              xys, conics, radii, comp, cum, rects, gids, bins, final_Ts,
+                              xys, conics, radii, cum, rects, splats, gids, bins, final_Ts,
                               final_idx)
         return img
 
     @staticmethod
     def backward(ctx, v_img):
         means3d, scales, quats, viewmat, colors, opacities, background, xys, conics, radii, \
-            comp, cum, rects, gids, bins, final_Ts, final_idx = ctx.saved_tensors
+            cum, rects, splats, gids, bins, final_Ts, final_idx = ctx.saved_tensors
         dev = means3d.device
         nch = colors.shape[1]
         ws, wsb = _staged_workspace(ctx.d, ctx.isect, nch, dev)
@@ -555,7 +579,7 @@ class _Render(torch.autograd.Function):
                 ctx.d, ctx.isect, ctx.adjoint, colors, opacities, background, xys, conics, gids,
                 bins, final_Ts, final_idx, radii, cum, rects, v_img)
             v_mean3d, v_scale, v_quat = _project_backward(
-                ctx.d, ctx.adjoint, means3d, scales, quats, viewmat, radii, conics, comp, v_xy,
+                ctx.d, ctx.adjoint, means3d, scales, quats, viewmat, radii, conics, None, v_xy,
                 None, v_conic, None)
             return None, v_mean3d, v_scale, v_quat, v_colors, v_opacity, None, None, None
         v_img = v_img.contiguous()
@@ -568,10 +592,10 @@ class _Render(torch.autograd.Function):
         with torch.cuda.device(dev):
             _lib.check(L.jaxsplat_render_backward(
                 ctypes.byref(ctx.d), _p(gids), _p(bins), ctx.isect, _p(means3d), _p(scales),
-                _p(quats), _p(viewmat), _p(xys), _p(conics), _p(colors), _p(opacities),
-                _p(background), nch, _p(radii), _p(rects), _p(cum), _p(final_Ts),
-                _p(final_idx), _p(v_img), _p(v_colors), _p(v_opacity), _p(v_mean3d),
-                _p(v_scale), _p(v_quat), ctx.adjoint, _p(ws), wsb, _stream(dev)))
+                _p(quats), _p(viewmat), _p(splats), _p(conics), _p(background), nch, _p(radii),
+                _p(rects), _p(cum), _p(final_Ts), _p(final_idx), _p(v_img), _p(v_colors),
+                _p(v_opacity), _p(v_mean3d), _p(v_scale), _p(v_quat), ctx.adjoint, _p(ws), wsb,
+                _stream(dev)))
         return None, v_mean3d, v_scale, v_quat, v_colors, v_opacity, None, None, None
 
 

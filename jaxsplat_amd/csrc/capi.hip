@@ -296,6 +296,35 @@ int jaxsplat_project_forward_rects(const jaxsplat_descriptor* d, const float* me
     return JAXSPLAT_OK;
 }
 
+int jaxsplat_project_forward_splats(const jaxsplat_descriptor* d, const float* means3d,
+                                    const float* scales, const float* quats,
+                                    const float* viewmat, const float* opacities,
+                                    const float* colors, uint32_t channels, float* xys,
+                                    float* depths, int32_t* radii, float* conics,
+                                    float* compensation, int32_t* num_tiles_hit,
+                                    int32_t* cum_tiles_hit, uint32_t* tile_rects, float* splats,
+                                    void* workspace, size_t workspace_bytes, void* stream) {
+    g_err.clear();
+    if (int st = validate(d)) return st;
+    if (channels < 1 || channels > 3)
+        return fail(JAXSPLAT_ERR_INVALID,
+                    "project_forward_splats: channels must be in [1, 3], got %u", channels);
+    if (d->num_points == 0) return JAXSPLAT_OK;
+    if (!means3d || !scales || !quats || !viewmat || !opacities || !colors || !xys || !depths ||
+        !radii || !conics || !num_tiles_hit || !cum_tiles_hit || !tile_rects || !splats)
+        return fail(JAXSPLAT_ERR_INVALID, "project_forward_splats: NULL required pointer");
+    if (!workspace || workspace_bytes < jaxsplat_project_workspace_size(d))
+        return fail(JAXSPLAT_ERR_WORKSPACE, "project_forward_splats: workspace too small");
+    JSA_CHECK_HIP(jsa::project_fwd(camera_of(d), means3d, scales, quats, viewmat, nullptr, xys,
+                                   depths, radii, conics, compensation, num_tiles_hit,
+                                   cum_tiles_hit, opacities,
+                                   reinterpret_cast<uint2*>(tile_rects), workspace,
+                                   static_cast<hipStream_t>(stream), colors, (int)channels,
+                                   reinterpret_cast<float4*>(splats)),
+                  "project_forward_splats");
+    return JAXSPLAT_OK;
+}
+
 int jaxsplat_project_backward(const jaxsplat_descriptor* d, const float* means3d,
                               const float* scales, const float* quats, const float* viewmat,
                               const int32_t* radii, const float* conics,
@@ -642,12 +671,11 @@ int jaxsplat_rasterize_backward_staged(
 int jaxsplat_render_backward(
     const jaxsplat_descriptor* d, const int32_t* gaussian_ids_sorted, const int32_t* tile_bins,
     uint32_t num_intersects, const float* means3d, const float* scales, const float* quats,
-    const float* viewmat, const float* xys, const float* conics, const float* colors,
-    const float* opacities, const float* background, uint32_t channels, const int32_t* radii,
-    const uint32_t* tile_rects, const int32_t* cum_tiles_hit, const float* final_Ts,
-    const int32_t* final_idx, const float* v_out_img, float* v_colors, float* v_opacity,
-    float* v_mean3d, float* v_scale, float* v_quat, int adjoint, void* workspace,
-    size_t workspace_bytes, void* stream) {
+    const float* viewmat, const float* splats, const float* conics, const float* background,
+    uint32_t channels, const int32_t* radii, const uint32_t* tile_rects,
+    const int32_t* cum_tiles_hit, const float* final_Ts, const int32_t* final_idx,
+    const float* v_out_img, float* v_colors, float* v_opacity, float* v_mean3d, float* v_scale,
+    float* v_quat, int adjoint, void* workspace, size_t workspace_bytes, void* stream) {
     g_err.clear();
     if (int st = validate(d)) return st;
     if (channels < 1 || channels > 3)
@@ -660,10 +688,10 @@ int jaxsplat_render_backward(
     if (d->num_points == 0) return JAXSPLAT_OK;
     if (!v_colors || !v_opacity || !v_mean3d || !v_scale || !v_quat)
         return fail(JAXSPLAT_ERR_INVALID, "render_backward: NULL gradient pointer");
-    if (!means3d || !scales || !quats || !viewmat || !xys || !radii || !tile_rects ||
-        !cum_tiles_hit || !conics ||
-        (num_intersects > 0 && (!gaussian_ids_sorted || !tile_bins || !colors || !opacities ||
-                                !background || !final_Ts || !final_idx || !v_out_img)))
+    if (!means3d || !scales || !quats || !viewmat || !splats || !conics || !radii ||
+        !tile_rects || !cum_tiles_hit ||
+        (num_intersects > 0 && (!gaussian_ids_sorted || !tile_bins || !background ||
+                                !final_Ts || !final_idx || !v_out_img)))
         return fail(JAXSPLAT_ERR_INVALID, "render_backward: NULL required pointer");
     const size_t need = align256(jsa::raster_bwd_staged_bytes((int)num_intersects));
     if (!workspace || workspace_bytes < need)
@@ -674,16 +702,47 @@ int jaxsplat_render_backward(
     hipStream_t s = static_cast<hipStream_t>(stream);
     const int consistent = adjoint == JAXSPLAT_ADJOINT_CONSISTENT;
     JSA_CHECK_HIP(jsa::raster_bwd_staged(c, (int)channels, gaussian_ids_sorted, tile_bins,
-                                         (int)num_intersects, xys, conics, colors, opacities,
+                                         (int)num_intersects, nullptr, nullptr, nullptr, nullptr,
                                          background, final_Ts, final_idx, v_out_img, nullptr,
                                          radii, rects, cum_tiles_hit, workspace, nullptr, nullptr,
-                                         nullptr, nullptr, nullptr, consistent, s, true),
+                                         nullptr, nullptr, nullptr, consistent, s, true,
+                                         reinterpret_cast<const float4*>(splats)),
                   "render_backward (records)");
     JSA_CHECK_HIP(jsa::render_bwd_tail(c, (int)channels, (int)num_intersects, means3d, scales,
                                        quats, viewmat, radii, rects, cum_tiles_hit, conics,
                                        workspace, v_colors, v_opacity, v_mean3d, v_scale, v_quat,
                                        consistent, s),
                   "render_backward (tail)");
+    return JAXSPLAT_OK;
+}
+
+int jaxsplat_rasterize_forward_splats(const jaxsplat_descriptor* d,
+                                      const int32_t* gaussian_ids_sorted,
+                                      const int32_t* tile_bins, uint32_t num_intersects,
+                                      const float* splats, const float* background,
+                                      uint32_t channels, float* final_Ts, int32_t* final_idx,
+                                      float* out_img, void* stream) {
+    g_err.clear();
+    if (int st = validate(d)) return st;
+    if (channels < 1 || channels > 3)
+        return fail(JAXSPLAT_ERR_INVALID,
+                    "rasterize_forward_splats: channels must be in [1, 3], got %u", channels);
+    if (!background || !final_Ts || !final_idx || !out_img)
+        return fail(JAXSPLAT_ERR_INVALID, "rasterize_forward_splats: NULL required pointer");
+    const jsa::Camera c = camera_of(d);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (num_intersects == 0 || d->num_points == 0) {
+        JSA_CHECK_HIP(jsa::raster_empty(c, background, final_Ts, final_idx, out_img, s,
+                                        (int)channels),
+                      "rasterize_forward_splats");
+        return JAXSPLAT_OK;
+    }
+    if (!gaussian_ids_sorted || !tile_bins || !splats)
+        return fail(JAXSPLAT_ERR_INVALID, "rasterize_forward_splats: NULL required pointer");
+    JSA_CHECK_HIP(jsa::raster_fwd_splats(c, (int)channels, gaussian_ids_sorted, tile_bins,
+                                         reinterpret_cast<const float4*>(splats), background,
+                                         final_Ts, final_idx, out_img, s),
+                  "rasterize_forward_splats");
     return JAXSPLAT_OK;
 }
 

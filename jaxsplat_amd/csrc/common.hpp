@@ -154,6 +154,14 @@ __device__ __forceinline__ void unpack_rect(uint2 rc, int& x0, int& x1, int& y0,
     y1 = y0 + (int)(rc.y >> 16);
 }
 
+// log2(255 o): a pair passes the per-pixel test o exp(-sigma) >= 1/255 with
+// sigma >= 0 iff sigma * log2(e) lies in [0, alpha_limit2(o)].  One function
+// for every kernel that evaluates or precomputes it (the projection's splat
+// records, the rectangles below and both rasterizers must agree bit for bit).
+__device__ __forceinline__ float alpha_limit2(float o) {
+    return __builtin_amdgcn_logf(255.f * o);
+}
+
 // Tile rectangle of a Gaussian (centre xy, radius rad > 0, conic (a, b, c),
 // opacity o) restricted to the tiles holding a pixel centre it can blend
 // into: those where o exp(-sigma) >= 1/255, i.e. the ellipse
@@ -169,7 +177,7 @@ __device__ __forceinline__ uint2 opacity_tile_rect(int bw, int tiles_x, int tile
                                                    int rad, float a, float b, float c, float o) {
     int x0, x1, y0, y1;
     tile_rect(bw, tiles_x, tiles_y, xy, rad, x0, x1, y0, y1);
-    const float lim2 = __builtin_amdgcn_logf(255.f * o);
+    const float lim2 = alpha_limit2(o);
     if (!(lim2 >= 0.f)) return make_uint2((uint32_t)x0 | ((uint32_t)y0 << 16), 0u);
     const float det = a * c - b * b;
     if (a > 0.f && c > 0.f && det > 0.f) {
@@ -196,9 +204,48 @@ __device__ __forceinline__ uint2 opacity_tile_rect(int bw, int tiles_x, int tile
 // Staged rasterize-backward records (raster.hip writes them; the reduction
 // kernels in raster.hip and project.hip sum them)
 // ---------------------------------------------------------------------------
-// A record is three float4 per intersection slot: (rgb 3, conic a),
-// (conic b, c, xy 2), (opacity, xy_abs 2, -); bits holds one bit per slot,
-// set when the slot's record was written.
+// A record holds one (Gaussian, tile) pair's sums at the pair's intersection
+// slot k: 9 floats at rec + 9 k, (rgb 3, conic 3, xy 2, opacity), or with the
+// v_xy_abs sums (ABS) 12 floats at rec + 12 k, (rgb 3, conic a), (conic b,
+// c, xy 2), (opacity, xy_abs 2, -).  bits holds one bit per slot, set when
+// the slot's record was written.  The workspace reserves 48 bytes per slot.
+template <bool ABS>
+__device__ __forceinline__ void load_record(const float* __restrict__ rec, size_t k, float4& a0,
+                                            float4& a1, float4& a2) {
+    if (ABS) {
+        const float4* q = reinterpret_cast<const float4*>(rec) + 3 * k;
+        a0 = q[0];
+        a1 = q[1];
+        a2 = q[2];
+    } else {
+        const float* q = rec + 9 * k;
+        a0 = make_float4(q[0], q[1], q[2], q[3]);
+        a1 = make_float4(q[4], q[5], q[6], q[7]);
+        a2 = make_float4(q[8], 0.f, 0.f, 0.f);
+    }
+}
+
+template <bool ABS>
+__device__ __forceinline__ void store_record(float* __restrict__ rec, size_t k, float4 a0,
+                                             float4 a1, float4 a2) {
+    if (ABS) {
+        float4* q = reinterpret_cast<float4*>(rec) + 3 * k;
+        q[0] = a0;
+        q[1] = a1;
+        q[2] = a2;
+    } else {
+        float* q = rec + 9 * k;
+        q[0] = a0.x;
+        q[1] = a0.y;
+        q[2] = a0.z;
+        q[3] = a0.w;
+        q[4] = a1.x;
+        q[5] = a1.y;
+        q[6] = a1.z;
+        q[7] = a1.w;
+        q[8] = a2.x;
+    }
+}
 // Sum of the flagged staged records of slots [off, end) (empty: off == end),
 // in slot order, into (a0, a1, a2) of the calling lane.  Ranges of at most 32
 // slots are summed by their own lane: the bitmap words covering the range,
@@ -210,7 +257,7 @@ __device__ __forceinline__ uint2 opacity_tile_rect(int bw, int tiles_x, int tile
 // its wave for tens of rounds; the results stay independent of scheduling.
 // Called by every lane of the wave.
 template <bool ABS>
-__device__ __forceinline__ void sum_records(int off, int end, const float4* __restrict__ rec,
+__device__ __forceinline__ void sum_records(int off, int end, const float* __restrict__ rec,
                                             const uint32_t* __restrict__ bits, int lane,
                                             float4& a0, float4& a1, float4& a2) {
     constexpr int kBig = 32;
@@ -246,11 +293,8 @@ __device__ __forceinline__ void sum_records(int off, int end, const float4* __re
                     }
                     float4 r[4][3];
 #pragma unroll
-                    for (int i = 0; i < 4; ++i) {
-                        r[i][0] = rec[3 * (size_t)k[i] + 0];
-                        r[i][1] = rec[3 * (size_t)k[i] + 1];
-                        r[i][2] = rec[3 * (size_t)k[i] + 2];
-                    }
+                    for (int i = 0; i < 4; ++i)
+                        load_record<ABS>(rec, (size_t)k[i], r[i][0], r[i][1], r[i][2]);
 #pragma unroll
                     for (int i = 0; i < 4; ++i) {
                         const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -273,8 +317,8 @@ __device__ __forceinline__ void sum_records(int off, int end, const float4* __re
         float4 b0 = make_float4(0.f, 0.f, 0.f, 0.f), b1 = b0, b2 = b0;
         for (int k = boff + lane; k < bend; k += 64) {
             if (!((bits[k >> 5] >> (k & 31)) & 1u)) continue;
-            const float4 r0 = rec[3 * (size_t)k + 0], r1 = rec[3 * (size_t)k + 1],
-                         r2 = rec[3 * (size_t)k + 2];
+            float4 r0, r1, r2;
+            load_record<ABS>(rec, (size_t)k, r0, r1, r2);
             b0.x += r0.x; b0.y += r0.y; b0.z += r0.z; b0.w += r0.w;
             b1.x += r1.x; b1.y += r1.y; b1.z += r1.z; b1.w += r1.w;
             b2.x += r2.x; b2.y += r2.y; b2.z += r2.z;
@@ -348,6 +392,11 @@ struct RasterGrid {
     // backward), zeroed by the launch; null: one workgroup per tile
     const int* order;
     int* queue_counters;
+    // splat records (optional, single view): per Gaussian three float4
+    // (x, y, opacity, alpha_limit2(opacity)), (conic a, b, c, tile rectangle
+    // word 0), (colour 3, rectangle word 1), staged instead of gathering
+    // xys, opacities, conics, colours and rectangles separately
+    const float4* splats;
 };
 
 ViewRec single_view(const Camera& c);
@@ -380,11 +429,14 @@ constexpr int kStaticProjectPoints = 1 << 26;
 hipError_t project_static_scratch(void** out);
 // With opacities and rects (both optional), the tile counts and their sums are
 // those of the opacity-limited rectangles (opacity_tile_rect), written to rects.
+// With opacities, colors (nch <= 3 channels per row) and splats as well, the
+// splat records of RasterGrid::splats are written too.
 hipError_t project_fwd(const Camera& c, const float* means3d, const float* scales,
                        const float* quats, const float* viewmat, float* cov3d,
                        float* xys, float* depths, int* radii, float* conics,
                        float* compensation, int* num_tiles_hit, int* cum_tiles_hit,
-                       const float* opacities, uint2* rects, void* scratch, hipStream_t s);
+                       const float* opacities, uint2* rects, void* scratch, hipStream_t s,
+                       const float* colors = nullptr, int nch = 0, float4* splats = nullptr);
 // Opacity-limited tile rectangles (opacity_tile_rect) of projected Gaussians,
 // their counts (num_tiles_hit optional) and inclusive sums; scratch
 // project_fwd_scratch_bytes(c.num_points).
@@ -558,7 +610,13 @@ hipError_t raster_bwd_staged(const Camera& c, int nch, const int* gids_sorted,
                              void* ws,
                              float* v_colors, float* v_opacity, float* v_xy, float* v_xy_abs,
                              float* v_conic, int consistent, hipStream_t s,
-                             bool records_only = false);
+                             bool records_only = false, const float4* splats = nullptr);
+// Single-view forward staging from splat records (RasterGrid::splats) of
+// nch <= 3 channels: no xys / conics / colours / opacities arrays.
+hipError_t raster_fwd_splats(const Camera& c, int nch, const int* gids_sorted,
+                             const int* tile_bins, const float4* splats,
+                             const float* background, float* final_Ts, int* final_idx,
+                             float* out_img, hipStream_t s);
 // Batch backward with staged gradients (3 channels): records at the slots
 // item_first[item] + the tile's row-major offset in the item's rectangle
 // (item_rects, the ones the binning used; workspace raster_bwd_staged_bytes(cap)),

@@ -237,7 +237,8 @@ __device__ __forceinline__ int project_one(
     float* __restrict__ cov3d_out, float* __restrict__ xys, float* __restrict__ depths,
     int* __restrict__ radii, float* __restrict__ conics, float* __restrict__ comp_out,
     int* __restrict__ num_tiles_hit, const float* __restrict__ opacities,
-    uint2* __restrict__ rects) {
+    uint2* __restrict__ rects, const float* __restrict__ colors, int nch,
+    float4* __restrict__ splats) {
     const Proj r = project_core(c, idx, W, pt, means3d, scales, quats);
     const float* cov = r.cov;
     const float* conic = r.conic;
@@ -262,6 +263,15 @@ __device__ __forceinline__ int project_one(
                                    : make_uint2(0u, 0u);
         rects[idx] = rc;
         count = (int)rect_count(rc);
+        if (splats) {   // the rasterizers' staged inputs in one 48-byte record
+            const float o = opacities[idx];
+            const float* col = colors + (size_t)nch * idx;
+            float4* sp = splats + 3 * (size_t)idx;
+            sp[0] = make_float4(r.cx, r.cy, o, alpha_limit2(o));
+            sp[1] = make_float4(conic[0], conic[1], conic[2], __uint_as_float(rc.x));
+            sp[2] = make_float4(col[0], nch > 1 ? col[1] : 0.f, nch > 2 ? col[2] : 0.f,
+                                __uint_as_float(rc.y));
+        }
     }
     num_tiles_hit[idx] = count;
     return count;
@@ -338,7 +348,8 @@ __global__ __launch_bounds__(256) void project_fwd_kernel(
     int* __restrict__ radii, float* __restrict__ conics, float* __restrict__ comp_out,
     int* __restrict__ num_tiles_hit, int* __restrict__ cum_tiles_hit,
     const float* __restrict__ opacities, uint2* __restrict__ rects,
-    uint32_t* __restrict__ part_tot) {
+    uint32_t* __restrict__ part_tot, const float* __restrict__ colors, int nch,
+    float4* __restrict__ splats) {
     __shared__ uint32_t s_wave_tot[kProjPer][4];
     const int tid = threadIdx.x;
     const int lane = lane_id();
@@ -357,7 +368,7 @@ __global__ __launch_bounds__(256) void project_fwd_kernel(
         cnt[k] = idx < c.num_points
                      ? project_one(c, idx, W, pt, means3d, scales, quats, cov3d_out, xys,
                                    depths, radii, conics, comp_out, num_tiles_hit, opacities,
-                                   rects)
+                                   rects, colors, nch, splats)
                      : 0;
         incl[k] = wave_inclusive_sum(cnt[k]);
         if (lane == 63) s_wave_tot[k][wave] = (uint32_t)incl[k];
@@ -649,7 +660,7 @@ __global__ __launch_bounds__(256) void render_bwd_tail_kernel(
     Camera c, int num_isect, const float* __restrict__ means3d, const float* __restrict__ scales,
     const float* __restrict__ quats, const float* __restrict__ viewmat,
     const int* __restrict__ radii, const uint2* __restrict__ rects, const int* __restrict__ cum,
-    const float* __restrict__ conics, const float4* __restrict__ rec,
+    const float* __restrict__ conics, const float* __restrict__ rec,
     const uint32_t* __restrict__ bits, int nch, float* __restrict__ v_colors,
     float* __restrict__ v_opacity, float* __restrict__ v_mean3d, float* __restrict__ v_scale,
     float* __restrict__ v_quat, int consistent) {
@@ -860,14 +871,18 @@ hipError_t project_fwd(const Camera& c, const float* means3d, const float* scale
                        const float* quats, const float* viewmat, float* cov3d, float* xys,
                        float* depths, int* radii, float* conics, float* compensation,
                        int* num_tiles_hit, int* cum_tiles_hit, const float* opacities,
-                       uint2* rects, void* scratch, hipStream_t s) {
+                       uint2* rects, void* scratch, hipStream_t s, const float* colors, int nch,
+                       float4* splats) {
     if (c.num_points <= 0) return hipSuccess;
+    if (splats && (!opacities || !rects || !colors || nch < 1 || nch > 3))
+        return hipErrorInvalidValue;
     const int parts = (c.num_points + kProjPart - 1) / kProjPart;
     uint32_t* part_tot = static_cast<uint32_t*>(scratch);
     uint32_t* part_off = part_tot + parts;
     project_fwd_kernel<<<parts, 256, 0, s>>>(c, means3d, scales, quats, viewmat, cov3d, xys,
                                              depths, radii, conics, compensation, num_tiles_hit,
-                                             cum_tiles_hit, opacities, rects, part_tot);
+                                             cum_tiles_hit, opacities, rects, part_tot, colors,
+                                             nch, splats);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || parts == 1) return e;
     project_scan_kernel<<<1, 1024, 0, s>>>(part_tot, parts, part_off);
@@ -924,7 +939,7 @@ hipError_t render_bwd_tail(const Camera& c, int nch, int num_isect, const float*
     // the layout of raster_bwd_staged's workspace: records, then the bitmap
     const uint8_t* w = static_cast<const uint8_t*>(ws);
     const size_t n = (size_t)(num_isect > 0 ? num_isect : 1);
-    const float4* rec = reinterpret_cast<const float4*>(w);
+    const float* rec = reinterpret_cast<const float*>(w);
     const uint32_t* bits = reinterpret_cast<const uint32_t*>(w + ((48 * n + 255) & ~size_t(255)));
     const int blocks = (c.num_points + 255) / 256;
     render_bwd_tail_kernel<<<blocks, 256, 0, s>>>(c, num_isect, means3d, scales, quats, viewmat,

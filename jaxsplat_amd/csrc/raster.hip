@@ -36,15 +36,20 @@
 namespace jsa {
 namespace {
 
-// Pixels per lane of the backward: 4 (one wave per tile, each Gaussian
-// reduced once per tile) for tiles of more than 128 pixels, else 2.  At 16 x 16
-// the 4-slot form cuts the VALU work per (Gaussian, tile) by about a third
-// (91 VGPRs, 5 waves per SIMD): 0.49 -> 0.46 ms for the hd1m backward stage.
+// Pixels per lane of the backward: 4 for tiles of more than 128 pixels, else
+// 2, so that one wave covers any tile (bw <= 16) and each Gaussian is reduced
+// once per tile.  At 16 x 16 the 4-slot form cuts the VALU work per
+// (Gaussian, tile) by about a third: 0.49 -> 0.46 ms for the hd1m backward
+// stage.  The kernel keeps one wave's sums per entry, so it is always
+// launched with 64 threads.
 inline int bwd_slots(int block_width) { return block_width * block_width > 128 ? 4 : 2; }
 inline int bwd_threads(int block_width, int slots) {
-    const int per = (block_width * block_width + slots - 1) / slots;
-    return ((per + 63) / 64) * 64;
+    (void)block_width;
+    (void)slots;
+    return 64;
 }
+static_assert((kMaxTileThreads + 3) / 4 <= 64 && 128 / 2 <= 64,
+              "one wave covers a tile at 4 (more than 128 pixels) or 2 pixels per lane");
 
 constexpr float kTEps = 1e-4f;
 // The batches hold the conic scaled to base 2: (a, b, c) * (log2(e)/2,
@@ -70,9 +75,7 @@ __device__ __forceinline__ float sigma2(float4 b, float dx, float dy) {
 // sigma2 in [0, lim2], lim2 = log2(255 o).  Gaussians with lim2 < 0 are culled
 // whole (CullInfo::any), so lim2 >= 0 here, and the range test is one unsigned
 // compare of the float bits (negative sigma2 and NaN have larger patterns).
-__device__ __forceinline__ float alpha_limit2(float o) {
-    return __builtin_amdgcn_logf(255.f * o);
-}
+// (alpha_limit2 is in common.hpp.)
 __device__ __forceinline__ bool pair_accepted(float s2, float lim2) {
     return __float_as_uint(s2) <= __float_as_uint(lim2);
 }
@@ -112,6 +115,49 @@ __device__ __forceinline__ float4 load_color(const RasterGrid& rg, const float* 
                                              size_t g) {
     const float* p = a + (size_t)rg.ch_stride * g + rg.ch_off;
     return make_float4(p[0], rg.ch_n > 1 ? p[1] : 0.f, rg.ch_n > 2 ? p[2] : 0.f, 0.f);
+}
+
+// One staged entry's inputs: from the splat records when the grid has them
+// (one 48-byte record per Gaussian), else gathered from the separate arrays
+// (item-indexed xys / conics, Gaussian-indexed colours / opacities).
+struct SplatIn {
+    float x, y, o, lim2, ca, cb, cc;
+    float4 col;
+    uint2 rect;         // splat records only
+};
+
+__device__ __forceinline__ SplatIn stage_splat(const RasterGrid& rg, int it, int g,
+                                               const float2* __restrict__ xys,
+                                               const float* __restrict__ conics,
+                                               const float* __restrict__ colors,
+                                               const float* __restrict__ opac) {
+    SplatIn sp;
+    if (rg.splats) {
+        const float4 a = rg.splats[3 * (size_t)it + 0];
+        const float4 b = rg.splats[3 * (size_t)it + 1];
+        const float4 c = rg.splats[3 * (size_t)it + 2];
+        sp.x = a.x;
+        sp.y = a.y;
+        sp.o = a.z;
+        sp.lim2 = a.w;
+        sp.ca = b.x;
+        sp.cb = b.y;
+        sp.cc = b.z;
+        sp.col = make_float4(c.x, c.y, c.z, 0.f);
+        sp.rect = make_uint2(__float_as_uint(b.w), __float_as_uint(c.w));
+        return sp;
+    }
+    const float2 xy = xys[it];
+    sp.x = xy.x;
+    sp.y = xy.y;
+    sp.o = opac[g];
+    sp.lim2 = alpha_limit2(sp.o);
+    sp.ca = conics[3 * it + 0];
+    sp.cb = conics[3 * it + 1];
+    sp.cc = conics[3 * it + 2];
+    sp.col = load_color(rg, colors, g);
+    sp.rect = make_uint2(0u, 0u);
+    return sp;
 }
 
 // Tile-local (row, column) of pixel number p of a bw x bw tile.  Pixels are
@@ -167,14 +213,14 @@ struct CullInfo {
 };
 
 __device__ __forceinline__ CullInfo cull_info(float gx, float gy, float a, float b, float c,
-                                              float o) {
+                                              float lim2) {
     CullInfo k;
     k.gx = gx;
     k.gy = gy;
     k.a = a;
     k.b = b;
     k.c = c;
-    k.lim2 = alpha_limit2(o);
+    k.lim2 = lim2;
     k.smax = k.lim2 * 0.69314718f + 0.01f;        // ln(255 o) + margin
     k.any = k.lim2 >= 0.f;
     k.exact = a > 0.f && c > 0.f && a * c - b * b > 0.f;
@@ -287,15 +333,12 @@ __device__ __forceinline__ void raster_fwd_tile(
             uint32_t bits = 0;
             const int it = (e < DEPTH && idx < range.y) ? gids[idx] : -1;
             if ((unsigned)(it - vr.item_off) < (unsigned)vr.n) {    // an item of this view
-                const int g = it + gshift;
-                const float2 xy = xys[it];
-                const float o = opac[g];
-                const float ca = conics[3 * it + 0], cb_ = conics[3 * it + 1],
-                            cc = conics[3 * it + 2];
-                const CullInfo ci = cull_info(xy.x, xy.y, ca, cb_, cc, o);
-                s_a[e] = make_float4(xy.x, xy.y, o, __int_as_float(it));
-                s_b[e] = make_float4(ca * kHalfLog2e, cb_ * kLog2e, cc * kHalfLog2e, ci.lim2);
-                s_c[e] = load_color(rg, colors, g);
+                const SplatIn sp = stage_splat(rg, it, it + gshift, xys, conics, colors, opac);
+                const CullInfo ci = cull_info(sp.x, sp.y, sp.ca, sp.cb, sp.cc, sp.lim2);
+                s_a[e] = make_float4(sp.x, sp.y, sp.o, __int_as_float(it));
+                s_b[e] = make_float4(sp.ca * kHalfLog2e, sp.cb * kLog2e, sp.cc * kHalfLog2e,
+                                     sp.lim2);
+                s_c[e] = sp.col;
                 for (int G = 0; G < nwaves; ++G)
                     if (s_rok[G] && group_hit(ci, s_rect[G])) bits |= 1u << G;
             }
@@ -565,16 +608,16 @@ __device__ __forceinline__ void finish_grads(float* r, float4 conic2, float o) {
 }
 
 // Staged gradient output of a single-view backward (no float atomics).  The
-// flush of (Gaussian g, tile) writes its sums as one 48-byte record at the
-// pair's intersection slot in the reference's emission order,
-// cum_tiles_hit[g] - num_tiles_hit[g] + the tile's row-major offset in g's
-// tile rectangle, and sets the slot's bit in a bitmap (one 32-bit atomic OR
-// per flush, where the direct path adds nine floats atomically);
-// raster_bwd_reduce_kernel then sums each Gaussian's flagged records in slot
-// order, so the result does not depend on the order in which tiles finish.  Record: (rgb 3, conic a), (conic b, c,
-// xy 2), (opacity, xy_abs 2, -).
+// flush of (Gaussian g, tile) writes its sums as one record (9 floats, or 12
+// with the v_xy_abs sums; load_record / store_record) at the pair's
+// intersection slot in the reference's emission order, cum_tiles_hit[g] -
+// num_tiles_hit[g] + the tile's row-major offset in g's tile rectangle, and
+// sets the slot's bit in a bitmap (one 32-bit atomic OR per flush, where the
+// direct path adds nine floats atomically); raster_bwd_reduce_kernel (or the
+// fused render's tail) then sums each Gaussian's flagged records in slot
+// order, so the result does not depend on the order in which tiles finish.
 struct StagedOut {
-    float4* rec;            // 3 float4 per slot
+    float* rec;             // one record per slot (load_record / store_record)
     uint32_t* bits;         // one bit per slot, zero before the backward
     const int* cum;         // cum_tiles_hit (inclusive), used when first is null
     const uint32_t* first;  // a batch's first slot per item (exclusive)
@@ -600,7 +643,9 @@ __device__ __forceinline__ void raster_bwd_tile(
     __shared__ float4 s_a[DEPTH];        // x, y, opacity, id
     __shared__ float4 s_b[DEPTH];        // base-2 conic a', b', c', staged slot
     __shared__ float4 s_c[DEPTH];        // colour, alpha_limit2(opacity)
-    __shared__ float s_g[DEPTH][2][NV];  // per-wave sums [entry][wave][value]
+    // one wave per workgroup (bwd_threads: NSLOT pixels per lane cover the
+    // tile in 64 lanes); its per-entry sums, kept until the batch's flush
+    __shared__ float s_g[DEPTH][NV];
     __shared__ int s_flag[DEPTH];
     __shared__ uint64_t s_mask[4][NW];   // [group = slot * nwaves + wave][64-entry word]
     __shared__ float4 s_rect[4];
@@ -661,8 +706,7 @@ __device__ __forceinline__ void raster_bwd_tile(
     for (int e = tid; e < DEPTH; e += nthr) {
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
-            s_g[e][0][v] = 0.f;
-            s_g[e][1][v] = 0.f;
+            s_g[e][v] = 0.f;
         }
         s_flag[e] = 0;
     }
@@ -686,19 +730,18 @@ __device__ __forceinline__ void raster_bwd_tile(
             uint32_t bits = 0;
             const int it = (e < DEPTH && idx >= range.x) ? gids[idx] : -1;
             if ((unsigned)(it - vr.item_off) < (unsigned)vr.n) {    // an item of this view
-                const int g = it + gshift;
-                const float2 xy = xys[it];
-                const float o = opac[g];
-                const float ca = conics[3 * it + 0], cb_ = conics[3 * it + 1],
-                            cc = conics[3 * it + 2];
-                s_a[e] = make_float4(xy.x, xy.y, o, __int_as_float(it));
+                const SplatIn sp = stage_splat(rg, it, it + gshift, xys, conics, colors, opac);
+                s_a[e] = make_float4(sp.x, sp.y, sp.o, __int_as_float(it));
                 int slot = -1;
                 if (STAGED) {
                     int x0, x1, y0, y1;
-                    if (st.rects)
+                    if (rg.splats)
+                        unpack_rect(sp.rect, x0, x1, y0, y1);
+                    else if (st.rects)
                         unpack_rect(st.rects[it], x0, x1, y0, y1);
                     else
-                        tile_rect(bw, vr.tiles_x, vr.tiles_y, xy, st.radii[it], x0, x1, y0, y1);
+                        tile_rect(bw, vr.tiles_x, vr.tiles_y, make_float2(sp.x, sp.y),
+                                  st.radii[it], x0, x1, y0, y1);
                     const int w = x1 - x0, lx = tx - x0, ly = ty - y0;
                     const int base = st.first ? (int)st.first[it] : st.cum[it] - w * (y1 - y0);
                     const int sl = base + ly * w + lx;
@@ -707,10 +750,10 @@ __device__ __forceinline__ void raster_bwd_tile(
                         sl < st.n_slots)
                         slot = sl;
                 }
-                s_b[e] = make_float4(ca * kHalfLog2e, cb_ * kLog2e, cc * kHalfLog2e,
+                s_b[e] = make_float4(sp.ca * kHalfLog2e, sp.cb * kLog2e, sp.cc * kHalfLog2e,
                                      __int_as_float(slot));
-                const CullInfo ci = cull_info(xy.x, xy.y, ca, cb_, cc, o);
-                float4 col = load_color(rg, colors, g);
+                const CullInfo ci = cull_info(sp.x, sp.y, sp.ca, sp.cb, sp.cc, sp.lim2);
+                float4 col = sp.col;
                 col.w = ci.lim2;                 // colour, acceptance limit
                 s_c[e] = col;
                 for (int G = 0; G < ngroups; ++G)
@@ -791,9 +834,9 @@ __device__ __forceinline__ void raster_bwd_tile(
                         }
                 }
                 if (any == 0) continue;
-                // lane j of every row gets the row sum of value j, then the four
-                // rows are summed with the row swaps; lanes 0..NV-1 store the
-                // wave's sums
+                // lane j of every row gets the row sum of value j, then the
+                // four rows are summed with the row swaps; lanes 0..NV-1 store
+                // the wave's sums
                 float val = row_sums_split<NV>(g, lane);
                 {
                     const auto r = __builtin_amdgcn_permlane16_swap(
@@ -805,8 +848,9 @@ __device__ __forceinline__ void raster_bwd_tile(
                         __float_as_uint(val), __float_as_uint(val), false, false);
                     val = __uint_as_float(r[0]) + __uint_as_float(r[1]);
                 }
-                if (lane < NV) s_g[t][wave][lane] = val;
+                if (lane < NV) s_g[t][lane] = val;
                 if (lane == 0) s_flag[t] = 1;
+
             }
         }
         __syncthreads();
@@ -815,19 +859,18 @@ __device__ __forceinline__ void raster_bwd_tile(
             float r[NV];
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
-                r[v] = s_g[e][0][v] + s_g[e][1][v];
-                s_g[e][0][v] = 0.f;
-                s_g[e][1][v] = 0.f;
+                r[v] = s_g[e][v];
+                s_g[e][v] = 0.f;
             }
             s_flag[e] = 0;
             finish_grads<NV>(r, s_b[e], s_a[e].z);
             if (STAGED) {
                 const int slot = __float_as_int(s_b[e].w);
                 if (slot < 0) continue;
-                st.rec[3 * (size_t)slot + 0] = make_float4(r[0], r[1], r[2], r[3]);
-                st.rec[3 * (size_t)slot + 1] = make_float4(r[4], r[5], r[6], r[7]);
-                st.rec[3 * (size_t)slot + 2] =
-                    make_float4(r[NV - 1], ABS ? r[8] : 0.f, ABS ? r[9] : 0.f, 0.f);
+                store_record<ABS>(st.rec, (size_t)slot, make_float4(r[0], r[1], r[2], r[3]),
+                                  make_float4(r[4], r[5], r[6], r[7]),
+                                  make_float4(r[NV - 1], ABS ? r[8] : 0.f, ABS ? r[9] : 0.f,
+                                              0.f));
                 atomicOr(&st.bits[slot >> 5], 1u << (slot & 31));
                 continue;
             }
@@ -895,7 +938,7 @@ __global__ __launch_bounds__(128) void raster_bwd_queue_kernel(
 template <bool ABS>
 __global__ __launch_bounds__(256) void raster_bwd_reduce_kernel(
     Camera c, int num_isect, const float2* __restrict__ xys, const int* __restrict__ radii,
-    const uint2* __restrict__ rects, const int* __restrict__ cum, const float4* __restrict__ rec,
+    const uint2* __restrict__ rects, const int* __restrict__ cum, const float* __restrict__ rec,
     const uint32_t* __restrict__ bits, int nch, float* __restrict__ v_rgb,
     float* __restrict__ v_opacity, float* __restrict__ v_xy, float* __restrict__ v_xy_abs,
     float* __restrict__ v_conic) {
@@ -941,7 +984,7 @@ __global__ __launch_bounds__(256) void raster_bwd_reduce_kernel(
 __global__ __launch_bounds__(256) void raster_bwd_batch_reduce_kernel(
     const ViewRec* __restrict__ views, int n_slots, const int* __restrict__ radii,
     const uint2* __restrict__ item_rects, const uint32_t* __restrict__ item_first,
-    const float4* __restrict__ rec, const uint32_t* __restrict__ bits, float* __restrict__ v_xy,
+    const float* __restrict__ rec, const uint32_t* __restrict__ bits, float* __restrict__ v_xy,
     float* __restrict__ v_conic, float4* __restrict__ v_rgbo) {
     const ViewRec vr = views[blockIdx.y];
     if ((int)blockIdx.x * 256 >= vr.n) return;        // uniform over the block
@@ -1147,6 +1190,19 @@ hipError_t raster_fwd_grid(const RasterGrid& g, const int* items_sorted, const i
     return hipGetLastError();
 }
 
+hipError_t raster_fwd_splats(const Camera& c, int nch, const int* gids_sorted,
+                             const int* tile_bins, const float4* splats,
+                             const float* background, float* final_Ts, int* final_idx,
+                             float* out_img, hipStream_t s) {
+    if (nch < 1 || nch > 3 || !splats) return hipErrorInvalidValue;
+    RasterGrid g = single_grid(c);
+    g.ch_stride = nch;
+    g.ch_n = nch;
+    g.splats = splats;
+    return raster_fwd_grid(g, gids_sorted, tile_bins, nullptr, nullptr, nullptr, nullptr,
+                           background, final_Ts, final_idx, out_img, s);
+}
+
 hipError_t raster_fwd(const Camera& c, const int* gids_sorted, const int* tile_bins,
                       const float* xys, const float* conics, const float* colors,
                       const float* opacities, const float* background, float* final_Ts,
@@ -1213,13 +1269,14 @@ hipError_t raster_bwd_staged(const Camera& c, int nch, const int* gids_sorted,
                              const int* radii, const uint2* rects, const int* cum_tiles_hit,
                              void* ws,
                              float* v_colors, float* v_opacity, float* v_xy, float* v_xy_abs,
-                             float* v_conic, int consistent, hipStream_t s, bool records_only) {
+                             float* v_conic, int consistent, hipStream_t s, bool records_only,
+                             const float4* splats) {
     if (c.num_points <= 0) return hipSuccess;
     if (nch < 1 || nch > 3) return hipErrorInvalidValue;
     uint8_t* w = static_cast<uint8_t*>(ws);
     const size_t n = (size_t)(num_isect > 0 ? num_isect : 1);
     StagedOut st;
-    st.rec = reinterpret_cast<float4*>(w);
+    st.rec = reinterpret_cast<float*>(w);
     st.bits = reinterpret_cast<uint32_t*>(w + ((48 * n + 255) & ~size_t(255)));
     st.cum = cum_tiles_hit;
     st.first = nullptr;
@@ -1233,6 +1290,7 @@ hipError_t raster_bwd_staged(const Camera& c, int nch, const int* gids_sorted,
         RasterGrid g = single_grid(c);
         g.ch_stride = nch;
         g.ch_n = nch;
+        g.splats = splats;
         const int slots = bwd_slots(g.block_width);
         const int nthr = bwd_threads(g.block_width, slots);
         const int grid = tile_grid(g.total_tiles);
@@ -1287,7 +1345,7 @@ hipError_t raster_bwd_batch_staged(const RasterGrid& g, int max_view_n, const in
     uint8_t* w = static_cast<uint8_t*>(ws);
     const size_t n = (size_t)(cap > 0 ? cap : 1);
     StagedOut st;
-    st.rec = reinterpret_cast<float4*>(w);
+    st.rec = reinterpret_cast<float*>(w);
     st.bits = reinterpret_cast<uint32_t*>(w + ((48 * n + 255) & ~size_t(255)));
     st.cum = nullptr;
     st.first = item_first;
