@@ -383,15 +383,15 @@ def stage_breakdown(torch, sc, cam, params, target, iters):
     vm, vs, vq = torch.empty((N, 3), **f32), torch.empty((N, 3), **f32), torch.empty((N, 4), **f32)
     bwsb = L.jaxsplat_rasterize_backward_staged_workspace_size(D, isect)
     bws = torch.empty(bwsb, dtype=torch.uint8, device=dev)
-    names = ["project_fwd", "depth_sort", "emission", "tile_sort", "bin_edges", "raster_fwd",
-             "loss_grad", "raster_bwd_records", "bwd_tail"]
+    names = ["project_fwd", "depth_keys", "emission", "tile_sort", "bin_edges", "tile_depth_sort",
+             "raster_fwd", "loss_grad", "raster_bwd_records", "bwd_tail"]
     acc = {k: 0.0 for k in names}
-    marks = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    marks = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
     for m in marks:
         m.record()                                  # creates the event
     torch.cuda.synchronize()
-    handles = (ctypes.c_void_p * 3)(*[m.cuda_event for m in marks])
-    L.jaxsplat_set_stage_marks(handles, 3)
+    handles = (ctypes.c_void_p * 4)(*[m.cuda_event for m in marks])
+    L.jaxsplat_set_stage_marks(handles, 4)
     try:
         for it in range(iters + 2):
             ev = [torch.cuda.Event(enable_timing=True) for _ in range(7)]
@@ -422,7 +422,8 @@ def stage_breakdown(torch, sc, cam, params, target, iters):
             if it >= 2:
                 t = [ev[0].elapsed_time(ev[1]), ev[1].elapsed_time(ev[2]),
                      ev[2].elapsed_time(marks[0]), marks[0].elapsed_time(marks[1]),
-                     marks[1].elapsed_time(ev[3]), ev[3].elapsed_time(ev[4]),
+                     marks[1].elapsed_time(marks[3]), marks[3].elapsed_time(ev[3]),
+                     ev[3].elapsed_time(ev[4]),
                      ev[4].elapsed_time(ev[5]), ev[5].elapsed_time(marks[2]),
                      marks[2].elapsed_time(ev[6])]
                 for k, n in enumerate(names):
@@ -463,23 +464,26 @@ def stage_breakdown(torch, sc, cam, params, target, iters):
     # sum; a flushed record is 36 B.
     stage_bytes = {
         "project_fwd": 164 * N,
-        "depth_sort": 104 * N,
-        "emission": 32 * N + 32 * E + 8 * isect,
+        "depth_keys": 28 * N,
+        "emission": 16 * N + 32 * E + 8 * isect,
         "tile_sort": 40 * isect,
         "bin_edges": 4 * isect + 8 * T,
+        "tile_depth_sort": 12 * isect + 8 * T,
         "raster_fwd": 52 * L_ft + 8 * T + 20 * Px,
         "loss_grad": 36 * Px,
         "raster_bwd_records": 56 * L_bt + 8 * T + 20 * Px + 36 * R + (isect + 7) // 8,
         "bwd_tail": 36 * R + (isect + 7) // 8 + 16 * N + 56 * V + 4 * (N - V) + 40 * N,
     }
-    valu_bound = {"raster_fwd", "raster_bwd_records"}
+    # the raster kernels are bound by VALU issue, the per-tile depth sort by
+    # the latency of its LDS ranking rounds and barriers, the rest by HBM
+    bound_by = {"raster_fwd": "valu", "raster_bwd_records": "valu", "tile_depth_sort": "lds"}
     stages = {}
     for k in names:
         ms_k = acc[k]
         gbs = stage_bytes[k] / (ms_k * 1e-3) / 1e9 if ms_k > 0 else 0.0
         stages[k] = {"ms": round(ms_k, 4), "alg_bytes": int(stage_bytes[k]),
                      "GBs": round(gbs, 1), "frac_hbm": round(gbs / HBM_PEAK_GBS, 4),
-                     "bound": "valu" if k in valu_bound else "hbm"}
+                     "bound": bound_by.get(k, "hbm")}
 
     # ---- roofline of the dominant kernels: the render backward ----
     # (the rasterize adjoint's record kernel and the per-Gaussian tail that

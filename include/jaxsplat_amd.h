@@ -69,12 +69,13 @@ const char* jaxsplat_last_error(void);
 
 /*
  * Profiling marks (no counterpart in the reference, which has no tracing).
- * events: up to 3 hipEvent_t (NULL entries allowed) that later calls from
+ * events: up to 4 hipEvent_t (NULL entries allowed) that later calls from
  * this thread record, on their stream, at internal stage boundaries:
  * [0] in jaxsplat_bin_emit_sort / jaxsplat_bin_and_sort after the emission,
  * [1] there after the tile sort (before the bin edges), [2] in the staged
  * rasterize backward after the per-intersection records (before the
- * per-Gaussian reduction).  events == NULL (or count 0) clears them.  The
+ * per-Gaussian reduction), [3] in the binning after the bin edges (before
+ * the per-tile depth order).  events == NULL (or count 0) clears them.  The
  * caller keeps the events alive while they are set.
  */
 void jaxsplat_set_stage_marks(void* const* events, int count);
@@ -187,9 +188,10 @@ int jaxsplat_bin_and_sort(const jaxsplat_descriptor* d, const float* xys, const 
  * The same binning in two stages, so that a caller that reads num_intersects
  * back from cum_tiles_hit can queue the first stage before it waits for the
  * count (the reference reads it between projection and sort_and_bin,
- * lib/ops.cu:645-660).  jaxsplat_bin_depth_sort needs no count: depth keys and
- * the stable depth sort of the Gaussians, into a workspace that depends on
- * num_points only.  jaxsplat_bin_emit_sort then emits, sorts and bins from
+ * lib/ops.cu:645-660).  jaxsplat_bin_depth_sort needs no count: the
+ * Gaussians' depth keys and tile rectangles, into a workspace that depends on
+ * num_points only.  jaxsplat_bin_emit_sort then emits (in Gaussian id order,
+ * as the reference), sorts by tile, bins and orders every tile by depth from
  * that workspace, with the outputs of jaxsplat_bin_and_sort; its workspace
  * (num_intersects dependent) is the one jaxsplat_bin_check reads.
  *
@@ -199,7 +201,8 @@ int jaxsplat_bin_and_sort(const jaxsplat_descriptor* d, const float* xys, const 
  * min(total, num_intersects), counted on the device, so a caller may pass a
  * capacity instead of the exact cum_tiles_hit[N-1] and skip reading the count
  * back (no host synchronisation).  A total above the capacity is an overflow:
- * only the first num_intersects in emission order are binned, and bit 8 of
+ * only the first num_intersects in emission order (the order of
+ * cum_tiles_hit) are binned, and bit 8 of
  * the error word is set.  After the call (in stream order) the workspace
  * holds the error word in bytes [0, 4) and the exact total, a uint64, in
  * bytes [8, 16), for callers that check them on the device or later.

@@ -9,26 +9,29 @@
 // the reference's final order is the lexicographic order
 // (tile, depth bits, gaussian id).
 //
-// This implementation produces exactly that order with far less sort traffic:
-//  1. depth_keys: one 32-bit key per Gaussian (depth bits, or 0xFFFFFFFF for
-//     Gaussians with an empty tile rectangle, which emit nothing);
-//  2. a stable LSD radix sort of the Gaussians by depth key (values = index),
-//     giving the (depth bits, gaussian id) order; its first pass drops the
-//     0xFFFFFFFF keys, so the later passes and the emission see only the
-//     Gaussians that emit (71% of them at 1080p with 1M Gaussians), counted
-//     on the device;
-//  3. emission in that order: a reduce-then-scan of the Gaussians' tile
-//     counts gives each Gaussian its first output slot, then every wave
-//     writes whole 64-slot chunks of (tile id, gaussian id) pairs (owner
-//     lookup by an LDS mark and a DPP max-scan), so neither a Gaussian
-//     covering hundreds of tiles nor a run of large near Gaussians serialises
-//     a lane or a wave;
-//  4. a stable LSD radix sort of the pairs on the tile id bits only
-//     (2 passes at 1080p/16x16 instead of 6 passes over 64-bit keys);
-//  5. tile_bins_kernel: [start, end) edges from the sorted tile ids, empty
-//     tiles (0, 0).
-// Stability of step 4 on input already ordered by (depth bits, id) yields
-// (tile, depth bits, id): identical to the reference order.
+// Single view, with far less sort traffic than a 64-bit key sort:
+//  1. depth_keys_kernel: per Gaussian its depth bits and packed tile
+//     rectangle;
+//  2. emission in Gaussian id order: a reduce-then-scan of the Gaussians'
+//     tile counts gives each Gaussian its first output slot (the reference's
+//     cum_tiles_hit order), then every wave writes whole 64-slot chunks of
+//     (tile id, gaussian id) pairs (owner lookup by an LDS mark and a DPP
+//     max-scan), so neither a Gaussian covering hundreds of tiles nor a run
+//     of large Gaussians serialises a lane or a wave; the expansion also
+//     counts the tile sort's first digit;
+//  3. a stable LSD radix sort of the pairs on the tile id bits only
+//     (2 passes at 1080p/16x16): order (tile, gaussian id);
+//  4. tile_bins_kernel: [start, end) edges from the sorted tile ids, empty
+//     tiles (0, 0);
+//  5. tile_depth_sort_kernel: every tile's run, already in id order, sorted
+//     stably in LDS by the depth bits that vary inside it (three 9-bit
+//     passes at most for float depths), giving (tile, depth bits, id):
+//     identical to the reference order.  Runs longer than the LDS holds are
+//     queued and sorted by tile_depth_sort_long_kernel in chunks through
+//     global memory.
+// The depth order is thus established per tile (about 800 entries each at
+// 1080p with 1M Gaussians, sorted where they lie in LDS) instead of by a
+// global sort of the Gaussians and a stable tile sort after it.
 #include "common.hpp"
 
 namespace jsa {
@@ -57,10 +60,11 @@ __global__ void depth_keys_kernel(Camera c, const float* __restrict__ depths,
     rects[g] = rc;
 }
 
-// View of each depth-sorted item (largest v with item_off[v] <= item), the
-// key of the stable pass that groups a batch's items by view.
+// View of each item of the emission order (largest v with item_off[v] <=
+// item); perm == nullptr: the items 0..n-1 in order, n = n_host.
 constexpr int kMaxBatchViews = 4096;
 __global__ __launch_bounds__(256) void view_keys_kernel(const uint32_t* __restrict__ perm,
+                                                        int n_host,
                                                         const int* __restrict__ n_dev,
                                                         const ViewRec* __restrict__ views,
                                                         int num_views,
@@ -68,10 +72,10 @@ __global__ __launch_bounds__(256) void view_keys_kernel(const uint32_t* __restri
     __shared__ int s_off[kMaxBatchViews];
     for (int v = threadIdx.x; v < num_views; v += blockDim.x) s_off[v] = views[v].item_off;
     __syncthreads();
-    const int n = *n_dev;
+    const int n = n_dev ? *n_dev : n_host;
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n) return;
-    const int it = (int)perm[r];
+    const int it = perm ? (int)perm[r] : r;
     int lo = 0, hi = num_views;
     while (hi - lo > 1) {
         const int mid = (lo + hi) >> 1;
@@ -84,11 +88,12 @@ __global__ __launch_bounds__(256) void view_keys_kernel(const uint32_t* __restri
 // ---------------------------------------------------------------------------
 // Emission: reduce-then-scan over the depth order, then slot-balanced expansion
 // ---------------------------------------------------------------------------
-// Depth order puts the large, near Gaussians together, so splitting the work
-// by Gaussians leaves a few waves with several times the average number of
-// slots (at 1080p with 1M Gaussians the busiest wave expanded 231 chunks of 64
-// slots against a mean of 35).  Here every wave of the expansion owns whole
-// 64-slot chunks of the output instead:
+// Tile counts per Gaussian range from 1 to the whole image, and an order such
+// as depth order puts the large, near Gaussians together, so splitting the
+// work by Gaussians leaves a few waves with several times the average number
+// of slots (at 1080p with 1M Gaussians in depth order the busiest wave
+// expanded 231 chunks of 64 slots against a mean of 35).  Here every wave of
+// the expansion owns whole 64-slot chunks of the output instead:
 //  1. emit_count_kernel: each depth-ordered entry's tile rectangle, packed
 //     (x0 | y0 << 16, w | h << 16), and per-partition totals of slots and of
 //     entries that emit (w * h > 0);
@@ -114,7 +119,8 @@ constexpr int kExpandWaves = 8192;           // waves of the expansion (a full c
 // Pass 1 over the entries perm[0..n): gathers each entry's packed rectangle
 // (one 8-byte read at a random id) into emission order and sums slots and
 // emitting entries per partition.  The lane's loads are issued stage by stage
-// so that they overlap.
+// so that they overlap.  perm == nullptr is the id order: the rectangles are
+// read where they lie and not copied.
 __global__ __launch_bounds__(256) void emit_count_kernel(
     int n_host, const int* __restrict__ n_dev, const uint32_t* __restrict__ perm,
     const uint2* __restrict__ rect_by_id, uint2* __restrict__ rects,
@@ -127,14 +133,16 @@ __global__ __launch_bounds__(256) void emit_count_kernel(
     uint32_t g[kEmitPer];
     uint2 rc[kEmitPer];
 #pragma unroll
-    for (int k = 0; k < kEmitPer; ++k) g[k] = r0 + k * 256 < n ? perm[r0 + k * 256] : 0xFFFFFFFFu;
+    for (int k = 0; k < kEmitPer; ++k)
+        g[k] = r0 + k * 256 < n ? (perm ? perm[r0 + k * 256] : (uint32_t)(r0 + k * 256))
+                                : 0xFFFFFFFFu;
 #pragma unroll
     for (int k = 0; k < kEmitPer; ++k)
         rc[k] = g[k] != 0xFFFFFFFFu ? rect_by_id[g[k]] : make_uint2(0u, 0u);
     uint32_t sum = 0, nz = 0;
 #pragma unroll
     for (int k = 0; k < kEmitPer; ++k) {
-        if (r0 + k * 256 < n) rects[r0 + k * 256] = rc[k];
+        if (perm && r0 + k * 256 < n) rects[r0 + k * 256] = rc[k];
         const uint32_t cnt = rect_count(rc[k]);
         sum += cnt;
         nz += cnt > 0 ? 1u : 0u;
@@ -279,7 +287,7 @@ __global__ __launch_bounds__(256) void emit_compact_kernel(
         base += slab;
         zbase += zslab;
         if (cnt[k] == 0) continue;          // also every r >= n
-        const uint32_t id = perm[r];
+        const uint32_t id = perm ? perm[r] : (uint32_t)r;
         E.off[j] = (uint32_t)(off < limit ? off : limit);
         E.rect[j] = rc[k];
         E.gid[j] = id;
@@ -442,6 +450,487 @@ __global__ __launch_bounds__(256) void tile_bins_kernel(const uint32_t* __restri
     }
 }
 
+// ---------------------------------------------------------------------------
+// Per-tile depth order
+// ---------------------------------------------------------------------------
+// A tile's run [bins[t].x, bins[t].y) of the tile-sorted ids is in ascending
+// id (the stable tile sort of pairs emitted in id order).  Ordering it by
+// (depth bits, position in the run) gives the reference order (depth bits,
+// id) inside the tile.
+//
+// Bucket order: the run's keys are split into 2^bb buckets linear in
+// key - min (the shift makes max - min fit bb bits, so at least half of the
+// buckets are in range); an LDS histogram with returning atomics gives every
+// entry its bucket and a slot in it, a scan the bucket starts, and the
+// entries are placed by bucket.  Each entry's final position is then its
+// bucket's start plus the number of entries of its bucket that precede it in
+// (key, position) order, counted by reading the bucket.  Depths inside a
+// tile are spread over a range of floats, so buckets are small (at 1080p with
+// 1M Gaussians: 2048 buckets for about 800 entries, at most 12 in a bucket);
+// the order is exact whatever the distribution, only the counting reads grow
+// with the bucket sizes.  A run whose keys are all equal is already in
+// order.
+//
+// Three kernels by run length: up to kSegCap entries in one 256-thread
+// workgroup with the placed entries in LDS; up to kBigCap in one 1024-thread
+// workgroup with the placed entries in a global scratch of the run's own
+// offsets (L2-resident); longer runs by a chunked LSD radix sort through
+// global memory.  The first kernel queues the runs it does not take, the
+// second the ones it passes on.
+constexpr int kSegThreads = 256;
+constexpr int kSegItems = 6;                           // entries per lane
+constexpr int kSegCap = kSegThreads * kSegItems;       // 1536
+constexpr int kSegBucketBits = 11;
+constexpr int kBigThreads = 1024;
+constexpr int kBigItems = 16;
+constexpr int kBigCap = kBigThreads * kBigItems;       // 16384
+constexpr int kBigMaxBucketBits = 14;
+constexpr int kSegQueueBlocks = 256;                   // workgroups of the queue kernels
+constexpr int kSegMaxBits = 9;                         // digit bits of the chunked radix sort
+constexpr int kSegBins = 1 << kSegMaxBits;
+
+// Unsigned min and max over the 64 lanes (full EXEC): DPP inside the rows,
+// then the four rows by readlane.
+__device__ __forceinline__ void wave_minmax_u32(uint32_t& mn, uint32_t& mx) {
+    int a = (int)mn, b = (int)mx;
+    a = (int)min((uint32_t)a, (uint32_t)dpp_i<kDppQuadPerm1032>(a));
+    b = (int)max((uint32_t)b, (uint32_t)dpp_i<kDppQuadPerm1032>(b));
+    a = (int)min((uint32_t)a, (uint32_t)dpp_i<kDppQuadPerm2301>(a));
+    b = (int)max((uint32_t)b, (uint32_t)dpp_i<kDppQuadPerm2301>(b));
+    a = (int)min((uint32_t)a, (uint32_t)dpp_i<kDppRowRor4>(a));
+    b = (int)max((uint32_t)b, (uint32_t)dpp_i<kDppRowRor4>(b));
+    a = (int)min((uint32_t)a, (uint32_t)dpp_i<kDppRowRor8>(a));
+    b = (int)max((uint32_t)b, (uint32_t)dpp_i<kDppRowRor8>(b));
+    mn = min(min((uint32_t)__builtin_amdgcn_readlane(a, 0), (uint32_t)__builtin_amdgcn_readlane(a, 16)),
+             min((uint32_t)__builtin_amdgcn_readlane(a, 32), (uint32_t)__builtin_amdgcn_readlane(a, 48)));
+    mx = max(max((uint32_t)__builtin_amdgcn_readlane(b, 0), (uint32_t)__builtin_amdgcn_readlane(b, 16)),
+             max((uint32_t)__builtin_amdgcn_readlane(b, 32), (uint32_t)__builtin_amdgcn_readlane(b, 48)));
+}
+
+// Block min / max; s_red holds 2 * WAVES words.  Ends with a barrier.
+template <int WAVES>
+__device__ __forceinline__ void block_minmax_u32(uint32_t& mn, uint32_t& mx, uint32_t* s_red) {
+    const int wave = threadIdx.x >> 6;
+    wave_minmax_u32(mn, mx);
+    if (lane_id() == 0) {
+        s_red[wave] = mn;
+        s_red[WAVES + wave] = mx;
+    }
+    __syncthreads();
+    mn = s_red[0];
+    mx = s_red[WAVES];
+#pragma unroll
+    for (int w = 1; w < WAVES; ++w) {
+        mn = min(mn, s_red[w]);
+        mx = max(mx, s_red[WAVES + w]);
+    }
+    __syncthreads();
+}
+
+// Exclusive scan of one value per thread over a block of WAVES waves (s_tmp:
+// WAVES words; two barriers).
+template <int WAVES>
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s_tmp) {
+    const int wave = threadIdx.x >> 6;
+    const uint32_t incl = wave_inclusive_sum(v);
+    if (lane_id() == 63) s_tmp[wave] = incl;
+    __syncthreads();
+    uint32_t off = 0;
+#pragma unroll
+    for (int w = 0; w < WAVES; ++w) off += (w < wave) ? s_tmp[w] : 0u;
+    __syncthreads();
+    return off + incl - v;
+}
+
+// (key, position) order of two entries.
+__device__ __forceinline__ bool entry_before(uint2 a, uint32_t key, uint32_t e) {
+    return a.x < key || (a.x == key && a.y < e);
+}
+
+// Bucket order of the run's entries held in registers (entry e = r * THREADS
+// + tid, valid[r] for e < n): pos[r] = the entry's position in (key, e)
+// order.  s_cnt: (1 << bb) + 1 LDS words; kv: n entries (LDS or global
+// scratch) for the placed (key, e).  Ends with a barrier after which kv and
+// s_cnt are no longer read.
+template <int THREADS, int ITEMS, typename KV>
+__device__ __forceinline__ void bucket_positions(int n, const uint32_t (&key)[ITEMS],
+                                                 const bool (&valid)[ITEMS], uint32_t kmin,
+                                                 uint32_t kmax, int bb, uint32_t* s_cnt,
+                                                 uint32_t* s_tmp, KV kv,
+                                                 uint32_t (&pos)[ITEMS]) {
+    constexpr int WAVES = THREADS / kWave;
+    const int tid = threadIdx.x;
+    const int nb = 1 << bb;                 // nb >= THREADS
+    const uint32_t span = kmax - kmin;
+    const int span_bits = 32 - __builtin_clz(span);
+    const int shift = span_bits > bb ? span_bits - bb : 0;
+    for (int i = tid; i <= nb; i += THREADS) s_cnt[i] = 0u;
+    __syncthreads();
+    uint32_t bk[ITEMS], slot[ITEMS];
+#pragma unroll
+    for (int r = 0; r < ITEMS; ++r) {
+        bk[r] = (key[r] - kmin) >> shift;
+        slot[r] = valid[r] ? atomicAdd(&s_cnt[bk[r]], 1u) : 0u;
+    }
+    __syncthreads();
+    // bucket starts: each thread scans nb / THREADS consecutive buckets
+    const int per = nb / THREADS;
+    const int b0 = tid * per;
+    uint32_t sum = 0;
+    for (int k = 0; k < per; ++k) sum += s_cnt[b0 + k];
+    uint32_t run = block_exclusive_scan<WAVES>(sum, s_tmp);
+    for (int k = 0; k < per; ++k) {
+        const uint32_t c = s_cnt[b0 + k];
+        s_cnt[b0 + k] = run;
+        run += c;
+    }
+    if (tid == THREADS - 1) s_cnt[nb] = run;           // = n
+    __syncthreads();
+    uint32_t st[ITEMS], cnt[ITEMS];
+#pragma unroll
+    for (int r = 0; r < ITEMS; ++r) {
+        st[r] = 0u;
+        cnt[r] = 0u;
+        if (valid[r]) {
+            st[r] = s_cnt[bk[r]];
+            cnt[r] = s_cnt[bk[r] + 1] - st[r];
+            kv[st[r] + slot[r]] = make_uint2(key[r], (uint32_t)(r * THREADS + tid));
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < ITEMS; ++r) {
+        uint32_t rank = 0;
+        if (valid[r]) {
+            const uint32_t e = (uint32_t)(r * THREADS + tid);
+            for (uint32_t j = st[r]; j < st[r] + cnt[r]; ++j)
+                rank += entry_before(kv[j], key[r], e) ? 1u : 0u;
+        }
+        pos[r] = st[r] + rank;
+    }
+    __syncthreads();
+}
+
+struct SegShared {
+    uint2 kv[kSegCap];                              // placed (key, position)
+    uint32_t cnt[(1 << kSegBucketBits) + 1];        // bucket counts / starts; then the ids
+    uint32_t red[8];
+    uint32_t tmp[4];
+};
+static_assert((1 << kSegBucketBits) + 1 >= kSegCap, "the ids are staged in cnt");
+static_assert((1 << kSegBucketBits) >= kSegThreads, "bucket scan: one or more per thread");
+static_assert((1 << kBigMaxBucketBits) <= kBigCap, "");
+
+// One workgroup per tile.  Runs of at most kSegCap entries are ordered in LDS
+// and written back in place; longer ones are queued for the big kernel.
+__global__ __launch_bounds__(kSegThreads) void tile_depth_sort_kernel(
+    const int2* __restrict__ bins, const uint32_t* __restrict__ dkeys,
+    uint32_t* __restrict__ ids, int* __restrict__ big_count, int* __restrict__ big_tiles) {
+    __shared__ SegShared sh;
+    const int t = blockIdx.x;
+    const int2 b = bins[t];
+    const int n = b.y - b.x;
+    if (n <= 1) return;                                 // uniform
+    if (n > kSegCap) {
+        if (threadIdx.x == 0) big_tiles[atomicAdd(big_count, 1)] = t;
+        return;
+    }
+    const int tid = threadIdx.x;
+    uint32_t key[kSegItems], val[kSegItems], pos[kSegItems];
+    bool valid[kSegItems];
+#pragma unroll
+    for (int r = 0; r < kSegItems; ++r) {
+        const int e = r * kSegThreads + tid;
+        valid[r] = e < n;
+        val[r] = valid[r] ? ids[b.x + e] : 0u;
+    }
+    uint32_t kmin = ~0u, kmax = 0u;
+#pragma unroll
+    for (int r = 0; r < kSegItems; ++r) {
+        key[r] = valid[r] ? dkeys[val[r]] : 0u;
+        if (valid[r]) {
+            kmin = min(kmin, key[r]);
+            kmax = max(kmax, key[r]);
+        }
+    }
+    block_minmax_u32<kSegThreads / kWave>(kmin, kmax, sh.red);
+    if (kmin == kmax) return;                           // all keys equal: already in order
+    bucket_positions<kSegThreads, kSegItems>(n, key, valid, kmin, kmax, kSegBucketBits, sh.cnt,
+                                             sh.tmp, sh.kv, pos);
+#pragma unroll
+    for (int r = 0; r < kSegItems; ++r)
+        if (valid[r]) sh.cnt[pos[r]] = val[r];
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kSegItems; ++r) {
+        const int e = r * kSegThreads + tid;
+        if (valid[r]) ids[b.x + e] = sh.cnt[e];
+    }
+}
+
+// Queued runs of kSegCap < n <= kBigCap entries, one 1024-thread workgroup
+// per run at a time; the placed entries go to kv_tmp at the run's offsets.
+// Longer runs are queued for the long kernel.
+__global__ __launch_bounds__(kBigThreads) void tile_depth_sort_big_kernel(
+    const int2* __restrict__ bins, const uint32_t* __restrict__ dkeys,
+    uint32_t* __restrict__ ids, const int* __restrict__ big_count,
+    const int* __restrict__ big_tiles, uint2* __restrict__ kv_tmp,
+    int* __restrict__ long_count, int* __restrict__ long_tiles) {
+    __shared__ uint32_t s_cnt[(1 << kBigMaxBucketBits) + 1];
+    __shared__ uint32_t s_red[2 * (kBigThreads / kWave)];
+    __shared__ uint32_t s_tmp[kBigThreads / kWave];
+    const int tid = threadIdx.x;
+    const int count = *big_count;
+    for (int qi = blockIdx.x; qi < count; qi += gridDim.x) {
+        const int t = big_tiles[qi];
+        const int2 b = bins[t];
+        const int n = b.y - b.x;
+        if (n > kBigCap) {
+            if (tid == 0) long_tiles[atomicAdd(long_count, 1)] = t;
+            continue;                                   // uniform
+        }
+        uint32_t key[kBigItems], val[kBigItems], pos[kBigItems];
+        bool valid[kBigItems];
+#pragma unroll
+        for (int r = 0; r < kBigItems; ++r) {
+            const int e = r * kBigThreads + tid;
+            valid[r] = e < n;
+            val[r] = valid[r] ? ids[b.x + e] : 0u;
+        }
+        uint32_t kmin = ~0u, kmax = 0u;
+#pragma unroll
+        for (int r = 0; r < kBigItems; ++r) {
+            key[r] = valid[r] ? dkeys[val[r]] : 0u;
+            if (valid[r]) {
+                kmin = min(kmin, key[r]);
+                kmax = max(kmax, key[r]);
+            }
+        }
+        block_minmax_u32<kBigThreads / kWave>(kmin, kmax, s_red);
+        if (kmin == kmax) continue;                     // uniform; the ids stay
+        // about two buckets per entry, at least one per thread
+        int bb = 32 - __builtin_clz((uint32_t)n);
+        bb = min(max(bb + 1, 10), kBigMaxBucketBits);
+        bucket_positions<kBigThreads, kBigItems>(n, key, valid, kmin, kmax, bb, s_cnt, s_tmp,
+                                                 kv_tmp + b.x, pos);
+        // every lane has read its ids and the placed entries: write in place
+#pragma unroll
+        for (int r = 0; r < kBigItems; ++r)
+            if (valid[r]) ids[b.x + pos[r]] = val[r];
+        __syncthreads();
+    }
+}
+
+// The chunked radix sort of the longest runs (more than kBigCap entries).
+// Layout of a chunk of m <= kSegCap entries: wave w holds entries
+// [w * R * 64, (w + 1) * R * 64), R = ceil(m / 256), entry w*R*64 + r*64 + l
+// in its register slot r of lane l.  Ranking is stable in that order: one
+// 64-lane ballot per digit bit gives the lanes with the same digit, and a
+// per-wave LDS histogram carries the counts across the wave's rounds; per
+// digit, the waves' counts are then turned into offsets in wave order.
+struct RadixShared {
+    uint2 kv[kSegCap];
+    uint16_t whist[4][kSegBins];            // per-wave digit counts, then wave offsets
+    uint32_t start[kSegBins + 1];           // digit start inside the chunk; [kSegBins]: its size
+    uint32_t red[8];
+    uint32_t tmp[4];
+};
+
+// One stable counting pass over the chunk held in registers: pos[r] = the
+// entry's position in digit order inside the chunk.  Leaves sh.start[d] =
+// digit d's start in the chunk and sh.start[kSegBins] = the chunk's size.
+// Begins and ends with a barrier.
+__device__ __forceinline__ void seg_rank_pass(const uint32_t (&key)[kSegItems],
+                                              const bool (&valid)[kSegItems], int R, int shift,
+                                              int bits, RadixShared& sh,
+                                              uint32_t (&pos)[kSegItems]) {
+    const int tid = threadIdx.x;
+    const int lane = lane_id();
+    const int wave = tid >> 6;
+    uint32_t* wh = reinterpret_cast<uint32_t*>(&sh.whist[0][0]);
+    for (int i = tid; i < 2 * kSegBins; i += kSegThreads) wh[i] = 0u;   // 4 x kSegBins halves
+    __syncthreads();
+    const uint32_t dmask = (1u << bits) - 1u;
+    const uint64_t lt = __lanemask_lt();
+    uint32_t rank[kSegItems];
+#pragma unroll
+    for (int r = 0; r < kSegItems; ++r) {
+        rank[r] = 0u;
+        if (r < R) {                                    // uniform
+            const uint32_t d = (key[r] >> shift) & dmask;
+            const uint64_t peers = match_digit(d, bits, ballot(valid[r]));
+            uint32_t prior = 0;
+            if (valid[r]) prior = sh.whist[wave][d];
+            // the highest lane of each digit group advances the wave's count
+            const bool leader = valid[r] && ((peers & ~(lt | (1ull << lane))) == 0);
+            if (leader) sh.whist[wave][d] = (uint16_t)(prior + (uint32_t)__popcll(peers));
+            rank[r] = prior + (uint32_t)__popcll(peers & lt);
+            wave_lds_sync();
+        }
+    }
+    __syncthreads();
+    // digits 2*tid and 2*tid + 1: offsets of the waves inside the digit, the
+    // digit's count, and (block scan) its start
+    uint32_t tot[2];
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        const int d = 2 * tid + j;
+        uint32_t t = 0;
+#pragma unroll
+        for (int w = 0; w < 4; ++w) {
+            const uint32_t c = sh.whist[w][d];
+            sh.whist[w][d] = (uint16_t)t;
+            t += c;
+        }
+        tot[j] = t;
+    }
+    const uint32_t ex = block_exclusive_scan256(tot[0] + tot[1], sh.tmp);
+    sh.start[2 * tid] = ex;
+    sh.start[2 * tid + 1] = ex + tot[0];
+    if (tid == kSegThreads - 1) sh.start[kSegBins] = ex + tot[0] + tot[1];
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kSegItems; ++r) {
+        pos[r] = 0u;
+        if (r < R && valid[r]) {
+            const uint32_t d = (key[r] >> shift) & dmask;
+            pos[r] = sh.start[d] + sh.whist[wave][d] + rank[r];
+        }
+    }
+    __syncthreads();
+}
+
+// Queued runs of more than kBigCap entries: one workgroup per run at a time.
+// The run's varying key bits (OR ^ AND) are sorted in passes of at most 9
+// bits; each pass counts the run's digits, then ranks it chunk by chunk
+// (kSegCap entries in order) and scatters every chunk to the digit cursors in
+// global memory, ping-ponging between (ka, va) and (kb, vb) at the run's own
+// offsets; the ids go back in place at the end.
+__global__ __launch_bounds__(kSegThreads) void tile_depth_sort_long_kernel(
+    const int2* __restrict__ bins, const uint32_t* __restrict__ dkeys,
+    uint32_t* __restrict__ ids, const int* __restrict__ long_count,
+    const int* __restrict__ long_tiles, uint32_t* __restrict__ ka, uint32_t* __restrict__ va,
+    uint32_t* __restrict__ kb, uint32_t* __restrict__ vb) {
+    __shared__ RadixShared sh;
+    __shared__ uint32_t s_cursor[kSegBins];
+    const int tid = threadIdx.x;
+    const int lane = lane_id();
+    const int wave = tid >> 6;
+    const int count = *long_count;
+    for (int li = blockIdx.x; li < count; li += gridDim.x) {
+        const int2 b = bins[long_tiles[li]];
+        const int n = b.y - b.x;
+        uint32_t o = 0u, a = ~0u;
+        for (int i = tid; i < n; i += kSegThreads) {
+            const uint32_t g = ids[b.x + i];
+            const uint32_t k = dkeys[g];
+            ka[b.x + i] = k;
+            va[b.x + i] = g;
+            o |= k;
+            a &= k;
+        }
+        // the bits that differ between some two keys: OR ^ AND of the run
+        uint32_t or_all = o, and_all = a;
+#pragma unroll
+        for (int off = 1; off < kWave; off <<= 1) {
+            or_all |= (uint32_t)__shfl_xor((int)or_all, off);
+            and_all &= (uint32_t)__shfl_xor((int)and_all, off);
+        }
+        if (lane == 0) {
+            sh.red[wave] = or_all;
+            sh.red[4 + wave] = and_all;
+        }
+        __syncthreads();
+        const uint32_t diff = ((sh.red[0] | sh.red[1]) | (sh.red[2] | sh.red[3])) ^
+                              ((sh.red[4] & sh.red[5]) & (sh.red[6] & sh.red[7]));
+        __syncthreads();
+        if (diff == 0u) continue;                       // uniform; the ids stay
+        const int lo = __builtin_ctz(diff);
+        const int hi = 32 - __builtin_clz(diff);
+        const int passes = (hi - lo + kSegMaxBits - 1) / kSegMaxBits;
+        const int bits = (hi - lo + passes - 1) / passes;
+        const uint32_t* sk = ka;
+        const uint32_t* sv = va;
+        uint32_t* dk = kb;
+        uint32_t* dv = vb;
+        for (int q = 0; q < passes; ++q) {
+            const int shift = lo + q * bits;
+            const int bq = min(bits, hi - shift);
+            const uint32_t dmask = (1u << bq) - 1u;
+            // the run's digit counts, scanned into the cursors
+            for (int d = tid; d < kSegBins; d += kSegThreads) s_cursor[d] = 0u;
+            __syncthreads();
+            for (int i = tid; i < n; i += kSegThreads)
+                atomicAdd(&s_cursor[(sk[b.x + i] >> shift) & dmask], 1u);
+            __syncthreads();
+            const uint32_t c0 = s_cursor[2 * tid], c1 = s_cursor[2 * tid + 1];
+            const uint32_t ex = block_exclusive_scan256(c0 + c1, sh.tmp);
+            s_cursor[2 * tid] = ex;
+            s_cursor[2 * tid + 1] = ex + c0;
+            __syncthreads();
+            for (int c = 0; c < n; c += kSegCap) {
+                const int m = min(kSegCap, n - c);
+                const int R = (m + kSegThreads - 1) / kSegThreads;
+                const int wbase = wave * R * kWave;
+                uint32_t key[kSegItems], val[kSegItems], pos[kSegItems];
+                bool valid[kSegItems];
+#pragma unroll
+                for (int r = 0; r < kSegItems; ++r) {
+                    const int i = wbase + r * kWave + lane;
+                    valid[r] = r < R && i < m;
+                    key[r] = valid[r] ? sk[b.x + c + i] : 0u;
+                    val[r] = valid[r] ? sv[b.x + c + i] : 0u;
+                }
+                seg_rank_pass(key, valid, R, shift, bq, sh, pos);
+#pragma unroll
+                for (int r = 0; r < kSegItems; ++r) {
+                    if (valid[r]) {
+                        const uint32_t d = (key[r] >> shift) & dmask;
+                        const uint32_t out = (uint32_t)b.x + s_cursor[d] + (pos[r] - sh.start[d]);
+                        dk[out] = key[r];
+                        dv[out] = val[r];
+                    }
+                }
+                __syncthreads();
+                s_cursor[2 * tid] += sh.start[2 * tid + 1] - sh.start[2 * tid];
+                s_cursor[2 * tid + 1] += sh.start[2 * tid + 2] - sh.start[2 * tid + 1];
+                __syncthreads();
+            }
+            const uint32_t* tk = dk;
+            const uint32_t* tv = dv;
+            dk = const_cast<uint32_t*>(sk);
+            dv = const_cast<uint32_t*>(sv);
+            sk = tk;
+            sv = tv;
+        }
+        for (int i = tid; i < n; i += kSegThreads) ids[b.x + i] = sv[b.x + i];
+        __syncthreads();
+    }
+}
+
+// The per-tile depth order of every run of bins[0..num_bins): queue[0] and
+// queue[1] (zeroed) count the runs passed to the big and the long kernel,
+// whose tiles go to queue_tiles[0..num_bins) and [num_bins..2 num_bins).
+// scratch: four free buffers of the intersection capacity (the first two
+// adjacent, as the big kernel's uint2 scratch).
+hipError_t tile_depth_order(const int2* bins, int num_bins, const uint32_t* dkeys,
+                            uint32_t* ids, int* queue, int* queue_tiles, uint32_t* ka,
+                            uint32_t* va, uint32_t* kb, uint32_t* vb, hipStream_t s) {
+    if (num_bins <= 0) return hipSuccess;
+    int* big_tiles = queue_tiles;
+    int* long_tiles = queue_tiles + num_bins;
+    tile_depth_sort_kernel<<<num_bins, kSegThreads, 0, s>>>(bins, dkeys, ids, queue, big_tiles);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const int qb = num_bins < kSegQueueBlocks ? num_bins : kSegQueueBlocks;
+    tile_depth_sort_big_kernel<<<qb, kBigThreads, 0, s>>>(
+        bins, dkeys, ids, queue, big_tiles, reinterpret_cast<uint2*>(ka), queue + 1, long_tiles);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    tile_depth_sort_long_kernel<<<qb, kSegThreads, 0, s>>>(bins, dkeys, ids, queue + 1,
+                                                          long_tiles, ka, va, kb, vb);
+    return hipGetLastError();
+}
+
 constexpr size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
 // Emission buffers (passes 1-4 above) for n entries and a capacity of cap slots.
@@ -467,8 +956,8 @@ size_t emit_layout(size_t o, int n, int cap, bool batch, EmitLayout& E) {
     return o;
 }
 
-// Passes 1-4 over the entries perm[0..n) in emission order, n = *n_dev when
-// given (the depth sort's kept keys), else n_host; launch shapes follow
+// Passes 1-4 over the entries perm[0..n) in emission order (perm == nullptr:
+// the ids 0..n-1), n = *n_dev when given, else n_host; launch shapes follow
 // n_host, an upper bound.  Writes the slots written (min(total, cap)) to
 // info[0] and the number of emitting entries to info[1] (info: the layout's
 // slot if null), the exact total to *total if total is not null, and bit 8 of
@@ -503,8 +992,9 @@ hipError_t emit(int n, const int* n_dev, int tiles_x_plain, const ViewRec* views
     emit_scan_kernel<<<1, 1024, 0, s>>>(part_tot, part_nz, parts, part_off, part_nzoff, cap,
                                         info, total, err);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    emit_compact_kernel<BATCH><<<parts, 256, 0, s>>>(n, n_dev, views, vkeys, perm, rects,
-                                                     part_off, part_nzoff, info, E);
+    emit_compact_kernel<BATCH><<<parts, 256, 0, s>>>(n, n_dev, views, vkeys, perm,
+                                                     perm ? rects : rect_by_id, part_off,
+                                                     part_nzoff, info, E);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     const long long chunks = ((long long)cap + 63) / 64;
     const int waves = (int)(chunks < kExpandWaves ? chunks : kExpandWaves);
@@ -514,34 +1004,31 @@ hipError_t emit(int n, const int* n_dev, int tiles_x_plain, const ViewRec* views
     return hipGetLastError();
 }
 
-// Depth stage: needs no intersection count, so a caller can run it while it
-// reads the count back.  Its workspace depends on the Gaussian count only.
+// Depth stage: per-Gaussian depth keys and rectangles.  It needs no
+// intersection count, so a caller can run it while it reads the count back;
+// its workspace depends on the Gaussian count only.
 struct DepthLayout {
-    size_t n_kept, depth_sort, dkeys, rect_by_id, dtmp_k, dtmp_v, dsorted_k, perm, end;
+    size_t dkeys, rect_by_id, end;
 };
 
 DepthLayout depth_layout(const BinPlan& p) {
     DepthLayout L;
     size_t o = 0;
-    L.n_kept = o; o = align256(o + 16);     // Gaussians that emit (kept by the depth sort)
-    L.depth_sort = o; o = align256(o + p.depth_sort.scratch_bytes);
     const size_t n4 = align256(4ull * (p.n > 0 ? p.n : 1));
     L.dkeys = o; o += n4;
     L.rect_by_id = o; o += 2 * n4;
-    L.dtmp_k = o; o += n4;
-    L.dtmp_v = o; o += n4;
-    L.dsorted_k = o; o += n4;
-    L.perm = o; o += n4;
     L.end = o;
     return L;
 }
 
-// Emission stage (emission, tile sort, bin edges); the error word comes first.
-// The zeroed header is part of the C ABI (jaxsplat_bin_emit_sort): the error
-// word at byte 0 and the exact intersection total (uint64) at byte 8.
+// Emission stage (emission, tile sort, bin edges, per-tile depth order); the
+// error word comes first.  The zeroed header is part of the C ABI
+// (jaxsplat_bin_emit_sort): the error word at byte 0 and the exact
+// intersection total (uint64) at byte 8; the counts of runs queued for the
+// longer per-tile sorts follow at bytes 16 and 20.
 struct BinLayout {
-    size_t err, total, zero_end;
-    size_t tile_sort, tiles, gids, ttmp_k, ttmp_v, tiles_sorted, end;
+    size_t err, total, queue, zero_end;
+    size_t tile_sort, queue_tiles, tiles, gids, ttmp_k, ttmp_v, tiles_sorted, end;
     EmitLayout emit;
 };
 
@@ -550,10 +1037,12 @@ BinLayout layout(const BinPlan& p) {
     size_t o = 0;
     L.err = o;
     L.total = o + 8;
-    o += 16;
+    L.queue = o + 16;
+    o += 32;
     L.zero_end = o;
     o = align256(o);
     L.tile_sort = o; o = align256(o + p.tile_sort.scratch_bytes);
+    L.queue_tiles = o; o = align256(o + 8ull * (p.num_tiles > 0 ? p.num_tiles : 1));
     o = emit_layout(o, p.n, p.isect_cap, false, L.emit);
     const size_t i4 = align256(4ull * (p.isect_cap > 0 ? p.isect_cap : 1));
     L.tiles = o; o += i4;
@@ -575,7 +1064,6 @@ BinPlan bin_plan(const Camera& c, int isect_cap) {
     p.isect_cap = isect_cap;
     p.num_tiles = c.tiles_x * c.tiles_y;
     p.tile_bits = bitlen((uint32_t)(p.num_tiles > 1 ? p.num_tiles - 1 : 1));
-    p.depth_sort = sort_plan(p.n, 0, 32);
     p.tile_sort = sort_plan(isect_cap, 0, p.tile_bits);
     p.depth_bytes = align256(depth_layout(p).end);
     p.emit_bytes = align256(layout(p).end);
@@ -592,16 +1080,10 @@ hipError_t bin_depth(const BinPlan& p, const Camera& c, const float* xys, const 
     if (p.n <= 0) return hipSuccess;
     const DepthLayout D = depth_layout(p);
     uint8_t* b = static_cast<uint8_t*>(depth_ws);
-    auto U = [&](size_t off) { return reinterpret_cast<uint32_t*>(b + off); };
     depth_keys_kernel<<<(p.n + 255) / 256, 256, 0, s>>>(
-        c, depths, radii, reinterpret_cast<const float2*>(xys), rects, U(D.dkeys),
-        reinterpret_cast<uint2*>(b + D.rect_by_id));
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    // only the Gaussians that emit are sorted: the first pass drops the others
-    return sort_pairs(p.depth_sort, U(D.dkeys), nullptr, U(D.dsorted_k), U(D.perm), U(D.dtmp_k),
-                      U(D.dtmp_v), p.n, nullptr, b + D.depth_sort, nullptr, s,
-                      reinterpret_cast<int*>(b + D.n_kept));
+        c, depths, radii, reinterpret_cast<const float2*>(xys), rects,
+        reinterpret_cast<uint32_t*>(b + D.dkeys), reinterpret_cast<uint2*>(b + D.rect_by_id));
+    return hipGetLastError();
 }
 
 hipError_t bin_emit(const BinPlan& p, const Camera& c, int num_isect_host,
@@ -618,32 +1100,41 @@ hipError_t bin_emit(const BinPlan& p, const Camera& c, int num_isect_host,
     auto U = [&](size_t off) { return reinterpret_cast<uint32_t*>(b + off); };
     const DepthLayout D = depth_layout(p);
     const uint8_t* d = static_cast<const uint8_t*>(depth_ws);
-    const uint32_t* perm = reinterpret_cast<const uint32_t*>(d + D.perm);
+    const uint32_t* dkeys = reinterpret_cast<const uint32_t*>(d + D.dkeys);
     const uint2* rect_by_id = reinterpret_cast<const uint2*>(d + D.rect_by_id);
     // the sort and the edges cover the slots the emission wrote, min(total,
     // capacity), read on the device: a count given too large (an inconsistent
     // cum_tiles_hit) then never sorts unwritten slots, and one too small is
     // an overflow recorded in the error word
     int* written = reinterpret_cast<int*>(b + L.emit.info);
-    // the expansion also counts the tile sort's first digit per sort partition
+    // emission in id order; the expansion also counts the tile sort's first
+    // digit per sort partition
     const FirstCounts fc = first_counts(p.tile_sort, b + L.tile_sort);
-    e = emit<false>(p.n, reinterpret_cast<const int*>(d + D.n_kept), c.tiles_x, nullptr,
-                    nullptr, perm, rect_by_id, p.isect_cap, b, L.emit, written,
+    e = emit<false>(p.n, nullptr, c.tiles_x, nullptr, nullptr, nullptr, rect_by_id,
+                    p.isect_cap, b, L.emit, written,
                     reinterpret_cast<unsigned long long*>(b + L.total), err, U(L.tiles),
                     U(L.gids), nullptr, fc, s);
     if (e != hipSuccess) return e;
     stage_mark(kMarkEmitted, s);
     uint32_t* tiles_out = tiles_sorted ? reinterpret_cast<uint32_t*>(tiles_sorted) : U(L.tiles_sorted);
-    e = sort_pairs(p.tile_sort, U(L.tiles), U(L.gids), tiles_out,
-                   reinterpret_cast<uint32_t*>(gids_sorted), U(L.ttmp_k), U(L.ttmp_v),
-                   num_isect_host, written, b + L.tile_sort, err, s, nullptr, fc.counts != nullptr);
+    uint32_t* ids = reinterpret_cast<uint32_t*>(gids_sorted);
+    e = sort_pairs(p.tile_sort, U(L.tiles), U(L.gids), tiles_out, ids, U(L.ttmp_k),
+                   U(L.ttmp_v), num_isect_host, written, b + L.tile_sort, err, s,
+                   fc.counts != nullptr);
     if (e != hipSuccess) return e;
     stage_mark(kMarkTileSorted, s);
     const int nb = (int)(((long long)p.isect_cap + 256 * kTileBinsPer - 1) / (256 * kTileBinsPer));
     tile_bins_kernel<<<nb > 0 ? nb : 1, 256, 0, s>>>(tiles_out, num_isect_host, written,
                                                    p.num_tiles,
                                                    reinterpret_cast<int2*>(tile_bins), err);
-    return hipGetLastError();
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    stage_mark(kMarkBinned, s);
+    // per-tile depth order; the sort's ping-pong buffers and the emission's
+    // pair buffers are free again and serve the longer runs
+    return tile_depth_order(reinterpret_cast<const int2*>(tile_bins), p.num_tiles, dkeys, ids,
+                            reinterpret_cast<int*>(b + L.queue),
+                            reinterpret_cast<int*>(b + L.queue_tiles), U(L.ttmp_k),
+                            U(L.ttmp_v), U(L.tiles), U(L.gids), s);
 }
 
 // One workspace: the emission stage's region (error word first), then the
@@ -662,21 +1153,20 @@ hipError_t bin_and_sort(const BinPlan& p, const Camera& c, const float* xys, con
 // ---------------------------------------------------------------------------
 // Batched binning (items of many views; see ViewRec)
 // ---------------------------------------------------------------------------
-//  1. stable sort of the items by depth key (4 passes): (depth bits, item);
-//  2. view_keys + one stable pass on the view bits: (view, depth bits, item),
-//     the emission order;
-//  3. emission (BATCH): keys view << tile_bits | tile, values = item ids,
-//     with the total and the clamped count published on the device;
-//  4. stable sort on the tile bits only (the view bits ride along): order
-//     (tile, view, depth bits, item), so every (view, tile) run is contiguous
-//     and in the reference's per-tile order;
-//  5. tile_bins_kernel on the full keys.
+// Items are numbered view by view (ViewRec::item_off), so the item order is
+// the (view, item) order:
+//  1. view_keys: every item's view;
+//  2. emission (BATCH) in item order: keys view << tile_bits | tile, values =
+//     item ids, with the total and the clamped count published on the device;
+//  3. stable sort on the tile bits only (the view bits ride along): order
+//     (tile, view, item), so every (view, tile) run is contiguous;
+//  4. tile_bins_kernel on the full keys;
+//  5. the per-tile depth order of every (view, tile) run (as for one view).
 namespace {
 
 struct BatchLayout {
-    size_t err, count, total, kept, zero_end;
-    size_t depth_sort, view_sort, tile_sort, dsorted_k, perm, dtmp_k, dtmp_v, vkeys, vsorted,
-        perm2, keys, vals, ttmp_k, ttmp_v, keys_sorted, end;
+    size_t err, count, total, queue, zero_end;
+    size_t tile_sort, queue_tiles, vkeys, keys, vals, ttmp_k, ttmp_v, keys_sorted, end;
     EmitLayout emit;
 };
 
@@ -686,22 +1176,15 @@ BatchLayout batch_layout(const BatchBinPlan& p) {
     L.err = o; o += 16;
     L.count = o; o += 16;
     L.total = o; o += 16;
-    L.kept = o; o += 16;
+    L.queue = o; o += 16;
     L.zero_end = o;
     o = align256(o);
-    L.depth_sort = o; o = align256(o + p.depth_sort.scratch_bytes);
-    L.view_sort = o; o = align256(o + p.view_sort.scratch_bytes);
     L.tile_sort = o; o = align256(o + p.tile_sort.scratch_bytes);
+    L.queue_tiles = o; o = align256(o + 8ull * (p.num_bins > 0 ? p.num_bins : 1));
     o = emit_layout(o, p.n_items, p.isect_cap, true, L.emit);
     const size_t n4 = align256(4ull * (p.n_items > 0 ? p.n_items : 1));
     const size_t i4 = align256(4ull * (p.isect_cap > 0 ? p.isect_cap : 1));
-    L.dsorted_k = o; o += n4;
-    L.perm = o; o += n4;
-    L.dtmp_k = o; o += n4;
-    L.dtmp_v = o; o += n4;
     L.vkeys = o; o += n4;
-    L.vsorted = o; o += n4;
-    L.perm2 = o; o += n4;
     L.keys = o; o += i4;
     L.vals = o; o += i4;
     L.ttmp_k = o; o += i4;
@@ -721,8 +1204,6 @@ BatchBinPlan batch_bin_plan(int n_items, int num_views, int max_view_tiles, int 
     p.tile_bits = bitlen((uint32_t)(max_view_tiles > 1 ? max_view_tiles - 1 : 1));
     p.view_bits = bitlen((uint32_t)(num_views > 1 ? num_views - 1 : 0));
     p.num_bins = num_views << p.tile_bits;
-    p.depth_sort = sort_plan(n_items, 0, 32);
-    p.view_sort = sort_plan(n_items, 0, p.view_bits);
     p.tile_sort = sort_plan(isect_cap, 0, p.tile_bits);
     p.scratch_bytes = batch_layout(p).end;
     return p;
@@ -758,38 +1239,26 @@ hipError_t batch_bin(const BatchBinPlan& p, const ViewRec* views, const uint2* i
     int* count = reinterpret_cast<int*>(b + L.count);
     auto U = [&](size_t off) { return reinterpret_cast<uint32_t*>(b + off); };
 
-    // only the items that emit are sorted: the first depth pass drops the
-    // others (key 0xFFFFFFFF) and counts the rest in `kept`
-    int* kept = reinterpret_cast<int*>(b + L.kept);
-    e = sort_pairs(p.depth_sort, depth_keys, nullptr, U(L.dsorted_k), U(L.perm), U(L.dtmp_k),
-                   U(L.dtmp_v), p.n_items, nullptr, b + L.depth_sort, err, s, kept);
-    if (e != hipSuccess) return e;
-    view_keys_kernel<<<(p.n_items + 255) / 256, 256, 0, s>>>(U(L.perm), kept, views,
+    view_keys_kernel<<<(p.n_items + 255) / 256, 256, 0, s>>>(nullptr, p.n_items, nullptr, views,
                                                               p.num_views, U(L.vkeys));
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    const uint32_t* order = U(L.perm);
-    const uint32_t* order_views = U(L.vkeys);
-    if (p.view_bits > 0) {
-        e = sort_pairs(p.view_sort, U(L.vkeys), U(L.perm), U(L.vsorted), U(L.perm2),
-                       U(L.dtmp_k), U(L.dtmp_v), p.n_items, kept, b + L.view_sort, err, s);
-        if (e != hipSuccess) return e;
-        order = U(L.perm2);
-        order_views = U(L.vsorted);
-    }
     const FirstCounts fc = first_counts(p.tile_sort, b + L.tile_sort);
-    e = emit<true>(p.n_items, kept, 0, views, order_views, order, item_rects, p.isect_cap, b,
-                   L.emit, count,
-                   reinterpret_cast<unsigned long long*>(b + L.total), err, U(L.keys),
-                   U(L.vals), item_first, fc, s);
+    e = emit<true>(p.n_items, nullptr, 0, views, U(L.vkeys), nullptr, item_rects, p.isect_cap,
+                   b, L.emit, count, reinterpret_cast<unsigned long long*>(b + L.total), err,
+                   U(L.keys), U(L.vals), item_first, fc, s);
     if (e != hipSuccess) return e;
-    e = sort_pairs(p.tile_sort, U(L.keys), U(L.vals), U(L.keys_sorted),
-                   reinterpret_cast<uint32_t*>(items_sorted), U(L.ttmp_k), U(L.ttmp_v), 0, count,
-                   b + L.tile_sort, err, s, nullptr, fc.counts != nullptr);
+    uint32_t* ids = reinterpret_cast<uint32_t*>(items_sorted);
+    e = sort_pairs(p.tile_sort, U(L.keys), U(L.vals), U(L.keys_sorted), ids, U(L.ttmp_k),
+                   U(L.ttmp_v), 0, count, b + L.tile_sort, err, s, fc.counts != nullptr);
     if (e != hipSuccess) return e;
     const int nb = (int)(((long long)p.isect_cap + 256 * kTileBinsPer - 1) / (256 * kTileBinsPer));
     tile_bins_kernel<<<nb > 0 ? nb : 1, 256, 0, s>>>(U(L.keys_sorted), 0, count, p.num_bins,
                                                    reinterpret_cast<int2*>(tile_bins), err);
-    return hipGetLastError();
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    return tile_depth_order(reinterpret_cast<const int2*>(tile_bins), p.num_bins, depth_keys,
+                            ids, reinterpret_cast<int*>(b + L.queue),
+                            reinterpret_cast<int*>(b + L.queue_tiles), U(L.ttmp_k),
+                            U(L.ttmp_v), U(L.keys), U(L.vals), s);
 }
 
 }  // namespace jsa

@@ -122,6 +122,34 @@ __device__ __forceinline__ T wave_inclusive_sum(T v) {
     return v;
 }
 
+// Mask of the lanes of this wave whose `bits`-bit digit equals this lane's
+// (restricted to lanes in `valid`): one ballot per digit bit.
+__device__ __forceinline__ uint64_t match_digit(uint32_t d, int bits, uint64_t valid) {
+    uint64_t peers = valid;
+    for (int b = 0; b < bits; ++b) {
+        const bool bit = (d >> b) & 1u;
+        const uint64_t bb = ballot(bit);
+        peers &= bit ? bb : ~bb;
+    }
+    return peers;
+}
+
+// Exclusive scan of one value per thread over a 256-thread block (s_tmp: 4
+// words of LDS; two barriers).
+__device__ __forceinline__ uint32_t block_exclusive_scan256(uint32_t v, uint32_t* s_tmp) {
+    const int tid = threadIdx.x;
+    const int lane = lane_id();
+    const int wave = tid >> 6;
+    const uint32_t incl = wave_inclusive_sum(v);
+    if (lane == 63) s_tmp[wave] = incl;
+    __syncthreads();
+    uint32_t off = 0;
+#pragma unroll
+    for (int w = 0; w < 4; ++w) off += (w < wave) ? s_tmp[w] : 0u;
+    __syncthreads();
+    return off + incl - v;
+}
+
 // Tile rectangle [x0, x1) x [y0, y1) of a Gaussian from its centre and integer
 // radius: forward.cu:152-161 / helpers.h:9-32 (the binning and the staged
 // backward must agree bit for bit, so both use this one function).
@@ -408,7 +436,8 @@ enum StageMark : int {
     kMarkEmitted = 0,       // bin_emit: emission done, tile sort next
     kMarkTileSorted = 1,    // bin_emit: tile sort done, bin edges next
     kMarkRecords = 2,       // staged rasterize backward: records done, reduction next
-    kNumStageMarks = 3,
+    kMarkBinned = 3,        // bin_emit: bin edges done, per-tile depth order next
+    kNumStageMarks = 4,
 };
 void stage_mark(int k, hipStream_t s);
 
@@ -485,14 +514,10 @@ uint32_t* sort_first_counts(const SortPlan& p, void* scratch);
 // tmp_keys/tmp_vals must hold n_cap entries (ping-pong buffers).  The scratch
 // area must be zeroed by the caller (sort_zero_bytes of it) before the call.
 size_t sort_zero_bytes(const SortPlan& p);
-// With n_kept (a device int), keys equal to 0xFFFFFFFF are dropped: the first
-// pass leaves them out and writes the number of keys kept to *n_kept, and the
-// output holds the kept keys only (the later passes read their count there).
 hipError_t sort_pairs(const SortPlan& p, const uint32_t* keys_in, const uint32_t* vals_in,
                       uint32_t* keys_out, uint32_t* vals_out, uint32_t* tmp_keys,
                       uint32_t* tmp_vals, int n_host, const int* n_dev, void* scratch,
-                      uint32_t* err, hipStream_t s, int* n_kept = nullptr,
-                      bool counted = false);
+                      uint32_t* err, hipStream_t s, bool counted = false);
 
 // ---- binning ----
 struct BinPlan {
@@ -500,19 +525,19 @@ struct BinPlan {
     int isect_cap;      // intersection capacity
     int num_tiles;
     int tile_bits;
-    SortPlan depth_sort;
     SortPlan tile_sort;
     size_t depth_bytes;     // depth stage workspace (depends on n only)
     size_t emit_bytes;      // emission stage workspace
     size_t scratch_bytes;   // both, for bin_and_sort: emission region first
 };
 BinPlan bin_plan(const Camera& c, int isect_cap);
-// Depth stage: depth keys, packed tile rectangles and the stable depth sort.
-// rects (optional, packed as pack_rect) replaces the rectangles computed from
-// xys and radii.
+// Depth stage: per-Gaussian depth keys (depth bits; 0xFFFFFFFF for Gaussians
+// that emit nothing) and packed tile rectangles.  rects (optional, packed as
+// pack_rect) replaces the rectangles computed from xys and radii.
 hipError_t bin_depth(const BinPlan& p, const Camera& c, const float* xys, const float* depths,
                      const int* radii, const uint2* rects, void* depth_ws, hipStream_t s);
-// Emission stage over the depth stage's workspace: emission, tile sort, edges.
+// Emission stage over the depth stage's workspace: emission in id order, the
+// stable tile sort, edges, and the per-tile depth order.
 // p.isect_cap bounds the slots written; the sort and the edges cover the slots
 // actually written, min(total, p.isect_cap), read on the device, so that a
 // num_isect_host larger than the total never sorts unwritten slots.
@@ -558,7 +583,7 @@ struct BatchBinPlan {
     int isect_cap;
     int tile_bits, view_bits;
     int num_bins;       // num_views << tile_bits
-    SortPlan depth_sort, view_sort, tile_sort;
+    SortPlan tile_sort;
     size_t scratch_bytes;
 };
 BatchBinPlan batch_bin_plan(int n_items, int num_views, int max_view_tiles, int isect_cap);

@@ -28,32 +28,18 @@ constexpr int kSortWaves = kSortThreads / kWave;
 constexpr int kDigitBits = 8;
 constexpr int kBins = 1 << kDigitBits;
 constexpr int kMaxPasses = 4;
-constexpr uint32_t kDropKey = 0xFFFFFFFFu;      // the key the first pass may drop
 
 __device__ __forceinline__ int read_n(int n_host, const int* n_dev) {
     return n_dev ? *n_dev : n_host;
 }
 
-// Mask of the lanes of this wave whose `bits`-bit digit equals this lane's
-// (restricted to lanes in `valid`): one ballot per digit bit.
-__device__ __forceinline__ uint64_t match_digit(uint32_t d, int bits, uint64_t valid) {
-    uint64_t peers = valid;
-    for (int b = 0; b < bits; ++b) {
-        const bool bit = (d >> b) & 1u;
-        const uint64_t bb = ballot(bit);
-        peers &= bit ? bb : ~bb;
-    }
-    return peers;
-}
-
 // Per-partition digit counts of one pass, stored digit-major:
 // counts[d * parts + part].  Per-wave LDS histograms keep same-digit runs
 // (frequent in emission order) from piling onto one LDS word.
-// With drop, keys equal to kDropKey are left out (counted nowhere).
 template <int ITEMS>
 __global__ __launch_bounds__(kSortThreads) void radix_upsweep(
     const uint32_t* __restrict__ keys, int n_host, const int* __restrict__ n_dev, int shift,
-    int bits, int parts, uint32_t* __restrict__ counts, int drop) {
+    int bits, int parts, uint32_t* __restrict__ counts) {
     constexpr int TILE = kSortThreads * ITEMS;
     __shared__ uint32_t s_hist[kSortWaves][kBins];
     const int tid = threadIdx.x;
@@ -68,10 +54,7 @@ __global__ __launch_bounds__(kSortThreads) void radix_upsweep(
 #pragma unroll
     for (int r = 0; r < ITEMS; ++r) {
         const int p = base + r * kSortThreads + tid;
-        if (p < n) {
-            const uint32_t k = keys[p];
-            if (!(drop && k == kDropKey)) atomicAdd(&s_hist[wave][(k >> shift) & mask], 1u);
-        }
+        if (p < n) atomicAdd(&s_hist[wave][(keys[p] >> shift) & mask], 1u);
     }
     __syncthreads();
     uint32_t v = 0;
@@ -143,31 +126,14 @@ __global__ __launch_bounds__(kSortThreads) void radix_colscan(uint32_t* __restri
     if (tid == 0) totals[blockIdx.x] = s_carry;
 }
 
-// Exclusive scan of one value per thread over the 256-thread block.
-__device__ __forceinline__ uint32_t block_exclusive_scan256(uint32_t v, uint32_t* s_tmp) {
-    const int tid = threadIdx.x;
-    const int lane = lane_id();
-    const int wave = tid >> 6;
-    const uint32_t incl = wave_inclusive_sum(v);
-    if (lane == 63) s_tmp[wave] = incl;
-    __syncthreads();
-    uint32_t off = 0;
-#pragma unroll
-    for (int w = 0; w < kSortWaves; ++w) off += (w < wave) ? s_tmp[w] : 0u;
-    __syncthreads();
-    return off + incl - v;
-}
-
-// Stable ranking + scatter of one partition with known global offsets.  With
-// drop, keys equal to kDropKey are left out, so the output is the kept keys
-// only; partition 0 then also writes their number to *n_kept.
+// Stable ranking + scatter of one partition with known global offsets.
 template <int ITEMS>
 __global__ __launch_bounds__(kSortThreads) void radix_downsweep(
     const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
     uint32_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out, int n_host,
     const int* __restrict__ n_dev, int shift, int bits, int parts,
     const uint32_t* __restrict__ counts /* scanned, digit-major */,
-    const uint32_t* __restrict__ totals /* per digit */, int drop, int* __restrict__ n_kept) {
+    const uint32_t* __restrict__ totals /* per digit */) {
     constexpr int TILE = kSortThreads * ITEMS;
     __shared__ uint32_t s_keys[TILE];
     __shared__ uint32_t s_vals[TILE];
@@ -182,10 +148,6 @@ __global__ __launch_bounds__(kSortThreads) void radix_downsweep(
     const int wave = tid >> 6;
     const uint32_t digit_mask = (1u << bits) - 1u;
     const int part = blockIdx.x;
-    if (n_kept && part == 0) {       // the kept keys: the sum of the digit totals
-        const uint32_t ex = block_exclusive_scan256(totals[tid], s_tmp);
-        if (tid == kSortThreads - 1) *n_kept = (int)(ex + totals[tid]);
-    }
 #pragma unroll
     for (int w = 0; w < kSortWaves; ++w) s_whist[w][tid] = 0u;
     __syncthreads();
@@ -205,7 +167,7 @@ __global__ __launch_bounds__(kSortThreads) void radix_downsweep(
         const bool in = p < n;
         key[r] = in ? keys_in[p] : 0u;
         val[r] = in ? (vals_in ? vals_in[p] : (uint32_t)p) : 0u;
-        keep[r] = in && !(drop && key[r] == kDropKey);
+        keep[r] = in;
     }
     // ---- stable wave-local ranking with 64-lane ballots ----
 #pragma unroll
@@ -264,8 +226,8 @@ __global__ __launch_bounds__(kSortThreads) void radix_downsweep(
 }  // namespace
 
 // Partition size: 16 keys per lane for large arrays (the tile sort of millions
-// of pairs), 8 from 512K keys (the depth sort of 1M Gaussians: measured
-// faster than 4 or 16), 4 for small ones (more workgroups in flight).
+// of pairs), 8 from 512K keys (measured faster than 4 or 16 on a depth sort
+// of 1M keys), 4 for small ones (more workgroups in flight).
 static int items_for(int n_cap) { return n_cap >= (1 << 22) ? 16 : (n_cap >= (1 << 19) ? 8 : 4); }
 
 SortPlan sort_plan(int n_cap, int begin_bit, int end_bit) {
@@ -296,7 +258,7 @@ uint32_t* sort_first_counts(const SortPlan& p, void* scratch) {
 hipError_t sort_pairs(const SortPlan& p, const uint32_t* keys_in, const uint32_t* vals_in,
                       uint32_t* keys_out, uint32_t* vals_out, uint32_t* tmp_keys,
                       uint32_t* tmp_vals, int n_host, const int* n_dev, void* scratch,
-                      uint32_t* err, hipStream_t s, int* n_kept, bool counted) {
+                      uint32_t* err, hipStream_t s, bool counted) {
     (void)err;
     if (p.passes <= 0 || p.passes > kMaxPasses) return hipErrorInvalidValue;
     uint32_t* totals = static_cast<uint32_t*>(scratch);
@@ -316,25 +278,20 @@ hipError_t sort_pairs(const SortPlan& p, const uint32_t* keys_in, const uint32_t
         uint32_t* vo = to_out ? vals_out : tmp_vals;
         const int shift = p.begin_bit + q * per;
         const int bits = min(per, p.end_bit - shift);
-        // with n_kept the first pass drops the kDropKey keys; the later passes
-        // sort the kept ones, counted on the device
-        const int drop = (n_kept && q == 0) ? 1 : 0;
-        const int* nd = (n_kept && q > 0) ? n_kept : n_dev;
-        int* nk = drop ? n_kept : nullptr;
         // the first pass's counts may come from the keys' producer
-        const bool count_here = !(counted && q == 0 && !drop);
+        const bool count_here = !(counted && q == 0);
 #define JSA_PASS(IT)                                                                         \
     do {                                                                                     \
         if (count_here) {                                                                    \
-            radix_upsweep<IT><<<p.parts, kSortThreads, 0, s>>>(ki, n_host, nd, shift, bits,  \
-                                                                p.parts, counts, drop);      \
+            radix_upsweep<IT><<<p.parts, kSortThreads, 0, s>>>(ki, n_host, n_dev, shift,     \
+                                                                bits, p.parts, counts);      \
             if ((e = hipGetLastError()) != hipSuccess) return e;                             \
         }                                                                                    \
         radix_colscan<<<kBins, kSortThreads, 0, s>>>(counts, p.parts, totals);               \
         if ((e = hipGetLastError()) != hipSuccess) return e;                                 \
-        radix_downsweep<IT><<<p.parts, kSortThreads, 0, s>>>(ki, vi, ko, vo, n_host, nd,     \
+        radix_downsweep<IT><<<p.parts, kSortThreads, 0, s>>>(ki, vi, ko, vo, n_host, n_dev,  \
                                                               shift, bits, p.parts, counts,  \
-                                                              totals, drop, nk);             \
+                                                              totals);                       \
         if ((e = hipGetLastError()) != hipSuccess) return e;                                 \
     } while (0)
         if (items == 16) JSA_PASS(16);
