@@ -369,8 +369,6 @@ def stage_breakdown(torch, sc, cam, params, target, iters):
     project(True)
     isect = int(tcum[-1, 0].item())                 # read once, outside the timed loop
     ntiles = d.grid_dim_2d.x * d.grid_dim_2d.y
-    dwsb = L.jaxsplat_bin_depth_workspace_size(D)
-    dws = torch.empty(dwsb, dtype=torch.uint8, device=dev)
     ewsb = L.jaxsplat_bin_emit_workspace_size(D, isect)
     ews = torch.empty(ewsb, dtype=torch.uint8, device=dev)
     gids = torch.empty(max(isect, 1), **i32)
@@ -383,7 +381,7 @@ def stage_breakdown(torch, sc, cam, params, target, iters):
     vm, vs, vq = torch.empty((N, 3), **f32), torch.empty((N, 3), **f32), torch.empty((N, 4), **f32)
     bwsb = L.jaxsplat_rasterize_backward_staged_workspace_size(D, isect)
     bws = torch.empty(bwsb, dtype=torch.uint8, device=dev)
-    names = ["project_fwd", "depth_keys", "emission", "tile_sort", "bin_edges", "tile_depth_sort",
+    names = ["project_fwd", "emission", "tile_sort", "bin_edges", "tile_depth_sort",
              "raster_fwd", "loss_grad", "raster_bwd_records", "bwd_tail"]
     acc = {k: 0.0 for k in names}
     marks = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
@@ -398,11 +396,9 @@ def stage_breakdown(torch, sc, cam, params, target, iters):
             ev[0].record()
             project(True)
             ev[1].record()
-            _lib.check(L.jaxsplat_bin_depth_sort_rects(D, _p(depths), _p(radii), _p(rects),
-                                                       _p(dws), dwsb, s))
             ev[2].record()
-            _lib.check(L.jaxsplat_bin_emit_sort(D, isect, _p(dws), _p(gids), None, _p(bins),
-                                                _p(ews), ewsb, s))
+            _lib.check(L.jaxsplat_bin_rects_sort(D, _p(depths), _p(rects), _p(tcum), isect,
+                                                 _p(gids), None, _p(bins), _p(ews), ewsb, s))
             ev[3].record()
             _lib.check(L.jaxsplat_rasterize_forward_splats(
                 D, _p(gids), _p(bins), isect, _p(splats), _p(bg), 3, _p(fT), _p(fI), _p(img),
@@ -420,7 +416,7 @@ def stage_breakdown(torch, sc, cam, params, target, iters):
             ev[6].record()
             torch.cuda.synchronize()
             if it >= 2:
-                t = [ev[0].elapsed_time(ev[1]), ev[1].elapsed_time(ev[2]),
+                t = [ev[0].elapsed_time(ev[1]),
                      ev[2].elapsed_time(marks[0]), marks[0].elapsed_time(marks[1]),
                      marks[1].elapsed_time(marks[3]), marks[3].elapsed_time(ev[3]),
                      ev[3].elapsed_time(ev[4]),
@@ -464,8 +460,7 @@ def stage_breakdown(torch, sc, cam, params, target, iters):
     # sum; a flushed record is 36 B.
     stage_bytes = {
         "project_fwd": 164 * N,
-        "depth_keys": 28 * N,
-        "emission": 16 * N + 32 * E + 8 * isect,
+        "emission": 24 * N + 8 * isect,
         "tile_sort": 40 * isect,
         "bin_edges": 4 * isect + 8 * T,
         "tile_depth_sort": 12 * isect + 8 * T,

@@ -258,6 +258,25 @@ int jaxsplat_bin_depth_sort_rects(const jaxsplat_descriptor* d, const float* dep
                                   void* stream);
 
 /*
+ * Binning of given tile rectangles in one call, for a fused render (the
+ * reference's render, jaxsplat/__init__.py:6-70, whose tile lists the caller
+ * never sees); no counterpart in the reference.  tile_rects and
+ * cum_tiles_hit as written by jaxsplat_project_forward_rects /
+ * jaxsplat_project_forward_splats (cum_tiles_hit[g] - cum_tiles_hit[g-1] must
+ * be the count of tile_rects[g]); depths as written by the projection.  The
+ * intersections are emitted from the rectangles in Gaussian id order, sorted
+ * by tile and ordered by (depth bits, id) inside every tile: the same outputs
+ * as jaxsplat_bin_depth_sort_rects + jaxsplat_bin_emit_sort, with the same
+ * workspace (jaxsplat_bin_emit_workspace_size), capacity semantics and error
+ * word, without the depth stage.
+ */
+int jaxsplat_bin_rects_sort(const jaxsplat_descriptor* d, const float* depths,
+                            const uint32_t* tile_rects, const int32_t* cum_tiles_hit,
+                            uint32_t num_intersects, int32_t* gaussian_ids_sorted,
+                            int32_t* tile_ids_sorted, int32_t* tile_bins, void* workspace,
+                            size_t workspace_bytes, void* stream);
+
+/*
  * Synchronise `stream` and report an error recorded in the workspace of the
  * last jaxsplat_bin_and_sort or jaxsplat_bin_emit_sort (JAXSPLAT_ERR_CAPACITY):
  * more intersections than num_intersects, or a tile key out of range.

@@ -33,7 +33,7 @@ EXPORTS = (
     "jaxsplat_project_forward", "jaxsplat_project_forward_noalloc", "jaxsplat_project_backward",
     "jaxsplat_bin_workspace_size", "jaxsplat_bin_and_sort", "jaxsplat_bin_check",
     "jaxsplat_bin_depth_workspace_size", "jaxsplat_bin_depth_sort",
-    "jaxsplat_bin_emit_workspace_size", "jaxsplat_bin_emit_sort",
+    "jaxsplat_bin_emit_workspace_size", "jaxsplat_bin_emit_sort", "jaxsplat_bin_rects_sort",
     "jaxsplat_rasterize_backward_staged_workspace_size", "jaxsplat_rasterize_backward_staged",
     "jaxsplat_tile_rects_workspace_size", "jaxsplat_tile_rects", "jaxsplat_bin_depth_sort_rects",
     "jaxsplat_project_forward_rects",
@@ -122,6 +122,7 @@ _SIGS = {
     "jaxsplat_bin_depth_sort": (I32, [DP, P, P, P, P, SZ, P]),
     "jaxsplat_bin_emit_workspace_size": (SZ, [DP, U32]),
     "jaxsplat_bin_emit_sort": (I32, [DP, U32, P, P, P, P, P, SZ, P]),
+    "jaxsplat_bin_rects_sort": (I32, [DP, P, P, P, U32, P, P, P, P, SZ, P]),
     "jaxsplat_rasterize_backward_staged_workspace_size": (SZ, [DP, U32]),
     "jaxsplat_tile_rects_workspace_size": (SZ, [DP]),
     "jaxsplat_tile_rects": (I32, [DP] + [P] * 7 + [P, SZ, P]),

@@ -364,29 +364,35 @@ def _bin_tight(d, xys, depths, radii, conics, opacities, dev, capacity=None):
         _lib.check(L.jaxsplat_tile_rects(ctypes.byref(d), _p(xys), _p(radii), _p(conics),
                                          _p(opacities), _p(rects), None, _p(cum), _p(rws),
                                          rws_bytes, _stream(dev)))
-    gids, bins, isect = _bin_rects(d, depths, radii, rects, cum, dev, capacity)
+    gids, bins, isect = _bin_rects(d, depths, rects, cum, dev, capacity)
     return gids, bins, isect, rects, cum
 
 
-def _bin_rects(d, depths, radii, rects, cum, dev, capacity=None):
-    """Binning into given tile rectangles (with their inclusive counts cum):
-    the depth stage on those rectangles, then emission -- with the host read
-    of the total overlapped with the depth stage, or (with a capacity) no
-    read.  Returns (gaussian_ids_sorted, tile_bins, isect)."""
+def _bin_rects(d, depths, rects, cum, dev, capacity=None):
+    """Binning into given tile rectangles with their inclusive counts cum
+    (jaxsplat_bin_rects_sort: emission straight from the rectangles and
+    counts, no depth stage), after a host read of the total, or (with a
+    capacity) no read.  Returns (gaussian_ids_sorted, tile_bins, isect)."""
     L = _lib.lib()
+    ntiles = d.grid_dim_2d.x * d.grid_dim_2d.y
     with torch.cuda.device(dev):
         if capacity is None:
             count, ready = _read_count(cum, dev)
-        dws_bytes = L.jaxsplat_bin_depth_workspace_size(ctypes.byref(d))
-        dws = torch.empty(max(dws_bytes, 1), dtype=torch.uint8, device=dev)
-        _lib.check(L.jaxsplat_bin_depth_sort_rects(ctypes.byref(d), _p(depths), _p(radii),
-                                                   _p(rects), _p(dws), dws_bytes, _stream(dev)))
-        if capacity is None:
             ready.synchronize()
             isect = int(count[0])
         else:
             isect = capacity.capacity
-        return _emit(d, dws, isect, dev, capacity)
+        gids = torch.empty(max(isect, 1), dtype=torch.int32, device=dev)
+        bins = torch.empty((ntiles, 2), dtype=torch.int32, device=dev)
+        ws_bytes = L.jaxsplat_bin_emit_workspace_size(ctypes.byref(d), isect)
+        ws = torch.empty(max(ws_bytes, 16), dtype=torch.uint8, device=dev)
+        _lib.check(L.jaxsplat_bin_rects_sort(ctypes.byref(d), _p(depths), _p(rects), _p(cum),
+                                             isect, _p(gids), None, _p(bins), _p(ws), ws_bytes,
+                                             _stream(dev)))
+    if capacity is not None:
+        capacity._record(ws)
+    return gids[:isect], bins, isect
+
 
 def staged_backward_applies(nch, isect):
     """Whether the rasterize backward takes the staged entry point: up to
@@ -482,8 +488,7 @@ class _Rasterize(torch.autograd.Function):
             if N == 0:
                 gids, bins, isect = _bin(d, xys, depths, radii, 0, dev)
             else:
-                gids, bins, isect = _bin_rects(d, depths, radii, rects, cum_tiles_hit, dev,
-                                               capacity)
+                gids, bins, isect = _bin_rects(d, depths, rects, cum_tiles_hit, dev, capacity)
         elif TIGHT_TILES:
             gids, bins, isect, rects, cum_tiles_hit = _bin_tight(d, xys, depths, radii, conics,
                                                                  opacities, dev, capacity)
@@ -551,7 +556,7 @@ class _Render(torch.autograd.Function):
         if N == 0:
             gids, bins, isect = _bin(d, xys, depths, radii, 0, dev)
         else:
-            gids, bins, isect = _bin_rects(d, depths, radii, rects, cum, dev, capacity)
+            gids, bins, isect = _bin_rects(d, depths, rects, cum, dev, capacity)
         final_Ts = torch.empty((H, W, 1), dtype=torch.float32, device=dev)
         final_idx = torch.empty((H, W, 1), dtype=torch.int32, device=dev)
         img = torch.empty((H, W, nch), dtype=torch.float32, device=dev)

@@ -215,6 +215,11 @@ struct EmitEntries {
     int2* key;          // BATCH: (bin_off, tiles_x) of the entry's view
     uint32_t* chunk_owner;
     uint32_t* first_by_id;  // optional: first slot by id, for emitting entries
+    // with cum set: the entries are all Gaussians in id order (entry j is
+    // Gaussian j, emitting or not), their rectangles rect_by_id and first
+    // slots cum[j] - count (cum: inclusive sums of the rectangles' counts)
+    const int* cum;
+    const uint2* rect_by_id;
 };
 
 // The tile sort's first-pass digit counts, produced by the expansion (see
@@ -313,8 +318,9 @@ __global__ __launch_bounds__(256) void emit_compact_kernel(
 // belonging to the owner of the previous chunk's last slot.  A mark is
 // (step << 7) | (lane + 1) with a step number that grows with every chunk the
 // wave processes, so older marks never win the max-scan and the mark row is
-// cleared only once.  Every entry emits at least one slot, so a window covers
-// at least 64 slots.
+// cleared only once.  With compacted entries every entry emits at least one
+// slot, so a window covers at least 64 slots; with the entries of E.cum (all
+// Gaussians) a window may cover fewer, even none, and the wave slides on.
 template <bool BATCH>
 __global__ __launch_bounds__(256) void emit_expand_kernel(
     int tiles_x_plain, const EmitEntries E, const int* __restrict__ info,
@@ -352,9 +358,16 @@ __global__ __launch_bounds__(256) void emit_expand_kernel(
             uint32_t off = limit, gid = 0u;
             uint2 rc = make_uint2(0u, 0u);
             if (j < nnz) {
-                off = E.off[j];
-                rc = E.rect[j];
-                gid = E.gid[j];
+                if (E.cum) {
+                    rc = E.rect_by_id[j];
+                    gid = j;
+                    const uint32_t first = (uint32_t)E.cum[j] - rect_count(rc);
+                    off = first < limit ? first : limit;
+                } else {
+                    off = E.off[j];
+                    rc = E.rect[j];
+                    gid = E.gid[j];
+                }
                 if (BATCH) s_key[wave][lane] = E.key[j];
             }
             const uint32_t cnt = rect_count(rc);
@@ -956,6 +969,34 @@ size_t emit_layout(size_t o, int n, int cap, bool batch, EmitLayout& E) {
     return o;
 }
 
+// Emission straight from the inclusive counts (E.cum): every 64-slot chunk's
+// owner, the Gaussian whose slots [cum - count, cum) hold the chunk's first
+// slot, plus the totals of emit_scan_kernel (info[0] = slots written =
+// min(total, cap), info[1] = n, *total_out, bit 8 of err on overflow).
+__global__ __launch_bounds__(256) void cum_owner_kernel(int n, const int* __restrict__ cum,
+                                                        const uint2* __restrict__ rects,
+                                                        int cap, int* __restrict__ info,
+                                                        unsigned long long* __restrict__ total_out,
+                                                        uint32_t* __restrict__ err,
+                                                        uint32_t* __restrict__ chunk_owner) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    const uint32_t total = (uint32_t)cum[n - 1];
+    const uint32_t limit = total < (uint32_t)cap ? total : (uint32_t)cap;
+    if (g == n - 1) {
+        if (total_out) *total_out = total;
+        info[0] = (int)limit;
+        info[1] = n;
+        if (total > (uint32_t)cap) atomicOr(err, 8u);
+    }
+    const uint32_t cnt = rect_count(rects[g]);
+    if (cnt == 0) return;
+    const uint32_t off = (uint32_t)cum[g] - cnt;
+    if (off >= limit) return;
+    const uint32_t end = cnt > limit - off ? limit : off + cnt;
+    for (uint32_t ch = (off + 63u) >> 6; (ch << 6) < end; ++ch) chunk_owner[ch] = (uint32_t)g;
+}
+
 // Passes 1-4 over the entries perm[0..n) in emission order (perm == nullptr:
 // the ids 0..n-1), n = *n_dev when given, else n_host; launch shapes follow
 // n_host, an upper bound.  Writes the slots written (min(total, cap)) to
@@ -985,6 +1026,8 @@ hipError_t emit(int n, const int* n_dev, int tiles_x_plain, const ViewRec* views
     E.key = BATCH ? reinterpret_cast<int2*>(P(L.key)) : nullptr;
     E.chunk_owner = reinterpret_cast<uint32_t*>(P(L.chunk_owner));
     E.first_by_id = first_by_id;
+    E.cum = nullptr;
+    E.rect_by_id = nullptr;
     emit_count_kernel<<<parts, 256, 0, s>>>(n, n_dev, perm, rect_by_id, rects, part_tot,
                                             part_nz);
     hipError_t e = hipGetLastError();
@@ -1132,6 +1175,60 @@ hipError_t bin_emit(const BinPlan& p, const Camera& c, int num_isect_host,
     // per-tile depth order; the sort's ping-pong buffers and the emission's
     // pair buffers are free again and serve the longer runs
     return tile_depth_order(reinterpret_cast<const int2*>(tile_bins), p.num_tiles, dkeys, ids,
+                            reinterpret_cast<int*>(b + L.queue),
+                            reinterpret_cast<int*>(b + L.queue_tiles), U(L.ttmp_k),
+                            U(L.ttmp_v), U(L.tiles), U(L.gids), s);
+}
+
+// Binning of given tile rectangles with their inclusive counts (the render
+// path, whose projection wrote both): the emission reads the rectangles and
+// first slots straight from them and the per-tile depth order the depth bits
+// straight from `depths`, so neither the depth stage nor the emission's
+// count, scan and compaction run.  Same workspace and outputs as bin_emit.
+hipError_t bin_rects(const BinPlan& p, const Camera& c, const float* depths,
+                     const uint2* rects, const int* cum, int num_isect_host, int* gids_sorted,
+                     int* tiles_sorted, int* tile_bins, void* scratch, hipStream_t s) {
+    const BinLayout L = layout(p);
+    uint8_t* b = static_cast<uint8_t*>(scratch);
+    hipError_t e = fill_zero(b, L.zero_end, s);
+    if (e != hipSuccess) return e;
+    e = fill_zero(tile_bins, 8ull * p.num_tiles, s);
+    if (e != hipSuccess) return e;
+    if (p.n <= 0) return hipSuccess;
+    uint32_t* err = reinterpret_cast<uint32_t*>(b + L.err);
+    auto U = [&](size_t off) { return reinterpret_cast<uint32_t*>(b + off); };
+    int* written = reinterpret_cast<int*>(b + L.emit.info);
+    EmitEntries E{};
+    E.chunk_owner = U(L.emit.chunk_owner);
+    E.cum = cum;
+    E.rect_by_id = rects;
+    cum_owner_kernel<<<(p.n + 255) / 256, 256, 0, s>>>(
+        p.n, cum, rects, p.isect_cap, written,
+        reinterpret_cast<unsigned long long*>(b + L.total), err, E.chunk_owner);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    const FirstCounts fc = first_counts(p.tile_sort, b + L.tile_sort);
+    const long long chunks = ((long long)p.isect_cap + 63) / 64;
+    const int waves = (int)(chunks < kExpandWaves ? chunks : kExpandWaves);
+    const int blocks = fc.counts ? fc.parts : (waves + 3) / 4;
+    emit_expand_kernel<false><<<blocks > 0 ? blocks : 1, 256, 0, s>>>(
+        c.tiles_x, E, written, U(L.tiles), U(L.gids), fc);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    stage_mark(kMarkEmitted, s);
+    uint32_t* tiles_out = tiles_sorted ? reinterpret_cast<uint32_t*>(tiles_sorted) : U(L.tiles_sorted);
+    uint32_t* ids = reinterpret_cast<uint32_t*>(gids_sorted);
+    e = sort_pairs(p.tile_sort, U(L.tiles), U(L.gids), tiles_out, ids, U(L.ttmp_k),
+                   U(L.ttmp_v), num_isect_host, written, b + L.tile_sort, err, s,
+                   fc.counts != nullptr);
+    if (e != hipSuccess) return e;
+    stage_mark(kMarkTileSorted, s);
+    const int nb = (int)(((long long)p.isect_cap + 256 * kTileBinsPer - 1) / (256 * kTileBinsPer));
+    tile_bins_kernel<<<nb > 0 ? nb : 1, 256, 0, s>>>(tiles_out, num_isect_host, written,
+                                                   p.num_tiles,
+                                                   reinterpret_cast<int2*>(tile_bins), err);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    stage_mark(kMarkBinned, s);
+    return tile_depth_order(reinterpret_cast<const int2*>(tile_bins), p.num_tiles,
+                            reinterpret_cast<const uint32_t*>(depths), ids,
                             reinterpret_cast<int*>(b + L.queue),
                             reinterpret_cast<int*>(b + L.queue_tiles), U(L.ttmp_k),
                             U(L.ttmp_v), U(L.tiles), U(L.gids), s);

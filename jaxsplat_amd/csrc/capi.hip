@@ -465,6 +465,35 @@ int jaxsplat_bin_emit_sort(const jaxsplat_descriptor* d, uint32_t num_intersects
     return JAXSPLAT_OK;
 }
 
+int jaxsplat_bin_rects_sort(const jaxsplat_descriptor* d, const float* depths,
+                            const uint32_t* tile_rects, const int32_t* cum_tiles_hit,
+                            uint32_t num_intersects, int32_t* gaussian_ids_sorted,
+                            int32_t* tile_ids_sorted, int32_t* tile_bins, void* workspace,
+                            size_t workspace_bytes, void* stream) {
+    g_err.clear();
+    if (int st = validate(d)) return st;
+    if (num_intersects >= (1u << 30))
+        return fail(JAXSPLAT_ERR_CAPACITY, "num_intersects %u exceeds 2^30", num_intersects);
+    if (!tile_bins) return fail(JAXSPLAT_ERR_INVALID, "bin_rects_sort: tile_bins is NULL");
+    const jsa::Camera c = camera_of(d);
+    const jsa::BinPlan p = jsa::bin_plan(c, (int)num_intersects);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (num_intersects == 0 || d->num_points == 0) {
+        JSA_CHECK_HIP(jsa::fill_zero(tile_bins, 8ull * p.num_tiles, s), "bin_rects_sort");
+        return JAXSPLAT_OK;
+    }
+    if (!depths || !tile_rects || !cum_tiles_hit || !gaussian_ids_sorted)
+        return fail(JAXSPLAT_ERR_INVALID, "bin_rects_sort: NULL required pointer");
+    if (!workspace || workspace_bytes < p.emit_bytes)
+        return fail(JAXSPLAT_ERR_WORKSPACE, "bin_rects_sort: workspace too small (%zu < %zu)",
+                    workspace_bytes, p.emit_bytes);
+    JSA_CHECK_HIP(jsa::bin_rects(p, c, depths, reinterpret_cast<const uint2*>(tile_rects),
+                                 cum_tiles_hit, (int)num_intersects, gaussian_ids_sorted,
+                                 tile_ids_sorted, tile_bins, workspace, s),
+                  "bin_rects_sort");
+    return JAXSPLAT_OK;
+}
+
 int jaxsplat_bin_check(const jaxsplat_descriptor* d, uint32_t num_intersects,
                        const void* workspace, void* stream) {
     g_err.clear();

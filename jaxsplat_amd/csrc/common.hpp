@@ -544,6 +544,12 @@ hipError_t bin_depth(const BinPlan& p, const Camera& c, const float* xys, const 
 hipError_t bin_emit(const BinPlan& p, const Camera& c, int num_isect_host,
                     const void* depth_ws, int* gids_sorted, int* tiles_sorted, int* tile_bins,
                     void* scratch, hipStream_t s);
+// Binning of given packed tile rectangles with their inclusive counts cum
+// (consistent: cum[g] - cum[g-1] = the rectangle's tile count), the depth
+// keys read from depths (float bits); the emission workspace (p.emit_bytes).
+hipError_t bin_rects(const BinPlan& p, const Camera& c, const float* depths,
+                     const uint2* rects, const int* cum, int num_isect_host, int* gids_sorted,
+                     int* tiles_sorted, int* tile_bins, void* scratch, hipStream_t s);
 hipError_t bin_and_sort(const BinPlan& p, const Camera& c, const float* xys,
                         const float* depths, const int* radii, int num_isect_host,
                         int* gids_sorted, int* tiles_sorted, int* tile_bins, void* scratch,

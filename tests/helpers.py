@@ -63,9 +63,30 @@ def gpu_project(means3d, scales, quats, viewmat, *, img_shape, f, c, glob_scale=
     return {k: host(v) for k, v in outs.items()}
 
 
-def gpu_bin(xys, depths, radii, cum_tiles_hit, *, img_shape, block_width=16, split=False):
+def packed_rects(xys, radii, img_shape, block_width):
+    """The reference tile rectangles packed as the library packs them
+    (x0 | y0 << 16, w | h << 16), zero for radius 0, as (N, 2) int32."""
+    import oracle
+    H_, W_ = img_shape
+    tx = (W_ + block_width - 1) // block_width
+    ty = (H_ + block_width - 1) // block_width
+    xys = np.asarray(xys, np.float32).reshape(-1, 2)
+    radii = np.asarray(radii).reshape(-1)
+    x0, x1, y0, y1 = oracle.project.tile_rect(xys[:, 0], xys[:, 1], radii.astype(np.float32),
+                                              tx, ty, block_width)
+    on = radii > 0
+    w = np.where(on, x1 - x0, 0).astype(np.uint32)
+    h = np.where(on, y1 - y0, 0).astype(np.uint32)
+    lo = np.where(on, x0.astype(np.uint32) | (y0.astype(np.uint32) << 16), 0).astype(np.uint32)
+    return np.stack([lo, w | (h << 16)], 1).view(np.int32)
+
+
+def gpu_bin(xys, depths, radii, cum_tiles_hit, *, img_shape, block_width=16, split=False,
+            rects=False):
     """Binning through jaxsplat_bin_and_sort, or (split) through the depth
-    stage and the emission stage with separate workspaces."""
+    stage and the emission stage with separate workspaces, or (rects) through
+    jaxsplat_bin_rects_sort with the reference rectangles packed on the host
+    and cum_tiles_hit."""
     t = torch()
     N = xys.shape[0]
     d = descriptor(N, img_shape, block_width=block_width)
@@ -78,7 +99,14 @@ def gpu_bin(xys, depths, radii, cum_tiles_hit, *, img_shape, block_width=16, spl
     x = dev(np.asarray(xys, np.float32).reshape(N, 2))
     dp = dev(np.asarray(depths, np.float32).reshape(N))
     r = dev(np.asarray(radii, np.int32).reshape(N))
-    if split:
+    if rects:
+        rc = dev(packed_rects(xys, radii, img_shape, block_width))
+        cm = dev(np.asarray(cum_tiles_hit, np.int32).reshape(N))
+        wsb = L.jaxsplat_bin_emit_workspace_size(ctypes.byref(d), isect)
+        ws = t.full((max(wsb, 1),), 0xA5, dtype=t.uint8, device="cuda")
+        _lib.check(L.jaxsplat_bin_rects_sort(ctypes.byref(d), P(dp), P(rc), P(cm), isect,
+                                             P(gids), P(tiles), P(bins), P(ws), wsb, stream()))
+    elif split:
         dwb = L.jaxsplat_bin_depth_workspace_size(ctypes.byref(d))
         dws = t.full((max(dwb, 1),), 0xA5, dtype=t.uint8, device="cuda")
         _lib.check(L.jaxsplat_bin_depth_sort(ctypes.byref(d), P(x), P(dp), P(r), P(dws), dwb,

@@ -9,7 +9,8 @@ ties broken by id -- four buckets holding every entry --, a few octaves of
 depth, depths over five decades) plus the case where nothing emits, with
 short runs only, with longer runs, and with short, long and longest runs at
 once (a cluster of Gaussians over a few tiles), bin them through
-``jaxsplat_bin_and_sort`` and the split depth + emission entry points, and
+``jaxsplat_bin_and_sort``, the split depth + emission entry points and
+``jaxsplat_bin_rects_sort`` (packed rectangles with their inclusive counts), and
 require the sorted ids, the reconstructed 64-bit keys and the tile bins to be
 bit-identical to the oracle's stable (tile, depth bits, id) order
 (``oracle.binning``, reference ``lib/ops.cu:722-760``).
@@ -67,12 +68,15 @@ def _case(kind, n=20_000, seed=7, cluster=False):
     return xys, depths, radii, cum
 
 
-@pytest.mark.parametrize("split", [False, True])
+PATHS = {"one": {}, "split": {"split": True}, "rects": {"rects": True}}
+
+
+@pytest.mark.parametrize("path", list(PATHS))
 @pytest.mark.parametrize("kind", ["equal", "ties", "bits9", "bits17", "octaves", "wide"])
-def test_depth_key_spreads_bitexact(kind, split):
+def test_depth_key_spreads_bitexact(kind, path):
     xys, depths, radii, cum = _case(kind)
     gids, tiles, bins, isect = H.gpu_bin(xys, depths, radii, cum, img_shape=IMG, block_width=BW,
-                                         split=split)
+                                         **PATHS[path])
     ks, gs, bs = oracle.binning.bin_and_sort(xys, depths, radii, cum, img_shape=IMG,
                                              block_width=BW)
     assert isect == ks.size > 0
@@ -82,14 +86,16 @@ def test_depth_key_spreads_bitexact(kind, split):
     assert np.array_equal(bins, bs), kind
 
 
+@pytest.mark.parametrize("path", ["one", "rects"])
 @pytest.mark.parametrize("n,cluster", [(40_000, False), (90_000, False), (30_000, True),
                                        (70_000, True)])
 @pytest.mark.parametrize("kind", ["equal", "ties", "bits9", "octaves", "wide"])
-def test_long_tile_runs_bitexact(kind, n, cluster):
+def test_long_tile_runs_bitexact(kind, n, cluster, path):
     # runs longer than 1536 entries (all three kernels with the largest
     # cluster: runs of up to 23.5k entries)
     xys, depths, radii, cum = _case(kind, n=n, seed=11, cluster=cluster)
-    gids, tiles, bins, isect = H.gpu_bin(xys, depths, radii, cum, img_shape=IMG, block_width=BW)
+    gids, tiles, bins, isect = H.gpu_bin(xys, depths, radii, cum, img_shape=IMG, block_width=BW,
+                                         **PATHS[path])
     ks, gs, bs = oracle.binning.bin_and_sort(xys, depths, radii, cum, img_shape=IMG,
                                              block_width=BW)
     lengths = bs[:, 1] - bs[:, 0]
@@ -110,7 +116,8 @@ def test_depth_stage_nothing_emits():
     # is emitted or sorted
     xys, depths, radii, cum = _case("none")
     assert int(cum[-1]) == 0
-    gids, tiles, bins, isect = H.gpu_bin(xys, depths, radii, cum, img_shape=IMG, block_width=BW,
-                                         split=True)
-    assert isect == 0
-    assert not bins.any()
+    for path in ("split", "rects"):
+        gids, tiles, bins, isect = H.gpu_bin(xys, depths, radii, cum, img_shape=IMG,
+                                             block_width=BW, **PATHS[path])
+        assert isect == 0
+        assert not bins.any()
