@@ -23,12 +23,9 @@
 //     (2 passes at 1080p/16x16): order (tile, gaussian id);
 //  4. tile_bins_kernel: [start, end) edges from the sorted tile ids, empty
 //     tiles (0, 0);
-//  5. tile_depth_sort_kernel: every tile's run, already in id order, sorted
-//     stably in LDS by the depth bits that vary inside it (three 9-bit
-//     passes at most for float depths), giving (tile, depth bits, id):
-//     identical to the reference order.  Runs longer than the LDS holds are
-//     queued and sorted by tile_depth_sort_long_kernel in chunks through
-//     global memory.
+//  5. tile_depth_sort_kernel: every tile's run ordered by (depth bits, id)
+//     with LDS buckets linear in the run's key range (global scratch for
+//     runs longer than the LDS holds): identical to the reference order.
 // The depth order is thus established per tile (about 800 entries each at
 // 1080p with 1M Gaussians, sorted where they lie in LDS) instead of by a
 // global sort of the Gaussians and a stable tile sort after it.
@@ -466,41 +463,32 @@ __global__ __launch_bounds__(256) void tile_bins_kernel(const uint32_t* __restri
 // ---------------------------------------------------------------------------
 // Per-tile depth order
 // ---------------------------------------------------------------------------
-// A tile's run [bins[t].x, bins[t].y) of the tile-sorted ids is in ascending
-// id (the stable tile sort of pairs emitted in id order).  Ordering it by
-// (depth bits, position in the run) gives the reference order (depth bits,
-// id) inside the tile.
+// A tile's run [bins[t].x, bins[t].y) of the tile-sorted ids holds the
+// Gaussians whose rectangle covers the tile.  Ordering it by (depth bits, id)
+// gives the reference order inside the tile.
 //
-// Bucket order: the run's keys are split into 2^bb buckets linear in
-// key - min (the shift makes max - min fit bb bits, so at least half of the
+// Bucket order: the run's keys are split into 2^11 buckets linear in
+// key - min (the shift makes max - min fit 11 bits, so at least half of the
 // buckets are in range); an LDS histogram with returning atomics gives every
 // entry its bucket and a slot in it, a scan the bucket starts, and the
 // entries are placed by bucket.  Each entry's final position is then its
 // bucket's start plus the number of entries of its bucket that precede it in
-// (key, position) order, counted by reading the bucket.  Depths inside a
-// tile are spread over a range of floats, so buckets are small (at 1080p with
-// 1M Gaussians: 2048 buckets for about 800 entries, at most 12 in a bucket);
-// the order is exact whatever the distribution, only the counting reads grow
-// with the bucket sizes.  A run whose keys are all equal is already in
-// order.
+// (key, id) order, counted by reading the bucket.  Depths inside a tile are
+// spread over a range of floats, so buckets are small (at 1080p with 1M
+// Gaussians about 800 entries per tile, at most 12 in a bucket); the order is
+// exact whatever the distribution, only the counting reads grow with the
+// bucket sizes.  A run whose keys are all equal is already in order (it is in
+// id order, from the stable tile sort of pairs emitted in id order).
 //
-// Three kernels by run length: up to kSegCap entries in one 256-thread
-// workgroup with the placed entries in LDS; up to kBigCap in one 1024-thread
-// workgroup with the placed entries in a global scratch of the run's own
-// offsets (L2-resident); longer runs by a chunked LSD radix sort through
-// global memory.  The first kernel queues the runs it does not take, the
-// second the ones it passes on.
+// One 256-thread workgroup per tile: runs of up to kSegCap entries are held
+// in registers and placed in LDS; longer ones go through global scratch at
+// the run's own offsets (L2-resident), with the bucket counts in LDS.
 constexpr int kSegThreads = 256;
+constexpr int kSegWaves = kSegThreads / kWave;
 constexpr int kSegItems = 6;                           // entries per lane
 constexpr int kSegCap = kSegThreads * kSegItems;       // 1536
 constexpr int kSegBucketBits = 11;
-constexpr int kBigThreads = 1024;
-constexpr int kBigItems = 16;
-constexpr int kBigCap = kBigThreads * kBigItems;       // 16384
-constexpr int kBigMaxBucketBits = 14;
-constexpr int kSegQueueBlocks = 256;                   // workgroups of the queue kernels
-constexpr int kSegMaxBits = 9;                         // digit bits of the chunked radix sort
-constexpr int kSegBins = 1 << kSegMaxBits;
+constexpr int kSegBuckets = 1 << kSegBucketBits;
 
 // Unsigned min and max over the 64 lanes (full EXEC): DPP inside the rows,
 // then the four rows by readlane.
@@ -520,136 +508,142 @@ __device__ __forceinline__ void wave_minmax_u32(uint32_t& mn, uint32_t& mx) {
              max((uint32_t)__builtin_amdgcn_readlane(b, 32), (uint32_t)__builtin_amdgcn_readlane(b, 48)));
 }
 
-// Block min / max; s_red holds 2 * WAVES words.  Ends with a barrier.
-template <int WAVES>
-__device__ __forceinline__ void block_minmax_u32(uint32_t& mn, uint32_t& mx, uint32_t* s_red) {
+// 20 KB, so that eight workgroups fit a CU.  red and tmp are used before the
+// entries are placed, so they share kv's space.
+struct SegShared {
+    union {
+        uint2 kv[kSegCap];                          // placed (key, id)
+        struct {
+            uint32_t red[2 * kSegWaves];
+            uint32_t tmp[kSegWaves];
+        };
+    };
+    uint32_t cnt[kSegBuckets];                      // bucket counts / starts; then the ids
+};
+static_assert(kSegBuckets >= kSegCap, "the ids are staged in cnt");
+static_assert(sizeof(SegShared) <= 20480, "eight workgroups per CU");
+static_assert(kSegBuckets % kSegThreads == 0, "bucket scan: whole runs per thread");
+
+// Block min / max.  Ends with a barrier.
+__device__ __forceinline__ void block_minmax_u32(uint32_t& mn, uint32_t& mx, SegShared& sh) {
     const int wave = threadIdx.x >> 6;
     wave_minmax_u32(mn, mx);
     if (lane_id() == 0) {
-        s_red[wave] = mn;
-        s_red[WAVES + wave] = mx;
+        sh.red[wave] = mn;
+        sh.red[kSegWaves + wave] = mx;
     }
     __syncthreads();
-    mn = s_red[0];
-    mx = s_red[WAVES];
+    mn = sh.red[0];
+    mx = sh.red[kSegWaves];
 #pragma unroll
-    for (int w = 1; w < WAVES; ++w) {
-        mn = min(mn, s_red[w]);
-        mx = max(mx, s_red[WAVES + w]);
+    for (int w = 1; w < kSegWaves; ++w) {
+        mn = min(mn, sh.red[w]);
+        mx = max(mx, sh.red[kSegWaves + w]);
     }
     __syncthreads();
 }
 
-// Exclusive scan of one value per thread over a block of WAVES waves (s_tmp:
-// WAVES words; two barriers).
-template <int WAVES>
-__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s_tmp) {
-    const int wave = threadIdx.x >> 6;
-    const uint32_t incl = wave_inclusive_sum(v);
-    if (lane_id() == 63) s_tmp[wave] = incl;
-    __syncthreads();
-    uint32_t off = 0;
-#pragma unroll
-    for (int w = 0; w < WAVES; ++w) off += (w < wave) ? s_tmp[w] : 0u;
-    __syncthreads();
-    return off + incl - v;
+// The shift that maps key - kmin of [kmin, kmax] onto the buckets.
+__device__ __forceinline__ int bucket_shift(uint32_t kmin, uint32_t kmax) {
+    const int span_bits = 32 - __builtin_clz(kmax - kmin);
+    return span_bits > kSegBucketBits ? span_bits - kSegBucketBits : 0;
 }
 
-// (key, position) order of two entries.
-__device__ __forceinline__ bool entry_before(uint2 a, uint32_t key, uint32_t e) {
-    return a.x < key || (a.x == key && a.y < e);
-}
-
-// Bucket order of the run's entries held in registers (entry e = r * THREADS
-// + tid, valid[r] for e < n): pos[r] = the entry's position in (key, e)
-// order.  s_cnt: (1 << bb) + 1 LDS words; kv: n entries (LDS or global
-// scratch) for the placed (key, e).  Ends with a barrier after which kv and
-// s_cnt are no longer read.
-template <int THREADS, int ITEMS, typename KV>
-__device__ __forceinline__ void bucket_positions(int n, const uint32_t (&key)[ITEMS],
-                                                 const bool (&valid)[ITEMS], uint32_t kmin,
-                                                 uint32_t kmax, int bb, uint32_t* s_cnt,
-                                                 uint32_t* s_tmp, KV kv,
-                                                 uint32_t (&pos)[ITEMS]) {
-    constexpr int WAVES = THREADS / kWave;
+// Counts in sh.cnt[0..kSegBuckets) -> exclusive starts (the last bucket
+// ends at the run's length).  Each thread scans a run of consecutive buckets.
+// Ends with a barrier.
+__device__ __forceinline__ void bucket_scan(SegShared& sh) {
+    constexpr int per = kSegBuckets / kSegThreads;
     const int tid = threadIdx.x;
-    const int nb = 1 << bb;                 // nb >= THREADS
-    const uint32_t span = kmax - kmin;
-    const int span_bits = 32 - __builtin_clz(span);
-    const int shift = span_bits > bb ? span_bits - bb : 0;
-    for (int i = tid; i <= nb; i += THREADS) s_cnt[i] = 0u;
-    __syncthreads();
-    uint32_t bk[ITEMS], slot[ITEMS];
-#pragma unroll
-    for (int r = 0; r < ITEMS; ++r) {
-        bk[r] = (key[r] - kmin) >> shift;
-        slot[r] = valid[r] ? atomicAdd(&s_cnt[bk[r]], 1u) : 0u;
-    }
-    __syncthreads();
-    // bucket starts: each thread scans nb / THREADS consecutive buckets
-    const int per = nb / THREADS;
     const int b0 = tid * per;
+    uint32_t c[per];
     uint32_t sum = 0;
-    for (int k = 0; k < per; ++k) sum += s_cnt[b0 + k];
-    uint32_t run = block_exclusive_scan<WAVES>(sum, s_tmp);
+#pragma unroll
     for (int k = 0; k < per; ++k) {
-        const uint32_t c = s_cnt[b0 + k];
-        s_cnt[b0 + k] = run;
-        run += c;
+        c[k] = sh.cnt[b0 + k];
+        sum += c[k];
     }
-    if (tid == THREADS - 1) s_cnt[nb] = run;           // = n
-    __syncthreads();
-    uint32_t st[ITEMS], cnt[ITEMS];
+    uint32_t run = block_exclusive_scan256(sum, sh.tmp);
 #pragma unroll
-    for (int r = 0; r < ITEMS; ++r) {
-        st[r] = 0u;
-        cnt[r] = 0u;
-        if (valid[r]) {
-            st[r] = s_cnt[bk[r]];
-            cnt[r] = s_cnt[bk[r] + 1] - st[r];
-            kv[st[r] + slot[r]] = make_uint2(key[r], (uint32_t)(r * THREADS + tid));
-        }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int r = 0; r < ITEMS; ++r) {
-        uint32_t rank = 0;
-        if (valid[r]) {
-            const uint32_t e = (uint32_t)(r * THREADS + tid);
-            for (uint32_t j = st[r]; j < st[r] + cnt[r]; ++j)
-                rank += entry_before(kv[j], key[r], e) ? 1u : 0u;
-        }
-        pos[r] = st[r] + rank;
+    for (int k = 0; k < per; ++k) {
+        sh.cnt[b0 + k] = run;
+        run += c[k];
     }
     __syncthreads();
 }
 
-struct SegShared {
-    uint2 kv[kSegCap];                              // placed (key, position)
-    uint32_t cnt[(1 << kSegBucketBits) + 1];        // bucket counts / starts; then the ids
-    uint32_t red[8];
-    uint32_t tmp[4];
-};
-static_assert((1 << kSegBucketBits) + 1 >= kSegCap, "the ids are staged in cnt");
-static_assert((1 << kSegBucketBits) >= kSegThreads, "bucket scan: one or more per thread");
-static_assert((1 << kBigMaxBucketBits) <= kBigCap, "");
+// End of bucket bk: the next bucket's start, or n for the last one.
+__device__ __forceinline__ uint32_t bucket_end(const SegShared& sh, uint32_t bk, int n) {
+    return bk + 1 < (uint32_t)kSegBuckets ? sh.cnt[bk + 1] : (uint32_t)n;
+}
 
-// One workgroup per tile.  Runs of at most kSegCap entries are ordered in LDS
-// and written back in place; longer ones are queued for the big kernel.
+// (key, id) order of two entries.
+__device__ __forceinline__ bool entry_before(uint2 a, uint32_t key, uint32_t id) {
+    return a.x < key || (a.x == key && a.y < id);
+}
+
+// Rank of (key, id) among the placed entries kv[s, s + c).
+template <typename KV>
+__device__ __forceinline__ uint32_t bucket_rank(KV kv, uint32_t s, uint32_t c, uint32_t key,
+                                                uint32_t id) {
+    uint32_t rank = 0;
+    for (uint32_t j = s; j < s + c; ++j) rank += entry_before(kv[j], key, id) ? 1u : 0u;
+    return rank;
+}
+
+// Runs of more than kSegCap entries: the same order through global scratch
+// at the run's offsets: ka the keys, va the slots in the buckets, kv the
+// placed (key, id).  Every lane of the block calls it (barriers inside).
+__device__ void long_run_order(int base, int n, const uint32_t* __restrict__ dkeys,
+                               uint32_t* __restrict__ ids, uint32_t* __restrict__ ka,
+                               uint32_t* __restrict__ va, uint2* __restrict__ kv,
+                               SegShared& sh) {
+    const int tid = threadIdx.x;
+    uint32_t kmin = ~0u, kmax = 0u;
+    for (int e = tid; e < n; e += kSegThreads) {
+        const uint32_t k = dkeys[ids[base + e]];
+        ka[base + e] = k;
+        kmin = min(kmin, k);
+        kmax = max(kmax, k);
+    }
+    block_minmax_u32(kmin, kmax, sh);
+    if (kmin == kmax) return;                           // uniform: already in order
+    const int shift = bucket_shift(kmin, kmax);
+    for (int i = tid; i < kSegBuckets; i += kSegThreads) sh.cnt[i] = 0u;
+    __syncthreads();
+    for (int e = tid; e < n; e += kSegThreads)
+        va[base + e] = atomicAdd(&sh.cnt[(ka[base + e] - kmin) >> shift], 1u);
+    __syncthreads();
+    bucket_scan(sh);
+    for (int e = tid; e < n; e += kSegThreads) {
+        const uint32_t k = ka[base + e];
+        kv[base + sh.cnt[(k - kmin) >> shift] + va[base + e]] = make_uint2(k, ids[base + e]);
+    }
+    __syncthreads();                                    // ids are no longer read
+    for (int j = tid; j < n; j += kSegThreads) {
+        const uint2 me = kv[base + j];
+        const uint32_t bk = (me.x - kmin) >> shift;
+        const uint32_t st = sh.cnt[bk];
+        const uint32_t rank = bucket_rank(kv + base, st, bucket_end(sh, bk, n) - st, me.x, me.y);
+        ids[base + st + rank] = me.y;
+    }
+}
+
+// One workgroup per tile (see above).
 __global__ __launch_bounds__(kSegThreads) void tile_depth_sort_kernel(
     const int2* __restrict__ bins, const uint32_t* __restrict__ dkeys,
-    uint32_t* __restrict__ ids, int* __restrict__ big_count, int* __restrict__ big_tiles) {
+    uint32_t* __restrict__ ids, uint32_t* __restrict__ ka, uint32_t* __restrict__ va,
+    uint2* __restrict__ kv_tmp) {
     __shared__ SegShared sh;
-    const int t = blockIdx.x;
-    const int2 b = bins[t];
+    const int2 b = bins[blockIdx.x];
     const int n = b.y - b.x;
     if (n <= 1) return;                                 // uniform
     if (n > kSegCap) {
-        if (threadIdx.x == 0) big_tiles[atomicAdd(big_count, 1)] = t;
+        long_run_order(b.x, n, dkeys, ids, ka, va, kv_tmp, sh);
         return;
     }
     const int tid = threadIdx.x;
-    uint32_t key[kSegItems], val[kSegItems], pos[kSegItems];
+    uint32_t key[kSegItems], val[kSegItems];
     bool valid[kSegItems];
 #pragma unroll
     for (int r = 0; r < kSegItems; ++r) {
@@ -657,6 +651,8 @@ __global__ __launch_bounds__(kSegThreads) void tile_depth_sort_kernel(
         valid[r] = e < n;
         val[r] = valid[r] ? ids[b.x + e] : 0u;
     }
+    // the bucket counts are cleared while the loads are in flight
+    for (int i = tid; i < kSegBuckets; i += kSegThreads) sh.cnt[i] = 0u;
     uint32_t kmin = ~0u, kmax = 0u;
 #pragma unroll
     for (int r = 0; r < kSegItems; ++r) {
@@ -666,10 +662,34 @@ __global__ __launch_bounds__(kSegThreads) void tile_depth_sort_kernel(
             kmax = max(kmax, key[r]);
         }
     }
-    block_minmax_u32<kSegThreads / kWave>(kmin, kmax, sh.red);
-    if (kmin == kmax) return;                           // all keys equal: already in order
-    bucket_positions<kSegThreads, kSegItems>(n, key, valid, kmin, kmax, kSegBucketBits, sh.cnt,
-                                             sh.tmp, sh.kv, pos);
+    block_minmax_u32(kmin, kmax, sh);
+    if (kmin == kmax) return;                           // uniform: already in order
+    const int shift = bucket_shift(kmin, kmax);
+    uint32_t bk[kSegItems], slot[kSegItems];
+#pragma unroll
+    for (int r = 0; r < kSegItems; ++r) {
+        bk[r] = (key[r] - kmin) >> shift;
+        slot[r] = valid[r] ? atomicAdd(&sh.cnt[bk[r]], 1u) : 0u;
+    }
+    __syncthreads();
+    bucket_scan(sh);
+    uint32_t st[kSegItems], cnt[kSegItems];
+#pragma unroll
+    for (int r = 0; r < kSegItems; ++r) {
+        st[r] = 0u;
+        cnt[r] = 0u;
+        if (valid[r]) {
+            st[r] = sh.cnt[bk[r]];
+            cnt[r] = bucket_end(sh, bk[r], n) - st[r];
+            sh.kv[st[r] + slot[r]] = make_uint2(key[r], val[r]);
+        }
+    }
+    __syncthreads();
+    uint32_t pos[kSegItems];
+#pragma unroll
+    for (int r = 0; r < kSegItems; ++r)
+        pos[r] = valid[r] ? st[r] + bucket_rank(sh.kv, st[r], cnt[r], key[r], val[r]) : 0u;
+    __syncthreads();                                    // sh.cnt is no longer read
 #pragma unroll
     for (int r = 0; r < kSegItems; ++r)
         if (valid[r]) sh.cnt[pos[r]] = val[r];
@@ -681,266 +701,13 @@ __global__ __launch_bounds__(kSegThreads) void tile_depth_sort_kernel(
     }
 }
 
-// Queued runs of kSegCap < n <= kBigCap entries, one 1024-thread workgroup
-// per run at a time; the placed entries go to kv_tmp at the run's offsets.
-// Longer runs are queued for the long kernel.
-__global__ __launch_bounds__(kBigThreads) void tile_depth_sort_big_kernel(
-    const int2* __restrict__ bins, const uint32_t* __restrict__ dkeys,
-    uint32_t* __restrict__ ids, const int* __restrict__ big_count,
-    const int* __restrict__ big_tiles, uint2* __restrict__ kv_tmp,
-    int* __restrict__ long_count, int* __restrict__ long_tiles) {
-    __shared__ uint32_t s_cnt[(1 << kBigMaxBucketBits) + 1];
-    __shared__ uint32_t s_red[2 * (kBigThreads / kWave)];
-    __shared__ uint32_t s_tmp[kBigThreads / kWave];
-    const int tid = threadIdx.x;
-    const int count = *big_count;
-    for (int qi = blockIdx.x; qi < count; qi += gridDim.x) {
-        const int t = big_tiles[qi];
-        const int2 b = bins[t];
-        const int n = b.y - b.x;
-        if (n > kBigCap) {
-            if (tid == 0) long_tiles[atomicAdd(long_count, 1)] = t;
-            continue;                                   // uniform
-        }
-        uint32_t key[kBigItems], val[kBigItems], pos[kBigItems];
-        bool valid[kBigItems];
-#pragma unroll
-        for (int r = 0; r < kBigItems; ++r) {
-            const int e = r * kBigThreads + tid;
-            valid[r] = e < n;
-            val[r] = valid[r] ? ids[b.x + e] : 0u;
-        }
-        uint32_t kmin = ~0u, kmax = 0u;
-#pragma unroll
-        for (int r = 0; r < kBigItems; ++r) {
-            key[r] = valid[r] ? dkeys[val[r]] : 0u;
-            if (valid[r]) {
-                kmin = min(kmin, key[r]);
-                kmax = max(kmax, key[r]);
-            }
-        }
-        block_minmax_u32<kBigThreads / kWave>(kmin, kmax, s_red);
-        if (kmin == kmax) continue;                     // uniform; the ids stay
-        // about two buckets per entry, at least one per thread
-        int bb = 32 - __builtin_clz((uint32_t)n);
-        bb = min(max(bb + 1, 10), kBigMaxBucketBits);
-        bucket_positions<kBigThreads, kBigItems>(n, key, valid, kmin, kmax, bb, s_cnt, s_tmp,
-                                                 kv_tmp + b.x, pos);
-        // every lane has read its ids and the placed entries: write in place
-#pragma unroll
-        for (int r = 0; r < kBigItems; ++r)
-            if (valid[r]) ids[b.x + pos[r]] = val[r];
-        __syncthreads();
-    }
-}
-
-// The chunked radix sort of the longest runs (more than kBigCap entries).
-// Layout of a chunk of m <= kSegCap entries: wave w holds entries
-// [w * R * 64, (w + 1) * R * 64), R = ceil(m / 256), entry w*R*64 + r*64 + l
-// in its register slot r of lane l.  Ranking is stable in that order: one
-// 64-lane ballot per digit bit gives the lanes with the same digit, and a
-// per-wave LDS histogram carries the counts across the wave's rounds; per
-// digit, the waves' counts are then turned into offsets in wave order.
-struct RadixShared {
-    uint2 kv[kSegCap];
-    uint16_t whist[4][kSegBins];            // per-wave digit counts, then wave offsets
-    uint32_t start[kSegBins + 1];           // digit start inside the chunk; [kSegBins]: its size
-    uint32_t red[8];
-    uint32_t tmp[4];
-};
-
-// One stable counting pass over the chunk held in registers: pos[r] = the
-// entry's position in digit order inside the chunk.  Leaves sh.start[d] =
-// digit d's start in the chunk and sh.start[kSegBins] = the chunk's size.
-// Begins and ends with a barrier.
-__device__ __forceinline__ void seg_rank_pass(const uint32_t (&key)[kSegItems],
-                                              const bool (&valid)[kSegItems], int R, int shift,
-                                              int bits, RadixShared& sh,
-                                              uint32_t (&pos)[kSegItems]) {
-    const int tid = threadIdx.x;
-    const int lane = lane_id();
-    const int wave = tid >> 6;
-    uint32_t* wh = reinterpret_cast<uint32_t*>(&sh.whist[0][0]);
-    for (int i = tid; i < 2 * kSegBins; i += kSegThreads) wh[i] = 0u;   // 4 x kSegBins halves
-    __syncthreads();
-    const uint32_t dmask = (1u << bits) - 1u;
-    const uint64_t lt = __lanemask_lt();
-    uint32_t rank[kSegItems];
-#pragma unroll
-    for (int r = 0; r < kSegItems; ++r) {
-        rank[r] = 0u;
-        if (r < R) {                                    // uniform
-            const uint32_t d = (key[r] >> shift) & dmask;
-            const uint64_t peers = match_digit(d, bits, ballot(valid[r]));
-            uint32_t prior = 0;
-            if (valid[r]) prior = sh.whist[wave][d];
-            // the highest lane of each digit group advances the wave's count
-            const bool leader = valid[r] && ((peers & ~(lt | (1ull << lane))) == 0);
-            if (leader) sh.whist[wave][d] = (uint16_t)(prior + (uint32_t)__popcll(peers));
-            rank[r] = prior + (uint32_t)__popcll(peers & lt);
-            wave_lds_sync();
-        }
-    }
-    __syncthreads();
-    // digits 2*tid and 2*tid + 1: offsets of the waves inside the digit, the
-    // digit's count, and (block scan) its start
-    uint32_t tot[2];
-#pragma unroll
-    for (int j = 0; j < 2; ++j) {
-        const int d = 2 * tid + j;
-        uint32_t t = 0;
-#pragma unroll
-        for (int w = 0; w < 4; ++w) {
-            const uint32_t c = sh.whist[w][d];
-            sh.whist[w][d] = (uint16_t)t;
-            t += c;
-        }
-        tot[j] = t;
-    }
-    const uint32_t ex = block_exclusive_scan256(tot[0] + tot[1], sh.tmp);
-    sh.start[2 * tid] = ex;
-    sh.start[2 * tid + 1] = ex + tot[0];
-    if (tid == kSegThreads - 1) sh.start[kSegBins] = ex + tot[0] + tot[1];
-    __syncthreads();
-#pragma unroll
-    for (int r = 0; r < kSegItems; ++r) {
-        pos[r] = 0u;
-        if (r < R && valid[r]) {
-            const uint32_t d = (key[r] >> shift) & dmask;
-            pos[r] = sh.start[d] + sh.whist[wave][d] + rank[r];
-        }
-    }
-    __syncthreads();
-}
-
-// Queued runs of more than kBigCap entries: one workgroup per run at a time.
-// The run's varying key bits (OR ^ AND) are sorted in passes of at most 9
-// bits; each pass counts the run's digits, then ranks it chunk by chunk
-// (kSegCap entries in order) and scatters every chunk to the digit cursors in
-// global memory, ping-ponging between (ka, va) and (kb, vb) at the run's own
-// offsets; the ids go back in place at the end.
-__global__ __launch_bounds__(kSegThreads) void tile_depth_sort_long_kernel(
-    const int2* __restrict__ bins, const uint32_t* __restrict__ dkeys,
-    uint32_t* __restrict__ ids, const int* __restrict__ long_count,
-    const int* __restrict__ long_tiles, uint32_t* __restrict__ ka, uint32_t* __restrict__ va,
-    uint32_t* __restrict__ kb, uint32_t* __restrict__ vb) {
-    __shared__ RadixShared sh;
-    __shared__ uint32_t s_cursor[kSegBins];
-    const int tid = threadIdx.x;
-    const int lane = lane_id();
-    const int wave = tid >> 6;
-    const int count = *long_count;
-    for (int li = blockIdx.x; li < count; li += gridDim.x) {
-        const int2 b = bins[long_tiles[li]];
-        const int n = b.y - b.x;
-        uint32_t o = 0u, a = ~0u;
-        for (int i = tid; i < n; i += kSegThreads) {
-            const uint32_t g = ids[b.x + i];
-            const uint32_t k = dkeys[g];
-            ka[b.x + i] = k;
-            va[b.x + i] = g;
-            o |= k;
-            a &= k;
-        }
-        // the bits that differ between some two keys: OR ^ AND of the run
-        uint32_t or_all = o, and_all = a;
-#pragma unroll
-        for (int off = 1; off < kWave; off <<= 1) {
-            or_all |= (uint32_t)__shfl_xor((int)or_all, off);
-            and_all &= (uint32_t)__shfl_xor((int)and_all, off);
-        }
-        if (lane == 0) {
-            sh.red[wave] = or_all;
-            sh.red[4 + wave] = and_all;
-        }
-        __syncthreads();
-        const uint32_t diff = ((sh.red[0] | sh.red[1]) | (sh.red[2] | sh.red[3])) ^
-                              ((sh.red[4] & sh.red[5]) & (sh.red[6] & sh.red[7]));
-        __syncthreads();
-        if (diff == 0u) continue;                       // uniform; the ids stay
-        const int lo = __builtin_ctz(diff);
-        const int hi = 32 - __builtin_clz(diff);
-        const int passes = (hi - lo + kSegMaxBits - 1) / kSegMaxBits;
-        const int bits = (hi - lo + passes - 1) / passes;
-        const uint32_t* sk = ka;
-        const uint32_t* sv = va;
-        uint32_t* dk = kb;
-        uint32_t* dv = vb;
-        for (int q = 0; q < passes; ++q) {
-            const int shift = lo + q * bits;
-            const int bq = min(bits, hi - shift);
-            const uint32_t dmask = (1u << bq) - 1u;
-            // the run's digit counts, scanned into the cursors
-            for (int d = tid; d < kSegBins; d += kSegThreads) s_cursor[d] = 0u;
-            __syncthreads();
-            for (int i = tid; i < n; i += kSegThreads)
-                atomicAdd(&s_cursor[(sk[b.x + i] >> shift) & dmask], 1u);
-            __syncthreads();
-            const uint32_t c0 = s_cursor[2 * tid], c1 = s_cursor[2 * tid + 1];
-            const uint32_t ex = block_exclusive_scan256(c0 + c1, sh.tmp);
-            s_cursor[2 * tid] = ex;
-            s_cursor[2 * tid + 1] = ex + c0;
-            __syncthreads();
-            for (int c = 0; c < n; c += kSegCap) {
-                const int m = min(kSegCap, n - c);
-                const int R = (m + kSegThreads - 1) / kSegThreads;
-                const int wbase = wave * R * kWave;
-                uint32_t key[kSegItems], val[kSegItems], pos[kSegItems];
-                bool valid[kSegItems];
-#pragma unroll
-                for (int r = 0; r < kSegItems; ++r) {
-                    const int i = wbase + r * kWave + lane;
-                    valid[r] = r < R && i < m;
-                    key[r] = valid[r] ? sk[b.x + c + i] : 0u;
-                    val[r] = valid[r] ? sv[b.x + c + i] : 0u;
-                }
-                seg_rank_pass(key, valid, R, shift, bq, sh, pos);
-#pragma unroll
-                for (int r = 0; r < kSegItems; ++r) {
-                    if (valid[r]) {
-                        const uint32_t d = (key[r] >> shift) & dmask;
-                        const uint32_t out = (uint32_t)b.x + s_cursor[d] + (pos[r] - sh.start[d]);
-                        dk[out] = key[r];
-                        dv[out] = val[r];
-                    }
-                }
-                __syncthreads();
-                s_cursor[2 * tid] += sh.start[2 * tid + 1] - sh.start[2 * tid];
-                s_cursor[2 * tid + 1] += sh.start[2 * tid + 2] - sh.start[2 * tid + 1];
-                __syncthreads();
-            }
-            const uint32_t* tk = dk;
-            const uint32_t* tv = dv;
-            dk = const_cast<uint32_t*>(sk);
-            dv = const_cast<uint32_t*>(sv);
-            sk = tk;
-            sv = tv;
-        }
-        for (int i = tid; i < n; i += kSegThreads) ids[b.x + i] = sv[b.x + i];
-        __syncthreads();
-    }
-}
-
-// The per-tile depth order of every run of bins[0..num_bins): queue[0] and
-// queue[1] (zeroed) count the runs passed to the big and the long kernel,
-// whose tiles go to queue_tiles[0..num_bins) and [num_bins..2 num_bins).
-// scratch: four free buffers of the intersection capacity (the first two
-// adjacent, as the big kernel's uint2 scratch).
+// The per-tile depth order of every run of bins[0..num_bins).  ka, va: free
+// buffers of the intersection capacity; kv_tmp: a free one of twice that.
 hipError_t tile_depth_order(const int2* bins, int num_bins, const uint32_t* dkeys,
-                            uint32_t* ids, int* queue, int* queue_tiles, uint32_t* ka,
-                            uint32_t* va, uint32_t* kb, uint32_t* vb, hipStream_t s) {
+                            uint32_t* ids, uint32_t* ka, uint32_t* va, uint2* kv_tmp,
+                            hipStream_t s) {
     if (num_bins <= 0) return hipSuccess;
-    int* big_tiles = queue_tiles;
-    int* long_tiles = queue_tiles + num_bins;
-    tile_depth_sort_kernel<<<num_bins, kSegThreads, 0, s>>>(bins, dkeys, ids, queue, big_tiles);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    const int qb = num_bins < kSegQueueBlocks ? num_bins : kSegQueueBlocks;
-    tile_depth_sort_big_kernel<<<qb, kBigThreads, 0, s>>>(
-        bins, dkeys, ids, queue, big_tiles, reinterpret_cast<uint2*>(ka), queue + 1, long_tiles);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    tile_depth_sort_long_kernel<<<qb, kSegThreads, 0, s>>>(bins, dkeys, ids, queue + 1,
-                                                          long_tiles, ka, va, kb, vb);
+    tile_depth_sort_kernel<<<num_bins, kSegThreads, 0, s>>>(bins, dkeys, ids, ka, va, kv_tmp);
     return hipGetLastError();
 }
 
@@ -1067,11 +834,10 @@ DepthLayout depth_layout(const BinPlan& p) {
 // Emission stage (emission, tile sort, bin edges, per-tile depth order); the
 // error word comes first.  The zeroed header is part of the C ABI
 // (jaxsplat_bin_emit_sort): the error word at byte 0 and the exact
-// intersection total (uint64) at byte 8; the counts of runs queued for the
-// longer per-tile sorts follow at bytes 16 and 20.
+// intersection total (uint64) at byte 8.
 struct BinLayout {
-    size_t err, total, queue, zero_end;
-    size_t tile_sort, queue_tiles, tiles, gids, ttmp_k, ttmp_v, tiles_sorted, end;
+    size_t err, total, zero_end;
+    size_t tile_sort, tiles, gids, ttmp_k, ttmp_v, tiles_sorted, end;
     EmitLayout emit;
 };
 
@@ -1080,12 +846,10 @@ BinLayout layout(const BinPlan& p) {
     size_t o = 0;
     L.err = o;
     L.total = o + 8;
-    L.queue = o + 16;
-    o += 32;
+    o += 16;
     L.zero_end = o;
     o = align256(o);
     L.tile_sort = o; o = align256(o + p.tile_sort.scratch_bytes);
-    L.queue_tiles = o; o = align256(o + 8ull * (p.num_tiles > 0 ? p.num_tiles : 1));
     o = emit_layout(o, p.n, p.isect_cap, false, L.emit);
     const size_t i4 = align256(4ull * (p.isect_cap > 0 ? p.isect_cap : 1));
     L.tiles = o; o += i4;
@@ -1172,12 +936,10 @@ hipError_t bin_emit(const BinPlan& p, const Camera& c, int num_isect_host,
                                                    reinterpret_cast<int2*>(tile_bins), err);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     stage_mark(kMarkBinned, s);
-    // per-tile depth order; the sort's ping-pong buffers and the emission's
-    // pair buffers are free again and serve the longer runs
+    // per-tile depth order; the emission's pair buffers and the sort's
+    // ping-pong buffers (adjacent) are free again and serve the longer runs
     return tile_depth_order(reinterpret_cast<const int2*>(tile_bins), p.num_tiles, dkeys, ids,
-                            reinterpret_cast<int*>(b + L.queue),
-                            reinterpret_cast<int*>(b + L.queue_tiles), U(L.ttmp_k),
-                            U(L.ttmp_v), U(L.tiles), U(L.gids), s);
+                            U(L.tiles), U(L.gids), reinterpret_cast<uint2*>(b + L.ttmp_k), s);
 }
 
 // Binning of given tile rectangles with their inclusive counts (the render
@@ -1228,10 +990,8 @@ hipError_t bin_rects(const BinPlan& p, const Camera& c, const float* depths,
     if ((e = hipGetLastError()) != hipSuccess) return e;
     stage_mark(kMarkBinned, s);
     return tile_depth_order(reinterpret_cast<const int2*>(tile_bins), p.num_tiles,
-                            reinterpret_cast<const uint32_t*>(depths), ids,
-                            reinterpret_cast<int*>(b + L.queue),
-                            reinterpret_cast<int*>(b + L.queue_tiles), U(L.ttmp_k),
-                            U(L.ttmp_v), U(L.tiles), U(L.gids), s);
+                            reinterpret_cast<const uint32_t*>(depths), ids, U(L.tiles),
+                            U(L.gids), reinterpret_cast<uint2*>(b + L.ttmp_k), s);
 }
 
 // One workspace: the emission stage's region (error word first), then the
@@ -1262,8 +1022,8 @@ hipError_t bin_and_sort(const BinPlan& p, const Camera& c, const float* xys, con
 namespace {
 
 struct BatchLayout {
-    size_t err, count, total, queue, zero_end;
-    size_t tile_sort, queue_tiles, vkeys, keys, vals, ttmp_k, ttmp_v, keys_sorted, end;
+    size_t err, count, total, zero_end;
+    size_t tile_sort, vkeys, keys, vals, ttmp_k, ttmp_v, keys_sorted, end;
     EmitLayout emit;
 };
 
@@ -1273,11 +1033,9 @@ BatchLayout batch_layout(const BatchBinPlan& p) {
     L.err = o; o += 16;
     L.count = o; o += 16;
     L.total = o; o += 16;
-    L.queue = o; o += 16;
     L.zero_end = o;
     o = align256(o);
     L.tile_sort = o; o = align256(o + p.tile_sort.scratch_bytes);
-    L.queue_tiles = o; o = align256(o + 8ull * (p.num_bins > 0 ? p.num_bins : 1));
     o = emit_layout(o, p.n_items, p.isect_cap, true, L.emit);
     const size_t n4 = align256(4ull * (p.n_items > 0 ? p.n_items : 1));
     const size_t i4 = align256(4ull * (p.isect_cap > 0 ? p.isect_cap : 1));
@@ -1353,9 +1111,8 @@ hipError_t batch_bin(const BatchBinPlan& p, const ViewRec* views, const uint2* i
                                                    reinterpret_cast<int2*>(tile_bins), err);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     return tile_depth_order(reinterpret_cast<const int2*>(tile_bins), p.num_bins, depth_keys,
-                            ids, reinterpret_cast<int*>(b + L.queue),
-                            reinterpret_cast<int*>(b + L.queue_tiles), U(L.ttmp_k),
-                            U(L.ttmp_v), U(L.keys), U(L.vals), s);
+                            ids, U(L.keys), U(L.vals), reinterpret_cast<uint2*>(b + L.ttmp_k),
+                            s);
 }
 
 }  // namespace jsa
