@@ -354,19 +354,22 @@ def stage_breakdown(torch, sc, cam, params, target, iters):
 
     splats = torch.empty((N, 12), **f32)
 
-    def project(opacity_limited):
+    def project(opacity_limited, analysis=False):
         if opacity_limited:
             # the render path: opacity-limited rectangles and splat records
+            # (xys and the per-Gaussian counts only for the analysis below:
+            # the render path writes neither)
             _lib.check(L.jaxsplat_project_forward_splats(
                 D, _p(means3d), _p(scales), _p(quats), _p(viewmat), _p(opac), _p(colors), 3,
-                _p(xys), _p(depths), _p(radii), _p(conics), None, _p(tnth), _p(tcum),
-                _p(rects), _p(splats), _p(ws), wsb, s))
+                _p(xys) if analysis else None, _p(depths), _p(radii), _p(conics), None,
+                _p(tnth) if analysis else None, _p(tcum), _p(rects), _p(splats), _p(ws), wsb,
+                s))
         else:
             _lib.check(L.jaxsplat_project_forward(
                 D, _p(means3d), _p(scales), _p(quats), _p(viewmat), None, _p(xys), _p(depths),
                 _p(radii), _p(conics), _p(comp), _p(nth), _p(cum), _p(ws), wsb, s))
 
-    project(True)
+    project(True, analysis=True)
     isect = int(tcum[-1, 0].item())                 # read once, outside the timed loop
     ntiles = d.grid_dim_2d.x * d.grid_dim_2d.y
     ewsb = L.jaxsplat_bin_emit_workspace_size(D, isect)
@@ -454,12 +457,12 @@ def stage_breakdown(torch, sc, cam, params, target, iters):
     # E with a non-empty opacity-limited rectangle, I = isect pairs binned,
     # L_ft / L_bt consumed by the forward / backward, R records flushed).
     # The projection reads 56 B per Gaussian (means, scales, quats, opacity,
-    # colour) and writes 100 (xys, depth, radius, conic, count, inclusive sum,
-    # rectangle, 48-byte splat record) + 8 of scan; a staged list entry costs
+    # colour) and writes 80 (depth, radius, conic, inclusive sum, rectangle,
+    # 48-byte splat record) + 8 of scan; a staged list entry costs
     # its id and its splat record (52 B), in the backward also its inclusive
     # sum; a flushed record is 36 B.
     stage_bytes = {
-        "project_fwd": 164 * N,
+        "project_fwd": 144 * N,
         "emission": 24 * N + 8 * isect,
         "tile_sort": 40 * isect,
         "bin_edges": 4 * isect + 8 * T,

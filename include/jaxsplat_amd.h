@@ -382,6 +382,8 @@ int jaxsplat_rasterize_backward_staged_rects(
  * records instead of gathering xys, conics, colours, opacities and
  * rectangles separately.  Results are those of jaxsplat_project_forward_rects
  * + jaxsplat_rasterize_forward_binned / the backward below, bit for bit.
+ * xys and num_tiles_hit may be NULL (the splat records hold the positions and
+ * cum_tiles_hit the counts): the fused render writes neither.
  */
 int jaxsplat_project_forward_splats(const jaxsplat_descriptor* d, const float* means3d,
                                     const float* scales, const float* quats,

@@ -537,11 +537,11 @@ class _Render(torch.autograd.Function):
         H, W = img_shape
         d = _lib.make_descriptor(N, img_shape, f, c, glob_scale, clip_thresh, block_width)
         L = _lib.lib()
-        xys = torch.empty((N, 2), dtype=torch.float32, device=dev)
+        # no xys or per-Gaussian counts: the splat records hold the positions
+        # and cum the counts
         depths = torch.empty((N, 1), dtype=torch.float32, device=dev)
         radii = torch.empty((N, 1), dtype=torch.int32, device=dev)
         conics = torch.empty((N, 3), dtype=torch.float32, device=dev)
-        nth = torch.empty((N, 1), dtype=torch.int32, device=dev)
         cum = torch.empty((N, 1), dtype=torch.int32, device=dev)
         rects = torch.empty((N, 2), dtype=torch.int32, device=dev)
         splats = torch.empty((N, 12), dtype=torch.float32, device=dev)
@@ -550,11 +550,11 @@ class _Render(torch.autograd.Function):
         with torch.cuda.device(dev):
             _lib.check(L.jaxsplat_project_forward_splats(
                 ctypes.byref(d), _p(means3d), _p(scales), _p(quats), _p(viewmat),
-                _p(opacities), _p(colors), nch, _p(xys), _p(depths), _p(radii), _p(conics),
-                None, _p(nth), _p(cum), _p(rects), _p(splats), _p(ws), ws_bytes,
+                _p(opacities), _p(colors), nch, None, _p(depths), _p(radii), _p(conics),
+                None, None, _p(cum), _p(rects), _p(splats), _p(ws), ws_bytes,
                 _stream(dev)))
         if N == 0:
-            gids, bins, isect = _bin(d, xys, depths, radii, 0, dev)
+            gids, bins, isect = _bin(d, splats[:, :2].contiguous(), depths, radii, 0, dev)
         else:
             gids, bins, isect = _bin_rects(d, depths, rects, cum, dev, capacity)
         final_Ts = torch.empty((H, W, 1), dtype=torch.float32, device=dev)
@@ -568,18 +568,19 @@ class _Render(torch.autograd.Function):
         ctx.isect = isect
         ctx.adjoint = adjoint
         ctx.save_for_backward(means3d, scales, quats, viewmat, colors, opacities, background,
-                              xys, conics, radii, cum, rects, splats, gids, bins, final_Ts,
+                              conics, radii, cum, rects, splats, gids, bins, final_Ts,
                               final_idx)
         return img
 
     @staticmethod
     def backward(ctx, v_img):
-        means3d, scales, quats, viewmat, colors, opacities, background, xys, conics, radii, \
+        means3d, scales, quats, viewmat, colors, opacities, background, conics, radii, \
             cum, rects, splats, gids, bins, final_Ts, final_idx = ctx.saved_tensors
         dev = means3d.device
         nch = colors.shape[1]
         ws, wsb = _staged_workspace(ctx.d, ctx.isect, nch, dev)
         if ws is None:
+            xys = splats[:, :2].contiguous()
             v_colors, v_opacity, v_xy, v_conic = _rasterize_backward(
                 ctx.d, ctx.isect, ctx.adjoint, colors, opacities, background, xys, conics, gids,
                 bins, final_Ts, final_idx, radii, cum, rects, v_img)

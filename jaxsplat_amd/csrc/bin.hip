@@ -739,22 +739,26 @@ size_t emit_layout(size_t o, int n, int cap, bool batch, EmitLayout& E) {
 // Emission straight from the inclusive counts (E.cum): every 64-slot chunk's
 // owner, the Gaussian whose slots [cum - count, cum) hold the chunk's first
 // slot, plus the totals of emit_scan_kernel (info[0] = slots written =
-// min(total, cap), info[1] = n, *total_out, bit 8 of err on overflow).
+// min(total, cap), info[1] = n, *total_out) and the binning's error word
+// (8 on overflow, else 0).  Threads t < num_tiles also clear bins[t] (the
+// grid covers max(n, num_tiles) threads).
 __global__ __launch_bounds__(256) void cum_owner_kernel(int n, const int* __restrict__ cum,
                                                         const uint2* __restrict__ rects,
                                                         int cap, int* __restrict__ info,
                                                         unsigned long long* __restrict__ total_out,
                                                         uint32_t* __restrict__ err,
-                                                        uint32_t* __restrict__ chunk_owner) {
+                                                        uint32_t* __restrict__ chunk_owner,
+                                                        int num_tiles, int2* __restrict__ bins) {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < num_tiles) bins[g] = make_int2(0, 0);
     if (g >= n) return;
     const uint32_t total = (uint32_t)cum[n - 1];
     const uint32_t limit = total < (uint32_t)cap ? total : (uint32_t)cap;
     if (g == n - 1) {
-        if (total_out) *total_out = total;
+        *total_out = total;
+        *err = total > (uint32_t)cap ? 8u : 0u;
         info[0] = (int)limit;
         info[1] = n;
-        if (total > (uint32_t)cap) atomicOr(err, 8u);
     }
     const uint32_t cnt = rect_count(rects[g]);
     if (cnt == 0) return;
@@ -952,11 +956,11 @@ hipError_t bin_rects(const BinPlan& p, const Camera& c, const float* depths,
                      int* tiles_sorted, int* tile_bins, void* scratch, hipStream_t s) {
     const BinLayout L = layout(p);
     uint8_t* b = static_cast<uint8_t*>(scratch);
-    hipError_t e = fill_zero(b, L.zero_end, s);
-    if (e != hipSuccess) return e;
-    e = fill_zero(tile_bins, 8ull * p.num_tiles, s);
-    if (e != hipSuccess) return e;
-    if (p.n <= 0) return hipSuccess;
+    hipError_t e;
+    if (p.n <= 0) {
+        if ((e = fill_zero(b, L.zero_end, s)) != hipSuccess) return e;
+        return fill_zero(tile_bins, 8ull * p.num_tiles, s);
+    }
     uint32_t* err = reinterpret_cast<uint32_t*>(b + L.err);
     auto U = [&](size_t off) { return reinterpret_cast<uint32_t*>(b + off); };
     int* written = reinterpret_cast<int*>(b + L.emit.info);
@@ -964,9 +968,12 @@ hipError_t bin_rects(const BinPlan& p, const Camera& c, const float* depths,
     E.chunk_owner = U(L.emit.chunk_owner);
     E.cum = cum;
     E.rect_by_id = rects;
-    cum_owner_kernel<<<(p.n + 255) / 256, 256, 0, s>>>(
+    // also writes the error word and the total, and clears the bins
+    const int threads = p.n > p.num_tiles ? p.n : p.num_tiles;
+    cum_owner_kernel<<<(threads + 255) / 256, 256, 0, s>>>(
         p.n, cum, rects, p.isect_cap, written,
-        reinterpret_cast<unsigned long long*>(b + L.total), err, E.chunk_owner);
+        reinterpret_cast<unsigned long long*>(b + L.total), err, E.chunk_owner, p.num_tiles,
+        reinterpret_cast<int2*>(tile_bins));
     if ((e = hipGetLastError()) != hipSuccess) return e;
     const FirstCounts fc = first_counts(p.tile_sort, b + L.tile_sort);
     const long long chunks = ((long long)p.isect_cap + 63) / 64;

@@ -310,8 +310,8 @@ int jaxsplat_project_forward_splats(const jaxsplat_descriptor* d, const float* m
         return fail(JAXSPLAT_ERR_INVALID,
                     "project_forward_splats: channels must be in [1, 3], got %u", channels);
     if (d->num_points == 0) return JAXSPLAT_OK;
-    if (!means3d || !scales || !quats || !viewmat || !opacities || !colors || !xys || !depths ||
-        !radii || !conics || !num_tiles_hit || !cum_tiles_hit || !tile_rects || !splats)
+    if (!means3d || !scales || !quats || !viewmat || !opacities || !colors || !depths ||
+        !radii || !conics || !cum_tiles_hit || !tile_rects || !splats)
         return fail(JAXSPLAT_ERR_INVALID, "project_forward_splats: NULL required pointer");
     if (!workspace || workspace_bytes < jaxsplat_project_workspace_size(d))
         return fail(JAXSPLAT_ERR_WORKSPACE, "project_forward_splats: workspace too small");

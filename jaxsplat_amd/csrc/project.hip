@@ -229,8 +229,8 @@ __device__ __forceinline__ Proj project_core(const Camera& c, int idx, const M3&
     return r;
 }
 
-// Plain projection of Gaussian idx: writes every output row and returns
-// num_tiles_hit.
+// Plain projection of Gaussian idx: writes every output row (xys and
+// num_tiles_hit only where given) and returns num_tiles_hit.
 __device__ __forceinline__ int project_one(
     const Camera& c, int idx, const M3& W, const float* pt, const float* __restrict__ means3d,
     const float* __restrict__ scales, const float* __restrict__ quats,
@@ -248,7 +248,7 @@ __device__ __forceinline__ int project_one(
         o[1] = make_float2(cov[2], cov[3]);
         o[2] = make_float2(cov[4], cov[5]);
     }
-    reinterpret_cast<float2*>(xys)[idx] = make_float2(r.cx, r.cy);
+    if (xys) reinterpret_cast<float2*>(xys)[idx] = make_float2(r.cx, r.cy);
     depths[idx] = r.depth;
     radii[idx] = r.rad;
     conics[3 * idx + 0] = conic[0];
@@ -273,7 +273,7 @@ __device__ __forceinline__ int project_one(
                                 __uint_as_float(rc.y));
         }
     }
-    num_tiles_hit[idx] = count;
+    if (num_tiles_hit) num_tiles_hit[idx] = count;
     return count;
 }
 
@@ -667,13 +667,23 @@ __global__ __launch_bounds__(256) void render_bwd_tail_kernel(
     const int idx = blockIdx.x * 256 + threadIdx.x;
     const int lane = lane_id();
     const bool own = idx < c.num_points;
-    const bool vis = own && radii[idx] > 0;
+    // every per-Gaussian input is loaded up front, so that only the bitmap
+    // and the records wait on the rectangle and the count
+    const int gi = own ? idx : 0;
+    const int rad = own ? radii[gi] : 0;
+    const uint2 rc = rects[gi];
+    const int cend = cum[gi];
+    const float m[3] = {means3d[3 * gi + 0], means3d[3 * gi + 1], means3d[3 * gi + 2]};
+    const float sc[3] = {scales[3 * gi + 0], scales[3 * gi + 1], scales[3 * gi + 2]};
+    const float4 q4 = reinterpret_cast<const float4*>(quats)[gi];
+    const float cn[3] = {conics[3 * gi + 0], conics[3 * gi + 1], conics[3 * gi + 2]};
+    const bool vis = own && rad > 0;
     float4 a0 = make_float4(0.f, 0.f, 0.f, 0.f), a1 = a0, a2 = a0;
     int off = 0, end = 0;
     if (vis) {
         int x0, x1, y0, y1;
-        unpack_rect(rects[idx], x0, x1, y0, y1);
-        end = cum[idx];
+        unpack_rect(rc, x0, x1, y0, y1);
+        end = cend;
         off = end - (x1 - x0) * (y1 - y0);
         if (off < 0 || end > num_isect) off = end = 0;     // inconsistent input: zeros
     }
@@ -689,16 +699,12 @@ __global__ __launch_bounds__(256) void render_bwd_tail_kernel(
         M3 W;
         float pt[3];
         load_view(viewmat, W, pt);
-        const float m[3] = {means3d[3 * idx + 0], means3d[3 * idx + 1], means3d[3 * idx + 2]};
-        const float sc[3] = {scales[3 * idx + 0], scales[3 * idx + 1], scales[3 * idx + 2]};
-        const float4 q4 = reinterpret_cast<const float4*>(quats)[idx];
         const float q[4] = {q4.x, q4.y, q4.z, q4.w};
         float cov[6];
         cov3d_of(sc, q, c.glob_scale, cov);
         float vc2[3], vc3[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-        view_vjp(c, W, pt, m, cov, conics[3 * idx + 0], conics[3 * idx + 1], conics[3 * idx + 2],
-                 make_float2(a1.z, a1.w), 0.f, a0.w, a1.x, a1.y, false, 0.f, 0.f, consistent,
-                 vmean, vc2, vc3);
+        view_vjp(c, W, pt, m, cov, cn[0], cn[1], cn[2], make_float2(a1.z, a1.w), 0.f, a0.w,
+                 a1.x, a1.y, false, 0.f, 0.f, consistent, vmean, vc2, vc3);
         cov3d_vjp(sc, q, c.glob_scale, vc3, vsc, vq);
     }
 #pragma unroll
