@@ -19,8 +19,9 @@
 //     max-scan), so neither a Gaussian covering hundreds of tiles nor a run
 //     of large Gaussians serialises a lane or a wave; the expansion also
 //     counts the tile sort's first digit;
-//  3. a stable LSD radix sort of the pairs on the tile id bits only
-//     (2 passes at 1080p/16x16): order (tile, gaussian id);
+//  3. an LSD radix sort of the pairs on the tile id bits only (2 passes at
+//     1080p/16x16; the first need not be stable, the order inside a tile
+//     being set in step 5);
 //  4. tile_bins_kernel: [start, end) edges from the sorted tile ids, empty
 //     tiles (0, 0);
 //  5. tile_depth_sort_kernel: every tile's run ordered by (depth bits, id)
@@ -464,21 +465,22 @@ __global__ __launch_bounds__(256) void tile_bins_kernel(const uint32_t* __restri
 // Per-tile depth order
 // ---------------------------------------------------------------------------
 // A tile's run [bins[t].x, bins[t].y) of the tile-sorted ids holds the
-// Gaussians whose rectangle covers the tile.  Ordering it by (depth bits, id)
-// gives the reference order inside the tile.
+// Gaussians whose rectangle covers the tile, in any order.  Ordering it by
+// (depth bits, id) gives the reference order inside the tile.
 //
-// Bucket order: the run's keys are split into 2^11 buckets linear in
-// key - min (the shift makes max - min fit 11 bits, so at least half of the
-// buckets are in range); an LDS histogram with returning atomics gives every
-// entry its bucket and a slot in it, a scan the bucket starts, and the
-// entries are placed by bucket.  Each entry's final position is then its
-// bucket's start plus the number of entries of its bucket that precede it in
-// (key, id) order, counted by reading the bucket.  Depths inside a tile are
-// spread over a range of floats, so buckets are small (at 1080p with 1M
-// Gaussians about 800 entries per tile, at most 12 in a bucket); the order is
-// exact whatever the distribution, only the counting reads grow with the
-// bucket sizes.  A run whose keys are all equal is already in order (it is in
-// id order, from the stable tile sort of pairs emitted in id order).
+// Bucket order: every entry gets the composite v = (key - kmin) << ib |
+// (id - imin), ib the bit length of the id range, whose order is the
+// (key, id) order; the run is split into 2^11 buckets linear in v (the shift
+// makes the largest v fit 11 bits, so at least half of the buckets are in
+// range).  An LDS histogram with returning atomics gives every entry its
+// bucket and a slot in it, a scan the bucket starts, and the entries are
+// placed by bucket.  Each entry's final position is then its bucket's start
+// plus the number of entries of its bucket that precede it in (key, id)
+// order, counted by reading the bucket.  Depths inside a tile are spread over
+// a range of floats (at 1080p with 1M Gaussians about 800 entries per tile,
+// at most 12 in a bucket), and where many keys are equal the ids spread the
+// buckets instead; the order is exact whatever the distribution, only the
+// counting reads grow with the bucket sizes.
 //
 // One 256-thread workgroup per tile: runs of up to kSegCap entries are held
 // in registers and placed in LDS; longer ones go through global scratch at
@@ -490,22 +492,18 @@ constexpr int kSegCap = kSegThreads * kSegItems;       // 1536
 constexpr int kSegBucketBits = 11;
 constexpr int kSegBuckets = 1 << kSegBucketBits;
 
-// Unsigned min and max over the 64 lanes (full EXEC): DPP inside the rows,
+// Unsigned min / max over the 64 lanes (full EXEC): DPP inside the rows,
 // then the four rows by readlane.
-__device__ __forceinline__ void wave_minmax_u32(uint32_t& mn, uint32_t& mx) {
-    int a = (int)mn, b = (int)mx;
-    a = (int)min((uint32_t)a, (uint32_t)dpp_i<kDppQuadPerm1032>(a));
-    b = (int)max((uint32_t)b, (uint32_t)dpp_i<kDppQuadPerm1032>(b));
-    a = (int)min((uint32_t)a, (uint32_t)dpp_i<kDppQuadPerm2301>(a));
-    b = (int)max((uint32_t)b, (uint32_t)dpp_i<kDppQuadPerm2301>(b));
-    a = (int)min((uint32_t)a, (uint32_t)dpp_i<kDppRowRor4>(a));
-    b = (int)max((uint32_t)b, (uint32_t)dpp_i<kDppRowRor4>(b));
-    a = (int)min((uint32_t)a, (uint32_t)dpp_i<kDppRowRor8>(a));
-    b = (int)max((uint32_t)b, (uint32_t)dpp_i<kDppRowRor8>(b));
-    mn = min(min((uint32_t)__builtin_amdgcn_readlane(a, 0), (uint32_t)__builtin_amdgcn_readlane(a, 16)),
-             min((uint32_t)__builtin_amdgcn_readlane(a, 32), (uint32_t)__builtin_amdgcn_readlane(a, 48)));
-    mx = max(max((uint32_t)__builtin_amdgcn_readlane(b, 0), (uint32_t)__builtin_amdgcn_readlane(b, 16)),
-             max((uint32_t)__builtin_amdgcn_readlane(b, 32), (uint32_t)__builtin_amdgcn_readlane(b, 48)));
+template <bool MAX>
+__device__ __forceinline__ uint32_t wave_extreme_u32(uint32_t v) {
+    auto op = [](uint32_t a, uint32_t b) { return MAX ? max(a, b) : min(a, b); };
+    int a = (int)v;
+    a = (int)op((uint32_t)a, (uint32_t)dpp_i<kDppQuadPerm1032>(a));
+    a = (int)op((uint32_t)a, (uint32_t)dpp_i<kDppQuadPerm2301>(a));
+    a = (int)op((uint32_t)a, (uint32_t)dpp_i<kDppRowRor4>(a));
+    a = (int)op((uint32_t)a, (uint32_t)dpp_i<kDppRowRor8>(a));
+    return op(op((uint32_t)__builtin_amdgcn_readlane(a, 0), (uint32_t)__builtin_amdgcn_readlane(a, 16)),
+              op((uint32_t)__builtin_amdgcn_readlane(a, 32), (uint32_t)__builtin_amdgcn_readlane(a, 48)));
 }
 
 // 20 KB, so that eight workgroups fit a CU.  red and tmp are used before the
@@ -514,7 +512,7 @@ struct SegShared {
     union {
         uint2 kv[kSegCap];                          // placed (key, id)
         struct {
-            uint32_t red[2 * kSegWaves];
+            uint32_t red[4 * kSegWaves];
             uint32_t tmp[kSegWaves];
         };
     };
@@ -524,29 +522,53 @@ static_assert(kSegBuckets >= kSegCap, "the ids are staged in cnt");
 static_assert(sizeof(SegShared) <= 20480, "eight workgroups per CU");
 static_assert(kSegBuckets % kSegThreads == 0, "bucket scan: whole runs per thread");
 
-// Block min / max.  Ends with a barrier.
-__device__ __forceinline__ void block_minmax_u32(uint32_t& mn, uint32_t& mx, SegShared& sh) {
+// The run's composite bucketing: v = (key - kmin) << ib | (id - imin),
+// bucket = v >> shift.
+struct Buckets {
+    uint32_t kmin, imin;
+    int ib, shift;
+    __device__ __forceinline__ uint32_t of(uint32_t key, uint32_t id) const {
+        const uint64_t v = ((uint64_t)(key - kmin) << ib) | (uint64_t)(id - imin);
+        return (uint32_t)(v >> shift);
+    }
+};
+
+// Block min / max of the keys and ids (ids of a run of two or more entries
+// differ, so the composite range is never empty).  Ends with a barrier.
+__device__ __forceinline__ Buckets block_buckets(uint32_t kmin, uint32_t kmax, uint32_t imin,
+                                                 uint32_t imax, SegShared& sh) {
     const int wave = threadIdx.x >> 6;
-    wave_minmax_u32(mn, mx);
+    kmin = wave_extreme_u32<false>(kmin);
+    kmax = wave_extreme_u32<true>(kmax);
+    imin = wave_extreme_u32<false>(imin);
+    imax = wave_extreme_u32<true>(imax);
     if (lane_id() == 0) {
-        sh.red[wave] = mn;
-        sh.red[kSegWaves + wave] = mx;
+        sh.red[wave] = kmin;
+        sh.red[kSegWaves + wave] = kmax;
+        sh.red[2 * kSegWaves + wave] = imin;
+        sh.red[3 * kSegWaves + wave] = imax;
     }
     __syncthreads();
-    mn = sh.red[0];
-    mx = sh.red[kSegWaves];
+    kmin = sh.red[0];
+    kmax = sh.red[kSegWaves];
+    imin = sh.red[2 * kSegWaves];
+    imax = sh.red[3 * kSegWaves];
 #pragma unroll
     for (int w = 1; w < kSegWaves; ++w) {
-        mn = min(mn, sh.red[w]);
-        mx = max(mx, sh.red[kSegWaves + w]);
+        kmin = min(kmin, sh.red[w]);
+        kmax = max(kmax, sh.red[kSegWaves + w]);
+        imin = min(imin, sh.red[2 * kSegWaves + w]);
+        imax = max(imax, sh.red[3 * kSegWaves + w]);
     }
     __syncthreads();
-}
-
-// The shift that maps key - kmin of [kmin, kmax] onto the buckets.
-__device__ __forceinline__ int bucket_shift(uint32_t kmin, uint32_t kmax) {
-    const int span_bits = 32 - __builtin_clz(kmax - kmin);
-    return span_bits > kSegBucketBits ? span_bits - kSegBucketBits : 0;
+    Buckets bk;
+    bk.kmin = kmin;
+    bk.imin = imin;
+    bk.ib = imax > imin ? 32 - __builtin_clz(imax - imin) : 0;
+    const uint64_t vmax = ((uint64_t)(kmax - kmin) << bk.ib) | (uint64_t)(imax - imin);
+    const int vbits = 64 - __builtin_clzll(vmax);
+    bk.shift = vbits > kSegBucketBits ? vbits - kSegBucketBits : 0;
+    return bk;
 }
 
 // Counts in sh.cnt[0..kSegBuckets) -> exclusive starts (the last bucket
@@ -599,32 +621,34 @@ __device__ void long_run_order(int base, int n, const uint32_t* __restrict__ dke
                                uint32_t* __restrict__ va, uint2* __restrict__ kv,
                                SegShared& sh) {
     const int tid = threadIdx.x;
-    uint32_t kmin = ~0u, kmax = 0u;
+    uint32_t kmin = ~0u, kmax = 0u, imin = ~0u, imax = 0u;
     for (int e = tid; e < n; e += kSegThreads) {
-        const uint32_t k = dkeys[ids[base + e]];
+        const uint32_t id = ids[base + e];
+        const uint32_t k = dkeys[id];
         ka[base + e] = k;
         kmin = min(kmin, k);
         kmax = max(kmax, k);
+        imin = min(imin, id);
+        imax = max(imax, id);
     }
-    block_minmax_u32(kmin, kmax, sh);
-    if (kmin == kmax) return;                           // uniform: already in order
-    const int shift = bucket_shift(kmin, kmax);
+    const Buckets bk = block_buckets(kmin, kmax, imin, imax, sh);
     for (int i = tid; i < kSegBuckets; i += kSegThreads) sh.cnt[i] = 0u;
     __syncthreads();
     for (int e = tid; e < n; e += kSegThreads)
-        va[base + e] = atomicAdd(&sh.cnt[(ka[base + e] - kmin) >> shift], 1u);
+        va[base + e] = atomicAdd(&sh.cnt[bk.of(ka[base + e], ids[base + e])], 1u);
     __syncthreads();
     bucket_scan(sh);
     for (int e = tid; e < n; e += kSegThreads) {
         const uint32_t k = ka[base + e];
-        kv[base + sh.cnt[(k - kmin) >> shift] + va[base + e]] = make_uint2(k, ids[base + e]);
+        const uint32_t id = ids[base + e];
+        kv[base + sh.cnt[bk.of(k, id)] + va[base + e]] = make_uint2(k, id);
     }
     __syncthreads();                                    // ids are no longer read
     for (int j = tid; j < n; j += kSegThreads) {
         const uint2 me = kv[base + j];
-        const uint32_t bk = (me.x - kmin) >> shift;
-        const uint32_t st = sh.cnt[bk];
-        const uint32_t rank = bucket_rank(kv + base, st, bucket_end(sh, bk, n) - st, me.x, me.y);
+        const uint32_t b = bk.of(me.x, me.y);
+        const uint32_t st = sh.cnt[b];
+        const uint32_t rank = bucket_rank(kv + base, st, bucket_end(sh, b, n) - st, me.x, me.y);
         ids[base + st + rank] = me.y;
     }
 }
@@ -653,22 +677,22 @@ __global__ __launch_bounds__(kSegThreads) void tile_depth_sort_kernel(
     }
     // the bucket counts are cleared while the loads are in flight
     for (int i = tid; i < kSegBuckets; i += kSegThreads) sh.cnt[i] = 0u;
-    uint32_t kmin = ~0u, kmax = 0u;
+    uint32_t kmin = ~0u, kmax = 0u, imin = ~0u, imax = 0u;
 #pragma unroll
     for (int r = 0; r < kSegItems; ++r) {
         key[r] = valid[r] ? dkeys[val[r]] : 0u;
         if (valid[r]) {
             kmin = min(kmin, key[r]);
             kmax = max(kmax, key[r]);
+            imin = min(imin, val[r]);
+            imax = max(imax, val[r]);
         }
     }
-    block_minmax_u32(kmin, kmax, sh);
-    if (kmin == kmax) return;                           // uniform: already in order
-    const int shift = bucket_shift(kmin, kmax);
+    const Buckets bkt = block_buckets(kmin, kmax, imin, imax, sh);
     uint32_t bk[kSegItems], slot[kSegItems];
 #pragma unroll
     for (int r = 0; r < kSegItems; ++r) {
-        bk[r] = (key[r] - kmin) >> shift;
+        bk[r] = valid[r] ? bkt.of(key[r], val[r]) : 0u;
         slot[r] = valid[r] ? atomicAdd(&sh.cnt[bk[r]], 1u) : 0u;
     }
     __syncthreads();
@@ -929,9 +953,11 @@ hipError_t bin_emit(const BinPlan& p, const Camera& c, int num_isect_host,
     stage_mark(kMarkEmitted, s);
     uint32_t* tiles_out = tiles_sorted ? reinterpret_cast<uint32_t*>(tiles_sorted) : U(L.tiles_sorted);
     uint32_t* ids = reinterpret_cast<uint32_t*>(gids_sorted);
+    // the order inside a tile comes from the per-tile sort: the first pass
+    // need not be stable
     e = sort_pairs(p.tile_sort, U(L.tiles), U(L.gids), tiles_out, ids, U(L.ttmp_k),
                    U(L.ttmp_v), num_isect_host, written, b + L.tile_sort, err, s,
-                   fc.counts != nullptr);
+                   fc.counts != nullptr, true);
     if (e != hipSuccess) return e;
     stage_mark(kMarkTileSorted, s);
     const int nb = (int)(((long long)p.isect_cap + 256 * kTileBinsPer - 1) / (256 * kTileBinsPer));
@@ -985,9 +1011,11 @@ hipError_t bin_rects(const BinPlan& p, const Camera& c, const float* depths,
     stage_mark(kMarkEmitted, s);
     uint32_t* tiles_out = tiles_sorted ? reinterpret_cast<uint32_t*>(tiles_sorted) : U(L.tiles_sorted);
     uint32_t* ids = reinterpret_cast<uint32_t*>(gids_sorted);
+    // the order inside a tile comes from the per-tile sort: the first pass
+    // need not be stable
     e = sort_pairs(p.tile_sort, U(L.tiles), U(L.gids), tiles_out, ids, U(L.ttmp_k),
                    U(L.ttmp_v), num_isect_host, written, b + L.tile_sort, err, s,
-                   fc.counts != nullptr);
+                   fc.counts != nullptr, true);
     if (e != hipSuccess) return e;
     stage_mark(kMarkTileSorted, s);
     const int nb = (int)(((long long)p.isect_cap + 256 * kTileBinsPer - 1) / (256 * kTileBinsPer));

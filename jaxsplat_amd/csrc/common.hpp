@@ -509,7 +509,8 @@ SortPlan sort_plan(int n_cap, int begin_bit, int end_bit);
 // fill them itself and pass counted = true to sort_pairs, which then skips
 // that pass's counting kernel.
 uint32_t* sort_first_counts(const SortPlan& p, void* scratch);
-// Sorts (keys_in, vals_in) -> (keys_out, vals_out) stably on [begin, end).
+// Sorts (keys_in, vals_in) -> (keys_out, vals_out) stably on [begin, end)
+// (with unstable_first, equal keys may leave in any order).
 // vals_in == nullptr means values = index.  n is read from n_dev when non-null.
 // tmp_keys/tmp_vals must hold n_cap entries (ping-pong buffers).  The scratch
 // area must be zeroed by the caller (sort_zero_bytes of it) before the call.
@@ -517,7 +518,8 @@ size_t sort_zero_bytes(const SortPlan& p);
 hipError_t sort_pairs(const SortPlan& p, const uint32_t* keys_in, const uint32_t* vals_in,
                       uint32_t* keys_out, uint32_t* vals_out, uint32_t* tmp_keys,
                       uint32_t* tmp_vals, int n_host, const int* n_dev, void* scratch,
-                      uint32_t* err, hipStream_t s, bool counted = false);
+                      uint32_t* err, hipStream_t s, bool counted = false,
+                      bool unstable_first = false);
 
 // ---- binning ----
 struct BinPlan {

@@ -126,8 +126,11 @@ __global__ __launch_bounds__(kSortThreads) void radix_colscan(uint32_t* __restri
     if (tid == 0) totals[blockIdx.x] = s_carry;
 }
 
-// Stable ranking + scatter of one partition with known global offsets.
-template <int ITEMS>
+// Ranking + scatter of one partition with known global offsets: stable, or
+// (STABLE false) in any order inside a digit, ranked by one returning LDS
+// atomic per key instead of a ballot per digit bit (measured 31.5 against
+// 41 us for 6.6M keys and 7-bit digits).
+template <int ITEMS, bool STABLE>
 __global__ __launch_bounds__(kSortThreads) void radix_downsweep(
     const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
     uint32_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out, int n_host,
@@ -169,18 +172,22 @@ __global__ __launch_bounds__(kSortThreads) void radix_downsweep(
         val[r] = in ? (vals_in ? vals_in[p] : (uint32_t)p) : 0u;
         keep[r] = in;
     }
-    // ---- stable wave-local ranking with 64-lane ballots ----
+    // ---- wave-local ranking: stable with 64-lane ballots, else atomics ----
 #pragma unroll
     for (int r = 0; r < ITEMS; ++r) {
         const bool valid = keep[r];
         const uint32_t d = (key[r] >> shift) & digit_mask;
-        const uint64_t peers = match_digit(d, bits, ballot(valid));
-        uint32_t prior = 0;
-        if (valid) prior = s_whist[wave][d];
-        const bool leader = valid && ((peers & ~(lt | (1ull << lane))) == 0);
-        if (leader) s_whist[wave][d] = prior + (uint32_t)__popcll(peers);
-        rank[r] = (uint16_t)(prior + (uint32_t)__popcll(peers & lt));
-        wave_lds_sync();
+        if (STABLE) {
+            const uint64_t peers = match_digit(d, bits, ballot(valid));
+            uint32_t prior = 0;
+            if (valid) prior = s_whist[wave][d];
+            const bool leader = valid && ((peers & ~(lt | (1ull << lane))) == 0);
+            if (leader) s_whist[wave][d] = prior + (uint32_t)__popcll(peers);
+            rank[r] = (uint16_t)(prior + (uint32_t)__popcll(peers & lt));
+            wave_lds_sync();
+        } else {
+            rank[r] = valid ? (uint16_t)atomicAdd(&s_whist[wave][d], 1u) : (uint16_t)0;
+        }
     }
     __syncthreads();
 
@@ -258,7 +265,7 @@ uint32_t* sort_first_counts(const SortPlan& p, void* scratch) {
 hipError_t sort_pairs(const SortPlan& p, const uint32_t* keys_in, const uint32_t* vals_in,
                       uint32_t* keys_out, uint32_t* vals_out, uint32_t* tmp_keys,
                       uint32_t* tmp_vals, int n_host, const int* n_dev, void* scratch,
-                      uint32_t* err, hipStream_t s, bool counted) {
+                      uint32_t* err, hipStream_t s, bool counted, bool unstable_first) {
     (void)err;
     if (p.passes <= 0 || p.passes > kMaxPasses) return hipErrorInvalidValue;
     uint32_t* totals = static_cast<uint32_t*>(scratch);
@@ -280,6 +287,9 @@ hipError_t sort_pairs(const SortPlan& p, const uint32_t* keys_in, const uint32_t
         const int bits = min(per, p.end_bit - shift);
         // the first pass's counts may come from the keys' producer
         const bool count_here = !(counted && q == 0);
+        // an LSD sort needs every pass after the first to be stable; the
+        // first one only where the caller needs equal keys in input order
+        const bool stable = !(unstable_first && q == 0);
 #define JSA_PASS(IT)                                                                         \
     do {                                                                                     \
         if (count_here) {                                                                    \
@@ -289,9 +299,12 @@ hipError_t sort_pairs(const SortPlan& p, const uint32_t* keys_in, const uint32_t
         }                                                                                    \
         radix_colscan<<<kBins, kSortThreads, 0, s>>>(counts, p.parts, totals);               \
         if ((e = hipGetLastError()) != hipSuccess) return e;                                 \
-        radix_downsweep<IT><<<p.parts, kSortThreads, 0, s>>>(ki, vi, ko, vo, n_host, n_dev,  \
-                                                              shift, bits, p.parts, counts,  \
-                                                              totals);                       \
+        if (stable)                                                                          \
+            radix_downsweep<IT, true><<<p.parts, kSortThreads, 0, s>>>(                      \
+                ki, vi, ko, vo, n_host, n_dev, shift, bits, p.parts, counts, totals);        \
+        else                                                                                 \
+            radix_downsweep<IT, false><<<p.parts, kSortThreads, 0, s>>>(                     \
+                ki, vi, ko, vo, n_host, n_dev, shift, bits, p.parts, counts, totals);        \
         if ((e = hipGetLastError()) != hipSuccess) return e;                                 \
     } while (0)
         if (items == 16) JSA_PASS(16);
