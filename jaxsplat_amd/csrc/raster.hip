@@ -487,7 +487,6 @@ struct PixState {
     float bv;
     int fin;
     bool inside;
-    uint64_t inside_m;  // ballot(inside), fixed for the kernel
 };
 
 // Row (16-lane) sums of v[0..NV) by recursive halving.  At each DPP level
@@ -551,9 +550,10 @@ __device__ __forceinline__ uint64_t bwd_pixel(PixState& s, float dx, float dy, f
 #define JSA_ACC(i, v) (g[i] += (v))
     constexpr float kClamp = CONSISTENT ? 0.999f : 0.99f;
     const float raw = a.z * __builtin_amdgcn_exp2f(-s2);
+    // a pixel outside the image has fin = -1 (k >= 0), so live implies inside
     const bool live = k <= s.fin;
     const bool acc = pair_accepted(s2, col.w);
-    const bool valid = s.inside && live && acc;
+    const bool valid = live && acc;
     // an invalid pair acts as alpha = 0: T unchanged (1 / (1 - 0) is exact), no weight
     const float rv = valid ? raw : 0.f;
     const float al = fminf(kClamp, rv);
@@ -583,7 +583,7 @@ __device__ __forceinline__ uint64_t bwd_pixel(PixState& s, float dx, float dy, f
     JSA_ACC(NV - 1, vs);
 #undef JSA_ACC
     // the lanes with a valid pair, from ballots of the plain compares
-    return ballot(live) & ballot(acc) & s.inside_m;
+    return ballot(live) & ballot(acc);
 }
 
 // Per-Gaussian factors of the partial sums of bwd_pixel (the batch's base-2
@@ -681,7 +681,6 @@ __device__ __forceinline__ void raster_bwd_tile(
         const int j = x_org + rc.y;
         PixState& s = ps[sl];
         s.inside = pixel && i < vr.img_h && j < vr.img_w;
-        s.inside_m = ballot(s.inside);
         s.px = (float)j + 0.5f;
         s.py = (float)i + 0.5f;
         const size_t pix = s.inside ? (size_t)vr.pix_off + (size_t)i * vr.img_w + j : 0;
