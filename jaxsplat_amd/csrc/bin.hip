@@ -571,9 +571,17 @@ __device__ __forceinline__ Buckets block_buckets(uint32_t kmin, uint32_t kmax, u
     return bk;
 }
 
-// Counts in sh.cnt[0..kSegBuckets) -> exclusive starts (the last bucket
-// ends at the run's length).  Each thread scans a run of consecutive buckets.
-// Ends with a barrier.
+// LDS word of bucket b's count.  The scan below has thread t read buckets
+// 8t..8t+7, which would put the lanes t, t+8, ... on one bank; rotating the
+// low three bits by the 64-word row (a permutation inside each 8-word group)
+// gives the 64 lanes of a read 64 different banks.
+__device__ __forceinline__ uint32_t bucket_word(uint32_t b) {
+    return (b & ~7u) | ((b + (b >> 6)) & 7u);
+}
+
+// Counts in sh.cnt[bucket_word(0..kSegBuckets)) -> exclusive starts (the last
+// bucket ends at the run's length).  Each thread scans a run of consecutive
+// buckets.  Ends with a barrier.
 __device__ __forceinline__ void bucket_scan(SegShared& sh) {
     constexpr int per = kSegBuckets / kSegThreads;
     const int tid = threadIdx.x;
@@ -582,13 +590,13 @@ __device__ __forceinline__ void bucket_scan(SegShared& sh) {
     uint32_t sum = 0;
 #pragma unroll
     for (int k = 0; k < per; ++k) {
-        c[k] = sh.cnt[b0 + k];
+        c[k] = sh.cnt[bucket_word(b0 + k)];
         sum += c[k];
     }
     uint32_t run = block_exclusive_scan256(sum, sh.tmp);
 #pragma unroll
     for (int k = 0; k < per; ++k) {
-        sh.cnt[b0 + k] = run;
+        sh.cnt[bucket_word(b0 + k)] = run;
         run += c[k];
     }
     __syncthreads();
@@ -596,7 +604,7 @@ __device__ __forceinline__ void bucket_scan(SegShared& sh) {
 
 // End of bucket bk: the next bucket's start, or n for the last one.
 __device__ __forceinline__ uint32_t bucket_end(const SegShared& sh, uint32_t bk, int n) {
-    return bk + 1 < (uint32_t)kSegBuckets ? sh.cnt[bk + 1] : (uint32_t)n;
+    return bk + 1 < (uint32_t)kSegBuckets ? sh.cnt[bucket_word(bk + 1)] : (uint32_t)n;
 }
 
 // (key, id) order of two entries.
@@ -635,31 +643,35 @@ __device__ void long_run_order(int base, int n, const uint32_t* __restrict__ dke
     for (int i = tid; i < kSegBuckets; i += kSegThreads) sh.cnt[i] = 0u;
     __syncthreads();
     for (int e = tid; e < n; e += kSegThreads)
-        va[base + e] = atomicAdd(&sh.cnt[bk.of(ka[base + e], ids[base + e])], 1u);
+        va[base + e] = atomicAdd(&sh.cnt[bucket_word(bk.of(ka[base + e], ids[base + e]))], 1u);
     __syncthreads();
     bucket_scan(sh);
     for (int e = tid; e < n; e += kSegThreads) {
         const uint32_t k = ka[base + e];
         const uint32_t id = ids[base + e];
-        kv[base + sh.cnt[bk.of(k, id)] + va[base + e]] = make_uint2(k, id);
+        kv[base + sh.cnt[bucket_word(bk.of(k, id))] + va[base + e]] = make_uint2(k, id);
     }
     __syncthreads();                                    // ids are no longer read
     for (int j = tid; j < n; j += kSegThreads) {
         const uint2 me = kv[base + j];
         const uint32_t b = bk.of(me.x, me.y);
-        const uint32_t st = sh.cnt[b];
+        const uint32_t st = sh.cnt[bucket_word(b)];
         const uint32_t rank = bucket_rank(kv + base, st, bucket_end(sh, b, n) - st, me.x, me.y);
         ids[base + st + rank] = me.y;
     }
 }
 
-// One workgroup per tile (see above).
+// One workgroup per tile (see above), the tiles mapped so that each XCD
+// orders a contiguous band of them: neighbouring tiles share Gaussians, whose
+// depth keys are then gathered through one L2.
 __global__ __launch_bounds__(kSegThreads) void tile_depth_sort_kernel(
-    const int2* __restrict__ bins, const uint32_t* __restrict__ dkeys,
+    const int2* __restrict__ bins, int num_bins, const uint32_t* __restrict__ dkeys,
     uint32_t* __restrict__ ids, uint32_t* __restrict__ ka, uint32_t* __restrict__ va,
     uint2* __restrict__ kv_tmp) {
     __shared__ SegShared sh;
-    const int2 b = bins[blockIdx.x];
+    const int t = xcd_tile(blockIdx.x, num_bins);
+    if (t >= num_bins) return;                          // uniform
+    const int2 b = bins[t];
     const int n = b.y - b.x;
     if (n <= 1) return;                                 // uniform
     if (n > kSegCap) {
@@ -693,7 +705,7 @@ __global__ __launch_bounds__(kSegThreads) void tile_depth_sort_kernel(
 #pragma unroll
     for (int r = 0; r < kSegItems; ++r) {
         bk[r] = valid[r] ? bkt.of(key[r], val[r]) : 0u;
-        slot[r] = valid[r] ? atomicAdd(&sh.cnt[bk[r]], 1u) : 0u;
+        slot[r] = valid[r] ? atomicAdd(&sh.cnt[bucket_word(bk[r])], 1u) : 0u;
     }
     __syncthreads();
     bucket_scan(sh);
@@ -703,7 +715,7 @@ __global__ __launch_bounds__(kSegThreads) void tile_depth_sort_kernel(
         st[r] = 0u;
         cnt[r] = 0u;
         if (valid[r]) {
-            st[r] = sh.cnt[bk[r]];
+            st[r] = sh.cnt[bucket_word(bk[r])];
             cnt[r] = bucket_end(sh, bk[r], n) - st[r];
             sh.kv[st[r] + slot[r]] = make_uint2(key[r], val[r]);
         }
@@ -731,7 +743,9 @@ hipError_t tile_depth_order(const int2* bins, int num_bins, const uint32_t* dkey
                             uint32_t* ids, uint32_t* ka, uint32_t* va, uint2* kv_tmp,
                             hipStream_t s) {
     if (num_bins <= 0) return hipSuccess;
-    tile_depth_sort_kernel<<<num_bins, kSegThreads, 0, s>>>(bins, dkeys, ids, ka, va, kv_tmp);
+    const int grid = 8 * ((num_bins + 7) / 8);         // xcd_tile's grid
+    tile_depth_sort_kernel<<<grid, kSegThreads, 0, s>>>(bins, num_bins, dkeys, ids, ka, va,
+                                                       kv_tmp);
     return hipGetLastError();
 }
 

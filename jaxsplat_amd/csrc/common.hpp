@@ -122,6 +122,15 @@ __device__ __forceinline__ T wave_inclusive_sum(T v) {
     return v;
 }
 
+// Tile of workgroup b in a grid of 8 * ceil(num_tiles / 8) workgroups:
+// workgroups are dealt to the 8 XCDs round robin, so XCD b & 7 gets the
+// contiguous band [x * per, (x + 1) * per) of tiles (indexes past num_tiles
+// are the caller's to skip).
+__device__ __forceinline__ int xcd_tile(int b, int num_tiles) {
+    const int per = (num_tiles + 7) / 8;
+    return (b & 7) * per + (b >> 3);
+}
+
 // Mask of the lanes of this wave whose `bits`-bit digit equals this lane's
 // (restricted to lanes in `valid`): one ballot per digit bit.
 __device__ __forceinline__ uint64_t match_digit(uint32_t d, int bits, uint64_t valid) {

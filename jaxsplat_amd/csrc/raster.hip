@@ -80,11 +80,6 @@ __device__ __forceinline__ bool pair_accepted(float s2, float lim2) {
     return __float_as_uint(s2) <= __float_as_uint(lim2);
 }
 
-__device__ __forceinline__ int xcd_tile(int b, int num_tiles) {
-    const int per = (num_tiles + 7) / 8;
-    return (b & 7) * per + (b >> 3);
-}
-
 __device__ __forceinline__ uint64_t uniform64(uint64_t v) {
     const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
     const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
