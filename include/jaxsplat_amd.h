@@ -536,7 +536,9 @@ int jaxsplat_fit_activate(int64_t n, const float* quats_raw, const float* colors
 
 /* v_img = 2 (img - target) / n and loss_sum = sum (img - target)^2 over n
  * floats (the gradient of mean((img - target)^2), single_image.py:26-29);
- * loss_sum is cleared first. */
+ * loss_sum is written (not accumulated) and does not depend on scheduling.
+ * The per-workgroup partial sums live in the library (one set per GPU), so
+ * calls on one GPU must not overlap: issue them in stream order. */
 int jaxsplat_fit_l2(int64_t n, const float* img, const float* target, float* v_img,
                     float* loss_sum, void* stream);
 

@@ -4,8 +4,9 @@
 //
 //  * fit_activate_kernel: quats / |quats|, sigmoid(colors), sigmoid(opacities)
 //    (single_image.py:143-166); means and scales are used raw;
-//  * fit_l2_kernel: v_img = 2 (img - target) / numel and the loss sum
-//    (single_image.py:26-29: mean of squares), one atomic per workgroup;
+//  * fit_l2_kernel + fit_l2_sum_kernel: v_img = 2 (img - target) / numel and
+//    the loss sum (single_image.py:26-29: mean of squares), the workgroups'
+//    partial sums added in workgroup order;
 //  * fit_adam_kernel: per Gaussian, the activation VJP (sigmoid' = s (1 - s);
 //    d(q/|q|) = (I - q^ q^T) / |q|) then Adam with bias correction
 //    (optax.adam defaults b1 = 0.9, b2 = 0.999, eps = 1e-8 added to sqrt(v^));
@@ -14,8 +15,8 @@
 //    turns the step into a no-op, so an overflowing render never updates the
 //    parameters;
 //  * fit_finish_kernel advances the step counter unless the step was skipped.
-// All stores are plain vector stores; the loss accumulator is cleared by a
-// kernel, so the whole step may be captured.
+// All stores are plain vector stores and nothing is cleared by a memset, so
+// the whole step may be captured.
 #include "../../include/jaxsplat_amd.h"
 #include "common.hpp"
 
@@ -38,25 +39,60 @@ __global__ __launch_bounds__(256) void fit_activate_kernel(
     opac[i] = sigmoidf_(opac_raw[i]);
 }
 
+// The loss's per-workgroup partial sums (one set per GPU, in the library):
+// fit_l2_kernel writes them and fit_l2_sum_kernel adds them in workgroup
+// order, so the loss does not depend on scheduling and no atomics on one
+// address serialise the launch's end.  Calls on one GPU are issued in stream
+// order by the callers (the fitting loop, the bench); calls that overlap would
+// share the partials.
+constexpr int kL2Blocks = 2048;
+__device__ float l2_partial[kL2Blocks];
+
+// v_img = scale (img - target) and the workgroup's partial loss sum; float4
+// accesses when the three arrays are 16-byte aligned and n is a multiple of 4.
 __global__ __launch_bounds__(256) void fit_l2_kernel(size_t n, const float* __restrict__ img,
                                                       const float* __restrict__ target,
                                                       float scale, float* __restrict__ v_img,
-                                                      float* __restrict__ loss_sum) {
+                                                      int vec4) {
     __shared__ float s_part[4];
     float acc = 0.f;
-    for (size_t i = blockIdx.x * (size_t)256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
-        const float d = img[i] - target[i];
-        v_img[i] = scale * d;
-        acc += d * d;
+    const size_t stride = (size_t)gridDim.x * 256;
+    if (vec4) {
+        const float4* a4 = reinterpret_cast<const float4*>(img);
+        const float4* t4 = reinterpret_cast<const float4*>(target);
+        float4* v4 = reinterpret_cast<float4*>(v_img);
+        for (size_t i = blockIdx.x * (size_t)256 + threadIdx.x; i < n / 4; i += stride) {
+            const float4 a = a4[i], t = t4[i];
+            const float4 d = make_float4(a.x - t.x, a.y - t.y, a.z - t.z, a.w - t.w);
+            v4[i] = make_float4(scale * d.x, scale * d.y, scale * d.z, scale * d.w);
+            acc += (d.x * d.x + d.y * d.y) + (d.z * d.z + d.w * d.w);
+        }
+    } else {
+        for (size_t i = blockIdx.x * (size_t)256 + threadIdx.x; i < n; i += stride) {
+            const float d = img[i] - target[i];
+            v_img[i] = scale * d;
+            acc += d * d;
+        }
     }
     acc = wave_sum_to_lane63(acc);
     const int lane = lane_id(), wave = threadIdx.x >> 6;
     if (lane == 63) s_part[wave] = acc;
     __syncthreads();
-    if (threadIdx.x == 0) atomicAdd(loss_sum, (s_part[0] + s_part[1]) + (s_part[2] + s_part[3]));
+    if (threadIdx.x == 0) l2_partial[blockIdx.x] = (s_part[0] + s_part[1]) + (s_part[2] + s_part[3]);
 }
 
-__global__ void fit_begin_kernel(float* __restrict__ loss_sum) { *loss_sum = 0.f; }
+// loss_sum = the partials of the first `blocks` workgroups, in order (one
+// workgroup of 256 threads).
+__global__ __launch_bounds__(256) void fit_l2_sum_kernel(int blocks, float* __restrict__ loss_sum) {
+    __shared__ float s_part[4];
+    float acc = 0.f;
+    for (int b = threadIdx.x; b < blocks; b += 256) acc += l2_partial[b];
+    acc = wave_sum_to_lane63(acc);
+    const int lane = lane_id(), wave = threadIdx.x >> 6;
+    if (lane == 63) s_part[wave] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) *loss_sum = (s_part[0] + s_part[1]) + (s_part[2] + s_part[3]);
+}
 
 __device__ __forceinline__ float adam_one(float& p, float& m, float& v, float g, float lr,
                                           float b1, float b2, float eps, float c1, float c2) {
@@ -170,10 +206,15 @@ int jaxsplat_fit_l2(int64_t n, const float* img, const float* target, float* v_i
                     float* loss_sum, void* stream) {
     if (n <= 0 || !img || !target || !v_img || !loss_sum) return JAXSPLAT_ERR_INVALID;
     hipStream_t s = static_cast<hipStream_t>(stream);
-    jsa::fit_begin_kernel<<<1, 1, 0, s>>>(loss_sum);
-    const int64_t blocks = (n + 255) / 256;
-    jsa::fit_l2_kernel<<<(int)(blocks < 2048 ? blocks : 2048), 256, 0, s>>>(
-        (size_t)n, img, target, 2.f / (float)n, v_img, loss_sum);
+    const auto aligned = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; };
+    const int vec4 = (n % 4 == 0 && aligned(img) && aligned(target) && aligned(v_img)) ? 1 : 0;
+    const int64_t items = vec4 ? n / 4 : n;
+    const int64_t want = (items + 255) / 256;
+    const int blocks = (int)(want < jsa::kL2Blocks ? want : jsa::kL2Blocks);
+    jsa::fit_l2_kernel<<<blocks, 256, 0, s>>>((size_t)n, img, target, 2.f / (float)n, v_img,
+                                               vec4);
+    if (hipGetLastError() != hipSuccess) return JAXSPLAT_ERR_HIP;
+    jsa::fit_l2_sum_kernel<<<1, 256, 0, s>>>(blocks, loss_sum);
     return hipGetLastError() == hipSuccess ? JAXSPLAT_OK : JAXSPLAT_ERR_HIP;
 }
 
