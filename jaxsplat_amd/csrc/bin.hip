@@ -661,16 +661,13 @@ __device__ void long_run_order(int base, int n, const uint32_t* __restrict__ dke
     }
 }
 
-// One workgroup per tile (see above), the tiles mapped so that each XCD
-// orders a contiguous band of them: neighbouring tiles share Gaussians, whose
-// depth keys are then gathered through one L2.
-__global__ __launch_bounds__(kSegThreads) void tile_depth_sort_kernel(
-    const int2* __restrict__ bins, int num_bins, const uint32_t* __restrict__ dkeys,
-    uint32_t* __restrict__ ids, uint32_t* __restrict__ ka, uint32_t* __restrict__ va,
-    uint2* __restrict__ kv_tmp) {
-    __shared__ SegShared sh;
-    const int t = xcd_tile(blockIdx.x, num_bins);
-    if (t >= num_bins) return;                          // uniform
+// Orders run t (all lanes of the block; barriers inside).
+__device__ __forceinline__ void order_tile_run(int t, const int2* __restrict__ bins,
+                                               const uint32_t* __restrict__ dkeys,
+                                               uint32_t* __restrict__ ids,
+                                               uint32_t* __restrict__ ka,
+                                               uint32_t* __restrict__ va,
+                                               uint2* __restrict__ kv_tmp, SegShared& sh) {
     const int2 b = bins[t];
     const int n = b.y - b.x;
     if (n <= 1) return;                                 // uniform
@@ -737,15 +734,52 @@ __global__ __launch_bounds__(kSegThreads) void tile_depth_sort_kernel(
     }
 }
 
+// One workgroup per tile (see above), mapped as the raster kernels map the
+// tiles (xcd_tile), so that each XCD orders a band of tiles whose ids its
+// raster workgroups then read through the same L2.
+__global__ __launch_bounds__(kSegThreads) void tile_depth_sort_kernel(
+    const int2* __restrict__ bins, int num_bins, const uint32_t* __restrict__ dkeys,
+    uint32_t* __restrict__ ids, uint32_t* __restrict__ ka, uint32_t* __restrict__ va,
+    uint2* __restrict__ kv_tmp) {
+    __shared__ SegShared sh;
+    const int t = xcd_tile(blockIdx.x, num_bins);
+    if (t < num_bins) order_tile_run(t, bins, dkeys, ids, ka, va, kv_tmp, sh);
+}
+
+// A batch's bins (view << tile_bits | tile: many empty, many short): each
+// workgroup takes the runs blockIdx.x, + gridDim.x, ..., so an empty one costs
+// a load rather than a workgroup.  At least 6 waves per SIMD (78 VGPRs; the
+// loop took 87 unbounded).
+__global__ __launch_bounds__(kSegThreads, 6) void tile_depth_sort_loop_kernel(
+    const int2* __restrict__ bins, int num_bins, const uint32_t* __restrict__ dkeys,
+    uint32_t* __restrict__ ids, uint32_t* __restrict__ ka, uint32_t* __restrict__ va,
+    uint2* __restrict__ kv_tmp) {
+    __shared__ SegShared sh;
+    for (int t = blockIdx.x; t < num_bins; t += gridDim.x) {
+        order_tile_run(t, bins, dkeys, ids, ka, va, kv_tmp, sh);
+        __syncthreads();                                // LDS is reused by the next run
+    }
+}
+
 // The per-tile depth order of every run of bins[0..num_bins).  ka, va: free
 // buffers of the intersection capacity; kv_tmp: a free one of twice that.
-hipError_t tile_depth_order(const int2* bins, int num_bins, const uint32_t* dkeys,
+// xcd: the bins are one view's tiles in the raster kernels' numbering (the
+// batch's bins, view << tile_bits | tile, are not: a band per XCD would also
+// be uneven there).
+hipError_t tile_depth_order(const int2* bins, int num_bins, bool xcd, const uint32_t* dkeys,
                             uint32_t* ids, uint32_t* ka, uint32_t* va, uint2* kv_tmp,
                             hipStream_t s) {
     if (num_bins <= 0) return hipSuccess;
-    const int grid = 8 * ((num_bins + 7) / 8);         // xcd_tile's grid
-    tile_depth_sort_kernel<<<grid, kSegThreads, 0, s>>>(bins, num_bins, dkeys, ids, ka, va,
-                                                       kv_tmp);
+    // xcd_tile's grid, or two runs per workgroup (A/B at 6 waves per SIMD:
+    // multi32 +7%, orbit64 -1% against one run per workgroup; 4 or 8 runs
+    // cost orbit64's long runs more)
+    const int grid = xcd ? 8 * ((num_bins + 7) / 8) : (num_bins + 1) / 2;
+    if (xcd)
+        tile_depth_sort_kernel<<<grid, kSegThreads, 0, s>>>(bins, num_bins, dkeys, ids, ka, va,
+                                                           kv_tmp);
+    else
+        tile_depth_sort_loop_kernel<<<grid, kSegThreads, 0, s>>>(bins, num_bins, dkeys, ids, ka,
+                                                                va, kv_tmp);
     return hipGetLastError();
 }
 
@@ -982,8 +1016,9 @@ hipError_t bin_emit(const BinPlan& p, const Camera& c, int num_isect_host,
     stage_mark(kMarkBinned, s);
     // per-tile depth order; the emission's pair buffers and the sort's
     // ping-pong buffers (adjacent) are free again and serve the longer runs
-    return tile_depth_order(reinterpret_cast<const int2*>(tile_bins), p.num_tiles, dkeys, ids,
-                            U(L.tiles), U(L.gids), reinterpret_cast<uint2*>(b + L.ttmp_k), s);
+    return tile_depth_order(reinterpret_cast<const int2*>(tile_bins), p.num_tiles, true, dkeys,
+                            ids, U(L.tiles), U(L.gids), reinterpret_cast<uint2*>(b + L.ttmp_k),
+                            s);
 }
 
 // Binning of given tile rectangles with their inclusive counts (the render
@@ -1038,7 +1073,7 @@ hipError_t bin_rects(const BinPlan& p, const Camera& c, const float* depths,
                                                    reinterpret_cast<int2*>(tile_bins), err);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     stage_mark(kMarkBinned, s);
-    return tile_depth_order(reinterpret_cast<const int2*>(tile_bins), p.num_tiles,
+    return tile_depth_order(reinterpret_cast<const int2*>(tile_bins), p.num_tiles, true,
                             reinterpret_cast<const uint32_t*>(depths), ids, U(L.tiles),
                             U(L.gids), reinterpret_cast<uint2*>(b + L.ttmp_k), s);
 }
@@ -1159,9 +1194,9 @@ hipError_t batch_bin(const BatchBinPlan& p, const ViewRec* views, const uint2* i
     tile_bins_kernel<<<nb > 0 ? nb : 1, 256, 0, s>>>(U(L.keys_sorted), 0, count, p.num_bins,
                                                    reinterpret_cast<int2*>(tile_bins), err);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    return tile_depth_order(reinterpret_cast<const int2*>(tile_bins), p.num_bins, depth_keys,
-                            ids, U(L.keys), U(L.vals), reinterpret_cast<uint2*>(b + L.ttmp_k),
-                            s);
+    return tile_depth_order(reinterpret_cast<const int2*>(tile_bins), p.num_bins, false,
+                            depth_keys, ids, U(L.keys), U(L.vals),
+                            reinterpret_cast<uint2*>(b + L.ttmp_k), s);
 }
 
 }  // namespace jsa
