@@ -374,7 +374,7 @@ __global__ __launch_bounds__(256) void project_fwd_kernel(
         if (lane == 63) s_wave_tot[k][wave] = (uint32_t)incl[k];
     }
     __syncthreads();
-    // partition-local inclusive sums; project_scan_fixup_kernel adds the
+    // partition-local inclusive sums; project_scan_add_kernel adds the
     // partition's offset
     uint32_t total = 0;
 #pragma unroll
@@ -395,7 +395,7 @@ __global__ __launch_bounds__(256) void project_fwd_kernel(
 
 // Opacity-limited tile rectangles (opacity_tile_rect) of the projected
 // Gaussians, their tile counts and the partition-local inclusive sums of the
-// counts (finished by project_scan_kernel / project_scan_fixup_kernel), laid
+// counts (finished by project_scan_add_kernel), laid
 // out like project_fwd_kernel's.  Gaussians with radius 0 get empty rectangles.
 __global__ __launch_bounds__(256) void tile_rects_kernel(
     Camera c, const float2* __restrict__ xys, const int* __restrict__ radii,
@@ -445,36 +445,26 @@ __global__ __launch_bounds__(256) void tile_rects_kernel(
     if (tid == 0) part_tot[part] = total;
 }
 
-// Exclusive scan of the partition totals, one workgroup of 1024 threads
-// (thread t scans a contiguous run of partitions).
-__global__ __launch_bounds__(1024) void project_scan_kernel(const uint32_t* __restrict__ part_tot,
-                                                            int nparts,
-                                                            uint32_t* __restrict__ part_off) {
-    __shared__ uint32_t s_w[16];
+// cum_tiles_hit += its partition's offset, one workgroup per partition
+// p >= 1: the workgroup sums the totals of partitions 0..p-1 itself (at most a
+// few KB from L2 at 1M Gaussians), so no separate scan kernel runs between
+// the projection and this one.
+__global__ __launch_bounds__(256) void project_scan_add_kernel(
+    int n, const uint32_t* __restrict__ part_tot, int* __restrict__ cum_tiles_hit) {
+    __shared__ uint32_t s_w[4];
     const int tid = threadIdx.x;
-    const int lane = lane_id();
-    const int wave = tid >> 6;
-    const int per = (nparts + 1023) / 1024;
-    const int b0 = min(tid * per, nparts), b1 = min(b0 + per, nparts);
+    const int p = blockIdx.x + 1;
     uint32_t sum = 0;
-    for (int i = b0; i < b1; ++i) sum += part_tot[i];
-    const uint32_t incl = wave_inclusive_sum(sum);
-    if (lane == 63) s_w[wave] = incl;
+    for (int q = tid; q < p; q += 256) sum += part_tot[q];
+    sum = wave_inclusive_sum(sum);
+    if (lane_id() == 63) s_w[tid >> 6] = sum;
     __syncthreads();
-    uint32_t run = incl - sum;
-    for (int w = 0; w < wave; ++w) run += s_w[w];
-    for (int i = b0; i < b1; ++i) {
-        part_off[i] = run;
-        run += part_tot[i];
+    const int off = (int)((s_w[0] + s_w[1]) + (s_w[2] + s_w[3]));
+#pragma unroll
+    for (int k = 0; k < kProjPer; ++k) {
+        const int idx = p * kProjPart + k * 256 + tid;
+        if (idx < n) cum_tiles_hit[idx] += off;
     }
-}
-
-// cum_tiles_hit += its partition's offset (partition 0 has none).
-__global__ __launch_bounds__(256) void project_scan_fixup_kernel(
-    int n, const uint32_t* __restrict__ part_off, int* __restrict__ cum_tiles_hit) {
-    const int idx = kProjPart + blockIdx.x * 256 + threadIdx.x;
-    if (idx >= n) return;
-    cum_tiles_hit[idx] += (int)part_off[idx / kProjPart];
 }
 
 // helpers.h:155-192 with vR[row][col]; the reference indexes v_R[col][row].
@@ -884,18 +874,13 @@ hipError_t project_fwd(const Camera& c, const float* means3d, const float* scale
         return hipErrorInvalidValue;
     const int parts = (c.num_points + kProjPart - 1) / kProjPart;
     uint32_t* part_tot = static_cast<uint32_t*>(scratch);
-    uint32_t* part_off = part_tot + parts;
     project_fwd_kernel<<<parts, 256, 0, s>>>(c, means3d, scales, quats, viewmat, cov3d, xys,
                                              depths, radii, conics, compensation, num_tiles_hit,
                                              cum_tiles_hit, opacities, rects, part_tot, colors,
                                              nch, splats);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || parts == 1) return e;
-    project_scan_kernel<<<1, 1024, 0, s>>>(part_tot, parts, part_off);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    const int rest = c.num_points - kProjPart;
-    project_scan_fixup_kernel<<<(rest + 255) / 256, 256, 0, s>>>(c.num_points, part_off,
-                                                                 cum_tiles_hit);
+    project_scan_add_kernel<<<parts - 1, 256, 0, s>>>(c.num_points, part_tot, cum_tiles_hit);
     return hipGetLastError();
 }
 
@@ -905,17 +890,12 @@ hipError_t tile_rects(const Camera& c, const float* xys, const int* radii, const
     if (c.num_points <= 0) return hipSuccess;
     const int parts = (c.num_points + kProjPart - 1) / kProjPart;
     uint32_t* part_tot = static_cast<uint32_t*>(scratch);
-    uint32_t* part_off = part_tot + parts;
     tile_rects_kernel<<<parts, 256, 0, s>>>(c, reinterpret_cast<const float2*>(xys), radii,
                                             conics, opacities, rects, num_tiles_hit,
                                             cum_tiles_hit, part_tot);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || parts == 1) return e;
-    project_scan_kernel<<<1, 1024, 0, s>>>(part_tot, parts, part_off);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    const int rest = c.num_points - kProjPart;
-    project_scan_fixup_kernel<<<(rest + 255) / 256, 256, 0, s>>>(c.num_points, part_off,
-                                                                 cum_tiles_hit);
+    project_scan_add_kernel<<<parts - 1, 256, 0, s>>>(c.num_points, part_tot, cum_tiles_hit);
     return hipGetLastError();
 }
 
