@@ -31,6 +31,11 @@ enum Status : int {
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
 
 __device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+// The lane's bit of a wave-uniform mask as a lane predicate (the mask itself
+// serves as the select mask: no per-lane shift or compare).
+__device__ __forceinline__ bool lane_in(uint64_t m) {
+    return __builtin_amdgcn_inverse_ballot_w64(m);
+}
 
 // DPP controls (GFX9 encoding).
 constexpr int kDppQuadPerm1032 = 0xB1;   // quad_perm [1,0,3,2]

@@ -306,21 +306,24 @@ __device__ __forceinline__ void raster_fwd_tile(
     // wave-uniform, so that the batch bounds and sorted indices stay scalar
     const int2 rb = bins[vr.bin_off + tile];
     const int2 range = make_int2(uniform_i(rb.x), uniform_i(rb.y));
+    // A pixel is done once it has terminated (or from the start when it lies
+    // outside the image).  The done set is a wave-uniform lane mask: the
+    // branches below test ballots of plain compares against it in scalar
+    // registers, and a done pixel takes alpha = 0, so it blends nothing, keeps
+    // T and never terminates again.  The common pair, on which no blending
+    // pixel terminates, then updates every lane unconditionally.
     float T = 1.f;
     float cr = 0.f, cg = 0.f, cb = 0.f;
     int cur = 0;
-    // 0 while the pixel blends, the float sign bit once it is done: OR-ed
-    // into the acceptance compare, so that done pixels accept nothing and
-    // every ballot below is taken on a plain compare
-    uint32_t dbits = inside ? 0u : 0x80000000u;
+    uint64_t dm = ballot(!inside);
 
     for (int bstart = range.x; bstart < range.y; bstart += DEPTH) {
-        const uint64_t dmask = ballot(dbits != 0u);
-        if (lane == 0) s_cnt[wave] = __popcll(dmask);
+        if (lane == 0) s_cnt[wave] = __popcll(dm);
         __syncthreads();
         int ndone = 0;
         for (int w = 0; w < nwaves; ++w) ndone += s_cnt[w];
-        if (ndone == nthr) break;
+        // uniform exit, so that bstart stays a scalar
+        if (uniform_i(ndone) == nthr) break;
         // stage entries e = e0 + tid of the batch (several per lane when DEPTH > nthr)
         for (int e0 = 0; e0 < DEPTH; e0 += nthr) {
             const int e = e0 + tid;
@@ -344,15 +347,17 @@ __device__ __forceinline__ void raster_fwd_tile(
             }
         }
         __syncthreads();
-        // Branch-free per pixel: the decisions are lane masks, the colour is
-        // read and blended under one uniform branch when some lane adds, and
-        // the wave's exit test runs only after some lane terminated (keeps the
-        // scalar unit from issuing exec-mask updates for every evaluation).
+        // Branch-free per pixel: the decisions are lane masks and every branch
+        // is uniform (no exec-mask updates per evaluation).  A pair that no
+        // blending pixel accepts ends after the acceptance ballot; otherwise
+        // every lane blends with its alpha (0 where not accepted, which keeps
+        // T and the colour exactly), and only a pair that terminates some
+        // pixel takes the selects that keep it from blending.
         // Software-pipelined over the set bits of each mask word: the next
         // entry's LDS reads are issued before the current pair is evaluated,
         // which takes their latency out of the dependent chain per pair.  Two
         // register sets alternate (unrolled by two), so nothing is copied.
-        bool live = ballot(dbits == 0u) != 0;
+        bool live = dm != ~0ull;
         for (int q = 0; q < NW && live; ++q) {
             uint64_t m = uniform64(s_mask[wave][q]);
             // branch-free: with no bit left it re-reads entry t (a valid index),
@@ -370,28 +375,31 @@ __device__ __forceinline__ void raster_fwd_tile(
             auto eval = [&](int t, const float4& a, const float4& b, const float4& col) -> bool {
                 const float dx = a.x - px, dy = a.y - py;
                 const float s2 = sigma2(b, dx, dy);
+                const bool ok = pair_accepted(s2, b.w);
+                const uint64_t take = ballot(ok) & ~dm;     // accepted by a blending pixel
+                if (take == 0) return false;
                 const float alpha = fminf(0.999f, a.z * __builtin_amdgcn_exp2f(-s2));
-                // pair_accepted(s2, lim2) for a blending pixel; lim2 >= 0, so
-                // the sign bit of a done pixel fails the compare
-                const bool ok = (__float_as_uint(s2) | dbits) <= __float_as_uint(b.w);
-                const float next_T = fmaf(-T, alpha, T);      // T (1 - alpha)
-                const bool term = next_T <= kTEps;
-                const uint64_t okm = ballot(ok), tm = ballot(term);
-                const bool fin = ok && term;
-                const bool add = ok && !term;
-                if (okm & ~tm) {
-                    const float vis = add ? alpha * T : 0.f;
-                    cr += col.x * vis;
-                    cg += col.y * vis;
-                    cb += col.z * vis;
-                    T = add ? next_T : T;
-                    cur = add ? bstart + t : cur;
+                const float al = lane_in(take) ? alpha : 0.f;
+                float vis = al * T;
+                // T (1 - alpha); exactly T where al = 0
+                float next_T = fmaf(-T, al, T);
+                // a pixel with al = 0 keeps its T > 1e-4: fin is within take
+                const uint64_t fin = ballot(next_T <= kTEps);
+                uint64_t add = take;
+                if (fin != 0) {
+                    // a terminating pixel keeps T and adds nothing
+                    const bool f = lane_in(fin);
+                    vis = f ? 0.f : vis;
+                    next_T = f ? T : next_T;
+                    add = take & ~fin;
+                    dm |= fin;
                 }
-                if (okm & tm) {
-                    dbits = fin ? 0x80000000u : dbits;
-                    if (ballot(dbits == 0u) == 0) return true;
-                }
-                return false;
+                cr += col.x * vis;
+                cg += col.y * vis;
+                cb += col.z * vis;
+                T = next_T;
+                cur = lane_in(add) ? bstart + t : cur;
+                return dm == ~0ull;
             };
             int t0 = 0, t1 = 0;
             float4 a0, b0, c0, a1, b1, c1;
