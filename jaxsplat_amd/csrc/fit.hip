@@ -50,28 +50,56 @@ __device__ float l2_partial[kL2Blocks];
 
 // v_img = scale (img - target) and the workgroup's partial loss sum; float4
 // accesses when the three arrays are 16-byte aligned and n is a multiple of 4.
+// Each thread takes kL2Unroll items kL2Unroll * 256 apart per pass and issues
+// all their loads before the first store, so a pass keeps 2 kL2Unroll loads
+// of every thread in flight (a 1080p image is one pass of the full grid).
+constexpr int kL2Unroll = 4;
 __global__ __launch_bounds__(256) void fit_l2_kernel(size_t n, const float* __restrict__ img,
                                                       const float* __restrict__ target,
                                                       float scale, float* __restrict__ v_img,
                                                       int vec4) {
     __shared__ float s_part[4];
     float acc = 0.f;
-    const size_t stride = (size_t)gridDim.x * 256;
+    const size_t stride = (size_t)gridDim.x * 256 * kL2Unroll;
+    const size_t first = blockIdx.x * (size_t)(256 * kL2Unroll) + threadIdx.x;
     if (vec4) {
         const float4* a4 = reinterpret_cast<const float4*>(img);
         const float4* t4 = reinterpret_cast<const float4*>(target);
         float4* v4 = reinterpret_cast<float4*>(v_img);
-        for (size_t i = blockIdx.x * (size_t)256 + threadIdx.x; i < n / 4; i += stride) {
-            const float4 a = a4[i], t = t4[i];
-            const float4 d = make_float4(a.x - t.x, a.y - t.y, a.z - t.z, a.w - t.w);
-            v4[i] = make_float4(scale * d.x, scale * d.y, scale * d.z, scale * d.w);
-            acc += (d.x * d.x + d.y * d.y) + (d.z * d.z + d.w * d.w);
+        const size_t m = n / 4;
+        for (size_t i0 = first; i0 < m; i0 += stride) {
+            float4 a[kL2Unroll], t[kL2Unroll];
+#pragma unroll
+            for (int k = 0; k < kL2Unroll; ++k) {
+                const size_t i = i0 + (size_t)k * 256;
+                a[k] = i < m ? a4[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+                t[k] = i < m ? t4[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+#pragma unroll
+            for (int k = 0; k < kL2Unroll; ++k) {
+                const size_t i = i0 + (size_t)k * 256;
+                const float4 d = make_float4(a[k].x - t[k].x, a[k].y - t[k].y, a[k].z - t[k].z,
+                                             a[k].w - t[k].w);
+                if (i < m) v4[i] = make_float4(scale * d.x, scale * d.y, scale * d.z, scale * d.w);
+                acc += (d.x * d.x + d.y * d.y) + (d.z * d.z + d.w * d.w);
+            }
         }
     } else {
-        for (size_t i = blockIdx.x * (size_t)256 + threadIdx.x; i < n; i += stride) {
-            const float d = img[i] - target[i];
-            v_img[i] = scale * d;
-            acc += d * d;
+        for (size_t i0 = first; i0 < n; i0 += stride) {
+            float a[kL2Unroll], t[kL2Unroll];
+#pragma unroll
+            for (int k = 0; k < kL2Unroll; ++k) {
+                const size_t i = i0 + (size_t)k * 256;
+                a[k] = i < n ? img[i] : 0.f;
+                t[k] = i < n ? target[i] : 0.f;
+            }
+#pragma unroll
+            for (int k = 0; k < kL2Unroll; ++k) {
+                const size_t i = i0 + (size_t)k * 256;
+                const float d = a[k] - t[k];
+                if (i < n) v_img[i] = scale * d;
+                acc += d * d;
+            }
         }
     }
     acc = wave_sum_to_lane63(acc);
@@ -209,7 +237,8 @@ int jaxsplat_fit_l2(int64_t n, const float* img, const float* target, float* v_i
     const auto aligned = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; };
     const int vec4 = (n % 4 == 0 && aligned(img) && aligned(target) && aligned(v_img)) ? 1 : 0;
     const int64_t items = vec4 ? n / 4 : n;
-    const int64_t want = (items + 255) / 256;
+    const int64_t per_block = 256 * jsa::kL2Unroll;
+    const int64_t want = (items + per_block - 1) / per_block;
     const int blocks = (int)(want < jsa::kL2Blocks ? want : jsa::kL2Blocks);
     jsa::fit_l2_kernel<<<blocks, 256, 0, s>>>((size_t)n, img, target, 2.f / (float)n, v_img,
                                                vec4);

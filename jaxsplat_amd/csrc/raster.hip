@@ -358,21 +358,26 @@ __device__ __forceinline__ void raster_fwd_tile(
         // which takes their latency out of the dependent chain per pair.  Two
         // register sets alternate (unrolled by two), so nothing is copied.
         bool live = dm != ~0ull;
+        int cur_off = -1;          // 16 x the batch entry last blended, -1 for none
         for (int q = 0; q < NW && live; ++q) {
             uint64_t m = uniform64(s_mask[wave][q]);
             // branch-free: with no bit left it re-reads entry t (a valid index),
-            // so the LDS waits before each evaluation stay exact
-            auto fetch = [&](int& t, float4& a, float4& b, float4& c) -> bool {
+            // so the LDS waits before each evaluation stay exact.  The entry's
+            // byte offset 16 t is moved to a vector register once: it is the
+            // LDS address of the three reads and the value that marks the
+            // pixel's last blended entry (cur_off).
+            auto fetch = [&](int& t, int& toff, float4& a, float4& b, float4& c) -> bool {
                 const bool h = m != 0;
                 t = h ? q * 64 + __ffsll((unsigned long long)m) - 1 : t;
                 m &= m - 1;
-                a = s_a[t];
-                b = s_b[t];
-                c = s_c[t];
+                __asm__("v_mov_b32 %0, %1" : "=v"(toff) : "s"(t * (int)sizeof(float4)));
+                a = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(s_a) + toff);
+                b = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(s_b) + toff);
+                c = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(s_c) + toff);
                 return h;
             };
             // one pair; true when every pixel of the wave is done
-            auto eval = [&](int t, const float4& a, const float4& b, const float4& col) -> bool {
+            auto eval = [&](int toff, const float4& a, const float4& b, const float4& col) -> bool {
                 const float dx = a.x - px, dy = a.y - py;
                 const float s2 = sigma2(b, dx, dy);
                 const bool ok = pair_accepted(s2, b.w);
@@ -398,26 +403,28 @@ __device__ __forceinline__ void raster_fwd_tile(
                 cg += col.y * vis;
                 cb += col.z * vis;
                 T = next_T;
-                cur = lane_in(add) ? bstart + t : cur;
+                cur_off = lane_in(add) ? toff : cur_off;
                 return dm == ~0ull;
             };
-            int t0 = 0, t1 = 0;
+            int t0 = 0, t1 = 0, o0, o1;
             float4 a0, b0, c0, a1, b1, c1;
-            bool h0 = fetch(t0, a0, b0, c0);
+            bool h0 = fetch(t0, o0, a0, b0, c0);
             while (h0) {
-                const bool h1 = fetch(t1, a1, b1, c1);
-                if (eval(t0, a0, b0, c0)) {
+                const bool h1 = fetch(t1, o1, a1, b1, c1);
+                if (eval(o0, a0, b0, c0)) {
                     live = false;
                     break;
                 }
                 if (!h1) break;
-                h0 = fetch(t0, a0, b0, c0);
-                if (eval(t1, a1, b1, c1)) {
+                h0 = fetch(t0, o0, a0, b0, c0);
+                if (eval(o1, a1, b1, c1)) {
                     live = false;
                     break;
                 }
             }
         }
+        // the batch's last blended entry, if any, as a sorted index
+        cur = cur_off >= 0 ? bstart + (cur_off >> 4) : cur;
         __syncthreads();
     }
     if (inside) {
