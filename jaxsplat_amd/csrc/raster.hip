@@ -271,7 +271,7 @@ __device__ __forceinline__ void raster_fwd_tile(
     const float* __restrict__ bgs, float* __restrict__ final_Ts, int* __restrict__ final_idx,
     float* __restrict__ out_img) {
     constexpr int NW = DEPTH / 64;            // 64-entry words per batch
-    __shared__ float4 s_a[DEPTH];             // x, y, opacity, id
+    __shared__ float4 s_a[DEPTH];             // tile-centred x, y, opacity, id
     __shared__ float4 s_b[DEPTH];             // base-2 conic a', b', c', alpha_limit2(opacity)
     __shared__ float4 s_c[DEPTH];             // colour
     __shared__ uint64_t s_mask[4][NW];        // [group][64-entry word]
@@ -295,8 +295,11 @@ __device__ __forceinline__ void raster_fwd_tile(
     const int i = y_org + rc.x;
     const int j = x_org + rc.y;
     const bool inside = pixel && i < vr.img_h && j < vr.img_w;
-    const float px = (float)j + 0.5f;
-    const float py = (float)i + 0.5f;
+    // pixel centre and staged Gaussian positions relative to the tile's centre
+    // pixel corner (x_org + bw / 2, y_org + bw / 2), as in the backward
+    const int cx0 = x_org + bw / 2, cy0 = y_org + bw / 2;
+    const float px = (float)(j - cx0) + 0.5f;
+    const float py = (float)(i - cy0) + 0.5f;
     if (tid < nwaves) {
         bool ok;
         s_rect[tid] = group_rect(tid * 64, bw, x_org, y_org, vr.img_w, vr.img_h, ok);
@@ -333,7 +336,8 @@ __device__ __forceinline__ void raster_fwd_tile(
             if ((unsigned)(it - vr.item_off) < (unsigned)vr.n) {    // an item of this view
                 const SplatIn sp = stage_splat(rg, it, it + gshift, xys, conics, colors, opac);
                 const CullInfo ci = cull_info(sp.x, sp.y, sp.ca, sp.cb, sp.cc, sp.lim2);
-                s_a[e] = make_float4(sp.x, sp.y, sp.o, __int_as_float(it));
+                s_a[e] = make_float4(sp.x - (float)cx0, sp.y - (float)cy0, sp.o,
+                                     __int_as_float(it));
                 s_b[e] = make_float4(sp.ca * kHalfLog2e, sp.cb * kLog2e, sp.cc * kHalfLog2e,
                                      sp.lim2);
                 s_c[e] = sp.col;
@@ -491,7 +495,9 @@ __global__ __launch_bounds__(256) void raster_fwd_queue_kernel(
 // Backward: NSLOT pixels per lane (slots); 64-pixel groups = (slot, wave).
 // ---------------------------------------------------------------------------
 struct PixState {
-    float px, py, T_final, T, vo_r, vo_g, vo_b;
+    // px, py: pixel centre relative to the tile's centre corner (exact);
+    // pxx, pxy, pyy: its products, the weights of the moment sums
+    float px, py, pxx, pxy, pyy, T_final, T, vo_r, vo_g, vo_b;
     // (sum over the Gaussians behind of colour * alpha * T, dotted with v_out)
     // - T_final (v_out_alpha - background . v_out): the only form the adjoint uses
     float bv;
@@ -547,6 +553,9 @@ __device__ __forceinline__ float row_sums_split(float* v, int lane) {
 //   g[6..7] = sum vs dx, vs dy                       (v_xy = conic * (sum vs d))
 //   [g[8..9] = sum |v_xy per pixel|]                 (v_xy_abs)
 //   g[NV-1] = sum vs                                 (v_opacity = -sum vs / o)
+// where without v_xy_abs g[3..7] hold the moments sum vs px^2, vs px py,
+// vs py^2, vs px, vs py of the tile-centred pixel positions instead (one
+// multiply-add each; finish_grads expands (gx - px)^2 etc. once per entry)
 // with vs = dL/dsigma = -o * vis * v_alpha (times the clamp's zero slope in
 // consistent mode).  The returned mask of lanes with a valid pair comes from
 // ballots taken inside, so every call must run in wave-uniform control flow
@@ -580,15 +589,24 @@ __device__ __forceinline__ uint64_t bwd_pixel(PixState& s, float dx, float dy, f
     s.bv += fac * cv;
     // the clamp's zero slope applies in consistent mode only
     const float vs = (CONSISTENT && raw >= kClamp) ? 0.f : -rv * v_alpha;
-    const float p = vs * dx, q = vs * dy;
-    JSA_ACC(3, p * dx);
-    JSA_ACC(4, p * dy);
-    JSA_ACC(5, q * dy);
-    JSA_ACC(6, p);
-    JSA_ACC(7, q);
-    if (NV == 11) {     // log2(e) |conic (p, q)|, rescaled by finish_grads
+    if (NV == 11) {
+        const float p = vs * dx, q = vs * dy;
+        JSA_ACC(3, p * dx);
+        JSA_ACC(4, p * dy);
+        JSA_ACC(5, q * dy);
+        JSA_ACC(6, p);
+        JSA_ACC(7, q);
+        // log2(e) |conic (p, q)|, rescaled by finish_grads
         JSA_ACC(8, fabsf(2.f * b.x * p + b.y * q));
         JSA_ACC(9, fabsf(b.y * p + 2.f * b.z * q));
+    } else {
+        // moments of vs over the pixel positions; finish_grads turns them
+        // into the sums over the offsets (dx, dy) = Gaussian - pixel
+        JSA_ACC(3, vs * s.pxx);
+        JSA_ACC(4, vs * s.pxy);
+        JSA_ACC(5, vs * s.pyy);
+        JSA_ACC(6, vs * s.px);
+        JSA_ACC(7, vs * s.py);
     }
     JSA_ACC(NV - 1, vs);
 #undef JSA_ACC
@@ -601,8 +619,22 @@ __device__ __forceinline__ uint64_t bwd_pixel(PixState& s, float dx, float dy, f
 // v_xy 2, [v_xy_abs 2], v_opacity).  o > 0 whenever a pair was valid
 // (alpha >= 1/255).
 template <int NV>
-__device__ __forceinline__ void finish_grads(float* r, float4 conic2, float o) {
+__device__ __forceinline__ void finish_grads(float* r, float4 conic2, float o, float gx,
+                                             float gy) {
     constexpr float kInvLog2e = 1.f / kLog2e;
+    if (NV == 9) {
+        // moments -> offset sums, with (gx, gy) the tile-centred Gaussian
+        // position: sum vs (gx - px) = gx S - Sx, sum vs (gx - px)^2 =
+        // gx (gx S - 2 Sx) + Sxx, sum vs (gx - px)(gy - py) =
+        // gx (gy S - Sy) - gy Sx + Sxy
+        const float S = r[NV - 1], Sx = r[6], Sy = r[7];
+        const float ex = fmaf(gx, S, -Sx), ey = fmaf(gy, S, -Sy);
+        r[3] = fmaf(gx, ex - Sx, r[3]);
+        r[4] = fmaf(gx, ey, fmaf(-gy, Sx, r[4]));
+        r[5] = fmaf(gy, ey - Sy, r[5]);
+        r[6] = ex;
+        r[7] = ey;
+    }
     const float ca = conic2.x * (2.f * kInvLog2e), cb = conic2.y * kInvLog2e,
                 cc = conic2.z * (2.f * kInvLog2e);
     r[3] *= 0.5f;
@@ -650,7 +682,7 @@ __device__ __forceinline__ void raster_bwd_tile(
     float* __restrict__ v_conic, const StagedOut& st) {
     constexpr int NV = ABS ? 11 : 9;     // rgb3, conic3, xy2, [xy_abs2], opacity
     constexpr int NW = DEPTH / 64;       // 64-entry words per batch
-    __shared__ float4 s_a[DEPTH];        // x, y, opacity, id
+    __shared__ float4 s_a[DEPTH];        // tile-centred x, y, opacity, id
     __shared__ float4 s_b[DEPTH];        // base-2 conic a', b', c', staged slot
     __shared__ float4 s_c[DEPTH];        // colour, alpha_limit2(opacity)
     // one wave per workgroup (bwd_threads: NSLOT pixels per lane cover the
@@ -677,6 +709,8 @@ __device__ __forceinline__ void raster_bwd_tile(
     const bool quad = bw == 16;
     const int ty = tile / vr.tiles_x, tx = tile - ty * vr.tiles_x;
     const int x_org = tx * bw, y_org = ty * bw;
+    // positions relative to the tile's centre corner, as in the forward
+    const int cx0 = x_org + bw / 2, cy0 = y_org + bw / 2;
     const float4 bg4 = load_color(rg, bgs, vr.view);
     const float bgr = bg4.x, bgg = bg4.y, bgb = bg4.z;
 
@@ -691,8 +725,11 @@ __device__ __forceinline__ void raster_bwd_tile(
         const int j = x_org + rc.y;
         PixState& s = ps[sl];
         s.inside = pixel && i < vr.img_h && j < vr.img_w;
-        s.px = (float)j + 0.5f;
-        s.py = (float)i + 0.5f;
+        s.px = (float)(j - cx0) + 0.5f;
+        s.py = (float)(i - cy0) + 0.5f;
+        s.pxx = s.px * s.px;
+        s.pxy = s.px * s.py;
+        s.pyy = s.py * s.py;
         const size_t pix = s.inside ? (size_t)vr.pix_off + (size_t)i * vr.img_w + j : 0;
         s.T_final = s.inside ? final_Ts[pix] : 0.f;
         s.T = s.T_final;
@@ -740,7 +777,8 @@ __device__ __forceinline__ void raster_bwd_tile(
             const int it = (e < DEPTH && idx >= range.x) ? gids[idx] : -1;
             if ((unsigned)(it - vr.item_off) < (unsigned)vr.n) {    // an item of this view
                 const SplatIn sp = stage_splat(rg, it, it + gshift, xys, conics, colors, opac);
-                s_a[e] = make_float4(sp.x, sp.y, sp.o, __int_as_float(it));
+                s_a[e] = make_float4(sp.x - (float)cx0, sp.y - (float)cy0, sp.o,
+                                     __int_as_float(it));
                 int slot = -1;
                 if (STAGED) {
                     int x0, x1, y0, y1;
@@ -872,7 +910,7 @@ __device__ __forceinline__ void raster_bwd_tile(
                 s_g[e][v] = 0.f;
             }
             s_flag[e] = 0;
-            finish_grads<NV>(r, s_b[e], s_a[e].z);
+            finish_grads<NV>(r, s_b[e], s_a[e].z, s_a[e].x, s_a[e].y);
             if (STAGED) {
                 const int slot = __float_as_int(s_b[e].w);
                 if (slot < 0) continue;
