@@ -252,20 +252,34 @@ __device__ __forceinline__ uint2 opacity_tile_rect(int bw, int tiles_x, int tile
 // c, xy 2), (opacity, xy_abs 2, -).  bits holds one bit per slot, set when
 // the slot's record was written.  The workspace reserves 48 bytes per slot.
 template <bool ABS>
-__device__ __forceinline__ void load_record(const float* __restrict__ rec, size_t k, float4& a0,
-                                            float4& a1, float4& a2) {
+__device__ __forceinline__ const float* record_at(const float* __restrict__ rec, size_t k) {
+    return rec + (ABS ? 12 : 9) * k;
+}
+
+template <bool ABS>
+__device__ __forceinline__ void load_record_from(const float* __restrict__ q, float4& a0,
+                                                 float4& a1, float4& a2) {
     if (ABS) {
-        const float4* q = reinterpret_cast<const float4*>(rec) + 3 * k;
-        a0 = q[0];
-        a1 = q[1];
-        a2 = q[2];
+        const float4* q4 = reinterpret_cast<const float4*>(q);
+        a0 = q4[0];
+        a1 = q4[1];
+        a2 = q4[2];
     } else {
-        const float* q = rec + 9 * k;
         a0 = make_float4(q[0], q[1], q[2], q[3]);
         a1 = make_float4(q[4], q[5], q[6], q[7]);
         a2 = make_float4(q[8], 0.f, 0.f, 0.f);
     }
 }
+
+template <bool ABS>
+__device__ __forceinline__ void load_record(const float* __restrict__ rec, size_t k, float4& a0,
+                                            float4& a1, float4& a2) {
+    load_record_from<ABS>(record_at<ABS>(rec, k), a0, a1, a2);
+}
+
+// An all-zero record (zero-initialised, never written): the source of the
+// absent records' loads in sum_records, whose +0 terms leave a sum unchanged.
+static __device__ __align__(16) float kZeroRecord[12];
 
 template <bool ABS>
 __device__ __forceinline__ void store_record(float* __restrict__ rec, size_t k, float4 a0,
@@ -293,7 +307,7 @@ __device__ __forceinline__ void store_record(float* __restrict__ rec, size_t k, 
 // slots are summed by their own lane: the bitmap words covering the range,
 // four independent loads per step (bits outside the range masked off), the set
 // bits taken four records at a time with their loads in flight together (a
-// missing fourth reads record 0 and is discarded).  Longer ranges are summed
+// missing one reads the zero record).  Longer ranges are summed
 // one at a time by the whole wave -- lane l takes slots off + l, off + l + 64,
 // ... -- and combined in a fixed tree, so that one large Gaussian does not hold
 // its wave for tens of rounds; the results stay independent of scheduling.
@@ -336,15 +350,19 @@ __device__ __forceinline__ void sum_records(int off, int end, const float* __res
                     float4 r[4][3];
 #pragma unroll
                     for (int i = 0; i < 4; ++i)
-                        load_record<ABS>(rec, (size_t)k[i], r[i][0], r[i][1], r[i][2]);
+                        load_record_from<ABS>(
+                            ok[i] ? record_at<ABS>(rec, (size_t)k[i]) : kZeroRecord, r[i][0],
+                            r[i][1], r[i][2]);
 #pragma unroll
                     for (int i = 0; i < 4; ++i) {
-                        const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
-                        const float4 q0 = ok[i] ? r[i][0] : z, q1 = ok[i] ? r[i][1] : z,
-                                     q2 = ok[i] ? r[i][2] : z;
+                        const float4 q0 = r[i][0], q1 = r[i][1], q2 = r[i][2];
                         a0.x += q0.x; a0.y += q0.y; a0.z += q0.z; a0.w += q0.w;
                         a1.x += q1.x; a1.y += q1.y; a1.z += q1.z; a1.w += q1.w;
-                        a2.x += q2.x; a2.y += q2.y; a2.z += q2.z;
+                        a2.x += q2.x;
+                        if (ABS) {
+                            a2.y += q2.y;
+                            a2.z += q2.z;
+                        }
                     }
                 }
             }

@@ -467,18 +467,70 @@ __global__ __launch_bounds__(256) void project_scan_add_kernel(
     }
 }
 
+// Fused dot products of two and three terms, in one fixed order (the
+// backward's helpers write every multiply-add explicitly: this file is built
+// without contraction, so the bits do not depend on the surrounding code).
+__device__ __forceinline__ float fdot2(float a0, float b0, float a1, float b1) {
+    return __builtin_fmaf(a0, b0, a1 * b1);
+}
+__device__ __forceinline__ float fdot3(float a0, float b0, float a1, float b1, float a2,
+                                       float b2) {
+    return __builtin_fmaf(a0, b0, __builtin_fmaf(a1, b1, a2 * b2));
+}
+
 // helpers.h:155-192 with vR[row][col]; the reference indexes v_R[col][row].
 __device__ __forceinline__ void quat_vjp(const float q[4], const M3& vR, float out[4]) {
     const float w = q[0], x = q[1], y = q[2], z = q[3];
 #define VR(col, row) vR.m[row][col]
-    out[0] = 2.f * (x * (VR(1, 2) - VR(2, 1)) + y * (VR(2, 0) - VR(0, 2)) + z * (VR(0, 1) - VR(1, 0)));
-    out[1] = 2.f * (-2.f * x * (VR(1, 1) + VR(2, 2)) + y * (VR(0, 1) + VR(1, 0)) +
-                    z * (VR(0, 2) + VR(2, 0)) + w * (VR(1, 2) - VR(2, 1)));
-    out[2] = 2.f * (x * (VR(0, 1) + VR(1, 0)) - 2.f * y * (VR(0, 0) + VR(2, 2)) +
-                    z * (VR(1, 2) + VR(2, 1)) + w * (VR(2, 0) - VR(0, 2)));
-    out[3] = 2.f * (x * (VR(0, 2) + VR(2, 0)) + y * (VR(1, 2) + VR(2, 1)) -
-                    2.f * z * (VR(0, 0) + VR(1, 1)) + w * (VR(0, 1) - VR(1, 0)));
+    out[0] = 2.f * fdot3(x, VR(1, 2) - VR(2, 1), y, VR(2, 0) - VR(0, 2), z, VR(0, 1) - VR(1, 0));
+    out[1] = 2.f * __builtin_fmaf(-2.f * x, VR(1, 1) + VR(2, 2),
+                                  fdot3(y, VR(0, 1) + VR(1, 0), z, VR(0, 2) + VR(2, 0), w,
+                                        VR(1, 2) - VR(2, 1)));
+    out[2] = 2.f * __builtin_fmaf(-2.f * y, VR(0, 0) + VR(2, 2),
+                                  fdot3(x, VR(0, 1) + VR(1, 0), z, VR(1, 2) + VR(2, 1), w,
+                                        VR(2, 0) - VR(0, 2)));
+    out[3] = 2.f * __builtin_fmaf(-2.f * z, VR(0, 0) + VR(1, 1),
+                                  fdot3(x, VR(0, 2) + VR(2, 0), y, VR(1, 2) + VR(2, 1), w,
+                                        VR(0, 1) - VR(1, 0)));
 #undef VR
+}
+
+// The backward's helpers below multiply out only the non-zero structure of
+// the matrices (the EWA Jacobian's zero row, the 2D cotangent's zero row and
+// column, the diagonal scale matrix), use the symmetry of the 2D cotangent G
+// and of the covariance V, and fuse products into multiply-adds (fdot2 /
+// fdot3): the gradients follow the reference formulas exactly in real
+// arithmetic but not its operation order (the forward keeps its order bit
+// for bit).
+
+// The rotation R, the scaled diagonal g s and cov3d = (R S)(R S)^T of one
+// Gaussian, for its VJPs (forward.cu:547-568; equal to cov3d_of up to rounding).
+struct CovParts {
+    M3 R;
+    float sg[3];
+    float cov[6];
+};
+
+__device__ __forceinline__ CovParts cov3d_parts(const float s[3], const float q[4], float g) {
+    CovParts p;
+    p.R = quat_to_rotmat(q[0], q[1], q[2], q[3]);
+    M3 M;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        p.sg[j] = g * s[j];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) M.m[i][j] = p.R.m[i][j] * p.sg[j];
+    }
+    const auto dot3 = [&](int i, int j) {
+        return fdot3(M.m[i][0], M.m[j][0], M.m[i][1], M.m[j][1], M.m[i][2], M.m[j][2]);
+    };
+    p.cov[0] = dot3(0, 0);
+    p.cov[1] = dot3(0, 1);
+    p.cov[2] = dot3(0, 2);
+    p.cov[3] = dot3(1, 1);
+    p.cov[4] = dot3(1, 2);
+    p.cov[5] = dot3(2, 2);
+    return p;
 }
 
 // VJP of one view's projection of one Gaussian with camera-space mean from
@@ -493,20 +545,21 @@ __device__ __forceinline__ void view_vjp(const Camera& c, const M3& W, const flo
                                          float vc3[6]) {
     float t[3];
     transform(W, pt, m[0], m[1], m[2], t);
-    // v_mean from v_xy: project_pix_vjp (helpers.h:131-138) then R^T (:90-98)
-    const float rw = 1.f / (t[2] + 1e-6f);
+    // v_mean from v_xy: project_pix_vjp (helpers.h:131-138) then R^T (:90-98);
+    // reciprocals to 1 ulp (v_rcp_f32) instead of correctly rounded divisions
+    const float rw = __builtin_amdgcn_rcpf(t[2] + 1e-6f);
     const float vpx = c.fx * gxy.x, vpy = c.fy * gxy.y;
-    const float vv[3] = {vpx * rw, vpy * rw, -(vpx * t[0] + vpy * t[1]) * rw * rw};
+    const float vv0 = vpx * rw, vv1 = vpy * rw;
+    const float vv2 = __builtin_fmaf(-fdot2(vpx, t[0], vpy, t[1]) * rw, rw, vz);
 #pragma unroll
-    for (int k = 0; k < 3; ++k)
-        vmean[k] = W.m[0][k] * vv[0] + W.m[1][k] * vv[1] + W.m[2][k] * vv[2] + W.m[2][k] * vz;
+    for (int k = 0; k < 3; ++k) vmean[k] = fdot3(W.m[0][k], vv0, W.m[1][k], vv1, W.m[2][k], vv2);
     // v_cov2d (helpers.h:60-87)
     const float gb = gb_raw * 0.5f;
-    const float xg00 = a * ga + b * gb, xg01 = a * gb + b * gc;
-    const float xg10 = b * ga + cc * gb, xg11 = b * gb + cc * gc;
-    vc2[0] = -(xg00 * a + xg01 * b);
-    vc2[1] = -(xg00 * b + xg01 * cc) - (xg10 * a + xg11 * b);
-    vc2[2] = -(xg10 * b + xg11 * cc);
+    const float xg00 = fdot2(a, ga, b, gb), xg01 = fdot2(a, gb, b, gc);
+    const float xg10 = fdot2(b, ga, cc, gb), xg11 = fdot2(b, gb, cc, gc);
+    vc2[0] = -fdot2(xg00, a, xg01, b);
+    vc2[1] = -(fdot2(xg00, b, xg01, cc) + fdot2(xg10, a, xg11, b));
+    vc2[2] = -fdot2(xg10, b, xg11, cc);
     if (has_comp) {
         const float inv_det = a * cc - b * b;
         const float omc = 1.f - cp * cp;
@@ -516,12 +569,13 @@ __device__ __forceinline__ void view_vjp(const Camera& c, const M3& W, const flo
         vc2[2] += vsq * (omc * cc - 0.3f * inv_det);
     }
     // EWA VJP, backward.cu:421-524
+    const float rz = __builtin_amdgcn_rcpf(t[2]);
     float tx = t[0], ty = t[1];
     bool inx = true, iny = true;
     float cxv = 0.f, cyv = 0.f;
     if (consistent) {
         const float lim_x = 1.3f * c.tan_fovx, lim_y = 1.3f * c.tan_fovy;
-        const float ux = t[0] / t[2], uy = t[1] / t[2];
+        const float ux = t[0] * rz, uy = t[1] * rz;
         inx = ux > -lim_x && ux < lim_x;
         iny = uy > -lim_y && uy < lim_y;
         cxv = fminf(lim_x, fmaxf(-lim_x, ux));
@@ -529,35 +583,52 @@ __device__ __forceinline__ void view_vjp(const Camera& c, const M3& W, const flo
         tx = t[2] * cxv;
         ty = t[2] * cyv;
     }
-    const float rz = 1.f / t[2];
     const float rz2 = rz * rz;
-    const M3 J = ewa_jacobian(c.fx, c.fy, tx, ty, rz, rz2);
+    // J = [[fx rz, 0, -fx tx rz2], [0, fy rz, -fy ty rz2], [0, 0, 0]]; T = J W
+    // has rows 0 and 1 only
+    const float j00 = c.fx * rz, j02 = -c.fx * tx * rz2;
+    const float j11 = c.fy * rz, j12 = -c.fy * ty * rz2;
+    float T0[3], T1[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        T0[j] = fdot2(j00, W.m[0][j], j02, W.m[2][j]);
+        T1[j] = fdot2(j11, W.m[1][j], j12, W.m[2][j]);
+    }
+    // G = [[g00, g01, 0], [g01, g11, 0], [0, 0, 0]]; H = G T (rows 0, 1)
+    const float g00 = vc2[0], g01 = 0.5f * vc2[1], g11 = vc2[2];
+    float H0[3], H1[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        H0[j] = fdot2(g00, T0[j], g01, T1[j]);
+        H1[j] = fdot2(g01, T0[j], g11, T1[j]);
+    }
+    // v_cov3d from vV = T^T G T = T^T H (symmetric: the off-diagonal pairs
+    // are twice one entry)
+    const auto vv_ = [&](int i, int j) { return fdot2(T0[i], H0[j], T1[i], H1[j]); };
+    vc3[0] = vv_(0, 0);
+    vc3[1] = 2.f * vv_(0, 1);
+    vc3[2] = 2.f * vv_(0, 2);
+    vc3[3] = vv_(1, 1);
+    vc3[4] = 2.f * vv_(1, 2);
+    vc3[5] = vv_(2, 2);
+    // vT = G T V^T + G^T T V = 2 H V (G, V symmetric), rows 0 and 1; only
+    // vJ = vT W^T entries (0,0), (0,2), (1,1), (1,2) are used
     const M3 V = sym3(cov);
-    M3 G;
-    G.m[0][0] = vc2[0]; G.m[0][1] = 0.5f * vc2[1]; G.m[0][2] = 0.f;
-    G.m[1][0] = 0.5f * vc2[1]; G.m[1][1] = vc2[2]; G.m[1][2] = 0.f;
-    G.m[2][0] = 0.f; G.m[2][1] = 0.f; G.m[2][2] = 0.f;
-    const M3 T = mm(J, W);
-    const M3 vV = mm(mm(tr(T), G), T);
-    const M3 vT1 = mm(mm(G, T), tr(V));
-    const M3 vT2 = mm(mm(tr(G), T), V);
-    M3 vT;
+    float P0[3], P1[3];
 #pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 3; ++j) vT.m[i][j] = vT1.m[i][j] + vT2.m[i][j];
-    vc3[0] = vV.m[0][0];
-    vc3[1] = vV.m[0][1] + vV.m[1][0];
-    vc3[2] = vV.m[0][2] + vV.m[2][0];
-    vc3[3] = vV.m[1][1];
-    vc3[4] = vV.m[1][2] + vV.m[2][1];
-    vc3[5] = vV.m[2][2];
-    const M3 vJ = mm(vT, tr(W));
+    for (int j = 0; j < 3; ++j) {
+        P0[j] = 2.f * fdot3(H0[0], V.m[0][j], H0[1], V.m[1][j], H0[2], V.m[2][j]);
+        P1[j] = 2.f * fdot3(H1[0], V.m[0][j], H1[1], V.m[1][j], H1[2], V.m[2][j]);
+    }
+    const auto wdot = [&](const float* P, int r) {
+        return fdot3(P[0], W.m[r][0], P[1], W.m[r][1], P[2], W.m[r][2]);
+    };
+    const float vJ00 = wdot(P0, 0), vJ02 = wdot(P0, 2), vJ11 = wdot(P1, 1), vJ12 = wdot(P1, 2);
     const float rz3 = rz2 * rz;
-    float vtx = -c.fx * rz2 * vJ.m[0][2];
-    float vty = -c.fy * rz2 * vJ.m[1][2];
-    float vtz = -c.fx * rz2 * vJ.m[0][0] + 2.f * c.fx * tx * rz3 * vJ.m[0][2] -
-                c.fy * rz2 * vJ.m[1][1] + 2.f * c.fy * ty * rz3 * vJ.m[1][2];
+    float vtx = -c.fx * rz2 * vJ02;
+    float vty = -c.fy * rz2 * vJ12;
+    float vtz = -c.fx * rz2 * vJ00 + 2.f * c.fx * tx * rz3 * vJ02 - c.fy * rz2 * vJ11 +
+                2.f * c.fy * ty * rz3 * vJ12;
     if (consistent) {
         vtz = vtz + (inx ? 0.f : cxv) * vtx + (iny ? 0.f : cyv) * vty;
         vtx = inx ? vtx : 0.f;
@@ -565,29 +636,34 @@ __device__ __forceinline__ void view_vjp(const Camera& c, const M3& W, const flo
     }
 #pragma unroll
     for (int k = 0; k < 3; ++k)
-        vmean[k] = vmean[k] + (vtx * W.m[0][k] + vty * W.m[1][k] + vtz * W.m[2][k]);
+        vmean[k] = vmean[k] + fdot3(vtx, W.m[0][k], vty, W.m[1][k], vtz, W.m[2][k]);
 }
 
-// Scale/rotation VJP of cov3d = (R S)(R S)^T, backward.cu:528-570.
-__device__ __forceinline__ void cov3d_vjp(const float s[3], const float q[4], float glob_scale,
+// Scale/rotation VJP of cov3d = (R S)(R S)^T, backward.cu:528-570, with R and
+// the scaled diagonal g s of cov3d_parts.
+__device__ __forceinline__ void cov3d_vjp(const CovParts& cp, const float q[4], float glob_scale,
                                           const float vc3[6], float vsc[3], float vq[4]) {
-    const M3 R = quat_to_rotmat(q[0], q[1], q[2], q[3]);
-    const M3 S = scale_mat(s[0], s[1], s[2], glob_scale);
-    const M3 M = mm(R, S);
-    M3 vVs;
-    vVs.m[0][0] = vc3[0]; vVs.m[0][1] = 0.5f * vc3[1]; vVs.m[0][2] = 0.5f * vc3[2];
-    vVs.m[1][0] = 0.5f * vc3[1]; vVs.m[1][1] = vc3[3]; vVs.m[1][2] = 0.5f * vc3[4];
-    vVs.m[2][0] = 0.5f * vc3[2]; vVs.m[2][1] = 0.5f * vc3[4]; vVs.m[2][2] = vc3[5];
-    M3 vM = mm(vVs, M);
+    const M3& R = cp.R;
+    // vM = 2 vVs M with M = R diag(g s) and vVs the symmetric v_cov3d
+    const float h1 = 0.5f * vc3[1], h2 = 0.5f * vc3[2], h4 = 0.5f * vc3[4];
+    const float Vs[3][3] = {{vc3[0], h1, h2}, {h1, vc3[3], h4}, {h2, h4, vc3[5]}};
+    M3 vM;
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
-        for (int j = 0; j < 3; ++j) vM.m[i][j] = 2.f * vM.m[i][j];
+        for (int j = 0; j < 3; ++j)
+            vM.m[i][j] = 2.f * cp.sg[j] *
+                         fdot3(Vs[i][0], R.m[0][j], Vs[i][1], R.m[1][j], Vs[i][2], R.m[2][j]);
 #pragma unroll
     for (int k = 0; k < 3; ++k)
-        vsc[k] = (R.m[0][k] * vM.m[0][k] + R.m[1][k] * vM.m[1][k] + R.m[2][k] * vM.m[2][k]) *
+        vsc[k] = fdot3(R.m[0][k], vM.m[0][k], R.m[1][k], vM.m[1][k], R.m[2][k], vM.m[2][k]) *
                  glob_scale;
-    const M3 vR = mm(vM, S);
+    // vR = vM S
+    M3 vR;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) vR.m[i][j] = vM.m[i][j] * cp.sg[j];
     quat_vjp(q, vR, vq);
 }
 
@@ -612,15 +688,14 @@ __global__ __launch_bounds__(256) void project_bwd_kernel(
         const float s[3] = {scales[3 * idx + 0], scales[3 * idx + 1], scales[3 * idx + 2]};
         const float4 q4 = reinterpret_cast<const float4*>(quats)[idx];
         const float q[4] = {q4.x, q4.y, q4.z, q4.w};
-        float cov[6];
-        cov3d_of(s, q, c.glob_scale, cov);
+        const CovParts cp = cov3d_parts(s, q, c.glob_scale);
         const bool has_comp = v_compensation != nullptr;
-        view_vjp(c, W, pt, m, cov, conics[3 * idx + 0], conics[3 * idx + 1], conics[3 * idx + 2],
+        view_vjp(c, W, pt, m, cp.cov, conics[3 * idx + 0], conics[3 * idx + 1], conics[3 * idx + 2],
                  reinterpret_cast<const float2*>(v_xy)[idx], v_depth ? v_depth[idx] : 0.f,
                  v_conic[3 * idx + 0], v_conic[3 * idx + 1], v_conic[3 * idx + 2], has_comp,
                  has_comp ? compensation[idx] : 0.f, has_comp ? v_compensation[idx] : 0.f,
                  consistent, vmean, vc2, vc3);
-        cov3d_vjp(s, q, c.glob_scale, vc3, vsc, vq);
+        cov3d_vjp(cp, q, c.glob_scale, vc3, vsc, vq);
     }
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
@@ -690,12 +765,11 @@ __global__ __launch_bounds__(256) void render_bwd_tail_kernel(
         float pt[3];
         load_view(viewmat, W, pt);
         const float q[4] = {q4.x, q4.y, q4.z, q4.w};
-        float cov[6];
-        cov3d_of(sc, q, c.glob_scale, cov);
+        const CovParts cp = cov3d_parts(sc, q, c.glob_scale);
         float vc2[3], vc3[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-        view_vjp(c, W, pt, m, cov, cn[0], cn[1], cn[2], make_float2(a1.z, a1.w), 0.f, a0.w,
+        view_vjp(c, W, pt, m, cp.cov, cn[0], cn[1], cn[2], make_float2(a1.z, a1.w), 0.f, a0.w,
                  a1.x, a1.y, false, 0.f, 0.f, consistent, vmean, vc2, vc3);
-        cov3d_vjp(sc, q, c.glob_scale, vc3, vsc, vq);
+        cov3d_vjp(cp, q, c.glob_scale, vc3, vsc, vq);
     }
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
@@ -729,7 +803,7 @@ __global__ __launch_bounds__(256) void project_batch_bwd_kernel(
     const float q[4] = {q4.x, q4.y, q4.z, q4.w};
     float vmean[3] = {0.f, 0.f, 0.f}, vc3[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     float4 rgbo = make_float4(0.f, 0.f, 0.f, 0.f);
-    float cov[6];
+    CovParts cp;
     bool any = false;
     for (int v = sc.v_begin; v < sc.v_end; ++v) {
         const ViewRec vr = views[v];
@@ -743,7 +817,7 @@ __global__ __launch_bounds__(256) void project_batch_bwd_kernel(
             rgbo.w += c.w;
         }
         if (!any) {
-            cov3d_of(s, q, glob_scale, cov);
+            cp = cov3d_parts(s, q, glob_scale);
             any = true;
         }
         const Camera c = view_camera(vr, glob_scale, clip_thresh, 1);
@@ -751,7 +825,7 @@ __global__ __launch_bounds__(256) void project_batch_bwd_kernel(
         float pt[3];
         load_view(viewmats + 16 * vr.view, W, pt);
         float vm[3], v2[3], v3[6];
-        view_vjp(c, W, pt, m, cov, conics[3 * it + 0], conics[3 * it + 1], conics[3 * it + 2],
+        view_vjp(c, W, pt, m, cp.cov, conics[3 * it + 0], conics[3 * it + 1], conics[3 * it + 2],
                  reinterpret_cast<const float2*>(v_xy)[it], 0.f, v_conic[3 * it + 0],
                  v_conic[3 * it + 1], v_conic[3 * it + 2], false, 0.f, 0.f, consistent, vm, v2,
                  v3);
@@ -761,7 +835,7 @@ __global__ __launch_bounds__(256) void project_batch_bwd_kernel(
         for (int j = 0; j < 6; ++j) vc3[j] += v3[j];
     }
     float vsc[3] = {0.f, 0.f, 0.f}, vq[4] = {0.f, 0.f, 0.f, 0.f};
-    if (any) cov3d_vjp(s, q, glob_scale, vc3, vsc, vq);
+    if (any) cov3d_vjp(cp, q, glob_scale, vc3, vsc, vq);
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
         v_mean3d[3 * g + j] = vmean[j];
