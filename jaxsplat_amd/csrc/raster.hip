@@ -15,9 +15,8 @@
 //  * one workgroup per tile, rounded up to whole 64-lane waves (bw*bw pixels;
 //    padding lanes take part in loads and reductions but own no pixel);
 //  * Gaussian batches staged in LDS as three float4 records
-//    (x, y, opacity, id) (a, b, c, -) (r, g, b, -), positions relative to the
-//    tile's centre, so each lane reads a batch entry with 128-bit broadcast
-//    LDS loads;
+//    (x, y, opacity, id) (a, b, c, -) (r, g, b, -) so each lane reads a batch
+//    entry with 128-bit broadcast LDS loads;
 //  * per-(Gaussian, 64-pixel group) culling bits built once per batch: the
 //    exact minimum of sigma over the group's pixel rectangle against
 //    ln(255 o); each wave then walks only its set bits (scalar s_ff1);
