@@ -467,8 +467,9 @@ void jaxsplat_batch_destroy(jaxsplat_batch* b);
 size_t jaxsplat_batch_workspace_size(const jaxsplat_batch* b);
 /* Gaussians staged in LDS per batch by the rasterize forward / backward:
  * 64, 128 or 256, or 0 for the defaults (forward: the workgroup size, bw^2
- * rounded up to whole waves; backward: 64).  The results do not depend on
- * it; the autotuner of the Python layer picks the fastest. */
+ * rounded up to whole waves, rounded up again to 64, 128 or 256; backward:
+ * 64).  The results do not depend on it; the autotuner of the Python layer
+ * picks the fastest. */
 int jaxsplat_batch_set_depths(jaxsplat_batch* b, int32_t fwd_depth, int32_t bwd_depth);
 
 /* How the batch rasterizer (forward and backward) assigns its (view, tile)

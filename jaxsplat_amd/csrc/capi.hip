@@ -993,7 +993,7 @@ struct jaxsplat_batch {
     long long num_pixels = 0;
     float glob_scale = 1.f, clip_thresh = 0.f;
     int block_width = 16;
-    int fwd_depth = 0, bwd_depth = 0;     // Gaussians per LDS batch (0: workgroup size)
+    int fwd_depth = 0, bwd_depth = 0;     // Gaussians per LDS batch (0: the defaults)
     int schedule = JAXSPLAT_SCHEDULE_GRID;
     jsa::BatchBinPlan bin{};
     std::vector<jsa::ViewRec> views;

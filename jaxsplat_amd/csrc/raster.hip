@@ -194,6 +194,18 @@ __device__ __forceinline__ float4 group_rect(int p0, int bw, int x_org, int y_or
     return make_float4((float)x0 + 0.5f, (float)x1 + 0.5f, (float)y0 + 0.5f, (float)y1 + 0.5f);
 }
 
+// Pixel-centre box of 8 x 8 quadrant q (rows 8 (q / 2).., columns 8 (q % 2)..)
+// of a bw x bw tile, bw <= 16, clipped to the tile and the image; ok = false
+// if no pixel is inside.
+__device__ __forceinline__ float4 quad_rect(int q, int bw, int x_org, int y_org, int W, int H,
+                                            bool& ok) {
+    const int r0 = (q >> 1) << 3, c0 = (q & 1) << 3;
+    const int x0 = x_org + c0, x1 = min(x_org + min(c0 + 7, bw - 1), W - 1);
+    const int y0 = y_org + r0, y1 = min(y_org + min(r0 + 7, bw - 1), H - 1);
+    ok = r0 < bw && c0 < bw && x0 <= x1 && y0 <= y1;
+    return make_float4((float)x0 + 0.5f, (float)x1 + 0.5f, (float)y0 + 0.5f, (float)y1 + 0.5f);
+}
+
 // Culling of a (Gaussian, pixel group) pair.  A pixel can pass the per-pixel
 // test only where o * exp(-sigma) >= 1/255, i.e. sigma <= ln(255 o).  The test
 // compares that threshold (plus 0.01, far above float32 rounding of sigma)
@@ -698,15 +710,17 @@ __device__ __forceinline__ void raster_bwd_tile(
     const int tile = gt - vr.tile_off;
     const int gshift = vr.g0 - vr.item_off;      // gaussian = item + gshift
     const int bw = rg.block_width;
-    const int nthr = blockDim.x;         // 64 or 128 (NSLOT pixels per lane)
+    constexpr int nthr = 64;             // one wave (bwd_threads)
     const int tid = threadIdx.x;
     const int lane = lane_id();
-    const int wave = tid >> 6;
-    const int nwaves = nthr >> 6;
-    const int ngroups = NSLOT * nwaves;  // <= 4
-    // slot sl of lane l is pixel sl * 64 + l: for 16 x 16 tiles (one wave) the
-    // four 8 x 8 quadrants at the same offset l
-    const bool quad = bw == 16;
+    constexpr int wave = 0;
+    constexpr int nwaves = 1;
+    constexpr int ngroups = NSLOT;
+    // Four slots (tiles of 12..16 pixels): slot sl of lane l is pixel
+    // (l / 8, l % 8) of the 8 x 8 quadrant sl (lanes past a smaller tile own no
+    // pixel), so slots 0/2 and 1/3 share a column and 0/1 and 2/3 a row.  Two
+    // slots (up to 11 pixels): pixel sl * 64 + l, row-major.
+    constexpr bool quad = NSLOT == 4;
     const int ty = tile / vr.tiles_x, tx = tile - ty * vr.tiles_x;
     const int x_org = tx * bw, y_org = ty * bw;
     // positions relative to the tile's centre corner, as in the forward
@@ -718,11 +732,20 @@ __device__ __forceinline__ void raster_bwd_tile(
     int slot_fin[NSLOT];                 // per slot: largest final index over the wave
 #pragma unroll
     for (int sl = 0; sl < NSLOT; ++sl) {
-        const int p = sl * nthr + tid;
-        const bool pixel = p < bw * bw;
-        const int2 rc = tile_pixel(pixel ? p : 0, bw);
-        const int i = y_org + rc.x;
-        const int j = x_org + rc.y;
+        int r, c;
+        bool pixel;
+        if constexpr (quad) {
+            r = ((sl >> 1) << 3) + (tid >> 3);
+            c = ((sl & 1) << 3) + (tid & 7);
+            pixel = r < bw && c < bw;
+        } else {
+            const int p = sl * nthr + tid;
+            pixel = p < bw * bw;
+            r = pixel ? p / bw : 0;
+            c = pixel ? p - r * bw : 0;
+        }
+        const int i = y_org + r;
+        const int j = x_org + c;
         PixState& s = ps[sl];
         s.inside = pixel && i < vr.img_h && j < vr.img_w;
         s.px = (float)(j - cx0) + 0.5f;
@@ -745,8 +768,8 @@ __device__ __forceinline__ void raster_bwd_tile(
     }
     if (tid < ngroups) {
         bool ok;
-        const int sl = tid / nwaves, w = tid - sl * nwaves;
-        s_rect[tid] = group_rect(sl * nthr + w * 64, bw, x_org, y_org, vr.img_w, vr.img_h, ok);
+        s_rect[tid] = quad ? quad_rect(tid, bw, x_org, y_org, vr.img_w, vr.img_h, ok)
+                           : group_rect(tid * 64, bw, x_org, y_org, vr.img_w, vr.img_h, ok);
         s_rok[tid] = ok;
     }
     for (int e = tid; e < DEPTH; e += nthr) {
@@ -845,9 +868,9 @@ __device__ __forceinline__ void raster_bwd_tile(
                 for (int v = 0; v < NV; ++v) g[v] = -0.f;
                 uint64_t any = 0;
                 bool shared = false;
-                if constexpr (NSLOT == 4) {
-                    if (quad) {
-                        // 16 x 16 quadrant slots: slots 0/2 and 1/3 share a
+                if constexpr (quad) {
+                    {
+                        // quadrant slots: slots 0/2 and 1/3 share a
                         // pixel column, 0/1 and 2/3 a row, so dx, dy and the
                         // row terms b' dy, c' dy of sigma are computed once per
                         // column / row (the same roundings as sigma2 per pixel)
@@ -943,7 +966,7 @@ __device__ __forceinline__ void raster_bwd_tile(
 }
 
 template <bool CONSISTENT, bool ABS, int DEPTH, bool STAGED, int NSLOT>
-__global__ __launch_bounds__(128) void raster_bwd_kernel(
+__global__ __launch_bounds__(64) void raster_bwd_kernel(
     RasterGrid rg, const int* __restrict__ gids, const int2* __restrict__ bins,
     const float2* __restrict__ xys, const float* __restrict__ conics,
     const float* __restrict__ colors, const float* __restrict__ opac,
@@ -961,7 +984,7 @@ __global__ __launch_bounds__(128) void raster_bwd_kernel(
 
 // Work-queue form of the staged batch backward (see raster_fwd_queue_kernel).
 template <bool CONSISTENT, int DEPTH, int NSLOT>
-__global__ __launch_bounds__(128) void raster_bwd_queue_kernel(
+__global__ __launch_bounds__(64) void raster_bwd_queue_kernel(
     RasterGrid rg, const int* __restrict__ order, int* __restrict__ counter,
     const int* __restrict__ gids, const int2* __restrict__ bins,
     const float2* __restrict__ xys, const float* __restrict__ conics,
@@ -1196,7 +1219,9 @@ hipError_t raster_fwd_grid(const RasterGrid& g, const int* items_sorted, const i
                            int* final_idx, float* out_img, hipStream_t s) {
     if (g.total_tiles <= 0) return hipSuccess;
     const int nthr = ((g.block_width * g.block_width + 63) / 64) * 64;
-    const int depth = g.fwd_depth > 0 ? g.fwd_depth : nthr;
+    // default: the smallest batch depth that holds one entry per thread (a
+    // 12- or 13-pixel tile has 192 threads: 256, two staging rounds)
+    const int depth = g.fwd_depth > 0 ? g.fwd_depth : (nthr <= 64 ? 64 : nthr <= 128 ? 128 : 256);
     const int grid = tile_grid(g.total_tiles);
     const int2* b2 = reinterpret_cast<const int2*>(tile_bins);
     const float2* x2 = reinterpret_cast<const float2*>(xys);

@@ -88,7 +88,8 @@ def _raster_case(name="small", n=None, bw=16, opacity=None, bg=(0.0, 0.0, 0.0), 
     return sc, cam, g, gids, bins, isect, op, np.asarray(bg, np.float32)
 
 
-@pytest.mark.parametrize("bw,bg", [(16, (0.0, 0.0, 0.0)), (8, (0.2, 0.5, 0.9)), (5, (1, 1, 1))])
+@pytest.mark.parametrize("bw,bg", [(16, (0.0, 0.0, 0.0)), (8, (0.2, 0.5, 0.9)), (5, (1, 1, 1)),
+                                  (12, (0.4, 0.4, 0.1)), (13, (0.0, 1.0, 0.5))])
 def test_raster_fwd_vs_oracle(bw, bg):
     sc, cam, g, gids, bins, isect, op, bgv = _raster_case(bw=bw, bg=bg)
     out = H.gpu_raster_fwd(gids, bins, isect, g["xys"], g["conics"], sc.colors, op, bgv,
@@ -163,8 +164,10 @@ def test_raster_partial_tiles_vs_oracle():
         assert okg, (k, worst)
 
 
+# tile widths 12..16 take the backward's four-slot quadrant layout (lanes past
+# a 12- or 13-pixel tile own no pixel), up to 11 the two-slot row-major one
 @pytest.mark.parametrize("adjoint,mode", [(0, "reference"), (1, "consistent")])
-@pytest.mark.parametrize("bw", [16, 8])
+@pytest.mark.parametrize("bw", [16, 13, 12, 8])
 def test_raster_bwd_vs_oracle(adjoint, mode, bw):
     sc, cam, g, gids, bins, isect, op, bgv = _raster_case(bw=bw, bg=(0.3, 0.1, 0.6),
                                                           opacity=lambda o: np.minimum(o * 1.3, 1.0))

@@ -97,6 +97,23 @@ def test_staged_matches_atomic_backward(nch, adjoint, with_abs):
         assert np.all(np.abs(s[k] - ref) <= tol), k
 
 
+@pytest.mark.parametrize("bw", [12, 13])
+def test_staged_matches_atomic_backward_partial_quadrants(bw):
+    # tiles of 12 and 13 pixels: the four-slot quadrant layout with lanes
+    # past the tile owning no pixel, in both the staged and the atomic form
+    case = _case(3, bw=bw)
+    sc, cam = case[0], case[1]
+    rng = np.random.default_rng(13)
+    v_img = rng.normal(size=(*cam.img_shape, 3)).astype(np.float32)
+    a = _run("atomic", *case, v_img, 0, False, bw=bw)
+    s = _run("staged", *case, v_img, 0, False, bw=bw)
+    assert np.abs(a["v_colors"]).max() > 0
+    for k in a:
+        ref = a[k].astype(np.float64)
+        tol = 1e-5 * np.abs(ref) + 1e-6 * np.abs(ref).max()
+        assert np.all(np.abs(s[k] - ref) <= tol), k
+
+
 def test_staged_intersection_count_not_multiple_of_four():
     """Regression: the per-slot flags of a workspace left dirty by an earlier
     call (here 0xA5) must all be cleared, also when the intersection count is
