@@ -302,15 +302,73 @@ __device__ __forceinline__ void store_record(float* __restrict__ rec, size_t k, 
         q[8] = a2.x;
     }
 }
+// Row (16-lane) sums of v[0..NV) by recursive halving.  At each DPP level
+// (partner lanes l^1, l^2, l+-4, l+-8 within the row: each flips one lane bit
+// and keeps the lower ones) the values are taken in pairs: a lane whose level
+// bit is set keeps the second value of a pair and passes the first, the other
+// lanes the reverse, and each lane adds what its partner passes.  A pair costs
+// two selects and one DPP add (two DPP adds as whole row sums, plus the
+// selects that pick lane j's value afterwards); an unpaired value is summed
+// whole.  Lane j of every row ends with the row sum of value j, j < NV <= 16.
+template <int L>
+__device__ __forceinline__ float dpp_level(float x) {
+    if constexpr (L == 0) return dpp_f<kDppQuadPerm1032>(x);
+    else if constexpr (L == 1) return dpp_f<kDppQuadPerm2301>(x);
+    else if constexpr (L == 2) return dpp_f<kDppRowRor4>(x);
+    else return dpp_f<kDppRowRor8>(x);
+}
+
+template <int L, int M>
+__device__ __forceinline__ void halve_level(float* v, int lane) {
+    const bool hi = (lane >> L) & 1;
+#pragma unroll
+    for (int i = 0; i < M / 2; ++i) {
+        const float a = v[2 * i], b = v[2 * i + 1];
+        const float keep = hi ? b : a, pass = hi ? a : b;
+        v[i] = keep + dpp_level<L>(pass);
+    }
+    if (M & 1) v[M / 2] = v[M - 1] + dpp_level<L>(v[M - 1]);
+}
+
+template <int NV>
+__device__ __forceinline__ float row_sums_split(float* v, int lane) {
+    constexpr int M1 = (NV + 1) / 2, M2 = (M1 + 1) / 2, M3 = (M2 + 1) / 2;
+    static_assert(NV <= 16 && (M3 + 1) / 2 == 1, "at most 16 values per row");
+    halve_level<0, NV>(v, lane);
+    halve_level<1, M1>(v, lane);
+    halve_level<2, M2>(v, lane);
+    halve_level<3, M3>(v, lane);
+    return v[0];
+}
+
+// Wave sums of v[0..NV): row_sums_split, then the four rows summed with the
+// gfx950 row swaps; lane j < NV returns the wave's sum of value j.
+template <int NV>
+__device__ __forceinline__ float wave_sums_split(float* v, int lane) {
+    float val = row_sums_split<NV>(v, lane);
+    {
+        const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(val),
+                                                        __float_as_uint(val), false, false);
+        val = __uint_as_float(r[0]) + __uint_as_float(r[1]);
+    }
+    {
+        const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(val),
+                                                        __float_as_uint(val), false, false);
+        val = __uint_as_float(r[0]) + __uint_as_float(r[1]);
+    }
+    return val;
+}
+
 // Sum of the flagged staged records of slots [off, end) (empty: off == end),
 // in slot order, into (a0, a1, a2) of the calling lane.  Ranges of at most 16
 // slots are summed by their own lane: the bitmap words covering the range,
 // four independent loads per step (bits outside the range masked off), the set
 // bits taken four records at a time with their loads in flight together (a
-// missing one reads the zero record).  Longer ranges are summed
-// one at a time by the whole wave -- lane l takes slots off + l, off + l + 64,
-// ... -- and combined in a fixed tree, so that one large Gaussian does not hold
-// its wave for tens of rounds; the results stay independent of scheduling.
+// missing one reads the zero record).  Longer ranges are summed one at a time
+// by the whole wave -- lane l takes slots off + l, off + l + 64, ... -- and
+// combined in a fixed tree (wave_sums_split), so that one large Gaussian does
+// not hold its wave for tens of rounds; the results stay independent of
+// scheduling.
 // Called by every lane of the wave.
 template <bool ABS>
 __device__ __forceinline__ void sum_records(int off, int end, const float* __restrict__ rec,
@@ -387,15 +445,17 @@ __device__ __forceinline__ void sum_records(int off, int end, const float* __res
             b2.x += r2.x; b2.y += r2.y; b2.z += r2.z;
         }
         float v[11] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w, b2.x, b2.y, b2.z};
+        // lane j < NV ends with the wave sum of value j (xy_abs sums only with ABS)
+        constexpr int NV = ABS ? 11 : 9;
+        const float val = wave_sums_split<NV>(v, lane);
+        float t[11];
 #pragma unroll
-        for (int i = 0; i < 11; ++i) {
-            if (!ABS && i >= 9) continue;    // xy_abs sums only with ABS
-            v[i] = wave_sum(v[i]);
-        }
+        for (int i = 0; i < NV; ++i)
+            t[i] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(val), i));
         if (lane == src) {
-            a0 = make_float4(v[0], v[1], v[2], v[3]);
-            a1 = make_float4(v[4], v[5], v[6], v[7]);
-            a2 = make_float4(v[8], ABS ? v[9] : 0.f, ABS ? v[10] : 0.f, 0.f);
+            a0 = make_float4(t[0], t[1], t[2], t[3]);
+            a1 = make_float4(t[4], t[5], t[6], t[7]);
+            a2 = make_float4(t[8], ABS ? t[9] : 0.f, ABS ? t[10] : 0.f, 0.f);
         }
     }
 }

@@ -517,45 +517,6 @@ struct PixState {
     bool inside;
 };
 
-// Row (16-lane) sums of v[0..NV) by recursive halving.  At each DPP level
-// (partner lanes l^1, l^2, l+-4, l+-8 within the row: each flips one lane bit
-// and keeps the lower ones) the values are taken in pairs: a lane whose level
-// bit is set keeps the second value of a pair and passes the first, the other
-// lanes the reverse, and each lane adds what its partner passes.  A pair costs
-// two selects and one DPP add (two DPP adds as whole row sums, plus the
-// selects that pick lane j's value afterwards); an unpaired value is summed
-// whole.  Lane j of every row ends with the row sum of value j, j < NV <= 16.
-template <int L>
-__device__ __forceinline__ float dpp_level(float x) {
-    if constexpr (L == 0) return dpp_f<kDppQuadPerm1032>(x);
-    else if constexpr (L == 1) return dpp_f<kDppQuadPerm2301>(x);
-    else if constexpr (L == 2) return dpp_f<kDppRowRor4>(x);
-    else return dpp_f<kDppRowRor8>(x);
-}
-
-template <int L, int M>
-__device__ __forceinline__ void halve_level(float* v, int lane) {
-    const bool hi = (lane >> L) & 1;
-#pragma unroll
-    for (int i = 0; i < M / 2; ++i) {
-        const float a = v[2 * i], b = v[2 * i + 1];
-        const float keep = hi ? b : a, pass = hi ? a : b;
-        v[i] = keep + dpp_level<L>(pass);
-    }
-    if (M & 1) v[M / 2] = v[M - 1] + dpp_level<L>(v[M - 1]);
-}
-
-template <int NV>
-__device__ __forceinline__ float row_sums_split(float* v, int lane) {
-    constexpr int M1 = (NV + 1) / 2, M2 = (M1 + 1) / 2, M3 = (M2 + 1) / 2;
-    static_assert(NV <= 16 && (M3 + 1) / 2 == 1, "at most 16 values per row");
-    halve_level<0, NV>(v, lane);
-    halve_level<1, M1>(v, lane);
-    halve_level<2, M2>(v, lane);
-    halve_level<3, M3>(v, lane);
-    return v[0];
-}
-
 // One pixel's contribution to the per-Gaussian partial sums, branch-free
 // (an invalid pair contributes zeros and leaves the pixel state unchanged).
 // Only pixel-dependent sums are accumulated; the per-Gaussian factors are
