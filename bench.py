@@ -956,7 +956,7 @@ def summarize_profiles(prof, workload, calib_mb, command):
     cmds = "\n".join(f"rocprofv3 {PROFILE_PASSES[s]} -d DIR/{s} -o run --output-format csv -- "
                      f"{command}" for s in rec["passes"])
     lines = [f"# rocprofv3 kernel statistics and HBM traffic: {workload} fwd+bwd step", "",
-             "Commands (repository root, MI355X, ROCm 7.2; DIR = gpurun_out/prof), then "
+             "Commands (repository root, MI355X, ROCm 7.2; DIR = an output directory), then "
              f"`python bench.py --summarize-profiles DIR --workload {workload} "
              f"--copy-calibration-mb {calib_mb}`:", "", "```", cmds, "```", "",
              f"Source hash of the profiled kernels: `{rec['source_sha16']}`.  HBM bytes = "
