@@ -509,10 +509,14 @@ __global__ __launch_bounds__(256) void raster_fwd_queue_kernel(
 struct PixState {
     // px, py: pixel centre relative to the tile's centre corner (exact);
     // pxx, pxy, pyy: its products, the weights of the moment sums
-    float px, py, pxx, pxy, pyy, T_final, T, vo_r, vo_g, vo_b;
-    // (sum over the Gaussians behind of colour * alpha * T, dotted with v_out)
-    // - T_final (v_out_alpha - background . v_out): the only form the adjoint uses
-    float bv;
+    float px, py, pxx, pxy, pyy, vo_r, vo_g, vo_b;
+    // U = 1 / T for the transmittance T in front of the Gaussians walked so
+    // far (back to front), and w = B / T for their blended colour behind,
+    // B = (sum of colour * alpha * T, dotted with v_out) - T_final (v_out_alpha
+    // - background . v_out), the only form the adjoint uses.  With these,
+    // stepping over a Gaussian of alpha a is U' = U (1 - a) (one FMA, no
+    // division), T' = 1 / U', v_alpha = T' (c.v_out - w), w' = w + a (c.v_out - w).
+    float U, w;
     int fin;
     bool inside;
 };
@@ -546,20 +550,22 @@ __device__ __forceinline__ uint64_t bwd_pixel(PixState& s, float dx, float dy, f
     const bool live = k <= s.fin;
     const bool acc = pair_accepted(s2, col.w);
     const bool valid = live && acc;
-    // an invalid pair acts as alpha = 0: T unchanged (1 / (1 - 0) is exact), no weight
+    // an invalid pair acts as alpha = 0: U and w unchanged exactly, no weight
     const float rv = valid ? raw : 0.f;
     const float al = fminf(kClamp, rv);
-    const float ra = __builtin_amdgcn_rcpf(1.f - al);
-    const float T = s.T * ra;                    // transmittance in front of this Gaussian
-    s.T = T;
+    const float U = __builtin_fmaf(-s.U, al, s.U);
+    s.U = U;
+    const float T = __builtin_amdgcn_rcpf(U);    // transmittance in front of this Gaussian
     const float fac = al * T;
     JSA_ACC(0, fac * s.vo_r);
     JSA_ACC(1, fac * s.vo_g);
     JSA_ACC(2, fac * s.vo_b);
-    // v_alpha = sum_ch (col T - buf ra) v_out + ra T_final (v_out_alpha - bg . v_out)
+    // v_alpha = sum_ch (col T - buf / (1 - alpha)) v_out
+    //           + T_final (v_out_alpha - bg . v_out) / (1 - alpha) = T (cv - w)
     const float cv = col.x * s.vo_r + col.y * s.vo_g + col.z * s.vo_b;
-    const float v_alpha = T * cv - ra * s.bv;
-    s.bv += fac * cv;
+    const float dcw = cv - s.w;
+    const float v_alpha = T * dcw;
+    s.w = __builtin_fmaf(al, dcw, s.w);
     // the clamp's zero slope applies in consistent mode only
     const float vs = (CONSISTENT && raw >= kClamp) ? 0.f : -rv * v_alpha;
     if (NV == 11) {
@@ -715,16 +721,18 @@ __device__ __forceinline__ void raster_bwd_tile(
         s.pxy = s.px * s.py;
         s.pyy = s.py * s.py;
         const size_t pix = s.inside ? (size_t)vr.pix_off + (size_t)i * vr.img_w + j : 0;
-        s.T_final = s.inside ? final_Ts[pix] : 0.f;
-        s.T = s.T_final;
+        // the forward leaves T_final > 1e-4; a zero from elsewhere gives a
+        // huge finite U (T ~ 0: no weight) rather than inf (inf * 0 = NaN)
+        const float T_final = s.inside ? final_Ts[pix] : 1.f;
+        s.U = T_final > 0.f ? 1.f / T_final : 3.0e38f;
         s.fin = s.inside ? final_idx[pix] : -1;
         const float4 vo = s.inside ? load_color(rg, v_out_img, pix) : make_float4(0, 0, 0, 0);
         s.vo_r = vo.x;
         s.vo_g = vo.y;
         s.vo_b = vo.z;
         const float vo_a = (s.inside && v_out_alpha) ? v_out_alpha[pix] : 0.f;
-        // background term of v_alpha: T_final * ra * (v_out_alpha - bg . v_out)
-        s.bv = -s.T_final * (vo_a - (bgr * s.vo_r + bgg * s.vo_g + bgb * s.vo_b));
+        // w = B / T_final with B's background term only
+        s.w = (bgr * s.vo_r + bgg * s.vo_g + bgb * s.vo_b) - vo_a;
         slot_fin[sl] = uniform_i(wave_max_i(s.fin));
     }
     if (tid < ngroups) {
