@@ -361,10 +361,10 @@ __device__ __forceinline__ float wave_sums_split(float* v, int lane) {
 
 // Sum of the flagged staged records of slots [off, end) (empty: off == end),
 // in slot order, into (a0, a1, a2) of the calling lane.  Ranges of at most 16
-// slots are summed by their own lane: the bitmap words covering the range,
-// four independent loads per step (bits outside the range masked off), the set
-// bits taken four records at a time with their loads in flight together (a
-// missing one reads the zero record).  Longer ranges are summed one at a time
+// slots are summed by their own lane: the (at most two) bitmap words covering
+// the range as one mask (bits outside the range masked off), the set bits
+// taken four records at a time with their loads in flight together (a missing
+// one reads the zero record).  Longer ranges are summed one at a time
 // by the whole wave -- lane l takes slots off + l, off + l + 64, ... -- and
 // combined in a fixed tree (wave_sums_split), so that one large Gaussian does
 // not hold its wave for tens of rounds; the results stay independent of
@@ -380,51 +380,42 @@ __device__ __forceinline__ void sum_records(int off, int end, const float* __res
     constexpr int kBig = 16;
     const bool big = end - off > kBig;
     const int cnt = end - off;
+    static_assert(kBig <= 33, "a per-lane range spans at most two bitmap words");
     if (cnt > 0 && !big) {
+        // the (at most two) bitmap words of the range as one 64-bit mask, so
+        // that four set bits of either word are taken per step
         const int wfirst = off >> 5, wlast = (end - 1) >> 5;
         const uint32_t lo_mask = ~0u << (off & 31);
         const int hb = (end - 1) & 31;
         const uint32_t hi_mask = hb == 31 ? ~0u : ((2u << hb) - 1u);
-        for (int w0 = wfirst; w0 <= wlast; w0 += 4) {
-            uint32_t W[4];
+        const uint32_t wa = bits[wfirst];
+        const uint32_t wb = wlast > wfirst ? bits[wlast] : 0u;
+        uint64_t v = wlast > wfirst ? ((uint64_t)(wb & hi_mask) << 32) | (wa & lo_mask)
+                                    : (uint64_t)(wa & lo_mask & hi_mask);
+        const int kb = wfirst << 5;
+        while (v) {
+            int k[4];
+            bool ok[4];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int w = w0 + q;
-                uint32_t v = w <= wlast ? bits[w] : 0u;
-                if (w == wfirst) v &= lo_mask;
-                if (w == wlast) v &= hi_mask;
-                W[q] = v;
+            for (int i = 0; i < 4; ++i) {
+                ok[i] = v != 0u;
+                k[i] = ok[i] ? kb + __ffsll((unsigned long long)v) - 1 : 0;
+                v &= v - 1u;
             }
+            float4 r[4][3];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                uint32_t v = W[q];
-                const int kb = (w0 + q) << 5;
-                while (v) {
-                    int k[4];
-                    bool ok[4];
+            for (int i = 0; i < 4; ++i)
+                load_record_from<ABS>(ok[i] ? record_at<ABS>(rec, (size_t)k[i]) : kZeroRecord,
+                                      r[i][0], r[i][1], r[i][2]);
 #pragma unroll
-                    for (int i = 0; i < 4; ++i) {
-                        ok[i] = v != 0u;
-                        k[i] = ok[i] ? kb + __ffs(v) - 1 : 0;
-                        v &= v - 1u;
-                    }
-                    float4 r[4][3];
-#pragma unroll
-                    for (int i = 0; i < 4; ++i)
-                        load_record_from<ABS>(
-                            ok[i] ? record_at<ABS>(rec, (size_t)k[i]) : kZeroRecord, r[i][0],
-                            r[i][1], r[i][2]);
-#pragma unroll
-                    for (int i = 0; i < 4; ++i) {
-                        const float4 q0 = r[i][0], q1 = r[i][1], q2 = r[i][2];
-                        a0.x += q0.x; a0.y += q0.y; a0.z += q0.z; a0.w += q0.w;
-                        a1.x += q1.x; a1.y += q1.y; a1.z += q1.z; a1.w += q1.w;
-                        a2.x += q2.x;
-                        if (ABS) {
-                            a2.y += q2.y;
-                            a2.z += q2.z;
-                        }
-                    }
+            for (int i = 0; i < 4; ++i) {
+                const float4 q0 = r[i][0], q1 = r[i][1], q2 = r[i][2];
+                a0.x += q0.x; a0.y += q0.y; a0.z += q0.z; a0.w += q0.w;
+                a1.x += q1.x; a1.y += q1.y; a1.z += q1.z; a1.w += q1.w;
+                a2.x += q2.x;
+                if (ABS) {
+                    a2.y += q2.y;
+                    a2.z += q2.z;
                 }
             }
         }
