@@ -374,9 +374,10 @@ template <bool ABS>
 __device__ __forceinline__ void sum_records(int off, int end, const float* __restrict__ rec,
                                             const uint32_t* __restrict__ bits, int lane,
                                             float4& a0, float4& a1, float4& a2) {
-    // 16: 59 us for the backward tail of 1M Gaussians at 1080p, 62 us at 12,
-    // 92 us at 8 (longer per-lane runs hold their wave; each wave-wide range
-    // costs one transposed wave reduction)
+    // 16: 56 us for the backward tail of 1M Gaussians at 1080p (24 the same,
+    // 32: 60 us; 12: 62 us and 8: 92 us measured before the two-word mask):
+    // longer per-lane runs hold their wave, and each wave-wide range costs one
+    // transposed wave reduction
     constexpr int kBig = 16;
     const bool big = end - off > kBig;
     const int cnt = end - off;
