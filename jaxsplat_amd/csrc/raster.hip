@@ -401,24 +401,25 @@ __device__ __forceinline__ void raster_fwd_tile(
                 if (take == 0) return false;
                 const float alpha = fminf(0.999f, a.z * __builtin_amdgcn_exp2f(-s2));
                 const float al = lane_in(take) ? alpha : 0.f;
-                float vis = al * T;
-                // T (1 - alpha); exactly T where al = 0
-                float next_T = fmaf(-T, al, T);
+                const float vis = al * T;
+                // T (1 - alpha) as T - alpha T, in place; exactly T where al = 0
+                T = T - vis;
                 // a pixel with al = 0 keeps its T > 1e-4: fin is within take
-                const uint64_t fin = ballot(next_T <= kTEps);
+                const uint64_t fin = ballot(T <= kTEps);
                 uint64_t add = take;
+                float v = vis;
                 if (fin != 0) {
-                    // a terminating pixel keeps T and adds nothing
+                    // a terminating pixel adds nothing and keeps its transmittance
+                    // (T + alpha T restores it to within an ulp)
                     const bool f = lane_in(fin);
-                    vis = f ? 0.f : vis;
-                    next_T = f ? T : next_T;
+                    T = f ? T + vis : T;
+                    v = f ? 0.f : vis;
                     add = take & ~fin;
                     dm |= fin;
                 }
-                cr += col.x * vis;
-                cg += col.y * vis;
-                cb += col.z * vis;
-                T = next_T;
+                cr += col.x * v;
+                cg += col.y * v;
+                cb += col.z * v;
                 cur_off = lane_in(add) ? toff : cur_off;
                 return dm == ~0ull;
             };
